@@ -1,0 +1,212 @@
+/*
+ * mv2.h -- C-ABI of the MI355X Mobile V2 granular-terrain library (libmv2.so).
+ *
+ * Plain pointers and sizes only.  Field pointers are device pointers on the
+ * context's GPU; records are host structs filled after one stream sync.
+ * Every entry point returns an mv2_status; mv2_last_error() gives the message.
+ *
+ * Each entry point names the reference interface it replaces (paths relative
+ * to the reference repository; pkg/ = src/isaac_bulk_pipeline/).
+ *
+ * Field layout: row-major H[y, x], ny rows by nx columns, one value per cell.
+ *   MV2_FP64: z_base, b_eff, mobile, momentum_x, momentum_y are double.
+ *   MV2_FP32: z_base, b_eff, momentum_x, momentum_y are float; mobile is int32
+ *             in quanta of mv2_mobile_quantum() metres (exact mass transfer).
+ */
+#ifndef MV2_H
+#define MV2_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MV2_ABI_VERSION 1
+
+typedef enum mv2_status {
+    MV2_OK = 0,
+    MV2_EINVAL = 1,    /* ValueError in the reference: shape/config/binding */
+    MV2_ENUMERIC = 2,  /* RuntimeError: positivity, maximum_substeps, non-convergence */
+    MV2_EDEVICE = 3,   /* HIP runtime failure */
+    MV2_ESTATE = 4     /* call order (e.g. step before bind) */
+} mv2_status;
+
+typedef enum mv2_precision { MV2_FP64 = 0, MV2_FP32 = 1 } mv2_precision;
+
+typedef struct mv2_context mv2_context;
+
+/* ---- lifecycle: replaces WarpRuntime (pkg/performance/warp_backend.py:131-190) ---- */
+int mv2_abi_version(void);
+double mv2_mobile_quantum(void);
+int mv2_create(int device, mv2_context **out);
+void mv2_destroy(mv2_context *ctx);
+/* last error of ctx (or of the calling thread when ctx is NULL) */
+const char *mv2_last_error(const mv2_context *ctx);
+int mv2_set_stream(mv2_context *ctx, void *hip_stream);
+int mv2_synchronize(mv2_context *ctx);
+
+/* ---- state binding: WarpProductionMobileV2Solver.bind_device_state
+ *      (pkg/bulk_interaction/warp_mobile_v2.py:103-115) over the DeviceBulkState
+ *      named arrays (pkg/runtime/bulk_state_authority.py:543-557). Scratch is
+ *      allocated here once; no per-substep allocation. ---- */
+int mv2_bind(mv2_context *ctx, int precision, int ny, int nx, double dx, double dy,
+             void *z_base, void *b_eff, void *mobile, void *momentum_x, void *momentum_y);
+
+/* ---- Mobile V2 step ---- */
+typedef struct mv2_step_config {
+    double dt;                        /* frame dt (s) */
+    double gravity;                   /* g */
+    double earth_pressure;            /* K */
+    double cfl;
+    double dry_tolerance;
+    double basal_friction;            /* mu */
+    double tan_start;                 /* tan(start angle) for the static early exit */
+    double external_ax, external_ay;  /* uniform external acceleration */
+    double density;                   /* ledger density (1.0 = per-density) */
+    int maximum_substeps;
+    int limit_mode;                   /* 0: mobile_v2_reference.py:237, 1: warp_mobile_v2.py:215 */
+    int weight_mode;                  /* 0: uniform dx*dy, 1: Triangle-A-C vertex weights */
+    int static_exit;                  /* apply the global static early exit */
+} mv2_step_config;
+
+/* summary[]: 0 mass(m^3), 1 px, 2 py, 3 max speed, 4 kinetic, 5 gravitational,
+ *            6 internal (each x density), 7 min h */
+typedef struct mv2_step_record {
+    int status;                       /* mv2_status of the step */
+    int substeps;
+    int cfl_limited;
+    int static_exit;
+    int numeric_error;                /* 1 positivity, 2 maximum_substeps, 3 overflow */
+    int reserved;
+    double maximum_cfl;
+    double remaining;
+    double summary_before[8];
+    double summary_after[8];
+    /* per-substep sums accumulated over the step:
+     * 0,1 sum(q_new - q_old) before dry zeroing (mobile_v2_warp.py:198-199)
+     * 2,3 external impulse  4,5 friction impulse  6 external work (x dt)
+     * 7 twice the friction dissipation  8,9 sum(q_new_zeroed - q_old) (mobile_v2_reference.py:258-272) */
+    double diag[10];
+    double mass_quanta_before;        /* FP32: exact sum of mobile quanta (as double) */
+    double mass_quanta_after;
+} mv2_step_record;
+
+/* One step_resident call (warp_mobile_v2.py:148-245) or standalone step
+ * (pkg/experimental/mobile_v2_warp.py:251-344): static check, device-side CFL
+ * loop with no host round trip per substep, ledgers before/after. */
+int mv2_step(mv2_context *ctx, const mv2_step_config *cfg, mv2_step_record *out);
+
+/* ledger reduction only (warp_mobile_v2.py:28-54 summarize) */
+int mv2_summary(mv2_context *ctx, const mv2_step_config *cfg, double out[8]);
+
+/* static diagnostic (warp_mobile_v2.py:56-85): out[0] max |grad H|, out[1] max |q| */
+int mv2_static_diagnostic(mv2_context *ctx, double out[2]);
+
+/* ---- entrainment / deposition contracts
+ *      DeviceBulkState.entrain_host_indices / deposit_host_indices
+ *      (pkg/runtime/bulk_state_authority.py:104-150, :920-1002).
+ *      idx and depth are DEVICE arrays of length n (unique indices). ---- */
+int mv2_entrain(mv2_context *ctx, const int32_t *idx, const double *depth, int n,
+                int weight_mode, double *volume_out);
+int mv2_deposit(mv2_context *ctx, const int32_t *idx, const double *depth, int n,
+                int weight_mode, double density, double out[3]);
+/* DeviceBulkState.reservoir_reduction (bulk_state_authority.py:60-75, :1004-1034):
+ * out = resting volume, mobile volume, density*px, density*py */
+int mv2_reservoirs(mv2_context *ctx, int weight_mode, double density, double out[4]);
+
+/* ---- plain angle-of-repose relaxation: slope_model.relax_critical_slope
+ *      (slope_model.py:273-343) behind MinimumSlopeAdapter._legacy_call
+ *      (pkg/solvers/minimum_slope_adapter.py:153-182, closed boundary),
+ *      on a canonical H[y, x] field of the given precision (double or float). */
+typedef struct mv2_relax_result {
+    int iterations;
+    int converged;
+    double max_excess_last;
+} mv2_relax_result;
+/* crit_diff[4] = critical slope * distance for the legacy directions
+ * (1,0) dx, (0,1) dy, (1,1) and (1,-1) hypot(dx, dy), as the reference forms them. */
+int mv2_relax(mv2_context *ctx, int precision, void *height, int ny, int nx,
+              const double crit_diff[4], int max_iterations, double tolerance,
+              mv2_relax_result *out);
+
+/* ---- compact colour-batch sweep: WarpCompactActiveEdgeOperator.relax_phase /
+ *      scan_phase (pkg/solvers/warp_compact_frontier.py:57-130, :296-392).
+ *      All arrays are device arrays; flags are zeroed by the call. ---- */
+int mv2_frontier_relax(mv2_context *ctx, double *height, const double *weights,
+                       int32_t *reached, const int32_t *a_i, const int32_t *a_j,
+                       const int32_t *b_i, const int32_t *b_j, int n_edges,
+                       int rows, int cols, int tile_size, int tiles_x, int tile_count,
+                       double critical_difference, int32_t *moved_tile_flags,
+                       int32_t *moved_count_out);
+int mv2_frontier_scan(mv2_context *ctx, const double *height, const int32_t *reached,
+                      const int32_t *tile_ids, const int32_t *a_i, const int32_t *a_j,
+                      const int32_t *b_i, const int32_t *b_j, int n_edges, int cols,
+                      int tile_count, double critical_difference, double tolerance,
+                      int32_t *unstable_tile_flags);
+
+/* ---- LargeAvalanche failure mask + connected components
+ *      DeviceLargeAvalancheBridge.observe_and_maybe_mobilize
+ *      (pkg/runtime/gpu_large_avalanche.py:541-767) device part. ---- */
+typedef struct mv2_avalanche_config {
+    double tan_start, tan_stop;
+    double start_rad, stop_rad;
+    double density, gravity, cohesion;
+    double mobilization_depth, dry_tolerance, activity_speed;
+    int connectivity;                 /* 4 or 8 */
+    int release_settled_latches;
+    int weight_mode;
+    int reserved;
+} mv2_avalanche_config;
+
+/* per-cell persistent/diagnostic arrays (double unless noted), all length ny*nx */
+typedef struct mv2_avalanche_fields {
+    int32_t *unstable;                /* int32 */
+    int32_t *labels;                  /* int32: canonical root (min index) or -1 */
+    int32_t *latch;                   /* int32 */
+    int32_t *previous;                /* int32 */
+    int32_t *activation_count;        /* int32 */
+    int32_t *counts;                  /* int32 scratch: cells per root */
+    double *owned_surface;
+    double *owned_export_baseline;
+    double *export_cumulative;
+    double *slope;
+    double *grad_x;
+    double *grad_y;
+    double *severity;
+    double *first_activation_time;
+    double *last_activation_time;
+    double *r2m_cumulative;
+} mv2_avalanche_fields;
+
+/* diag_i: 0 regions, 1 largest count, 2 winner root, 3 unstable cells, 4 overlap
+ * diag_f: 0 sum excess_start, 1 max excess_start, 2 sum excess_stop, 3 max excess_stop,
+ *         4 area, 5 mobilizable volume, 6 root excess sum */
+typedef struct mv2_label_result {
+    int32_t diag_i[8];
+    double diag_f[8];
+    int union_passes;
+    int reserved;
+} mv2_label_result;
+
+int mv2_avalanche_label(mv2_context *ctx, const mv2_avalanche_config *cfg,
+                        const mv2_avalanche_fields *f, mv2_label_result *out);
+/* update_previous_and_tiles (gpu_large_avalanche.py:284-306) */
+int mv2_avalanche_previous(mv2_context *ctx, const mv2_avalanche_fields *f, int32_t winner,
+                           int keep_candidate, int seed_frontier, int32_t *frontier_reached,
+                           int32_t *tile_flags, int tile_size, int tiles_x, int tile_count);
+/* mobilize (gpu_large_avalanche.py:309-376); diag_f[16] as there */
+int mv2_avalanche_mobilize(mv2_context *ctx, const mv2_avalanche_config *cfg,
+                           const mv2_avalanche_fields *f, int32_t winner,
+                           double activation_time, double diag_f[16]);
+
+/* generic canonical connected-component labelling of an int32 mask
+ * (LargeAvalanche _largest_connected_region semantics, large_avalanche.py:711-778):
+ * labels = min linear index of the component, -1 off-mask. */
+int mv2_label_mask(mv2_context *ctx, const int32_t *mask, int32_t *labels, int ny, int nx,
+                   int connectivity);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,494 @@
+// LargeAvalanche failure mask, connected-component labelling, component
+// statistics and Resting->Mobile activation (pkg/runtime/gpu_large_avalanche.py).
+//
+// Labelling is a single-pass lock-free union-find instead of the reference's
+// host-driven "union_pass + compress until unchanged" loop (:576-599, up to
+// max(ny, nx) launches each followed by a host sync): every masked cell
+// unites itself with its E, S (and SE, SW for 8-connectivity) neighbours by
+// hooking the larger root onto the smaller with compare-and-swap, retrying
+// when another thread hooked first.  Parent indices only ever decrease, so a
+// component's final root is its minimum linear index -- the same canonical
+// label as the reference (root = min index, tie -> smallest root).  One
+// flatten pass then writes every cell's root and counts cells per root.
+//
+// Floating-point sums over the winning component use per-workgroup partials
+// combined in fixed order (the reference uses FP64 atomics).
+#include "mv2_internal.hpp"
+
+#include <cmath>
+#include <cstring>
+
+namespace mv2 {
+
+constexpr int kLabelParts = 512;   // workgroups of the statistics kernels
+constexpr int kWinF = 7;           // winner float statistics
+
+struct LabelAux {
+    int32_t diag_i[8];
+    double diag_f[8];
+};
+
+__device__ inline double ld_field(const void *p, int f32, int64_t i)
+{
+    return f32 ? (double)((const float *)p)[i] : ((const double *)p)[i];
+}
+__device__ inline double ld_mobile(const void *p, int f32, int64_t i)
+{
+    return f32 ? (double)((const int32_t *)p)[i] * kQuantum : ((const double *)p)[i];
+}
+
+__device__ inline double clampd(double x, double lo, double hi) { return x < lo ? lo : (x > hi ? hi : x); }
+
+// initialize (gpu_large_avalanche.py:59-162)
+__global__ void __launch_bounds__(kBlock) avalanche_init_kernel(
+    const void *z_base, const void *resting, const void *mobile, const void *qx, const void *qy, int f32,
+    int ny, int nx, double dx, double dy, mv2_avalanche_config cfg, double area, double f6,
+    mv2_avalanche_fields f)
+{
+    const int64_t n = (int64_t)ny * nx;
+    for (int64_t index = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; index < n;
+         index += (int64_t)gridDim.x * blockDim.x) {
+        const int row = (int)(index / nx), col = (int)(index - (int64_t)row * nx);
+        int64_t left = index, right = index, down = index, up = index;
+        double denom_x = dx, denom_y = dy;
+        if (col > 0) left = index - 1;
+        if (col + 1 < nx) right = index + 1;
+        if (col > 0 && col + 1 < nx) denom_x = 2.0 * dx;
+        if (row > 0) down = index - nx;
+        if (row + 1 < ny) up = index + nx;
+        if (row > 0 && row + 1 < ny) denom_y = 2.0 * dy;
+        double gx = (ld_field(resting, f32, right) + ld_mobile(mobile, f32, right) - ld_field(resting, f32, left) -
+                     ld_mobile(mobile, f32, left)) / denom_x;
+        double gy = (ld_field(resting, f32, up) + ld_mobile(mobile, f32, up) - ld_field(resting, f32, down) -
+                     ld_mobile(mobile, f32, down)) / denom_y;
+        double magnitude = sqrt(gx * gx + gy * gy);
+        f.grad_x[index] = gx;
+        f.grad_y[index] = gy;
+        f.slope[index] = atan(magnitude);
+        double rest = ld_field(resting, f32, index);
+        double layer_depth = fmin(fmax(rest - ld_field(z_base, f32, index), 0.0), cfg.mobilization_depth);
+        double normalization = sqrt(1.0 + magnitude * magnitude);
+        double drive = cfg.density * cfg.gravity * layer_depth * magnitude / normalization;
+        double normal = cfg.density * cfg.gravity * layer_depth / normalization;
+        double start_margin = drive - (cfg.cohesion + normal * cfg.tan_start);
+        double stop_margin = drive - (cfg.cohesion + normal * cfg.tan_stop);
+        double denom = normal * (cfg.tan_start - cfg.tan_stop);
+        f.severity[index] = clampd(stop_margin / (denom >= 1.0e-12 ? denom : 1.0e-12), 0.0, 1.0);
+        const bool active = layer_depth > cfg.dry_tolerance && start_margin > 0.0;
+        f.unstable[index] = active ? 1 : 0;
+        f.labels[index] = active ? (int32_t)index : -1;
+        double m = ld_mobile(mobile, f32, index);
+        double speed = 0.0;
+        if (m > cfg.dry_tolerance) {
+            double vx = ld_field(qx, f32, index) / m, vy = ld_field(qy, f32, index) / m;
+            speed = sqrt(vx * vx + vy * vy);
+        }
+        const double w = cell_weight(cfg.weight_mode, ny, nx, area, f6, row, col);
+        double exported = f.export_cumulative[index] - f.owned_export_baseline[index];
+        bool resolved = exported > cfg.dry_tolerance * w;
+        if (cfg.release_settled_latches && speed <= cfg.activity_speed && (resolved || stop_margin <= 0.0))
+            f.latch[index] = 0;
+    }
+}
+
+__global__ void init_from_mask(const int32_t *mask, int32_t *labels, int64_t n)
+{
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        labels[i] = mask[i] ? (int32_t)i : -1;
+}
+
+__device__ inline int32_t load_parent(int32_t *p, int32_t i)
+{
+    return __hip_atomic_load(&p[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// root of x with path halving; parents satisfy p[x] <= x, roots p[r] == r
+__device__ inline int32_t find_root(int32_t *p, int32_t x)
+{
+    int32_t cur = load_parent(p, x);
+    if (cur != x) {
+        int32_t prev = x, next;
+        while (cur > (next = load_parent(p, cur))) {
+            __hip_atomic_store(&p[prev], next, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            prev = cur;
+            cur = next;
+        }
+    }
+    return cur;
+}
+
+__device__ inline void unite(int32_t *p, int32_t a, int32_t b)
+{
+    int32_t ra = find_root(p, a), rb = find_root(p, b);
+    while (ra != rb) {
+        if (ra < rb) {
+            int32_t expected = rb;
+            if (__hip_atomic_compare_exchange_strong(&p[rb], &expected, ra, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
+                                                     __HIP_MEMORY_SCOPE_AGENT))
+                return;
+            rb = find_root(p, expected);
+        } else {
+            int32_t expected = ra;
+            if (__hip_atomic_compare_exchange_strong(&p[ra], &expected, rb, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
+                                                     __HIP_MEMORY_SCOPE_AGENT))
+                return;
+            ra = find_root(p, expected);
+        }
+    }
+}
+
+__global__ void __launch_bounds__(kBlock) union_kernel(int32_t *p, int ny, int nx, int connectivity)
+{
+    const int64_t n = (int64_t)ny * nx;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        if (load_parent(p, (int32_t)i) < 0) continue;
+        const int r = (int)(i / nx), c = (int)(i - (int64_t)r * nx);
+        const int32_t ii = (int32_t)i;
+        if (c + 1 < nx && load_parent(p, ii + 1) >= 0) unite(p, ii, ii + 1);
+        if (r + 1 < ny) {
+            if (load_parent(p, ii + nx) >= 0) unite(p, ii, ii + nx);
+            if (connectivity == 8) {
+                if (c + 1 < nx && load_parent(p, ii + nx + 1) >= 0) unite(p, ii, ii + nx + 1);
+                if (c > 0 && load_parent(p, ii + nx - 1) >= 0) unite(p, ii, ii + nx - 1);
+            }
+        }
+    }
+}
+
+// final labels (full path to the root) and cell counts per root
+__global__ void __launch_bounds__(kBlock) flatten_kernel(int32_t *p, int64_t n, int32_t *counts)
+{
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        int32_t v = p[i];
+        if (v < 0) continue;
+        int32_t r = (int32_t)i;
+        while (true) {
+            int32_t nx_ = p[r];
+            if (nx_ == r) break;
+            r = nx_;
+        }
+        // every read above observes a completed union pass; the write of this
+        // cell's own entry cannot change any other cell's root
+        p[i] = r;
+        if (counts) atomicAdd(&counts[r], 1);
+    }
+}
+
+__global__ void fill32(int32_t *p, int64_t n, int32_t v)
+{
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) p[i] = v;
+}
+
+// region count / largest / unstable (reduce_components :234-243)
+__global__ void __launch_bounds__(kBlock) count_kernel(const int32_t *counts, int64_t n, LabelAux *aux)
+{
+    __shared__ double smem[16];
+    double v[2] = {0.0, 0.0};       // regions, unstable (exact in double below 2^53)
+    double mx[1] = {0.0};
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        int32_t c = counts[i];
+        if (c > 0) {
+            v[0] += 1.0;
+            v[1] += (double)c;
+            mx[0] = mx[0] >= (double)c ? mx[0] : (double)c;
+        }
+    }
+    block_reduce<double, 2, 0>(v, smem);
+    block_reduce<double, 1, 1>(mx, smem);
+    if (threadIdx.x == 0) {
+        atomicAdd(&aux->diag_i[0], (int32_t)v[0]);
+        atomicAdd(&aux->diag_i[3], (int32_t)v[1]);
+        atomicMax(&aux->diag_i[1], (int32_t)mx[0]);
+    }
+}
+
+// select_winner (:246-252): smallest root among the largest components
+__global__ void __launch_bounds__(kBlock) winner_kernel(const int32_t *counts, int64_t n, LabelAux *aux)
+{
+    const int32_t largest = aux->diag_i[1];
+    if (largest <= 0) return;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        if (counts[i] == largest) atomicMin(&aux->diag_i[2], (int32_t)i);
+}
+
+// aggregate + winner_diagnostics (:201-231, :255-281) restricted to the winner
+__global__ void __launch_bounds__(kBlock) winner_stats_kernel(
+    const int32_t *labels, const void *z_base, const void *resting, int f32, const int32_t *latch,
+    const int32_t *previous, const double *slope, const double *severity, int ny, int nx,
+    mv2_avalanche_config cfg, double area, double f6, LabelAux *aux, double *partials)
+{
+    __shared__ double smem[kWinF * 4];
+    const int32_t winner = aux->diag_i[2];
+    double s[5] = {0.0, 0.0, 0.0, 0.0, 0.0};   // sum es, sum et, area, volume, root excess
+    double m[2] = {0.0, 0.0};                   // max es, max et
+    int prev = 0;
+    const int64_t n = (int64_t)ny * nx;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        if (labels[i] < 0 || labels[i] != winner) continue;
+        const int r = (int)(i / nx), c = (int)(i - (int64_t)r * nx);
+        const double w = cell_weight(cfg.weight_mode, ny, nx, area, f6, r, c);
+        prev += previous ? previous[i] : 0;
+        double es = fmax(slope[i] - cfg.start_rad, 0.0);
+        double et = fmax(slope[i] - cfg.stop_rad, 0.0);
+        s[0] += es;
+        s[1] += et;
+        m[0] = fmax(m[0], es);
+        m[1] = fmax(m[1], et);
+        double transfer = 0.0;
+        if (latch[i] == 0)
+            transfer = fmin(fmax(ld_field(resting, f32, i) - ld_field(z_base, f32, i), 0.0),
+                            cfg.mobilization_depth * severity[i]);
+        s[2] += w;
+        s[3] += transfer * w;
+        s[4] += es;
+    }
+    block_reduce<double, 5, 0>(s, smem);
+    block_reduce<double, 2, 1>(m, smem);
+    int ptot = prev;
+    for (int off = kWave / 2; off > 0; off >>= 1) ptot += __shfl_down(ptot, off, kWave);
+    if ((threadIdx.x & (kWave - 1)) == 0 && ptot) atomicAdd(&aux->diag_i[4], ptot);
+    if (threadIdx.x == 0) {
+        double *pp = partials + (size_t)blockIdx.x * kWinF;
+        pp[0] = s[0]; pp[1] = m[0]; pp[2] = s[1]; pp[3] = m[1]; pp[4] = s[2]; pp[5] = s[3]; pp[6] = s[4];
+    }
+}
+
+__global__ void __launch_bounds__(kBlock) winner_final_kernel(const double *partials, int nparts, LabelAux *aux)
+{
+    __shared__ double smem[kWinF * 4];
+    double s[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+    double m[2] = {0.0, 0.0};
+    for (int p = threadIdx.x; p < nparts; p += blockDim.x) {
+        const double *pp = partials + (size_t)p * kWinF;
+        s[0] += pp[0]; s[1] += pp[2]; s[2] += pp[4]; s[3] += pp[5]; s[4] += pp[6];
+        m[0] = fmax(m[0], pp[1]);
+        m[1] = fmax(m[1], pp[3]);
+    }
+    block_reduce<double, 5, 0>(s, smem);
+    block_reduce<double, 2, 1>(m, smem);
+    if (threadIdx.x == 0) {
+        aux->diag_f[0] = s[0];
+        aux->diag_f[1] = m[0];
+        aux->diag_f[2] = s[1];
+        aux->diag_f[3] = m[1];
+        aux->diag_f[4] = s[2];
+        aux->diag_f[5] = s[3];
+        aux->diag_f[6] = s[4];
+        aux->diag_f[7] = 0.0;
+    }
+}
+
+__global__ void aux_init(LabelAux *aux, int32_t n_plus_one)
+{
+    if (threadIdx.x == 0) {
+        for (int k = 0; k < 8; k++) { aux->diag_i[k] = 0; aux->diag_f[k] = 0.0; }
+        aux->diag_i[2] = n_plus_one;
+    }
+}
+
+// update_previous_and_tiles (:284-306)
+__global__ void __launch_bounds__(kBlock) previous_kernel(const int32_t *labels, int32_t *previous, int32_t *reached,
+                                                          int32_t *tile_flags, int64_t n, int nx, int32_t winner,
+                                                          int keep, int seed, int tile_size, int tiles_x)
+{
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        bool belongs = labels[i] >= 0 && labels[i] == winner;
+        previous[i] = (belongs && keep) ? 1 : 0;
+        if (belongs && seed) {
+            reached[i] = 1;
+            int row = (int)(i / nx), col = (int)(i - (int64_t)row * nx);
+            atomicMax(&tile_flags[(row / tile_size) * tiles_x + col / tile_size], 1);
+        }
+    }
+}
+
+// mobilize (:309-376): zero-momentum Resting -> Mobile transfer on the winner
+constexpr int kMobF = 5;
+__global__ void __launch_bounds__(kBlock) mobilize_kernel(
+    const void *z_base, void *resting, void *mobile, int f32, int ny, int nx, mv2_avalanche_config cfg,
+    double area, double f6, mv2_avalanche_fields f, int32_t winner, double activation_time, double *partials)
+{
+    __shared__ double smem[kMobF * 4];
+    double s[kMobF] = {0.0, 0.0, 0.0, 0.0, 0.0};   // total, new vol, re vol, new area, re area
+    const int64_t n = (int64_t)ny * nx;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        if (f.labels[i] < 0 || f.labels[i] != winner || f.latch[i] != 0) continue;
+        double rest = ld_field(resting, f32, i);
+        double transfer = fmin(fmax(rest - ld_field(z_base, f32, i), 0.0), cfg.mobilization_depth * f.severity[i]);
+        if (transfer <= cfg.dry_tolerance) continue;
+        double mob = ld_mobile(mobile, f32, i);
+        if (f32) {
+            long long dq = (long long)floor(transfer * kInvQuantum);
+            int32_t *mq = (int32_t *)mobile;
+            long long room = 2147483647LL - (long long)mq[i];
+            if (dq > room) dq = room;
+            transfer = (double)dq * kQuantum;
+            f.owned_surface[i] = rest + mob;
+            ((float *)resting)[i] = (float)(rest - transfer);
+            mq[i] = (int32_t)((long long)mq[i] + dq);
+        } else {
+            f.owned_surface[i] = rest + mob;
+            ((double *)resting)[i] = rest - transfer;
+            ((double *)mobile)[i] = mob + transfer;
+        }
+        f.owned_export_baseline[i] = f.export_cumulative[i];
+        f.latch[i] = 1;
+        const int r = (int)(i / nx), c = (int)(i - (int64_t)r * nx);
+        const double w = cell_weight(cfg.weight_mode, ny, nx, area, f6, r, c);
+        double weighted = transfer * w;
+        int32_t prev = f.activation_count[i];
+        f.activation_count[i] = prev + 1;
+        if (prev == 0) {
+            f.first_activation_time[i] = activation_time;
+            s[1] += weighted;
+            s[3] += w;
+        } else {
+            s[2] += weighted;
+            s[4] += w;
+        }
+        f.last_activation_time[i] = activation_time;
+        f.r2m_cumulative[i] = f.r2m_cumulative[i] + weighted;
+        s[0] += weighted;
+    }
+    block_reduce<double, kMobF, 0>(s, smem);
+    if (threadIdx.x == 0)
+        for (int k = 0; k < kMobF; k++) partials[(size_t)blockIdx.x * kMobF + k] = s[k];
+}
+
+__global__ void __launch_bounds__(kBlock) mobilize_final_kernel(const double *partials, int nparts, double *out)
+{
+    __shared__ double smem[kMobF * 4];
+    double s[kMobF] = {0.0, 0.0, 0.0, 0.0, 0.0};
+    for (int p = threadIdx.x; p < nparts; p += blockDim.x)
+        for (int k = 0; k < kMobF; k++) s[k] += partials[(size_t)p * kMobF + k];
+    block_reduce<double, kMobF, 0>(s, smem);
+    if (threadIdx.x == 0)
+        for (int k = 0; k < kMobF; k++) out[k] = s[k];
+}
+
+static int check_ctx(mv2_context *ctx)
+{
+    if (!ctx) return set_error(nullptr, MV2_EINVAL, "[GpuLargeAvalanche] null context");
+    if (!ctx->bound) return set_error(ctx, MV2_ESTATE, "[GpuLargeAvalanche] bind state first");
+    hipSetDevice(ctx->device);
+    return MV2_OK;
+}
+
+static int label_core(mv2_context *ctx, int32_t *labels, int ny, int nx, int connectivity, int32_t *counts)
+{
+    const int64_t n = (int64_t)ny * nx;
+    const int grid = grid_for(n, kBlock, 4096);
+    if (counts) hipLaunchKernelGGL(fill32, dim3(grid), dim3(kBlock), 0, ctx->stream, counts, n, 0);
+    hipLaunchKernelGGL(union_kernel, dim3(grid), dim3(kBlock), 0, ctx->stream, labels, ny, nx, connectivity);
+    hipLaunchKernelGGL(flatten_kernel, dim3(grid), dim3(kBlock), 0, ctx->stream, labels, n, counts);
+    MV2_HIP(ctx, hipGetLastError());
+    return MV2_OK;
+}
+
+}  // namespace mv2
+
+using namespace mv2;
+
+extern "C" int mv2_label_mask(mv2_context *ctx, const int32_t *mask, int32_t *labels, int ny, int nx,
+                              int connectivity)
+{
+    if (!ctx) return set_error(nullptr, MV2_EINVAL, "[LargeAvalanche] null context");
+    if (!mask || !labels || ny < 1 || nx < 1)
+        return set_error(ctx, MV2_EINVAL, "[LargeAvalanche] connected mask must be 2-D");
+    if (connectivity != 4 && connectivity != 8)
+        return set_error(ctx, MV2_EINVAL, "[LargeAvalanche] connectivity must be 4 or 8");
+    if ((int64_t)ny * nx >= ((int64_t)1 << 31))
+        return set_error(ctx, MV2_EINVAL, "[LargeAvalanche] mask too large for int32 labels");
+    hipSetDevice(ctx->device);
+    const int64_t n = (int64_t)ny * nx;
+    hipLaunchKernelGGL(init_from_mask, dim3(grid_for(n, kBlock, 4096)), dim3(kBlock), 0, ctx->stream, mask, labels, n);
+    if (int rc = label_core(ctx, labels, ny, nx, connectivity, nullptr)) return rc;
+    MV2_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return MV2_OK;
+}
+
+extern "C" int mv2_avalanche_label(mv2_context *ctx, const mv2_avalanche_config *cfg,
+                                   const mv2_avalanche_fields *f, mv2_label_result *out)
+{
+    if (int rc = check_ctx(ctx)) return rc;
+    if (!cfg || !f || !out) return set_error(ctx, MV2_EINVAL, "[GpuLargeAvalanche] null argument");
+    if (cfg->connectivity != 4 && cfg->connectivity != 8)
+        return set_error(ctx, MV2_EINVAL, "[LargeAvalanche] connectivity must be 4 or 8");
+    const int ny = ctx->ny, nx = ctx->nx;
+    const int64_t n = (int64_t)ny * nx;
+    const int f32 = ctx->precision == MV2_FP32;
+    const double area = ctx->dx * ctx->dy, f6 = area / 6.0;
+    if (int rc = ensure_scratch(ctx, ctx->label_aux, sizeof(LabelAux) + sizeof(double) * kWinF * kLabelParts)) return rc;
+    LabelAux *aux = (LabelAux *)ctx->label_aux.ptr;
+    double *partials = (double *)((char *)ctx->label_aux.ptr + sizeof(LabelAux));
+    const int grid = grid_for(n, kBlock, 4096);
+    hipLaunchKernelGGL(avalanche_init_kernel, dim3(grid), dim3(kBlock), 0, ctx->stream, ctx->z_base, ctx->b_eff,
+                       ctx->mobile, ctx->qx, ctx->qy, f32, ny, nx, ctx->dx, ctx->dy, *cfg, area, f6, *f);
+    if (int rc = label_core(ctx, f->labels, ny, nx, cfg->connectivity, f->counts)) return rc;
+    hipLaunchKernelGGL(aux_init, dim3(1), dim3(64), 0, ctx->stream, aux, (int32_t)(n + 1));
+    const int sgrid = grid_for(n, kBlock, kLabelParts);
+    hipLaunchKernelGGL(count_kernel, dim3(sgrid), dim3(kBlock), 0, ctx->stream, f->counts, n, aux);
+    hipLaunchKernelGGL(winner_kernel, dim3(sgrid), dim3(kBlock), 0, ctx->stream, f->counts, n, aux);
+    hipLaunchKernelGGL(winner_stats_kernel, dim3(sgrid), dim3(kBlock), 0, ctx->stream, f->labels, ctx->z_base,
+                       ctx->b_eff, f32, f->latch, f->previous, f->slope, f->severity, ny, nx, *cfg, area, f6, aux,
+                       partials);
+    hipLaunchKernelGGL(winner_final_kernel, dim3(1), dim3(kBlock), 0, ctx->stream, partials, sgrid, aux);
+    MV2_HIP(ctx, hipGetLastError());
+    MV2_HIP(ctx, hipMemcpyAsync(ctx->host_result.ptr, aux, sizeof(LabelAux), hipMemcpyDeviceToHost, ctx->stream));
+    MV2_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    LabelAux host;
+    memcpy(&host, ctx->host_result.ptr, sizeof(LabelAux));
+    memcpy(out->diag_i, host.diag_i, sizeof(host.diag_i));
+    memcpy(out->diag_f, host.diag_f, sizeof(host.diag_f));
+    out->union_passes = 1;
+    out->reserved = 0;
+    return MV2_OK;
+}
+
+extern "C" int mv2_avalanche_previous(mv2_context *ctx, const mv2_avalanche_fields *f, int32_t winner,
+                                      int keep_candidate, int seed_frontier, int32_t *frontier_reached,
+                                      int32_t *tile_flags, int tile_size, int tiles_x, int tile_count)
+{
+    if (int rc = check_ctx(ctx)) return rc;
+    if (!f || !frontier_reached || !tile_flags || tile_size < 1 || tile_count < 1)
+        return set_error(ctx, MV2_EINVAL, "[GpuLargeAvalanche] invalid previous/tiles arguments");
+    const int64_t n = (int64_t)ctx->ny * ctx->nx;
+    hipLaunchKernelGGL(fill32, dim3(grid_for(tile_count)), dim3(kBlock), 0, ctx->stream, tile_flags,
+                       (int64_t)tile_count, 0);
+    hipLaunchKernelGGL(previous_kernel, dim3(grid_for(n, kBlock, 4096)), dim3(kBlock), 0, ctx->stream, f->labels,
+                       f->previous, frontier_reached, tile_flags, n, ctx->nx, winner, keep_candidate, seed_frontier,
+                       tile_size, tiles_x);
+    MV2_HIP(ctx, hipGetLastError());
+    return MV2_OK;
+}
+
+extern "C" int mv2_avalanche_mobilize(mv2_context *ctx, const mv2_avalanche_config *cfg,
+                                      const mv2_avalanche_fields *f, int32_t winner, double activation_time,
+                                      double diag_f[16])
+{
+    if (int rc = check_ctx(ctx)) return rc;
+    if (!cfg || !f || !diag_f) return set_error(ctx, MV2_EINVAL, "[GpuLargeAvalanche] null argument");
+    const int ny = ctx->ny, nx = ctx->nx;
+    const int64_t n = (int64_t)ny * nx;
+    const int f32 = ctx->precision == MV2_FP32;
+    const double area = ctx->dx * ctx->dy, f6 = area / 6.0;
+    if (int rc = ensure_scratch(ctx, ctx->label_aux, sizeof(LabelAux) + sizeof(double) * kWinF * kLabelParts)) return rc;
+    double *partials = (double *)((char *)ctx->label_aux.ptr + sizeof(LabelAux));
+    double *dres = (double *)((char *)ctx->result.ptr + 3584);
+    const int sgrid = grid_for(n, kBlock, kLabelParts);
+    hipLaunchKernelGGL(mobilize_kernel, dim3(sgrid), dim3(kBlock), 0, ctx->stream, ctx->z_base, ctx->b_eff,
+                       ctx->mobile, f32, ny, nx, *cfg, area, f6, *f, winner, activation_time, partials);
+    hipLaunchKernelGGL(mobilize_final_kernel, dim3(1), dim3(kBlock), 0, ctx->stream, partials, sgrid, dres);
+    MV2_HIP(ctx, hipGetLastError());
+    MV2_HIP(ctx, hipMemcpyAsync(ctx->host_result.ptr, dres, sizeof(double) * kMobF, hipMemcpyDeviceToHost, ctx->stream));
+    MV2_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    double s[kMobF];
+    memcpy(s, ctx->host_result.ptr, sizeof(s));
+    for (int k = 0; k < 16; k++) diag_f[k] = 0.0;
+    diag_f[7] = s[0];
+    diag_f[10] = s[1];
+    diag_f[11] = s[2];
+    diag_f[12] = s[3];
+    diag_f[13] = s[4];
+    return MV2_OK;
+}

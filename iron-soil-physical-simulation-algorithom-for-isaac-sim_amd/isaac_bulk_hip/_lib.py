@@ -1,0 +1,150 @@
+"""ctypes binding of libmv2.so (declarations in include/mv2.h).
+
+The HIP library is the only compute path: if it is missing or cannot find a
+GPU, the call raises -- there is no CPU fallback.
+"""
+
+from __future__ import annotations
+
+import ctypes
+from pathlib import Path
+
+_HERE = Path(__file__).resolve().parent
+LIB_PATH = _HERE / "libmv2.so"
+
+_D = ctypes.c_double
+_I = ctypes.c_int
+_I32 = ctypes.c_int32
+_P = ctypes.c_void_p
+
+MV2_OK, MV2_EINVAL, MV2_ENUMERIC, MV2_EDEVICE, MV2_ESTATE = 0, 1, 2, 3, 4
+MV2_FP64, MV2_FP32 = 0, 1
+
+
+class HipBackendUnavailable(RuntimeError):
+    """The HIP library or a GPU is unavailable (WarpBackendUnavailable equivalent)."""
+
+
+class StepConfig(ctypes.Structure):
+    _fields_ = [
+        ("dt", _D), ("gravity", _D), ("earth_pressure", _D), ("cfl", _D),
+        ("dry_tolerance", _D), ("basal_friction", _D), ("tan_start", _D),
+        ("external_ax", _D), ("external_ay", _D), ("density", _D),
+        ("maximum_substeps", _I), ("limit_mode", _I), ("weight_mode", _I), ("static_exit", _I),
+    ]
+
+
+class StepRecord(ctypes.Structure):
+    _fields_ = [
+        ("status", _I), ("substeps", _I), ("cfl_limited", _I), ("static_exit", _I),
+        ("numeric_error", _I), ("reserved", _I),
+        ("maximum_cfl", _D), ("remaining", _D),
+        ("summary_before", _D * 8), ("summary_after", _D * 8), ("diag", _D * 10),
+        ("mass_quanta_before", _D), ("mass_quanta_after", _D),
+    ]
+
+
+class RelaxResult(ctypes.Structure):
+    _fields_ = [("iterations", _I), ("converged", _I), ("max_excess_last", _D)]
+
+
+class AvalancheConfig(ctypes.Structure):
+    _fields_ = [
+        ("tan_start", _D), ("tan_stop", _D), ("start_rad", _D), ("stop_rad", _D),
+        ("density", _D), ("gravity", _D), ("cohesion", _D), ("mobilization_depth", _D),
+        ("dry_tolerance", _D), ("activity_speed", _D),
+        ("connectivity", _I), ("release_settled_latches", _I), ("weight_mode", _I), ("reserved", _I),
+    ]
+
+
+class AvalancheFields(ctypes.Structure):
+    _fields_ = [(name, _P) for name in (
+        "unstable", "labels", "latch", "previous", "activation_count", "counts",
+        "owned_surface", "owned_export_baseline", "export_cumulative", "slope", "grad_x", "grad_y",
+        "severity", "first_activation_time", "last_activation_time", "r2m_cumulative")]
+
+
+class LabelResult(ctypes.Structure):
+    _fields_ = [("diag_i", _I32 * 8), ("diag_f", _D * 8), ("union_passes", _I), ("reserved", _I)]
+
+
+class BatchDesc(ctypes.Structure):
+    _fields_ = [
+        ("ny", _I), ("nx", _I), ("offset", ctypes.c_int64),
+    ]
+
+
+# exported symbol -> (restype, argtypes); the CPU test suite checks that every
+# symbol declared in include/mv2.h is exported
+SIGNATURES: dict[str, tuple[object, list[object]]] = {
+    "mv2_abi_version": (_I, []),
+    "mv2_mobile_quantum": (_D, []),
+    "mv2_create": (_I, [_I, ctypes.POINTER(_P)]),
+    "mv2_destroy": (None, [_P]),
+    "mv2_last_error": (ctypes.c_char_p, [_P]),
+    "mv2_set_stream": (_I, [_P, _P]),
+    "mv2_synchronize": (_I, [_P]),
+    "mv2_bind": (_I, [_P, _I, _I, _I, _D, _D, _P, _P, _P, _P, _P]),
+    "mv2_step": (_I, [_P, ctypes.POINTER(StepConfig), ctypes.POINTER(StepRecord)]),
+    "mv2_summary": (_I, [_P, ctypes.POINTER(StepConfig), _P]),
+    "mv2_static_diagnostic": (_I, [_P, _P]),
+    "mv2_entrain": (_I, [_P, _P, _P, _I, _I, ctypes.POINTER(_D)]),
+    "mv2_deposit": (_I, [_P, _P, _P, _I, _I, _D, _P]),
+    "mv2_reservoirs": (_I, [_P, _I, _D, _P]),
+    "mv2_relax": (_I, [_P, _I, _P, _I, _I, _P, _I, _D, ctypes.POINTER(RelaxResult)]),
+    "mv2_frontier_relax": (_I, [_P, _P, _P, _P, _P, _P, _P, _P, _I, _I, _I, _I, _I, _I, _D, _P,
+                                ctypes.POINTER(_I32)]),
+    "mv2_frontier_scan": (_I, [_P, _P, _P, _P, _P, _P, _P, _P, _I, _I, _I, _D, _D, _P]),
+    "mv2_avalanche_label": (_I, [_P, ctypes.POINTER(AvalancheConfig), ctypes.POINTER(AvalancheFields),
+                                 ctypes.POINTER(LabelResult)]),
+    "mv2_avalanche_previous": (_I, [_P, ctypes.POINTER(AvalancheFields), _I32, _I, _I, _P, _P, _I, _I,
+                                    _I]),
+    "mv2_avalanche_mobilize": (_I, [_P, ctypes.POINTER(AvalancheConfig),
+                                    ctypes.POINTER(AvalancheFields), _I32, _D, _P]),
+    "mv2_label_mask": (_I, [_P, _P, _P, _I, _I, _I]),
+}
+
+_lib: ctypes.CDLL | None = None
+
+
+def load() -> ctypes.CDLL:
+    """Load libmv2.so and declare its signatures (raises if it is not built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not LIB_PATH.exists():
+        raise HipBackendUnavailable(
+            f"libmv2.so is not built at {LIB_PATH}; run __graft_entry__.build()")
+    lib = ctypes.CDLL(str(LIB_PATH))
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(lib, name, None)
+        if fn is None:
+            continue
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def exported_symbols() -> list[str]:
+    lib = load()
+    return [name for name in SIGNATURES if hasattr(lib, name)]
+
+
+def last_error(ctx: int | None) -> str:
+    msg = load().mv2_last_error(ctx)
+    return msg.decode() if msg else ""
+
+
+def raise_for(code: int, ctx: int | None, *, value_error=ValueError, runtime_error=RuntimeError) -> None:
+    """Map an mv2_status to the reference's exception types."""
+    if code == MV2_OK:
+        return
+    msg = last_error(ctx)
+    if code == MV2_EINVAL:
+        raise value_error(msg)
+    if code == MV2_ENUMERIC:
+        raise runtime_error(msg)
+    if code == MV2_ESTATE:
+        raise RuntimeError(msg)
+    raise HipBackendUnavailable(msg)

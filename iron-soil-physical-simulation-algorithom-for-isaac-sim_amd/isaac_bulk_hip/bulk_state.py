@@ -1,0 +1,412 @@
+"""Device-authoritative bulk terrain state (the plugin surface of the hot path).
+
+Mirrors DeviceBulkState / DeviceMaterialLedger of the reference
+(pkg/runtime/bulk_state_authority.py:416-1406): the same named arrays
+(``z_base``, ``b_eff`` with its ``resting`` alias, ``mobile``,
+``momentum_x``, ``momentum_y``, ``weights`` and the avalanche/frontier
+scratch), the entrain/deposit contracts, the reservoir ledger, explicit host
+views and checkpoint restore.  Arrays are torch tensors on the MI355X; the
+contracts run as HIP kernels through libmv2.
+
+precision="fp64" stores every field as float64, exactly like the reference.
+precision="fp32" stores z_base/b_eff/momentum as float32 and ``mobile`` as
+int32 quanta of ``MOBILE_QUANTUM_M`` (2^-28 m): the Mobile V2 step then moves
+mass in whole quanta, so its mass ledger is exact.  ``mobile_m()`` and the host
+views convert quanta to metres.
+"""
+
+from __future__ import annotations
+
+import ctypes
+from dataclasses import dataclass
+from enum import Enum
+from typing import Iterable
+
+import numpy as np
+
+from . import _lib
+from .config import TerrainGrid, triangle_vertex_weights
+from .runtime import HipRuntime
+
+MOBILE_QUANTUM_M = 1.0 / 268435456.0
+
+
+class BulkStateAuthority(str, Enum):
+    HOST = "HOST"
+    DEVICE = "DEVICE"
+
+
+class BulkStateAuthorityError(RuntimeError):
+    """Raised when a non-authoritative view attempts to mutate physics state."""
+
+
+@dataclass(frozen=True)
+class DeviceMaterialLedgerSnapshot:
+    initial_total_m3: float
+    resting_m3: float
+    mobile_m3: float
+    payload_m3: float
+    airborne_m3: float
+    outflow_m3: float
+    absolute_volume_error_m3: float
+    relative_volume_error: float
+
+
+@dataclass(frozen=True)
+class HostBulkStateView:
+    """Read-only host observation taken at an explicit boundary."""
+
+    H_resting_m: np.ndarray
+    H_mobile_m: np.ndarray
+    mobile_momentum_m2_s: np.ndarray
+    timestamp_device_s: float
+    reset_generation: int
+    source: str
+    dirty_tile_ids: np.ndarray
+    z_base_m: np.ndarray
+
+    @property
+    def b_eff_m(self) -> np.ndarray:
+        return self.H_resting_m
+
+    @property
+    def H_free_m(self) -> np.ndarray:
+        return self.b_eff_m + self.H_mobile_m
+
+    @property
+    def h_resting_derived_m(self) -> np.ndarray:
+        return self.b_eff_m - self.z_base_m
+
+    def commit(self, *args, **kwargs) -> None:
+        raise BulkStateAuthorityError(
+            "[HostBulkStateView] host views are non-authoritative and cannot commit physics")
+
+
+class DeviceBulkState:
+    """HIP-resident authoritative state shared by the Mobile V2 operators."""
+
+    backend_identity = "GPU_RUNTIME_DEVICE_AUTHORITATIVE_HIP"
+    _INT_FIELDS = ("active_mask", "material_mask", "dirty_mask", "frontier_reached",
+                   "avalanche_unstable", "avalanche_parent", "avalanche_component_count",
+                   "avalanche_latch", "avalanche_previous_component",
+                   "avalanche_activation_count", "deposition_mask")
+    _F64_FIELDS = ("avalanche_slope", "avalanche_gradient_x", "avalanche_gradient_y",
+                   "avalanche_severity", "avalanche_first_activation_time",
+                   "avalanche_last_activation_time", "avalanche_owned_surface",
+                   "avalanche_owned_export_baseline", "mobile_export_cumulative",
+                   "avalanche_r2m_cumulative", "avalanche_m2r_cumulative")
+
+    def __init__(self, grid: TerrainGrid, initial_resting_m, *, device: str = "cuda:0",
+                 tile_size: int = 64, precision: str = "fp64",
+                 runtime: HipRuntime | None = None) -> None:
+        if precision not in ("fp64", "fp32"):
+            raise ValueError("[DeviceBulkState] precision must be 'fp64' or 'fp32'")
+        self.grid = grid
+        self.shape = grid.shape
+        self.size = int(grid.nx * grid.ny)
+        self.tile_size = int(tile_size)
+        if self.tile_size < 4:
+            raise ValueError("[DeviceBulkState] tile_size must be >= 4")
+        self.precision = precision
+        self.runtime = runtime or HipRuntime(device)
+        self.authority = BulkStateAuthority.DEVICE
+        self._timestamp_device_s = 0.0
+        self._timestamp_host_view_s = -1.0
+        self._reset_generation = 0
+        self._dirty_tile_ids: set[int] = set()
+        torch = __import__("torch")
+        self._fdtype = torch.float64 if precision == "fp64" else torch.float32
+        self._mdtype = torch.float64 if precision == "fp64" else torch.int32
+        initial = np.asarray(grid.validate_heightmap(initial_resting_m), dtype=np.float64)
+        rt = self.runtime
+        rt.upload("z_base", np.zeros(self.shape).ravel(), self._fdtype)
+        rt.upload("initial_b_eff", initial.ravel(), self._fdtype)
+        rt.upload("b_eff", initial.ravel(), self._fdtype)
+        rt.zeros("mobile", self.size, self._mdtype)
+        rt.zeros("momentum_x", self.size, self._fdtype)
+        rt.zeros("momentum_y", self.size, self._fdtype)
+        rt.upload("weights", triangle_vertex_weights(grid.ny, grid.nx, grid.dx, grid.dy).ravel(),
+                  torch.float64)
+        rt.arrays["initial_resting"] = rt.arrays["initial_b_eff"]
+        rt.arrays["resting"] = rt.arrays["b_eff"]
+        for name in self._INT_FIELDS:
+            rt.zeros(name, self.size, torch.int32)
+        for name in self._F64_FIELDS:
+            rt.zeros(name, self.size, torch.float64)
+        rt.zeros("dirty_tile_flags", self.tile_count, torch.int32)
+        self._bind()
+        self.begin_physics_step()
+
+    # -- binding -----------------------------------------------------------------
+    def _bind(self) -> None:
+        rt = self.runtime
+        prec = _lib.MV2_FP64 if self.precision == "fp64" else _lib.MV2_FP32
+        rc = rt.lib.mv2_bind(rt.ctx, prec, self.grid.ny, self.grid.nx, float(self.grid.dx),
+                             float(self.grid.dy), rt.ptr("z_base"), rt.ptr("b_eff"), rt.ptr("mobile"),
+                             rt.ptr("momentum_x"), rt.ptr("momentum_y"))
+        rt.check(rc)
+
+    @property
+    def timestamp_device_s(self) -> float:
+        return self._timestamp_device_s
+
+    @property
+    def reset_generation(self) -> int:
+        return self._reset_generation
+
+    @property
+    def tile_shape(self) -> tuple[int, int]:
+        return ((self.shape[0] + self.tile_size - 1) // self.tile_size,
+                (self.shape[1] + self.tile_size - 1) // self.tile_size)
+
+    @property
+    def tile_count(self) -> int:
+        ty, tx = self.tile_shape
+        return int(ty * tx)
+
+    def begin_physics_step(self) -> None:
+        self.runtime.telemetry.reset()
+
+    def advance_time(self, dt_s: float) -> None:
+        dt = float(dt_s)
+        if not np.isfinite(dt) or dt < 0.0:
+            raise ValueError("[DeviceBulkState] dt_s must be finite/non-negative")
+        self._timestamp_device_s += dt
+
+    # -- field access ----------------------------------------------------------------
+    def mobile_m(self):
+        """Mobile thickness in metres as a device float64 tensor."""
+        m = self.runtime.arrays["mobile"]
+        return m.double() * MOBILE_QUANTUM_M if self.precision == "fp32" else m
+
+    def set_fields(self, *, z_base=None, b_eff=None, mobile=None, momentum_x=None,
+                   momentum_y=None) -> None:
+        """Explicit full-field initialisation (acceptance/checkpoint boundary)."""
+        torch = __import__("torch")
+        rt = self.runtime
+        for name, value in (("z_base", z_base), ("b_eff", b_eff), ("momentum_x", momentum_x),
+                            ("momentum_y", momentum_y)):
+            if value is None:
+                continue
+            arr = np.asarray(value, dtype=np.float64).reshape(self.shape)
+            rt.arrays[name].copy_(torch.from_numpy(arr.ravel()).to(rt.device, self._fdtype))
+        if mobile is not None:
+            arr = np.asarray(mobile, dtype=np.float64).reshape(self.shape)
+            if np.any(arr < 0.0):
+                raise ValueError("MobileV2 h must be nonnegative")
+            if self.precision == "fp32":
+                q = np.rint(arr * (1.0 / MOBILE_QUANTUM_M))
+                if np.any(q > np.iinfo(np.int32).max):
+                    raise ValueError("[DeviceBulkState] mobile thickness exceeds the FP32 storage range")
+                t = torch.from_numpy(q.astype(np.int32).ravel())
+            else:
+                t = torch.from_numpy(arr.ravel())
+            rt.arrays["mobile"].copy_(t.to(rt.device))
+
+    # -- contracts -------------------------------------------------------------------
+    def _compact(self, flat_indices, values, what: str):
+        torch = __import__("torch")
+        indices = np.asarray(flat_indices, dtype=np.int64).reshape(-1)
+        depth = np.asarray(values, dtype=np.float64).reshape(-1)
+        if depth.shape != indices.shape or not np.all(np.isfinite(depth)):
+            raise ValueError(f"[DeviceBulkState] invalid {what} request")
+        if np.any(indices < 0) or np.any(indices >= self.size):
+            raise ValueError(f"[DeviceBulkState] {what} index outside state")
+        if np.unique(indices).size != indices.size:
+            raise ValueError(f"[DeviceBulkState] duplicate {what} indices")
+        idx_t = torch.from_numpy(indices.astype(np.int32)).to(self.runtime.device)
+        dep_t = torch.from_numpy(depth).to(self.runtime.device)
+        self.runtime.telemetry.record_h2d(indices.astype(np.int32))
+        self.runtime.telemetry.record_h2d(depth)
+        return indices, idx_t, dep_t
+
+    def entrain_host_indices(self, flat_indices, requested_depth_m, *, reason: str) -> float:
+        """Resting -> Mobile transfer (bulk_state_authority.py:920-960)."""
+        if not reason:
+            raise ValueError("[DeviceBulkState] entrainment reason must be non-empty")
+        indices, idx_t, dep_t = self._compact(flat_indices, requested_depth_m, "entrainment")
+        if not indices.size:
+            return 0.0
+        rt = self.runtime
+        rt.bind_stream()
+        out = ctypes.c_double(0.0)
+        rc = rt.lib.mv2_entrain(rt.ctx, idx_t.data_ptr(), dep_t.data_ptr(), int(indices.size), 1,
+                                ctypes.byref(out))
+        rt.check(rc)
+        rt.telemetry.record_call()
+        rows, cols = np.divmod(indices, self.shape[1])
+        self.mark_dirty_bbox((int(rows.min()), int(rows.max()) + 1, int(cols.min()), int(cols.max()) + 1))
+        return float(out.value)
+
+    def deposit_host_indices(self, flat_indices, requested_depth_m, *, density_kg_m3: float,
+                             reason: str) -> dict:
+        """Mobile -> Resting transfer with reported impulse (bulk_state_authority.py:962-1002)."""
+        if not reason:
+            raise ValueError("[DeviceBulkState] deposition reason must be non-empty")
+        density = float(density_kg_m3)
+        if not density > 0.0:
+            raise ValueError("[DeviceBulkState] invalid deposition indices/density")
+        indices, idx_t, dep_t = self._compact(flat_indices, requested_depth_m, "deposition")
+        if not indices.size:
+            return {"deposited_volume_m3": 0.0, "removed_mobile_momentum_kg_m_s": np.zeros(3)}
+        rt = self.runtime
+        rt.bind_stream()
+        out = np.zeros(3)
+        rc = rt.lib.mv2_deposit(rt.ctx, idx_t.data_ptr(), dep_t.data_ptr(), int(indices.size), 1,
+                                density, out.ctypes.data)
+        rt.check(rc)
+        rt.telemetry.record_call()
+        rows, cols = np.divmod(indices, self.shape[1])
+        self.mark_dirty_bbox((int(rows.min()), int(rows.max()) + 1, int(cols.min()), int(cols.max()) + 1))
+        return {"deposited_volume_m3": float(out[0]),
+                "removed_mobile_momentum_kg_m_s": np.asarray([out[1], out[2], 0.0])}
+
+    def reservoir_reduction(self, density_kg_m3: float) -> dict:
+        density = float(density_kg_m3)
+        if not np.isfinite(density) or density <= 0.0:
+            raise ValueError("[DeviceBulkState] density must be finite/positive")
+        rt = self.runtime
+        rt.bind_stream()
+        out = np.zeros(4)
+        rt.check(rt.lib.mv2_reservoirs(rt.ctx, 1, density, out.ctypes.data))
+        rt.telemetry.record_d2h(out)
+        return {"resting_volume_m3": float(out[0]), "mobile_volume_m3": float(out[1]),
+                "mobile_momentum_kg_m_s": np.asarray([out[2], out[3], 0.0])}
+
+    # -- dirty tiles -----------------------------------------------------------------
+    def mark_dirty_tiles(self, tile_ids: Iterable[int]) -> None:
+        for tile_id in np.asarray(tuple(tile_ids), dtype=np.int64).ravel():
+            if tile_id < 0 or tile_id >= self.tile_count:
+                raise ValueError("[DeviceBulkState] dirty tile ID outside grid")
+            self._dirty_tile_ids.add(int(tile_id))
+
+    def mark_dirty_bbox(self, bbox_yx) -> None:
+        row0, row1, col0, col1 = (int(v) for v in bbox_yx)
+        if row1 <= row0 or col1 <= col0:
+            return
+        ty, tx = self.tile_shape
+        fy, ly = max(0, row0 // self.tile_size), min(ty - 1, (row1 - 1) // self.tile_size)
+        fx, lx = max(0, col0 // self.tile_size), min(tx - 1, (col1 - 1) // self.tile_size)
+        self.mark_dirty_tiles(r * tx + c for r in range(fy, ly + 1) for c in range(fx, lx + 1))
+
+    def consume_dirty_tile_ids(self) -> np.ndarray:
+        out = np.asarray(sorted(self._dirty_tile_ids), dtype=np.int32)
+        self._dirty_tile_ids.clear()
+        return out
+
+    # -- explicit boundaries ---------------------------------------------------------
+    def explicit_host_view(self, *, source: str) -> HostBulkStateView:
+        allowed = {"debug", "acceptance", "checkpoint", "reset", "visualization"}
+        if source not in allowed:
+            raise BulkStateAuthorityError(
+                "[DeviceBulkState] full host views are restricted to explicit boundaries, "
+                f"got source={source!r}")
+        rt = self.runtime
+        rt.synchronize()
+        resting = rt.download("b_eff").astype(np.float64).reshape(self.shape)
+        z = rt.download("z_base").astype(np.float64).reshape(self.shape)
+        mobile = rt.download("mobile").astype(np.float64).reshape(self.shape)
+        if self.precision == "fp32":
+            mobile = mobile * MOBILE_QUANTUM_M
+        q = np.stack((rt.download("momentum_x").astype(np.float64).reshape(self.shape),
+                      rt.download("momentum_y").astype(np.float64).reshape(self.shape)), axis=-1)
+        self._timestamp_host_view_s = self._timestamp_device_s
+        return HostBulkStateView(resting, mobile, q, self._timestamp_device_s,
+                                 self._reset_generation, source, self.consume_dirty_tile_ids(), z)
+
+    def restore_checkpoint_fields(self, fields: dict, *, timestamp_s: float, source: str) -> None:
+        if source not in {"acceptance", "checkpoint"}:
+            raise BulkStateAuthorityError(
+                "[DeviceBulkState] checkpoint restore requires an explicit boundary")
+        torch = __import__("torch")
+        float_state = {"z_base", "b_eff", "resting", "mobile", "momentum_x", "momentum_y"}
+        known = float_state | set(self._F64_FIELDS) | set(self._INT_FIELDS)
+        unknown = sorted(set(fields) - known)
+        if unknown:
+            raise BulkStateAuthorityError(f"[DeviceBulkState] unsupported checkpoint fields: {unknown}")
+        state_values = {}
+        for name, value in fields.items():
+            key = "b_eff" if name == "resting" else name
+            arr = np.asarray(value)
+            if arr.shape != self.shape or not np.all(np.isfinite(arr)):
+                raise BulkStateAuthorityError(f"[DeviceBulkState] invalid checkpoint field {key!r}")
+            if key in float_state:
+                state_values[key] = arr
+            else:
+                dtype = torch.float64 if key in self._F64_FIELDS else torch.int32
+                self.runtime.arrays[key].copy_(torch.from_numpy(np.ascontiguousarray(arr).ravel()).to(
+                    self.runtime.device, dtype))
+        self.set_fields(**state_values)
+        t = float(timestamp_s)
+        if not np.isfinite(t) or t < 0.0:
+            raise BulkStateAuthorityError("[DeviceBulkState] checkpoint timestamp must be finite/non-negative")
+        self._timestamp_device_s = t
+        self._timestamp_host_view_s = -1.0
+        self.mark_dirty_tiles(range(self.tile_count))
+        self.begin_physics_step()
+
+    def reset(self) -> None:
+        rt = self.runtime
+        rt.arrays["b_eff"].copy_(rt.arrays["initial_b_eff"])
+        for name in ("z_base", "mobile", "momentum_x", "momentum_y", "dirty_tile_flags",
+                     *self._INT_FIELDS, *self._F64_FIELDS):
+            rt.arrays[name].zero_()
+        rt.synchronize()
+        self._timestamp_device_s = 0.0
+        self._timestamp_host_view_s = -1.0
+        self._dirty_tile_ids = set(range(self.tile_count))
+        self._reset_generation += 1
+        self.begin_physics_step()
+
+    def assert_consistent(self) -> None:
+        if self.authority is not BulkStateAuthority.DEVICE:
+            raise BulkStateAuthorityError("[DeviceBulkState] state authority is not DEVICE")
+        if self.runtime.arrays["resting"] is not self.runtime.arrays["b_eff"]:
+            raise BulkStateAuthorityError("[DeviceBulkState] resting alias diverged from b_eff")
+
+    def diagnostics(self) -> dict:
+        self.assert_consistent()
+        return {
+            "backend_identity": self.backend_identity,
+            "state_authority": self.authority.value,
+            "authoritative_geometry": ["z_base", "b_eff", "mobile"],
+            "precision": self.precision,
+            "mobile_storage": "int32 quanta of 2^-28 m" if self.precision == "fp32" else "float64 m",
+            "shape_yx": list(self.shape),
+            "tile_size": self.tile_size,
+            "timestamp_device_s": self._timestamp_device_s,
+            "reset_generation": self._reset_generation,
+            "resident_array_names": sorted(self.runtime.arrays),
+        }
+
+
+class DeviceMaterialLedger:
+    """Scalar material ledger whose terrain entries come from device reductions."""
+
+    def __init__(self, state: DeviceBulkState, density_kg_m3: float, *, payload_m3: float = 0.0,
+                 airborne_m3: float = 0.0, outflow_m3: float = 0.0,
+                 initial_total_m3: float | None = None) -> None:
+        values = state.reservoir_reduction(density_kg_m3)
+        self._density_kg_m3 = float(density_kg_m3)
+        computed = float(values["resting_volume_m3"] + values["mobile_volume_m3"]
+                         + payload_m3 + airborne_m3 + outflow_m3)
+        self._initial_total_m3 = computed if initial_total_m3 is None else float(initial_total_m3)
+        if not np.isfinite(self._initial_total_m3) or self._initial_total_m3 < 0.0:
+            raise ValueError("[DeviceMaterialLedger] initial total must be finite/non-negative")
+
+    @property
+    def initial_total_m3(self) -> float:
+        return self._initial_total_m3
+
+    def snapshot(self, state: DeviceBulkState, *, payload_m3: float, airborne_m3: float,
+                 outflow_m3: float) -> DeviceMaterialLedgerSnapshot:
+        s = state.reservoir_reduction(self._density_kg_m3)
+        current = float(s["resting_volume_m3"] + s["mobile_volume_m3"] + float(payload_m3)
+                        + float(airborne_m3) + float(outflow_m3))
+        error = current - self._initial_total_m3
+        return DeviceMaterialLedgerSnapshot(
+            initial_total_m3=self._initial_total_m3, resting_m3=float(s["resting_volume_m3"]),
+            mobile_m3=float(s["mobile_volume_m3"]), payload_m3=float(payload_m3),
+            airborne_m3=float(airborne_m3), outflow_m3=float(outflow_m3),
+            absolute_volume_error_m3=error,
+            relative_volume_error=abs(error) / max(abs(self._initial_total_m3), 1.0e-12))

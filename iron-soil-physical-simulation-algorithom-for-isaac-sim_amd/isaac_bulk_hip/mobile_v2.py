@@ -1,0 +1,279 @@
+"""Mobile V2 granular-terrain step on the MI355X.
+
+Drop-in counterparts of the reference's device solvers:
+  HipProductionMobileV2Solver  <- WarpProductionMobileV2Solver
+                                  (pkg/bulk_interaction/warp_mobile_v2.py:91-253)
+  HipMobileV2ReferenceSolver   <- WarpMobileV2ReferenceSolver
+                                  (pkg/experimental/mobile_v2_warp.py:238-344)
+with the same records (WarpMobileStep, warp_mobile_layer.py:349-371;
+MobileV2Step, mobile_v2_reference.py:63-90).  The whole step -- ledger,
+static early exit, every CFL substep and the closing ledger -- is one HIP
+launch (libmv2 ``mv2_step``); the host reads one fixed-size record.
+"""
+
+from __future__ import annotations
+
+import ctypes
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import _lib
+from .bulk_state import DeviceBulkState
+from .config import MobileV2Config, TerrainGrid
+
+
+@dataclass(frozen=True)
+class WarpMobileStep:
+    volume_before_m3: float
+    volume_after_m3: float
+    momentum_before_terrain_kg_m_s: np.ndarray
+    momentum_after_terrain_kg_m_s: np.ndarray
+    gravity_pressure_impulse_terrain_ns: np.ndarray
+    basal_friction_impulse_terrain_ns: np.ndarray
+    tool_impulse_on_mobile_terrain_ns: np.ndarray
+    numerical_dissipative_impulse_terrain_ns: np.ndarray
+    maximum_speed_m_s: float
+    substeps: int
+    cfl_limited: bool
+    kinetic_energy_before_j: float
+    kinetic_energy_after_j: float
+    gravity_pressure_work_j: float
+    basal_friction_work_j: float
+    tool_work_j: float
+    transport_numerical_energy_residual_j: float
+    donor_export_m3: float
+    receiver_import_m3: float
+    transport_mass_residual_m3: float
+    advected_momentum_crossings_kg_m_s: np.ndarray
+
+
+@dataclass(frozen=True)
+class MobileV2State:
+    b_eff_m: np.ndarray
+    h_m: np.ndarray
+    q_m2_s: np.ndarray
+
+    def __post_init__(self) -> None:
+        b = np.asarray(self.b_eff_m, dtype=np.float64)
+        h = np.asarray(self.h_m, dtype=np.float64)
+        q = np.asarray(self.q_m2_s, dtype=np.float64)
+        if b.ndim != 2 or h.shape != b.shape or q.shape != b.shape + (2,):
+            raise ValueError("MobileV2 state shape mismatch")
+        if not np.all(np.isfinite(b)) or not np.all(np.isfinite(h)) or not np.all(np.isfinite(q)):
+            raise ValueError("MobileV2 state must be finite")
+        if np.any(h < -1.0e-14):
+            raise ValueError("MobileV2 h must be nonnegative")
+
+    @property
+    def H_free_m(self) -> np.ndarray:
+        return np.asarray(self.b_eff_m) + np.asarray(self.h_m)
+
+
+@dataclass(frozen=True)
+class MobileV2Step:
+    state: MobileV2State
+    dt_s: float
+    substeps: int
+    maximum_cfl: float
+    mass_before_m3: float
+    mass_after_m3: float
+    mass_residual_m3: float
+    momentum_before_m4_s: np.ndarray
+    momentum_after_m4_s: np.ndarray
+    conservative_topography_wall_impulse_m4_s: np.ndarray
+    friction_impulse_m4_s: np.ndarray
+    external_impulse_m4_s: np.ndarray
+    momentum_accounting_residual_m4_s: np.ndarray
+    kinetic_before_j_per_density: float
+    kinetic_after_j_per_density: float
+    gravitational_before_j_per_density: float
+    gravitational_after_j_per_density: float
+    internal_before_j_per_density: float
+    internal_after_j_per_density: float
+    energy_before_j_per_density: float
+    energy_after_j_per_density: float
+    friction_dissipation_j_per_density: float
+    external_work_j_per_density: float
+    entrainment_deposition_energy_transfer_j_per_density: float
+    numerical_energy_residual_j_per_density: float
+    minimum_h_m: float
+
+
+def _step_config(config: MobileV2Config, dt: float, *, density: float, limit_mode: int,
+                 weight_mode: int, external=(0.0, 0.0)) -> _lib.StepConfig:
+    return _lib.StepConfig(
+        dt=float(dt), gravity=config.gravity_m_s2, earth_pressure=config.earth_pressure_coefficient,
+        cfl=config.cfl, dry_tolerance=config.dry_tolerance_m,
+        basal_friction=config.basal_friction_coefficient, tan_start=config.tan_start,
+        external_ax=float(external[0]), external_ay=float(external[1]), density=float(density),
+        maximum_substeps=int(config.maximum_substeps), limit_mode=int(limit_mode),
+        weight_mode=int(weight_mode), static_exit=1)
+
+
+def run_step(state: DeviceBulkState, config: MobileV2Config, dt: float, *, density: float,
+             limit_mode: int, weight_mode: int, external=(0.0, 0.0),
+             error_prefix: str = "") -> _lib.StepRecord:
+    """Run one device step on ``state`` and return the raw record."""
+    rt = state.runtime
+    rt.bind_stream()
+    cfg = _step_config(config, dt, density=density, limit_mode=limit_mode,
+                       weight_mode=weight_mode, external=external)
+    rec = _lib.StepRecord()
+    rc = rt.lib.mv2_step(rt.ctx, ctypes.byref(cfg), ctypes.byref(rec))
+    rt.telemetry.record_call(launches=3, syncs=1)
+    rt.telemetry.record_d2h(np.zeros(ctypes.sizeof(rec) // 8))
+    if rc != _lib.MV2_OK:
+        msg = _lib.last_error(rt.ctx)
+        if rc == _lib.MV2_EINVAL:
+            raise ValueError(msg)
+        if rc == _lib.MV2_ENUMERIC:
+            raise RuntimeError(f"{error_prefix}{msg}")
+        _lib.raise_for(rc, rt.ctx)
+    return rec
+
+
+class HipProductionMobileV2Solver:
+    """Accepted V2 equations consuming the persistent device fields (step_resident)."""
+
+    backend_name = "GPU_HIP_MOBILE_V2_PRODUCTION_SHARED_STATE"
+
+    def __init__(self, *, runtime=None, config: MobileV2Config | None = None) -> None:
+        self.runtime = runtime
+        self.config = config or MobileV2Config()
+        self.state: DeviceBulkState | None = None
+        self.material = None
+        self.grid = None
+
+    def bind_device_state(self, state: DeviceBulkState, material, grid, integrator) -> None:
+        del integrator
+        if self.runtime is None:
+            self.runtime = state.runtime
+        if state.runtime is not self.runtime:
+            raise ValueError("[MobileV2Production] runtime mismatch")
+        required = {"b_eff", "mobile", "momentum_x", "momentum_y", "weights"}
+        missing = required - set(self.runtime.arrays)
+        if missing:
+            raise ValueError(f"[MobileV2Production] missing state: {sorted(missing)}")
+        self.state, self.material, self.grid = state, material, grid
+
+    def step_resident(self, dt_s: float, **_) -> WarpMobileStep:
+        if self.state is None:
+            raise RuntimeError("[MobileV2Production] bind state first")
+        state = self.state
+        density = float(self.material.assumed_bulk_density_kg_m3)
+        rec = run_step(state, self.config, float(dt_s), density=density, limit_mode=1,
+                       weight_mode=1, error_prefix="[MobileV2Production] ")
+        before = np.asarray(rec.summary_before[:])
+        after = np.asarray(rec.summary_after[:])
+        diag = np.asarray(rec.diag[:])
+        area = state.grid.dx * state.grid.dy
+        before_p = np.asarray([before[1], before[2], 0.0])
+        if rec.static_exit:
+            return WarpMobileStep(
+                volume_before_m3=float(before[0]), volume_after_m3=float(before[0]),
+                momentum_before_terrain_kg_m_s=before_p, momentum_after_terrain_kg_m_s=before_p.copy(),
+                gravity_pressure_impulse_terrain_ns=np.zeros(3),
+                basal_friction_impulse_terrain_ns=np.zeros(3),
+                tool_impulse_on_mobile_terrain_ns=np.zeros(3),
+                numerical_dissipative_impulse_terrain_ns=np.zeros(3),
+                maximum_speed_m_s=float(before[3]), substeps=0, cfl_limited=False,
+                kinetic_energy_before_j=float(before[4]), kinetic_energy_after_j=float(before[4]),
+                gravity_pressure_work_j=0.0, basal_friction_work_j=0.0, tool_work_j=0.0,
+                transport_numerical_energy_residual_j=0.0, donor_export_m3=0.0,
+                receiver_import_m3=0.0, transport_mass_residual_m3=0.0,
+                advected_momentum_crossings_kg_m_s=np.zeros(3))
+        gravity = np.asarray([diag[0], diag[1], 0.0]) * density * area
+        friction = np.asarray([diag[4], diag[5], 0.0]) * density * area
+        after_p = np.asarray([after[1], after[2], 0.0])
+        friction_work = float(0.5 * diag[7] * density * area)
+        return WarpMobileStep(
+            volume_before_m3=float(before[0]), volume_after_m3=float(after[0]),
+            momentum_before_terrain_kg_m_s=before_p, momentum_after_terrain_kg_m_s=after_p,
+            gravity_pressure_impulse_terrain_ns=gravity, basal_friction_impulse_terrain_ns=friction,
+            tool_impulse_on_mobile_terrain_ns=np.zeros(3),
+            numerical_dissipative_impulse_terrain_ns=after_p - before_p - gravity - friction,
+            maximum_speed_m_s=float(after[3]), substeps=int(rec.substeps),
+            cfl_limited=bool(rec.cfl_limited), kinetic_energy_before_j=float(before[4]),
+            kinetic_energy_after_j=float(after[4]),
+            gravity_pressure_work_j=float((after[5] + after[6]) - (before[5] + before[6])),
+            basal_friction_work_j=friction_work, tool_work_j=0.0,
+            transport_numerical_energy_residual_j=float(
+                (after[4] + after[5] + after[6]) - (before[4] + before[5] + before[6]) + friction_work),
+            donor_export_m3=0.0, receiver_import_m3=0.0,
+            transport_mass_residual_m3=float(after[0] - before[0]),
+            advected_momentum_crossings_kg_m_s=np.zeros(3))
+
+    def diagnostics(self) -> dict:
+        return {
+            "backend_name": self.backend_name,
+            "state": ["z_base", "b_eff", "mobile", "momentum_x", "momentum_y"],
+            "pressure_flux": "0.5*K*g*h^2_CONSERVATIVE_FACE_FLUX",
+            "K_classification": "ENGINEERING_CLOSURE_UNCALIBRATED",
+            "precision": None if self.state is None else self.state.precision,
+            "substep_loop": "DEVICE_RESIDENT_COOPERATIVE_LAUNCH",
+        }
+
+
+# Name under which callers of the reference bind the production solver.
+WarpProductionMobileV2Solver = HipProductionMobileV2Solver
+
+
+class HipMobileV2ReferenceSolver:
+    """Standalone device step of a host ``MobileV2State`` (never bound to production state)."""
+
+    def __init__(self, config: MobileV2Config | None = None, *, device: str = "cuda:0",
+                 precision: str = "fp64") -> None:
+        self.config = config or MobileV2Config()
+        self.device = device
+        self.precision = precision
+        self._state: DeviceBulkState | None = None
+
+    def _device_state(self, shape) -> DeviceBulkState:
+        ny, nx = shape
+        if self._state is None or self._state.shape != (ny, nx):
+            grid = TerrainGrid(int(nx), int(ny), self.config.dx_m, self.config.dy_m)
+            runtime = None if self._state is None else self._state.runtime
+            self._state = DeviceBulkState(grid, np.zeros((ny, nx)), device=self.device,
+                                          precision=self.precision, runtime=runtime)
+        return self._state
+
+    def step(self, state: MobileV2State, dt_s: float, *, external_acceleration_m_s2=None) -> MobileV2Step:
+        dt = float(dt_s)
+        if not np.isfinite(dt) or dt <= 0.0:
+            raise ValueError("dt must be finite/positive")
+        ext = (0.0, 0.0)
+        if external_acceleration_m_s2 is not None:
+            e = np.asarray(external_acceleration_m_s2, dtype=np.float64)
+            if e.shape != (2,):
+                raise ValueError("device MobileV2 supports a uniform (2,) external acceleration")
+            ext = (float(e[0]), float(e[1]))
+        dev = self._device_state(state.h_m.shape)
+        q = np.asarray(state.q_m2_s, dtype=np.float64)
+        dev.set_fields(z_base=np.zeros(state.h_m.shape), b_eff=state.b_eff_m, mobile=state.h_m,
+                       momentum_x=q[..., 0], momentum_y=q[..., 1])
+        rec = run_step(dev, self.config, dt, density=1.0, limit_mode=0, weight_mode=0,
+                       external=ext, error_prefix="Warp ")
+        view = dev.explicit_host_view(source="acceptance")
+        final = MobileV2State(np.array(state.b_eff_m, dtype=np.float64, copy=True),
+                              view.H_mobile_m, view.mobile_momentum_m2_s)
+        b, a = np.asarray(rec.summary_before[:]), np.asarray(rec.summary_after[:])
+        d = np.asarray(rec.diag[:])
+        area = self.config.dx_m * self.config.dy_m
+        k0, g0, i0 = b[4], b[5], b[6]
+        k1, g1, i1 = a[4], a[5], a[6]
+        e0, e1 = k0 + g0 + i0, k1 + g1 + i1
+        cons = d[8:10] * area
+        fric = d[4:6] * area
+        ext_imp = d[2:4] * area
+        fdis = float(0.5 * d[7] * area)
+        ework = float(d[6] * area)
+        p0, p1 = b[1:3] * 1.0, a[1:3] * 1.0
+        return MobileV2Step(
+            final, dt, int(rec.substeps), float(rec.maximum_cfl), float(b[0]), float(a[0]),
+            float(a[0] - b[0]), p0, p1, cons, fric, ext_imp, p1 - p0 - (cons + fric + ext_imp),
+            float(k0), float(k1), float(g0), float(g1), float(i0), float(i1), float(e0), float(e1),
+            fdis, ework, 0.0, float(e1 - e0 - ework + fdis), float(min(a[7], 0.0)))
+
+
+WarpMobileV2ReferenceSolver = HipMobileV2ReferenceSolver

@@ -1,0 +1,330 @@
+"""ctypes front-end of the CPU oracle (``liboracle.so``).
+
+TEST INFRASTRUCTURE ONLY: imported by ``tests/``, ``__graft_entry__.smoke()``
+and the ``cpu_baseline`` leg of ``bench.py`` -- never by the product package.
+The C restatement in ``mv2_oracle.c`` cites the reference routine each entry
+follows.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+from pathlib import Path
+
+import numpy as np
+
+_HERE = Path(__file__).resolve().parent
+_LIB_PATH = _HERE / "liboracle.so"
+_lib: ctypes.CDLL | None = None
+
+_D = ctypes.c_double
+_I = ctypes.c_int
+_P = ctypes.c_void_p
+
+
+class V2Params(ctypes.Structure):
+    _fields_ = [
+        ("dx", _D), ("dy", _D), ("gravity", _D), ("earth_pressure", _D),
+        ("cfl", _D), ("dry_tolerance", _D), ("basal_friction", _D),
+        ("tan_start", _D), ("external_ax", _D), ("external_ay", _D),
+        ("maximum_substeps", _I), ("limit_mode", _I), ("static_exit", _I),
+    ]
+
+
+class V2Result(ctypes.Structure):
+    _fields_ = [
+        ("status", _I), ("substeps", _I), ("cfl_limited", _I), ("static_exit", _I),
+        ("maximum_cfl", _D), ("remaining", _D),
+        ("conservative_impulse", _D * 2), ("friction_impulse", _D * 2),
+        ("external_impulse", _D * 2), ("friction_dissipation", _D),
+        ("external_work", _D), ("diag", _D * 8),
+    ]
+
+
+class AvalancheParams(ctypes.Structure):
+    _fields_ = [
+        ("dx", _D), ("dy", _D), ("tan_start", _D), ("tan_stop", _D),
+        ("density", _D), ("gravity", _D), ("cohesion", _D),
+        ("mobilization_depth", _D), ("dry_tolerance", _D), ("activity_speed", _D),
+        ("release_settled_latches", _I),
+    ]
+
+
+def build() -> Path:
+    """Compile the oracle with its Makefile (idempotent)."""
+    subprocess.run(["make", "-s", "-C", str(_HERE)], check=True)
+    return _LIB_PATH
+
+
+def lib() -> ctypes.CDLL:
+    global _lib
+    if _lib is None:
+        if not _LIB_PATH.exists():
+            build()
+        _lib = ctypes.CDLL(str(_LIB_PATH))
+        _lib.orc_pairwise_sum.restype = _D
+        _lib.orc_pairwise_sum.argtypes = [_P, ctypes.c_int64, ctypes.c_int64]
+        _lib.orc_v2_step.argtypes = [_I, _I, ctypes.POINTER(V2Params), _P, _P, _P, _P, _D,
+                                     ctypes.POINTER(V2Result)]
+        _lib.orc_v2_step.restype = _I
+        _lib.orc_v2_face.argtypes = [_D] * 8 + [_I, _D, _D, _D, _D, _P]
+        _lib.orc_v2_static.argtypes = [_I, _I, _D, _D, _P, _P, _P, _P, _P]
+        _lib.orc_vertex_weight.restype = _D
+        _lib.orc_vertex_weight.argtypes = [_I, _I, _D, _D, _I, _I]
+        _lib.orc_v2_summary.argtypes = [_I, _I, _D, _D, _I, _D, _D, _D, _D, _P, _P, _P, _P, _P]
+        _lib.orc_entrain.restype = _D
+        _lib.orc_entrain.argtypes = [_I, _P, _P, _P, _P, _P, _P, _I]
+        _lib.orc_deposit.argtypes = [_I, _P, _P, _P, _P, _P, _P, _P, _I, _D, _P]
+        _lib.orc_reservoirs.argtypes = [_I, _P, _P, _P, _P, _P, _P, _D, _P]
+        for name in ("orc_relax_legacy_f64", "orc_relax_legacy_f32"):
+            fn = getattr(_lib, name)
+            fn.restype = _I
+            fn.argtypes = [_I, _I, _D, _D, _D, _I, _D, _P, ctypes.POINTER(_I),
+                           ctypes.POINTER(_D)]
+        _lib.orc_frontier_relax.restype = _I
+        _lib.orc_frontier_relax.argtypes = [_I, _I, _I, _I, _P, _P, _P, _P, _P, _P, _P, _I,
+                                            _D, _P]
+        _lib.orc_frontier_scan.argtypes = [_I, _P, _P, _P, _P, _P, _P, _P, _I, _D, _D, _P]
+        _lib.orc_avalanche_initialize.argtypes = [
+            _I, _I, ctypes.POINTER(AvalancheParams), _P, _P, _P, _P, _P, _P, _P, _P,
+            _P, _P, _P, _P, _P, _P]
+        _lib.orc_label.argtypes = [_I, _I, _I, _P, _P]
+        _lib.orc_components.argtypes = [_I, _I, _P, _P, _P, _P, _P, _P, _P, _P, _D, _D, _D,
+                                        _P, _P]
+        _lib.orc_mobilize.argtypes = [_I, _P, ctypes.c_int32, _P, _P, _P, _P, _P, _P, _P, _P,
+                                      _P, _P, _P, _P, _P, _D, _D, _D, _P]
+    return _lib
+
+
+def _ptr(a: np.ndarray) -> int:
+    return a.ctypes.data
+
+
+def _f64(a) -> np.ndarray:
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def _i32(a) -> np.ndarray:
+    return np.ascontiguousarray(a, dtype=np.int32)
+
+
+def pairwise_sum(a: np.ndarray) -> float:
+    a = _f64(a).ravel()
+    return float(lib().orc_pairwise_sum(_ptr(a), a.size, 1))
+
+
+def v2_step(b, h, qx, qy, dt, *, dx=0.05, dy=0.05, gravity=9.81, K=0.45, cfl=0.18,
+            dry=1.0e-10, mu=0.55, tan_start=None, start_angle_deg=34.0,
+            external=(0.0, 0.0), maximum_substeps=4096, limit_mode=0, static_exit=True):
+    """Restated ``MobileV2ReferenceSolver.step`` (FP64).  Returns new fields + record."""
+    b = _f64(b)
+    h = _f64(h).copy()
+    qx = _f64(qx).copy()
+    qy = _f64(qy).copy()
+    ny, nx = b.shape
+    if tan_start is None:
+        tan_start = float(np.tan(np.deg2rad(start_angle_deg)))
+    p = V2Params(dx, dy, gravity, K, cfl, dry, mu, tan_start, float(external[0]),
+                 float(external[1]), int(maximum_substeps), int(limit_mode), int(bool(static_exit)))
+    r = V2Result()
+    lib().orc_v2_step(ny, nx, ctypes.byref(p), _ptr(b), _ptr(h), _ptr(qx), _ptr(qy),
+                      float(dt), ctypes.byref(r))
+    rec = {
+        "status": r.status, "substeps": r.substeps, "cfl_limited": bool(r.cfl_limited),
+        "static_exit": bool(r.static_exit), "maximum_cfl": r.maximum_cfl,
+        "remaining": r.remaining,
+        "conservative_impulse": np.array(r.conservative_impulse[:]),
+        "friction_impulse": np.array(r.friction_impulse[:]),
+        "external_impulse": np.array(r.external_impulse[:]),
+        "friction_dissipation": r.friction_dissipation,
+        "external_work": r.external_work,
+        "diag": np.array(r.diag[:]),
+    }
+    return h, qx, qy, rec
+
+
+def v2_face(bl, br, hl, hr, qxl, qyl, qxr, qyr, normal, factor, K=0.45, g=9.81, dry=1.0e-10):
+    """One shared face (already scaled by factor): [F_h, qx_left, qy_left, qx_right, qy_right]."""
+    out = np.zeros(5)
+    lib().orc_v2_face(bl, br, hl, hr, qxl, qyl, qxr, qyr, int(normal), K, g, dry, factor,
+                      _ptr(out))
+    return out
+
+
+def v2_static(b, h, qx, qy, dx=0.05, dy=0.05):
+    b, h, qx, qy = (_f64(v) for v in (b, h, qx, qy))
+    out = np.zeros(2)
+    lib().orc_v2_static(b.shape[0], b.shape[1], dx, dy, _ptr(b), _ptr(h), _ptr(qx), _ptr(qy),
+                        _ptr(out))
+    return out
+
+
+def vertex_weights(ny, nx, dx, dy) -> np.ndarray:
+    w = np.empty((ny, nx))
+    for r in range(ny):
+        for c in range(nx):
+            w[r, c] = lib().orc_vertex_weight(ny, nx, dx, dy, r, c)
+    return w
+
+
+def v2_summary(b, h, qx, qy, *, dx, dy, weight_mode, density, K=0.45, g=9.81, dry=1.0e-10):
+    b, h, qx, qy = (_f64(v) for v in (b, h, qx, qy))
+    out = np.zeros(7)
+    lib().orc_v2_summary(b.shape[0], b.shape[1], dx, dy, int(weight_mode), density, K, g,
+                         dry, _ptr(b), _ptr(h), _ptr(qx), _ptr(qy), _ptr(out))
+    return out
+
+
+def entrain(z, b, m, w, idx, req):
+    z, w, req = _f64(z), _f64(w), _f64(req)
+    b, m = _f64(b).copy(), _f64(m).copy()
+    idx = _i32(idx)
+    v = lib().orc_entrain(b.size, _ptr(z), _ptr(b), _ptr(m), _ptr(w), _ptr(idx), _ptr(req),
+                          idx.size)
+    return b, m, float(v)
+
+
+def deposit(b, m, qx, qy, w, idx, req, density):
+    b, m, qx, qy = (_f64(v).copy() for v in (b, m, qx, qy))
+    w, req, idx = _f64(w), _f64(req), _i32(idx)
+    out = np.zeros(3)
+    lib().orc_deposit(b.size, _ptr(b), _ptr(m), _ptr(qx), _ptr(qy), _ptr(w), _ptr(idx),
+                      _ptr(req), idx.size, density, _ptr(out))
+    return b, m, qx, qy, out
+
+
+def reservoirs(z, b, m, qx, qy, w, density):
+    z, b, m, qx, qy, w = (_f64(v) for v in (z, b, m, qx, qy, w))
+    out = np.zeros(4)
+    lib().orc_reservoirs(b.size, _ptr(z), _ptr(b), _ptr(m), _ptr(qx), _ptr(qy), _ptr(w),
+                         density, _ptr(out))
+    return out
+
+
+def relax_legacy(h, d0, d1, crit_slope, max_iterations=20000, tolerance=1.0e-8,
+                 dtype=np.float64):
+    """``relax_critical_slope`` on a legacy-layout array; returns (h, iterations, converged, last)."""
+    h = np.ascontiguousarray(h, dtype=dtype).copy()
+    n0, n1 = h.shape
+    conv = _I(0)
+    last = _D(0.0)
+    fn = lib().orc_relax_legacy_f64 if dtype == np.float64 else lib().orc_relax_legacy_f32
+    it = fn(n0, n1, d0, d1, float(crit_slope), int(max_iterations), float(tolerance),
+            _ptr(h), ctypes.byref(conv), ctypes.byref(last))
+    return h, int(it), bool(conv.value), float(last.value)
+
+
+def relax_yx(h_yx, dx, dy, crit_slope, max_iterations=20000, tolerance=1.0e-8,
+             dtype=np.float64):
+    """``MinimumSlopeAdapter`` closed-boundary semantics on canonical ``H[y,x]``."""
+    legacy = np.ascontiguousarray(np.asarray(h_yx).T)
+    out, it, conv, last = relax_legacy(legacy, dx, dy, crit_slope, max_iterations, tolerance,
+                                       dtype)
+    return np.ascontiguousarray(out.T), it, conv, last
+
+
+def frontier_relax(height, weights, reached, a_i, a_j, b_i, b_j, critical_difference,
+                   tile_size, tiles_x):
+    height = _f64(height).copy()
+    reached = _i32(reached).copy()
+    weights = _f64(weights)
+    rows, cols = height.shape
+    a_i, a_j, b_i, b_j = (_i32(v) for v in (a_i, a_j, b_i, b_j))
+    tiles_y = (rows + tile_size - 1) // tile_size
+    flags = np.zeros(tiles_y * tiles_x, dtype=np.int32)
+    moved = lib().orc_frontier_relax(rows, cols, tile_size, tiles_x, _ptr(height), _ptr(weights),
+                                     _ptr(reached), _ptr(a_i), _ptr(a_j), _ptr(b_i), _ptr(b_j),
+                                     a_i.size, critical_difference, _ptr(flags))
+    return height, reached, int(moved), flags
+
+
+def frontier_scan(height, reached, tile_ids, a_i, a_j, b_i, b_j, critical_difference, tolerance,
+                  tile_count):
+    height, reached = _f64(height), _i32(reached)
+    tile_ids, a_i, a_j, b_i, b_j = (_i32(v) for v in (tile_ids, a_i, a_j, b_i, b_j))
+    flags = np.zeros(tile_count, dtype=np.int32)
+    lib().orc_frontier_scan(height.shape[1], _ptr(height), _ptr(reached), _ptr(tile_ids),
+                            _ptr(a_i), _ptr(a_j), _ptr(b_i), _ptr(b_j), a_i.size,
+                            critical_difference, tolerance, _ptr(flags))
+    return flags
+
+
+def avalanche_initialize(z, resting, mobile, qx, qy, weights, owned_baseline, export_cum, latch,
+                         *, dx, dy, tan_start, tan_stop, density, gravity, cohesion,
+                         mobilization_depth, dry, activity_speed, release):
+    z, resting, mobile, qx, qy, weights, owned_baseline, export_cum = (
+        _f64(v) for v in (z, resting, mobile, qx, qy, weights, owned_baseline, export_cum))
+    ny, nx = resting.shape
+    latch = _i32(latch).copy()
+    unstable = np.zeros((ny, nx), np.int32)
+    slope, gx, gy, sev = (np.zeros((ny, nx)) for _ in range(4))
+    p = AvalancheParams(dx, dy, tan_start, tan_stop, density, gravity, cohesion,
+                        mobilization_depth, dry, activity_speed, int(bool(release)))
+    lib().orc_avalanche_initialize(ny, nx, ctypes.byref(p), _ptr(z), _ptr(resting), _ptr(mobile),
+                                   _ptr(qx), _ptr(qy), _ptr(weights), _ptr(owned_baseline),
+                                   _ptr(export_cum), _ptr(unstable), _ptr(latch), _ptr(slope),
+                                   _ptr(gx), _ptr(gy), _ptr(sev))
+    return {"unstable": unstable, "latch": latch, "slope": slope, "grad_x": gx, "grad_y": gy,
+            "severity": sev}
+
+
+def label(mask, connectivity=8) -> np.ndarray:
+    mask = _i32(mask)
+    ny, nx = mask.shape
+    out = np.empty((ny, nx), np.int32)
+    lib().orc_label(ny, nx, int(connectivity), _ptr(mask), _ptr(out))
+    return out
+
+
+def components(labels, z, resting, latch, weights, slope, severity, previous, mobilization_depth,
+               start_rad, stop_rad):
+    labels, latch, previous = _i32(labels), _i32(latch), _i32(previous)
+    z, resting, weights, slope, severity = (_f64(v) for v in (z, resting, weights, slope, severity))
+    ny, nx = labels.shape
+    di = np.zeros(8, np.int32)
+    df = np.zeros(8)
+    lib().orc_components(ny, nx, _ptr(labels), _ptr(z), _ptr(resting), _ptr(latch), _ptr(weights),
+                         _ptr(slope), _ptr(severity), _ptr(previous), mobilization_depth,
+                         start_rad, stop_rad, _ptr(di), _ptr(df))
+    return di, df
+
+
+def mobilize(labels, winner, z, resting, mobile, weights, latch, owned_surface, owned_baseline,
+             export_cum, severity, activation_count, first_time, last_time, r2m, activation_time,
+             mobilization_depth, dry):
+    labels = _i32(labels)
+    z, weights, export_cum, severity = (_f64(v) for v in (z, weights, export_cum, severity))
+    resting, mobile, owned_surface, owned_baseline, first_time, last_time, r2m = (
+        _f64(v).copy() for v in (resting, mobile, owned_surface, owned_baseline, first_time,
+                                 last_time, r2m))
+    latch, activation_count = _i32(latch).copy(), _i32(activation_count).copy()
+    diag = np.zeros(16)
+    lib().orc_mobilize(labels.size, _ptr(labels), int(winner), _ptr(z), _ptr(resting),
+                       _ptr(mobile), _ptr(weights), _ptr(latch), _ptr(owned_surface),
+                       _ptr(owned_baseline), _ptr(export_cum), _ptr(severity),
+                       _ptr(activation_count), _ptr(first_time), _ptr(last_time), _ptr(r2m),
+                       activation_time, mobilization_depth, dry, _ptr(diag))
+    return {"resting": resting, "mobile": mobile, "latch": latch, "owned_surface": owned_surface,
+            "owned_baseline": owned_baseline, "activation_count": activation_count,
+            "first_time": first_time, "last_time": last_time, "r2m": r2m, "diag": diag}
+
+
+def energy_terms(h, qx, qy, b, dx, dy, g=9.81, K=0.45, dry=1.0e-10):
+    """``MobileV2ReferenceSolver.energy_terms`` (mobile_v2_reference.py:110-127)."""
+    h = np.asarray(h)
+    q = np.stack((qx, qy), axis=-1)
+    area = dx * dy
+    speed2 = np.divide(np.sum(q * q, axis=-1), h * h, out=np.zeros_like(h), where=h > dry)
+    kinetic = float(0.5 * np.sum(h * speed2, dtype=np.float64) * area)
+    grav = float(g * np.sum(h * b, dtype=np.float64) * area)
+    internal = float(0.5 * K * g * np.sum(h * h, dtype=np.float64) * area)
+    return kinetic, grav, internal, kinetic + grav + internal
+
+
+if os.environ.get("MV2_ORACLE_AUTOBUILD", "1") == "1" and not _LIB_PATH.exists():
+    try:
+        build()
+    except Exception:  # pragma: no cover - reported on first lib() call
+        pass

@@ -1,0 +1,171 @@
+"""GPU parity of relaxation, compact frontier sweeps and labelling vs the oracle."""
+
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+import oracle
+
+pytestmark = pytest.mark.gpu
+
+
+def _device(a, dtype):
+    import torch
+
+    return torch.from_numpy(np.ascontiguousarray(a)).to("cuda:0", dtype)
+
+
+@pytest.mark.parametrize("shape,angle,seed", [((40, 37), 34.0, 0), ((65, 130), 30.0, 1),
+                                              ((2, 2), 34.0, 2), ((97, 33), 38.0, 3)])
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_relax_bit_exact(shape, angle, seed, precision):
+    import torch
+    from isaac_bulk_hip import relax_device
+
+    rng = np.random.default_rng(seed)
+    h = rng.uniform(0.0, 1.5, size=shape)
+    dtype = torch.float64 if precision == "fp64" else torch.float32
+    npdt = np.float64 if precision == "fp64" else np.float32
+    t = _device(h.astype(npdt), dtype)
+    it, conv, last = relax_device(t, 0.05, 0.05, angle, 5000, 1e-8 if precision == "fp64" else 1e-6)
+    crit = float(np.tan(np.deg2rad(angle)))
+    o, oit, oconv, olast = oracle.relax_yx(h.astype(npdt), 0.05, 0.05, crit, 5000,
+                                           1e-8 if precision == "fp64" else 1e-6, dtype=npdt)
+    assert (it, conv) == (oit, oconv)
+    np.testing.assert_array_equal(t.cpu().numpy(), o)
+
+
+def test_relax_toe_701_iteration_count_and_field():
+    import torch
+    from isaac_bulk_hip import relax_device
+
+    nx, dx = 701, 0.05
+    drop = np.tan(np.deg2rad(30.0)) * dx
+    base = np.repeat((1.0 + np.arange(nx) * 0.95 * drop)[None, :], nx, axis=0)
+    toe = base.copy()
+    toe[346:355, 0] -= 0.10
+    t = _device(toe, torch.float64)
+    it, conv, _ = relax_device(t, dx, dx, 30.0, 5000, 1e-8)
+    assert (it, conv) == (183, True)
+    o, oit, _, _ = oracle.relax_yx(toe, dx, dx, float(np.tan(np.deg2rad(30.0))), 5000, 1e-8)
+    np.testing.assert_array_equal(t.cpu().numpy(), o)
+
+
+def test_relax_host_api_legacy_layout():
+    from isaac_bulk_hip import relax_critical_slope
+
+    h = np.zeros((31, 31))
+    h[15, 15] = 10.0
+    out, stats = relax_critical_slope(h, (0.1, 0.1), 30.0)
+    o, it, conv, _ = oracle.relax_legacy(h, 0.1, 0.1, float(np.tan(np.deg2rad(30.0))))
+    assert stats.converged and stats.iterations == it
+    np.testing.assert_array_equal(out, o)
+    assert stats.max_slope <= np.tan(np.deg2rad(30.0)) + 1e-7
+
+
+@pytest.mark.parametrize("density", [0.0, 0.3, 0.5, 0.6, 1.0])
+@pytest.mark.parametrize("connectivity", [4, 8])
+def test_label_mask_bit_exact(density, connectivity):
+    from isaac_bulk_hip import label_mask
+
+    rng = np.random.default_rng(int(density * 100) + connectivity)
+    for shape in ((64, 64), (1, 17), (130, 257)):
+        mask = (rng.random(shape) < density).astype(np.int32)
+        got = label_mask(mask, connectivity=connectivity)
+        np.testing.assert_array_equal(got, oracle.label(mask, connectivity))
+
+
+def test_label_large_spiral_single_component():
+    """A long serpentine path is one component whose root is its minimum index."""
+    from isaac_bulk_hip import label_mask
+
+    n = 301
+    mask = np.zeros((n, n), np.int32)
+    mask[::2, :] = 1
+    for r in range(1, n, 2):
+        mask[r, n - 1 if (r // 2) % 2 == 0 else 0] = 1
+    got = label_mask(mask, connectivity=4)
+    assert set(np.unique(got)) == {-1, 0}
+    np.testing.assert_array_equal(got, oracle.label(mask, 4))
+
+
+def _avalanche_state(seed=0, n=96, precision="fp64"):
+    from isaac_bulk_hip import DeviceBulkState, TerrainGrid
+    from isaac_bulk_hip.workloads import cut_face_terrain
+
+    b, h, _ = cut_face_terrain(n, seed=seed, face_deg=45.0)
+    grid = TerrainGrid(n, n, 0.05, 0.05)
+    state = DeviceBulkState(grid, b, precision=precision)
+    return state, grid, b
+
+
+def test_avalanche_label_and_mobilize_match_oracle():
+    from isaac_bulk_hip import (DeviceLargeAvalancheBridge, LargeAvalancheTransitionConfig,
+                                MaterialScenario, TerrainVolumeIntegrator)
+
+    state, grid, b = _avalanche_state()
+    material = MaterialScenario("cohesionless", 2200.0, 34.0, 0.0, 0.45, 38.0, 30.0, 0.35)
+    cfg = LargeAvalancheTransitionConfig(minimum_connected_cells=16, minimum_connected_area_m2=0.02,
+                                         minimum_mobilizable_volume_m3=0.001, persistence_time_s=0.01,
+                                         mobilization_depth_m=0.05)
+    bridge = DeviceLargeAvalancheBridge(state, material, grid, TerrainVolumeIntegrator.from_grid(grid), cfg)
+    res = bridge.observe_and_maybe_mobilize(0.02)
+    n = grid.nx
+    z = np.zeros((n, n))
+    w = TerrainVolumeIntegrator.from_grid(grid).vertex_weights_m2
+    o = oracle.avalanche_initialize(
+        z, b, z, z, z, w, z, z, np.zeros((n, n), np.int32), dx=0.05, dy=0.05,
+        tan_start=float(np.tan(np.deg2rad(38.0))), tan_stop=float(np.tan(np.deg2rad(30.0))),
+        density=2200.0, gravity=9.81, cohesion=0.0, mobilization_depth=0.05, dry=1e-8,
+        activity_speed=0.05, release=True)
+    a = state.runtime.arrays
+    np.testing.assert_array_equal(a["avalanche_unstable"].cpu().numpy().reshape(n, n), o["unstable"])
+    lab = oracle.label(o["unstable"], 8)
+    di, df = oracle.components(lab, z, b, np.zeros((n, n), np.int32), w, o["slope"], o["severity"],
+                               np.zeros((n, n), np.int32), 0.05, float(np.deg2rad(38.0)),
+                               float(np.deg2rad(30.0)))
+    assert res.unstable_cell_count == di[3] > 0
+    assert res.connected_region_count == di[0]
+    assert res.largest_connected_cell_count == di[1]
+    assert bridge.last_label.diag_i[2] == di[2]
+    assert res.largest_connected_area_m2 == pytest.approx(df[4], rel=1e-12)
+    assert res.largest_connected_mobilizable_volume_m3 == pytest.approx(df[5], rel=1e-12)
+    assert res.classification == bridge.LARGE_AVALANCHE_MOBILE_PATH
+    mob = oracle.mobilize(lab, di[2], z, b, z, w, np.zeros((n, n), np.int32), z, z, z, o["severity"],
+                          np.zeros((n, n), np.int32), z, z, z, 0.0, 0.05, 1e-8)
+    view = state.explicit_host_view(source="acceptance")
+    np.testing.assert_array_equal(view.b_eff_m, mob["resting"])
+    np.testing.assert_array_equal(view.H_mobile_m, mob["mobile"])
+    assert res.transferred_volume_m3 == pytest.approx(mob["diag"][7], rel=1e-12)
+
+
+def test_frontier_phase_matches_oracle():
+    import torch
+    from isaac_bulk_hip import CompactTileFrontier, HipCompactActiveEdgeOperator
+
+    rng = np.random.default_rng(4)
+    shape = (70, 90)
+    h = rng.uniform(0.0, 1.0, size=shape)
+    reached = (rng.random(shape) < 0.3).astype(np.int32)
+    w = rng.uniform(0.5, 1.5, size=shape)
+    op = HipCompactActiveEdgeOperator(shape, tile_size=32)
+    op.initialize(h, reached, w)
+    frontier = CompactTileFrontier(shape, 32)
+    ho, ro = h.copy(), reached.copy()
+    crit = float(np.tan(np.deg2rad(30.0))) * 0.05
+    for d, (di, dj) in enumerate(((1, 0), (0, 1), (1, 1), (1, -1))):
+        for phase in (0, 1):
+            batch = frontier.edge_batch(np.arange(frontier.tile_count), direction_index=d, di=di,
+                                        dj=dj, phase=phase)
+            r = op.run_phase(batch, critical_difference_m=crit, tolerance_m=1e-8)
+            ho, ro, moved, flags = oracle.frontier_relax(ho, w, ro, batch.a_i, batch.a_j, batch.b_i,
+                                                         batch.b_j, crit, 32, frontier.tile_shape[1])
+            uflags = oracle.frontier_scan(ho, ro, batch.tile_ids, batch.a_i, batch.a_j, batch.b_i,
+                                          batch.b_j, crit, 1e-8, frontier.tile_count)
+            assert r.moved_edge_count == moved
+            assert r.triggered_tile_ids.tolist() == np.flatnonzero(flags).tolist()
+            assert r.unstable_owner_tile_ids.tolist() == np.flatnonzero(uflags).tolist()
+    hd, rd = op.download_state()
+    np.testing.assert_array_equal(hd, ho)
+    np.testing.assert_array_equal(rd, ro.astype(bool))

@@ -1,0 +1,279 @@
+"""Pin the CPU oracle against the reference's own evidence and closed forms.
+
+Golden data: tests/golden/mobile_v2_cpu_contracts.json (copied from the
+reference's committed outputs by tests/golden/make_contract_fixture.py).
+"""
+
+from __future__ import annotations
+
+import json
+from pathlib import Path
+
+import numpy as np
+import pytest
+import scipy.ndimage
+
+import contract_cases
+import oracle
+
+GOLD = json.loads((Path(__file__).parent / "golden" / "mobile_v2_cpu_contracts.json").read_text())
+
+
+@pytest.mark.parametrize("name", sorted(contract_cases.cases()))
+def test_contract_case_scalars_bit_exact(name):
+    """substeps, max CFL, energies (16 digits) and mass residual match the reference run."""
+    b, h, q, dt = contract_cases.cases()[name]
+    g = GOLD["contract_cases"][name]
+    h1, qx1, qy1, rec = oracle.v2_step(b, h, q[..., 0], q[..., 1], dt)
+    assert rec["status"] == 0
+    assert rec["substeps"] == g["substeps"]
+    assert rec["maximum_cfl"] == g["maximum_CFL"]
+    e = g["energy"]
+    k0, g0, i0, e0 = oracle.energy_terms(h, q[..., 0], q[..., 1], b, 0.05, 0.05)
+    k1, g1, i1, e1 = oracle.energy_terms(h1, qx1, qy1, b, 0.05, 0.05)
+    assert (k0, g0, i0, e0) == (e["kinetic_before_j_per_density"], e["gravitational_before_j_per_density"],
+                                e["internal_before_j_per_density"], e["before_j_per_density"])
+    assert (k1, g1, i1, e1) == (e["kinetic_after_j_per_density"], e["gravitational_after_j_per_density"],
+                                e["internal_after_j_per_density"], e["after_j_per_density"])
+    assert rec["friction_dissipation"] == e["friction_dissipation_j_per_density"]
+    assert rec["external_work"] == e["external_work_j_per_density"]
+    m0 = float(np.sum(h, dtype=np.float64) * 0.05 * 0.05)
+    m1 = float(np.sum(h1, dtype=np.float64) * 0.05 * 0.05)
+    assert m1 - m0 == g["mass_residual_m3"]
+    assert float(np.min(h1, initial=0.0)) == g["minimum_h_m"]
+    # momentum accounting residuals are ~1e-17 sums whose last bits depend on
+    # the NumPy build's reduction order; they agree to 1e-16.
+    p0 = np.sum(q, axis=(0, 1)) * 0.05 * 0.05
+    p1 = np.sum(np.stack((qx1, qy1), -1), axis=(0, 1)) * 0.05 * 0.05
+    res = p1 - p0 - (rec["conservative_impulse"] + rec["friction_impulse"] + rec["external_impulse"])
+    assert np.max(np.abs(res - np.asarray(g["momentum_accounting_residual_m4_s"]))) <= 1.0e-16
+
+
+def test_valley_single_face_response_exact():
+    """One shared face (columns 20|21) reproduces the reference's uphill-impulse check."""
+    b, h, q, dt = contract_cases.cases()["VALLEY"]
+    vals = [0.0 - oracle.v2_face(b[r, 20], b[r, 21], h[r, 20], h[r, 21], q[r, 20, 0], q[r, 20, 1],
+                                 q[r, 21, 0], q[r, 21, 1], 0, dt / 0.05)[1] for r in range(41)]
+    assert float(np.mean(vals)) == GOLD["contract_cases"]["VALLEY"]["low_to_forward_face_delta_q_m2_s"]
+
+
+def test_lake_at_rest_is_balanced_in_the_interior():
+    """b + K h constant with q = 0: zero mass flux, and interior momentum updates cancel.
+
+    Closed walls have no boundary face (mobile_v2_reference.py:378-431), so only
+    interior cells are in exact hydrostatic balance.
+    """
+    K = 0.45
+    n = 17
+    y, x = np.indices((n, n))
+    b = 0.2 + 0.01 * np.sin(0.7 * x) * np.cos(0.5 * y)
+    h = (0.4 - b) / K
+    h1, qx1, qy1, rec = oracle.v2_step(b, h, np.zeros_like(h), np.zeros_like(h), 1.0e-3,
+                                       static_exit=False)
+    assert rec["substeps"] == 1
+    assert np.max(np.abs(h1 - h)) < 1e-15
+    assert np.max(np.abs(qx1[1:-1, 1:-1])) < 1e-15 and np.max(np.abs(qy1[1:-1, 1:-1])) < 1e-15
+
+
+def test_friction_decrement_closed_form():
+    """A uniform sheet on a flat bed loses mu*g*dt of speed per substep (no flux)."""
+    n = 12
+    b = np.zeros((n, n))
+    h = np.full((n, n), 0.1)
+    v = 0.5
+    qx = h * v
+    dt = 0.01
+    h1, qx1, qy1, rec = oracle.v2_step(b, h, qx, np.zeros_like(h), dt, static_exit=False)
+    # interior cells see no net flux (uniform state); speed drops by mu*g*sum(dt)
+    expect = v - 0.55 * 9.81 * dt
+    got = qx1[n // 2, n // 2] / h1[n // 2, n // 2]
+    assert abs(got - expect) < 1e-12
+
+
+def test_mass_invariance_and_mirror_symmetry():
+    n = 33
+    y, x = np.indices((n, n))
+    b = 0.1 + 0.002 * ((x - 16) ** 2)
+    h = 0.05 + 0.2 * np.exp(-((x - 16) ** 2 + (y - 16) ** 2) / 12.0)
+    h1, qx1, qy1, rec = oracle.v2_step(b, h, np.zeros_like(h), np.zeros_like(h), 0.05)
+    assert rec["substeps"] > 1
+    assert abs(np.sum(h1) - np.sum(h)) <= 1e-12 * np.sum(h)
+    np.testing.assert_allclose(h1, h1[:, ::-1], atol=1e-13)
+    np.testing.assert_allclose(h1, h1[::-1, :], atol=1e-13)
+    np.testing.assert_allclose(qx1, -qx1[:, ::-1], atol=1e-13)
+
+
+def test_static_cases_take_zero_substeps():
+    for name in ("STATIC_FLAT", "STATIC_SLOPE"):
+        b, h, q, dt = contract_cases.cases()[name]
+        h1, qx1, qy1, rec = oracle.v2_step(b, h, q[..., 0], q[..., 1], dt)
+        assert rec["static_exit"] and rec["substeps"] == 0
+        np.testing.assert_array_equal(h1, h)
+
+
+def test_entrain_deposit_gate_a_closed_forms():
+    """Gate A: 16x16 ones, entrain [0.03, 2.0] then deposit [0.01, 0.25] at (7,7),(7,8)."""
+    import sys
+    sys.path.insert(0, str(Path(__file__).resolve().parents[1] /
+                           "iron-soil-physical-simulation-algorithom-for-isaac-sim_amd"))
+    from isaac_bulk_hip.config import triangle_vertex_weights
+
+    n = 16
+    w = triangle_vertex_weights(n, n, 0.05, 0.05)
+    z = np.zeros((n, n))
+    b = np.ones((n, n))
+    m = np.zeros((n, n))
+    idx = np.asarray([7 * n + 7, 7 * n + 8])
+    b1, m1, vol = oracle.entrain(z.ravel(), b.ravel(), m.ravel(), w.ravel(), idx, [0.03, 2.0])
+    assert vol == GOLD["gate_a"]["entrained_volume_m3"]
+    assert np.max(np.abs((b1 + m1) - (b.ravel() + m.ravel()))) == 0.0
+    b2, m2, qx2, qy2, out = oracle.deposit(b1, m1, np.zeros(n * n), np.zeros(n * n), w.ravel(), idx,
+                                           [0.01, 0.25], 1370.0)
+    assert out[0] == GOLD["gate_a"]["deposited_volume_m3"]
+    assert list(out[1:]) == GOLD["gate_a"]["deposition_removed_momentum_kg_m_s"][:2]
+    assert np.max(np.abs((b2 + m2) - (b1 + m1))) == 0.0
+
+
+def _toe_and_whole(nx=701, dx=0.05):
+    critical_drop = np.tan(np.deg2rad(30.0)) * dx
+    baseline = np.repeat((1.0 + np.arange(nx) * 0.95 * critical_drop)[None, :], nx, axis=0)
+    toe = baseline.copy()
+    toe[346:355, 0] -= 0.10
+    rows, cols = np.indices((nx, nx))
+    whole = np.ones((nx, nx)) + 0.030 * ((rows + cols) & 1)
+    return toe, whole
+
+
+def test_relaxation_published_iteration_counts_701():
+    """docs/MINISLOPE_SPARSE_TILE_PERFORMANCE.md: toe 183, whole pile 2 (30 deg, tol 1e-8)."""
+    crit = float(np.tan(np.deg2rad(30.0)))
+    toe, whole = _toe_and_whole()
+    _, it, conv, _ = oracle.relax_yx(toe, 0.05, 0.05, crit, 5000, 1e-8)
+    assert (it, conv) == (183, True)
+    _, it2, conv2, _ = oracle.relax_yx(whole, 0.05, 0.05, crit, 5000, 1e-8)
+    assert (it2, conv2) == (2, True)
+
+
+def test_relaxation_invariants_spike_and_plane():
+    """test_slope_model.py:18-34: conservation, slope <= tan+1e-7, stable plane unchanged."""
+    import isaac_bulk_hip.relaxation as R
+
+    h = np.zeros((31, 31))
+    h[15, 15] = 10.0
+    crit = float(np.tan(np.deg2rad(30.0)))
+    out, it, conv, _ = oracle.relax_legacy(h, 0.1, 0.1, crit)
+    assert conv
+    assert abs(out.sum() - h.sum()) < 1e-10
+    assert R.max_neighbor_slope(out, (0.1, 0.1)) <= crit + 1e-7
+    assert out.min() >= 0.0
+    x = np.arange(20)[:, None] * 0.02
+    plane = np.broadcast_to(x, (20, 17)).copy()
+    out2, _, _, _ = oracle.relax_legacy(plane, 0.1, 0.1, crit)
+    np.testing.assert_array_equal(out2, plane)
+
+
+def test_relaxation_two_cell_pair_brute_force():
+    """A 2x2 block: the first sub-phase moves exactly half the excess of the (1,0) edges."""
+    crit = float(np.tan(np.deg2rad(30.0)))
+    h = np.array([[1.0, 0.2], [0.3, 0.2]])
+    out, it, conv, _ = oracle.relax_legacy(h, 0.1, 0.1, crit, max_iterations=1)
+    d = 1.0 - 0.3
+    ex = d - crit * 0.1
+    a = 1.0 - 0.5 * ex
+    # later sub-phases also move mass; replay them by hand in numpy
+    hh = h.copy()
+    dist = [0.1, 0.1, float(np.hypot(0.1, 0.1)), float(np.hypot(0.1, 0.1))]
+    views = [lambda z: (z[:-1, :], z[1:, :]), lambda z: (z[:, :-1], z[:, 1:]),
+             lambda z: (z[:-1, :-1], z[1:, 1:]), lambda z: (z[:-1, 1:], z[1:, :-1])]
+    for di, view in enumerate(views):
+        for phase in (0, 1):
+            A, B = view(hh)
+            coord = np.arange(A.shape[0])[:, None] if di != 1 else np.arange(A.shape[1])[None, :]
+            active = (coord & 1) == phase
+            diff = A - B
+            exc = np.maximum(np.abs(diff) - crit * dist[di], 0)
+            t = 0.5 * exc * np.sign(diff) * active
+            A -= t
+            B += t
+            if di == 0 and phase == 0:
+                assert hh[0, 0] == a
+    np.testing.assert_array_equal(out, hh)
+
+
+def test_float_relaxation_tracks_double():
+    rng = np.random.default_rng(5)
+    h = rng.uniform(0.0, 2.0, size=(40, 37))
+    crit = float(np.tan(np.deg2rad(34.0)))
+    o64, it64, _, _ = oracle.relax_legacy(h, 0.05, 0.05, crit, 2000, 1e-6)
+    o32, it32, _, _ = oracle.relax_legacy(h, 0.05, 0.05, crit, 2000, 1e-6, dtype=np.float32)
+    assert np.max(np.abs(o64 - o32)) < 1e-4
+    assert abs(it64 - it32) <= max(3, it64 // 10)
+
+
+def _canonical_scipy(mask, connectivity):
+    structure = np.ones((3, 3)) if connectivity == 8 else None
+    lab, n = scipy.ndimage.label(mask, structure=structure)
+    out = np.full(mask.shape, -1, dtype=np.int64)
+    flat = lab.ravel()
+    for k in range(1, n + 1):
+        idx = np.flatnonzero(flat == k)
+        out.ravel()[idx] = idx.min()
+    return out
+
+
+@pytest.mark.parametrize("density", [0.3, 0.5, 0.6])
+@pytest.mark.parametrize("connectivity", [4, 8])
+def test_oracle_labels_match_scipy(density, connectivity):
+    rng = np.random.default_rng(int(density * 10) + connectivity)
+    mask = rng.random((64, 57)) < density
+    lab = oracle.label(mask.astype(np.int32), connectivity)
+    np.testing.assert_array_equal(lab, _canonical_scipy(mask, connectivity))
+
+
+def test_yield_analytic_numbers_and_cohesive_plane_is_stable():
+    """test_large_avalanche_transition.py:387-410: drive 763.0389..., resistance 2096.1513..."""
+    rho, g, d = 2200.0, 9.81, 0.05
+    slope = np.deg2rad(45.0)
+    mag = np.tan(slope)
+    norm = np.sqrt(1.0 + mag * mag)
+    drive = rho * g * d * mag / norm
+    normal = rho * g * d / norm
+    resist = 1500.0 + normal * np.tan(np.deg2rad(38.0))
+    assert abs(drive - 763.0389275784034) < 1e-6
+    assert abs(resist - 2096.1513465821067) < 1e-6
+    n = 21
+    x = np.arange(n) * 0.05
+    plane = np.repeat((2.0 - np.tan(slope) * x)[None, :], n, axis=0)
+    z = np.zeros((n, n))
+    out = oracle.avalanche_initialize(
+        z, plane, z, z, z, np.ones((n, n)), z, z, np.zeros((n, n), np.int32), dx=0.05, dy=0.05,
+        tan_start=float(np.tan(np.deg2rad(38.0))), tan_stop=float(np.tan(np.deg2rad(30.0))),
+        density=rho, gravity=g, cohesion=1500.0, mobilization_depth=d, dry=1e-8,
+        activity_speed=0.05, release=True)
+    assert int(out["unstable"].sum()) == 0
+    out0 = oracle.avalanche_initialize(
+        z, plane, z, z, z, np.ones((n, n)), z, z, np.zeros((n, n), np.int32), dx=0.05, dy=0.05,
+        tan_start=float(np.tan(np.deg2rad(38.0))), tan_stop=float(np.tan(np.deg2rad(30.0))),
+        density=rho, gravity=g, cohesion=0.0, mobilization_depth=d, dry=1e-8,
+        activity_speed=0.05, release=True)
+    assert int(out0["unstable"].sum()) == n * n
+
+
+def test_disconnected_spikes_give_four_regions():
+    """test_large_avalanche_transition.py:307-328: four isolated spikes -> >= 4 regions."""
+    n = 21
+    resting = np.ones((n, n))
+    for r, c in ((4, 4), (4, 16), (16, 4), (16, 16)):
+        resting[r, c] += 0.2
+    z = np.zeros((n, n))
+    out = oracle.avalanche_initialize(
+        z, resting, z, z, z, np.ones((n, n)), z, z, np.zeros((n, n), np.int32), dx=0.05, dy=0.05,
+        tan_start=float(np.tan(np.deg2rad(38.0))), tan_stop=float(np.tan(np.deg2rad(30.0))),
+        density=2200.0, gravity=9.81, cohesion=0.0, mobilization_depth=0.05, dry=1e-8,
+        activity_speed=0.05, release=True)
+    lab = oracle.label(out["unstable"], 8)
+    di, df = oracle.components(lab, z, resting, np.zeros((n, n), np.int32), np.ones((n, n)),
+                               out["slope"], out["severity"], np.zeros((n, n), np.int32), 0.05,
+                               float(np.deg2rad(38.0)), float(np.deg2rad(30.0)))
+    assert di[0] >= 4
+    assert di[3] == int(out["unstable"].sum())
