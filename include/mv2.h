@@ -90,6 +90,7 @@ typedef struct mv2_step_record {
     double diag[10];
     double mass_quanta_before;        /* FP32: exact sum of mobile quanta (as double) */
     double mass_quanta_after;
+    double kernel_ms;                 /* device time of the step launch (HIP events) */
 } mv2_step_record;
 
 /* One step_resident call (warp_mobile_v2.py:148-245) or standalone step
@@ -205,6 +206,32 @@ int mv2_avalanche_mobilize(mv2_context *ctx, const mv2_avalanche_config *cfg,
  * labels = min linear index of the component, -1 off-mask. */
 int mv2_label_mask(mv2_context *ctx, const int32_t *mask, int32_t *labels, int ny, int nx,
                    int connectivity);
+
+/* ---- learned avalanche warm start (no reference counterpart: the reference
+ *      has no learned model; the plain relaxation above is its judge).
+ *      Float/BF16 device buffers, NHWC activations; see csrc/mv2_surrogate.hip. ---- */
+int mv2_nn_features(mv2_context *ctx, const float *b, const float *h, int B, int H, int W,
+                    double dx, double dy, float *feat);
+int mv2_nn_conv1(mv2_context *ctx, const float *feat, const float *w1, const float *b1,
+                 int B, int H, int W, int dil, void *y);
+int mv2_nn_conv_mfma(mv2_context *ctx, const void *x, int cx, const void *w, const float *bias,
+                     const void *mask, void *y, int B, int H, int W, int dil, int relu, int mode);
+int mv2_nn_pack_weights(mv2_context *ctx, const float *w, int cout, int cin, int cxf, int cxb,
+                        void *fwd, void *bwd);
+int mv2_nn_expand_grad(mv2_context *ctx, const float *g, int64_t npix, void *out16);
+int mv2_nn_weight_grad(mv2_context *ctx, const void *dz, int zmode, const void *x, int xmode,
+                       int B, int H, int W, int dil, int cout, int cin, float *dw, float *db);
+int mv2_nn_mse(mv2_context *ctx, const float *pred, const float *target, int64_t n, float *grad,
+               double *loss_out);
+int mv2_nn_adam(mv2_context *ctx, float *param, const float *grad, float *m, float *v, int n,
+                double lr, double beta1, double beta2, double eps, int step);
+int mv2_nn_project_w1(mv2_context *ctx, float *w1);
+int mv2_nn_apply_delta(mv2_context *ctx, float *b, const float *delta, int64_t n, double eps);
+/* batched relaxation of independent FP32 patches (<= 4096 cells each), one
+ * workgroup per patch, same schedule as mv2_relax; iterations per patch out */
+int mv2_relax_patches(mv2_context *ctx, float *h, int npatch, int H, int W,
+                      const double crit_diff[4], int max_iterations, double tolerance,
+                      int *iterations);
 
 #ifdef __cplusplus
 }

@@ -97,6 +97,10 @@ extern "C" int mv2_create(int device, mv2_context **out)
     if (int rc = ensure_scratch(ctx, ctx->sync, 4096)) { delete ctx; return rc; }
     if (int rc = ensure_scratch(ctx, ctx->result, 4096)) { delete ctx; return rc; }
     if (int rc = ensure_scratch(ctx, ctx->host_result, 4096, true)) { delete ctx; return rc; }
+    if (hipEventCreate(&ctx->ev_start) != hipSuccess || hipEventCreate(&ctx->ev_stop) != hipSuccess) {
+        delete ctx;
+        return set_error(nullptr, MV2_EDEVICE, "hipEventCreate failed");
+    }
     *out = ctx;
     return MV2_OK;
 }
@@ -115,7 +119,11 @@ extern "C" void mv2_destroy(mv2_context *ctx)
     release(ctx->result);
     release(ctx->relax_alt);
     release(ctx->label_aux);
+    release(ctx->nn_partials);
+    release(ctx->nn_loss);
     release(ctx->host_result, true);
+    if (ctx->ev_start) hipEventDestroy(ctx->ev_start);
+    if (ctx->ev_stop) hipEventDestroy(ctx->ev_stop);
     delete ctx;
 }
 

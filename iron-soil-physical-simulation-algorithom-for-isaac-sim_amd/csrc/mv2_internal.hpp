@@ -47,6 +47,9 @@ struct mv2_context {
     mv2::Scratch relax_alt;               // ping-pong height of the relaxation
     mv2::Scratch host_result;             // pinned host staging
     mv2::Scratch label_aux;               // labelling partials
+    mv2::Scratch nn_partials;             // weight-gradient partials of the surrogate
+    mv2::Scratch nn_loss;                 // loss / projection partials
+    hipEvent_t ev_start = nullptr, ev_stop = nullptr;   // device timing of step launches
 };
 
 namespace mv2 {
@@ -121,20 +124,23 @@ __device__ inline void block_reduce(T (&v)[NV], T *smem)
 
 // ---- grid barrier for cooperatively launched (co-resident) grids ---------------
 // `counter` starts at 0 for the launch; barrier k completes when the counter
-// reaches k * gridDim.x.  Agent-scope release/acquire fences make the writes of
-// every XCD visible before any workgroup proceeds.
+// reaches k * gridDim.x.  One agent-scope release fence before arriving and one
+// acquire fence after the wait make every XCD's writes visible.  The wait
+// itself polls with relaxed loads: an acquire load invalidates the XCD's L2 on
+// every poll, and a thousand spinning workgroups would keep the L2 of all
+// other workgroups cold.
 __device__ inline void grid_barrier(unsigned *counter, unsigned &epoch)
 {
     __syncthreads();
     epoch += 1;
     if (threadIdx.x == 0) {
         const unsigned target = epoch * gridDim.x;
-        __threadfence();
-        __hip_atomic_fetch_add(counter, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-        while (__hip_atomic_load(counter, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < target) {
-            __builtin_amdgcn_s_sleep(1);
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+        __hip_atomic_fetch_add(counter, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        while (__hip_atomic_load(counter, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
+            __builtin_amdgcn_s_sleep(2);
         }
-        __threadfence();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
     }
     __syncthreads();
 }

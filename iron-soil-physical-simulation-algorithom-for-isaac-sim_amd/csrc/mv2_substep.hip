@@ -378,6 +378,225 @@ __device__ inline CellOut cell32(const V2Args<F32P> &a, int src, int r, int c, d
     return out;
 }
 
+// ---------------------------------------------------------------------------
+// FP32 tile substep: each shared face is evaluated once per tile from LDS.
+// A 16 x 32 tile stages an (18 x 34) halo block of cells (depth, quanta,
+// momentum, wet-masked velocity), computes its 16 x 33 x-faces and 17 x 32
+// y-faces into LDS, then updates its cells.  Faces on a tile border are
+// computed by both neighbouring tiles from bit-identical operands, so both
+// sides move the same integer number of quanta.
+// ---------------------------------------------------------------------------
+constexpr int kTH = 16, kTW = 32;
+constexpr int kLH = kTH + 2, kLW = kTW + 2;
+constexpr int kFX = kTH * (kTW + 1);
+constexpr int kFY = (kTH + 1) * kTW;
+
+struct Tile32 {
+    float b[kLH * kLW];
+    float h[kLH * kLW];
+    int hq[kLH * kLW];
+    float qx[kLH * kLW];
+    float qy[kLH * kLW];
+    float vx[kLH * kLW];
+    float vy[kLH * kLW];
+    int fxq[kFX];
+    float fxl[kFX], fxr[kFX], fxt[kFX];
+    int fyq[kFY];
+    float fyl[kFY], fyr[kFY], fyt[kFY];
+};
+
+// face from cell-centred velocities (zero for dry cells); the reconstructed
+// velocity is the cell velocity, zeroed where the reconstruction is dry
+template <int NORMAL>
+__device__ inline Face32 face32v(float bl, float br, int hql, int hqr, float hl, float hr,
+                                 float vxl, float vyl, float vxr, float vyr,
+                                 const Consts32 &k, float factor, int clamp_quarter)
+{
+    float head_l = bl + k.K * hl;
+    float head_r = br + k.K * hr;
+    float b_face = fmaxf(bl, br);
+    float hs_l = fmaxf((head_l - b_face) * k.invK, 0.0f);
+    float hs_r = fmaxf((head_r - b_face) * k.invK, 0.0f);
+    bool wl = hs_l > k.dry, wr = hs_r > k.dry;
+    float vnl = wl ? (NORMAL == 0 ? vxl : vyl) : 0.0f, vtl = wl ? (NORMAL == 0 ? vyl : vxl) : 0.0f;
+    float vnr = wr ? (NORMAL == 0 ? vxr : vyr) : 0.0f, vtr = wr ? (NORMAL == 0 ? vyr : vxr) : 0.0f;
+    float qnl = hs_l * vnl, qtl = hs_l * vtl;
+    float qnr = hs_r * vnr, qtr = hs_r * vtr;
+    float psl = (k.c * hs_l) * hs_l, psr = (k.c * hs_r) * hs_r;
+    float fln = qnl * vnl + psl, frn = qnr * vnr + psr;
+    float flt = qnl * vtl, frt = qnr * vtr;
+    float speed = fmaxf(fabsf(vnl) + __builtin_sqrtf(k.Kg * hs_l),
+                        fabsf(vnr) + __builtin_sqrtf(k.Kg * hs_r));
+    float hsp = 0.5f * speed;
+    float fh = 0.5f * (qnl + qnr) - hsp * (hs_r - hs_l);
+    float fn = 0.5f * (fln + frn) - hsp * (qnr - qnl);
+    float ft = 0.5f * (flt + frt) - hsp * (qtr - qtl);
+    float pl = (k.c * hl) * hl, pr = (k.c * hr) * hr;
+    Face32 o;
+    int q = (int)rintf(fh * factor * (float)kInvQuantum);
+    if (clamp_quarter) {
+        if (q > 0) q = min(q, hql >> 2);
+        else q = max(q, -(hqr >> 2));
+    }
+    o.h = q;
+    float nl = (fn + (pl - psl)) * factor;
+    float nr = (fn + (pr - psr)) * factor;
+    float tt = ft * factor;
+    if (NORMAL == 0) {
+        o.qxl = nl; o.qyl = tt; o.qxr = nr; o.qyr = tt;
+    } else {
+        o.qxl = tt; o.qyl = nl; o.qxr = tt; o.qyr = nr;
+    }
+    return o;
+}
+
+__device__ inline Consts32 consts32(const mv2_step_config &cfg)
+{
+    Consts32 k;
+    k.K = (float)cfg.earth_pressure;
+    k.invK = (float)(1.0 / cfg.earth_pressure);
+    k.Kg = (float)(cfg.earth_pressure * cfg.gravity);
+    k.c = (float)(0.5 * cfg.earth_pressure * cfg.gravity);
+    k.dry = (float)cfg.dry_tolerance;
+    return k;
+}
+
+// tile t (row-major over tiles_x) -> substep update of its cells
+__device__ void tile_substep32(const V2Args<F32P> &a, Tile32 &T, int src, int t, int tiles_x,
+                               double sub_dt, Acc<F32P> &acc, double &wmax, double &hmin, int &ovf)
+{
+    const int ny = a.ny, nx = a.nx;
+    const int r0 = (t / tiles_x) * kTH, c0 = (t % tiles_x) * kTW;
+    const Consts32 k = consts32(a.cfg);
+    const float Qf = (float)kQuantum;
+    const float *gb = a.b;
+    const int32_t *gh = a.h[src];
+    const float *gqx = a.qx[src];
+    const float *gqy = a.qy[src];
+    for (int e = threadIdx.x; e < kLH * kLW; e += blockDim.x) {
+        const int lr = e / kLW, lc = e - lr * kLW;
+        const int gr = r0 - 1 + lr, gc = c0 - 1 + lc;
+        float b = 0.0f, h = 0.0f, qx = 0.0f, qy = 0.0f, vx = 0.0f, vy = 0.0f;
+        int hq = 0;
+        if (gr >= 0 && gr < ny && gc >= 0 && gc < nx) {
+            const int64_t i = (int64_t)gr * nx + gc;
+            b = gb[i];
+            hq = gh[i];
+            qx = gqx[i];
+            qy = gqy[i];
+            h = (float)hq * Qf;
+            if (h > k.dry) {
+                float inv = rcpf(h);
+                vx = qx * inv;
+                vy = qy * inv;
+            }
+        }
+        T.b[e] = b; T.h[e] = h; T.hq[e] = hq; T.qx[e] = qx; T.qy[e] = qy; T.vx[e] = vx; T.vy[e] = vy;
+    }
+    __syncthreads();
+    const float fxf = (float)(sub_dt / a.dx), fyf = (float)(sub_dt / a.dy);
+    const int cq = a.clamp_quarter;
+    for (int e = threadIdx.x; e < kFX + kFY; e += blockDim.x) {
+        if (e < kFX) {
+            const int lr = e / (kTW + 1), kk = e - lr * (kTW + 1);
+            const int gr = r0 + lr, gcl = c0 - 1 + kk;
+            if (gr >= ny || gcl < 0 || gcl + 1 >= nx) continue;
+            const int L = (lr + 1) * kLW + kk, R = L + 1;
+            Face32 f = face32v<0>(T.b[L], T.b[R], T.hq[L], T.hq[R], T.h[L], T.h[R], T.vx[L], T.vy[L],
+                                  T.vx[R], T.vy[R], k, fxf, cq);
+            T.fxq[e] = f.h; T.fxl[e] = f.qxl; T.fxr[e] = f.qxr; T.fxt[e] = f.qyl;
+        } else {
+            const int ey = e - kFX;
+            const int kk = ey / kTW, lc = ey - kk * kTW;
+            const int grl = r0 - 1 + kk, gc = c0 + lc;
+            if (gc >= nx || grl < 0 || grl + 1 >= ny) continue;
+            const int L = kk * kLW + lc + 1, R = L + kLW;
+            Face32 f = face32v<1>(T.b[L], T.b[R], T.hq[L], T.hq[R], T.h[L], T.h[R], T.vx[L], T.vy[L],
+                                  T.vx[R], T.vy[R], k, fyf, cq);
+            T.fyq[ey] = f.h; T.fyl[ey] = f.qyl; T.fyr[ey] = f.qyr; T.fyt[ey] = f.qxl;
+        }
+    }
+    __syncthreads();
+    const int dst = src ^ 1;
+    for (int e = threadIdx.x; e < kTH * kTW; e += blockDim.x) {
+        const int ir = e / kTW, ic = e - ir * kTW;
+        const int gr = r0 + ir, gc = c0 + ic;
+        if (gr >= ny || gc >= nx) continue;
+        const int L = (ir + 1) * kLW + ic + 1;
+        long long dq = 0;
+        float dqx = 0.0f, dqy = 0.0f;
+        if (gc + 1 < nx) {
+            const int f = ir * (kTW + 1) + ic + 1;
+            dq -= T.fxq[f]; dqx = dqx - T.fxl[f]; dqy = dqy - T.fxt[f];
+        }
+        if (gc > 0) {
+            const int f = ir * (kTW + 1) + ic;
+            dq += T.fxq[f]; dqx = dqx + T.fxr[f]; dqy = dqy + T.fxt[f];
+        }
+        if (gr + 1 < ny) {
+            const int f = (ir + 1) * kTW + ic;
+            dq -= T.fyq[f]; dqx = dqx - T.fyt[f]; dqy = dqy - T.fyl[f];
+        }
+        if (gr > 0) {
+            const int f = ir * kTW + ic;
+            dq += T.fyq[f]; dqx = dqx + T.fyt[f]; dqy = dqy + T.fyr[f];
+        }
+        const int hqi = T.hq[L];
+        const float qxi = T.qx[L], qyi = T.qy[L];
+        long long hn = (long long)hqi + dq;
+        ovf |= hn > 2147483647LL;
+        hmin = npmin(hmin, (double)hn * kQuantum);
+        int hq = (int)(hn < 0 ? 0 : (hn > 2147483647LL ? 2147483647LL : hn));
+        float qxn = qxi + dqx, qyn = qyi + dqy;
+        acc.diag[0] += (double)(qxn - qxi);
+        acc.diag[1] += (double)(qyn - qyi);
+        float depth = (float)hq * Qf;
+        float qax = 0.0f, qay = 0.0f, wave_v = 0.0f;
+        if (depth > k.dry) {
+            const float dt = (float)sub_dt;
+            const float ax = (float)a.cfg.external_ax, ay = (float)a.cfg.external_ay;
+            float inv = rcpf(depth);
+            float vbx = qxn * inv, vby = qyn * inv;
+            float vex = vbx + ax * dt, vey = vby + ay * dt;
+            float speed = __builtin_sqrtf(vex * vex + vey * vey);
+            float decrement = (float)(a.cfg.basal_friction * a.cfg.gravity) * dt;
+            float factor = speed > 1.0e-15f ? fmaxf(0.0f, 1.0f - decrement * rcpf(speed)) : 0.0f;
+            float vax = vex * factor, vay = vey * factor;
+            qax = depth * vax;
+            qay = depth * vay;
+            wave_v = speed * factor;
+            double dd = (double)depth;
+            acc.diag[2] += dd * (double)(vex - vbx);
+            acc.diag[3] += dd * (double)(vey - vby);
+            acc.diag[6] += dd * (0.5 * (double)(vbx + vex) * a.cfg.external_ax +
+                                 0.5 * (double)(vby + vey) * a.cfg.external_ay) * sub_dt;
+            acc.diag[4] += (double)qax - dd * (double)vex;
+            acc.diag[5] += (double)qay - dd * (double)vey;
+            acc.diag[7] += dd * ((double)speed * speed - ((double)vax * vax + (double)vay * vay));
+        }
+        acc.diag[8] += (double)((depth > k.dry ? qxn : 0.0f) - qxi);
+        acc.diag[9] += (double)((depth > k.dry ? qyn : 0.0f) - qyi);
+        const int64_t i = (int64_t)gr * nx + gc;
+        a.h[dst][i] = hq;
+        a.qx[dst][i] = qax;
+        a.qy[dst][i] = qay;
+        wmax = npmax(wmax, (double)(wave_v + __builtin_sqrtf(k.Kg * depth)));
+    }
+    __syncthreads();
+}
+
+// Workgroups are dispatched round-robin over the 8 XCDs.  Renumber them so
+// that the workgroups of one XCD own one contiguous band of tiles: halos
+// then mostly hit that XCD's L2.
+__device__ inline void xcd_tile_range(int tiles, int &first, int &last)
+{
+    const int G = gridDim.x, b = blockIdx.x;
+    const int x = b % 8, j = b / 8;
+    const int pb = x * (G / 8) + min(x, G % 8) + j;
+    first = (int)((long long)pb * tiles / G);
+    last = (int)((long long)(pb + 1) * tiles / G);
+}
+
 template <class P>
 __device__ inline CellOut cell_update(const V2Args<P> &a, int src, int r, int c, double sub_dt, Acc<P> &acc);
 template <>
@@ -651,13 +870,23 @@ __global__ void __launch_bounds__(kBlock) v2_step_kernel(V2Args<P> a)
         double wmax[1] = {0.0};
         double hmin = 0.0;
         int ovf = 0;
-        for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
-             i += (int64_t)gridDim.x * blockDim.x) {
-            int r = (int)(i / nx), c = (int)(i - (int64_t)r * nx);
-            CellOut o = cell_update<P>(a, src, r, c, sub_dt, acc);
-            wmax[0] = npmax(wmax[0], o.wave);
-            hmin = npmin(hmin, o.hmin);
-            ovf |= o.overflow;
+        if constexpr (P::kQuant) {
+            __shared__ Tile32 tile;
+            const int tiles_x = (nx + kTW - 1) / kTW;
+            const int tiles = tiles_x * ((ny + kTH - 1) / kTH);
+            int first, last;
+            xcd_tile_range(tiles, first, last);
+            for (int t = first; t < last; t++)
+                tile_substep32(a, tile, src, t, tiles_x, sub_dt, acc, wmax[0], hmin, ovf);
+        } else {
+            for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+                 i += (int64_t)gridDim.x * blockDim.x) {
+                int r = (int)(i / nx), c = (int)(i - (int64_t)r * nx);
+                CellOut o = cell_update<P>(a, src, r, c, sub_dt, acc);
+                wmax[0] = npmax(wmax[0], o.wave);
+                hmin = npmin(hmin, o.hmin);
+                ovf |= o.overflow;
+            }
         }
         block_reduce<double, 1, 1>(wmax, smem);
         double hm[1] = {hmin};
@@ -727,6 +956,7 @@ static int launch_step(mv2_context *ctx, const mv2_step_config *cfg, mv2_step_re
     if (per_cu < 1) return set_error(ctx, MV2_EDEVICE, "[MobileV2] kernel cannot be resident");
     int grid = per_cu * ctx->cu_count;
     int need = grid_for(n);
+    if (P::kQuant) need = ((ctx->nx + kTW - 1) / kTW) * ((ctx->ny + kTH - 1) / kTH);
     if (grid > need) grid = need;
     if (grid < 1) grid = 1;
     if (int rc = ensure_scratch(ctx, ctx->partials, sizeof(Partial) * (size_t)grid)) return rc;
@@ -751,12 +981,17 @@ static int launch_step(mv2_context *ctx, const mv2_step_config *cfg, mv2_step_re
                        (int)(sizeof(SyncBlock) / 4));
     hipLaunchKernelGGL(zero_record, dim3(1), dim3(64), 0, ctx->stream, a.rec);
     void *args[] = {&a};
+    MV2_HIP(ctx, hipEventRecord(ctx->ev_start, ctx->stream));
     MV2_HIP(ctx, hipLaunchCooperativeKernel((const void *)v2_step_kernel<P>, dim3(grid), dim3(kBlock),
                                             args, 0, ctx->stream));
+    MV2_HIP(ctx, hipEventRecord(ctx->ev_stop, ctx->stream));
     MV2_HIP(ctx, hipMemcpyAsync(ctx->host_result.ptr, a.rec, sizeof(mv2_step_record),
                                 hipMemcpyDeviceToHost, ctx->stream));
     MV2_HIP(ctx, hipStreamSynchronize(ctx->stream));
     *out = *(const mv2_step_record *)ctx->host_result.ptr;
+    float ms = 0.0f;
+    MV2_HIP(ctx, hipEventElapsedTime(&ms, ctx->ev_start, ctx->ev_stop));
+    out->kernel_ms = ms;
     return MV2_OK;
 }
 

@@ -1,0 +1,609 @@
+// Learned avalanche warm start: a 5-layer dilated 3x3 conv-net (3 -> 32 -> 32
+// -> 32 -> 32 -> 1 channels, dilations 1, 2, 4, 8, 16: receptive field 63
+// cells) predicting the change of b_eff under the angle-of-repose relaxation,
+// plus the kernels that train it and the batched patch relaxation that makes
+// its training pairs.
+//
+// * Layer 1 (3 input channels) runs in FP32 on the VALU: terrain heights of a
+//   few metres need FP32 to resolve millimetre differences, and its taps on
+//   the b_eff channel are constrained to sum to zero, so the network's output
+//   is exactly invariant to adding a constant to the terrain.
+// * Layers 2-5 run as implicit GEMMs on the matrix cores:
+//   v_mfma_f32_32x32x16_bf16, BF16 activations/weights, FP32 accumulation.
+//   One wave computes 32 consecutive pixels of one row x 32 output channels;
+//   K = 9 taps x input channels, 16 per MFMA.  Activations are NHWC so each
+//   lane's 8 K-values of one pixel are one 16-byte load.
+// * Backward-data reuses the same MFMA kernel with flipped, transposed
+//   weights; the weight gradient is an LDS-tiled FP32 reduction with
+//   per-workgroup partials combined in fixed order (deterministic).
+#include "mv2_internal.hpp"
+
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+namespace mv2 {
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+
+__device__ inline __bf16 to_bf16(float v) { return (__bf16)v; }
+__device__ inline float from_bf16(__bf16 v) { return (float)v; }
+
+// ---------------------------------------------------------------------------
+// input features: f0 = b_eff, f1 = h_mobile, f2 = |grad(b_eff + h_mobile)|
+// (centred differences inside, one-sided at the edges), FP32 [B][H][W][4]
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(kBlock) features_kernel(const float *b, const float *h, int B, int H, int W,
+                                                          float dx, float dy, float *feat)
+{
+    const int64_t total = (int64_t)B * H * W;
+    for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < total;
+         p += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t n = p / ((int64_t)H * W);
+        const int rem = (int)(p - n * H * W);
+        const int y = rem / W, x = rem - y * W;
+        auto Hf = [&](int yy, int xx) {
+            const int64_t i = n * H * W + (int64_t)yy * W + xx;
+            return b[i] + (h ? h[i] : 0.0f);
+        };
+        const int xl = x > 0 ? x - 1 : 0, xr = x + 1 < W ? x + 1 : W - 1;
+        const int yd = y > 0 ? y - 1 : 0, yu = y + 1 < H ? y + 1 : H - 1;
+        const float gx = (Hf(y, xr) - Hf(y, xl)) / ((float)(xr - xl) * dx);
+        const float gy = (Hf(yu, x) - Hf(yd, x)) / ((float)(yu - yd) * dy);
+        feat[p * 4 + 0] = b[p];
+        feat[p * 4 + 1] = h ? h[p] : 0.0f;
+        feat[p * 4 + 2] = __builtin_sqrtf(gx * gx + gy * gy);
+        feat[p * 4 + 3] = 0.0f;
+    }
+}
+
+// layer 1: FP32 3x3 conv over 3 feature channels, bias, ReLU -> BF16 NHWC32
+// w1: [9][32][4] FP32 (channel 3 unused)
+__global__ void __launch_bounds__(kBlock) conv1_fp32_kernel(const float *feat, const float *w1, const float *b1,
+                                                            int B, int H, int W, int dil, __bf16 *y)
+{
+    __shared__ float ws[9 * 32 * 4];
+    __shared__ float bs[32];
+    for (int i = threadIdx.x; i < 9 * 32 * 4; i += blockDim.x) ws[i] = w1[i];
+    if (threadIdx.x < 32) bs[threadIdx.x] = b1[threadIdx.x];
+    __syncthreads();
+    const int64_t total = (int64_t)B * H * W;
+    for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < total;
+         p += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t n = p / ((int64_t)H * W);
+        const int rem = (int)(p - n * H * W);
+        const int yc = rem / W, xc = rem - yc * W;
+        float f[9][3];
+        for (int t = 0; t < 9; t++) {
+            const int yy = yc + (t / 3 - 1) * dil, xx = xc + (t % 3 - 1) * dil;
+            const bool in = yy >= 0 && yy < H && xx >= 0 && xx < W;
+            const int64_t q = n * H * W + (int64_t)yy * W + xx;
+            for (int c = 0; c < 3; c++) f[t][c] = in ? feat[q * 4 + c] : 0.0f;
+        }
+        // zero padding of the b_eff channel would break the constant-shift
+        // invariance at the domain edge: replicate the centre value instead
+        for (int t = 0; t < 9; t++) {
+            const int yy = yc + (t / 3 - 1) * dil, xx = xc + (t % 3 - 1) * dil;
+            if (!(yy >= 0 && yy < H && xx >= 0 && xx < W)) f[t][0] = feat[p * 4 + 0];
+        }
+        for (int co = 0; co < 32; co++) {
+            float acc = bs[co];
+            for (int t = 0; t < 9; t++)
+                for (int c = 0; c < 3; c++) acc += ws[(t * 32 + co) * 4 + c] * f[t][c];
+            y[p * 32 + co] = to_bf16(acc > 0.0f ? acc : 0.0f);
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// MFMA implicit-GEMM 3x3 conv.  x: BF16 NHWC with CX = 16*KS channels;
+// w: BF16 [9][32][CX] (output channel major, input channel innermost).
+// mode 0: BF16 NHWC32 output (+bias, optional ReLU, optional multiply by
+//         (mask > 0) where mask is BF16 NHWC32)
+// mode 1: FP32 single-channel output = channel 0 (+bias[0])
+// Grid: (ceil(W/32), ceil(H/4), B); 4 waves, one row each.
+// ---------------------------------------------------------------------------
+struct ConvArgs {
+    const __bf16 *x;
+    const __bf16 *w;
+    const float *bias;
+    const __bf16 *mask;
+    void *y;
+    int B, H, W, dil;
+    int relu, mode;
+};
+
+template <int KS>
+__global__ void __launch_bounds__(kBlock) conv_mfma_kernel(ConvArgs a)
+{
+    constexpr int CX = 16 * KS;
+    constexpr int VPR = CX / 8;                    // bf16x8 vectors per (tap, co) row
+    __shared__ bf16x8 wl[9 * 32 * VPR];
+    const bf16x8 *wg = reinterpret_cast<const bf16x8 *>(a.w);
+    for (int i = threadIdx.x; i < 9 * 32 * VPR; i += blockDim.x) wl[i] = wg[i];
+    __syncthreads();
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int x0 = blockIdx.x * 32;
+    const int yrow = blockIdx.y * 4 + wave;
+    const int n = blockIdx.z;
+    if (yrow >= a.H) return;
+    const int m = lane & 31, half = lane >> 5;
+    const int x = x0 + m;
+    f32x16 acc;
+    for (int r = 0; r < 16; r++) acc[r] = 0.0f;
+    const bf16x8 zero = {};
+    for (int tap = 0; tap < 9; tap++) {
+        const int yy = yrow + (tap / 3 - 1) * a.dil, xx = x + (tap % 3 - 1) * a.dil;
+        const bool in = x < a.W && yy >= 0 && yy < a.H && xx >= 0 && xx < a.W;
+        const bf16x8 *xp = reinterpret_cast<const bf16x8 *>(
+            a.x + (((int64_t)n * a.H + (in ? yy : 0)) * a.W + (in ? xx : 0)) * CX);
+#pragma unroll
+        for (int ks = 0; ks < KS; ks++) {
+            const bf16x8 av = in ? xp[ks * 2 + half] : zero;
+            const bf16x8 bv = wl[(tap * 32 + m) * VPR + ks * 2 + half];
+            acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(av, bv, acc, 0, 0, 0);
+        }
+    }
+    const int co = lane & 31;
+    const float bias = a.bias ? a.bias[co] : 0.0f;
+#pragma unroll
+    for (int r = 0; r < 16; r++) {
+        const int mm = (r / 4) * 8 + half * 4 + (r % 4);
+        const int px = x0 + mm;
+        if (px >= a.W) continue;
+        const int64_t p = ((int64_t)n * a.H + yrow) * a.W + px;
+        float v = acc[r] + bias;
+        if (a.mode == 1) {
+            if (co == 0) reinterpret_cast<float *>(a.y)[p] = v;
+            continue;
+        }
+        if (a.relu) v = v > 0.0f ? v : 0.0f;
+        if (a.mask && !(from_bf16(a.mask[p * 32 + co]) > 0.0f)) v = 0.0f;
+        reinterpret_cast<__bf16 *>(a.y)[p * 32 + co] = to_bf16(v);
+    }
+}
+
+// pack FP32 master weights of an MFMA layer into BF16 forward / backward forms.
+// master w: [9][cout][cin] FP32 with cout, cin <= 32.
+//   fwd  [9][32][CXF]: fwd[t][co][ci] = w[t][co][ci]
+//   bwd  [9][32][CXB]: bwd[t][ci][co] = w[8 - t][co][ci]   (for dX = conv(dZ, bwd))
+__global__ void pack_weights_kernel(const float *w, int cout, int cin, int cxf, int cxb, __bf16 *fwd,
+                                    __bf16 *bwd)
+{
+    const int total = 9 * 32 * 32;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < total; i += gridDim.x * blockDim.x) {
+        const int t = i / 1024, r = (i / 32) % 32, c = i % 32;
+        if (fwd && c < cxf) fwd[(t * 32 + r) * cxf + c] = to_bf16(r < cout && c < cin ? w[(t * cout + r) * cin + c] : 0.0f);
+        if (bwd && c < cxb)
+            bwd[(t * 32 + r) * cxb + c] = to_bf16(c < cout && r < cin ? w[((8 - t) * cout + c) * cin + r] : 0.0f);
+    }
+}
+
+// single-channel FP32 gradient -> BF16 NHWC16 (channel 0 = value, others 0)
+__global__ void expand_grad_kernel(const float *g, int64_t npix, __bf16 *out16)
+{
+    for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < npix; p += (int64_t)gridDim.x * blockDim.x) {
+        bf16x8 v0 = {}, v1 = {};
+        v0[0] = to_bf16(g[p]);
+        reinterpret_cast<bf16x8 *>(out16)[p * 2] = v0;
+        reinterpret_cast<bf16x8 *>(out16)[p * 2 + 1] = v1;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// weight gradient: dW[t][co][ci] = sum_p dZ[p][co] * X[p + off(t)][ci],
+// db[co] = sum_p dZ[p][co].  dZ: BF16 NHWC32 (zmode 0) or FP32 channel-0
+// (zmode 1, cout = 1); X: BF16 NHWC32 (xmode 0) or FP32 features stride 4
+// (xmode 1, cin = 3).  One workgroup = one tap x one chunk of pixels; 256
+// threads own 4 (co, ci) pairs each; 64-pixel slabs staged in LDS.
+// partials: [nchunks][9][32][32] (+ [nchunks][32] bias, tap 0 only)
+// ---------------------------------------------------------------------------
+constexpr int kGradSlab = 64;
+
+__global__ void __launch_bounds__(kBlock) weight_grad_kernel(const void *dz, int zmode, const void *xin, int xmode,
+                                                             int B, int H, int W, int dil, int64_t chunk,
+                                                             float *partials, float *bias_partials)
+{
+    __shared__ float zs[kGradSlab][33];
+    __shared__ float xs[kGradSlab][33];
+    const int tap = blockIdx.y;
+    const int dyo = (tap / 3 - 1) * dil, dxo = (tap % 3 - 1) * dil;
+    const int64_t total = (int64_t)B * H * W;
+    const int64_t p0 = (int64_t)blockIdx.x * chunk;
+    const int64_t p1 = p0 + chunk < total ? p0 + chunk : total;
+    const int co = threadIdx.x / 8, ci0 = (threadIdx.x % 8) * 4;
+    float acc[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+    float bacc = 0.0f;
+    for (int64_t s = p0; s < p1; s += kGradSlab) {
+        for (int e = threadIdx.x; e < kGradSlab * 32; e += blockDim.x) {
+            const int pl = e / 32, c = e % 32;
+            const int64_t p = s + pl;
+            float zv = 0.0f, xv = 0.0f;
+            if (p < p1) {
+                if (zmode == 0) zv = from_bf16(reinterpret_cast<const __bf16 *>(dz)[p * 32 + c]);
+                else zv = c == 0 ? reinterpret_cast<const float *>(dz)[p] : 0.0f;
+                const int64_t n = p / ((int64_t)H * W);
+                const int rem = (int)(p - n * H * W);
+                const int yy = rem / W + dyo, xx = rem % W + dxo;
+                if (yy >= 0 && yy < H && xx >= 0 && xx < W) {
+                    const int64_t q = n * H * W + (int64_t)yy * W + xx;
+                    if (xmode == 0) xv = from_bf16(reinterpret_cast<const __bf16 *>(xin)[q * 32 + c]);
+                    else xv = c < 3 ? reinterpret_cast<const float *>(xin)[q * 4 + c] : 0.0f;
+                } else if (xmode == 1 && c == 0) {
+                    // replicate-padding of the b_eff channel (conv1_fp32_kernel)
+                    const int64_t q = n * H * W + (int64_t)(rem / W) * W + rem % W;
+                    xv = reinterpret_cast<const float *>(xin)[q * 4];
+                }
+            }
+            zs[pl][c] = zv;
+            xs[pl][c] = xv;
+        }
+        __syncthreads();
+        for (int pl = 0; pl < kGradSlab; pl++) {
+            const float z = zs[pl][co];
+            acc[0] += z * xs[pl][ci0 + 0];
+            acc[1] += z * xs[pl][ci0 + 1];
+            acc[2] += z * xs[pl][ci0 + 2];
+            acc[3] += z * xs[pl][ci0 + 3];
+            if (tap == 0 && (threadIdx.x % 8) == 0) bacc += z;
+        }
+        __syncthreads();
+    }
+    float *dst = partials + ((int64_t)blockIdx.x * 9 + tap) * 1024 + co * 32 + ci0;
+    for (int k = 0; k < 4; k++) dst[k] = acc[k];
+    if (tap == 0 && (threadIdx.x % 8) == 0) bias_partials[(int64_t)blockIdx.x * 32 + co] = bacc;
+}
+
+// fixed-order reduction of the partials into dW [9][cout][cin] and db [cout]
+__global__ void __launch_bounds__(kBlock) reduce_grad_kernel(const float *partials, const float *bias_partials,
+                                                             int nchunks, int cout, int cin, float *dw, float *db)
+{
+    const int total = 9 * 32 * 32 + 32;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < total; i += gridDim.x * blockDim.x) {
+        if (i < 9 * 1024) {
+            const int t = i / 1024, co = (i / 32) % 32, ci = i % 32;
+            if (co >= cout || ci >= cin) continue;
+            float s = 0.0f;
+            for (int c = 0; c < nchunks; c++) s += partials[((int64_t)c * 9 + t) * 1024 + co * 32 + ci];
+            dw[(t * cout + co) * cin + ci] = s;
+        } else {
+            const int co = i - 9 * 1024;
+            if (co >= cout) continue;
+            float s = 0.0f;
+            for (int c = 0; c < nchunks; c++) s += bias_partials[(int64_t)c * 32 + co];
+            db[co] = s;
+        }
+    }
+}
+
+// MSE loss on the predicted b_eff change; grad = 2 (pred - target) / N,
+// per-workgroup partial loss sums.  `weight` (optional) scales per pixel.
+__global__ void __launch_bounds__(kBlock) mse_kernel(const float *pred, const float *target, int64_t n,
+                                                     float *grad, double *partials)
+{
+    __shared__ double smem[4];
+    double s[1] = {0.0};
+    const float scale = 2.0f / (float)n;
+    for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < n; p += (int64_t)gridDim.x * blockDim.x) {
+        const float d = pred[p] - target[p];
+        grad[p] = scale * d;
+        s[0] += (double)d * (double)d;
+    }
+    block_reduce<double, 1, 0>(s, smem);
+    if (threadIdx.x == 0) partials[blockIdx.x] = s[0];
+}
+
+// Adam step on FP32 master parameters; `zero_sum_stride` > 0 re-projects the
+// b_eff-channel taps of layer 1 onto zero sum (per output channel).
+__global__ void adam_kernel(float *param, const float *grad, float *m, float *v, int n, float lr, float beta1,
+                            float beta2, float eps, float bc1, float bc2)
+{
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const float g = grad[i];
+        const float mi = beta1 * m[i] + (1.0f - beta1) * g;
+        const float vi = beta2 * v[i] + (1.0f - beta2) * g * g;
+        m[i] = mi;
+        v[i] = vi;
+        param[i] -= lr * (mi / bc1) / (__builtin_sqrtf(vi / bc2) + eps);
+    }
+}
+
+// layer-1 weights [9][32][4]: b_eff channel (0) taps sum to zero per output
+// channel; channel 3 stays zero
+__global__ void project_w1_kernel(float *w1)
+{
+    const int co = threadIdx.x;
+    if (co >= 32) return;
+    float s = 0.0f;
+    for (int t = 0; t < 9; t++) s += w1[(t * 32 + co) * 4];
+    s /= 9.0f;
+    for (int t = 0; t < 9; t++) {
+        w1[(t * 32 + co) * 4] -= s;
+        w1[(t * 32 + co) * 4 + 3] = 0.0f;
+    }
+}
+
+// layer-1 input gradient is not needed; the layer-1 pre-activation gradient
+// is dX1 * (Y1 > 0), produced by conv_mfma_kernel's mask path.
+
+// ---------------------------------------------------------------------------
+// batched patch relaxation: one workgroup relaxes one independent H x W
+// patch (FP32, closed boundary, legacy direction/parity schedule) entirely
+// in LDS until max excess <= tol or max_iterations.
+// ---------------------------------------------------------------------------
+constexpr int kPatchMax = 64 * 64;
+
+__global__ void __launch_bounds__(kBlock) relax_patches_kernel(float *h, int npatch, int H, int W, float c0, float c1,
+                                                               float c2, float c3, int max_iterations, float tol,
+                                                               int *iterations)
+{
+    __shared__ float t[kPatchMax];
+    __shared__ float red[8];
+    __shared__ int done;
+    const int patch = blockIdx.x;
+    if (patch >= npatch) return;
+    float *hp = h + (int64_t)patch * H * W;
+    for (int i = threadIdx.x; i < H * W; i += blockDim.x) t[i] = hp[i];
+    __syncthreads();
+    const float crit[4] = {c0, c1, c2, c3};
+    int it = 0;
+    for (it = 1; it <= max_iterations; it++) {
+        float mx = 0.0f;
+        for (int d = 0; d < 4; d++) {
+            for (int phase = 0; phase < 2; phase++) {
+                // view index (r, c) over the edge set of direction d
+                const int rows = (d == 0) ? H : H - 1;
+                const int cols = (d == 1) ? W : W - 1;
+                for (int e = threadIdx.x; e < rows * cols; e += blockDim.x) {
+                    const int r = e / cols, c = e - r * cols;
+                    int ia, ib;
+                    if (d == 0) { ia = r * W + c; ib = ia + 1; }
+                    else if (d == 1) { ia = r * W + c; ib = ia + W; }
+                    else if (d == 2) { ia = r * W + c; ib = ia + W + 1; }
+                    else { ia = (r + 1) * W + c; ib = r * W + c + 1; }
+                    float ex = fabsf(t[ia] - t[ib]) - crit[d];
+                    ex = ex > 0.0f ? ex : 0.0f;
+                    mx = mx >= ex ? mx : ex;
+                }
+                __syncthreads();
+                for (int e = threadIdx.x; e < rows * cols; e += blockDim.x) {
+                    const int r = e / cols, c = e - r * cols;
+                    const int coord = d == 1 ? r : c;
+                    if ((coord & 1) != phase) continue;
+                    int ia, ib;
+                    if (d == 0) { ia = r * W + c; ib = ia + 1; }
+                    else if (d == 1) { ia = r * W + c; ib = ia + W; }
+                    else if (d == 2) { ia = r * W + c; ib = ia + W + 1; }
+                    else { ia = (r + 1) * W + c; ib = r * W + c + 1; }
+                    const float diff = t[ia] - t[ib];
+                    float ex = fabsf(diff) - crit[d];
+                    ex = ex > 0.0f ? ex : 0.0f;
+                    const float sgn = diff > 0.0f ? 1.0f : (diff < 0.0f ? -1.0f : 0.0f);
+                    const float tr = (0.5f * ex) * sgn;
+                    t[ia] = t[ia] - tr;
+                    t[ib] = t[ib] + tr;
+                }
+                __syncthreads();
+            }
+        }
+        float m1[1] = {mx};
+        // wave max then workgroup max
+        for (int off = kWave / 2; off > 0; off >>= 1) {
+            float o = __shfl_down(m1[0], off, kWave);
+            m1[0] = o > m1[0] ? o : m1[0];
+        }
+        if ((threadIdx.x & (kWave - 1)) == 0) red[threadIdx.x / kWave] = m1[0];
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            float g = red[0];
+            for (int w = 1; w < (int)(blockDim.x / kWave); w++) g = red[w] > g ? red[w] : g;
+            done = g <= tol;
+        }
+        __syncthreads();
+        if (done) break;
+    }
+    for (int i = threadIdx.x; i < H * W; i += blockDim.x) hp[i] = t[i];
+    if (threadIdx.x == 0 && iterations) iterations[patch] = it > max_iterations ? max_iterations : it;
+}
+
+// mass projection of the predicted change: remove the mean of the change over
+// the cells it moves (|delta| > eps) so sum(delta) = 0, then keep heights >= 0
+__global__ void __launch_bounds__(kBlock) delta_stats_kernel(const float *delta, int64_t n, float eps, double *partials)
+{
+    __shared__ double smem[8];
+    double s[2] = {0.0, 0.0};
+    for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < n; p += (int64_t)gridDim.x * blockDim.x) {
+        const float d = delta[p];
+        if (fabsf(d) > eps) {
+            s[0] += (double)d;
+            s[1] += 1.0;
+        }
+    }
+    block_reduce<double, 2, 0>(s, smem);
+    if (threadIdx.x == 0) {
+        partials[blockIdx.x * 2] = s[0];
+        partials[blockIdx.x * 2 + 1] = s[1];
+    }
+}
+
+__global__ void __launch_bounds__(kBlock) apply_delta_kernel(float *b, const float *delta, int64_t n, float eps,
+                                                             const double *partials, int nparts)
+{
+    __shared__ float shift;
+    if (threadIdx.x == 0) {
+        double s = 0.0, c = 0.0;
+        for (int k = 0; k < nparts; k++) {
+            s += partials[2 * k];
+            c += partials[2 * k + 1];
+        }
+        shift = c > 0.0 ? (float)(s / c) : 0.0f;
+    }
+    __syncthreads();
+    for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < n; p += (int64_t)gridDim.x * blockDim.x) {
+        const float d = delta[p];
+        if (fabsf(d) > eps) {
+            const float v = b[p] + (d - shift);
+            b[p] = v > 0.0f ? v : 0.0f;
+        }
+    }
+}
+
+}  // namespace mv2
+
+using namespace mv2;
+
+static int check_plain(mv2_context *ctx)
+{
+    if (!ctx) return set_error(nullptr, MV2_EINVAL, "[Surrogate] null context");
+    hipSetDevice(ctx->device);
+    return MV2_OK;
+}
+
+extern "C" int mv2_nn_features(mv2_context *ctx, const float *b, const float *h, int B, int H, int W, double dx,
+                               double dy, float *feat)
+{
+    if (int rc = check_plain(ctx)) return rc;
+    const int64_t n = (int64_t)B * H * W;
+    hipLaunchKernelGGL(features_kernel, dim3(grid_for(n, kBlock, 8192)), dim3(kBlock), 0, ctx->stream, b, h, B, H,
+                       W, (float)dx, (float)dy, feat);
+    MV2_HIP(ctx, hipGetLastError());
+    return MV2_OK;
+}
+
+extern "C" int mv2_nn_conv1(mv2_context *ctx, const float *feat, const float *w1, const float *b1, int B, int H,
+                            int W, int dil, void *y)
+{
+    if (int rc = check_plain(ctx)) return rc;
+    const int64_t n = (int64_t)B * H * W;
+    hipLaunchKernelGGL(conv1_fp32_kernel, dim3(grid_for(n, kBlock, 8192)), dim3(kBlock), 0, ctx->stream, feat, w1,
+                       b1, B, H, W, dil, (__bf16 *)y);
+    MV2_HIP(ctx, hipGetLastError());
+    return MV2_OK;
+}
+
+extern "C" int mv2_nn_conv_mfma(mv2_context *ctx, const void *x, int cx, const void *w, const float *bias,
+                                const void *mask, void *y, int B, int H, int W, int dil, int relu, int mode)
+{
+    if (int rc = check_plain(ctx)) return rc;
+    if (cx != 16 && cx != 32) return set_error(ctx, MV2_EINVAL, "[Surrogate] input channels must be 16 or 32");
+    if (B < 1 || H < 1 || W < 1 || B > 65535 || (H + 3) / 4 > 65535)
+        return set_error(ctx, MV2_EINVAL, "[Surrogate] invalid activation shape");
+    ConvArgs a{(const __bf16 *)x, (const __bf16 *)w, bias, (const __bf16 *)mask, y, B, H, W, dil, relu, mode};
+    dim3 grid((W + 31) / 32, (H + 3) / 4, B);
+    if (cx == 16) hipLaunchKernelGGL(conv_mfma_kernel<1>, grid, dim3(kBlock), 0, ctx->stream, a);
+    else hipLaunchKernelGGL(conv_mfma_kernel<2>, grid, dim3(kBlock), 0, ctx->stream, a);
+    MV2_HIP(ctx, hipGetLastError());
+    return MV2_OK;
+}
+
+extern "C" int mv2_nn_pack_weights(mv2_context *ctx, const float *w, int cout, int cin, int cxf, int cxb, void *fwd,
+                                   void *bwd)
+{
+    if (int rc = check_plain(ctx)) return rc;
+    hipLaunchKernelGGL(pack_weights_kernel, dim3(36), dim3(kBlock), 0, ctx->stream, w, cout, cin, cxf, cxb,
+                       (__bf16 *)fwd, (__bf16 *)bwd);
+    MV2_HIP(ctx, hipGetLastError());
+    return MV2_OK;
+}
+
+extern "C" int mv2_nn_expand_grad(mv2_context *ctx, const float *g, int64_t npix, void *out16)
+{
+    if (int rc = check_plain(ctx)) return rc;
+    hipLaunchKernelGGL(expand_grad_kernel, dim3(grid_for(npix, kBlock, 8192)), dim3(kBlock), 0, ctx->stream, g,
+                       npix, (__bf16 *)out16);
+    MV2_HIP(ctx, hipGetLastError());
+    return MV2_OK;
+}
+
+extern "C" int mv2_nn_weight_grad(mv2_context *ctx, const void *dz, int zmode, const void *x, int xmode, int B,
+                                  int H, int W, int dil, int cout, int cin, float *dw, float *db)
+{
+    if (int rc = check_plain(ctx)) return rc;
+    const int64_t total = (int64_t)B * H * W;
+    int64_t chunk = 4096;
+    int nchunks = (int)((total + chunk - 1) / chunk);
+    if (nchunks > 2048) {
+        nchunks = 2048;
+        chunk = (total + nchunks - 1) / nchunks;
+        chunk = (chunk + kGradSlab - 1) / kGradSlab * kGradSlab;
+        nchunks = (int)((total + chunk - 1) / chunk);
+    }
+    const size_t need = sizeof(float) * ((size_t)nchunks * 9 * 1024 + (size_t)nchunks * 32);
+    if (int rc = ensure_scratch(ctx, ctx->nn_partials, need)) return rc;
+    float *parts = (float *)ctx->nn_partials.ptr;
+    float *bparts = parts + (size_t)nchunks * 9 * 1024;
+    hipLaunchKernelGGL(weight_grad_kernel, dim3(nchunks, 9), dim3(kBlock), 0, ctx->stream, dz, zmode, x, xmode, B, H,
+                       W, dil, chunk, parts, bparts);
+    hipLaunchKernelGGL(reduce_grad_kernel, dim3(37), dim3(kBlock), 0, ctx->stream, parts, bparts, nchunks, cout, cin,
+                       dw, db);
+    MV2_HIP(ctx, hipGetLastError());
+    return MV2_OK;
+}
+
+extern "C" int mv2_nn_mse(mv2_context *ctx, const float *pred, const float *target, int64_t n, float *grad,
+                          double *loss_out)
+{
+    if (int rc = check_plain(ctx)) return rc;
+    const int grid = grid_for(n, kBlock, 1024);
+    if (int rc = ensure_scratch(ctx, ctx->nn_loss, sizeof(double) * 2048)) return rc;
+    double *parts = (double *)ctx->nn_loss.ptr;
+    hipLaunchKernelGGL(mse_kernel, dim3(grid), dim3(kBlock), 0, ctx->stream, pred, target, n, grad, parts);
+    MV2_HIP(ctx, hipGetLastError());
+    if (loss_out) {
+        std::vector<double> host(grid);
+        MV2_HIP(ctx, hipMemcpyAsync(host.data(), parts, sizeof(double) * grid, hipMemcpyDeviceToHost, ctx->stream));
+        MV2_HIP(ctx, hipStreamSynchronize(ctx->stream));
+        double s = 0.0;
+        for (double v : host) s += v;
+        *loss_out = s / (double)n;
+    }
+    return MV2_OK;
+}
+
+extern "C" int mv2_nn_adam(mv2_context *ctx, float *param, const float *grad, float *m, float *v, int n, double lr,
+                           double beta1, double beta2, double eps, int step)
+{
+    if (int rc = check_plain(ctx)) return rc;
+    const float bc1 = (float)(1.0 - pow(beta1, step)), bc2 = (float)(1.0 - pow(beta2, step));
+    hipLaunchKernelGGL(adam_kernel, dim3(grid_for(n)), dim3(kBlock), 0, ctx->stream, param, grad, m, v, n, (float)lr,
+                       (float)beta1, (float)beta2, (float)eps, bc1, bc2);
+    MV2_HIP(ctx, hipGetLastError());
+    return MV2_OK;
+}
+
+extern "C" int mv2_nn_project_w1(mv2_context *ctx, float *w1)
+{
+    if (int rc = check_plain(ctx)) return rc;
+    hipLaunchKernelGGL(project_w1_kernel, dim3(1), dim3(64), 0, ctx->stream, w1);
+    MV2_HIP(ctx, hipGetLastError());
+    return MV2_OK;
+}
+
+extern "C" int mv2_relax_patches(mv2_context *ctx, float *h, int npatch, int H, int W, const double crit_diff[4],
+                                 int max_iterations, double tolerance, int *iterations)
+{
+    if (int rc = check_plain(ctx)) return rc;
+    if (H < 2 || W < 2 || H * W > kPatchMax)
+        return set_error(ctx, MV2_EINVAL, "[MiniSlope] patch must be between 2x2 and 4096 cells");
+    if (npatch < 0) return set_error(ctx, MV2_EINVAL, "[MiniSlope] negative patch count");
+    if (npatch == 0) return MV2_OK;
+    hipLaunchKernelGGL(relax_patches_kernel, dim3(npatch), dim3(kBlock), 0, ctx->stream, h, npatch, H, W,
+                       (float)crit_diff[0], (float)crit_diff[1], (float)crit_diff[2], (float)crit_diff[3],
+                       max_iterations, (float)tolerance, iterations);
+    MV2_HIP(ctx, hipGetLastError());
+    return MV2_OK;
+}
+
+extern "C" int mv2_nn_apply_delta(mv2_context *ctx, float *b, const float *delta, int64_t n, double eps)
+{
+    if (int rc = check_plain(ctx)) return rc;
+    const int grid = grid_for(n, kBlock, 1024);
+    if (int rc = ensure_scratch(ctx, ctx->nn_loss, sizeof(double) * 2048)) return rc;
+    double *parts = (double *)ctx->nn_loss.ptr;
+    hipLaunchKernelGGL(delta_stats_kernel, dim3(grid), dim3(kBlock), 0, ctx->stream, delta, n, (float)eps, parts);
+    hipLaunchKernelGGL(apply_delta_kernel, dim3(grid), dim3(kBlock), 0, ctx->stream, b, delta, n, (float)eps, parts,
+                       grid);
+    MV2_HIP(ctx, hipGetLastError());
+    return MV2_OK;
+}

@@ -40,7 +40,7 @@ class StepRecord(ctypes.Structure):
         ("numeric_error", _I), ("reserved", _I),
         ("maximum_cfl", _D), ("remaining", _D),
         ("summary_before", _D * 8), ("summary_after", _D * 8), ("diag", _D * 10),
-        ("mass_quanta_before", _D), ("mass_quanta_after", _D),
+        ("mass_quanta_before", _D), ("mass_quanta_after", _D), ("kernel_ms", _D),
     ]
 
 
@@ -102,6 +102,17 @@ SIGNATURES: dict[str, tuple[object, list[object]]] = {
     "mv2_avalanche_mobilize": (_I, [_P, ctypes.POINTER(AvalancheConfig),
                                     ctypes.POINTER(AvalancheFields), _I32, _D, _P]),
     "mv2_label_mask": (_I, [_P, _P, _P, _I, _I, _I]),
+    "mv2_nn_features": (_I, [_P, _P, _P, _I, _I, _I, _D, _D, _P]),
+    "mv2_nn_conv1": (_I, [_P, _P, _P, _P, _I, _I, _I, _I, _P]),
+    "mv2_nn_conv_mfma": (_I, [_P, _P, _I, _P, _P, _P, _P, _I, _I, _I, _I, _I, _I]),
+    "mv2_nn_pack_weights": (_I, [_P, _P, _I, _I, _I, _I, _P, _P]),
+    "mv2_nn_expand_grad": (_I, [_P, _P, ctypes.c_int64, _P]),
+    "mv2_nn_weight_grad": (_I, [_P, _P, _I, _P, _I, _I, _I, _I, _I, _I, _I, _P, _P]),
+    "mv2_nn_mse": (_I, [_P, _P, _P, ctypes.c_int64, _P, ctypes.POINTER(_D)]),
+    "mv2_nn_adam": (_I, [_P, _P, _P, _P, _P, _I, _D, _D, _D, _D, _I]),
+    "mv2_nn_project_w1": (_I, [_P, _P]),
+    "mv2_nn_apply_delta": (_I, [_P, _P, _P, ctypes.c_int64, _D]),
+    "mv2_relax_patches": (_I, [_P, _P, _I, _I, _I, _P, _I, _D, _P]),
 }
 
 _lib: ctypes.CDLL | None = None

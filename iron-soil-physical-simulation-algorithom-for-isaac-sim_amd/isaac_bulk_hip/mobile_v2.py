@@ -144,6 +144,7 @@ class HipProductionMobileV2Solver:
         self.state: DeviceBulkState | None = None
         self.material = None
         self.grid = None
+        self.last_record: _lib.StepRecord | None = None
 
     def bind_device_state(self, state: DeviceBulkState, material, grid, integrator) -> None:
         del integrator
@@ -164,6 +165,7 @@ class HipProductionMobileV2Solver:
         density = float(self.material.assumed_bulk_density_kg_m3)
         rec = run_step(state, self.config, float(dt_s), density=density, limit_mode=1,
                        weight_mode=1, error_prefix="[MobileV2Production] ")
+        self.last_record = rec
         before = np.asarray(rec.summary_before[:])
         after = np.asarray(rec.summary_after[:])
         diag = np.asarray(rec.diag[:])

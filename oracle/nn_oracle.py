@@ -1,0 +1,137 @@
+"""NumPy oracle of the learned warm-start network (forward + backward).
+
+TEST INFRASTRUCTURE ONLY.  Restates the layer definitions of the HIP kernels
+in csrc/mv2_surrogate.hip: features (b_eff, h_mobile, |grad H|), layer 1 in
+FP32 with replicate padding on the b_eff channel, layers 2-5 as dilated 3x3
+convolutions with zero padding, ReLU after layers 1-4, a single-channel
+output.  ``bf16=True`` rounds activations and weights to BF16 at the points
+where the device does; ``bf16=False`` is plain float64 and is what the
+finite-difference test pins.
+"""
+
+from __future__ import annotations
+
+import numpy as np
+
+DILATIONS = (1, 2, 4, 8, 16)
+TAPS = [(t // 3 - 1, t % 3 - 1) for t in range(9)]
+
+
+def bf16(x: np.ndarray) -> np.ndarray:
+    """Round float32 values to bfloat16 (round to nearest even), returned as float32."""
+    a = np.ascontiguousarray(np.asarray(x, dtype=np.float32)).view(np.uint32).astype(np.uint64)
+    r = ((a + 0x7FFF + ((a >> 16) & 1)) >> 16) << 16
+    return (r & 0xFFFFFFFF).astype(np.uint32).view(np.float32)
+
+
+def features(b: np.ndarray, h: np.ndarray, dx: float, dy: float) -> np.ndarray:
+    """[B,H,W] -> [B,H,W,3] (b_eff, h_mobile, |grad(b+h)|), float32 arithmetic."""
+    b = np.asarray(b, np.float32)
+    h = np.zeros_like(b) if h is None else np.asarray(h, np.float32)
+    Hf = b + h
+    B, H, W = b.shape
+    xs = np.arange(W)
+    xl, xr = np.maximum(xs - 1, 0), np.minimum(xs + 1, W - 1)
+    ys = np.arange(H)
+    yd, yu = np.maximum(ys - 1, 0), np.minimum(ys + 1, H - 1)
+    gx = (Hf[:, :, xr] - Hf[:, :, xl]) / ((xr - xl).astype(np.float32) * np.float32(dx))[None, None, :]
+    gy = (Hf[:, yu, :] - Hf[:, yd, :]) / ((yu - yd).astype(np.float32) * np.float32(dy))[None, :, None]
+    slope = np.sqrt(gx * gx + gy * gy).astype(np.float32)
+    return np.stack((b, h, slope), axis=-1)
+
+
+def _shift(x: np.ndarray, dy: int, dx: int, fill_center: bool = False) -> np.ndarray:
+    """out[n, y, x] = x[n, y+dy, x+dx] (zero outside, or the centre value)."""
+    B, H, W = x.shape[:3]
+    out = np.array(x, copy=True) if fill_center else np.zeros_like(x)
+    ys0, ys1 = max(0, -dy), min(H, H - dy)
+    xs0, xs1 = max(0, -dx), min(W, W - dx)
+    if ys1 > ys0 and xs1 > xs0:
+        out[:, ys0:ys1, xs0:xs1] = x[:, ys0 + dy:ys1 + dy, xs0 + dx:xs1 + dx]
+    return out
+
+
+def conv(x: np.ndarray, w: np.ndarray, bias, dil: int, *, replicate_channel0: bool = False) -> np.ndarray:
+    """x [B,H,W,Cin], w [9,Cout,Cin] -> [B,H,W,Cout] (float64 accumulation)."""
+    x = np.asarray(x, np.float64)
+    out = np.zeros(x.shape[:3] + (w.shape[1],))
+    for t, (ty, tx) in enumerate(TAPS):
+        xs = _shift(x, ty * dil, tx * dil)
+        if replicate_channel0:
+            xs[..., 0] = _shift(x[..., :1], ty * dil, tx * dil, fill_center=True)[..., 0]
+        out += xs @ np.asarray(w[t], np.float64).T
+    if bias is not None:
+        out += np.asarray(bias, np.float64)
+    return out
+
+
+def init_params(seed: int = 0) -> dict:
+    rng = np.random.default_rng(seed)
+    p = {}
+    p["w1"] = (rng.standard_normal((9, 32, 3)) * np.sqrt(2.0 / 27.0)).astype(np.float32)
+    p["w1"][:, :, 0] -= p["w1"][:, :, 0].mean(axis=0, keepdims=True)
+    p["b1"] = np.zeros(32, np.float32)
+    for k in (2, 3, 4):
+        p[f"w{k}"] = (rng.standard_normal((9, 32, 32)) * np.sqrt(2.0 / 288.0)).astype(np.float32)
+        p[f"b{k}"] = np.zeros(32, np.float32)
+    p["w5"] = (rng.standard_normal((9, 1, 32)) * np.sqrt(1.0 / 288.0) * 0.1).astype(np.float32)
+    p["b5"] = np.zeros(1, np.float32)
+    return p
+
+
+def forward(params: dict, feat: np.ndarray, *, bf16_mode: bool = True):
+    """Return (output [B,H,W], activations list) for features [B,H,W,3]."""
+    rnd = bf16 if bf16_mode else (lambda a: np.asarray(a, np.float64))
+    acts = [np.asarray(feat, np.float64)]
+    z = conv(feat, params["w1"], params["b1"], DILATIONS[0], replicate_channel0=True)
+    y = rnd(np.maximum(z, 0.0).astype(np.float32) if bf16_mode else np.maximum(z, 0.0))
+    acts.append(np.asarray(y, np.float64))
+    for k in (2, 3, 4):
+        w = rnd(params[f"w{k}"]) if bf16_mode else params[f"w{k}"]
+        z = conv(acts[-1], w, params[f"b{k}"], DILATIONS[k - 1])
+        y = rnd(np.maximum(z, 0.0).astype(np.float32) if bf16_mode else np.maximum(z, 0.0))
+        acts.append(np.asarray(y, np.float64))
+    w5 = rnd(params["w5"]) if bf16_mode else params["w5"]
+    out = conv(acts[-1], w5, params["b5"], DILATIONS[4])[..., 0]
+    return out, acts
+
+
+def _conv_backward_input(gz: np.ndarray, w: np.ndarray, dil: int) -> np.ndarray:
+    """dX[p] = sum_t W[t]^T gz[p - off(t)]."""
+    out = np.zeros(gz.shape[:3] + (w.shape[2],))
+    for t, (ty, tx) in enumerate(TAPS):
+        out += _shift(gz, -ty * dil, -tx * dil) @ np.asarray(w[t], np.float64)
+    return out
+
+
+def _conv_backward_weight(gz: np.ndarray, x: np.ndarray, dil: int, *, replicate_channel0=False):
+    cout, cin = gz.shape[-1], x.shape[-1]
+    dw = np.zeros((9, cout, cin))
+    for t, (ty, tx) in enumerate(TAPS):
+        xs = _shift(x, ty * dil, tx * dil)
+        if replicate_channel0:
+            xs[..., 0] = _shift(x[..., :1], ty * dil, tx * dil, fill_center=True)[..., 0]
+        dw[t] = np.einsum("bhwo,bhwi->oi", gz, xs)
+    return dw, gz.sum(axis=(0, 1, 2))
+
+
+def backward(params: dict, acts: list, grad_out: np.ndarray, *, bf16_mode: bool = True) -> dict:
+    """Gradients of sum(grad_out * output) w.r.t. every parameter."""
+    rnd = bf16 if bf16_mode else (lambda a: np.asarray(a, np.float64))
+    grads = {}
+    g = np.asarray(grad_out, np.float64)[..., None]
+    w5 = rnd(params["w5"]) if bf16_mode else params["w5"]
+    grads["w5"], grads["b5"] = _conv_backward_weight(g, acts[4], DILATIONS[4])
+    gx = _conv_backward_input(rnd(g.astype(np.float32)) if bf16_mode else g, w5, DILATIONS[4])
+    for k in (4, 3, 2):
+        gz = gx * (acts[k] > 0)
+        if bf16_mode:
+            gz = np.asarray(bf16(gz.astype(np.float32)), np.float64)
+        grads[f"w{k}"], grads[f"b{k}"] = _conv_backward_weight(gz, acts[k - 1], DILATIONS[k - 1])
+        wk = rnd(params[f"w{k}"]) if bf16_mode else params[f"w{k}"]
+        gx = _conv_backward_input(gz, wk, DILATIONS[k - 1])
+    gz = gx * (acts[1] > 0)
+    if bf16_mode:
+        gz = np.asarray(bf16(gz.astype(np.float32)), np.float64)
+    grads["w1"], grads["b1"] = _conv_backward_weight(gz, acts[0], DILATIONS[0], replicate_channel0=True)
+    return grads

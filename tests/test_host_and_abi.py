@@ -47,7 +47,7 @@ def test_abi_constants():
 def test_struct_layouts_match_header():
     # sizes follow from the C declarations (natural alignment, no packing)
     assert ctypes.sizeof(_lib.StepConfig) == 10 * 8 + 4 * 4
-    assert ctypes.sizeof(_lib.StepRecord) == 6 * 4 + 2 * 8 + 8 * 8 * 2 + 10 * 8 + 2 * 8
+    assert ctypes.sizeof(_lib.StepRecord) == 6 * 4 + 2 * 8 + 8 * 8 * 2 + 10 * 8 + 3 * 8
     assert ctypes.sizeof(_lib.RelaxResult) == 16
     assert ctypes.sizeof(_lib.AvalancheConfig) == 10 * 8 + 4 * 4
     assert ctypes.sizeof(_lib.AvalancheFields) == 16 * 8
