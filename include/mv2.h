@@ -78,7 +78,7 @@ typedef struct mv2_step_record {
     int cfl_limited;
     int static_exit;
     int numeric_error;                /* 1 positivity, 2 maximum_substeps, 3 overflow */
-    int reserved;
+    int complete;                     /* 1: the substep loop ended (always 1 on return) */
     double maximum_cfl;
     double remaining;
     double summary_before[8];
@@ -90,13 +90,21 @@ typedef struct mv2_step_record {
     double diag[10];
     double mass_quanta_before;        /* FP32: exact sum of mobile quanta (as double) */
     double mass_quanta_after;
-    double kernel_ms;                 /* device time of the step launch (HIP events) */
+    double kernel_ms;                 /* device time of the whole call (HIP events) */
+    double substep_kernel_ms;         /* sum over the substep launches (timing mode only, else 0) */
+    int launches;                     /* substep launches enqueued (substeps + ended-loop launches) */
+    int pad;
 } mv2_step_record;
 
 /* One step_resident call (warp_mobile_v2.py:148-245) or standalone step
  * (pkg/experimental/mobile_v2_warp.py:251-344): static check, device-side CFL
  * loop with no host round trip per substep, ledgers before/after. */
 int mv2_step(mv2_context *ctx, const mv2_step_config *cfg, mv2_step_record *out);
+
+/* Timing mode (no counterpart in the reference; a measurement aid): when
+ * enabled, mv2_step brackets every substep launch with HIP events and reports
+ * their summed duration in substep_kernel_ms. */
+int mv2_set_timing(mv2_context *ctx, int enable);
 
 /* ledger reduction only (warp_mobile_v2.py:28-54 summarize) */
 int mv2_summary(mv2_context *ctx, const mv2_step_config *cfg, double out[8]);
@@ -124,6 +132,8 @@ typedef struct mv2_relax_result {
     int iterations;
     int converged;
     double max_excess_last;
+    int complete;                     /* 1: the iteration loop ended (always 1 on return) */
+    int launches;                     /* iteration launches enqueued */
 } mv2_relax_result;
 /* crit_diff[4] = critical slope * distance for the legacy directions
  * (1,0) dx, (0,1) dy, (1,1) and (1,-1) hypot(dx, dy), as the reference forms them. */

@@ -121,9 +121,12 @@ extern "C" void mv2_destroy(mv2_context *ctx)
     release(ctx->label_aux);
     release(ctx->nn_partials);
     release(ctx->nn_loss);
+    release(ctx->step_diag);
+    release(ctx->spread);
     release(ctx->host_result, true);
     if (ctx->ev_start) hipEventDestroy(ctx->ev_start);
     if (ctx->ev_stop) hipEventDestroy(ctx->ev_stop);
+    for (hipEvent_t e : ctx->sub_events) hipEventDestroy(e);
     delete ctx;
 }
 

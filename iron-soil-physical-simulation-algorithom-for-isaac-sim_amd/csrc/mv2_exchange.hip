@@ -140,7 +140,7 @@ __global__ void __launch_bounds__(kBlock) reservoir_kernel(const void *z_base, c
     const int64_t n = (int64_t)ny * nx;
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
          i += (int64_t)gridDim.x * blockDim.x) {
-        const int r = (int)(i / nx), c = (int)(i - (int64_t)r * nx);
+        const int r = (int)i / nx, c = (int)i - r * nx;
         const double w = cell_weight(weight_mode, ny, nx, area, f6, r, c);
         double z, b, m, qx, qy;
         if (!F32) {

@@ -6,6 +6,7 @@
 #include <cstdint>
 #include <cstdio>
 #include <string>
+#include <vector>
 
 #include "mv2.h"
 
@@ -49,7 +50,13 @@ struct mv2_context {
     mv2::Scratch label_aux;               // labelling partials
     mv2::Scratch nn_partials;             // weight-gradient partials of the surrogate
     mv2::Scratch nn_loss;                 // loss / projection partials
-    hipEvent_t ev_start = nullptr, ev_stop = nullptr;   // device timing of step launches
+    mv2::Scratch step_diag;               // per-workgroup substep diagnostics
+    mv2::Scratch spread;                  // spread-maximum slots of the step and the relaxation
+    hipEvent_t ev_start = nullptr, ev_stop = nullptr;   // device timing of step calls
+    int timing = 0;                       // bracket each substep launch with events
+    std::vector<hipEvent_t> sub_events;   // event pairs of the substep launches
+    int v2_predicted = 0;                 // substep launches to enqueue up front
+    int relax_predicted = 0;              // relaxation launches to enqueue up front
 };
 
 namespace mv2 {
@@ -64,83 +71,114 @@ int ensure_scratch(mv2_context *ctx, Scratch &s, size_t bytes, bool host_pinned 
         if (_e != hipSuccess) return mv2::check_hip(ctx, _e, #expr); \
     } while (0)
 
-// ---- reductions (wave64) ------------------------------------------------------
-template <class T>
-__device__ inline T wave_sum(T v)
+// ---- reductions (wave64, DPP) ----------------------------------------------------
+// A wave reduction is a Hillis-Steele scan inside each 16-lane row (row_shr 1,
+// 2, 4, 8), then row_bcast:15 and row_bcast:31 carry the row totals up to lane
+// 63, which is read back as a scalar.  Lanes without a source keep the
+// identity.  Each step is a VALU move with a fixed lane pattern, so the
+// result is deterministic; it replaces LDS-crossbar shuffles, whose latency
+// dominated multi-value block reductions.
+template <int CTRL, int ROW_MASK>
+__device__ inline unsigned dpp_u32(unsigned old, unsigned v)
 {
-    for (int off = kWave / 2; off > 0; off >>= 1) v += __shfl_down(v, off, kWave);
-    return v;
-}
-template <class T>
-__device__ inline T wave_max(T v)
-{
-    for (int off = kWave / 2; off > 0; off >>= 1) {
-        T o = __shfl_down(v, off, kWave);
-        v = o > v ? o : v;
-    }
-    return v;
-}
-template <class T>
-__device__ inline T wave_min(T v)
-{
-    for (int off = kWave / 2; off > 0; off >>= 1) {
-        T o = __shfl_down(v, off, kWave);
-        v = o < v ? o : v;
-    }
-    return v;
+    return (unsigned)__builtin_amdgcn_update_dpp((int)old, (int)v, CTRL, ROW_MASK, 0xf, false);
 }
 
-// Block-wide reduction of NV values; result valid in thread 0.  `smem` must
-// hold NV * (blockDim/64) elements.  Combines waves in fixed order, so the
-// result is deterministic for a given launch shape.
-template <class T, int NV, int OP>  // OP 0 sum, 1 max, 2 min
+template <int CTRL, int ROW_MASK, class T>
+__device__ inline T dpp_move(T old, T v)
+{
+    static_assert(sizeof(T) == 4 || sizeof(T) == 8, "32- or 64-bit values");
+    if constexpr (sizeof(T) == 4) {
+        return __builtin_bit_cast(T, dpp_u32<CTRL, ROW_MASK>(__builtin_bit_cast(unsigned, old),
+                                                              __builtin_bit_cast(unsigned, v)));
+    } else {
+        struct W { unsigned lo, hi; };
+        W o = __builtin_bit_cast(W, old), x = __builtin_bit_cast(W, v), r;
+        r.lo = dpp_u32<CTRL, ROW_MASK>(o.lo, x.lo);
+        r.hi = dpp_u32<CTRL, ROW_MASK>(o.hi, x.hi);
+        return __builtin_bit_cast(T, r);
+    }
+}
+
+template <int OP, class T>  // OP 0 sum, 1 max, 2 min
+__device__ inline T red_op(T a, T b)
+{
+    if constexpr (OP == 0) return a + b;
+    else if constexpr (OP == 1) return b > a ? b : a;
+    else return b < a ? b : a;
+}
+
+template <class T> struct RedLimits;
+template <> struct RedLimits<float> {
+    __device__ static float lowest() { return -__builtin_huge_valf(); }
+    __device__ static float highest() { return __builtin_huge_valf(); }
+};
+template <> struct RedLimits<double> {
+    __device__ static double lowest() { return -__builtin_huge_val(); }
+    __device__ static double highest() { return __builtin_huge_val(); }
+};
+template <> struct RedLimits<int> {
+    __device__ static int lowest() { return (int)0x80000000u; }
+    __device__ static int highest() { return 0x7fffffff; }
+};
+template <> struct RedLimits<long long> {
+    __device__ static long long lowest() { return (long long)0x8000000000000000ull; }
+    __device__ static long long highest() { return 0x7fffffffffffffffll; }
+};
+
+template <int OP, class T>
+__device__ inline T wave_reduce(T v)
+{
+    const T id = OP == 0 ? (T)0 : (OP == 1 ? RedLimits<T>::lowest() : RedLimits<T>::highest());
+    v = red_op<OP>(v, dpp_move<0x111, 0xf>(id, v));   // row_shr:1
+    v = red_op<OP>(v, dpp_move<0x112, 0xf>(id, v));   // row_shr:2
+    v = red_op<OP>(v, dpp_move<0x114, 0xf>(id, v));   // row_shr:4
+    v = red_op<OP>(v, dpp_move<0x118, 0xf>(id, v));   // row_shr:8
+    v = red_op<OP>(v, dpp_move<0x142, 0xa>(id, v));   // row_bcast:15 into rows 1, 3
+    v = red_op<OP>(v, dpp_move<0x143, 0xc>(id, v));   // row_bcast:31 into rows 2, 3
+    if constexpr (sizeof(T) == 4) {
+        return __builtin_bit_cast(T, __builtin_amdgcn_readlane(__builtin_bit_cast(int, v), 63));
+    } else {
+        struct W { int lo, hi; };
+        W w = __builtin_bit_cast(W, v);
+        w.lo = __builtin_amdgcn_readlane(w.lo, 63);
+        w.hi = __builtin_amdgcn_readlane(w.hi, 63);
+        return __builtin_bit_cast(T, w);
+    }
+}
+template <class T> __device__ inline T wave_sum(T v) { return wave_reduce<0>(v); }
+template <class T> __device__ inline T wave_max(T v) { return wave_reduce<1>(v); }
+template <class T> __device__ inline T wave_min(T v) { return wave_reduce<2>(v); }
+
+// Block-wide reduction of NV values; result valid in thread 0.  NW is the
+// number of waves of the block (blockDim.x / 64) and `smem` must hold NV * NW
+// elements.  Waves are combined in fixed order, so the result is
+// deterministic for a given launch shape.  Thread k < NV folds value k (a
+// single thread folding every value serialises hundreds of dependent LDS
+// reads on one wave).  All threads of the block must call it (full waves).
+template <class T, int NV, int OP, int NW = kBlock / kWave>  // OP 0 sum, 1 max, 2 min
 __device__ inline void block_reduce(T (&v)[NV], T *smem)
 {
+    static_assert(NV <= kWave, "one wave folds the values");
     const int lane = threadIdx.x & (kWave - 1);
     const int wave = threadIdx.x / kWave;
-    const int nwaves = blockDim.x / kWave;
+#pragma unroll
     for (int k = 0; k < NV; k++) {
-        T x = v[k];
-        if (OP == 0) x = wave_sum(x);
-        else if (OP == 1) x = wave_max(x);
-        else x = wave_min(x);
-        if (lane == 0) smem[k * nwaves + wave] = x;
+        const T x = wave_reduce<OP>(v[k]);
+        if (lane == 0) smem[k * NW + wave] = x;
+    }
+    __syncthreads();
+    if (threadIdx.x < NV) {
+        const int k = threadIdx.x;
+        T acc = smem[k * NW];
+#pragma unroll
+        for (int w = 1; w < NW; w++) acc = red_op<OP>(acc, smem[k * NW + w]);
+        smem[k * NW] = acc;            // only thread k touches row k
     }
     __syncthreads();
     if (threadIdx.x == 0) {
-        for (int k = 0; k < NV; k++) {
-            T acc = smem[k * nwaves];
-            for (int w = 1; w < nwaves; w++) {
-                T o = smem[k * nwaves + w];
-                if (OP == 0) acc += o;
-                else if (OP == 1) acc = o > acc ? o : acc;
-                else acc = o < acc ? o : acc;
-            }
-            v[k] = acc;
-        }
-    }
-    __syncthreads();
-}
-
-// ---- grid barrier for cooperatively launched (co-resident) grids ---------------
-// `counter` starts at 0 for the launch; barrier k completes when the counter
-// reaches k * gridDim.x.  One agent-scope release fence before arriving and one
-// acquire fence after the wait make every XCD's writes visible.  The wait
-// itself polls with relaxed loads: an acquire load invalidates the XCD's L2 on
-// every poll, and a thousand spinning workgroups would keep the L2 of all
-// other workgroups cold.
-__device__ inline void grid_barrier(unsigned *counter, unsigned &epoch)
-{
-    __syncthreads();
-    epoch += 1;
-    if (threadIdx.x == 0) {
-        const unsigned target = epoch * gridDim.x;
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-        __hip_atomic_fetch_add(counter, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        while (__hip_atomic_load(counter, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
-            __builtin_amdgcn_s_sleep(2);
-        }
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+#pragma unroll
+        for (int k = 0; k < NV; k++) v[k] = smem[k * NW];
     }
     __syncthreads();
 }
@@ -162,11 +200,68 @@ __device__ inline double cell_weight(int mode, int ny, int nx, double area, doub
     return mode ? vertex_weight(ny, nx, f6, r, c) : area;
 }
 
+// Load a small record written by an earlier launch, word by word with
+// device-coherent loads (never served from a stale per-CU cache).
+template <class T>
+__device__ inline T load_coherent(const T *p)
+{
+    static_assert(sizeof(T) % 4 == 0, "record size must be a multiple of 4 bytes");
+    unsigned w[sizeof(T) / 4];
+    const unsigned *src = reinterpret_cast<const unsigned *>(p);
+#pragma unroll
+    for (int i = 0; i < (int)(sizeof(T) / 4); i++)
+        w[i] = __hip_atomic_load(src + i, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    T v;
+    __builtin_memcpy(&v, w, sizeof(T));
+    return v;
+}
+
 // atomic max of a non-negative double via its ordered bit pattern
 __device__ inline void atomic_max_nonneg(unsigned long long *slot, double v)
 {
     if (!(v >= 0.0)) return;
     atomicMax(slot, (unsigned long long)__double_as_longlong(v));
+}
+
+// ---- spread maxima -----------------------------------------------------------------
+// Device-scope atomics on one address serialise (~12 ns each on MI355X, so a
+// launch of ~1000 workgroups pays ~12 us for one global maximum).  A spread
+// maximum gives every 64th workgroup the same slot, 128 bytes apart, so the
+// atomics of one launch proceed in parallel; the consumer (a later launch)
+// folds the 64 slots with one wave.  Max is exact, so the order of arrival
+// does not change the result.
+constexpr int kSpread = 64;
+constexpr int kSpreadStride = 16;                    // u64 words between slots
+constexpr int kSpreadWords = kSpread * kSpreadStride;
+// the step uses sets 0..4, the relaxation sets 8..10, the labelling 12..14
+constexpr int kRelaxSpreadSet = 8;
+constexpr int kLabelSpreadSet = 12;
+constexpr size_t kSpreadScratchBytes = (size_t)16 * kSpreadWords * 8;
+
+__device__ inline void spread_max(unsigned long long *slots, double v)
+{
+    atomic_max_nonneg(&slots[(blockIdx.x % kSpread) * kSpreadStride], v);
+}
+
+// maximum over the slots (written by earlier launches); every thread of the
+// block calls it and receives the value
+__device__ inline double spread_read_max(const unsigned long long *slots, double *smem1)
+{
+    if (threadIdx.x < kWave) {
+        double v = __longlong_as_double((long long)slots[threadIdx.x * kSpreadStride]);
+        v = wave_max(v);
+        if (threadIdx.x == 0) *smem1 = v;
+    }
+    __syncthreads();
+    const double r = *smem1;
+    __syncthreads();
+    return r;
+}
+
+// zero the slots (threads 0..63 of one workgroup)
+__device__ inline void spread_clear(unsigned long long *slots)
+{
+    if (threadIdx.x < kSpread) slots[threadIdx.x * kSpreadStride] = 0ull;
 }
 
 // ---- launch helpers --------------------------------------------------------------

@@ -48,7 +48,7 @@ __global__ void __launch_bounds__(kBlock) avalanche_init_kernel(
     const int64_t n = (int64_t)ny * nx;
     for (int64_t index = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; index < n;
          index += (int64_t)gridDim.x * blockDim.x) {
-        const int row = (int)(index / nx), col = (int)(index - (int64_t)row * nx);
+        const int row = (int)index / nx, col = (int)index - row * nx;
         int64_t left = index, right = index, down = index, up = index;
         double denom_x = dx, denom_y = dy;
         if (col > 0) left = index - 1;
@@ -137,41 +137,191 @@ __device__ inline void unite(int32_t *p, int32_t a, int32_t b)
     }
 }
 
-__global__ void __launch_bounds__(kBlock) union_kernel(int32_t *p, int ny, int nx, int connectivity)
+// ---------------------------------------------------------------------------
+// Tile-based union-find.  The grid is cut into 32 x 32 tiles, one workgroup
+// each.  (1) ccl_local_kernel: union-find of the tile's cells in LDS with
+// workgroup-scope compare-and-swap, then every masked cell gets the global
+// index of its tile-local root (the minimum index of its tile-local
+// component: row-major order inside a tile agrees with global row-major
+// order).  (2) ccl_border_kernel: the edges that cross a tile boundary are
+// united in the global parent array with the same larger-root-onto-smaller
+// hooking, so the final root of a component is its minimum linear index.
+// (3) ccl_flatten_kernel: every cell follows its parents to the root and the
+// tile counts cells per root in an LDS table before one global atomicAdd per
+// distinct root, so a component spanning many tiles costs one atomic per tile
+// instead of one per cell.
+// ---------------------------------------------------------------------------
+constexpr int kCcl = 32;
+constexpr int kCclCells = kCcl * kCcl;
+constexpr int kCclPer = kCclCells / kBlock;
+constexpr int kCclHash = 128;
+static_assert(kCclCells % kBlock == 0, "tile must split evenly over the block");
+
+__device__ inline int lds_load(int *p, int i)
 {
-    const int64_t n = (int64_t)ny * nx;
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
-        if (load_parent(p, (int32_t)i) < 0) continue;
-        const int r = (int)(i / nx), c = (int)(i - (int64_t)r * nx);
-        const int32_t ii = (int32_t)i;
-        if (c + 1 < nx && load_parent(p, ii + 1) >= 0) unite(p, ii, ii + 1);
-        if (r + 1 < ny) {
-            if (load_parent(p, ii + nx) >= 0) unite(p, ii, ii + nx);
+    return __hip_atomic_load(&p[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+
+// root of local cell x with path halving (parents satisfy lp[x] <= x)
+__device__ inline int lds_find(int *lp, int x)
+{
+    int cur = lds_load(lp, x);
+    if (cur != x) {
+        int prev = x, next;
+        while (cur > (next = lds_load(lp, cur))) {
+            __hip_atomic_store(&lp[prev], next, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+            prev = cur;
+            cur = next;
+        }
+    }
+    return cur;
+}
+
+__device__ inline void lds_unite(int *lp, int a, int b)
+{
+    int ra = lds_find(lp, a), rb = lds_find(lp, b);
+    while (ra != rb) {
+        if (ra < rb) {
+            int expected = rb;
+            if (__hip_atomic_compare_exchange_strong(&lp[rb], &expected, ra, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
+                                                     __HIP_MEMORY_SCOPE_WORKGROUP))
+                return;
+            rb = lds_find(lp, expected);
+        } else {
+            int expected = ra;
+            if (__hip_atomic_compare_exchange_strong(&lp[ra], &expected, rb, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
+                                                     __HIP_MEMORY_SCOPE_WORKGROUP))
+                return;
+            ra = lds_find(lp, expected);
+        }
+    }
+}
+
+__global__ void __launch_bounds__(kBlock) ccl_local_kernel(int32_t *p, int ny, int nx, int connectivity, int tiles_x)
+{
+    __shared__ int lp[kCclCells];
+    const int t = blockIdx.x;
+    const int tr0 = (t / tiles_x) * kCcl, tc0 = (t % tiles_x) * kCcl;
+    bool masked[kCclPer];
+#pragma unroll
+    for (int j = 0; j < kCclPer; j++) {
+        const int e = threadIdx.x + j * kBlock;
+        const int gr = tr0 + e / kCcl, gc = tc0 + e % kCcl;
+        masked[j] = gr < ny && gc < nx && p[gr * nx + gc] >= 0;
+    }
+#pragma unroll
+    for (int j = 0; j < kCclPer; j++) lp[threadIdx.x + j * kBlock] = masked[j] ? threadIdx.x + j * kBlock : -1;
+    __syncthreads();
+    // masked cells keep lp >= 0 and unmasked ones -1 throughout, so the
+    // neighbour test may read lp while other threads hook roots
+#pragma unroll
+    for (int j = 0; j < kCclPer; j++) {
+        if (!masked[j]) continue;
+        const int e = threadIdx.x + j * kBlock;
+        const int lr = e / kCcl, lc = e % kCcl;
+        if (lc + 1 < kCcl && lds_load(lp, e + 1) >= 0) lds_unite(lp, e, e + 1);
+        if (lr + 1 < kCcl) {
+            if (lds_load(lp, e + kCcl) >= 0) lds_unite(lp, e, e + kCcl);
             if (connectivity == 8) {
-                if (c + 1 < nx && load_parent(p, ii + nx + 1) >= 0) unite(p, ii, ii + nx + 1);
-                if (c > 0 && load_parent(p, ii + nx - 1) >= 0) unite(p, ii, ii + nx - 1);
+                if (lc + 1 < kCcl && lds_load(lp, e + kCcl + 1) >= 0) lds_unite(lp, e, e + kCcl + 1);
+                if (lc > 0 && lds_load(lp, e + kCcl - 1) >= 0) lds_unite(lp, e, e + kCcl - 1);
+            }
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < kCclPer; j++) {
+        if (!masked[j]) continue;
+        const int e = threadIdx.x + j * kBlock;
+        const int r = lds_find(lp, e);
+        p[(tr0 + e / kCcl) * nx + tc0 + e % kCcl] = (tr0 + r / kCcl) * nx + tc0 + r % kCcl;
+    }
+}
+
+// edges crossing tile boundaries: per tile 32 right-column cells (E, SE),
+// 32 bottom-row cells (S, SE, SW) and 32 left-column cells (SW)
+__global__ void __launch_bounds__(kBlock) ccl_border_kernel(int32_t *p, int ny, int nx, int connectivity,
+                                                             int tiles_x, int tiles)
+{
+    const int total = tiles * 3 * kCcl;
+    for (int idx = blockIdx.x * blockDim.x + threadIdx.x; idx < total; idx += gridDim.x * blockDim.x) {
+        const int t = idx / (3 * kCcl), q = idx % (3 * kCcl);
+        const int tr0 = (t / tiles_x) * kCcl, tc0 = (t % tiles_x) * kCcl;
+        int r, c;
+        if (q < kCcl) { r = tr0 + q; c = tc0 + kCcl - 1; }
+        else if (q < 2 * kCcl) { r = tr0 + kCcl - 1; c = tc0 + q - kCcl; }
+        else { r = tr0 + q - 2 * kCcl; c = tc0; }
+        if (r >= ny || c >= nx) continue;
+        const int i = r * nx + c;
+        if (load_parent(p, i) < 0) continue;
+        const bool right = q < kCcl, bottom = q >= kCcl && q < 2 * kCcl, left = q >= 2 * kCcl;
+        if (right && c + 1 < nx && load_parent(p, i + 1) >= 0) unite(p, i, i + 1);
+        if (r + 1 < ny) {
+            if (bottom && load_parent(p, i + nx) >= 0) unite(p, i, i + nx);
+            if (connectivity == 8) {
+                if ((right || bottom) && c + 1 < nx && load_parent(p, i + nx + 1) >= 0) unite(p, i, i + nx + 1);
+                if ((left || bottom) && c > 0 && load_parent(p, i + nx - 1) >= 0) unite(p, i, i + nx - 1);
             }
         }
     }
 }
 
 // final labels (full path to the root) and cell counts per root
-__global__ void __launch_bounds__(kBlock) flatten_kernel(int32_t *p, int64_t n, int32_t *counts)
+__global__ void __launch_bounds__(kBlock) ccl_flatten_kernel(int32_t *p, int ny, int nx, int tiles_x, int32_t *counts)
 {
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
-        int32_t v = p[i];
-        if (v < 0) continue;
-        int32_t r = (int32_t)i;
-        while (true) {
-            int32_t nx_ = p[r];
-            if (nx_ == r) break;
-            r = nx_;
-        }
-        // every read above observes a completed union pass; the write of this
-        // cell's own entry cannot change any other cell's root
-        p[i] = r;
-        if (counts) atomicAdd(&counts[r], 1);
+    __shared__ int keys[kCclHash];
+    __shared__ int vals[kCclHash];
+    const int t = blockIdx.x;
+    const int tr0 = (t / tiles_x) * kCcl, tc0 = (t % tiles_x) * kCcl;
+    if (counts && threadIdx.x < kCclHash) {
+        keys[threadIdx.x] = -1;
+        vals[threadIdx.x] = 0;
     }
+    int root[kCclPer];
+#pragma unroll
+    for (int j = 0; j < kCclPer; j++) {
+        const int e = threadIdx.x + j * kBlock;
+        const int gr = tr0 + e / kCcl, gc = tc0 + e % kCcl;
+        root[j] = -1;
+        if (gr >= ny || gc >= nx) continue;
+        const int i = gr * nx + gc;
+        int r = p[i];
+        if (r < 0) continue;
+        while (true) {
+            const int up = p[r];
+            if (up == r) break;
+            r = up;
+        }
+        root[j] = r;
+    }
+    // every read above observes the completed border pass; writing this
+    // cell's own entry (to its root, an ancestor) cannot change another root
+#pragma unroll
+    for (int j = 0; j < kCclPer; j++) {
+        if (root[j] < 0) continue;
+        const int e = threadIdx.x + j * kBlock;
+        p[(tr0 + e / kCcl) * nx + tc0 + e % kCcl] = root[j];
+    }
+    if (!counts) return;
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < kCclPer; j++) {
+        const int r = root[j];
+        if (r < 0) continue;
+        int h = (int)(((unsigned)r * 2654435761u) >> 25) & (kCclHash - 1);
+        bool done = false;
+        for (int probe = 0; probe < kCclHash && !done; probe++) {
+            const int prev = atomicCAS(&keys[h], -1, r);
+            if (prev == -1 || prev == r) {
+                atomicAdd(&vals[h], 1);
+                done = true;
+            }
+            h = (h + 1) & (kCclHash - 1);
+        }
+        if (!done) atomicAdd(&counts[r], 1);
+    }
+    __syncthreads();
+    if (threadIdx.x < kCclHash && keys[threadIdx.x] >= 0) atomicAdd(&counts[keys[threadIdx.x]], vals[threadIdx.x]);
 }
 
 __global__ void fill32(int32_t *p, int64_t n, int32_t v)
@@ -179,33 +329,68 @@ __global__ void fill32(int32_t *p, int64_t n, int32_t v)
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) p[i] = v;
 }
 
-// region count / largest / unstable (reduce_components :234-243)
-__global__ void __launch_bounds__(kBlock) count_kernel(const int32_t *counts, int64_t n, LabelAux *aux)
+// zero the per-root counts, the statistics record and the spread sets
+__global__ void label_reset_kernel(int32_t *counts, int64_t n, LabelAux *aux, int32_t n_plus_one,
+                                   unsigned long long *spread)
 {
-    __shared__ double smem[16];
-    double v[2] = {0.0, 0.0};       // regions, unstable (exact in double below 2^53)
-    double mx[1] = {0.0};
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
-        int32_t c = counts[i];
-        if (c > 0) {
-            v[0] += 1.0;
-            v[1] += (double)c;
-            mx[0] = mx[0] >= (double)c ? mx[0] : (double)c;
-        }
-    }
-    block_reduce<double, 2, 0>(v, smem);
-    block_reduce<double, 1, 1>(mx, smem);
-    if (threadIdx.x == 0) {
-        atomicAdd(&aux->diag_i[0], (int32_t)v[0]);
-        atomicAdd(&aux->diag_i[3], (int32_t)v[1]);
-        atomicMax(&aux->diag_i[1], (int32_t)mx[0]);
+    const int64_t tid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t nthreads = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i = tid; i < n; i += nthreads) counts[i] = 0;
+    for (int64_t i = tid; i < 3 * kSpread; i += nthreads) spread[i * kSpreadStride] = 0ull;
+    if (tid == 0) {
+        for (int k = 0; k < 8; k++) { aux->diag_i[k] = 0; aux->diag_f[k] = 0.0; }
+        aux->diag_i[2] = n_plus_one;
     }
 }
 
-// select_winner (:246-252): smallest root among the largest components
-__global__ void __launch_bounds__(kBlock) winner_kernel(const int32_t *counts, int64_t n, LabelAux *aux)
+// region count / largest / unstable (reduce_components :234-243) into spread
+// sets 0 (regions), 1 (unstable cells) and 2 (largest); integer sums, so the
+// order of the atomics does not change the result
+__global__ void __launch_bounds__(kBlock) count_kernel(const int32_t *counts, int64_t n, unsigned long long *spread)
 {
-    const int32_t largest = aux->diag_i[1];
+    __shared__ long long smem[16];
+    long long v[2] = {0, 0};
+    long long mx[1] = {0};
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        const int32_t c = counts[i];
+        if (c > 0) {
+            v[0] += 1;
+            v[1] += c;
+            mx[0] = mx[0] >= c ? mx[0] : c;
+        }
+    }
+    block_reduce<long long, 2, 0>(v, smem);
+    block_reduce<long long, 1, 1>(mx, smem);
+    if (threadIdx.x == 0) {
+        const int slot = (blockIdx.x % kSpread) * kSpreadStride;
+        if (v[0]) atomicAdd(&spread[slot], (unsigned long long)v[0]);
+        if (v[1]) atomicAdd(&spread[kSpreadWords + slot], (unsigned long long)v[1]);
+        if (mx[0]) atomicMax(&spread[2 * kSpreadWords + slot], (unsigned long long)mx[0]);
+    }
+}
+
+// select_winner (:246-252): smallest root among the largest components; the
+// first workgroup also folds the spread sets into the statistics record
+__global__ void __launch_bounds__(kBlock) winner_kernel(const int32_t *counts, int64_t n,
+                                                         const unsigned long long *spread, LabelAux *aux)
+{
+    __shared__ long long smem[4];
+    if (threadIdx.x < kWave) {
+        const int slot = threadIdx.x * kSpreadStride;
+        const long long regions = wave_sum((long long)spread[slot]);
+        const long long unstable = wave_sum((long long)spread[kSpreadWords + slot]);
+        const long long largest = wave_max((long long)spread[2 * kSpreadWords + slot]);
+        if (threadIdx.x == 0) {
+            smem[0] = largest;
+            if (blockIdx.x == 0) {
+                aux->diag_i[0] = (int32_t)regions;
+                aux->diag_i[1] = (int32_t)largest;
+                aux->diag_i[3] = (int32_t)unstable;
+            }
+        }
+    }
+    __syncthreads();
+    const int32_t largest = (int32_t)smem[0];
     if (largest <= 0) return;
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
         if (counts[i] == largest) atomicMin(&aux->diag_i[2], (int32_t)i);
@@ -225,7 +410,7 @@ __global__ void __launch_bounds__(kBlock) winner_stats_kernel(
     const int64_t n = (int64_t)ny * nx;
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
         if (labels[i] < 0 || labels[i] != winner) continue;
-        const int r = (int)(i / nx), c = (int)(i - (int64_t)r * nx);
+        const int r = (int)i / nx, c = (int)i - r * nx;
         const double w = cell_weight(cfg.weight_mode, ny, nx, area, f6, r, c);
         prev += previous ? previous[i] : 0;
         double es = fmax(slope[i] - cfg.start_rad, 0.0);
@@ -244,12 +429,12 @@ __global__ void __launch_bounds__(kBlock) winner_stats_kernel(
     }
     block_reduce<double, 5, 0>(s, smem);
     block_reduce<double, 2, 1>(m, smem);
-    int ptot = prev;
-    for (int off = kWave / 2; off > 0; off >>= 1) ptot += __shfl_down(ptot, off, kWave);
+    const int ptot = wave_sum(prev);
     if ((threadIdx.x & (kWave - 1)) == 0 && ptot) atomicAdd(&aux->diag_i[4], ptot);
     if (threadIdx.x == 0) {
-        double *pp = partials + (size_t)blockIdx.x * kWinF;
-        pp[0] = s[0]; pp[1] = m[0]; pp[2] = s[1]; pp[3] = m[1]; pp[4] = s[2]; pp[5] = s[3]; pp[6] = s[4];
+        // field-major partials: field f of workgroup g at [f * gridDim.x + g]
+        const double vals[kWinF] = {s[0], m[0], s[1], m[1], s[2], s[3], s[4]};
+        for (int f = 0; f < kWinF; f++) partials[(size_t)f * gridDim.x + blockIdx.x] = vals[f];
     }
 }
 
@@ -259,7 +444,9 @@ __global__ void __launch_bounds__(kBlock) winner_final_kernel(const double *part
     double s[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
     double m[2] = {0.0, 0.0};
     for (int p = threadIdx.x; p < nparts; p += blockDim.x) {
-        const double *pp = partials + (size_t)p * kWinF;
+        double pp[kWinF];
+#pragma unroll
+        for (int f = 0; f < kWinF; f++) pp[f] = partials[(size_t)f * nparts + p];
         s[0] += pp[0]; s[1] += pp[2]; s[2] += pp[4]; s[3] += pp[5]; s[4] += pp[6];
         m[0] = fmax(m[0], pp[1]);
         m[1] = fmax(m[1], pp[3]);
@@ -278,14 +465,6 @@ __global__ void __launch_bounds__(kBlock) winner_final_kernel(const double *part
     }
 }
 
-__global__ void aux_init(LabelAux *aux, int32_t n_plus_one)
-{
-    if (threadIdx.x == 0) {
-        for (int k = 0; k < 8; k++) { aux->diag_i[k] = 0; aux->diag_f[k] = 0.0; }
-        aux->diag_i[2] = n_plus_one;
-    }
-}
-
 // update_previous_and_tiles (:284-306)
 __global__ void __launch_bounds__(kBlock) previous_kernel(const int32_t *labels, int32_t *previous, int32_t *reached,
                                                           int32_t *tile_flags, int64_t n, int nx, int32_t winner,
@@ -296,7 +475,7 @@ __global__ void __launch_bounds__(kBlock) previous_kernel(const int32_t *labels,
         previous[i] = (belongs && keep) ? 1 : 0;
         if (belongs && seed) {
             reached[i] = 1;
-            int row = (int)(i / nx), col = (int)(i - (int64_t)row * nx);
+            int row = (int)i / nx, col = (int)i - row * nx;
             atomicMax(&tile_flags[(row / tile_size) * tiles_x + col / tile_size], 1);
         }
     }
@@ -333,7 +512,7 @@ __global__ void __launch_bounds__(kBlock) mobilize_kernel(
         }
         f.owned_export_baseline[i] = f.export_cumulative[i];
         f.latch[i] = 1;
-        const int r = (int)(i / nx), c = (int)(i - (int64_t)r * nx);
+        const int r = (int)i / nx, c = (int)i - r * nx;
         const double w = cell_weight(cfg.weight_mode, ny, nx, area, f6, r, c);
         double weighted = transfer * w;
         int32_t prev = f.activation_count[i];
@@ -352,7 +531,7 @@ __global__ void __launch_bounds__(kBlock) mobilize_kernel(
     }
     block_reduce<double, kMobF, 0>(s, smem);
     if (threadIdx.x == 0)
-        for (int k = 0; k < kMobF; k++) partials[(size_t)blockIdx.x * kMobF + k] = s[k];
+        for (int k = 0; k < kMobF; k++) partials[(size_t)k * gridDim.x + blockIdx.x] = s[k];
 }
 
 __global__ void __launch_bounds__(kBlock) mobilize_final_kernel(const double *partials, int nparts, double *out)
@@ -360,7 +539,7 @@ __global__ void __launch_bounds__(kBlock) mobilize_final_kernel(const double *pa
     __shared__ double smem[kMobF * 4];
     double s[kMobF] = {0.0, 0.0, 0.0, 0.0, 0.0};
     for (int p = threadIdx.x; p < nparts; p += blockDim.x)
-        for (int k = 0; k < kMobF; k++) s[k] += partials[(size_t)p * kMobF + k];
+        for (int k = 0; k < kMobF; k++) s[k] += partials[(size_t)k * nparts + p];
     block_reduce<double, kMobF, 0>(s, smem);
     if (threadIdx.x == 0)
         for (int k = 0; k < kMobF; k++) out[k] = s[k];
@@ -376,11 +555,14 @@ static int check_ctx(mv2_context *ctx)
 
 static int label_core(mv2_context *ctx, int32_t *labels, int ny, int nx, int connectivity, int32_t *counts)
 {
-    const int64_t n = (int64_t)ny * nx;
-    const int grid = grid_for(n, kBlock, 4096);
-    if (counts) hipLaunchKernelGGL(fill32, dim3(grid), dim3(kBlock), 0, ctx->stream, counts, n, 0);
-    hipLaunchKernelGGL(union_kernel, dim3(grid), dim3(kBlock), 0, ctx->stream, labels, ny, nx, connectivity);
-    hipLaunchKernelGGL(flatten_kernel, dim3(grid), dim3(kBlock), 0, ctx->stream, labels, n, counts);
+    const int tiles_x = (nx + kCcl - 1) / kCcl;
+    const int tiles = tiles_x * ((ny + kCcl - 1) / kCcl);
+    hipLaunchKernelGGL(ccl_local_kernel, dim3(tiles), dim3(kBlock), 0, ctx->stream, labels, ny, nx, connectivity,
+                       tiles_x);
+    hipLaunchKernelGGL(ccl_border_kernel, dim3(grid_for((int64_t)tiles * 3 * kCcl)), dim3(kBlock), 0, ctx->stream,
+                       labels, ny, nx, connectivity, tiles_x, tiles);
+    hipLaunchKernelGGL(ccl_flatten_kernel, dim3(tiles), dim3(kBlock), 0, ctx->stream, labels, ny, nx, tiles_x,
+                       counts);
     MV2_HIP(ctx, hipGetLastError());
     return MV2_OK;
 }
@@ -424,11 +606,14 @@ extern "C" int mv2_avalanche_label(mv2_context *ctx, const mv2_avalanche_config 
     const int grid = grid_for(n, kBlock, 4096);
     hipLaunchKernelGGL(avalanche_init_kernel, dim3(grid), dim3(kBlock), 0, ctx->stream, ctx->z_base, ctx->b_eff,
                        ctx->mobile, ctx->qx, ctx->qy, f32, ny, nx, ctx->dx, ctx->dy, *cfg, area, f6, *f);
+    if (int rc = ensure_scratch(ctx, ctx->spread, kSpreadScratchBytes)) return rc;
+    unsigned long long *spread = (unsigned long long *)ctx->spread.ptr + (size_t)kLabelSpreadSet * kSpreadWords;
+    hipLaunchKernelGGL(label_reset_kernel, dim3(grid_for(n, kBlock, 1024)), dim3(kBlock), 0, ctx->stream, f->counts,
+                       n, aux, (int32_t)(n + 1), spread);
     if (int rc = label_core(ctx, f->labels, ny, nx, cfg->connectivity, f->counts)) return rc;
-    hipLaunchKernelGGL(aux_init, dim3(1), dim3(64), 0, ctx->stream, aux, (int32_t)(n + 1));
     const int sgrid = grid_for(n, kBlock, kLabelParts);
-    hipLaunchKernelGGL(count_kernel, dim3(sgrid), dim3(kBlock), 0, ctx->stream, f->counts, n, aux);
-    hipLaunchKernelGGL(winner_kernel, dim3(sgrid), dim3(kBlock), 0, ctx->stream, f->counts, n, aux);
+    hipLaunchKernelGGL(count_kernel, dim3(sgrid), dim3(kBlock), 0, ctx->stream, f->counts, n, spread);
+    hipLaunchKernelGGL(winner_kernel, dim3(sgrid), dim3(kBlock), 0, ctx->stream, f->counts, n, spread, aux);
     hipLaunchKernelGGL(winner_stats_kernel, dim3(sgrid), dim3(kBlock), 0, ctx->stream, f->labels, ctx->z_base,
                        ctx->b_eff, f32, f->latch, f->previous, f->slope, f->severity, ny, nx, *cfg, area, f6, aux,
                        partials);

@@ -2,16 +2,17 @@
 // slope_model.py:273-343) and the compact colour-batch sweep
 // (pkg/solvers/warp_compact_frontier.py:57-130).
 //
-// relax: one cooperative launch runs the whole data-dependent iteration loop.
-// Each workgroup stages a tile plus an 8-cell halo in LDS and performs all 8
-// red-black sub-phases of an iteration there.  A sub-phase moves information
-// one cell along its direction, so after the 8 sub-phases (two per direction,
-// directions x, y and both diagonals) at most 6 halo layers are stale; the
-// interior and every edge it owns are exact.  Then the interior is written
-// (ping-pong buffers), the per-iteration maximum excess is reduced with one
-// atomicMax per workgroup, and one grid barrier decides convergence.  So an
-// iteration costs one read + one write of the field instead of eight
-// full-array passes, and the host is not involved per iteration.
+// relax: one launch per iteration.  Each workgroup stages a tile plus an
+// 8-cell halo in LDS and performs all 8 red-black sub-phases of an iteration
+// there.  A sub-phase moves information one cell along its direction, so after
+// the 8 sub-phases (two per direction, directions x, y and both diagonals) at
+// most 6 halo layers are stale; the interior and every edge it owns are exact.
+// Then the interior is written (ping-pong buffers) and the iteration's maximum
+// excess is reduced into a spread maximum (64 slots).  The next launch tests
+// it against the tolerance on the device and returns at once once converged,
+// so the host enqueues batches of launches and reads one result per batch.
+// An iteration costs one read + one write of the field instead of eight
+// full-array passes.
 //
 // Directions and parities follow the reference's legacy H[x, y] convention
 // behind MinimumSlopeAdapter (pkg/solvers/minimum_slope_adapter.py:170-179):
@@ -33,10 +34,18 @@ constexpr int kRelaxTW = 48;   // interior columns per tile
 constexpr int kRelaxLH = kRelaxTH + 2 * kRelaxHalo;
 constexpr int kRelaxLW = kRelaxTW + 2 * kRelaxHalo;
 
+// Iteration state: launch k reads ctl[k & 1], its workgroup 0 writes
+// ctl[(k + 1) & 1]; launch k reduces its maximum excess into spread set k % 3,
+// tests set (k - 1) % 3 (the previous iteration) against the tolerance and
+// clears set (k + 1) % 3 for the next launch.
+struct RelaxControl {
+    int iterations;      // iterations applied to the field
+    int src;             // buffer holding the current field
+    int done, converged;
+    double last;         // maximum excess of the last applied iteration
+};
 struct RelaxSync {
-    unsigned counter;
-    unsigned pad;
-    unsigned long long slot[3];
+    RelaxControl ctl[2];
 };
 
 template <class R>
@@ -47,10 +56,30 @@ struct RelaxArgs {
     int max_iterations;
     double tolerance;
     RelaxSync *sync;
+    unsigned long long *spread;   // three spread-maximum sets
     mv2_relax_result *out;
     int tiles_x, tiles;
 };
 
+// loop state for launch k: did iteration k (applied by launch k - 1) converge?
+// Every thread of the block calls it.
+template <class R>
+__device__ inline RelaxControl relax_resolve(const RelaxArgs<R> &a, int k, double *smem1)
+{
+    RelaxControl C = a.sync->ctl[k & 1];      // written by an earlier launch: plain load
+    if (!C.done && k > 0) {
+        C.last = spread_read_max(a.spread + ((k - 1) % 3) * kSpreadWords, smem1);
+        if (C.last <= a.tolerance) {
+            C.converged = 1;
+            C.done = 1;
+        } else if (C.iterations >= a.max_iterations) {
+            C.done = 1;
+        }
+    }
+    return C;
+}
+
+// cells of edge direction d anchored at (r, c): a and b as in the table above
 __device__ inline void edge_cells(int d, int r, int c, int &ra, int &ca, int &rb, int &cb)
 {
     switch (d) {
@@ -68,112 +97,171 @@ __device__ inline R excess_of(R diff, R crit)
     return ex > (R)0 ? ex : (R)0;
 }
 
+// Per-thread work of the LDS phases has a fixed trip count: the tile is
+// kRelaxLH x kRelaxLW = 3072 cells, the owned edges of one direction are the
+// 1536 interior anchors, and one parity selects 1536 anchors of the halo tile.
+// 512 threads (8 waves) per tile: the kernel is latency-bound on LDS and
+// barriers, so more waves per tile hide more of it at the same total work.
+constexpr int kRelaxBlock = 512;
+constexpr int kRelaxStage = kRelaxLH * kRelaxLW / kRelaxBlock;           // 6
+constexpr int kRelaxOwned = kRelaxTH * kRelaxTW / kRelaxBlock;           // 3
+constexpr int kRelaxActive = kRelaxLH * kRelaxLW / 2 / kRelaxBlock;      // 3
+static_assert(kRelaxLH * kRelaxLW % kRelaxBlock == 0 && kRelaxTH * kRelaxTW % kRelaxBlock == 0 &&
+              kRelaxLH * kRelaxLW / 2 % kRelaxBlock == 0, "relaxation tile must split evenly over the block");
+static_assert(kRelaxHalo % 2 == 0 && kRelaxTH % 2 == 0 && kRelaxTW % 2 == 0,
+              "tile origins must be even so local and global parities agree");
+
+// One relaxation iteration (slope_model.py:296-338) per launch; see the file
+// comment.  Every phase first loads all the operands a thread needs, then
+// computes, then stores: the edges of one parity are disjoint, so the loads
+// of one thread's edges never depend on its own stores.
 template <class R>
-__global__ void __launch_bounds__(kBlock) relax_kernel(RelaxArgs<R> a)
+__global__ void __launch_bounds__(kRelaxBlock) relax_kernel(RelaxArgs<R> a, int k)
 {
     __shared__ R tile[kRelaxLH * kRelaxLW];
     __shared__ double smem[8];
-    unsigned epoch = 0;
     const int ny = a.ny, nx = a.nx;
-    int src = 0;
-    int iteration = 0;
-    int converged = 0;
-    double last = 0.0;
-    for (iteration = 1; iteration <= a.max_iterations; iteration++) {
-        if (blockIdx.x == 0 && threadIdx.x == 0)
-            __hip_atomic_store(&a.sync->slot[(iteration + 1) % 3], 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        double wg_max = 0.0;
-        for (int t = blockIdx.x; t < a.tiles; t += gridDim.x) {
-            const int r0 = (t / a.tiles_x) * kRelaxTH, c0 = (t % a.tiles_x) * kRelaxTW;
-            const int R0 = r0 - kRelaxHalo, C0 = c0 - kRelaxHalo;
-            // stage tile + halo (cells outside the grid are never touched)
-            for (int k = threadIdx.x; k < kRelaxLH * kRelaxLW; k += blockDim.x) {
-                int lr = k / kRelaxLW, lc = k - lr * kRelaxLW;
-                int gr = R0 + lr, gc = C0 + lc;
-                R v = (R)0;
-                if (gr >= 0 && gr < ny && gc >= 0 && gc < nx) v = a.h[src][(int64_t)gr * nx + gc];
-                tile[k] = v;
-            }
-            __syncthreads();
-            R tmax = (R)0;
-            for (int d = 0; d < 4; d++) {
-                const R crit = a.crit[d];
-                for (int phase = 0; phase < 2; phase++) {
-                    // maximum excess over the edges this tile owns (pre-phase state)
-                    for (int k = threadIdx.x; k < kRelaxTH * kRelaxTW; k += blockDim.x) {
-                        int ir = k / kRelaxTW, ic = k - ir * kRelaxTW;
-                        int r = r0 + ir, c = c0 + ic;
-                        int ra, ca, rb, cb;
-                        edge_cells(d, r, c, ra, ca, rb, cb);
-                        if (r >= ny || c >= nx || ra >= ny || rb >= ny || ca >= nx || cb >= nx) continue;
-                        R va = tile[(ra - R0) * kRelaxLW + (ca - C0)];
-                        R vb = tile[(rb - R0) * kRelaxLW + (cb - C0)];
-                        R ex = excess_of<R>(va - vb, crit);
-                        tmax = tmax >= ex ? tmax : ex;
-                    }
-                    __syncthreads();
-                    // red-black transfer on the active parity (disjoint edges)
-                    for (int k = threadIdx.x; k < kRelaxLH * kRelaxLW; k += blockDim.x) {
-                        int lr = k / kRelaxLW, lc = k - lr * kRelaxLW;
-                        int r = R0 + lr, c = C0 + lc;
-                        int coord = d == 1 ? r : c;
-                        if (r < 0 || c < 0 || ((coord & 1) != phase)) continue;
-                        int ra, ca, rb, cb;
-                        edge_cells(d, r, c, ra, ca, rb, cb);
-                        if (ra >= ny || rb >= ny || ca >= nx || cb >= nx) continue;
-                        int la_r = ra - R0, la_c = ca - C0, lb_r = rb - R0, lb_c = cb - C0;
-                        if (la_r < 0 || la_r >= kRelaxLH || lb_r < 0 || lb_r >= kRelaxLH ||
-                            la_c < 0 || la_c >= kRelaxLW || lb_c < 0 || lb_c >= kRelaxLW)
-                            continue;
-                        R &va = tile[la_r * kRelaxLW + la_c];
-                        R &vb = tile[lb_r * kRelaxLW + lb_c];
-                        R diff = va - vb;
-                        R ex = excess_of<R>(diff, crit);
-                        R sgn = diff > (R)0 ? (R)1 : (diff < (R)0 ? (R)-1 : (R)0);
-                        R tr = ((R)0.5 * ex) * sgn;
-                        va = va - tr;
-                        vb = vb + tr;
-                    }
-                    __syncthreads();
+    RelaxSync *s = a.sync;
+    const bool lead = blockIdx.x == 0 && threadIdx.x == 0;
+    const RelaxControl C = relax_resolve(a, k, smem);
+    if (C.done) {
+        if (lead) s->ctl[(k + 1) & 1] = C;
+        return;
+    }
+    const int src = C.src;
+    if (blockIdx.x == 0) spread_clear(a.spread + ((k + 1) % 3) * kSpreadWords);
+    if (lead) {
+        RelaxControl nc = C;
+        nc.iterations = C.iterations + 1;
+        nc.src = src ^ 1;
+        s->ctl[(k + 1) & 1] = nc;
+    }
+    const int t = blockIdx.x;
+    const int r0 = (t / a.tiles_x) * kRelaxTH, c0 = (t % a.tiles_x) * kRelaxTW;
+    const int R0 = r0 - kRelaxHalo, C0 = c0 - kRelaxHalo;
+    // stage tile + halo in one round trip (cells outside the grid are never touched)
+    {
+        R v[kRelaxStage];
+#pragma unroll
+        for (int j = 0; j < kRelaxStage; j++) {
+            const int e = threadIdx.x + j * kRelaxBlock;
+            const int gr = R0 + e / kRelaxLW, gc = C0 + e % kRelaxLW;
+            v[j] = (R)0;
+            if (gr >= 0 && gr < ny && gc >= 0 && gc < nx) v[j] = a.h[src][gr * nx + gc];
+        }
+#pragma unroll
+        for (int j = 0; j < kRelaxStage; j++) tile[threadIdx.x + j * kRelaxBlock] = v[j];
+    }
+    __syncthreads();
+    R tmax = (R)0;
+#pragma unroll 1
+    for (int d = 0; d < 4; d++) {
+        const R crit = a.crit[d];
+#pragma unroll 1
+        for (int phase = 0; phase < 2; phase++) {
+            // maximum excess over every edge of direction d that this tile owns
+            // (anchor in the interior), in the state before the sub-phase
+            {
+                R va[kRelaxOwned], vb[kRelaxOwned];
+                bool ok[kRelaxOwned];
+#pragma unroll
+                for (int j = 0; j < kRelaxOwned; j++) {
+                    const int e = threadIdx.x + j * kRelaxBlock;
+                    const int r = r0 + e / kRelaxTW, c = c0 + e % kRelaxTW;
+                    int ra, ca, rb, cb;
+                    edge_cells(d, r, c, ra, ca, rb, cb);
+                    ok[j] = r < ny && c < nx && ra < ny && rb < ny && ca < nx && cb < nx;
+                    va[j] = tile[(ra - R0) * kRelaxLW + (ca - C0)];
+                    vb[j] = tile[(rb - R0) * kRelaxLW + (cb - C0)];
+                }
+#pragma unroll
+                for (int j = 0; j < kRelaxOwned; j++) {
+                    const R ex = ok[j] ? excess_of<R>(va[j] - vb[j], crit) : (R)0;
+                    tmax = tmax >= ex ? tmax : ex;
                 }
             }
-            wg_max = wg_max >= (double)tmax ? wg_max : (double)tmax;
-            for (int k = threadIdx.x; k < kRelaxTH * kRelaxTW; k += blockDim.x) {
-                int ir = k / kRelaxTW, ic = k - ir * kRelaxTW;
-                int r = r0 + ir, c = c0 + ic;
-                if (r < ny && c < nx)
-                    a.h[src ^ 1][(int64_t)r * nx + c] = tile[(ir + kRelaxHalo) * kRelaxLW + ic + kRelaxHalo];
+            __syncthreads();
+            // red-black transfer on the active parity (disjoint edges); the
+            // parity coordinate is c (d = 0, 2, 3) or r (d = 1)
+            {
+                int ia[kRelaxActive], ib[kRelaxActive];
+                R va[kRelaxActive], vb[kRelaxActive];
+                bool ok[kRelaxActive];
+#pragma unroll
+                for (int j = 0; j < kRelaxActive; j++) {
+                    const int e = threadIdx.x + j * kRelaxBlock;
+                    int lr, lc;
+                    if (d == 1) {
+                        lr = 2 * (e / kRelaxLW) + phase;
+                        lc = e % kRelaxLW;
+                    } else {
+                        lr = e / (kRelaxLW / 2);
+                        lc = 2 * (e % (kRelaxLW / 2)) + phase;
+                    }
+                    const int r = R0 + lr, c = C0 + lc;
+                    int ra, ca, rb, cb;
+                    edge_cells(d, r, c, ra, ca, rb, cb);
+                    const int la_r = ra - R0, la_c = ca - C0, lb_r = rb - R0, lb_c = cb - C0;
+                    ok[j] = r >= 0 && c >= 0 && ra < ny && rb < ny && ca < nx && cb < nx && la_r >= 0 &&
+                            la_r < kRelaxLH && lb_r >= 0 && lb_r < kRelaxLH && la_c >= 0 && la_c < kRelaxLW &&
+                            lb_c >= 0 && lb_c < kRelaxLW;
+                    ia[j] = ok[j] ? la_r * kRelaxLW + la_c : 0;
+                    ib[j] = ok[j] ? lb_r * kRelaxLW + lb_c : 0;
+                    va[j] = tile[ia[j]];
+                    vb[j] = tile[ib[j]];
+                }
+#pragma unroll
+                for (int j = 0; j < kRelaxActive; j++) {
+                    if (!ok[j]) continue;
+                    const R diff = va[j] - vb[j];
+                    const R ex = excess_of<R>(diff, crit);
+                    const R sgn = diff > (R)0 ? (R)1 : (diff < (R)0 ? (R)-1 : (R)0);
+                    const R tr = ((R)0.5 * ex) * sgn;
+                    tile[ia[j]] = va[j] - tr;
+                    tile[ib[j]] = vb[j] + tr;
+                }
             }
             __syncthreads();
         }
-        double m[1] = {wg_max};
-        block_reduce<double, 1, 1>(m, smem);
-        if (threadIdx.x == 0) atomic_max_nonneg(&a.sync->slot[iteration % 3], m[0]);
-        grid_barrier(&a.sync->counter, epoch);
-        src ^= 1;
-        last = __longlong_as_double((long long)__hip_atomic_load(&a.sync->slot[iteration % 3], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
-        if (last <= a.tolerance) { converged = 1; break; }
     }
-    if (iteration > a.max_iterations) iteration = a.max_iterations;
-    if (src == 1) {
-        const int64_t n = (int64_t)ny * nx;
+    // write the interior
+#pragma unroll
+    for (int j = 0; j < kRelaxOwned; j++) {
+        const int e = threadIdx.x + j * kRelaxBlock;
+        const int ir = e / kRelaxTW, ic = e % kRelaxTW;
+        const int r = r0 + ir, c = c0 + ic;
+        if (r < ny && c < nx) a.h[src ^ 1][r * nx + c] = tile[(ir + kRelaxHalo) * kRelaxLW + ic + kRelaxHalo];
+    }
+    double m[1] = {(double)tmax};
+    block_reduce<double, 1, 1, kRelaxBlock / kWave>(m, smem);
+    if (threadIdx.x == 0) spread_max(a.spread + (k % 3) * kSpreadWords, m[0]);
+}
+
+// after launch k - 1: copy back when the field sits in scratch, write the result
+template <class R>
+__global__ void __launch_bounds__(kBlock) relax_end_kernel(RelaxArgs<R> a, int k)
+{
+    __shared__ double smem[1];
+    const RelaxControl C = relax_resolve(a, k, smem);
+    if (C.src == 1) {
+        const int64_t n = (int64_t)a.ny * a.nx;
         for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
              i += (int64_t)gridDim.x * blockDim.x)
             a.h[0][i] = a.h[1][i];
     }
     if (blockIdx.x == 0 && threadIdx.x == 0) {
-        a.out->iterations = iteration;
-        a.out->converged = converged;
-        a.out->max_excess_last = last;
+        a.out->iterations = C.iterations;
+        a.out->converged = C.converged;
+        a.out->max_excess_last = C.last;
+        a.out->complete = C.done;
     }
 }
 
-__global__ void zero_relax_sync(RelaxSync *s)
+__global__ void zero_relax_sync(RelaxSync *s, unsigned long long *spread)
 {
-    if (threadIdx.x == 0) {
-        s->counter = 0;
-        s->slot[0] = s->slot[1] = s->slot[2] = 0ull;
-    }
+    unsigned *p = reinterpret_cast<unsigned *>(s);
+    for (int i = threadIdx.x; i < (int)(sizeof(RelaxSync) / 4); i += blockDim.x) p[i] = 0u;
+    for (int i = threadIdx.x; i < 3 * kSpread; i += blockDim.x) spread[(int64_t)i * kSpreadStride] = 0ull;
 }
 
 template <class R>
@@ -192,23 +280,37 @@ static int launch_relax(mv2_context *ctx, void *height, int ny, int nx, const do
     a.tolerance = tolerance;
     // the relaxation keeps its own barrier block after the V2 sync block
     a.sync = (RelaxSync *)((char *)ctx->sync.ptr + 1024);
+    if (int rc = ensure_scratch(ctx, ctx->spread, kSpreadScratchBytes)) return rc;
+    a.spread = (unsigned long long *)ctx->spread.ptr + (size_t)kRelaxSpreadSet * kSpreadWords;
     a.out = (mv2_relax_result *)((char *)ctx->result.ptr + 2048);
     a.tiles_x = (nx + kRelaxTW - 1) / kRelaxTW;
     a.tiles = a.tiles_x * ((ny + kRelaxTH - 1) / kRelaxTH);
-    int per_cu = 0;
-    MV2_HIP(ctx, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, relax_kernel<R>, kBlock, 0));
-    if (per_cu < 1) return set_error(ctx, MV2_EDEVICE, "[MiniSlope] relaxation kernel cannot be resident");
-    int grid = per_cu * ctx->cu_count;
-    if (grid > a.tiles) grid = a.tiles;
-    hipLaunchKernelGGL(zero_relax_sync, dim3(1), dim3(64), 0, ctx->stream, a.sync);
+    hipStream_t st = ctx->stream;
+    const int gend = grid_for((int64_t)n, kBlock, 4 * ctx->cu_count);
+    hipLaunchKernelGGL(zero_relax_sync, dim3(1), dim3(kBlock), 0, st, a.sync, a.spread);
     MV2_HIP(ctx, hipGetLastError());
-    void *args[] = {&a};
-    MV2_HIP(ctx, hipLaunchCooperativeKernel((const void *)relax_kernel<R>, dim3(grid), dim3(kBlock), args, 0,
-                                            ctx->stream));
-    MV2_HIP(ctx, hipMemcpyAsync(ctx->host_result.ptr, a.out, sizeof(mv2_relax_result), hipMemcpyDeviceToHost,
-                                ctx->stream));
-    MV2_HIP(ctx, hipStreamSynchronize(ctx->stream));
-    memcpy(out, ctx->host_result.ptr, sizeof(mv2_relax_result));
+    // One launch per iteration; launches after convergence return at once.
+    // Enqueue about as many as the previous call needed (at most 32), then
+    // grow the batch geometrically while the field has not converged.
+    int chunk = ctx->relax_predicted + 1;
+    if (chunk < 2) chunk = 2;
+    if (chunk > 32) chunk = 32;
+    int k = 0;
+    const mv2_relax_result *hres = (const mv2_relax_result *)ctx->host_result.ptr;
+    for (;;) {
+        if (chunk > max_iterations - k) chunk = max_iterations - k;
+        for (int j = 0; j < chunk; j++, k++)
+            hipLaunchKernelGGL(relax_kernel<R>, dim3(a.tiles), dim3(kRelaxBlock), 0, st, a, k);
+        hipLaunchKernelGGL(relax_end_kernel<R>, dim3(gend), dim3(kBlock), 0, st, a, k);
+        MV2_HIP(ctx, hipGetLastError());
+        MV2_HIP(ctx, hipMemcpyAsync(ctx->host_result.ptr, a.out, sizeof(mv2_relax_result), hipMemcpyDeviceToHost, st));
+        MV2_HIP(ctx, hipStreamSynchronize(st));
+        if (hres->complete || k >= max_iterations) break;
+        chunk = k / 2 > 8 ? k / 2 : 8;
+    }
+    memcpy(out, hres, sizeof(mv2_relax_result));
+    out->launches = k;
+    ctx->relax_predicted = out->iterations;
     return MV2_OK;
 }
 

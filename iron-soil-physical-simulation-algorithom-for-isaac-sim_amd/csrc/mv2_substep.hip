@@ -1,12 +1,21 @@
 // Mobile V2 finite-volume step on the device (gfx950).
 //
-// One cooperative (co-resident) launch runs a whole step_resident call:
-// ledger + static diagnostic + first CFL reduction, then the data-dependent
-// substep loop with ONE grid barrier per substep, then the final ledger.
-// The global CFL maximum of the state produced by substep n is reduced while
-// that state is written (wave shuffles + one atomicMax per workgroup), so the
-// host never sees the loop.  Fields ping-pong between the caller's arrays and
-// library scratch; an odd substep count ends with a copy back.
+// A step_resident call is a short stream of ordinary launches:
+//   reset     zero the loop state and the diagnostic partials
+//   begin     ledger before, static diagnostic, CFL maximum of the input
+//   substep k one launch per substep; the loop state lives on the device
+//             (Control below), so launches past the end of the loop return
+//             at once and the host never reads a per-substep value
+//   end       copy back when the final state sits in scratch, ledger after
+//   finalize  one workgroup folds the partials in fixed order into the record
+// The host enqueues as many substep launches as the previous call needed
+// (plus one) and reads the record once; only a call that needs more substeps
+// than enqueued pays a second round trip.  A kernel boundary costs a few
+// microseconds, a grid-wide barrier inside a persistent launch of ~1000
+// workgroups costs tens, which is why the loop is not one cooperative kernel.
+// The CFL maximum of the state produced by substep k is reduced while that
+// state is written (wave shuffles + one atomicMax per workgroup) into a
+// rotating slot that substep k+1 reads.
 //
 // Numerics follow pkg/experimental/mobile_v2_reference.py (CPU reference):
 //   FP64 mode: every face and cell update repeats the reference's operation
@@ -24,28 +33,46 @@
 
 namespace mv2 {
 
-struct SyncBlock {
-    unsigned counter;
-    unsigned pad0;
-    unsigned long long wave_slot[3];     // double bits of the CFL maximum
-    unsigned long long static_grad;      // double bits
-    unsigned long long static_q;         // double bits
-    int positivity;                      // 1 once any cell failed positivity
-    int overflow;                        // 1 once a thickness left the int32 range
-    int pad1[2];
+// Loop state of one step_resident call.  Substep launch k reads ctl[k & 1]
+// and its workgroup 0 writes the successor state into ctl[(k + 1) & 1], so a
+// launch never reads what it writes; the stream order between launches makes
+// the write visible to launch k + 1.
+struct Control {
+    double remaining;          // simulated time still to integrate
+    double max_cfl;
+    double last_remaining;     // state before the most recent substep
+    int substeps, src;         // substeps taken; buffer holding the current state
+    int last_substeps, last_src;
+    int done;                  // loop finished (normally or by an error)
+    int numeric;               // 0 ok, 1 positivity, 2 substep limit, 3 int32 range
+    int cfl_limited, static_exit;
+    int rolled, pad;           // 1 once a failed substep was rolled back
 };
 
-// per-workgroup partial record
+struct SyncBlock {
+    unsigned long long static_grad;      // double bits, folded by the finalize pass
+    unsigned long long static_q;         // double bits
+    int flags[2];                        // launch k ORs into [(k+1)&1]: bit 0 positivity, bit 1 range
+    Control ctl[2];
+};
+
+// spread-maximum sets of one step (kSpreadWords u64 each): the CFL maximum
+// rotates over three sets (launch k reads k % 3, fills (k+1) % 3 and clears
+// (k+2) % 3); two more hold the static diagnostic of the input state
+constexpr int kWaveSets = 3;
+constexpr int kSetStaticGrad = 3;
+constexpr int kSetStaticQ = 4;
+constexpr int kStepSpreadSets = 5;
+
+// per-workgroup partial ledger of the begin / end passes
 constexpr int kSum = 8;                  // summary entries
 constexpr int kDiag = 10;
 constexpr int kClasses = 9;              // weight classes of the exact FP32 mass ledger
-struct Partial {
-    double before[kSum];
-    double after[kSum];
-    double diag[kDiag];
-    long long mass_before[kClasses];
-    long long mass_after[kClasses];
-};
+// Partials are stored field-major (structure of arrays): field f of
+// workgroup g at [f * groups + g], so the folding pass reads each field of
+// consecutive workgroups with coalesced loads.
+constexpr int kLedgerF = 2 * kSum;        // doubles per workgroup: before[8], after[8]
+constexpr int kLedgerQ = 2 * kClasses;    // int64 per workgroup: quanta before[9], after[9]
 
 struct F64P {
     using real = double;
@@ -70,7 +97,12 @@ struct V2Args {
     typename P::real *qy[2];
     mv2_step_config cfg;
     SyncBlock *sync;
-    Partial *partials;
+    double *ledger_f;                    // kLedgerF x groups of the begin / end passes
+    long long *ledger_q;                 // kLedgerQ x groups
+    int groups;                          // workgroups of the begin / end passes
+    double *diag_part;                   // kDiag x (workgroups of the substep launches)
+    int diag_groups;
+    unsigned long long *spread;          // kStepSpreadSets spread-maximum sets
     mv2_step_record *rec;
     int clamp_quarter;                   // FP32: cap outflow per face at h/4
 };
@@ -390,6 +422,7 @@ constexpr int kTH = 16, kTW = 32;
 constexpr int kLH = kTH + 2, kLW = kTW + 2;
 constexpr int kFX = kTH * (kTW + 1);
 constexpr int kFY = (kTH + 1) * kTW;
+static_assert(kTH * kTW % kBlock == 0, "tile cells must split evenly over the block");
 
 struct Tile32 {
     float b[kLH * kLW];
@@ -461,42 +494,66 @@ __device__ inline Consts32 consts32(const mv2_step_config &cfg)
     return k;
 }
 
-// tile t (row-major over tiles_x) -> substep update of its cells
-__device__ void tile_substep32(const V2Args<F32P> &a, Tile32 &T, int src, int t, int tiles_x,
-                               double sub_dt, Acc<F32P> &acc, double &wmax, double &hmin, int &ovf)
+// Stage tile t (row-major over tiles_x) plus its one-cell halo into LDS.  The
+// loop has a fixed trip count and is unrolled, so every load of the thread is
+// in flight at once (one memory round trip for the whole tile).  The caller
+// synchronises before the LDS is read.
+constexpr int kStageIters = (kLH * kLW + kBlock - 1) / kBlock;
+__device__ inline void tile_stage32(const V2Args<F32P> &a, Tile32 &T, int src, int t, int tiles_x)
+{
+    const int ny = a.ny, nx = a.nx;
+    const int r0 = (t / tiles_x) * kTH, c0 = (t % tiles_x) * kTW;
+    const float Qf = (float)kQuantum;
+    const float dry = (float)a.cfg.dry_tolerance;
+    const float *gb = a.b;
+    const int32_t *gh = a.h[src];
+    const float *gqx = a.qx[src];
+    const float *gqy = a.qy[src];
+    float b[kStageIters], qx[kStageIters], qy[kStageIters];
+    int hq[kStageIters];
+#pragma unroll
+    for (int j = 0; j < kStageIters; j++) {
+        const int e = threadIdx.x + j * kBlock;
+        const int lr = e / kLW, lc = e - lr * kLW;
+        const int gr = r0 - 1 + lr, gc = c0 - 1 + lc;
+        b[j] = 0.0f; qx[j] = 0.0f; qy[j] = 0.0f; hq[j] = 0;
+        if (e < kLH * kLW && gr >= 0 && gr < ny && gc >= 0 && gc < nx) {
+            const int i = gr * nx + gc;
+            b[j] = gb[i];
+            hq[j] = gh[i];
+            qx[j] = gqx[i];
+            qy[j] = gqy[i];
+        }
+    }
+#pragma unroll
+    for (int j = 0; j < kStageIters; j++) {
+        const int e = threadIdx.x + j * kBlock;
+        if (e >= kLH * kLW) continue;
+        const float h = (float)hq[j] * Qf;
+        float vx = 0.0f, vy = 0.0f;
+        if (h > dry) {
+            float inv = rcpf(h);
+            vx = qx[j] * inv;
+            vy = qy[j] * inv;
+        }
+        T.b[e] = b[j]; T.h[e] = h; T.hq[e] = hq[j]; T.qx[e] = qx[j]; T.qy[e] = qy[j]; T.vx[e] = vx; T.vy[e] = vy;
+    }
+}
+
+// faces and cell update of the staged tile t (after a barrier)
+__device__ void tile_update32(const V2Args<F32P> &a, Tile32 &T, int src, int t, int tiles_x,
+                              double sub_dt, Acc<F32P> &acc, double &wmax, double &hmin, int &ovf)
 {
     const int ny = a.ny, nx = a.nx;
     const int r0 = (t / tiles_x) * kTH, c0 = (t % tiles_x) * kTW;
     const Consts32 k = consts32(a.cfg);
     const float Qf = (float)kQuantum;
-    const float *gb = a.b;
-    const int32_t *gh = a.h[src];
-    const float *gqx = a.qx[src];
-    const float *gqy = a.qy[src];
-    for (int e = threadIdx.x; e < kLH * kLW; e += blockDim.x) {
-        const int lr = e / kLW, lc = e - lr * kLW;
-        const int gr = r0 - 1 + lr, gc = c0 - 1 + lc;
-        float b = 0.0f, h = 0.0f, qx = 0.0f, qy = 0.0f, vx = 0.0f, vy = 0.0f;
-        int hq = 0;
-        if (gr >= 0 && gr < ny && gc >= 0 && gc < nx) {
-            const int64_t i = (int64_t)gr * nx + gc;
-            b = gb[i];
-            hq = gh[i];
-            qx = gqx[i];
-            qy = gqy[i];
-            h = (float)hq * Qf;
-            if (h > k.dry) {
-                float inv = rcpf(h);
-                vx = qx * inv;
-                vy = qy * inv;
-            }
-        }
-        T.b[e] = b; T.h[e] = h; T.hq[e] = hq; T.qx[e] = qx; T.qy[e] = qy; T.vx[e] = vx; T.vy[e] = vy;
-    }
-    __syncthreads();
     const float fxf = (float)(sub_dt / a.dx), fyf = (float)(sub_dt / a.dy);
     const int cq = a.clamp_quarter;
-    for (int e = threadIdx.x; e < kFX + kFY; e += blockDim.x) {
+#pragma unroll
+    for (int j = 0; j < (kFX + kFY + kBlock - 1) / kBlock; j++) {
+        const int e = threadIdx.x + j * kBlock;
+        if (e >= kFX + kFY) continue;
         if (e < kFX) {
             const int lr = e / (kTW + 1), kk = e - lr * (kTW + 1);
             const int gr = r0 + lr, gcl = c0 - 1 + kk;
@@ -518,7 +575,19 @@ __device__ void tile_substep32(const V2Args<F32P> &a, Tile32 &T, int src, int t,
     }
     __syncthreads();
     const int dst = src ^ 1;
-    for (int e = threadIdx.x; e < kTH * kTW; e += blockDim.x) {
+    // per-thread sums of the substep diagnostics over this thread's cells, in
+    // FP32 (two cells per thread), promoted to FP64 for the block reduction
+    float dg[kDiag];
+#pragma unroll
+    for (int d = 0; d < kDiag; d++) dg[d] = 0.0f;
+    long long hn_min = 0;
+    float wv = 0.0f;
+    const float dt = (float)sub_dt;
+    const float ax = (float)a.cfg.external_ax, ay = (float)a.cfg.external_ay;
+    const float decrement = (float)(a.cfg.basal_friction * a.cfg.gravity) * dt;
+#pragma unroll
+    for (int j = 0; j < kTH * kTW / kBlock; j++) {
+        const int e = threadIdx.x + j * kBlock;
         const int ir = e / kTW, ic = e - ir * kTW;
         const int gr = r0 + ir, gc = c0 + ic;
         if (gr >= ny || gc >= nx) continue;
@@ -543,45 +612,45 @@ __device__ void tile_substep32(const V2Args<F32P> &a, Tile32 &T, int src, int t,
         }
         const int hqi = T.hq[L];
         const float qxi = T.qx[L], qyi = T.qy[L];
-        long long hn = (long long)hqi + dq;
+        const long long hn = (long long)hqi + dq;
         ovf |= hn > 2147483647LL;
-        hmin = npmin(hmin, (double)hn * kQuantum);
-        int hq = (int)(hn < 0 ? 0 : (hn > 2147483647LL ? 2147483647LL : hn));
-        float qxn = qxi + dqx, qyn = qyi + dqy;
-        acc.diag[0] += (double)(qxn - qxi);
-        acc.diag[1] += (double)(qyn - qyi);
-        float depth = (float)hq * Qf;
+        hn_min = hn < hn_min ? hn : hn_min;
+        const int hq = (int)(hn < 0 ? 0 : (hn > 2147483647LL ? 2147483647LL : hn));
+        const float qxn = qxi + dqx, qyn = qyi + dqy;
+        dg[0] += qxn - qxi;
+        dg[1] += qyn - qyi;
+        const float depth = (float)hq * Qf;
         float qax = 0.0f, qay = 0.0f, wave_v = 0.0f;
-        if (depth > k.dry) {
-            const float dt = (float)sub_dt;
-            const float ax = (float)a.cfg.external_ax, ay = (float)a.cfg.external_ay;
-            float inv = rcpf(depth);
-            float vbx = qxn * inv, vby = qyn * inv;
-            float vex = vbx + ax * dt, vey = vby + ay * dt;
-            float speed = __builtin_sqrtf(vex * vex + vey * vey);
-            float decrement = (float)(a.cfg.basal_friction * a.cfg.gravity) * dt;
-            float factor = speed > 1.0e-15f ? fmaxf(0.0f, 1.0f - decrement * rcpf(speed)) : 0.0f;
-            float vax = vex * factor, vay = vey * factor;
+        const bool wet = depth > k.dry;
+        if (wet) {
+            const float inv = rcpf(depth);
+            const float vbx = qxn * inv, vby = qyn * inv;
+            const float vex = vbx + ax * dt, vey = vby + ay * dt;
+            const float speed = __builtin_sqrtf(vex * vex + vey * vey);
+            const float factor = speed > 1.0e-15f ? fmaxf(0.0f, 1.0f - decrement * rcpf(speed)) : 0.0f;
+            const float vax = vex * factor, vay = vey * factor;
             qax = depth * vax;
             qay = depth * vay;
             wave_v = speed * factor;
-            double dd = (double)depth;
-            acc.diag[2] += dd * (double)(vex - vbx);
-            acc.diag[3] += dd * (double)(vey - vby);
-            acc.diag[6] += dd * (0.5 * (double)(vbx + vex) * a.cfg.external_ax +
-                                 0.5 * (double)(vby + vey) * a.cfg.external_ay) * sub_dt;
-            acc.diag[4] += (double)qax - dd * (double)vex;
-            acc.diag[5] += (double)qay - dd * (double)vey;
-            acc.diag[7] += dd * ((double)speed * speed - ((double)vax * vax + (double)vay * vay));
+            dg[2] += depth * (vex - vbx);
+            dg[3] += depth * (vey - vby);
+            dg[6] += depth * (0.5f * (vbx + vex) * ax + 0.5f * (vby + vey) * ay) * dt;
+            dg[4] += qax - depth * vex;
+            dg[5] += qay - depth * vey;
+            dg[7] += depth * (speed * speed - (vax * vax + vay * vay));
         }
-        acc.diag[8] += (double)((depth > k.dry ? qxn : 0.0f) - qxi);
-        acc.diag[9] += (double)((depth > k.dry ? qyn : 0.0f) - qyi);
-        const int64_t i = (int64_t)gr * nx + gc;
+        dg[8] += (wet ? qxn : 0.0f) - qxi;
+        dg[9] += (wet ? qyn : 0.0f) - qyi;
+        const int i = gr * nx + gc;
         a.h[dst][i] = hq;
         a.qx[dst][i] = qax;
         a.qy[dst][i] = qay;
-        wmax = npmax(wmax, (double)(wave_v + __builtin_sqrtf(k.Kg * depth)));
+        wv = fmaxf(wv, wave_v + __builtin_sqrtf(k.Kg * depth));
     }
+#pragma unroll
+    for (int d = 0; d < kDiag; d++) acc.diag[d] += (double)dg[d];
+    hmin = npmin(hmin, (double)hn_min * kQuantum);
+    wmax = npmax(wmax, (double)wv);
     __syncthreads();
 }
 
@@ -610,36 +679,62 @@ __device__ inline CellOut cell_update<F32P>(const V2Args<F32P> &a, int src, int 
     return cell32(a, src, r, c, sub_dt, acc);
 }
 
-// wave speed of an existing state (CFL of the first substep)
+// ---------------------------------------------------------------------------
+// Ledger passes.  Each thread takes kLedgerU cells per grid stride and loads
+// all their fields (and the four neighbours the static diagnostic needs)
+// before any arithmetic, so a pass over the grid costs about one memory round
+// trip instead of one per cell.
+// ---------------------------------------------------------------------------
+constexpr int kLedgerU = 4;
+constexpr int kFinalizeBlock = 512;
+
+struct CellVals {
+    double b, depth, qx, qy;
+    long long quanta;
+};
+
 template <class P>
-__device__ inline double cell_wave(const V2Args<P> &a, int64_t i)
+__device__ inline CellVals load_cell(const V2Args<P> &a, int buf, int i)
+{
+    CellVals v;
+    v.b = (double)a.b[i];
+    const typename P::hst hr = a.h[buf][i];
+    v.depth = (double)P::depth(hr);
+    v.quanta = P::kQuant ? (long long)hr : 0;
+    v.qx = (double)a.qx[buf][i];
+    v.qy = (double)a.qy[buf][i];
+    return v;
+}
+
+// wave speed of an existing state (CFL of the first substep), in the
+// arithmetic of the substep that follows
+template <class P>
+__device__ inline double cell_wave(const V2Args<P> &a, const CellVals &v)
 {
     const double K = a.cfg.earth_pressure, g = a.cfg.gravity, dry = a.cfg.dry_tolerance;
     if (P::kQuant) {
-        float d = P::depth(a.h[0][i]);
+        const float d = (float)v.depth;
         float vx = 0.0f, vy = 0.0f;
         if (d > (float)dry) {
             float inv = rcpf(d);
-            vx = (float)a.qx[0][i] * inv;
-            vy = (float)a.qy[0][i] * inv;
+            vx = (float)v.qx * inv;
+            vy = (float)v.qy * inv;
         }
         return (double)(__builtin_sqrtf(vx * vx + vy * vy) + __builtin_sqrtf((float)(K * g) * d));
     } else {
-        double d = (double)a.h[0][i];
-        double vx = d > dry ? (double)a.qx[0][i] / d : 0.0;
-        double vy = d > dry ? (double)a.qy[0][i] / d : 0.0;
+        const double d = v.depth;
+        double vx = d > dry ? v.qx / d : 0.0;
+        double vy = d > dry ? v.qy / d : 0.0;
         return sqrt(vx * vx + vy * vy) + sqrt((K * g) * npmax(d, 0.0));
     }
 }
 
 // ledger terms of one cell (warp_mobile_v2.py:41-54 with weight w, density rho)
 template <class P>
-__device__ inline void cell_summary(const V2Args<P> &a, int buf, int64_t i, double w, double (&s)[kSum])
+__device__ inline void cell_summary(const V2Args<P> &a, const CellVals &v, double w, double (&s)[kSum])
 {
     const double dry = a.cfg.dry_tolerance, rho = a.cfg.density;
-    double depth = (double)P::depth(a.h[buf][i]);
-    double qx = (double)a.qx[buf][i], qy = (double)a.qy[buf][i];
-    double b = (double)a.b[i];
+    const double depth = v.depth, qx = v.qx, qy = v.qy, b = v.b;
     s[0] += depth * w;
     s[1] += rho * qx * w;
     s[2] += rho * qy * w;
@@ -661,305 +756,411 @@ __device__ inline int weight_class(int ny, int nx, int r, int c)
     return rc * 3 + cc;
 }
 
-// reduce per-workgroup partials in fixed order (workgroup 0 only)
-template <class P>
-__device__ void finalize_record(const V2Args<P> &a, double *smem, bool after_valid)
-{
-    const int nparts = gridDim.x;
-    double v[kSum * 2 + kDiag];
-    for (int k = 0; k < kSum * 2 + kDiag; k++) v[k] = 0.0;
-    double mx[4] = {0.0, 0.0, 0.0, 0.0};          // max speed before/after; -min h before/after
-    mx[2] = -1e300; mx[3] = -1e300;
-    long long mq[2 * kClasses];
-    for (int k = 0; k < 2 * kClasses; k++) mq[k] = 0;
-    for (int p = threadIdx.x; p < nparts; p += blockDim.x) {
-        const Partial &pp = a.partials[p];
-        for (int k = 0; k < kSum; k++) {
-            v[k] += pp.before[k];
-            v[kSum + k] += pp.after[k];
-        }
-        for (int k = 0; k < kDiag; k++) v[2 * kSum + k] += pp.diag[k];
-        mx[0] = npmax(mx[0], pp.before[3]);
-        mx[1] = npmax(mx[1], pp.after[3]);
-        mx[2] = npmax(mx[2], -pp.before[7]);
-        mx[3] = npmax(mx[3], -pp.after[7]);
-        for (int k = 0; k < kClasses; k++) {
-            mq[k] += pp.mass_before[k];
-            mq[kClasses + k] += pp.mass_after[k];
+// per-thread accumulators of one ledger pass
+struct LedgerAcc {
+    double s[kSum];
+    long long mq[kClasses];
+    __device__ void init()
+    {
+        for (int k = 0; k < kSum; k++) s[k] = 0.0;
+        s[7] = 1e300;
+        for (int k = 0; k < kClasses; k++) mq[k] = 0;
+    }
+    template <class P>
+    __device__ void add(const V2Args<P> &a, const CellVals &v, int i)
+    {
+        const int r = i / a.nx, c = i - r * a.nx;
+        const double area = a.dx * a.dy, f6 = a.dx * a.dy / 6.0;
+        cell_summary<P>(a, v, cell_weight(a.cfg.weight_mode, a.ny, a.nx, area, f6, r, c), s);
+        if (P::kQuant) {
+            const int cls = weight_class(a.ny, a.nx, r, c);
+#pragma unroll
+            for (int q = 0; q < kClasses; q++) mq[q] += cls == q ? v.quanta : 0;   // no dynamic indexing
         }
     }
-    block_reduce<double, kSum * 2 + kDiag, 0>(v, smem);
-    block_reduce<double, 4, 1>(mx, smem);
-    // exact integer sums: reduce through LDS as 64-bit integers
-    long long *ls = reinterpret_cast<long long *>(smem);
-    for (int k = 0; k < 2 * kClasses; k++) {
-        long long x = mq[k];
-        for (int off = kWave / 2; off > 0; off >>= 1) x += __shfl_down(x, off, kWave);
-        if ((threadIdx.x & (kWave - 1)) == 0) ls[k * (blockDim.x / kWave) + threadIdx.x / kWave] = x;
+};
+
+// reduce the pass over the block and store the workgroup's partial ledger
+template <class P>
+__device__ inline void store_ledger(const V2Args<P> &a, LedgerAcc &L, bool before, double *smem)
+{
+    double sums[6] = {L.s[0], L.s[1], L.s[2], L.s[4], L.s[5], L.s[6]};
+    double mxs[2] = {L.s[3], -L.s[7]};
+    block_reduce<double, 6, 0>(sums, smem);
+    block_reduce<double, 2, 1>(mxs, smem);
+    if (P::kQuant) block_reduce<long long, kClasses, 0>(L.mq, reinterpret_cast<long long *>(smem));
+    if (threadIdx.x == 0) {
+        const int G = a.groups, g = blockIdx.x;
+        double *dst = a.ledger_f + (before ? 0 : kSum) * G + g;
+        const double vals[kSum] = {sums[0], sums[1], sums[2], mxs[0], sums[3], sums[4], sums[5], -mxs[1]};
+        for (int k = 0; k < kSum; k++) dst[k * G] = vals[k];
+        long long *q = a.ledger_q + (before ? 0 : kClasses) * G + g;
+        for (int k = 0; k < kClasses; k++) q[k * G] = L.mq[k];
+    }
+}
+
+// Loop state that launch k acts on.  A failure flag raised by the launch that
+// wrote ctl[k & 1] rolls that substep back (the state before it is kept and
+// the loop ends with the error); flags of older launches were already folded
+// in by the launch after them (rolled == 1).  Both words were written by
+// earlier launches of the stream, so plain loads see them.
+__device__ inline Control resolve_control(const SyncBlock *s, int k)
+{
+    Control C = s->ctl[k & 1];
+    const int f = s->flags[k & 1];
+    if (f && !C.rolled) {
+        C.numeric = (f & 2) ? 3 : 1;
+        C.src = C.last_src;
+        C.remaining = C.last_remaining;
+        C.substeps = C.last_substeps;
+        C.done = 1;
+        C.rolled = 1;
+    }
+    return C;
+}
+
+__global__ void v2_reset_kernel(SyncBlock *s, mv2_step_record *rec, double *diag_part, int ndiag,
+                                unsigned long long *spread)
+{
+    const int tid = blockIdx.x * blockDim.x + threadIdx.x;
+    const int nthreads = gridDim.x * blockDim.x;
+    unsigned *ps = reinterpret_cast<unsigned *>(s);
+    for (int i = tid; i < (int)(sizeof(SyncBlock) / 4); i += nthreads) ps[i] = 0u;
+    unsigned *pr = reinterpret_cast<unsigned *>(rec);
+    for (int i = tid; i < (int)(sizeof(mv2_step_record) / 4); i += nthreads) pr[i] = 0u;
+    for (int i = tid; i < ndiag * kDiag; i += nthreads) diag_part[i] = 0.0;
+    for (int i = tid; i < kStepSpreadSets * kSpread; i += nthreads) spread[(int64_t)i * kSpreadStride] = 0ull;
+}
+
+// One pass over the input state: ledger before, static diagnostic
+// (warp_mobile_v2.py:56-85: gradient of H = b + h, free surface first, then the
+// centred / one-sided difference) and the CFL maximum of the first substep.
+template <class P>
+__global__ void __launch_bounds__(kBlock) v2_begin_kernel(V2Args<P> a)
+{
+    __shared__ double smem[64 * 4];
+    const int ny = a.ny, nx = a.nx, n = ny * nx;
+    const int stride = gridDim.x * blockDim.x;
+    const bool stat = a.cfg.static_exit != 0;
+    LedgerAcc L;
+    L.init();
+    double m[3] = {0.0, 0.0, 0.0};              // max |grad H|, max |q|, max wave speed
+    for (int base = blockIdx.x * blockDim.x + threadIdx.x; base < n; base += stride * kLedgerU) {
+        CellVals v[kLedgerU];
+        double hn[kLedgerU][4];                 // H at left, right, down, up (clamped)
+#pragma unroll
+        for (int u = 0; u < kLedgerU; u++) {
+            const int i = base + u * stride;
+            if (i >= n) continue;
+            v[u] = load_cell<P>(a, 0, i);
+            if (stat) {
+                const int r = i / nx, c = i - r * nx;
+                const int nb[4] = {c > 0 ? i - 1 : i, c + 1 < nx ? i + 1 : i, r > 0 ? i - nx : i,
+                                   r + 1 < ny ? i + nx : i};
+#pragma unroll
+                for (int q = 0; q < 4; q++) hn[u][q] = (double)a.b[nb[q]] + (double)P::depth(a.h[0][nb[q]]);
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < kLedgerU; u++) {
+            const int i = base + u * stride;
+            if (i >= n) continue;
+            L.add<P>(a, v[u], i);
+            if (stat) {
+                const int r = i / nx, c = i - r * nx;
+                const double denx = (c > 0 && c + 1 < nx) ? 2.0 * a.dx : a.dx;
+                const double deny = (r > 0 && r + 1 < ny) ? 2.0 * a.dy : a.dy;
+                const double gx = (hn[u][1] - hn[u][0]) / denx;
+                const double gy = (hn[u][3] - hn[u][2]) / deny;
+                m[0] = npmax(m[0], sqrt(gx * gx + gy * gy));
+                m[1] = npmax(m[1], fabs(v[u].qx));
+                m[1] = npmax(m[1], fabs(v[u].qy));
+            }
+            m[2] = npmax(m[2], cell_wave<P>(a, v[u]));
+        }
+    }
+    store_ledger<P>(a, L, true, smem);
+    block_reduce<double, 3, 1>(m, smem);
+    if (threadIdx.x == 0) {
+        if (stat) {
+            spread_max(a.spread + kSetStaticGrad * kSpreadWords, m[0]);
+            spread_max(a.spread + kSetStaticQ * kSpreadWords, m[1]);
+        }
+        spread_max(a.spread, m[2]);
+        if (blockIdx.x == 0) a.sync->ctl[0].remaining = a.cfg.dt;
+    }
+}
+
+// Substep k of the loop of mobile_v2_reference.py:236-307: decide from the
+// loop state whether the loop has ended (then return at once), otherwise take
+// sub_dt from the CFL maximum of the current state, update every cell into the
+// other buffer and reduce the next CFL maximum, positivity and range flags and
+// the diagnostics of the substep.
+//
+// A working launch k always finds the current state in buffer k & 1 (every
+// earlier launch worked and flipped it).  With `eager` set, the launch stages
+// its tile from that buffer before the loop state and the CFL maximum arrive,
+// so all three loads share one memory round trip; launches expected to find
+// the loop ended (eager = 0) read the loop state first and skip the staging.
+// LDS of the substep launch: the FP32 tile, nothing for the FP64 gather path
+template <class P> struct TileStore { int unused; };
+template <> struct TileStore<F32P> { Tile32 tile; };
+
+template <class P>
+__global__ void __launch_bounds__(kBlock) v2_substep_kernel(V2Args<P> a, int k, int eager)
+{
+    __shared__ double smem[64 * 4];
+    SyncBlock *s = a.sync;
+    const mv2_step_config &cfg = a.cfg;
+    const bool lead = blockIdx.x == 0 && threadIdx.x == 0;
+    const int src = k & 1;
+    const int ny = a.ny, nx = a.nx;
+    const int tiles_x = (nx + kTW - 1) / kTW;
+    const int tiles = tiles_x * ((ny + kTH - 1) / kTH);
+    int t = 0;
+    if constexpr (P::kQuant) {
+        int last;
+        xcd_tile_range(tiles, t, last);
+    }
+    // requests that do not depend on the loop state
+    double slot = 0.0;
+    if (threadIdx.x < kWave)
+        slot = __longlong_as_double((long long)a.spread[(k % kWaveSets) * kSpreadWords + threadIdx.x * kSpreadStride]);
+    double dprev[kDiag];
+    if (threadIdx.x == 0)
+        for (int d = 0; d < kDiag; d++) dprev[d] = a.diag_part[(int64_t)d * a.diag_groups + blockIdx.x];
+    __shared__ TileStore<P> store;
+    if constexpr (P::kQuant) {
+        if (eager) tile_stage32(a, store.tile, src, t, tiles_x);
+    }
+    Control C = resolve_control(s, k);
+    if (!C.done && k == 0 && cfg.static_exit) {
+        // warp_mobile_v2.py:148-160 - globally static state below the start angle
+        const double ext_max = npmax(0.0, npmax(fabs(cfg.external_ax), fabs(cfg.external_ay)));
+        const double qmax = spread_read_max(a.spread + kSetStaticQ * kSpreadWords, smem);
+        const double gmax = spread_read_max(a.spread + kSetStaticGrad * kSpreadWords, smem);
+        if (ext_max <= 1.0e-14 && qmax <= 1.0e-14 && gmax <= cfg.tan_start + 1.0e-14) {
+            C.static_exit = 1;
+            C.done = 1;
+        }
+    }
+    if (!C.done && !(C.remaining > 1.0e-14)) C.done = 1;
+    if (!C.done && cfg.limit_mode == 0 && C.substeps >= cfg.maximum_substeps) {
+        C.numeric = 2;
+        C.done = 1;
+    }
+    if (C.done) {
+        if (lead) s->ctl[(k + 1) & 1] = C;
+        return;
+    }
+    if constexpr (P::kQuant) {
+        if (!eager) tile_stage32(a, store.tile, src, t, tiles_x);
+    }
+    // CFL maximum of the current state (mobile_v2_reference.py:239-251)
+    if (threadIdx.x < kWave) {
+        const double v = wave_max(slot);
+        if (threadIdx.x == 0) smem[0] = v;
     }
     __syncthreads();
-    if (threadIdx.x == 0) {
-        long long tot[2 * kClasses];
-        for (int k = 0; k < 2 * kClasses; k++) {
-            tot[k] = 0;
-            for (int w = 0; w < (int)(blockDim.x / kWave); w++) tot[k] += ls[k * (blockDim.x / kWave) + w];
+    const double maxw = smem[0];
+    const double mind = a.dx <= a.dy ? a.dx : a.dy;
+    const double stable = maxw <= 1.0e-14 ? C.remaining : cfg.cfl * mind / maxw;
+    const double sub_dt = npmin(C.remaining, stable);
+    if (blockIdx.x == 0) spread_clear(a.spread + ((k + 2) % kWaveSets) * kSpreadWords);
+    if (lead) {
+        Control nc = C;
+        nc.last_remaining = C.remaining;
+        nc.last_substeps = C.substeps;
+        nc.last_src = C.src;
+        nc.max_cfl = npmax(C.max_cfl, maxw * sub_dt / mind);
+        if (sub_dt < C.remaining - 1.0e-14) nc.cfl_limited = 1;
+        nc.remaining = C.remaining - sub_dt;
+        nc.substeps = C.substeps + 1;
+        nc.src = C.src ^ 1;
+        // the loop's own exits, in the reference's order
+        if (cfg.limit_mode == 1 && nc.substeps > cfg.maximum_substeps) {
+            nc.numeric = 2;
+            nc.done = 1;
+        } else if (!(nc.remaining > 1.0e-14)) {
+            nc.done = 1;
+        } else if (cfg.limit_mode == 0 && nc.substeps >= cfg.maximum_substeps) {
+            nc.numeric = 2;
+            nc.done = 1;
         }
+        s->ctl[(k + 1) & 1] = nc;
+    }
+    Acc<P> acc;
+    for (int d = 0; d < kDiag; d++) acc.diag[d] = 0.0;
+    double wmax[1] = {0.0};
+    double hmin = 0.0;
+    int ovf = 0;
+    if constexpr (P::kQuant) {
+        tile_update32(a, store.tile, src, t, tiles_x, sub_dt, acc, wmax[0], hmin, ovf);
+    } else {
+        const int n = ny * nx;
+        for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+            const int r = i / nx, c = i - r * nx;
+            CellOut o = cell_update<P>(a, src, r, c, sub_dt, acc);
+            wmax[0] = npmax(wmax[0], o.wave);
+            hmin = npmin(hmin, o.hmin);
+            ovf |= o.overflow;
+        }
+    }
+    block_reduce<double, 1, 1>(wmax, smem);
+    double hm[1] = {hmin};
+    block_reduce<double, 1, 2>(hm, smem);
+    const int anyovf = __syncthreads_or(ovf);
+    block_reduce<double, kDiag, 0>(acc.diag, smem);
+    if (threadIdx.x == 0) {
+        spread_max(a.spread + ((k + 1) % kWaveSets) * kSpreadWords, wmax[0]);
+        const double tol = P::kQuant ? -1.0e-300 : -2.0e-13;
+        int f = 0;
+        if (hm[0] < tol) f |= 1;
+        if (anyovf) f |= 2;
+        if (f) atomicOr(&s->flags[(k + 1) & 1], f);
+        // each workgroup owns its diagnostic slot across the substep launches
+        for (int d = 0; d < kDiag; d++) a.diag_part[(int64_t)d * a.diag_groups + blockIdx.x] = dprev[d] + acc.diag[d];
+    }
+}
+
+// copy back when the final state sits in scratch; ledger after
+template <class P>
+__global__ void __launch_bounds__(kBlock) v2_end_kernel(V2Args<P> a, int k)
+{
+    __shared__ double smem[64 * 4];
+    const Control C = resolve_control(a.sync, k);
+    if (C.static_exit) return;
+    const int fsrc = C.src;
+    const int n = a.ny * a.nx;
+    const int stride = gridDim.x * blockDim.x;
+    LedgerAcc L;
+    L.init();
+    for (int base = blockIdx.x * blockDim.x + threadIdx.x; base < n; base += stride * kLedgerU) {
+        typename P::hst h[kLedgerU];
+        typename P::real qx[kLedgerU], qy[kLedgerU], b[kLedgerU];
+#pragma unroll
+        for (int u = 0; u < kLedgerU; u++) {
+            const int i = base + u * stride;
+            if (i >= n) continue;
+            b[u] = a.b[i];
+            h[u] = a.h[fsrc][i];
+            qx[u] = a.qx[fsrc][i];
+            qy[u] = a.qy[fsrc][i];
+        }
+#pragma unroll
+        for (int u = 0; u < kLedgerU; u++) {
+            const int i = base + u * stride;
+            if (i >= n) continue;
+            if (fsrc == 1) {
+                a.h[0][i] = h[u];
+                a.qx[0][i] = qx[u];
+                a.qy[0][i] = qy[u];
+            }
+            CellVals v;
+            v.b = (double)b[u];
+            v.depth = (double)P::depth(h[u]);
+            v.quanta = P::kQuant ? (long long)h[u] : 0;
+            v.qx = (double)qx[u];
+            v.qy = (double)qy[u];
+            L.add<P>(a, v, i);
+        }
+    }
+    store_ledger<P>(a, L, false, smem);
+}
+
+// Fold the partial ledgers and diagnostics in fixed order into the record.
+// One workgroup of kFinalizeBlock threads; thread p folds workgroups p,
+// p + kFinalizeBlock, ... of each field (coalesced across the threads).
+template <class P>
+__global__ void __launch_bounds__(kFinalizeBlock) v2_finalize_kernel(V2Args<P> a, int k, int nparts, int ndiag)
+{
+    __shared__ double smem[kFinalizeBlock / kWave * 40];
+    const Control C = resolve_control(a.sync, k);
+    const bool after_valid = !C.static_exit;
+    double v[kSum * 2 + kDiag];
+    for (int q = 0; q < kSum * 2 + kDiag; q++) v[q] = 0.0;
+    double mx[6] = {0.0, 0.0, -1e300, -1e300, 0.0, 0.0};   // max speed b/a; -min h b/a; static grad, q
+    long long mq[2 * kClasses];
+    for (int q = 0; q < 2 * kClasses; q++) mq[q] = 0;
+    for (int p = threadIdx.x; p < ndiag; p += blockDim.x)
+        for (int d = 0; d < kDiag; d++) v[2 * kSum + d] += a.diag_part[(int64_t)d * ndiag + p];
+    for (int p = threadIdx.x; p < nparts; p += blockDim.x) {
+        double f[kLedgerF];
+        long long q8[kLedgerQ];
+#pragma unroll
+        for (int q = 0; q < kLedgerF; q++) f[q] = a.ledger_f[(int64_t)q * nparts + p];
+#pragma unroll
+        for (int q = 0; q < kLedgerQ; q++) q8[q] = a.ledger_q[(int64_t)q * nparts + p];
+        for (int q = 0; q < kSum; q++) {
+            v[q] += f[q];
+            v[kSum + q] += f[kSum + q];
+        }
+        mx[0] = npmax(mx[0], f[3]);
+        mx[1] = npmax(mx[1], f[kSum + 3]);
+        mx[2] = npmax(mx[2], -f[7]);
+        mx[3] = npmax(mx[3], -f[kSum + 7]);
+        for (int q = 0; q < kLedgerQ; q++) mq[q] += q8[q];
+    }
+    if (threadIdx.x < kWave) {
+        mx[4] = __longlong_as_double((long long)a.spread[kSetStaticGrad * kSpreadWords + threadIdx.x * kSpreadStride]);
+        mx[5] = __longlong_as_double((long long)a.spread[kSetStaticQ * kSpreadWords + threadIdx.x * kSpreadStride]);
+    }
+    constexpr int kNW = kFinalizeBlock / kWave;
+    block_reduce<double, kSum * 2 + kDiag, 0, kNW>(v, smem);
+    block_reduce<double, 6, 1, kNW>(mx, smem);
+    block_reduce<long long, 2 * kClasses, 0, kNW>(mq, reinterpret_cast<long long *>(smem));
+    if (threadIdx.x == 0) {
         mv2_step_record *rec = a.rec;
-        for (int k = 0; k < kSum; k++) {
-            rec->summary_before[k] = v[k];
-            rec->summary_after[k] = after_valid ? v[kSum + k] : v[k];
+        for (int q = 0; q < kSum; q++) {
+            rec->summary_before[q] = v[q];
+            rec->summary_after[q] = after_valid ? v[kSum + q] : v[q];
         }
         rec->summary_before[3] = mx[0];
         rec->summary_after[3] = after_valid ? mx[1] : mx[0];
         rec->summary_before[7] = -mx[2];
         rec->summary_after[7] = after_valid ? -mx[3] : -mx[2];
-        for (int k = 0; k < kDiag; k++) rec->diag[k] = after_valid ? v[2 * kSum + k] : 0.0;
+        for (int d = 0; d < kDiag; d++) rec->diag[d] = after_valid ? v[2 * kSum + d] : 0.0;
         if (P::kQuant) {
             // exact mobile volume: integer quanta per weight class times its weight
             const double area = a.dx * a.dy, f6 = a.dx * a.dy / 6.0;
             const int rep_r[3] = {0, 1, a.ny - 1}, rep_c[3] = {0, 1, a.nx - 1};
             double mb = 0.0, ma = 0.0, qb = 0.0, qa = 0.0;
-            for (int k = 0; k < kClasses; k++) {
-                double w = a.cfg.weight_mode ? vertex_weight(a.ny, a.nx, f6, rep_r[k / 3], rep_c[k % 3]) : area;
-                mb += ((double)tot[k] * kQuantum) * w;
-                ma += ((double)tot[kClasses + k] * kQuantum) * w;
-                qb += (double)tot[k];
-                qa += (double)tot[kClasses + k];
+            for (int q = 0; q < kClasses; q++) {
+                double w = a.cfg.weight_mode ? vertex_weight(a.ny, a.nx, f6, rep_r[q / 3], rep_c[q % 3]) : area;
+                mb += ((double)mq[q] * kQuantum) * w;
+                ma += ((double)mq[kClasses + q] * kQuantum) * w;
+                qb += (double)mq[q];
+                qa += (double)mq[kClasses + q];
             }
             rec->summary_before[0] = mb;
             rec->summary_after[0] = after_valid ? ma : mb;
             rec->mass_quanta_before = qb;
             rec->mass_quanta_after = after_valid ? qa : qb;
         }
-    }
-    __syncthreads();
-}
-
-template <class P>
-__device__ void summary_pass(const V2Args<P> &a, int buf, bool before, double *smem)
-{
-    double s[kSum];
-    for (int k = 0; k < kSum; k++) s[k] = 0.0;
-    s[7] = 1e300;
-    long long mq[kClasses];
-    for (int k = 0; k < kClasses; k++) mq[k] = 0;
-    const int64_t n = (int64_t)a.ny * a.nx;
-    const double area = a.dx * a.dy, f6 = a.dx * a.dy / 6.0;
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
-         i += (int64_t)gridDim.x * blockDim.x) {
-        int r = (int)(i / a.nx), c = (int)(i - (int64_t)r * a.nx);
-        double w = cell_weight(a.cfg.weight_mode, a.ny, a.nx, area, f6, r, c);
-        cell_summary<P>(a, buf, i, w, s);
-        if (P::kQuant) mq[weight_class(a.ny, a.nx, r, c)] += (long long)a.h[buf][i];
-    }
-    double sums[6] = {s[0], s[1], s[2], s[4], s[5], s[6]};
-    double mxs[2] = {s[3], -s[7]};
-    block_reduce<double, 6, 0>(sums, smem);
-    block_reduce<double, 2, 1>(mxs, smem);
-    long long *ls = reinterpret_cast<long long *>(smem);
-    long long tot[kClasses];
-    for (int k = 0; k < kClasses; k++) {
-        long long x = mq[k];
-        for (int off = kWave / 2; off > 0; off >>= 1) x += __shfl_down(x, off, kWave);
-        if ((threadIdx.x & (kWave - 1)) == 0) ls[k * (blockDim.x / kWave) + threadIdx.x / kWave] = x;
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        for (int k = 0; k < kClasses; k++) {
-            tot[k] = 0;
-            for (int w = 0; w < (int)(blockDim.x / kWave); w++) tot[k] += ls[k * (blockDim.x / kWave) + w];
-        }
-        Partial &pp = a.partials[blockIdx.x];
-        double *dst = before ? pp.before : pp.after;
-        dst[0] = sums[0]; dst[1] = sums[1]; dst[2] = sums[2]; dst[3] = mxs[0];
-        dst[4] = sums[3]; dst[5] = sums[4]; dst[6] = sums[5]; dst[7] = -mxs[1];
-        for (int k = 0; k < kClasses; k++) (before ? pp.mass_before : pp.mass_after)[k] = tot[k];
-    }
-    __syncthreads();
-}
-
-// static diagnostic (warp_mobile_v2.py:56-85, gradient of H = b + h in the
-// CPU form: free surface first, then the centred/one-sided difference)
-template <class P>
-__device__ void static_pass(const V2Args<P> &a, double *smem)
-{
-    double m[2] = {0.0, 0.0};
-    const int ny = a.ny, nx = a.nx;
-    const int64_t n = (int64_t)ny * nx;
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
-         i += (int64_t)gridDim.x * blockDim.x) {
-        int r = (int)(i / nx), c = (int)(i - (int64_t)r * nx);
-        int cl = c > 0 ? c - 1 : 0, cr = c + 1 < nx ? c + 1 : nx - 1;
-        int rd = r > 0 ? r - 1 : 0, ru = r + 1 < ny ? r + 1 : ny - 1;
-        double denx = (c > 0 && c + 1 < nx) ? 2.0 * a.dx : a.dx;
-        double deny = (r > 0 && r + 1 < ny) ? 2.0 * a.dy : a.dy;
-        auto H = [&](int rr, int cc) {
-            int64_t j = (int64_t)rr * nx + cc;
-            return (double)a.b[j] + (double)P::depth(a.h[0][j]);
-        };
-        double gx = (H(r, cr) - H(r, cl)) / denx;
-        double gy = (H(ru, c) - H(rd, c)) / deny;
-        m[0] = npmax(m[0], sqrt(gx * gx + gy * gy));
-        m[1] = npmax(m[1], fabs((double)a.qx[0][i]));
-        m[1] = npmax(m[1], fabs((double)a.qy[0][i]));
-    }
-    block_reduce<double, 2, 1>(m, smem);
-    if (threadIdx.x == 0) {
-        atomic_max_nonneg(&a.sync->static_grad, m[0]);
-        atomic_max_nonneg(&a.sync->static_q, m[1]);
+        a.sync->static_grad = (unsigned long long)__double_as_longlong(mx[4]);
+        a.sync->static_q = (unsigned long long)__double_as_longlong(mx[5]);
+        rec->substeps = C.substeps;
+        rec->cfl_limited = C.cfl_limited;
+        rec->static_exit = C.static_exit;
+        rec->maximum_cfl = C.max_cfl;
+        rec->remaining = C.remaining;
+        rec->numeric_error = C.numeric;
+        rec->status = C.numeric ? MV2_ENUMERIC : MV2_OK;
+        rec->complete = C.done;
     }
 }
 
 template <class P>
-__global__ void __launch_bounds__(kBlock) v2_step_kernel(V2Args<P> a)
-{
-    __shared__ double smem[64 * 4];
-    unsigned epoch = 0;
-    const int ny = a.ny, nx = a.nx;
-    const int64_t n = (int64_t)ny * nx;
-    const mv2_step_config &cfg = a.cfg;
-    // ---- phase 0: ledger before, static diagnostic, first CFL maximum ----
-    summary_pass<P>(a, 0, true, smem);
-    if (cfg.static_exit) static_pass<P>(a, smem);
-    {
-        double wmax[1] = {0.0};
-        for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
-             i += (int64_t)gridDim.x * blockDim.x)
-            wmax[0] = npmax(wmax[0], cell_wave<P>(a, i));
-        block_reduce<double, 1, 1>(wmax, smem);
-        if (threadIdx.x == 0) atomic_max_nonneg(&a.sync->wave_slot[0], wmax[0]);
-    }
-    grid_barrier(&a.sync->counter, epoch);
-    const double ext_max = npmax(0.0, npmax(fabs(cfg.external_ax), fabs(cfg.external_ay)));
-    if (cfg.static_exit && ext_max <= 1.0e-14) {
-        double gmax = __longlong_as_double((long long)__hip_atomic_load(&a.sync->static_grad, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
-        double qmax = __longlong_as_double((long long)__hip_atomic_load(&a.sync->static_q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
-        if (qmax <= 1.0e-14 && gmax <= cfg.tan_start + 1.0e-14) {
-            if (blockIdx.x == 0) {
-                finalize_record<P>(a, smem, false);
-                if (threadIdx.x == 0) {
-                    a.rec->static_exit = 1;
-                    a.rec->substeps = 0;
-                    a.rec->remaining = cfg.dt;
-                    a.rec->maximum_cfl = 0.0;
-                }
-            }
-            return;
-        }
-    }
-    // ---- substep loop (mobile_v2_reference.py:236-307) ----
-    const double mind = a.dx <= a.dy ? a.dx : a.dy;
-    double remaining = cfg.dt;
-    double max_cfl = 0.0;
-    int substeps = 0;
-    int cfl_limited = 0;
-    int numeric = 0;
-    int src = 0;
-    Acc<P> acc;
-    for (int k = 0; k < kDiag; k++) acc.diag[k] = 0.0;
-    while (remaining > 1.0e-14) {
-        if (cfg.limit_mode == 0 && substeps >= cfg.maximum_substeps) { numeric = 2; break; }
-        const int slot = substeps % 3;
-        const double maxw = __longlong_as_double((long long)__hip_atomic_load(&a.sync->wave_slot[slot], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
-        const double stable = maxw <= 1.0e-14 ? remaining : cfg.cfl * mind / maxw;
-        const double sub_dt = npmin(remaining, stable);
-        max_cfl = npmax(max_cfl, maxw * sub_dt / mind);
-        if (sub_dt < remaining - 1.0e-14) cfl_limited = 1;
-        if (blockIdx.x == 0 && threadIdx.x == 0)
-            __hip_atomic_store(&a.sync->wave_slot[(substeps + 2) % 3], 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        double wmax[1] = {0.0};
-        double hmin = 0.0;
-        int ovf = 0;
-        if constexpr (P::kQuant) {
-            __shared__ Tile32 tile;
-            const int tiles_x = (nx + kTW - 1) / kTW;
-            const int tiles = tiles_x * ((ny + kTH - 1) / kTH);
-            int first, last;
-            xcd_tile_range(tiles, first, last);
-            for (int t = first; t < last; t++)
-                tile_substep32(a, tile, src, t, tiles_x, sub_dt, acc, wmax[0], hmin, ovf);
-        } else {
-            for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
-                 i += (int64_t)gridDim.x * blockDim.x) {
-                int r = (int)(i / nx), c = (int)(i - (int64_t)r * nx);
-                CellOut o = cell_update<P>(a, src, r, c, sub_dt, acc);
-                wmax[0] = npmax(wmax[0], o.wave);
-                hmin = npmin(hmin, o.hmin);
-                ovf |= o.overflow;
-            }
-        }
-        block_reduce<double, 1, 1>(wmax, smem);
-        double hm[1] = {hmin};
-        block_reduce<double, 1, 2>(hm, smem);
-        int anyovf = __syncthreads_or(ovf);
-        if (threadIdx.x == 0) {
-            atomic_max_nonneg(&a.sync->wave_slot[(substeps + 1) % 3], wmax[0]);
-            const double tol = P::kQuant ? -1.0e-300 : -2.0e-13;
-            if (hm[0] < tol) atomicOr(&a.sync->positivity, 1);
-            if (anyovf) atomicOr(&a.sync->overflow, 1);
-        }
-        grid_barrier(&a.sync->counter, epoch);
-        if (__hip_atomic_load(&a.sync->overflow, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) { numeric = 3; break; }
-        if (__hip_atomic_load(&a.sync->positivity, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) { numeric = 1; break; }
-        src ^= 1;
-        remaining -= sub_dt;
-        substeps += 1;
-        if (cfg.limit_mode == 1 && substeps > cfg.maximum_substeps) { numeric = 2; break; }
-    }
-    // ---- copy back when the final state sits in scratch ----
-    if (src == 1) {
-        for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
-             i += (int64_t)gridDim.x * blockDim.x) {
-            a.h[0][i] = a.h[1][i];
-            a.qx[0][i] = a.qx[1][i];
-            a.qy[0][i] = a.qy[1][i];
-        }
-        grid_barrier(&a.sync->counter, epoch);
-    }
-    // ---- ledger after + per-workgroup diagnostics ----
-    summary_pass<P>(a, 0, false, smem);
-    block_reduce<double, kDiag, 0>(acc.diag, smem);
-    if (threadIdx.x == 0)
-        for (int k = 0; k < kDiag; k++) a.partials[blockIdx.x].diag[k] = acc.diag[k];
-    grid_barrier(&a.sync->counter, epoch);
-    if (blockIdx.x == 0) {
-        finalize_record<P>(a, smem, true);
-        if (threadIdx.x == 0) {
-            a.rec->substeps = substeps;
-            a.rec->cfl_limited = cfl_limited;
-            a.rec->static_exit = 0;
-            a.rec->maximum_cfl = max_cfl;
-            a.rec->remaining = remaining;
-            a.rec->numeric_error = numeric;
-            a.rec->status = numeric ? MV2_ENUMERIC : MV2_OK;
-        }
-    }
-}
-
-__global__ void zero_words(unsigned *p, int n)
-{
-    for (int i = threadIdx.x; i < n; i += blockDim.x) p[i] = 0u;
-}
-
-__global__ void zero_record(mv2_step_record *rec)
-{
-    unsigned *p = reinterpret_cast<unsigned *>(rec);
-    for (int i = threadIdx.x; i < (int)(sizeof(mv2_step_record) / 4); i += blockDim.x) p[i] = 0u;
-}
-
-template <class P>
-static int launch_step(mv2_context *ctx, const mv2_step_config *cfg, mv2_step_record *out)
+static int launch_step(mv2_context *ctx, const mv2_step_config *cfg, mv2_step_record *out, int fixed_chunk = 0)
 {
     const int64_t n = (int64_t)ctx->ny * ctx->nx;
-    int per_cu = 0;
-    MV2_HIP(ctx, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, v2_step_kernel<P>, kBlock, 0));
-    if (per_cu < 1) return set_error(ctx, MV2_EDEVICE, "[MobileV2] kernel cannot be resident");
-    int grid = per_cu * ctx->cu_count;
-    int need = grid_for(n);
-    if (P::kQuant) need = ((ctx->nx + kTW - 1) / kTW) * ((ctx->ny + kTH - 1) / kTH);
-    if (grid > need) grid = need;
-    if (grid < 1) grid = 1;
-    if (int rc = ensure_scratch(ctx, ctx->partials, sizeof(Partial) * (size_t)grid)) return rc;
+    const int gs = grid_for(n, kBlock * kLedgerU, kFinalizeBlock);   // begin / end passes
+    const int gu = P::kQuant ? ((ctx->nx + kTW - 1) / kTW) * ((ctx->ny + kTH - 1) / kTH)
+                             : grid_for(n);                              // substep launches
+    if (int rc = ensure_scratch(ctx, ctx->partials, (sizeof(double) * kLedgerF + sizeof(long long) * kLedgerQ) * (size_t)gs))
+        return rc;
+    if (int rc = ensure_scratch(ctx, ctx->step_diag, sizeof(double) * kDiag * (size_t)gu)) return rc;
+    if (int rc = ensure_scratch(ctx, ctx->spread, kSpreadScratchBytes)) return rc;
     V2Args<P> a;
     a.ny = ctx->ny;
     a.nx = ctx->nx;
@@ -974,24 +1175,67 @@ static int launch_step(mv2_context *ctx, const mv2_step_config *cfg, mv2_step_re
     a.qy[1] = (typename P::real *)ctx->alt_qy.ptr;
     a.cfg = *cfg;
     a.sync = (SyncBlock *)ctx->sync.ptr;
-    a.partials = (Partial *)ctx->partials.ptr;
+    a.ledger_f = (double *)ctx->partials.ptr;
+    a.ledger_q = (long long *)(a.ledger_f + (size_t)kLedgerF * gs);
+    a.groups = gs;
+    a.diag_part = (double *)ctx->step_diag.ptr;
+    a.diag_groups = gu;
+    a.spread = (unsigned long long *)ctx->spread.ptr;
     a.rec = (mv2_step_record *)ctx->result.ptr;
     a.clamp_quarter = cfg->cfl <= 0.25 ? 1 : 0;
-    hipLaunchKernelGGL(zero_words, dim3(1), dim3(64), 0, ctx->stream, (unsigned *)ctx->sync.ptr,
-                       (int)(sizeof(SyncBlock) / 4));
-    hipLaunchKernelGGL(zero_record, dim3(1), dim3(64), 0, ctx->stream, a.rec);
-    void *args[] = {&a};
-    MV2_HIP(ctx, hipEventRecord(ctx->ev_start, ctx->stream));
-    MV2_HIP(ctx, hipLaunchCooperativeKernel((const void *)v2_step_kernel<P>, dim3(grid), dim3(kBlock),
-                                            args, 0, ctx->stream));
-    MV2_HIP(ctx, hipEventRecord(ctx->ev_stop, ctx->stream));
-    MV2_HIP(ctx, hipMemcpyAsync(ctx->host_result.ptr, a.rec, sizeof(mv2_step_record),
-                                hipMemcpyDeviceToHost, ctx->stream));
-    MV2_HIP(ctx, hipStreamSynchronize(ctx->stream));
-    *out = *(const mv2_step_record *)ctx->host_result.ptr;
+    hipStream_t st = ctx->stream;
+    const bool timing = ctx->timing != 0;
+    MV2_HIP(ctx, hipEventRecord(ctx->ev_start, st));
+    hipLaunchKernelGGL(v2_reset_kernel, dim3(grid_for((int64_t)gu * kDiag, kBlock, 64)), dim3(kBlock), 0, st,
+                       a.sync, a.rec, a.diag_part, gu, a.spread);
+    hipLaunchKernelGGL(v2_begin_kernel<P>, dim3(gs), dim3(kBlock), 0, st, a);
+    MV2_HIP(ctx, hipGetLastError());
+    int k = 0;
+    int chunk = fixed_chunk > 0 ? fixed_chunk : (ctx->v2_predicted > 0 ? ctx->v2_predicted + 1 : 4);
+    // launches expected to work stage their tile before the loop state arrives
+    int eager_until = fixed_chunk > 0 ? 0 : (ctx->v2_predicted > 0 ? ctx->v2_predicted : 4);
+    const mv2_step_record *hrec = (const mv2_step_record *)ctx->host_result.ptr;
+    for (;;) {
+        if (timing && (int)ctx->sub_events.size() < 2 * (k + chunk)) {
+            while ((int)ctx->sub_events.size() < 2 * (k + chunk)) {
+                hipEvent_t e;
+                MV2_HIP(ctx, hipEventCreate(&e));
+                ctx->sub_events.push_back(e);
+            }
+        }
+        for (int j = 0; j < chunk; j++, k++) {
+            if (timing) MV2_HIP(ctx, hipEventRecord(ctx->sub_events[2 * k], st));
+            hipLaunchKernelGGL(v2_substep_kernel<P>, dim3(gu), dim3(kBlock), 0, st, a, k, k < eager_until ? 1 : 0);
+            if (timing) MV2_HIP(ctx, hipEventRecord(ctx->sub_events[2 * k + 1], st));
+        }
+        hipLaunchKernelGGL(v2_end_kernel<P>, dim3(gs), dim3(kBlock), 0, st, a, k);
+        hipLaunchKernelGGL(v2_finalize_kernel<P>, dim3(1), dim3(kFinalizeBlock), 0, st, a, k, gs, gu);
+        MV2_HIP(ctx, hipGetLastError());
+        MV2_HIP(ctx, hipEventRecord(ctx->ev_stop, st));
+        MV2_HIP(ctx, hipMemcpyAsync(ctx->host_result.ptr, a.rec, sizeof(mv2_step_record), hipMemcpyDeviceToHost, st));
+        MV2_HIP(ctx, hipStreamSynchronize(st));
+        if (hrec->complete) break;
+        // more substeps than enqueued: the state so far is valid, enqueue more
+        chunk = chunk * 2 > 4 ? chunk * 2 : 4;
+        eager_until = k + chunk;
+    }
+    *out = *hrec;
+    if (fixed_chunk == 0) ctx->v2_predicted = out->substeps;
+    out->launches = k;
     float ms = 0.0f;
     MV2_HIP(ctx, hipEventElapsedTime(&ms, ctx->ev_start, ctx->ev_stop));
     out->kernel_ms = ms;
+    out->substep_kernel_ms = 0.0;
+    if (timing) {
+        // substeps are the first launches; the ones after the loop ended return at once
+        double sum = 0.0;
+        for (int j = 0; j < out->substeps && j < k; j++) {
+            float e = 0.0f;
+            MV2_HIP(ctx, hipEventElapsedTime(&e, ctx->sub_events[2 * j], ctx->sub_events[2 * j + 1]));
+            sum += e;
+        }
+        out->substep_kernel_ms = sum;
+    }
     return MV2_OK;
 }
 
@@ -1008,7 +1252,7 @@ static int launch_summary(mv2_context *ctx, const mv2_step_config *cfg, double o
     // A state with |q| > 1e-14 does not take the static exit; in that case the
     // zero-substep limit stops the loop before any update (limit_mode 0).
     c.limit_mode = 0;
-    if (int rc = launch_step<P>(ctx, &c, &rec)) return rc;
+    if (int rc = launch_step<P>(ctx, &c, &rec, 1)) return rc;
     for (int k = 0; k < 8; k++) out[k] = rec.summary_before[k];
     return MV2_OK;
 }
@@ -1066,12 +1310,19 @@ extern "C" int mv2_static_diagnostic(mv2_context *ctx, double out[2])
     c.limit_mode = 0;
     c.static_exit = 1;
     mv2_step_record rec;
-    int rc = ctx->precision == MV2_FP64 ? launch_step<F64P>(ctx, &c, &rec)
-                                        : launch_step<F32P>(ctx, &c, &rec);
+    int rc = ctx->precision == MV2_FP64 ? launch_step<F64P>(ctx, &c, &rec, 1)
+                                        : launch_step<F32P>(ctx, &c, &rec, 1);
     if (rc) return rc;
     SyncBlock sb;
     MV2_HIP(ctx, hipMemcpy(&sb, ctx->sync.ptr, sizeof(SyncBlock), hipMemcpyDeviceToHost));
     out[0] = __builtin_bit_cast(double, sb.static_grad);
     out[1] = __builtin_bit_cast(double, sb.static_q);
+    return MV2_OK;
+}
+
+extern "C" int mv2_set_timing(mv2_context *ctx, int enable)
+{
+    if (!ctx) return set_error(nullptr, MV2_EINVAL, "[MobileV2] null context");
+    ctx->timing = enable ? 1 : 0;
     return MV2_OK;
 }

@@ -37,15 +37,17 @@ class StepConfig(ctypes.Structure):
 class StepRecord(ctypes.Structure):
     _fields_ = [
         ("status", _I), ("substeps", _I), ("cfl_limited", _I), ("static_exit", _I),
-        ("numeric_error", _I), ("reserved", _I),
+        ("numeric_error", _I), ("complete", _I),
         ("maximum_cfl", _D), ("remaining", _D),
         ("summary_before", _D * 8), ("summary_after", _D * 8), ("diag", _D * 10),
         ("mass_quanta_before", _D), ("mass_quanta_after", _D), ("kernel_ms", _D),
+        ("substep_kernel_ms", _D), ("launches", _I), ("pad", _I),
     ]
 
 
 class RelaxResult(ctypes.Structure):
-    _fields_ = [("iterations", _I), ("converged", _I), ("max_excess_last", _D)]
+    _fields_ = [("iterations", _I), ("converged", _I), ("max_excess_last", _D), ("complete", _I),
+                ("launches", _I)]
 
 
 class AvalancheConfig(ctypes.Structure):
@@ -88,6 +90,7 @@ SIGNATURES: dict[str, tuple[object, list[object]]] = {
     "mv2_step": (_I, [_P, ctypes.POINTER(StepConfig), ctypes.POINTER(StepRecord)]),
     "mv2_summary": (_I, [_P, ctypes.POINTER(StepConfig), _P]),
     "mv2_static_diagnostic": (_I, [_P, _P]),
+    "mv2_set_timing": (_I, [_P, _I]),
     "mv2_entrain": (_I, [_P, _P, _P, _I, _I, ctypes.POINTER(_D)]),
     "mv2_deposit": (_I, [_P, _P, _P, _I, _I, _D, _P]),
     "mv2_reservoirs": (_I, [_P, _I, _D, _P]),

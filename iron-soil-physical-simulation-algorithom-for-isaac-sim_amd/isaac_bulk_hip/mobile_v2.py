@@ -121,7 +121,7 @@ def run_step(state: DeviceBulkState, config: MobileV2Config, dt: float, *, densi
                        weight_mode=weight_mode, external=external)
     rec = _lib.StepRecord()
     rc = rt.lib.mv2_step(rt.ctx, ctypes.byref(cfg), ctypes.byref(rec))
-    rt.telemetry.record_call(launches=3, syncs=1)
+    rt.telemetry.record_call(launches=int(rec.launches) + 4, syncs=1)
     rt.telemetry.record_d2h(np.zeros(ctypes.sizeof(rec) // 8))
     if rc != _lib.MV2_OK:
         msg = _lib.last_error(rt.ctx)
@@ -157,6 +157,11 @@ class HipProductionMobileV2Solver:
         if missing:
             raise ValueError(f"[MobileV2Production] missing state: {sorted(missing)}")
         self.state, self.material, self.grid = state, material, grid
+
+    def set_kernel_timing(self, enable: bool) -> None:
+        """Bracket each substep launch with HIP events (record.substep_kernel_ms)."""
+        rt = self.runtime if self.runtime is not None else self.state.runtime
+        rt.check(rt.lib.mv2_set_timing(rt.ctx, int(bool(enable))))
 
     def step_resident(self, dt_s: float, **_) -> WarpMobileStep:
         if self.state is None:

@@ -44,14 +44,40 @@ def test_abi_constants():
     assert lib.mv2_mobile_quantum() == 2.0 ** -28
 
 
-def test_struct_layouts_match_header():
-    # sizes follow from the C declarations (natural alignment, no packing)
-    assert ctypes.sizeof(_lib.StepConfig) == 10 * 8 + 4 * 4
-    assert ctypes.sizeof(_lib.StepRecord) == 6 * 4 + 2 * 8 + 8 * 8 * 2 + 10 * 8 + 3 * 8
-    assert ctypes.sizeof(_lib.RelaxResult) == 16
-    assert ctypes.sizeof(_lib.AvalancheConfig) == 10 * 8 + 4 * 4
-    assert ctypes.sizeof(_lib.AvalancheFields) == 16 * 8
-    assert ctypes.sizeof(_lib.LabelResult) == 8 * 4 + 8 * 8 + 2 * 4
+STRUCTS = {
+    "StepConfig": "mv2_step_config",
+    "StepRecord": "mv2_step_record",
+    "RelaxResult": "mv2_relax_result",
+    "AvalancheConfig": "mv2_avalanche_config",
+    "AvalancheFields": "mv2_avalanche_fields",
+    "LabelResult": "mv2_label_result",
+}
+
+
+def test_struct_layouts_match_header(tmp_path):
+    """Every ctypes record has the C compiler's size and field offsets for include/mv2.h."""
+    import shutil
+    import subprocess
+
+    if shutil.which("gcc") is None:
+        pytest.skip("gcc not available")
+    lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "mv2.h"', "int main(void) {"]
+    expected = []
+    for py_name, c_name in STRUCTS.items():
+        cls = getattr(_lib, py_name)
+        lines.append(f'printf("%zu\\n", sizeof({c_name}));')
+        expected.append(ctypes.sizeof(cls))
+        for field, _ in cls._fields_:
+            lines.append(f'printf("%zu\\n", offsetof({c_name}, {field}));')
+            expected.append(getattr(cls, field).offset)
+    lines += ["return 0;", "}"]
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines))
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-I", str(ROOT / "include"), str(src), "-o", str(exe)], check=True)
+    got = [int(v) for v in subprocess.run([str(exe)], check=True, capture_output=True,
+                                          text=True).stdout.split()]
+    assert got == expected
 
 
 def test_create_without_gpu_reports_backend_unavailable():
