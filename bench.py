@@ -89,6 +89,35 @@ def run_frame(state, solver, bridge, precision, relax_iters):
     return rec.substeps, it, int(lab.diag_i[3])
 
 
+def learned_relax_to_rest(state, plain_rest) -> dict:
+    """Learned warm start (committed weights) + exact sweeps to the plain tolerance."""
+    import torch
+    from isaac_bulk_hip import relax_device
+    from isaac_bulk_hip.surrogate import warm_start_relax
+    from isaac_bulk_hip.warm_start_training import load_net
+
+    net, meta = load_net(state.runtime)
+    b0 = state.runtime.arrays["b_eff"].view(N, N)
+    runs = []
+    for _ in range(3):                     # the first call also sizes the network buffers
+        b = b0.clone()
+        res = warm_start_relax(net, b, None, DX, 34.0, relax_tol("fp32"))
+        runs.append(res)
+    res = runs[-1]
+    # the plain relaxation as the judge: one more exact iteration finds nothing to move
+    check_it, check_conv, check_last = relax_device(b.clone(), DX, DX, 34.0, 1, relax_tol("fp32"),
+                                                    runtime=state.runtime)
+    mass0 = float(b0.double().sum().item())
+    return {"ms": res.elapsed_ms, "ms_runs": [r.elapsed_ms for r in runs],
+            "exact_iterations": res.iterations_exact + res.fallback_iterations,
+            "envelope_launches": res.envelope_launches, "converged": bool(res.converged),
+            "judge_max_excess": check_last, "judge_converged": bool(check_conv),
+            "mass_rel_change": abs(float(b.double().sum().item()) - mass0) / mass0,
+            "max_abs_vs_plain_m": float((b - plain_rest).abs().max().item()),
+            "weights": {"margin_deg": meta.get("margin_deg"), "pairs": meta.get("pairs"),
+                        "steps": meta.get("steps")}}
+
+
 def cpu_baseline(b, h, frames: int) -> dict:
     """FP64 CPU oracle (C, one thread) on the first frames of the same workload."""
     sys.path.insert(0, str(ROOT / "oracle"))
@@ -128,14 +157,18 @@ def main() -> int:
     from isaac_bulk_hip import relax_device
 
     relax_to_rest_ms, rest_it, rest_conv = None, None, None
+    learned = None
     if not args.skip_relax_to_rest:
-        rest = state.runtime.arrays["b_eff"].view(N, N).clone()
-        torch.cuda.synchronize()
-        t_rest = time.perf_counter()
-        rest_it, rest_conv, _ = relax_device(rest, DX, DX, 34.0, 20000, relax_tol(args.precision),
-                                             runtime=state.runtime)
-        torch.cuda.synchronize()
-        relax_to_rest_ms = (time.perf_counter() - t_rest) * 1e3
+        for _ in range(2):                 # the second run is reported (the first sizes scratch)
+            rest = state.runtime.arrays["b_eff"].view(N, N).clone()
+            torch.cuda.synchronize()
+            t_rest = time.perf_counter()
+            rest_it, rest_conv, _ = relax_device(rest, DX, DX, 34.0, 20000, relax_tol(args.precision),
+                                                 runtime=state.runtime)
+            torch.cuda.synchronize()
+            relax_to_rest_ms = (time.perf_counter() - t_rest) * 1e3
+        if args.precision == "fp32":
+            learned = learned_relax_to_rest(state, rest)
 
     for _ in range(args.warmup):
         run_frame(state, solver, bridge, args.precision, args.relax_max_iterations)
@@ -195,7 +228,8 @@ def main() -> int:
         "relax_to_rest_iterations": rest_it,
         "relax_to_rest_converged": rest_conv,
         "relax_iterations_timed": total_relax_it,
-        "learned_relax_to_rest_ms": None,
+        "learned_relax_to_rest_ms": learned["ms"] if learned else None,
+        "learned_relax_to_rest": learned,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,
                      "kernel": "v2_substep_kernel", "bytes_per_cell_substep": bpc,

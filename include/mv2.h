@@ -236,7 +236,22 @@ int mv2_nn_mse(mv2_context *ctx, const float *pred, const float *target, int64_t
 int mv2_nn_adam(mv2_context *ctx, float *param, const float *grad, float *m, float *v, int n,
                 double lr, double beta1, double beta2, double eps, int step);
 int mv2_nn_project_w1(mv2_context *ctx, float *w1);
+/* slope-excess penalty of the warm-started surface pre + pred (no counterpart in
+ * the reference; training aid of the learned warm start): adds its gradient to
+ * grad in place; loss_out = weight / N * sum over edges of max(|dz| - crit, 0)^2 */
+int mv2_nn_excess_loss(mv2_context *ctx, const float *pred, const float *pre, int B, int H, int W,
+                       const double crit_diff[4], double weight, float *grad, double *loss_out);
 int mv2_nn_apply_delta(mv2_context *ctx, float *b, const float *delta, int64_t n, double eps);
+/* b <- projected(z): the change z - b, shifted to zero sum over the cells it
+ * moves (|change| > eps), applied to b (heights kept >= 0).  No counterpart in
+ * the reference (learned warm start). */
+int mv2_nn_commit_surface(mv2_context *ctx, float *b, const float *z, int64_t n, double eps);
+/* Lipschitz envelope (no counterpart in the reference; learned warm start):
+ * z <- the largest surface below z (or below b + delta when delta is given)
+ * with |z_a - z_b| <= crit_diff[d] across every edge of the four legacy
+ * directions; device float32 H[y, x] fields.  launches_out[0] = launches used. */
+int mv2_envelope(mv2_context *ctx, const float *b, const float *delta, float *z, int ny, int nx,
+                 const double crit_diff[4], int max_launches, int *launches_out);
 /* batched relaxation of independent FP32 patches (<= 4096 cells each), one
  * workgroup per patch, same schedule as mv2_relax; iterations per patch out */
 int mv2_relax_patches(mv2_context *ctx, float *h, int npatch, int H, int W,

@@ -233,7 +233,8 @@ __device__ inline void atomic_max_nonneg(unsigned long long *slot, double v)
 constexpr int kSpread = 64;
 constexpr int kSpreadStride = 16;                    // u64 words between slots
 constexpr int kSpreadWords = kSpread * kSpreadStride;
-// the step uses sets 0..4, the relaxation sets 8..10, the labelling 12..14
+// the step uses sets 0..4, the envelope 5..7, the relaxation 8..10, the labelling 12..14
+constexpr int kEnvSpreadSet = 5;
 constexpr int kRelaxSpreadSet = 8;
 constexpr int kLabelSpreadSet = 12;
 constexpr size_t kSpreadScratchBytes = (size_t)16 * kSpreadWords * 8;

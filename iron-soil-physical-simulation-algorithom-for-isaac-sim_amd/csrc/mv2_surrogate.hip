@@ -367,33 +367,49 @@ __global__ void __launch_bounds__(kBlock) relax_patches_kernel(float *h, int npa
                     mx = mx >= ex ? mx : ex;
                 }
                 __syncthreads();
-                for (int e = threadIdx.x; e < rows * cols; e += blockDim.x) {
-                    const int r = e / cols, c = e - r * cols;
-                    const int coord = d == 1 ? r : c;
-                    if ((coord & 1) != phase) continue;
-                    int ia, ib;
-                    if (d == 0) { ia = r * W + c; ib = ia + 1; }
-                    else if (d == 1) { ia = r * W + c; ib = ia + W; }
-                    else if (d == 2) { ia = r * W + c; ib = ia + W + 1; }
-                    else { ia = (r + 1) * W + c; ib = r * W + c + 1; }
-                    const float diff = t[ia] - t[ib];
-                    float ex = fabsf(diff) - crit[d];
-                    ex = ex > 0.0f ? ex : 0.0f;
-                    const float sgn = diff > 0.0f ? 1.0f : (diff < 0.0f ? -1.0f : 0.0f);
-                    const float tr = (0.5f * ex) * sgn;
-                    t[ia] = t[ia] - tr;
-                    t[ib] = t[ib] + tr;
+                // red-black transfer: enumerate only the anchors of the active
+                // parity (the coordinate is c for d = 0, 2, 3 and r for d = 1);
+                // their edges are disjoint, so a thread loads the operands of
+                // four edges before it stores any of them
+                const int arows = (d == 1) ? (rows - phase + 1) / 2 : rows;
+                const int acols = (d == 1) ? cols : (cols - phase + 1) / 2;
+                const int nact = arows * acols;
+                for (int e0 = threadIdx.x; e0 < nact; e0 += 4 * blockDim.x) {
+                    int ia[4], ib[4];
+                    float va[4], vb[4];
+#pragma unroll
+                    for (int u = 0; u < 4; u++) {
+                        const int e = e0 + u * blockDim.x;
+                        ia[u] = -1;
+                        if (e >= nact) continue;
+                        const int ar = e / acols, ac = e - ar * acols;
+                        const int r = d == 1 ? 2 * ar + phase : ar;
+                        const int c = d == 1 ? ac : 2 * ac + phase;
+                        if (d == 0) { ia[u] = r * W + c; ib[u] = ia[u] + 1; }
+                        else if (d == 1) { ia[u] = r * W + c; ib[u] = ia[u] + W; }
+                        else if (d == 2) { ia[u] = r * W + c; ib[u] = ia[u] + W + 1; }
+                        else { ia[u] = (r + 1) * W + c; ib[u] = r * W + c + 1; }
+                        va[u] = t[ia[u]];
+                        vb[u] = t[ib[u]];
+                    }
+#pragma unroll
+                    for (int u = 0; u < 4; u++) {
+                        if (ia[u] < 0) continue;
+                        const float diff = va[u] - vb[u];
+                        float ex = fabsf(diff) - crit[d];
+                        ex = ex > 0.0f ? ex : 0.0f;
+                        const float sgn = diff > 0.0f ? 1.0f : (diff < 0.0f ? -1.0f : 0.0f);
+                        const float tr = (0.5f * ex) * sgn;
+                        t[ia[u]] = va[u] - tr;
+                        t[ib[u]] = vb[u] + tr;
+                    }
                 }
                 __syncthreads();
             }
         }
-        float m1[1] = {mx};
         // wave max then workgroup max
-        for (int off = kWave / 2; off > 0; off >>= 1) {
-            float o = __shfl_down(m1[0], off, kWave);
-            m1[0] = o > m1[0] ? o : m1[0];
-        }
-        if ((threadIdx.x & (kWave - 1)) == 0) red[threadIdx.x / kWave] = m1[0];
+        const float wm = wave_max(mx);
+        if ((threadIdx.x & (kWave - 1)) == 0) red[threadIdx.x / kWave] = wm;
         __syncthreads();
         if (threadIdx.x == 0) {
             float g = red[0];
@@ -407,14 +423,75 @@ __global__ void __launch_bounds__(kBlock) relax_patches_kernel(float *h, int npa
     if (threadIdx.x == 0 && iterations) iterations[patch] = it > max_iterations ? max_iterations : it;
 }
 
-// mass projection of the predicted change: remove the mean of the change over
-// the cells it moves (|delta| > eps) so sum(delta) = 0, then keep heights >= 0
-__global__ void __launch_bounds__(kBlock) delta_stats_kernel(const float *delta, int64_t n, float eps, double *partials)
+// Slope-excess penalty on the warm-started surface z = pre + pred:
+//   L_ex = weight / N * sum over edges e of max(|z_a - z_b| - c_e, 0)^2
+// over the four legacy edge directions (a -> b = a + off_d: d0 (0,+1),
+// d1 (+1,0), d2 (+1,+1), d3 (-1,+1) in (row, col); c_e = crit[d]).  Each
+// thread owns one cell and gathers the gradient of the (up to) eight edges it
+// belongs to, so the in-place update of grad needs no atomics; the loss adds
+// each edge once, on its anchor cell a.
+__global__ void __launch_bounds__(kBlock) excess_loss_kernel(const float *pred, const float *pre, int B, int H,
+                                                             int W, float c0, float c1, float c2, float c3,
+                                                             float weight, float *grad, double *partials)
+{
+    __shared__ double smem[4];
+    const float crit[4] = {c0, c1, c2, c3};
+    const int dr[4] = {0, 1, 1, -1}, dc[4] = {1, 0, 1, 1};
+    const int64_t n = (int64_t)B * H * W;
+    const float scale = weight / (float)n;
+    double s[1] = {0.0};
+    for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < n; p += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t img = p / ((int64_t)H * W);
+        const int rem = (int)(p - img * H * W);
+        const int r = rem / W, c = rem - r * W;
+        const float zp = pre[p] + pred[p];
+        float g = 0.0f;
+#pragma unroll
+        for (int d = 0; d < 4; d++) {
+            // cell as a: partner b = p + off
+            int rr = r + dr[d], cc = c + dc[d];
+            if (rr >= 0 && rr < H && cc >= 0 && cc < W) {
+                const int64_t q = p + (int64_t)dr[d] * W + dc[d];
+                const float diff = zp - (pre[q] + pred[q]);
+                const float ex = fabsf(diff) - crit[d];
+                if (ex > 0.0f) {
+                    g += 2.0f * ex * (diff > 0.0f ? 1.0f : -1.0f);
+                    s[0] += (double)ex * (double)ex;
+                }
+            }
+            // cell as b: partner a = p - off
+            rr = r - dr[d]; cc = c - dc[d];
+            if (rr >= 0 && rr < H && cc >= 0 && cc < W) {
+                const int64_t q = p - (int64_t)dr[d] * W - dc[d];
+                const float diff = (pre[q] + pred[q]) - zp;
+                const float ex = fabsf(diff) - crit[d];
+                if (ex > 0.0f) g -= 2.0f * ex * (diff > 0.0f ? 1.0f : -1.0f);
+            }
+        }
+        grad[p] += scale * g;
+    }
+    block_reduce<double, 1, 0>(s, smem);
+    if (threadIdx.x == 0) partials[blockIdx.x] = s[0];
+}
+
+// Mass projection of a change delta of b (delta = given array, or z - b for a
+// candidate surface z): remove the mean of the change over the cells it moves
+// (|delta| > eps) so the change sums to zero there, then keep heights >= 0.
+// Field-major partials: sum at [g], count at [gridDim.x + g].
+template <bool FROM_SURFACE>
+__device__ inline float change_at(const float *b, const float *src, int64_t p)
+{
+    return FROM_SURFACE ? src[p] - b[p] : src[p];
+}
+
+template <bool FROM_SURFACE>
+__global__ void __launch_bounds__(kBlock) delta_stats_kernel(const float *b, const float *src, int64_t n, float eps,
+                                                             double *partials)
 {
     __shared__ double smem[8];
     double s[2] = {0.0, 0.0};
     for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < n; p += (int64_t)gridDim.x * blockDim.x) {
-        const float d = delta[p];
+        const float d = change_at<FROM_SURFACE>(b, src, p);
         if (fabsf(d) > eps) {
             s[0] += (double)d;
             s[1] += 1.0;
@@ -422,30 +499,152 @@ __global__ void __launch_bounds__(kBlock) delta_stats_kernel(const float *delta,
     }
     block_reduce<double, 2, 0>(s, smem);
     if (threadIdx.x == 0) {
-        partials[blockIdx.x * 2] = s[0];
-        partials[blockIdx.x * 2 + 1] = s[1];
+        partials[blockIdx.x] = s[0];
+        partials[gridDim.x + blockIdx.x] = s[1];
     }
 }
 
-__global__ void __launch_bounds__(kBlock) apply_delta_kernel(float *b, const float *delta, int64_t n, float eps,
-                                                             const double *partials, int nparts)
+// fold the partials (one workgroup, fixed order) into the shift
+__global__ void __launch_bounds__(kBlock) delta_shift_kernel(const double *partials, int nparts, float *shift)
 {
-    __shared__ float shift;
-    if (threadIdx.x == 0) {
-        double s = 0.0, c = 0.0;
-        for (int k = 0; k < nparts; k++) {
-            s += partials[2 * k];
-            c += partials[2 * k + 1];
-        }
-        shift = c > 0.0 ? (float)(s / c) : 0.0f;
+    __shared__ double smem[8];
+    double s[2] = {0.0, 0.0};
+    for (int k = threadIdx.x; k < nparts; k += blockDim.x) {
+        s[0] += partials[k];
+        s[1] += partials[nparts + k];
     }
-    __syncthreads();
+    block_reduce<double, 2, 0>(s, smem);
+    if (threadIdx.x == 0) *shift = s[1] > 0.0 ? (float)(s[0] / s[1]) : 0.0f;
+}
+
+template <bool FROM_SURFACE>
+__global__ void __launch_bounds__(kBlock) apply_delta_kernel(float *b, const float *src, int64_t n, float eps,
+                                                             const float *shift_ptr)
+{
+    const float shift = *shift_ptr;
     for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < n; p += (int64_t)gridDim.x * blockDim.x) {
-        const float d = delta[p];
+        const float d = change_at<FROM_SURFACE>(b, src, p);
         if (fabsf(d) > eps) {
             const float v = b[p] + (d - shift);
             b[p] = v > 0.0f ? v : 0.0f;
         }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Lipschitz envelope: the largest surface below z whose differences across
+// every edge of the four legacy directions obey |z_a - z_b| <= crit[d], i.e.
+// z(p) = min over q of (z(q) + shortest edge-weighted path length q -> p).
+// It only lowers cells, by exactly the amount their slopes exceed the limit,
+// so after it the exact relaxation finds no excess.  One launch per batch of
+// kEnvSweeps in-place sweeps over a tile plus a kEnvSweeps-cell halo in LDS:
+// values only ever decrease and always stay above the envelope, so
+// concurrent updates are benign, and after s sweeps a cell is at least as low
+// as after s Jacobi steps, which the halo makes exact for the interior.  The
+// next launch stops the loop when no interior cell of any tile changed.  The
+// fixed point is the minimum over paths of the path sums evaluated from the
+// source outwards; float rounding is monotone, so every update order reaches
+// the same bits.
+// ---------------------------------------------------------------------------
+constexpr int kEnvSweeps = 8;
+constexpr int kEnvTH = 32, kEnvTW = 64;
+constexpr int kEnvLH = kEnvTH + 2 * kEnvSweeps, kEnvLW = kEnvTW + 2 * kEnvSweeps;
+
+struct EnvControl {
+    int launches, done, pad[2];
+};
+
+__global__ void add_surface_kernel(const float *b, const float *delta, float *z, int64_t n)
+{
+    for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < n; p += (int64_t)gridDim.x * blockDim.x)
+        z[p] = b[p] + delta[p];
+}
+
+__global__ void __launch_bounds__(kBlock) envelope_kernel(float *z, int ny, int nx, float c0, float c1, float c2,
+                                                          float c3, int tiles_x, EnvControl *ctl,
+                                                          unsigned long long *spread, int k)
+{
+    __shared__ float t[kEnvLH * kEnvLW];
+    __shared__ double smem[1];
+    __shared__ int changed;
+    // launch k - 1 recorded whether any interior cell changed in spread set (k-1) % 3
+    EnvControl C = ctl[k & 1];
+    if (!C.done && k > 0) {
+        const double last = spread_read_max(spread + ((k - 1) % 3) * kSpreadWords, smem);
+        if (last <= 0.0) C.done = 1;
+    }
+    if (C.done) {
+        if (blockIdx.x == 0 && threadIdx.x == 0) ctl[(k + 1) & 1] = C;
+        return;
+    }
+    if (blockIdx.x == 0) spread_clear(spread + ((k + 1) % 3) * kSpreadWords);
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        EnvControl nc = C;
+        nc.launches = C.launches + 1;
+        ctl[(k + 1) & 1] = nc;
+    }
+    const int r0 = (blockIdx.x / tiles_x) * kEnvTH, c0i = (blockIdx.x % tiles_x) * kEnvTW;
+    const int R0 = r0 - kEnvSweeps, C0 = c0i - kEnvSweeps;
+    if (threadIdx.x == 0) changed = 0;
+    for (int e = threadIdx.x; e < kEnvLH * kEnvLW; e += blockDim.x) {
+        const int gr = R0 + e / kEnvLW, gc = C0 + e % kEnvLW;
+        t[e] = (gr >= 0 && gr < ny && gc >= 0 && gc < nx) ? z[gr * nx + gc] : __builtin_huge_valf();
+    }
+    __syncthreads();
+    // neighbours (dr, dc) and their limits: both ends of d0 (0, +-1), d1 (+-1, 0),
+    // d2 (+-1, +-1 same sign), d3 (-+1, +-1 opposite sign)
+    const int nr[8] = {0, 0, 1, -1, 1, -1, -1, 1};
+    const int ncol[8] = {1, -1, 0, 0, 1, -1, 1, -1};
+    const float lim[8] = {c0, c0, c1, c1, c2, c2, c3, c3};
+    int moved = 0;
+    for (int sweep = 0; sweep < kEnvSweeps; sweep++) {
+        for (int e = threadIdx.x; e < kEnvLH * kEnvLW; e += blockDim.x) {
+            const int lr = e / kEnvLW, lc = e % kEnvLW;
+            // other threads lower their cells concurrently: word-sized relaxed
+            // LDS accesses, any value read is a valid upper bound
+            const float v = __hip_atomic_load(&t[e], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+            if (v == __builtin_huge_valf()) continue;        // outside the grid
+            float m = v;
+#pragma unroll
+            for (int q = 0; q < 8; q++) {
+                const int rr = lr + nr[q], cc = lc + ncol[q];
+                if (rr < 0 || rr >= kEnvLH || cc < 0 || cc >= kEnvLW) continue;
+                const float cand = __hip_atomic_load(&t[rr * kEnvLW + cc], __ATOMIC_RELAXED,
+                                                     __HIP_MEMORY_SCOPE_WORKGROUP) + lim[q];
+                m = cand < m ? cand : m;
+            }
+            if (m < v) {
+                __hip_atomic_store(&t[e], m, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                if (lr >= kEnvSweeps && lr < kEnvSweeps + kEnvTH && lc >= kEnvSweeps && lc < kEnvSweeps + kEnvTW)
+                    moved = 1;
+            }
+        }
+        __syncthreads();
+    }
+    for (int e = threadIdx.x; e < kEnvTH * kEnvTW; e += blockDim.x) {
+        const int ir = e / kEnvTW, ic = e % kEnvTW;
+        const int gr = r0 + ir, gc = c0i + ic;
+        if (gr < ny && gc < nx) z[gr * nx + gc] = t[(ir + kEnvSweeps) * kEnvLW + ic + kEnvSweeps];
+    }
+    if (moved) changed = 1;     // benign race: every writer stores 1
+    __syncthreads();
+    if (threadIdx.x == 0 && changed) spread_max(spread + (k % 3) * kSpreadWords, 1.0);
+}
+
+__global__ void envelope_reset_kernel(EnvControl *ctl, unsigned long long *spread)
+{
+    if (threadIdx.x < 2) ctl[threadIdx.x] = EnvControl{0, 0, {0, 0}};
+    for (int i = threadIdx.x; i < 3 * kSpread; i += blockDim.x) spread[(int64_t)i * kSpreadStride] = 0ull;
+}
+
+__global__ void envelope_result_kernel(const EnvControl *ctl, const unsigned long long *spread, int k, int *out)
+{
+    __shared__ double smem[1];
+    EnvControl C = ctl[k & 1];
+    if (!C.done && k > 0 && spread_read_max(spread + ((k - 1) % 3) * kSpreadWords, smem) <= 0.0) C.done = 1;
+    if (threadIdx.x == 0) {
+        out[0] = C.launches;
+        out[1] = C.done;
     }
 }
 
@@ -561,6 +760,31 @@ extern "C" int mv2_nn_mse(mv2_context *ctx, const float *pred, const float *targ
     return MV2_OK;
 }
 
+extern "C" int mv2_nn_excess_loss(mv2_context *ctx, const float *pred, const float *pre, int B, int H, int W,
+                                  const double crit_diff[4], double weight, float *grad, double *loss_out)
+{
+    if (int rc = check_plain(ctx)) return rc;
+    if (!pred || !pre || !grad || !crit_diff || B < 1 || H < 1 || W < 1)
+        return set_error(ctx, MV2_EINVAL, "[WarmStart] invalid excess-loss arguments");
+    const int64_t n = (int64_t)B * H * W;
+    const int grid = grid_for(n, kBlock, 1024);
+    if (int rc = ensure_scratch(ctx, ctx->nn_loss, sizeof(double) * 2048)) return rc;
+    double *parts = (double *)ctx->nn_loss.ptr;
+    hipLaunchKernelGGL(excess_loss_kernel, dim3(grid), dim3(kBlock), 0, ctx->stream, pred, pre, B, H, W,
+                       (float)crit_diff[0], (float)crit_diff[1], (float)crit_diff[2], (float)crit_diff[3],
+                       (float)weight, grad, parts);
+    MV2_HIP(ctx, hipGetLastError());
+    if (loss_out) {
+        std::vector<double> host(grid);
+        MV2_HIP(ctx, hipMemcpyAsync(host.data(), parts, sizeof(double) * grid, hipMemcpyDeviceToHost, ctx->stream));
+        MV2_HIP(ctx, hipStreamSynchronize(ctx->stream));
+        double acc = 0.0;
+        for (double v : host) acc += v;
+        *loss_out = weight * acc / (double)n;
+    }
+    return MV2_OK;
+}
+
 extern "C" int mv2_nn_adam(mv2_context *ctx, float *param, const float *grad, float *m, float *v, int n, double lr,
                            double beta1, double beta2, double eps, int step)
 {
@@ -595,15 +819,70 @@ extern "C" int mv2_relax_patches(mv2_context *ctx, float *h, int npatch, int H, 
     return MV2_OK;
 }
 
+template <bool FROM_SURFACE>
+static int project_change(mv2_context *ctx, float *b, const float *src, int64_t n, double eps)
+{
+    const int grid = grid_for(n, kBlock, 1024);
+    if (int rc = ensure_scratch(ctx, ctx->nn_loss, sizeof(double) * 2 * 1024 + 64)) return rc;
+    double *parts = (double *)ctx->nn_loss.ptr;
+    float *shift = (float *)(parts + 2 * 1024);
+    hipLaunchKernelGGL(delta_stats_kernel<FROM_SURFACE>, dim3(grid), dim3(kBlock), 0, ctx->stream, b, src, n,
+                       (float)eps, parts);
+    hipLaunchKernelGGL(delta_shift_kernel, dim3(1), dim3(kBlock), 0, ctx->stream, parts, grid, shift);
+    hipLaunchKernelGGL(apply_delta_kernel<FROM_SURFACE>, dim3(grid), dim3(kBlock), 0, ctx->stream, b, src, n,
+                       (float)eps, shift);
+    MV2_HIP(ctx, hipGetLastError());
+    return MV2_OK;
+}
+
 extern "C" int mv2_nn_apply_delta(mv2_context *ctx, float *b, const float *delta, int64_t n, double eps)
 {
     if (int rc = check_plain(ctx)) return rc;
-    const int grid = grid_for(n, kBlock, 1024);
-    if (int rc = ensure_scratch(ctx, ctx->nn_loss, sizeof(double) * 2048)) return rc;
-    double *parts = (double *)ctx->nn_loss.ptr;
-    hipLaunchKernelGGL(delta_stats_kernel, dim3(grid), dim3(kBlock), 0, ctx->stream, delta, n, (float)eps, parts);
-    hipLaunchKernelGGL(apply_delta_kernel, dim3(grid), dim3(kBlock), 0, ctx->stream, b, delta, n, (float)eps, parts,
-                       grid);
-    MV2_HIP(ctx, hipGetLastError());
+    if (!b || !delta || n < 1) return set_error(ctx, MV2_EINVAL, "[WarmStart] invalid apply_delta arguments");
+    return project_change<false>(ctx, b, delta, n, eps);
+}
+
+extern "C" int mv2_nn_commit_surface(mv2_context *ctx, float *b, const float *z, int64_t n, double eps)
+{
+    if (int rc = check_plain(ctx)) return rc;
+    if (!b || !z || n < 1) return set_error(ctx, MV2_EINVAL, "[WarmStart] invalid commit arguments");
+    return project_change<true>(ctx, b, z, n, eps);
+}
+
+extern "C" int mv2_envelope(mv2_context *ctx, const float *b, const float *delta, float *z, int ny, int nx,
+                            const double crit_diff[4], int max_launches, int *launches_out)
+{
+    if (int rc = check_plain(ctx)) return rc;
+    if (!z || !crit_diff || ny < 1 || nx < 1 || max_launches < 1 || (delta && !b))
+        return set_error(ctx, MV2_EINVAL, "[WarmStart] invalid envelope arguments");
+    if ((int64_t)ny * nx >= ((int64_t)1 << 31)) return set_error(ctx, MV2_EINVAL, "[WarmStart] grid too large");
+    for (int d = 0; d < 4; d++)
+        if (!(crit_diff[d] >= 0.0)) return set_error(ctx, MV2_EINVAL, "[WarmStart] critical differences must be >= 0");
+    const int64_t n = (int64_t)ny * nx;
+    hipStream_t st = ctx->stream;
+    if (int rc = ensure_scratch(ctx, ctx->spread, kSpreadScratchBytes)) return rc;
+    unsigned long long *spread = (unsigned long long *)ctx->spread.ptr + (size_t)kEnvSpreadSet * kSpreadWords;
+    EnvControl *ctl = (EnvControl *)((char *)ctx->sync.ptr + 2048);
+    int *dres = (int *)((char *)ctx->result.ptr + 3800);
+    if (delta) hipLaunchKernelGGL(add_surface_kernel, dim3(grid_for(n, kBlock, 2048)), dim3(kBlock), 0, st, b, delta, z, n);
+    hipLaunchKernelGGL(envelope_reset_kernel, dim3(1), dim3(kBlock), 0, st, ctl, spread);
+    const int tiles_x = (nx + kEnvTW - 1) / kEnvTW;
+    const int tiles = tiles_x * ((ny + kEnvTH - 1) / kEnvTH);
+    int k = 0, chunk = 4;
+    const int *hres = (const int *)ctx->host_result.ptr;
+    for (;;) {
+        if (chunk > max_launches - k) chunk = max_launches - k;
+        for (int j = 0; j < chunk; j++, k++)
+            hipLaunchKernelGGL(envelope_kernel, dim3(tiles), dim3(kBlock), 0, st, z, ny, nx, (float)crit_diff[0],
+                               (float)crit_diff[1], (float)crit_diff[2], (float)crit_diff[3], tiles_x, ctl, spread, k);
+        hipLaunchKernelGGL(envelope_result_kernel, dim3(1), dim3(kWave), 0, st, ctl, spread, k, dres);
+        MV2_HIP(ctx, hipGetLastError());
+        MV2_HIP(ctx, hipMemcpyAsync(ctx->host_result.ptr, dres, 2 * sizeof(int), hipMemcpyDeviceToHost, st));
+        MV2_HIP(ctx, hipStreamSynchronize(st));
+        if (hres[1] || k >= max_launches) break;
+        chunk = 2 * chunk;
+    }
+    if (launches_out) launches_out[0] = hres[0];
+    if (!hres[1]) return set_error(ctx, MV2_ENUMERIC, "[WarmStart] envelope did not converge in %d launches", max_launches);
     return MV2_OK;
 }

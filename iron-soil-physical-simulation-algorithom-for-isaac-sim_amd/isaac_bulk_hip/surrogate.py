@@ -162,8 +162,14 @@ class WarmStartNet:
                                         DILATIONS[0], 32, 4, grads["w1"].data_ptr(), grads["b1"].data_ptr()))
         return grads
 
-    def train_step(self, b, h, target, dx: float, dy: float, *, lr: float = 1e-3) -> float:
-        torch = _torch()
+    def train_step(self, b, h, target, dx: float, dy: float, *, lr: float = 1e-3,
+                   excess_weight: float = 0.0, excess_crit=None) -> float:
+        """One Adam step on MSE(pred, target) [+ excess_weight * slope excess of b + pred].
+
+        ``excess_crit`` is the legacy critical-difference vector of the angle the
+        warm-started surface must not exceed.  Returns the total loss; the two
+        terms are kept in ``last_losses``.
+        """
         rt = self.rt
         pred = self.forward(b, h, dx, dy)
         B, H, W = b.shape
@@ -171,6 +177,12 @@ class WarmStartNet:
         loss = ctypes.c_double(0.0)
         rt.check(rt.lib.mv2_nn_mse(rt.ctx, pred.data_ptr(), target.data_ptr(), B * H * W,
                                    buf["g5"].data_ptr(), ctypes.byref(loss)))
+        ex = ctypes.c_double(0.0)
+        if excess_weight > 0.0:
+            crit = np.ascontiguousarray(excess_crit, dtype=np.float64)
+            rt.check(rt.lib.mv2_nn_excess_loss(rt.ctx, pred.data_ptr(), b.data_ptr(), B, H, W, crit.ctypes.data,
+                                               float(excess_weight), buf["g5"].data_ptr(), ctypes.byref(ex)))
+        self.last_losses = (float(loss.value), float(ex.value))
         grads = self.backward(buf["g5"], (B, H, W))
         self.step_count += 1
         for k, p in self.p.items():
@@ -179,7 +191,7 @@ class WarmStartNet:
                                         self.step_count))
         rt.check(rt.lib.mv2_nn_project_w1(rt.ctx, self.p["w1"].data_ptr()))
         self.pack()
-        return float(loss.value)
+        return float(loss.value) + float(ex.value)
 
 
 # ---------------------------------------------------------------------------
@@ -237,19 +249,37 @@ class WarmStartResult:
     used_fallback: bool
     residual_after_sweeps: float
     elapsed_ms: float
+    envelope_launches: int = 0
 
 
 def warm_start_relax(net: WarmStartNet, b_field, h_field, dx: float, critical_angle_deg: float,
                      tolerance: float, *, max_iterations: int = 20000, sweeps: int = 4,
-                     fallback_threshold: float = 1e-6, eps: float = 1e-5) -> WarmStartResult:
-    """In-place accelerated relaxation of a device float32 [H, W] b_eff field."""
+                     fallback_threshold: float = 1e-6, eps: float = 1e-5,
+                     envelope_angle_deg: float | None = None) -> WarmStartResult:
+    """In-place accelerated relaxation of a device float32 [H, W] b_eff field.
+
+    1. the network predicts the relaxed change of b_eff;
+    2. the Lipschitz envelope (slopes <= envelope angle, default the critical
+       angle) removes the local over-steep spots of the predicted surface;
+    3. mass projection: the change is shifted to zero sum over the moved cells;
+    4. at most ``sweeps`` exact relaxation iterations; if the maximum excess
+       still exceeds the tolerance, the plain relaxation continues to the same
+       tolerance (fallback; ``used_fallback`` when the residual after the sweeps
+       was above ``fallback_threshold``).
+    """
     torch = _torch()
     rt = net.rt
     torch.cuda.synchronize(rt.device)
     t0 = time.perf_counter()
     H, W = b_field.shape
     pred = net.forward(b_field.view(1, H, W), None if h_field is None else h_field.view(1, H, W), dx, dx)
-    rt.check(rt.lib.mv2_nn_apply_delta(rt.ctx, b_field.data_ptr(), pred.data_ptr(), H * W, eps))
+    env_deg = critical_angle_deg if envelope_angle_deg is None else envelope_angle_deg
+    crit = legacy_critical_differences(dx, dx, env_deg)
+    launches = (ctypes.c_int * 1)()
+    zbuf = pred.view(H, W)                      # the surface overwrites the prediction in place
+    rt.check(rt.lib.mv2_envelope(rt.ctx, b_field.data_ptr(), pred.data_ptr(), zbuf.data_ptr(), H, W,
+                                 crit.ctypes.data, 256, launches))
+    rt.check(rt.lib.mv2_nn_commit_surface(rt.ctx, b_field.data_ptr(), zbuf.data_ptr(), H * W, eps))
     it, conv, last = relax_device(b_field, dx, dx, critical_angle_deg, sweeps, tolerance, runtime=rt)
     fb_it = 0
     used = False
@@ -259,4 +289,4 @@ def warm_start_relax(net: WarmStartNet, b_field, h_field, dx: float, critical_an
         fb_it, conv, last2 = relax_device(b_field, dx, dx, critical_angle_deg, max_iterations, tolerance,
                                           runtime=rt)
     torch.cuda.synchronize(rt.device)
-    return WarmStartResult(it, fb_it, conv, used, last, (time.perf_counter() - t0) * 1e3)
+    return WarmStartResult(it, fb_it, conv, used, last, (time.perf_counter() - t0) * 1e3, int(launches[0]))

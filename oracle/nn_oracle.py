@@ -135,3 +135,68 @@ def backward(params: dict, acts: list, grad_out: np.ndarray, *, bf16_mode: bool 
         gz = np.asarray(bf16(gz.astype(np.float32)), np.float64)
     grads["w1"], grads["b1"] = _conv_backward_weight(gz, acts[0], DILATIONS[0], replicate_channel0=True)
     return grads
+
+
+EDGE_OFFSETS = ((0, 1), (1, 0), (1, 1), (-1, 1))   # legacy directions d0..d3 as (row, col) a -> b
+
+
+def excess_loss(pred: np.ndarray, pre: np.ndarray, crit, weight: float):
+    """Slope-excess penalty of z = pre + pred over [B, H, W] and its gradient.
+
+    loss = weight / N * sum over edges (a, b = a + off_d) of max(|z_a - z_b| - crit[d], 0)^2.
+    """
+    z = np.asarray(pre, np.float64) + np.asarray(pred, np.float64)
+    B, H, W = z.shape
+    n = B * H * W
+    grad = np.zeros_like(z)
+    total = 0.0
+    for d, (dr, dc) in enumerate(EDGE_OFFSETS):
+        r0, r1 = max(0, -dr), min(H, H - dr)
+        c0, c1 = max(0, -dc), min(W, W - dc)
+        za = z[:, r0:r1, c0:c1]
+        zb = z[:, r0 + dr:r1 + dr, c0 + dc:c1 + dc]
+        diff = za - zb
+        ex = np.maximum(np.abs(diff) - crit[d], 0.0)
+        total += float((ex * ex).sum())
+        g = 2.0 * ex * np.sign(diff)
+        grad[:, r0:r1, c0:c1] += g
+        grad[:, r0 + dr:r1 + dr, c0 + dc:c1 + dc] -= g
+    return weight * total / n, grad * (weight / n)
+
+
+NEIGHBOURS = ((0, 1, 0), (0, -1, 0), (1, 0, 1), (-1, 0, 1), (1, 1, 2), (-1, -1, 2), (-1, 1, 3), (1, -1, 3))
+
+
+def envelope(z: np.ndarray, crit, *, max_sweeps: int = 100000) -> np.ndarray:
+    """Largest surface below z with |z_a - z_b| <= crit[d] across every legacy edge.
+
+    Jacobi iteration z <- min(z, z_nbr + crit) over the 8 neighbours (edge
+    direction d of each neighbour as in NEIGHBOURS) until nothing changes, in the
+    dtype of z (float32 in, float32 arithmetic).
+    """
+    z = np.array(z, copy=True)
+    H, W = z.shape
+    c = np.asarray(crit, dtype=z.dtype)
+    big = np.array(np.inf, dtype=z.dtype)
+    for _ in range(max_sweeps):
+        pad = np.full((H + 2, W + 2), big, dtype=z.dtype)
+        pad[1:-1, 1:-1] = z
+        m = z.copy()
+        for dr, dc, d in NEIGHBOURS:
+            cand = pad[1 + dr:1 + dr + H, 1 + dc:1 + dc + W] + c[d]
+            np.minimum(m, cand, out=m)
+        if np.array_equal(m, z):
+            return z
+        z = m
+    raise RuntimeError("envelope did not converge")
+
+
+def commit_surface(b: np.ndarray, z: np.ndarray, eps: float) -> np.ndarray:
+    """b + (z - b) with the change shifted to zero mean over |z - b| > eps (float32), heights >= 0."""
+    b = np.asarray(b, np.float32)
+    d = np.asarray(z, np.float32) - b
+    m = np.abs(d) > np.float32(eps)
+    shift = np.float32(d[m].astype(np.float64).sum() / m.sum()) if m.any() else np.float32(0.0)
+    out = b.copy()
+    out[m] = np.maximum(b[m] + (d[m] - shift), np.float32(0.0))
+    return out

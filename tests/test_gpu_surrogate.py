@@ -33,8 +33,9 @@ def test_mfma_conv_lane_layout_against_oracle():
     rt.bind_stream()
     rt.check(rt.lib.mv2_nn_pack_weights(rt.ctx, wt.data_ptr(), 32, 32, 32, 32, wf.data_ptr(), None))
     xt = _t(x, torch.bfloat16)
+    bt = _t(bias, torch.float32)
     y = torch.zeros((B, H, W, 32), dtype=torch.bfloat16, device="cuda:0")
-    rt.check(rt.lib.mv2_nn_conv_mfma(rt.ctx, xt.data_ptr(), 32, wf.data_ptr(), _t(bias, torch.float32).data_ptr(),
+    rt.check(rt.lib.mv2_nn_conv_mfma(rt.ctx, xt.data_ptr(), 32, wf.data_ptr(), bt.data_ptr(),
                                      None, y.data_ptr(), B, H, W, 3, 1, 0))
     ref = np.maximum(nn_oracle.conv(x, w, bias, 3), 0.0)
     got = y.float().cpu().numpy()
@@ -86,3 +87,66 @@ def test_batched_patch_relaxation_bit_exact():
         o, it, conv, _ = oracle.relax_yx(pre[i], 0.05, 0.05, crit, 3000, 1e-6, dtype=np.float32)
         assert its[i] == it
         np.testing.assert_array_equal(got[i], o)
+
+
+def test_excess_loss_kernel_against_oracle():
+    import ctypes
+
+    import torch
+    from isaac_bulk_hip.relaxation import _runtime, legacy_critical_differences
+
+    rt = _runtime("cuda:0")
+    rng = np.random.default_rng(11)
+    B, H, W = 3, 33, 47
+    pre = rng.uniform(0.0, 0.3, size=(B, H, W)).astype(np.float32)
+    pred = rng.uniform(-0.03, 0.03, size=(B, H, W)).astype(np.float32)
+    g0 = rng.standard_normal((B, H, W)).astype(np.float32) * 1e-3
+    crit = legacy_critical_differences(0.05, 0.05, 34.0)
+    grad = _t(g0, torch.float32)
+    pred_t, pre_t = _t(pred, torch.float32), _t(pre, torch.float32)   # held across the launch
+    loss = ctypes.c_double(0.0)
+    rt.bind_stream()
+    rt.check(rt.lib.mv2_nn_excess_loss(rt.ctx, pred_t.data_ptr(), pre_t.data_ptr(), B, H, W, crit.ctypes.data,
+                                       2.5, grad.data_ptr(), ctypes.byref(loss)))
+    ref_loss, ref_grad = nn_oracle.excess_loss(pred.astype(np.float64), pre.astype(np.float64), crit, 2.5)
+    assert ref_loss > 0.0
+    # FP32 surface and edge differences on the device
+    assert abs(loss.value - ref_loss) <= 1e-4 * ref_loss
+    np.testing.assert_allclose(grad.cpu().numpy(), g0 + ref_grad, rtol=1e-3, atol=1e-9)
+
+
+def test_envelope_kernel_bit_exact_against_oracle():
+    import ctypes
+
+    import torch
+    from isaac_bulk_hip.relaxation import _runtime, legacy_critical_differences
+
+    rt = _runtime("cuda:0")
+    rng = np.random.default_rng(21)
+    for (H, W) in ((37, 53), (130, 77)):
+        b = rng.uniform(0.0, 0.5, size=(H, W)).astype(np.float32)
+        delta = (rng.standard_normal((H, W)) * 0.05).astype(np.float32)
+        crit = legacy_critical_differences(0.05, 0.05, 34.0)
+        bt, dt = _t(b, torch.float32), _t(delta, torch.float32)
+        z = torch.empty_like(bt)
+        launches = (ctypes.c_int * 1)()
+        rt.bind_stream()
+        rt.check(rt.lib.mv2_envelope(rt.ctx, bt.data_ptr(), dt.data_ptr(), z.data_ptr(), H, W, crit.ctypes.data, 200,
+                                     launches))
+        ref = nn_oracle.envelope(b + delta, crit.astype(np.float32))
+        np.testing.assert_array_equal(z.cpu().numpy(), ref)
+        assert launches[0] >= 1
+
+
+def test_commit_surface_against_oracle():
+    import torch
+    from isaac_bulk_hip.relaxation import _runtime
+
+    rt = _runtime("cuda:0")
+    rng = np.random.default_rng(22)
+    b = rng.uniform(0.5, 1.0, size=(64, 96)).astype(np.float32)
+    z = b + np.where(rng.random(b.shape) < 0.3, rng.standard_normal(b.shape) * 0.01, 0.0).astype(np.float32)
+    bt, zt = _t(b, torch.float32), _t(z, torch.float32)
+    rt.bind_stream()
+    rt.check(rt.lib.mv2_nn_commit_surface(rt.ctx, bt.data_ptr(), zt.data_ptr(), b.size, 1e-5))
+    np.testing.assert_allclose(bt.cpu().numpy(), nn_oracle.commit_surface(b, z, 1e-5), rtol=0, atol=2e-7)

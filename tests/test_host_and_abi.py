@@ -164,3 +164,16 @@ def test_workload_generators_are_seeded():
     bb, hh, face = cut_face_terrain(101, seed=1)
     assert bb.min() >= 0.0 and hh.min() >= 0.0 and face.any()
     assert np.all(hh[~face] == 0.0)
+
+
+def test_committed_warm_start_weights_load():
+    from isaac_bulk_hip.warm_start_training import DEFAULT_WEIGHTS, load_weights
+
+    params, meta = load_weights(DEFAULT_WEIGHTS)
+    assert params["w1"].shape == (9, 32, 3) and params["w5"].shape == (9, 1, 32)
+    for k in (2, 3, 4):
+        assert params[f"w{k}"].shape == (9, 32, 32)
+    assert meta["critical_deg"] == 34.0 and 0.0 <= meta["margin_deg"] <= 1.0
+    # zero-sum taps on the b_eff channel (constant-shift invariance)
+    assert np.abs(params["w1"][:, :, 0].sum(axis=0)).max() < 1e-5
+    assert all(np.isfinite(v).all() for v in params.values())

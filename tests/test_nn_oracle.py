@@ -40,3 +40,104 @@ def test_zero_sum_b_taps_make_the_net_shift_invariant():
     o2, _ = nn_oracle.forward(params, f2, bf16_mode=False)
     # invariant up to the float32 rounding of the shifted heights
     assert np.max(np.abs(o1 - o2)) < 1e-5
+
+
+def _excess_brute(z, crit):
+    """Edge-by-edge restatement with the relaxation's legacy edge table."""
+    B, H, W = z.shape
+    total = 0.0
+    for n in range(B):
+        for r in range(H):
+            for c in range(W):
+                edges = []
+                if c + 1 < W:
+                    edges.append((0, (r, c), (r, c + 1)))
+                if r + 1 < H:
+                    edges.append((1, (r, c), (r + 1, c)))
+                if r + 1 < H and c + 1 < W:
+                    edges.append((2, (r, c), (r + 1, c + 1)))
+                    edges.append((3, (r + 1, c), (r, c + 1)))
+                for d, a, b in edges:
+                    ex = max(abs(z[n][a] - z[n][b]) - crit[d], 0.0)
+                    total += ex * ex
+    return total
+
+
+def test_excess_loss_matches_edge_enumeration_and_finite_differences():
+    rng = np.random.default_rng(5)
+    pre = rng.uniform(0.0, 0.2, size=(2, 7, 9))
+    pred = rng.uniform(-0.05, 0.05, size=(2, 7, 9))
+    crit = np.array([0.02, 0.03, 0.04, 0.045])
+    loss, grad = nn_oracle.excess_loss(pred, pre, crit, 3.0)
+    n = pre.size
+    assert abs(loss - 3.0 * _excess_brute(pre + pred, crit) / n) < 1e-14
+    eps = 1e-7
+    for idx in ((0, 0, 0), (1, 3, 4), (0, 6, 8), (1, 0, 8), (0, 5, 1)):
+        p2 = pred.copy(); p2[idx] += eps
+        p3 = pred.copy(); p3[idx] -= eps
+        fd = (nn_oracle.excess_loss(p2, pre, crit, 3.0)[0] - nn_oracle.excess_loss(p3, pre, crit, 3.0)[0]) / (2 * eps)
+        assert abs(fd - grad[idx]) <= 1e-6 * max(1.0, abs(fd)), (idx, fd, grad[idx])
+
+
+def test_excess_loss_is_zero_on_subcritical_surfaces():
+    dx = 0.05
+    crit = np.tan(np.deg2rad(34.0)) * np.array([dx, dx, np.hypot(dx, dx), np.hypot(dx, dx)])
+    x = np.arange(16)[None, None, :] * dx
+    plane = np.tan(np.deg2rad(30.0)) * x + np.zeros((1, 12, 1))
+    loss, grad = nn_oracle.excess_loss(np.zeros_like(plane), plane, crit, 1.0)
+    assert loss == 0.0 and not grad.any()
+    steep = np.tan(np.deg2rad(40.0)) * x + np.zeros((1, 12, 1))
+    loss, grad = nn_oracle.excess_loss(np.zeros_like(steep), steep, crit, 1.0)
+    assert loss > 0.0 and grad.any()
+
+
+def _envelope_dijkstra(z, crit):
+    """min over q of z(q) + shortest path length q -> p (float64, brute force)."""
+    import heapq
+
+    H, W = z.shape
+    best = np.array(z, dtype=np.float64)
+    heap = [(best[r, c], r, c) for r in range(H) for c in range(W)]
+    heapq.heapify(heap)
+    done = np.zeros((H, W), bool)
+    while heap:
+        v, r, c = heapq.heappop(heap)
+        if done[r, c]:
+            continue
+        done[r, c] = True
+        for dr, dc, d in nn_oracle.NEIGHBOURS:
+            rr, cc = r + dr, c + dc
+            if 0 <= rr < H and 0 <= cc < W and v + crit[d] < best[rr, cc]:
+                best[rr, cc] = v + crit[d]
+                heapq.heappush(heap, (best[rr, cc], rr, cc))
+    return best
+
+
+def test_envelope_is_the_maximal_lipschitz_minorant():
+    rng = np.random.default_rng(9)
+    z = rng.uniform(0.0, 0.4, size=(9, 11))
+    crit = np.array([0.03, 0.035, 0.05, 0.045])
+    env = nn_oracle.envelope(z, crit)
+    np.testing.assert_allclose(env, _envelope_dijkstra(z, crit), rtol=0, atol=1e-15)
+    assert np.all(env <= z)
+    # every lowered cell is held down by a neighbour at exactly the limit
+    for r in range(9):
+        for c in range(11):
+            if env[r, c] < z[r, c]:
+                tight = [env[r + dr, c + dc] + crit[d] for dr, dc, d in nn_oracle.NEIGHBOURS
+                         if 0 <= r + dr < 9 and 0 <= c + dc < 11]
+                assert abs(min(tight) - env[r, c]) < 1e-15
+    # the excess of the envelope is zero up to the rounding of z_b + c - z_b
+    loss, _ = nn_oracle.excess_loss(np.zeros((1, 9, 11)), env[None], crit, 1.0)
+    assert loss < 1e-30
+
+
+def test_commit_surface_conserves_the_moved_mass():
+    rng = np.random.default_rng(4)
+    b = rng.uniform(0.5, 1.0, size=(20, 20)).astype(np.float32)
+    z = b.copy()
+    z[5:9, 5:9] -= 0.05
+    z[12:15, 3:8] += 0.02
+    out = nn_oracle.commit_surface(b, z, 1e-5)
+    assert abs(float(out.astype(np.float64).sum() - b.astype(np.float64).sum())) < 1e-5
+    assert np.array_equal(out[0:4, :], b[0:4, :])
