@@ -35,6 +35,11 @@ N = 701
 DX = 0.05
 FRAME_DT = 1.0 / 60.0
 HBM_PEAK_GBS = 8000.0
+# HBM bytes per working substep launch of the FP32 kernel on this workload,
+# from rocprofv3 FETCH_SIZE / WRITE_SIZE scaled by a 64 MiB copy
+# (profiles/v2_substep_701x701_cutface_traffic.md); counters cannot be read
+# from inside the timed run.
+MEASURED_TRAFFIC_BYTES = {"fp32": 15.99e6}
 BYTES_PER_CELL_SUBSTEP = {"fp32": 28, "fp64": 56}
 
 
@@ -50,6 +55,8 @@ def parse() -> argparse.Namespace:
     p.add_argument("--no-cpu-baseline", action="store_true")
     p.add_argument("--skip-relax-to-rest", action="store_true",
                    help="omit the relax-to-rest measurement (short profiling runs)")
+    p.add_argument("--calibrate-copy", action="store_true",
+                   help="only copy a 64 MiB float32 tensor 10 times (scale of the HBM byte counters)")
     return p.parse_args()
 
 
@@ -151,6 +158,15 @@ def main() -> int:
     import torch
 
     torch.cuda.set_device(0)
+    if args.calibrate_copy:
+        src = torch.rand(16 << 20, device="cuda:0")
+        dst = torch.empty_like(src)
+        for _ in range(10):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"calibrate_copy_bytes_read": src.numel() * 4, "bytes_written": dst.numel() * 4,
+                          "launches": 10}))
+        return 0
     state, solver, bridge, b0, h0 = build_state(args.precision, args.seed)
 
     # relax-to-rest of the initial terrain (plain relaxation, same tolerance as the frames)
@@ -231,7 +247,7 @@ def main() -> int:
         "learned_relax_to_rest_ms": learned["ms"] if learned else None,
         "learned_relax_to_rest": learned,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": MEASURED_TRAFFIC_BYTES.get(args.precision),
                      "kernel": "v2_substep_kernel", "bytes_per_cell_substep": bpc,
                      "cells_per_launch": cells, "avg_launch_ms": avg_substep_ms,
                      "launches_timed": total_sub},
