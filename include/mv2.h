@@ -124,6 +124,37 @@ int mv2_deposit(mv2_context *ctx, const int32_t *idx, const double *depth, int n
  * out = resting volume, mobile volume, density*px, density*py */
 int mv2_reservoirs(mv2_context *ctx, int weight_mode, double density, double out[4]);
 
+/* ---- batched Mobile V2 (no single counterpart in the reference: the
+ *      reference steps one terrain per solver, warp_mobile_v2.py:148-245;
+ *      this advances many independent terrains, each with its own CFL loop,
+ *      in the same launches).  FP32 fields packed back to back: instance i
+ *      occupies [offset[i], offset[i] + ny[i]*nx[i]) of each device array;
+ *      h_quanta in quanta of mv2_mobile_quantum() metres. ---- */
+typedef struct mv2_batch_record {
+    int substeps;
+    int numeric_error;                /* 1 positivity, 2 maximum_substeps, 3 int32 range */
+    int static_exit;
+    int complete;
+    double maximum_cfl;
+    double remaining;
+    double mass_quanta_before;        /* exact sums of the mobile quanta */
+    double mass_quanta_after;
+    double momentum_before[2];        /* sum of q * dx * dy (per unit density) */
+    double momentum_after[2];
+    double kinetic_before;            /* sum of 0.5 h |v|^2 dx dy (per unit density) */
+    double kinetic_after;
+} mv2_batch_record;
+int mv2_batch_bind(mv2_context *ctx, int count, const int32_t *ny, const int32_t *nx,
+                   const int64_t *offset, double dx, double dy, float *b, int32_t *h_quanta,
+                   float *qx, float *qy);
+/* tile shape of the batched kernels: 0 16x32, 1 8x64, 2 32x16, 3 16x64 */
+int mv2_batch_set_tile_shape(mv2_context *ctx, int shape);
+int mv2_batch_tile_shape(int shape, int *th, int *tw);
+/* one frame of every instance (cfg->dt each); use_graph replays a HIP graph of
+ * eight substep launches; out has one record per instance */
+int mv2_batch_step(mv2_context *ctx, const mv2_step_config *cfg, int use_graph,
+                   mv2_batch_record *out, double *kernel_ms);
+
 /* ---- plain angle-of-repose relaxation: slope_model.relax_critical_slope
  *      (slope_model.py:273-343) behind MinimumSlopeAdapter._legacy_call
  *      (pkg/solvers/minimum_slope_adapter.py:153-182, closed boundary),

@@ -1,0 +1,706 @@
+// Batched Mobile V2 step: hundreds of independent terrains of uneven size
+// advance together (FP32 fields, int32 mobile quanta).
+//
+// The instances' fields are packed back to back in caller-owned arrays; every
+// instance keeps its own CFL loop (own dt sequence, own substep count, own
+// static early exit), exactly as if it were stepped alone.  One launch per
+// substep covers the tiles of *all* instances, so a launch moves the whole
+// batch (hundreds of MB) and the per-launch costs that dominate a single 701^2
+// terrain are amortised: this is the HBM-bound form of the substep.
+//
+// Per-instance loop state follows the single-terrain design: launch k reads
+// ctl[k & 1][i] and the instance's first tile writes ctl[(k+1) & 1][i]; the
+// instance's CFL maximum is a spread maximum over 16 slots of the instance
+// (tile g uses slot g mod 16) in one of four rotating sets (launch k reads set
+// k % 4, fills k+1, clears k+2).  Launch indices enter only as k mod 8, so the
+// host can capture eight substep launches once into a HIP graph and replay it.
+//
+// Tile shapes are a template parameter (chosen by the host-side autotuner).
+// Every face is computed from the same two cells with the same instructions
+// whatever the tile shape, so fields are bit-identical across shapes and to
+// the single-terrain step of each instance.
+#include "mv2_internal.hpp"
+#include "mv2_tile32.hpp"
+
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+namespace mv2 {
+
+constexpr int kInstSlots = 16;          // spread slots per instance
+constexpr int kInstStride = 16;         // u64 words between slots (128 B)
+constexpr int kInstWords = kInstSlots * kInstStride;
+constexpr int kWaveSetsB = 4;           // rotating CFL sets
+constexpr int kBLedger = 3;             // px, py, kinetic (doubles) per tile and pass
+
+struct BControl {
+    double remaining;
+    double max_cfl;
+    double last_remaining;
+    int substeps, src;
+    int last_substeps, last_src;
+    int done, numeric, cfl_limited, static_exit;
+    int rolled, started;
+};
+
+struct BatchArgs {
+    int count, tiles;
+    double dx, dy;
+    const float *b;
+    int32_t *h[2];
+    float *qx[2];
+    float *qy[2];
+    const int32_t *inst_ny, *inst_nx;
+    const int64_t *inst_off;
+    const int32_t *inst_first_tile, *inst_tiles;
+    const int32_t *tile_inst, *tile_r0, *tile_c0;
+    BControl *ctl;                      // [2][count]
+    int *flags;                         // [2][count]
+    unsigned long long *wave;           // [4][count][kInstWords]
+    unsigned long long *stat;           // [2][count][kInstWords]: max |grad H|, max |q|
+    double *ledger;                     // [2 passes][kBLedger][tiles]
+    long long *quanta;                  // [2 passes][tiles]
+    mv2_batch_record *rec;              // [count]
+    mv2_step_config cfg;
+    int clamp_quarter;
+};
+
+// the arguments of one instance in the shape the tile functions expect
+struct InstView {
+    int ny, nx;
+    double dx, dy;
+    const float *b;
+    int32_t *h[2];
+    float *qx[2];
+    float *qy[2];
+    mv2_step_config cfg;                // a copy: no address of the kernel arguments is taken
+    int clamp_quarter;
+    __device__ InstView(const BatchArgs &a, int i)
+        : ny(a.inst_ny[i]), nx(a.inst_nx[i]), dx(a.dx), dy(a.dy), cfg(a.cfg), clamp_quarter(a.clamp_quarter)
+    {
+        const int64_t off = a.inst_off[i];
+        b = a.b + off;
+        for (int s = 0; s < 2; s++) {
+            h[s] = a.h[s] + off;
+            qx[s] = a.qx[s] + off;
+            qy[s] = a.qy[s] + off;
+        }
+    }
+};
+
+__device__ inline unsigned long long *inst_set(unsigned long long *base, int set, int count, int i)
+{
+    return base + ((int64_t)set * count + i) * kInstWords;
+}
+
+__device__ inline void inst_max(unsigned long long *slots, double v)
+{
+    atomic_max_nonneg(&slots[(blockIdx.x % kInstSlots) * kInstStride], v);
+}
+
+// fold an instance's 16 slots: every thread of the block receives the value
+__device__ inline double inst_read_max(const unsigned long long *slots, double *smem1)
+{
+    if (threadIdx.x < kWave) {
+        double v = threadIdx.x < kInstSlots ? __longlong_as_double((long long)slots[threadIdx.x * kInstStride]) : 0.0;
+        v = wave_max(v);
+        if (threadIdx.x == 0) *smem1 = v;
+    }
+    __syncthreads();
+    const double r = *smem1;
+    __syncthreads();
+    return r;
+}
+
+__device__ inline BControl resolve_bcontrol(const BatchArgs &a, int k, int i)
+{
+    BControl C = a.ctl[(k & 1) * a.count + i];
+    const int f = a.flags[(k & 1) * a.count + i];
+    if (f && !C.rolled) {
+        C.numeric = (f & 2) ? 3 : 1;
+        C.src = C.last_src;
+        C.remaining = C.last_remaining;
+        C.substeps = C.last_substeps;
+        C.done = 1;
+        C.rolled = 1;
+    }
+    return C;
+}
+
+__global__ void batch_reset_kernel(BatchArgs a)
+{
+    const int64_t tid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t nth = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i = tid; i < 2 * (int64_t)a.count; i += nth) {
+        BControl c;
+        memset(&c, 0, sizeof(c));
+        c.remaining = a.cfg.dt;
+        a.ctl[i] = c;
+        a.flags[i] = 0;
+    }
+    for (int64_t i = tid; i < (int64_t)kWaveSetsB * a.count * kInstSlots; i += nth) a.wave[i * kInstStride] = 0ull;
+    for (int64_t i = tid; i < 2 * (int64_t)a.count * kInstSlots; i += nth) a.stat[i * kInstStride] = 0ull;
+}
+
+// one tile: ledger before, static diagnostic, CFL maximum of the input
+template <int TH, int TW>
+__global__ void __launch_bounds__(kBlock) batch_begin_kernel(BatchArgs a)
+{
+    __shared__ double smem[64 * 4];
+    const int g = blockIdx.x;
+    const int i = a.tile_inst[g];
+    const InstView v(a, i);
+    const int r0 = a.tile_r0[g], c0 = a.tile_c0[g];
+    const int ny = v.ny, nx = v.nx;
+    const double K = a.cfg.earth_pressure, gr = a.cfg.gravity, dry = a.cfg.dry_tolerance;
+    double s[kBLedger] = {0.0, 0.0, 0.0};
+    long long q[1] = {0};
+    double m[3] = {0.0, 0.0, 0.0};                // max |grad H|, max |q|, max wave speed
+#pragma unroll
+    for (int j = 0; j < TH * TW / kBlock; j++) {
+        const int e = threadIdx.x + j * kBlock;
+        const int r = r0 + e / TW, c = c0 + e % TW;
+        if (r >= ny || c >= nx) continue;
+        const int idx = r * nx + c;
+        const int32_t hq = v.h[0][idx];
+        const float d = (float)hq * (float)kQuantum;
+        const float qxv = v.qx[0][idx], qyv = v.qy[0][idx];
+        q[0] += hq;
+        const double depth = (double)d;
+        s[0] += (double)qxv;
+        s[1] += (double)qyv;
+        float vx = 0.0f, vy = 0.0f;
+        if (d > (float)dry) {
+            const float inv = rcpf(d);
+            vx = qxv * inv;
+            vy = qyv * inv;
+            const double dvx = (double)qxv / depth, dvy = (double)qyv / depth;
+            s[2] += 0.5 * depth * (dvx * dvx + dvy * dvy);
+        }
+        m[2] = npmax(m[2], (double)(__builtin_sqrtf(vx * vx + vy * vy) + __builtin_sqrtf((float)(K * gr) * d)));
+        if (a.cfg.static_exit) {
+            const int cl = c > 0 ? idx - 1 : idx, cr = c + 1 < nx ? idx + 1 : idx;
+            const int rd = r > 0 ? idx - nx : idx, ru = r + 1 < ny ? idx + nx : idx;
+            auto H = [&](int t) { return (double)v.b[t] + (double)((float)v.h[0][t] * (float)kQuantum); };
+            const double denx = (c > 0 && c + 1 < nx) ? 2.0 * a.dx : a.dx;
+            const double deny = (r > 0 && r + 1 < ny) ? 2.0 * a.dy : a.dy;
+            const double gx = (H(cr) - H(cl)) / denx, gy = (H(ru) - H(rd)) / deny;
+            m[0] = npmax(m[0], sqrt(gx * gx + gy * gy));
+            m[1] = npmax(m[1], fabs((double)qxv));
+            m[1] = npmax(m[1], fabs((double)qyv));
+        }
+    }
+    block_reduce<double, kBLedger, 0>(s, smem);
+    block_reduce<long long, 1, 0>(q, reinterpret_cast<long long *>(smem));
+    block_reduce<double, 3, 1>(m, smem);
+    if (threadIdx.x == 0) {
+        for (int f = 0; f < kBLedger; f++) a.ledger[(int64_t)f * a.tiles + g] = s[f];
+        a.quanta[g] = q[0];
+        inst_max(inst_set(a.wave, 0, a.count, i), m[2]);
+        if (a.cfg.static_exit) {
+            inst_max(inst_set(a.stat, 0, a.count, i), m[0]);
+            inst_max(inst_set(a.stat, 1, a.count, i), m[1]);
+        }
+    }
+}
+
+// One substep of every instance whose loop has not ended (k is used mod 8).
+template <int TH, int TW>
+__global__ void __launch_bounds__(kBlock) batch_substep_kernel(BatchArgs a, int k)
+{
+    __shared__ double smem[64 * 4];
+    __shared__ TileT<TH, TW> tile;
+    const int g = blockIdx.x;
+    const int i = a.tile_inst[g];
+    const bool lead = g == a.inst_first_tile[i] && threadIdx.x == 0;
+    const mv2_step_config &cfg = a.cfg;
+    BControl C = resolve_bcontrol(a, k, i);
+    if (!C.done && !C.started) {
+        C.started = 1;
+        if (cfg.static_exit) {
+            const double ext_max = npmax(0.0, npmax(fabs(cfg.external_ax), fabs(cfg.external_ay)));
+            const double qmax = inst_read_max(inst_set(a.stat, 1, a.count, i), smem);
+            const double gmax = inst_read_max(inst_set(a.stat, 0, a.count, i), smem);
+            if (ext_max <= 1.0e-14 && qmax <= 1.0e-14 && gmax <= cfg.tan_start + 1.0e-14) {
+                C.static_exit = 1;
+                C.done = 1;
+            }
+        }
+    }
+    if (!C.done && !(C.remaining > 1.0e-14)) C.done = 1;
+    if (!C.done && cfg.limit_mode == 0 && C.substeps >= cfg.maximum_substeps) {
+        C.numeric = 2;
+        C.done = 1;
+    }
+    if (C.done) {
+        if (lead) a.ctl[((k + 1) & 1) * a.count + i] = C;
+        return;
+    }
+    const InstView v(a, i);
+    const int src = C.src;
+    const int r0 = a.tile_r0[g], c0 = a.tile_c0[g];
+    tile_stage<TH, TW>(v, tile, src, r0, c0);
+    const double maxw = inst_read_max(inst_set(a.wave, k % kWaveSetsB, a.count, i), smem);
+    const double mind = a.dx <= a.dy ? a.dx : a.dy;
+    const double stable = maxw <= 1.0e-14 ? C.remaining : cfg.cfl * mind / maxw;
+    const double sub_dt = npmin(C.remaining, stable);
+    if (g == a.inst_first_tile[i] && threadIdx.x < kInstSlots)
+        inst_set(a.wave, (k + 2) % kWaveSetsB, a.count, i)[threadIdx.x * kInstStride] = 0ull;
+    if (lead) {
+        BControl nc = C;
+        nc.last_remaining = C.remaining;
+        nc.last_substeps = C.substeps;
+        nc.last_src = C.src;
+        nc.max_cfl = npmax(C.max_cfl, maxw * sub_dt / mind);
+        if (sub_dt < C.remaining - 1.0e-14) nc.cfl_limited = 1;
+        nc.remaining = C.remaining - sub_dt;
+        nc.substeps = C.substeps + 1;
+        nc.src = C.src ^ 1;
+        if (cfg.limit_mode == 1 && nc.substeps > cfg.maximum_substeps) {
+            nc.numeric = 2;
+            nc.done = 1;
+        } else if (!(nc.remaining > 1.0e-14)) {
+            nc.done = 1;
+        } else if (cfg.limit_mode == 0 && nc.substeps >= cfg.maximum_substeps) {
+            nc.numeric = 2;
+            nc.done = 1;
+        }
+        a.ctl[((k + 1) & 1) * a.count + i] = nc;
+    }
+    double diag[kDiag];
+    for (int d = 0; d < kDiag; d++) diag[d] = 0.0;
+    double wmax[1] = {0.0};
+    double hmin = 0.0;
+    int ovf = 0;
+    tile_update<TH, TW>(v, tile, src, r0, c0, sub_dt, diag, wmax[0], hmin, ovf);
+    block_reduce<double, 1, 1>(wmax, smem);
+    double hm[1] = {hmin};
+    block_reduce<double, 1, 2>(hm, smem);
+    const int anyovf = __syncthreads_or(ovf);
+    if (threadIdx.x == 0) {
+        inst_max(inst_set(a.wave, (k + 1) % kWaveSetsB, a.count, i), wmax[0]);
+        int f = 0;
+        if (hm[0] < -1.0e-300) f |= 1;
+        if (anyovf) f |= 2;
+        if (f) atomicOr(&a.flags[((k + 1) & 1) * a.count + i], f);
+    }
+}
+
+// copy back when an instance's final state sits in scratch; ledger after
+template <int TH, int TW>
+__global__ void __launch_bounds__(kBlock) batch_end_kernel(BatchArgs a, int k)
+{
+    __shared__ double smem[64 * 4];
+    const int g = blockIdx.x;
+    const int i = a.tile_inst[g];
+    const BControl C = resolve_bcontrol(a, k, i);
+    const InstView v(a, i);
+    const int r0 = a.tile_r0[g], c0 = a.tile_c0[g];
+    const int ny = v.ny, nx = v.nx;
+    const double dry = a.cfg.dry_tolerance;
+    const int fsrc = C.src;
+    double s[kBLedger] = {0.0, 0.0, 0.0};
+    long long q[1] = {0};
+#pragma unroll
+    for (int j = 0; j < TH * TW / kBlock; j++) {
+        const int e = threadIdx.x + j * kBlock;
+        const int r = r0 + e / TW, c = c0 + e % TW;
+        if (r >= ny || c >= nx) continue;
+        const int idx = r * nx + c;
+        const int32_t hq = (fsrc ? v.h[1] : v.h[0])[idx];
+        const float qxv = (fsrc ? v.qx[1] : v.qx[0])[idx], qyv = (fsrc ? v.qy[1] : v.qy[0])[idx];
+        if (fsrc == 1) {
+            v.h[0][idx] = hq;
+            v.qx[0][idx] = qxv;
+            v.qy[0][idx] = qyv;
+        }
+        q[0] += hq;
+        const double depth = (double)((float)hq * (float)kQuantum);
+        s[0] += (double)qxv;
+        s[1] += (double)qyv;
+        if (depth > dry) {
+            const double dvx = (double)qxv / depth, dvy = (double)qyv / depth;
+            s[2] += 0.5 * depth * (dvx * dvx + dvy * dvy);
+        }
+    }
+    block_reduce<double, kBLedger, 0>(s, smem);
+    block_reduce<long long, 1, 0>(q, reinterpret_cast<long long *>(smem));
+    if (threadIdx.x == 0) {
+        for (int f = 0; f < kBLedger; f++) a.ledger[(int64_t)(kBLedger + f) * a.tiles + g] = s[f];
+        a.quanta[a.tiles + g] = q[0];
+    }
+}
+
+// one workgroup per instance: fold its tiles' partials in fixed order
+__global__ void __launch_bounds__(kBlock) batch_finalize_kernel(BatchArgs a, int k)
+{
+    __shared__ double smem[64 * 4];
+    const int i = blockIdx.x;
+    const BControl C = resolve_bcontrol(a, k, i);
+    const int t0 = a.inst_first_tile[i], nt = a.inst_tiles[i];
+    double s[2 * kBLedger];
+    for (int f = 0; f < 2 * kBLedger; f++) s[f] = 0.0;
+    long long q[2] = {0, 0};
+    for (int t = threadIdx.x; t < nt; t += blockDim.x) {
+        for (int f = 0; f < 2 * kBLedger; f++) s[f] += a.ledger[(int64_t)f * a.tiles + t0 + t];
+        q[0] += a.quanta[t0 + t];
+        q[1] += a.quanta[a.tiles + t0 + t];
+    }
+    block_reduce<double, 2 * kBLedger, 0>(s, smem);
+    block_reduce<long long, 2, 0>(q, reinterpret_cast<long long *>(smem));
+    if (threadIdx.x == 0) {
+        const double area = a.dx * a.dy;
+        const bool after = !C.static_exit;
+        mv2_batch_record r;
+        r.substeps = C.substeps;
+        r.numeric_error = C.numeric;
+        r.static_exit = C.static_exit;
+        r.complete = C.done;
+        r.maximum_cfl = C.max_cfl;
+        r.remaining = C.remaining;
+        r.mass_quanta_before = (double)q[0];
+        r.mass_quanta_after = after ? (double)q[1] : (double)q[0];
+        r.momentum_before[0] = s[0] * area;
+        r.momentum_before[1] = s[1] * area;
+        r.momentum_after[0] = (after ? s[kBLedger + 0] : s[0]) * area;
+        r.momentum_after[1] = (after ? s[kBLedger + 1] : s[1]) * area;
+        r.kinetic_before = s[2] * area;
+        r.kinetic_after = (after ? s[kBLedger + 2] : s[2]) * area;
+        a.rec[i] = r;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// host side
+// ---------------------------------------------------------------------------
+struct TileShape {
+    int th, tw;
+};
+constexpr TileShape kShapes[] = {{16, 32}, {8, 64}, {32, 16}, {16, 64}};
+constexpr int kShapeCount = 4;
+
+template <int TH, int TW>
+struct ShapeKernels {
+    static void begin(dim3 g, hipStream_t s, const BatchArgs &a)
+    {
+        batch_begin_kernel<TH, TW><<<g, dim3(kBlock), 0, s>>>(a);
+    }
+    static void substep(dim3 g, hipStream_t s, const BatchArgs &a, int k)
+    {
+        batch_substep_kernel<TH, TW><<<g, dim3(kBlock), 0, s>>>(a, k);
+    }
+    static void end(dim3 g, hipStream_t s, const BatchArgs &a, int k)
+    {
+        batch_end_kernel<TH, TW><<<g, dim3(kBlock), 0, s>>>(a, k);
+    }
+};
+
+static void launch_begin(int shape, dim3 g, hipStream_t s, const BatchArgs &a)
+{
+    switch (shape) {
+    case 0: ShapeKernels<16, 32>::begin(g, s, a); break;
+    case 1: ShapeKernels<8, 64>::begin(g, s, a); break;
+    case 2: ShapeKernels<32, 16>::begin(g, s, a); break;
+    default: ShapeKernels<16, 64>::begin(g, s, a); break;
+    }
+}
+static void launch_substep(int shape, dim3 g, hipStream_t s, const BatchArgs &a, int k)
+{
+    switch (shape) {
+    case 0: ShapeKernels<16, 32>::substep(g, s, a, k); break;
+    case 1: ShapeKernels<8, 64>::substep(g, s, a, k); break;
+    case 2: ShapeKernels<32, 16>::substep(g, s, a, k); break;
+    default: ShapeKernels<16, 64>::substep(g, s, a, k); break;
+    }
+}
+static void launch_end(int shape, dim3 g, hipStream_t s, const BatchArgs &a, int k)
+{
+    switch (shape) {
+    case 0: ShapeKernels<16, 32>::end(g, s, a, k); break;
+    case 1: ShapeKernels<8, 64>::end(g, s, a, k); break;
+    case 2: ShapeKernels<32, 16>::end(g, s, a, k); break;
+    default: ShapeKernels<16, 64>::end(g, s, a, k); break;
+    }
+}
+
+}  // namespace mv2
+
+using namespace mv2;
+
+// batch state lives in the context (one batch per context)
+struct mv2_batch_state {
+    int count = 0;
+    int64_t cells = 0;
+    double dx = 0.0, dy = 0.0;
+    float *b = nullptr;
+    int32_t *h = nullptr;
+    float *qx = nullptr, *qy = nullptr;
+    std::vector<int32_t> ny, nx;
+    std::vector<int64_t> off;
+    int shape = 0;
+    int tiles = 0;
+    Scratch inst;                        // ny, nx, off, first tile, tile count
+    Scratch tiles_tab;                   // tile -> instance, r0, c0
+    Scratch alt_h, alt_qx, alt_qy;
+    Scratch ctl, wave, stat, ledger, rec;
+    int predicted = 0;
+    // graph of eight substep launches
+    hipGraphExec_t graph = nullptr;
+    hipStream_t capture = nullptr;
+    std::vector<unsigned char> graph_key;
+};
+
+static mv2_batch_state *batch_of(mv2_context *ctx)
+{
+    if (!ctx->batch) ctx->batch = new mv2_batch_state();
+    return ctx->batch;
+}
+
+void mv2_batch_release(mv2_context *ctx)
+{
+    mv2_batch_state *bs = ctx->batch;
+    if (!bs) return;
+    if (bs->graph) hipGraphExecDestroy(bs->graph);
+    if (bs->capture) hipStreamDestroy(bs->capture);
+    for (Scratch *s : {&bs->inst, &bs->tiles_tab, &bs->alt_h, &bs->alt_qx, &bs->alt_qy, &bs->ctl, &bs->wave,
+                       &bs->stat, &bs->ledger, &bs->rec})
+        if (s->ptr) hipFree(s->ptr);
+    delete bs;
+    ctx->batch = nullptr;
+}
+
+static int build_tiles(mv2_context *ctx, mv2_batch_state *bs, int shape)
+{
+    const int th = kShapes[shape].th, tw = kShapes[shape].tw;
+    std::vector<int32_t> first(bs->count), ntiles(bs->count), ti, tr, tc;
+    for (int i = 0; i < bs->count; i++) {
+        first[i] = (int32_t)ti.size();
+        const int tx = (bs->nx[i] + tw - 1) / tw, ty = (bs->ny[i] + th - 1) / th;
+        // XCD-aware order inside an instance is unnecessary here: consecutive
+        // tiles of one instance land on different XCDs, and the batch is large
+        for (int r = 0; r < ty; r++)
+            for (int c = 0; c < tx; c++) {
+                ti.push_back(i);
+                tr.push_back(r * th);
+                tc.push_back(c * tw);
+            }
+        ntiles[i] = ty * tx;
+    }
+    const int T = (int)ti.size();
+    if (int rc = ensure_scratch(ctx, bs->tiles_tab, sizeof(int32_t) * 3 * (size_t)T)) return rc;
+    int32_t *tab = (int32_t *)bs->tiles_tab.ptr;
+    MV2_HIP(ctx, hipMemcpy(tab, ti.data(), sizeof(int32_t) * T, hipMemcpyHostToDevice));
+    MV2_HIP(ctx, hipMemcpy(tab + T, tr.data(), sizeof(int32_t) * T, hipMemcpyHostToDevice));
+    MV2_HIP(ctx, hipMemcpy(tab + 2 * (size_t)T, tc.data(), sizeof(int32_t) * T, hipMemcpyHostToDevice));
+    int32_t *inst = (int32_t *)bs->inst.ptr;
+    MV2_HIP(ctx, hipMemcpy(inst + 2 * (size_t)bs->count + 2 * (size_t)bs->count, first.data(),
+                           sizeof(int32_t) * bs->count, hipMemcpyHostToDevice));
+    MV2_HIP(ctx, hipMemcpy(inst + 5 * (size_t)bs->count, ntiles.data(), sizeof(int32_t) * bs->count,
+                           hipMemcpyHostToDevice));
+    bs->tiles = T;
+    bs->shape = shape;
+    if (int rc = ensure_scratch(ctx, bs->ledger, (sizeof(double) * 2 * kBLedger + sizeof(long long) * 2) * (size_t)T))
+        return rc;
+    if (bs->graph) {
+        hipGraphExecDestroy(bs->graph);
+        bs->graph = nullptr;
+    }
+    return MV2_OK;
+}
+
+static BatchArgs batch_args(mv2_batch_state *bs, const mv2_step_config *cfg)
+{
+    BatchArgs a;
+    a.count = bs->count;
+    a.tiles = bs->tiles;
+    a.dx = bs->dx;
+    a.dy = bs->dy;
+    a.b = bs->b;
+    a.h[0] = bs->h;
+    a.h[1] = (int32_t *)bs->alt_h.ptr;
+    a.qx[0] = bs->qx;
+    a.qx[1] = (float *)bs->alt_qx.ptr;
+    a.qy[0] = bs->qy;
+    a.qy[1] = (float *)bs->alt_qy.ptr;
+    int32_t *inst = (int32_t *)bs->inst.ptr;
+    const size_t n = (size_t)bs->count;
+    a.inst_ny = inst;
+    a.inst_nx = inst + n;
+    a.inst_off = (const int64_t *)(inst + 2 * n);      // 2n int32 words
+    a.inst_first_tile = inst + 4 * n;
+    a.inst_tiles = inst + 5 * n;
+    const int32_t *tab = (const int32_t *)bs->tiles_tab.ptr;
+    a.tile_inst = tab;
+    a.tile_r0 = tab + bs->tiles;
+    a.tile_c0 = tab + 2 * (size_t)bs->tiles;
+    a.ctl = (BControl *)bs->ctl.ptr;
+    a.flags = (int *)((char *)bs->ctl.ptr + sizeof(BControl) * 2 * n);
+    a.wave = (unsigned long long *)bs->wave.ptr;
+    a.stat = (unsigned long long *)bs->stat.ptr;
+    a.ledger = (double *)bs->ledger.ptr;
+    a.quanta = (long long *)(a.ledger + (size_t)2 * kBLedger * bs->tiles);
+    a.rec = (mv2_batch_record *)bs->rec.ptr;
+    a.cfg = *cfg;
+    a.clamp_quarter = cfg->cfl <= 0.25 ? 1 : 0;
+    return a;
+}
+
+extern "C" int mv2_batch_bind(mv2_context *ctx, int count, const int32_t *ny, const int32_t *nx,
+                              const int64_t *offset, double dx, double dy, float *b, int32_t *h_quanta, float *qx,
+                              float *qy)
+{
+    if (!ctx) return set_error(nullptr, MV2_EINVAL, "[BatchedMobileV2] null context");
+    if (count < 1 || !ny || !nx || !offset || !b || !h_quanta || !qx || !qy)
+        return set_error(ctx, MV2_EINVAL, "[BatchedMobileV2] invalid batch description");
+    if (!(dx > 0.0) || !(dy > 0.0)) return set_error(ctx, MV2_EINVAL, "[BatchedMobileV2] dx, dy must be positive");
+    hipSetDevice(ctx->device);
+    int64_t cells = 0;
+    for (int i = 0; i < count; i++) {
+        if (ny[i] < 2 || nx[i] < 2 || (int64_t)ny[i] * nx[i] >= ((int64_t)1 << 31))
+            return set_error(ctx, MV2_EINVAL, "[BatchedMobileV2] instance %d has an invalid size %d x %d", i, ny[i], nx[i]);
+        if (offset[i] < cells)
+            return set_error(ctx, MV2_EINVAL, "[BatchedMobileV2] instance %d overlaps the previous one", i);
+        cells = offset[i] + (int64_t)ny[i] * nx[i];
+    }
+    mv2_batch_state *bs = batch_of(ctx);
+    bs->count = count;
+    bs->cells = cells;
+    bs->dx = dx;
+    bs->dy = dy;
+    bs->b = b;
+    bs->h = h_quanta;
+    bs->qx = qx;
+    bs->qy = qy;
+    bs->ny.assign(ny, ny + count);
+    bs->nx.assign(nx, nx + count);
+    bs->off.assign(offset, offset + count);
+    bs->predicted = 0;
+    if (int rc = ensure_scratch(ctx, bs->inst, sizeof(int32_t) * 6 * (size_t)count)) return rc;
+    int32_t *inst = (int32_t *)bs->inst.ptr;
+    MV2_HIP(ctx, hipMemcpy(inst, ny, sizeof(int32_t) * count, hipMemcpyHostToDevice));
+    MV2_HIP(ctx, hipMemcpy(inst + count, nx, sizeof(int32_t) * count, hipMemcpyHostToDevice));
+    MV2_HIP(ctx, hipMemcpy(inst + 2 * (size_t)count, offset, sizeof(int64_t) * count, hipMemcpyHostToDevice));
+    if (int rc = ensure_scratch(ctx, bs->alt_h, sizeof(int32_t) * (size_t)cells)) return rc;
+    if (int rc = ensure_scratch(ctx, bs->alt_qx, sizeof(float) * (size_t)cells)) return rc;
+    if (int rc = ensure_scratch(ctx, bs->alt_qy, sizeof(float) * (size_t)cells)) return rc;
+    if (int rc = ensure_scratch(ctx, bs->ctl, (sizeof(BControl) + sizeof(int)) * 2 * (size_t)count)) return rc;
+    if (int rc = ensure_scratch(ctx, bs->wave, sizeof(unsigned long long) * kWaveSetsB * kInstWords * (size_t)count))
+        return rc;
+    if (int rc = ensure_scratch(ctx, bs->stat, sizeof(unsigned long long) * 2 * kInstWords * (size_t)count)) return rc;
+    if (int rc = ensure_scratch(ctx, bs->rec, sizeof(mv2_batch_record) * (size_t)count)) return rc;
+    return build_tiles(ctx, bs, bs->shape);
+}
+
+extern "C" int mv2_batch_set_tile_shape(mv2_context *ctx, int shape)
+{
+    if (!ctx || !ctx->batch || !ctx->batch->count)
+        return set_error(ctx, MV2_ESTATE, "[BatchedMobileV2] bind a batch first");
+    if (shape < 0 || shape >= kShapeCount) return set_error(ctx, MV2_EINVAL, "[BatchedMobileV2] unknown tile shape %d", shape);
+    hipSetDevice(ctx->device);
+    if (shape == ctx->batch->shape && ctx->batch->tiles) return MV2_OK;
+    return build_tiles(ctx, ctx->batch, shape);
+}
+
+extern "C" int mv2_batch_tile_shape(int shape, int *th, int *tw)
+{
+    if (shape < 0 || shape >= kShapeCount || !th || !tw) return MV2_EINVAL;
+    *th = kShapes[shape].th;
+    *tw = kShapes[shape].tw;
+    return MV2_OK;
+}
+
+// capture eight substep launches (k = 0..7) into a graph for this config
+static int ensure_graph(mv2_context *ctx, mv2_batch_state *bs, const BatchArgs &a)
+{
+    std::vector<unsigned char> key(sizeof(BatchArgs) + sizeof(int));
+    memcpy(key.data(), &a, sizeof(BatchArgs));
+    memcpy(key.data() + sizeof(BatchArgs), &bs->shape, sizeof(int));
+    if (bs->graph && key == bs->graph_key) return MV2_OK;
+    if (bs->graph) {
+        hipGraphExecDestroy(bs->graph);
+        bs->graph = nullptr;
+    }
+    if (!bs->capture) MV2_HIP(ctx, hipStreamCreateWithFlags(&bs->capture, hipStreamNonBlocking));
+    MV2_HIP(ctx, hipStreamBeginCapture(bs->capture, hipStreamCaptureModeThreadLocal));
+    for (int k = 0; k < 8; k++) launch_substep(bs->shape, dim3(bs->tiles), bs->capture, a, k);
+    hipGraph_t graph = nullptr;
+    MV2_HIP(ctx, hipStreamEndCapture(bs->capture, &graph));
+    hipError_t e = hipGraphInstantiate(&bs->graph, graph, nullptr, nullptr, 0);
+    hipGraphDestroy(graph);
+    if (e != hipSuccess) {
+        bs->graph = nullptr;
+        return check_hip(ctx, e, "hipGraphInstantiate");
+    }
+    bs->graph_key = key;
+    return MV2_OK;
+}
+
+extern "C" int mv2_batch_step(mv2_context *ctx, const mv2_step_config *cfg, int use_graph, mv2_batch_record *out,
+                              double *kernel_ms)
+{
+    if (!ctx) return set_error(nullptr, MV2_EINVAL, "[BatchedMobileV2] null context");
+    mv2_batch_state *bs = ctx->batch;
+    if (!bs || !bs->count) return set_error(ctx, MV2_ESTATE, "[BatchedMobileV2] bind a batch first");
+    if (!cfg || !out) return set_error(ctx, MV2_EINVAL, "[BatchedMobileV2] null config/record");
+    if (!(cfg->dt > 0.0) || !std::isfinite(cfg->dt)) return set_error(ctx, MV2_EINVAL, "dt must be finite/positive");
+    if (!(cfg->cfl > 0.0) || !(cfg->earth_pressure > 0.0) || !(cfg->gravity > 0.0) || !(cfg->dry_tolerance > 0.0) ||
+        !(cfg->basal_friction > 0.0))
+        return set_error(ctx, MV2_EINVAL, "MobileV2 configuration must be finite/positive");
+    hipSetDevice(ctx->device);
+    hipStream_t st = ctx->stream;
+    const BatchArgs a = batch_args(bs, cfg);
+    if (use_graph)
+        if (int rc = ensure_graph(ctx, bs, a)) return rc;
+    const dim3 grid(bs->tiles);
+    MV2_HIP(ctx, hipEventRecord(ctx->ev_start, st));
+    hipLaunchKernelGGL(batch_reset_kernel, dim3(grid_for(2 * (int64_t)bs->count * kInstWords, kBlock, 256)),
+                       dim3(kBlock), 0, st, a);
+    launch_begin(bs->shape, grid, st, a);
+    MV2_HIP(ctx, hipGetLastError());
+    int k = 0;
+    int want = bs->predicted > 0 ? bs->predicted + 1 : 8;
+    const size_t rec_bytes = sizeof(mv2_batch_record) * (size_t)bs->count;
+    if (int rc = ensure_scratch(ctx, ctx->host_result, rec_bytes + 4096, true)) return rc;
+    const mv2_batch_record *hrec = (const mv2_batch_record *)ctx->host_result.ptr;
+    for (;;) {
+        if (use_graph) {
+            const int reps = (want + 7) / 8;
+            for (int r = 0; r < reps; r++, k += 8) MV2_HIP(ctx, hipGraphLaunch(bs->graph, st));
+        } else {
+            for (int j = 0; j < want; j++, k++) launch_substep(bs->shape, grid, st, a, k % 8);
+        }
+        launch_end(bs->shape, grid, st, a, k % 8);
+        hipLaunchKernelGGL(batch_finalize_kernel, dim3(bs->count), dim3(kBlock), 0, st, a, k % 8);
+        MV2_HIP(ctx, hipGetLastError());
+        MV2_HIP(ctx, hipEventRecord(ctx->ev_stop, st));
+        MV2_HIP(ctx, hipMemcpyAsync(ctx->host_result.ptr, a.rec, rec_bytes, hipMemcpyDeviceToHost, st));
+        MV2_HIP(ctx, hipStreamSynchronize(st));
+        bool all = true;
+        int most = 0;
+        for (int i = 0; i < bs->count; i++) {
+            all = all && hrec[i].complete;
+            most = hrec[i].substeps > most ? hrec[i].substeps : most;
+        }
+        if (all) {
+            bs->predicted = most;
+            break;
+        }
+        // more substeps than enqueued: the states so far are valid, continue
+        want = want * 2 > 8 ? want * 2 : 8;
+        (void)most;
+    }
+    memcpy(out, hrec, rec_bytes);
+    if (kernel_ms) {
+        float ms = 0.0f;
+        MV2_HIP(ctx, hipEventElapsedTime(&ms, ctx->ev_start, ctx->ev_stop));
+        *kernel_ms = ms;
+    }
+    for (int i = 0; i < bs->count; i++)
+        if (out[i].numeric_error)
+            return set_error(ctx, MV2_ENUMERIC, "[BatchedMobileV2] instance %d: %s", i,
+                             out[i].numeric_error == 1 ? "MobileV2 positivity failure"
+                             : out[i].numeric_error == 2 ? "MobileV2 maximum_substeps exceeded"
+                                                         : "mobile thickness exceeds the FP32 storage range");
+    return MV2_OK;
+}

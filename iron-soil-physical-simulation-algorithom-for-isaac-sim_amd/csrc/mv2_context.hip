@@ -121,6 +121,7 @@ extern "C" void mv2_destroy(mv2_context *ctx)
     release(ctx->label_aux);
     release(ctx->nn_partials);
     release(ctx->nn_loss);
+    mv2_batch_release(ctx);
     release(ctx->step_diag);
     release(ctx->spread);
     release(ctx->host_result, true);

@@ -28,6 +28,8 @@ struct Scratch {
 
 }  // namespace mv2
 
+struct mv2_batch_state;
+
 struct mv2_context {
     int device = 0;
     hipStream_t stream = nullptr;
@@ -57,7 +59,10 @@ struct mv2_context {
     std::vector<hipEvent_t> sub_events;   // event pairs of the substep launches
     int v2_predicted = 0;                 // substep launches to enqueue up front
     int relax_predicted = 0;              // relaxation launches to enqueue up front
+    mv2_batch_state *batch = nullptr;     // batched terrains bound to this context
 };
+
+void mv2_batch_release(mv2_context *ctx);
 
 namespace mv2 {
 

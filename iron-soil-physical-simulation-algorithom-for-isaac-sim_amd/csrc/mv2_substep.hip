@@ -28,6 +28,7 @@
 //              quanta (never active in exact arithmetic for CFL <= 0.25), so
 //              thickness stays non-negative without a mass-creating clamp.
 #include "mv2_internal.hpp"
+#include "mv2_tile32.hpp"
 
 #include <cmath>
 
@@ -66,7 +67,6 @@ constexpr int kStepSpreadSets = 5;
 
 // per-workgroup partial ledger of the begin / end passes
 constexpr int kSum = 8;                  // summary entries
-constexpr int kDiag = 10;
 constexpr int kClasses = 9;              // weight classes of the exact FP32 mass ledger
 // Partials are stored field-major (structure of arrays): field f of
 // workgroup g at [f * groups + g], so the folding pass reads each field of
@@ -107,8 +107,6 @@ struct V2Args {
     int clamp_quarter;                   // FP32: cap outflow per face at h/4
 };
 
-__device__ inline double npmax(double a, double b) { return a >= b ? a : b; }
-__device__ inline double npmin(double a, double b) { return a <= b ? a : b; }
 
 // ---------------------------------------------------------------------------
 // FP64 face: mobile_v2_reference.py:391-431 (+ _physical_flux :433-449).
@@ -179,14 +177,6 @@ __device__ inline Face64 face64(double bl, double br, double hl, double hr,
 // rounded once to whole quanta.  Deterministic: both cells that share the
 // face evaluate it from identical operands with identical instructions.
 // ---------------------------------------------------------------------------
-struct Face32 { int h; float qxl, qyl, qxr, qyr; };
-
-struct Consts32 {
-    float K, invK, Kg, c, dry;
-};
-
-__device__ inline float rcpf(float x) { return __builtin_amdgcn_rcpf(x); }
-
 template <int NORMAL>
 __device__ inline Face32 face32(float bl, float br, int hql, int hqr, float hl, float hr,
                                 float qxl, float qyl, float qxr, float qyr,
@@ -410,249 +400,9 @@ __device__ inline CellOut cell32(const V2Args<F32P> &a, int src, int r, int c, d
     return out;
 }
 
-// ---------------------------------------------------------------------------
-// FP32 tile substep: each shared face is evaluated once per tile from LDS.
-// A 16 x 32 tile stages an (18 x 34) halo block of cells (depth, quanta,
-// momentum, wet-masked velocity), computes its 16 x 33 x-faces and 17 x 32
-// y-faces into LDS, then updates its cells.  Faces on a tile border are
-// computed by both neighbouring tiles from bit-identical operands, so both
-// sides move the same integer number of quanta.
-// ---------------------------------------------------------------------------
+// FP32 tile of the single-terrain step (faces in mv2_tile32.hpp)
 constexpr int kTH = 16, kTW = 32;
-constexpr int kLH = kTH + 2, kLW = kTW + 2;
-constexpr int kFX = kTH * (kTW + 1);
-constexpr int kFY = (kTH + 1) * kTW;
-static_assert(kTH * kTW % kBlock == 0, "tile cells must split evenly over the block");
-
-struct Tile32 {
-    float b[kLH * kLW];
-    float h[kLH * kLW];
-    int hq[kLH * kLW];
-    float qx[kLH * kLW];
-    float qy[kLH * kLW];
-    float vx[kLH * kLW];
-    float vy[kLH * kLW];
-    int fxq[kFX];
-    float fxl[kFX], fxr[kFX], fxt[kFX];
-    int fyq[kFY];
-    float fyl[kFY], fyr[kFY], fyt[kFY];
-};
-
-// face from cell-centred velocities (zero for dry cells); the reconstructed
-// velocity is the cell velocity, zeroed where the reconstruction is dry
-template <int NORMAL>
-__device__ inline Face32 face32v(float bl, float br, int hql, int hqr, float hl, float hr,
-                                 float vxl, float vyl, float vxr, float vyr,
-                                 const Consts32 &k, float factor, int clamp_quarter)
-{
-    float head_l = bl + k.K * hl;
-    float head_r = br + k.K * hr;
-    float b_face = fmaxf(bl, br);
-    float hs_l = fmaxf((head_l - b_face) * k.invK, 0.0f);
-    float hs_r = fmaxf((head_r - b_face) * k.invK, 0.0f);
-    bool wl = hs_l > k.dry, wr = hs_r > k.dry;
-    float vnl = wl ? (NORMAL == 0 ? vxl : vyl) : 0.0f, vtl = wl ? (NORMAL == 0 ? vyl : vxl) : 0.0f;
-    float vnr = wr ? (NORMAL == 0 ? vxr : vyr) : 0.0f, vtr = wr ? (NORMAL == 0 ? vyr : vxr) : 0.0f;
-    float qnl = hs_l * vnl, qtl = hs_l * vtl;
-    float qnr = hs_r * vnr, qtr = hs_r * vtr;
-    float psl = (k.c * hs_l) * hs_l, psr = (k.c * hs_r) * hs_r;
-    float fln = qnl * vnl + psl, frn = qnr * vnr + psr;
-    float flt = qnl * vtl, frt = qnr * vtr;
-    float speed = fmaxf(fabsf(vnl) + __builtin_sqrtf(k.Kg * hs_l),
-                        fabsf(vnr) + __builtin_sqrtf(k.Kg * hs_r));
-    float hsp = 0.5f * speed;
-    float fh = 0.5f * (qnl + qnr) - hsp * (hs_r - hs_l);
-    float fn = 0.5f * (fln + frn) - hsp * (qnr - qnl);
-    float ft = 0.5f * (flt + frt) - hsp * (qtr - qtl);
-    float pl = (k.c * hl) * hl, pr = (k.c * hr) * hr;
-    Face32 o;
-    int q = (int)rintf(fh * factor * (float)kInvQuantum);
-    if (clamp_quarter) {
-        if (q > 0) q = min(q, hql >> 2);
-        else q = max(q, -(hqr >> 2));
-    }
-    o.h = q;
-    float nl = (fn + (pl - psl)) * factor;
-    float nr = (fn + (pr - psr)) * factor;
-    float tt = ft * factor;
-    if (NORMAL == 0) {
-        o.qxl = nl; o.qyl = tt; o.qxr = nr; o.qyr = tt;
-    } else {
-        o.qxl = tt; o.qyl = nl; o.qxr = tt; o.qyr = nr;
-    }
-    return o;
-}
-
-__device__ inline Consts32 consts32(const mv2_step_config &cfg)
-{
-    Consts32 k;
-    k.K = (float)cfg.earth_pressure;
-    k.invK = (float)(1.0 / cfg.earth_pressure);
-    k.Kg = (float)(cfg.earth_pressure * cfg.gravity);
-    k.c = (float)(0.5 * cfg.earth_pressure * cfg.gravity);
-    k.dry = (float)cfg.dry_tolerance;
-    return k;
-}
-
-// Stage tile t (row-major over tiles_x) plus its one-cell halo into LDS.  The
-// loop has a fixed trip count and is unrolled, so every load of the thread is
-// in flight at once (one memory round trip for the whole tile).  The caller
-// synchronises before the LDS is read.
-constexpr int kStageIters = (kLH * kLW + kBlock - 1) / kBlock;
-__device__ inline void tile_stage32(const V2Args<F32P> &a, Tile32 &T, int src, int t, int tiles_x)
-{
-    const int ny = a.ny, nx = a.nx;
-    const int r0 = (t / tiles_x) * kTH, c0 = (t % tiles_x) * kTW;
-    const float Qf = (float)kQuantum;
-    const float dry = (float)a.cfg.dry_tolerance;
-    const float *gb = a.b;
-    const int32_t *gh = a.h[src];
-    const float *gqx = a.qx[src];
-    const float *gqy = a.qy[src];
-    float b[kStageIters], qx[kStageIters], qy[kStageIters];
-    int hq[kStageIters];
-#pragma unroll
-    for (int j = 0; j < kStageIters; j++) {
-        const int e = threadIdx.x + j * kBlock;
-        const int lr = e / kLW, lc = e - lr * kLW;
-        const int gr = r0 - 1 + lr, gc = c0 - 1 + lc;
-        b[j] = 0.0f; qx[j] = 0.0f; qy[j] = 0.0f; hq[j] = 0;
-        if (e < kLH * kLW && gr >= 0 && gr < ny && gc >= 0 && gc < nx) {
-            const int i = gr * nx + gc;
-            b[j] = gb[i];
-            hq[j] = gh[i];
-            qx[j] = gqx[i];
-            qy[j] = gqy[i];
-        }
-    }
-#pragma unroll
-    for (int j = 0; j < kStageIters; j++) {
-        const int e = threadIdx.x + j * kBlock;
-        if (e >= kLH * kLW) continue;
-        const float h = (float)hq[j] * Qf;
-        float vx = 0.0f, vy = 0.0f;
-        if (h > dry) {
-            float inv = rcpf(h);
-            vx = qx[j] * inv;
-            vy = qy[j] * inv;
-        }
-        T.b[e] = b[j]; T.h[e] = h; T.hq[e] = hq[j]; T.qx[e] = qx[j]; T.qy[e] = qy[j]; T.vx[e] = vx; T.vy[e] = vy;
-    }
-}
-
-// faces and cell update of the staged tile t (after a barrier)
-__device__ void tile_update32(const V2Args<F32P> &a, Tile32 &T, int src, int t, int tiles_x,
-                              double sub_dt, Acc<F32P> &acc, double &wmax, double &hmin, int &ovf)
-{
-    const int ny = a.ny, nx = a.nx;
-    const int r0 = (t / tiles_x) * kTH, c0 = (t % tiles_x) * kTW;
-    const Consts32 k = consts32(a.cfg);
-    const float Qf = (float)kQuantum;
-    const float fxf = (float)(sub_dt / a.dx), fyf = (float)(sub_dt / a.dy);
-    const int cq = a.clamp_quarter;
-#pragma unroll
-    for (int j = 0; j < (kFX + kFY + kBlock - 1) / kBlock; j++) {
-        const int e = threadIdx.x + j * kBlock;
-        if (e >= kFX + kFY) continue;
-        if (e < kFX) {
-            const int lr = e / (kTW + 1), kk = e - lr * (kTW + 1);
-            const int gr = r0 + lr, gcl = c0 - 1 + kk;
-            if (gr >= ny || gcl < 0 || gcl + 1 >= nx) continue;
-            const int L = (lr + 1) * kLW + kk, R = L + 1;
-            Face32 f = face32v<0>(T.b[L], T.b[R], T.hq[L], T.hq[R], T.h[L], T.h[R], T.vx[L], T.vy[L],
-                                  T.vx[R], T.vy[R], k, fxf, cq);
-            T.fxq[e] = f.h; T.fxl[e] = f.qxl; T.fxr[e] = f.qxr; T.fxt[e] = f.qyl;
-        } else {
-            const int ey = e - kFX;
-            const int kk = ey / kTW, lc = ey - kk * kTW;
-            const int grl = r0 - 1 + kk, gc = c0 + lc;
-            if (gc >= nx || grl < 0 || grl + 1 >= ny) continue;
-            const int L = kk * kLW + lc + 1, R = L + kLW;
-            Face32 f = face32v<1>(T.b[L], T.b[R], T.hq[L], T.hq[R], T.h[L], T.h[R], T.vx[L], T.vy[L],
-                                  T.vx[R], T.vy[R], k, fyf, cq);
-            T.fyq[ey] = f.h; T.fyl[ey] = f.qyl; T.fyr[ey] = f.qyr; T.fyt[ey] = f.qxl;
-        }
-    }
-    __syncthreads();
-    const int dst = src ^ 1;
-    // per-thread sums of the substep diagnostics over this thread's cells, in
-    // FP32 (two cells per thread), promoted to FP64 for the block reduction
-    float dg[kDiag];
-#pragma unroll
-    for (int d = 0; d < kDiag; d++) dg[d] = 0.0f;
-    long long hn_min = 0;
-    float wv = 0.0f;
-    const float dt = (float)sub_dt;
-    const float ax = (float)a.cfg.external_ax, ay = (float)a.cfg.external_ay;
-    const float decrement = (float)(a.cfg.basal_friction * a.cfg.gravity) * dt;
-#pragma unroll
-    for (int j = 0; j < kTH * kTW / kBlock; j++) {
-        const int e = threadIdx.x + j * kBlock;
-        const int ir = e / kTW, ic = e - ir * kTW;
-        const int gr = r0 + ir, gc = c0 + ic;
-        if (gr >= ny || gc >= nx) continue;
-        const int L = (ir + 1) * kLW + ic + 1;
-        long long dq = 0;
-        float dqx = 0.0f, dqy = 0.0f;
-        if (gc + 1 < nx) {
-            const int f = ir * (kTW + 1) + ic + 1;
-            dq -= T.fxq[f]; dqx = dqx - T.fxl[f]; dqy = dqy - T.fxt[f];
-        }
-        if (gc > 0) {
-            const int f = ir * (kTW + 1) + ic;
-            dq += T.fxq[f]; dqx = dqx + T.fxr[f]; dqy = dqy + T.fxt[f];
-        }
-        if (gr + 1 < ny) {
-            const int f = (ir + 1) * kTW + ic;
-            dq -= T.fyq[f]; dqx = dqx - T.fyt[f]; dqy = dqy - T.fyl[f];
-        }
-        if (gr > 0) {
-            const int f = ir * kTW + ic;
-            dq += T.fyq[f]; dqx = dqx + T.fyt[f]; dqy = dqy + T.fyr[f];
-        }
-        const int hqi = T.hq[L];
-        const float qxi = T.qx[L], qyi = T.qy[L];
-        const long long hn = (long long)hqi + dq;
-        ovf |= hn > 2147483647LL;
-        hn_min = hn < hn_min ? hn : hn_min;
-        const int hq = (int)(hn < 0 ? 0 : (hn > 2147483647LL ? 2147483647LL : hn));
-        const float qxn = qxi + dqx, qyn = qyi + dqy;
-        dg[0] += qxn - qxi;
-        dg[1] += qyn - qyi;
-        const float depth = (float)hq * Qf;
-        float qax = 0.0f, qay = 0.0f, wave_v = 0.0f;
-        const bool wet = depth > k.dry;
-        if (wet) {
-            const float inv = rcpf(depth);
-            const float vbx = qxn * inv, vby = qyn * inv;
-            const float vex = vbx + ax * dt, vey = vby + ay * dt;
-            const float speed = __builtin_sqrtf(vex * vex + vey * vey);
-            const float factor = speed > 1.0e-15f ? fmaxf(0.0f, 1.0f - decrement * rcpf(speed)) : 0.0f;
-            const float vax = vex * factor, vay = vey * factor;
-            qax = depth * vax;
-            qay = depth * vay;
-            wave_v = speed * factor;
-            dg[2] += depth * (vex - vbx);
-            dg[3] += depth * (vey - vby);
-            dg[6] += depth * (0.5f * (vbx + vex) * ax + 0.5f * (vby + vey) * ay) * dt;
-            dg[4] += qax - depth * vex;
-            dg[5] += qay - depth * vey;
-            dg[7] += depth * (speed * speed - (vax * vax + vay * vay));
-        }
-        dg[8] += (wet ? qxn : 0.0f) - qxi;
-        dg[9] += (wet ? qyn : 0.0f) - qyi;
-        const int i = gr * nx + gc;
-        a.h[dst][i] = hq;
-        a.qx[dst][i] = qax;
-        a.qy[dst][i] = qay;
-        wv = fmaxf(wv, wave_v + __builtin_sqrtf(k.Kg * depth));
-    }
-#pragma unroll
-    for (int d = 0; d < kDiag; d++) acc.diag[d] += (double)dg[d];
-    hmin = npmin(hmin, (double)hn_min * kQuantum);
-    wmax = npmax(wmax, (double)wv);
-    __syncthreads();
-}
+using Tile32 = TileT<kTH, kTW>;
 
 // Workgroups are dispatched round-robin over the 8 XCDs.  Renumber them so
 // that the workgroups of one XCD own one contiguous band of tiles: halos
@@ -931,7 +681,7 @@ __global__ void __launch_bounds__(kBlock) v2_substep_kernel(V2Args<P> a, int k, 
         for (int d = 0; d < kDiag; d++) dprev[d] = a.diag_part[(int64_t)d * a.diag_groups + blockIdx.x];
     __shared__ TileStore<P> store;
     if constexpr (P::kQuant) {
-        if (eager) tile_stage32(a, store.tile, src, t, tiles_x);
+        if (eager) tile_stage<kTH, kTW>(a, store.tile, src, (t / tiles_x) * kTH, (t % tiles_x) * kTW);
     }
     Control C = resolve_control(s, k);
     if (!C.done && k == 0 && cfg.static_exit) {
@@ -954,7 +704,7 @@ __global__ void __launch_bounds__(kBlock) v2_substep_kernel(V2Args<P> a, int k, 
         return;
     }
     if constexpr (P::kQuant) {
-        if (!eager) tile_stage32(a, store.tile, src, t, tiles_x);
+        if (!eager) tile_stage<kTH, kTW>(a, store.tile, src, (t / tiles_x) * kTH, (t % tiles_x) * kTW);
     }
     // CFL maximum of the current state (mobile_v2_reference.py:239-251)
     if (threadIdx.x < kWave) {
@@ -995,7 +745,8 @@ __global__ void __launch_bounds__(kBlock) v2_substep_kernel(V2Args<P> a, int k, 
     double hmin = 0.0;
     int ovf = 0;
     if constexpr (P::kQuant) {
-        tile_update32(a, store.tile, src, t, tiles_x, sub_dt, acc, wmax[0], hmin, ovf);
+        tile_update<kTH, kTW>(a, store.tile, src, (t / tiles_x) * kTH, (t % tiles_x) * kTW, sub_dt, acc.diag, wmax[0],
+                              hmin, ovf);
     } else {
         const int n = ny * nx;
         for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {

@@ -45,6 +45,14 @@ class StepRecord(ctypes.Structure):
     ]
 
 
+class BatchRecord(ctypes.Structure):
+    _fields_ = [
+        ("substeps", _I), ("numeric_error", _I), ("static_exit", _I), ("complete", _I),
+        ("maximum_cfl", _D), ("remaining", _D), ("mass_quanta_before", _D), ("mass_quanta_after", _D),
+        ("momentum_before", _D * 2), ("momentum_after", _D * 2), ("kinetic_before", _D), ("kinetic_after", _D),
+    ]
+
+
 class RelaxResult(ctypes.Structure):
     _fields_ = [("iterations", _I), ("converged", _I), ("max_excess_last", _D), ("complete", _I),
                 ("launches", _I)]
@@ -91,6 +99,10 @@ SIGNATURES: dict[str, tuple[object, list[object]]] = {
     "mv2_summary": (_I, [_P, ctypes.POINTER(StepConfig), _P]),
     "mv2_static_diagnostic": (_I, [_P, _P]),
     "mv2_set_timing": (_I, [_P, _I]),
+    "mv2_batch_bind": (_I, [_P, _I, _P, _P, _P, _D, _D, _P, _P, _P, _P]),
+    "mv2_batch_set_tile_shape": (_I, [_P, _I]),
+    "mv2_batch_tile_shape": (_I, [_I, _P, _P]),
+    "mv2_batch_step": (_I, [_P, ctypes.POINTER(StepConfig), _I, _P, ctypes.POINTER(_D)]),
     "mv2_entrain": (_I, [_P, _P, _P, _I, _I, ctypes.POINTER(_D)]),
     "mv2_deposit": (_I, [_P, _P, _P, _I, _I, _D, _P]),
     "mv2_reservoirs": (_I, [_P, _I, _D, _P]),

@@ -1,0 +1,168 @@
+"""Batched Mobile V2: many independent terrains of uneven size on one GPU.
+
+Each instance keeps the semantics of ``WarpMobileV2ReferenceSolver.step`` /
+``step_resident`` (its own CFL substep loop, static early exit and limits,
+pkg/experimental/mobile_v2_reference.py:226-307) but all instances advance in
+the same launches: the fields of every instance are packed back to back in one
+set of device arrays (FP32 b_eff and momentum, int32 mobile quanta of 2^-28 m),
+one launch per substep covers the tiles of the whole batch, and eight substep
+launches are replayed from a captured HIP graph.  ``autotune`` picks the tile
+shape of the substep kernel by timing the candidates on a copy of the batch.
+
+Records per instance: substeps, CFL, exact mobile quanta before/after,
+momentum and kinetic energy before/after (uniform cell weights, per unit
+density).
+"""
+
+from __future__ import annotations
+
+import ctypes
+import json
+import time
+from dataclasses import dataclass
+from pathlib import Path
+
+import numpy as np
+
+from . import _lib
+from .bulk_state import MOBILE_QUANTUM_M
+from .config import MobileV2Config
+from .mobile_v2 import _step_config
+from .runtime import HipRuntime
+
+TILE_SHAPES = {0: (16, 32), 1: (8, 64), 2: (32, 16), 3: (16, 64)}
+
+
+def _torch():
+    import torch
+
+    return torch
+
+
+@dataclass
+class BatchStepRecord:
+    substeps: int
+    numeric_error: int
+    static_exit: bool
+    maximum_cfl: float
+    remaining: float
+    mass_quanta_before: int
+    mass_quanta_after: int
+    momentum_before: tuple
+    momentum_after: tuple
+    kinetic_before: float
+    kinetic_after: float
+
+
+def to_quanta(h: np.ndarray) -> np.ndarray:
+    q = np.rint(np.asarray(h, dtype=np.float64) * (1.0 / MOBILE_QUANTUM_M))
+    if np.any(q < 0) or np.any(q > np.iinfo(np.int32).max):
+        raise ValueError("[BatchedMobileV2] mobile thickness outside 0 .. 8 m")
+    return q.astype(np.int32)
+
+
+class BatchedMobileV2:
+    """Independent terrains advanced together; ``terrains`` is a list of (b_eff, h_mobile)."""
+
+    backend_name = "GPU_HIP_MOBILE_V2_BATCHED"
+
+    def __init__(self, terrains, dx: float, dy: float | None = None, *, runtime: HipRuntime | None = None,
+                 config: MobileV2Config | None = None, use_graph: bool = True, tile_shape: int = 0) -> None:
+        torch = _torch()
+        self.rt = runtime if runtime is not None else HipRuntime("cuda:0")
+        self.dx = float(dx)
+        self.dy = float(dx if dy is None else dy)
+        self.config = config or MobileV2Config(dx_m=self.dx, dy_m=self.dy)
+        self.use_graph = bool(use_graph)
+        shapes = [np.asarray(b).shape for b, _ in terrains]
+        self.ny = np.asarray([s[0] for s in shapes], np.int32)
+        self.nx = np.asarray([s[1] for s in shapes], np.int32)
+        sizes = self.ny.astype(np.int64) * self.nx.astype(np.int64)
+        self.offset = np.concatenate([[0], np.cumsum(sizes)[:-1]]).astype(np.int64)
+        self.cells = int(sizes.sum())
+        b = np.concatenate([np.asarray(t[0], np.float32).ravel() for t in terrains])
+        h = np.concatenate([to_quanta(t[1]).ravel() for t in terrains])
+        dev = self.rt.device
+        self.b = torch.from_numpy(b).to(dev)
+        self.h = torch.from_numpy(h).to(dev)
+        self.qx = torch.zeros(self.cells, dtype=torch.float32, device=dev)
+        self.qy = torch.zeros(self.cells, dtype=torch.float32, device=dev)
+        self.tile_shape = int(tile_shape)
+        self._bind(self.b, self.h, self.qx, self.qy)
+        self.last_kernel_ms = 0.0
+
+    @property
+    def count(self) -> int:
+        return int(self.ny.size)
+
+    def _bind(self, b, h, qx, qy) -> None:
+        rt = self.rt
+        rt.bind_stream()
+        rt.check(rt.lib.mv2_batch_bind(rt.ctx, self.count, self.ny.ctypes.data, self.nx.ctypes.data,
+                                       self.offset.ctypes.data, self.dx, self.dy, b.data_ptr(), h.data_ptr(),
+                                       qx.data_ptr(), qy.data_ptr()))
+        rt.check(rt.lib.mv2_batch_set_tile_shape(rt.ctx, self.tile_shape))
+
+    def set_tile_shape(self, shape: int) -> None:
+        if shape not in TILE_SHAPES:
+            raise ValueError(f"[BatchedMobileV2] unknown tile shape {shape}")
+        self.tile_shape = int(shape)
+        self.rt.check(self.rt.lib.mv2_batch_set_tile_shape(self.rt.ctx, self.tile_shape))
+
+    def step(self, dt: float, *, limit_mode: int = 0) -> list:
+        """Advance every instance by dt; one record per instance."""
+        rt = self.rt
+        rt.bind_stream()
+        cfg = _step_config(self.config, dt, density=1.0, limit_mode=limit_mode, weight_mode=0)
+        recs = (_lib.BatchRecord * self.count)()
+        ms = ctypes.c_double(0.0)
+        rc = rt.lib.mv2_batch_step(rt.ctx, ctypes.byref(cfg), int(self.use_graph), recs, ctypes.byref(ms))
+        self.last_kernel_ms = ms.value
+        if rc == _lib.MV2_ENUMERIC:
+            raise RuntimeError(_lib.last_error(rt.ctx))
+        rt.check(rc)
+        return [BatchStepRecord(r.substeps, r.numeric_error, bool(r.static_exit), r.maximum_cfl, r.remaining,
+                                int(r.mass_quanta_before), int(r.mass_quanta_after),
+                                tuple(r.momentum_before), tuple(r.momentum_after), r.kinetic_before,
+                                r.kinetic_after) for r in recs]
+
+    def fields(self, i: int):
+        """(b_eff, h_mobile [m], qx, qy) of instance i as numpy arrays."""
+        o, n = int(self.offset[i]), int(self.ny[i]) * int(self.nx[i])
+        shape = (int(self.ny[i]), int(self.nx[i]))
+        b = self.b[o:o + n].cpu().numpy().reshape(shape)
+        h = self.h[o:o + n].cpu().numpy().reshape(shape).astype(np.float64) * MOBILE_QUANTUM_M
+        return b, h, self.qx[o:o + n].cpu().numpy().reshape(shape), self.qy[o:o + n].cpu().numpy().reshape(shape)
+
+    def autotune(self, dt: float, *, shapes=tuple(TILE_SHAPES), frames: int = 2, cache: Path | None = None) -> dict:
+        """Time each tile shape on a copy of the batch; keep the fastest (and cache it)."""
+        torch = _torch()
+        key = f"{self.count}:{int(self.ny.sum())}:{int(self.cells)}:{dt:.6g}"
+        if cache is not None and Path(cache).exists():
+            known = json.loads(Path(cache).read_text())
+            if key in known:
+                self.set_tile_shape(int(known[key]["best"]))
+                return known[key]
+        times = {}
+        b, h, qx, qy = self.b.clone(), self.h.clone(), self.qx.clone(), self.qy.clone()
+        try:
+            for s in shapes:
+                self.tile_shape = int(s)
+                for _ in range(frames + 1):        # first frame: graph capture, buffers
+                    b.copy_(self.b); h.copy_(self.h); qx.copy_(self.qx); qy.copy_(self.qy)
+                    self._bind(b, h, qx, qy)
+                    torch.cuda.synchronize()
+                    t0 = time.perf_counter()
+                    self.step(dt)
+                    times.setdefault(s, []).append(time.perf_counter() - t0)
+        finally:
+            best = min(times, key=lambda s: min(times[s][1:] or times[s]))
+            self.tile_shape = int(best)
+            self._bind(self.b, self.h, self.qx, self.qy)
+        result = {"best": int(best), "tile": TILE_SHAPES[int(best)],
+                  "ms": {int(s): 1e3 * min(v[1:] or v) for s, v in times.items()}}
+        if cache is not None:
+            known = json.loads(Path(cache).read_text()) if Path(cache).exists() else {}
+            known[key] = result
+            Path(cache).write_text(json.dumps(known, indent=1))
+        return result

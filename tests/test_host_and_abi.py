@@ -51,6 +51,7 @@ STRUCTS = {
     "AvalancheConfig": "mv2_avalanche_config",
     "AvalancheFields": "mv2_avalanche_fields",
     "LabelResult": "mv2_label_result",
+    "BatchRecord": "mv2_batch_record",
 }
 
 
