@@ -151,9 +151,11 @@ int mv2_batch_bind(mv2_context *ctx, int count, const int32_t *ny, const int32_t
 int mv2_batch_set_tile_shape(mv2_context *ctx, int shape);
 int mv2_batch_tile_shape(int shape, int *th, int *tw);
 /* one frame of every instance (cfg->dt each); use_graph replays a HIP graph of
- * eight substep launches; out has one record per instance */
+ * eight substep launches; out has one record per instance; timing_ms (NULL or
+ * 3 doubles): device ms of the call, device ms of the substep launches, and
+ * the number of substep launches */
 int mv2_batch_step(mv2_context *ctx, const mv2_step_config *cfg, int use_graph,
-                   mv2_batch_record *out, double *kernel_ms);
+                   mv2_batch_record *out, double *timing_ms);
 
 /* ---- plain angle-of-repose relaxation: slope_model.relax_critical_slope
  *      (slope_model.py:273-343) behind MinimumSlopeAdapter._legacy_call

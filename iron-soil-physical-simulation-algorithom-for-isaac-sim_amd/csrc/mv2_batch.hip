@@ -448,6 +448,7 @@ struct mv2_batch_state {
     // graph of eight substep launches
     hipGraphExec_t graph = nullptr;
     hipStream_t capture = nullptr;
+    hipEvent_t ev_sub0 = nullptr, ev_sub1 = nullptr;   // bracket the substep launches
     std::vector<unsigned char> graph_key;
 };
 
@@ -463,6 +464,8 @@ void mv2_batch_release(mv2_context *ctx)
     if (!bs) return;
     if (bs->graph) hipGraphExecDestroy(bs->graph);
     if (bs->capture) hipStreamDestroy(bs->capture);
+    if (bs->ev_sub0) hipEventDestroy(bs->ev_sub0);
+    if (bs->ev_sub1) hipEventDestroy(bs->ev_sub1);
     for (Scratch *s : {&bs->inst, &bs->tiles_tab, &bs->alt_h, &bs->alt_qx, &bs->alt_qy, &bs->ctl, &bs->wave,
                        &bs->stat, &bs->ledger, &bs->rec})
         if (s->ptr) hipFree(s->ptr);
@@ -637,7 +640,7 @@ static int ensure_graph(mv2_context *ctx, mv2_batch_state *bs, const BatchArgs &
 }
 
 extern "C" int mv2_batch_step(mv2_context *ctx, const mv2_step_config *cfg, int use_graph, mv2_batch_record *out,
-                              double *kernel_ms)
+                              double *timing_ms)
 {
     if (!ctx) return set_error(nullptr, MV2_EINVAL, "[BatchedMobileV2] null context");
     mv2_batch_state *bs = ctx->batch;
@@ -663,19 +666,29 @@ extern "C" int mv2_batch_step(mv2_context *ctx, const mv2_step_config *cfg, int 
     const size_t rec_bytes = sizeof(mv2_batch_record) * (size_t)bs->count;
     if (int rc = ensure_scratch(ctx, ctx->host_result, rec_bytes + 4096, true)) return rc;
     const mv2_batch_record *hrec = (const mv2_batch_record *)ctx->host_result.ptr;
+    if (!bs->ev_sub0) {
+        MV2_HIP(ctx, hipEventCreate(&bs->ev_sub0));
+        MV2_HIP(ctx, hipEventCreate(&bs->ev_sub1));
+    }
+    double sub_ms = 0.0;
     for (;;) {
+        MV2_HIP(ctx, hipEventRecord(bs->ev_sub0, st));
         if (use_graph) {
             const int reps = (want + 7) / 8;
             for (int r = 0; r < reps; r++, k += 8) MV2_HIP(ctx, hipGraphLaunch(bs->graph, st));
         } else {
             for (int j = 0; j < want; j++, k++) launch_substep(bs->shape, grid, st, a, k % 8);
         }
+        MV2_HIP(ctx, hipEventRecord(bs->ev_sub1, st));
         launch_end(bs->shape, grid, st, a, k % 8);
         hipLaunchKernelGGL(batch_finalize_kernel, dim3(bs->count), dim3(kBlock), 0, st, a, k % 8);
         MV2_HIP(ctx, hipGetLastError());
         MV2_HIP(ctx, hipEventRecord(ctx->ev_stop, st));
         MV2_HIP(ctx, hipMemcpyAsync(ctx->host_result.ptr, a.rec, rec_bytes, hipMemcpyDeviceToHost, st));
         MV2_HIP(ctx, hipStreamSynchronize(st));
+        float sms = 0.0f;
+        MV2_HIP(ctx, hipEventElapsedTime(&sms, bs->ev_sub0, bs->ev_sub1));
+        sub_ms += sms;
         bool all = true;
         int most = 0;
         for (int i = 0; i < bs->count; i++) {
@@ -691,10 +704,12 @@ extern "C" int mv2_batch_step(mv2_context *ctx, const mv2_step_config *cfg, int 
         (void)most;
     }
     memcpy(out, hrec, rec_bytes);
-    if (kernel_ms) {
+    if (timing_ms) {
         float ms = 0.0f;
         MV2_HIP(ctx, hipEventElapsedTime(&ms, ctx->ev_start, ctx->ev_stop));
-        *kernel_ms = ms;
+        timing_ms[0] = ms;          // the whole call on the device
+        timing_ms[1] = sub_ms;      // the substep launches (including those after every loop ended)
+        timing_ms[2] = (double)k;   // substep launches
     }
     for (int i = 0; i < bs->count; i++)
         if (out[i].numeric_error)

@@ -12,11 +12,12 @@
 
 namespace mv2 {
 
-// Mobile thickness quantum of the FP32 storage mode: 2^-28 m (3.7e-9 m).
+// Mobile thickness quantum of the FP32 storage mode: 2^-24 m (6.0e-8 m).
 // Mass moves between cells as whole quanta, so FP32 runs conserve mass exactly
-// and int32 storage covers 0 .. 8 m of mobile depth.
-constexpr double kQuantum = 1.0 / 268435456.0;
-constexpr double kInvQuantum = 268435456.0;
+// and int32 storage covers 0 .. 128 m of mobile depth (material pooling in a
+// pit of a steep pile can exceed 8 m, the range of a 2^-28 m quantum).
+constexpr double kQuantum = 1.0 / 16777216.0;
+constexpr double kInvQuantum = 16777216.0;
 
 constexpr int kBlock = 256;      // threads per workgroup for streaming kernels
 constexpr int kWave = 64;

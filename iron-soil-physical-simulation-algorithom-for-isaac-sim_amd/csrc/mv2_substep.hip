@@ -21,7 +21,7 @@
 //   FP64 mode: every face and cell update repeats the reference's operation
 //              order (mobile_v2_reference.py:236-307, :365-449) with
 //              -ffp-contract=off, so fields match the CPU oracle bit for bit.
-//   FP32 mode: float fluxes; mobile thickness in int32 quanta of 2^-28 m.
+//   FP32 mode: float fluxes; mobile thickness in int32 quanta of 2^-24 m.
 //              Each face's mass flux is rounded once to whole quanta and added
 //              to one cell / removed from the other, so total mass is exact;
 //              outflow through one face is capped at a quarter of the donor's

@@ -10,7 +10,7 @@ contracts run as HIP kernels through libmv2.
 
 precision="fp64" stores every field as float64, exactly like the reference.
 precision="fp32" stores z_base/b_eff/momentum as float32 and ``mobile`` as
-int32 quanta of ``MOBILE_QUANTUM_M`` (2^-28 m): the Mobile V2 step then moves
+int32 quanta of ``MOBILE_QUANTUM_M`` (2^-24 m): the Mobile V2 step then moves
 mass in whole quanta, so its mass ledger is exact.  ``mobile_m()`` and the host
 views convert quanta to metres.
 """
@@ -28,7 +28,7 @@ from . import _lib
 from .config import TerrainGrid, triangle_vertex_weights
 from .runtime import HipRuntime
 
-MOBILE_QUANTUM_M = 1.0 / 268435456.0
+MOBILE_QUANTUM_M = 1.0 / 16777216.0
 
 
 class BulkStateAuthority(str, Enum):
@@ -371,7 +371,7 @@ class DeviceBulkState:
             "state_authority": self.authority.value,
             "authoritative_geometry": ["z_base", "b_eff", "mobile"],
             "precision": self.precision,
-            "mobile_storage": "int32 quanta of 2^-28 m" if self.precision == "fp32" else "float64 m",
+            "mobile_storage": "int32 quanta of 2^-24 m" if self.precision == "fp32" else "float64 m",
             "shape_yx": list(self.shape),
             "tile_size": self.tile_size,
             "timestamp_device_s": self._timestamp_device_s,

@@ -9,6 +9,7 @@ import pytest
 
 pytestmark = pytest.mark.gpu
 
+QUANTUM = 2.0 ** -24
 SIZES = [(40, 50), (64, 64), (33, 97), (130, 70), (96, 128), (17, 23)]
 
 
@@ -32,7 +33,7 @@ def _single_reference(terrains, dt, frames, config):
     solver = HipMobileV2ReferenceSolver(config, precision="fp32")
     results = []
     for b, h in terrains:
-        q = np.rint(h * 2.0 ** 28) * 2.0 ** -28
+        q = np.rint(h / QUANTUM) * QUANTUM
         state = MobileV2State(b.astype(np.float64), q, np.zeros(h.shape + (2,)))
         subs = []
         for _ in range(frames):
@@ -61,7 +62,7 @@ def test_batch_matches_single_terrain_bit_exact(shape, use_graph):
     for i, (state, subs) in enumerate(ref):
         b, h, qx, qy = batch.fields(i)
         assert [recs[f][i].substeps for f in range(frames)] == subs, i
-        np.testing.assert_array_equal(np.rint(h * 2.0 ** 28), np.rint(state.h_m * 2.0 ** 28))
+        np.testing.assert_array_equal(np.rint(h / QUANTUM), np.rint(state.h_m / QUANTUM))
         np.testing.assert_array_equal(qx.astype(np.float64), state.q_m2_s[..., 0])
         np.testing.assert_array_equal(qy.astype(np.float64), state.q_m2_s[..., 1])
         for f in range(frames):

@@ -41,7 +41,7 @@ def test_library_exports_every_declared_symbol():
 def test_abi_constants():
     lib = _lib.load()
     assert lib.mv2_abi_version() == 1
-    assert lib.mv2_mobile_quantum() == 2.0 ** -28
+    assert lib.mv2_mobile_quantum() == 2.0 ** -24
 
 
 STRUCTS = {
