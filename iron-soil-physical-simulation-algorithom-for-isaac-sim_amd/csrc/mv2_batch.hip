@@ -6,7 +6,7 @@
 // static early exit), exactly as if it were stepped alone.  One launch per
 // substep covers the tiles of *all* instances, so a launch moves the whole
 // batch (hundreds of MB) and the per-launch costs that dominate a single 701^2
-// terrain are amortised: this is the HBM-bound form of the substep.
+// terrain are amortised.
 //
 // Per-instance loop state follows the single-terrain design: launch k reads
 // ctl[k & 1][i] and the instance's first tile writes ctl[(k+1) & 1][i]; the
@@ -15,12 +15,13 @@
 // k % 4, fills k+1, clears k+2).  Launch indices enter only as k mod 8, so the
 // host can capture eight substep launches once into a HIP graph and replay it.
 //
-// Tile shapes are a template parameter (chosen by the host-side autotuner).
-// Every face is computed from the same two cells with the same instructions
-// whatever the tile shape, so fields are bit-identical across shapes and to
-// the single-terrain step of each instance.
+// Tiles are 62 columns by four wave bands of SH rows (the strip layout of
+// mv2_strip32.hpp); SH is a template parameter chosen by the host-side
+// autotuner.  Every face is computed from the same two cells with the same
+// instructions whatever the band height, so fields are bit-identical across
+// shapes and to the single-terrain step of each instance.
 #include "mv2_internal.hpp"
-#include "mv2_tile32.hpp"
+#include "mv2_strip32.hpp"
 
 #include <cmath>
 #include <cstring>
@@ -143,50 +144,88 @@ __global__ void batch_reset_kernel(BatchArgs a)
     for (int64_t i = tid; i < 2 * (int64_t)a.count * kInstSlots; i += nth) a.stat[i * kInstStride] = 0ull;
 }
 
-// one tile: ledger before, static diagnostic, CFL maximum of the input
-template <int TH, int TW>
+// double-precision value of lane l + 1 / lane l - 1 (two 32-bit DPP moves)
+__device__ inline double from_next_lane_d(double v)
+{
+    const unsigned long long u = __builtin_bit_cast(unsigned long long, v);
+    const int lo = from_next_lane((int)(unsigned)u), hi = from_next_lane((int)(unsigned)(u >> 32));
+    return __builtin_bit_cast(double, ((unsigned long long)(unsigned)hi << 32) | (unsigned)lo);
+}
+__device__ inline double from_prev_lane_d(double v)
+{
+    const unsigned long long u = __builtin_bit_cast(unsigned long long, v);
+    const int lo = from_prev_lane((int)(unsigned)u), hi = from_prev_lane((int)(unsigned)(u >> 32));
+    return __builtin_bit_cast(double, ((unsigned long long)(unsigned)hi << 32) | (unsigned)lo);
+}
+
+// One tile: ledger before, static diagnostic, CFL maximum of the input.  The
+// tile's waves use the strip layout of the substep: lanes 1..62 own the
+// columns, a lane holds its column's rows, and the static gradient takes its
+// x-neighbours by DPP and its y-neighbours from the rows in registers.
+template <int SH>
 __global__ void __launch_bounds__(kBlock) batch_begin_kernel(BatchArgs a)
 {
     __shared__ double smem[64 * 4];
     const int g = blockIdx.x;
     const int i = a.tile_inst[g];
     const InstView v(a, i);
-    const int r0 = a.tile_r0[g], c0 = a.tile_c0[g];
+    const int lane = threadIdx.x & (kWave - 1);
+    const int r0 = a.tile_r0[g] + wave_index() * SH, c0 = a.tile_c0[g];
     const int ny = v.ny, nx = v.nx;
+    const int gc = c0 - 1 + lane;
+    const bool owner = lane >= 1 && lane <= kStripW && gc < nx;
     const double K = a.cfg.earth_pressure, gr = a.cfg.gravity, dry = a.cfg.dry_tolerance;
+    const float Qf = (float)kQuantum;
+    StripRows<SH> R;
+    strip_load<SH>(v, 0, r0, c0, R);
+    double H[SH + 2];
+#pragma unroll
+    for (int j = 0; j < SH + 2; j++) H[j] = (double)R.b[j] + (double)((float)R.hq[j] * Qf);
     double s[kBLedger] = {0.0, 0.0, 0.0};
     long long q[1] = {0};
-    double m[3] = {0.0, 0.0, 0.0};                // max |grad H|, max |q|, max wave speed
+    double m[3] = {0.0, 0.0, 0.0};                // max |grad H|^2, max |q|, max wave speed
+    // The static test only asks whether every |grad H| <= tan_start + 1e-14.
+    // |grad H| is taken with reciprocal multiplies (a few ulps from the
+    // quotients); a cell within a relative 1e-12 of the bound is redone with
+    // the quotients, so the test decides exactly as with quotients everywhere.
+    const double bound = a.cfg.tan_start + 1.0e-14;
+    const double band_lo = bound * bound * (1.0 - 1.0e-12), band_hi = bound * bound * (1.0 + 1.0e-12);
+    const double denx = (gc > 0 && gc + 1 < nx) ? 2.0 * a.dx : a.dx;
+    const double inv_denx = 1.0 / denx;
+    const double inv_dy2 = 1.0 / (2.0 * a.dy), inv_dy1 = 1.0 / a.dy;
+    const float Kg = uniform((float)(K * gr)), dryf = uniform((float)dry);
 #pragma unroll
-    for (int j = 0; j < TH * TW / kBlock; j++) {
-        const int e = threadIdx.x + j * kBlock;
-        const int r = r0 + e / TW, c = c0 + e % TW;
-        if (r >= ny || c >= nx) continue;
-        const int idx = r * nx + c;
-        const int32_t hq = v.h[0][idx];
-        const float d = (float)hq * (float)kQuantum;
-        const float qxv = v.qx[0][idx], qyv = v.qy[0][idx];
+    for (int j = 1; j <= SH; j++) {
+        const int r = r0 - 1 + j;
+        const double HR = from_next_lane_d(H[j]), HL = from_prev_lane_d(H[j]);
+        if (!owner || r >= ny) continue;
+        const int32_t hq = R.hq[j];
+        const float d = (float)hq * Qf;
+        const float qxv = R.qx[j], qyv = R.qy[j];
         q[0] += hq;
-        const double depth = (double)d;
         s[0] += (double)qxv;
         s[1] += (double)qyv;
         float vx = 0.0f, vy = 0.0f;
-        if (d > (float)dry) {
+        if (d > dryf) {
             const float inv = rcpf(d);
             vx = qxv * inv;
             vy = qyv * inv;
-            const double dvx = (double)qxv / depth, dvy = (double)qyv / depth;
-            s[2] += 0.5 * depth * (dvx * dvx + dvy * dvy);
+            s[2] += (double)(0.5f * (qxv * vx + qyv * vy));
         }
-        m[2] = npmax(m[2], (double)(__builtin_sqrtf(vx * vx + vy * vy) + __builtin_sqrtf((float)(K * gr) * d)));
+        m[2] = npmax(m[2], (double)(sqrtf_hw(vx * vx + vy * vy) + sqrtf_hw(Kg * d)));
         if (a.cfg.static_exit) {
-            const int cl = c > 0 ? idx - 1 : idx, cr = c + 1 < nx ? idx + 1 : idx;
-            const int rd = r > 0 ? idx - nx : idx, ru = r + 1 < ny ? idx + nx : idx;
-            auto H = [&](int t) { return (double)v.b[t] + (double)((float)v.h[0][t] * (float)kQuantum); };
-            const double denx = (c > 0 && c + 1 < nx) ? 2.0 * a.dx : a.dx;
-            const double deny = (r > 0 && r + 1 < ny) ? 2.0 * a.dy : a.dy;
-            const double gx = (H(cr) - H(cl)) / denx, gy = (H(ru) - H(rd)) / deny;
-            m[0] = npmax(m[0], sqrt(gx * gx + gy * gy));
+            // central differences inside, one-sided at the terrain edge
+            const double hl = gc > 0 ? HL : H[j], hr = gc + 1 < nx ? HR : H[j];
+            const double hd = r > 0 ? H[j - 1] : H[j], hu = r + 1 < ny ? H[j + 1] : H[j];
+            const bool inner_y = r > 0 && r + 1 < ny;
+            const double gx = (hr - hl) * inv_denx, gy = (hu - hd) * (inner_y ? inv_dy2 : inv_dy1);
+            double g2 = gx * gx + gy * gy;
+            if (g2 >= band_lo && g2 <= band_hi) {
+                const double deny = inner_y ? 2.0 * a.dy : a.dy;
+                const double ex = (hr - hl) / denx, ey = (hu - hd) / deny;
+                g2 = ex * ex + ey * ey;
+            }
+            m[0] = npmax(m[0], g2);
             m[1] = npmax(m[1], fabs((double)qxv));
             m[1] = npmax(m[1], fabs((double)qyv));
         }
@@ -199,18 +238,18 @@ __global__ void __launch_bounds__(kBlock) batch_begin_kernel(BatchArgs a)
         a.quanta[g] = q[0];
         inst_max(inst_set(a.wave, 0, a.count, i), m[2]);
         if (a.cfg.static_exit) {
-            inst_max(inst_set(a.stat, 0, a.count, i), m[0]);
+            // the square root is monotonic: the root of the largest square is the largest root
+            inst_max(inst_set(a.stat, 0, a.count, i), sqrt(m[0]));
             inst_max(inst_set(a.stat, 1, a.count, i), m[1]);
         }
     }
 }
 
 // One substep of every instance whose loop has not ended (k is used mod 8).
-template <int TH, int TW>
+template <int SH>
 __global__ void __launch_bounds__(kBlock) batch_substep_kernel(BatchArgs a, int k)
 {
     __shared__ double smem[64 * 4];
-    __shared__ TileT<TH, TW> tile;
     const int g = blockIdx.x;
     const int i = a.tile_inst[g];
     const bool lead = g == a.inst_first_tile[i] && threadIdx.x == 0;
@@ -239,8 +278,9 @@ __global__ void __launch_bounds__(kBlock) batch_substep_kernel(BatchArgs a, int 
     }
     const InstView v(a, i);
     const int src = C.src;
-    const int r0 = a.tile_r0[g], c0 = a.tile_c0[g];
-    tile_stage<TH, TW>(v, tile, src, r0, c0);
+    const int r0 = a.tile_r0[g] + wave_index() * SH, c0 = a.tile_c0[g];
+    StripRows<SH> rows;
+    strip_load<SH>(v, src, r0, c0, rows);
     const double maxw = inst_read_max(inst_set(a.wave, k % kWaveSetsB, a.count, i), smem);
     const double mind = a.dx <= a.dy ? a.dx : a.dy;
     const double stable = maxw <= 1.0e-14 ? C.remaining : cfg.cfl * mind / maxw;
@@ -268,14 +308,15 @@ __global__ void __launch_bounds__(kBlock) batch_substep_kernel(BatchArgs a, int 
         }
         a.ctl[((k + 1) & 1) * a.count + i] = nc;
     }
-    double diag[kDiag];
-    for (int d = 0; d < kDiag; d++) diag[d] = 0.0;
-    double wmax[1] = {0.0};
-    double hmin = 0.0;
+    float dg[kDiag];
+    for (int d = 0; d < kDiag; d++) dg[d] = 0.0f;
+    float wv = 0.0f;
+    long long hn_min = 0;
     int ovf = 0;
-    tile_update<TH, TW>(v, tile, src, r0, c0, sub_dt, diag, wmax[0], hmin, ovf);
+    strip_update<SH>(v, rows, src, r0, c0, sub_dt, dg, wv, hn_min, ovf);
+    double wmax[1] = {(double)wv};
     block_reduce<double, 1, 1>(wmax, smem);
-    double hm[1] = {hmin};
+    double hm[1] = {(double)hn_min * kQuantum};
     block_reduce<double, 1, 2>(hm, smem);
     const int anyovf = __syncthreads_or(ovf);
     if (threadIdx.x == 0) {
@@ -288,7 +329,7 @@ __global__ void __launch_bounds__(kBlock) batch_substep_kernel(BatchArgs a, int 
 }
 
 // copy back when an instance's final state sits in scratch; ledger after
-template <int TH, int TW>
+template <int SH>
 __global__ void __launch_bounds__(kBlock) batch_end_kernel(BatchArgs a, int k)
 {
     __shared__ double smem[64 * 4];
@@ -296,32 +337,47 @@ __global__ void __launch_bounds__(kBlock) batch_end_kernel(BatchArgs a, int k)
     const int i = a.tile_inst[g];
     const BControl C = resolve_bcontrol(a, k, i);
     const InstView v(a, i);
-    const int r0 = a.tile_r0[g], c0 = a.tile_c0[g];
+    const int lane = threadIdx.x & (kWave - 1);
+    const int r0 = a.tile_r0[g] + wave_index() * SH, c0 = a.tile_c0[g];
     const int ny = v.ny, nx = v.nx;
+    const int gc = c0 - 1 + lane;
+    const bool owner = lane >= 1 && lane <= kStripW && gc < nx;
     const double dry = a.cfg.dry_tolerance;
     const int fsrc = C.src;
+    const int32_t *gh = fsrc ? v.h[1] : v.h[0];
+    const float *gqx = fsrc ? v.qx[1] : v.qx[0], *gqy = fsrc ? v.qy[1] : v.qy[0];
+    int32_t hq[SH];
+    float qxv[SH], qyv[SH];
+#pragma unroll
+    for (int j = 0; j < SH; j++) {
+        const int r = r0 + j;
+        hq[j] = 0; qxv[j] = 0.0f; qyv[j] = 0.0f;
+        if (owner && r < ny) {
+            const int idx = r * nx + gc;
+            hq[j] = gh[idx];
+            qxv[j] = gqx[idx];
+            qyv[j] = gqy[idx];
+        }
+    }
     double s[kBLedger] = {0.0, 0.0, 0.0};
     long long q[1] = {0};
 #pragma unroll
-    for (int j = 0; j < TH * TW / kBlock; j++) {
-        const int e = threadIdx.x + j * kBlock;
-        const int r = r0 + e / TW, c = c0 + e % TW;
-        if (r >= ny || c >= nx) continue;
-        const int idx = r * nx + c;
-        const int32_t hq = (fsrc ? v.h[1] : v.h[0])[idx];
-        const float qxv = (fsrc ? v.qx[1] : v.qx[0])[idx], qyv = (fsrc ? v.qy[1] : v.qy[0])[idx];
+    for (int j = 0; j < SH; j++) {
+        const int r = r0 + j;
+        if (!owner || r >= ny) continue;
         if (fsrc == 1) {
-            v.h[0][idx] = hq;
-            v.qx[0][idx] = qxv;
-            v.qy[0][idx] = qyv;
+            const int idx = r * nx + gc;
+            v.h[0][idx] = hq[j];
+            v.qx[0][idx] = qxv[j];
+            v.qy[0][idx] = qyv[j];
         }
-        q[0] += hq;
-        const double depth = (double)((float)hq * (float)kQuantum);
-        s[0] += (double)qxv;
-        s[1] += (double)qyv;
+        q[0] += hq[j];
+        const double depth = (double)((float)hq[j] * (float)kQuantum);
+        s[0] += (double)qxv[j];
+        s[1] += (double)qyv[j];
         if (depth > dry) {
-            const double dvx = (double)qxv / depth, dvy = (double)qyv / depth;
-            s[2] += 0.5 * depth * (dvx * dvx + dvy * dvy);
+            const float inv = rcpf((float)depth);
+            s[2] += (double)(0.5f * (qxv[j] * (qxv[j] * inv) + qyv[j] * (qyv[j] * inv)));
         }
     }
     block_reduce<double, kBLedger, 0>(s, smem);
@@ -374,53 +430,55 @@ __global__ void __launch_bounds__(kBlock) batch_finalize_kernel(BatchArgs a, int
 // ---------------------------------------------------------------------------
 // host side
 // ---------------------------------------------------------------------------
+// tile shapes: kStripW columns by 4 bands of SH rows (one band per wave)
 struct TileShape {
     int th, tw;
 };
-constexpr TileShape kShapes[] = {{16, 32}, {8, 64}, {32, 16}, {16, 64}};
+constexpr TileShape kShapes[] = {{8, kStripW}, {16, kStripW}, {32, kStripW}, {64, kStripW}};
 constexpr int kShapeCount = 4;
 
-template <int TH, int TW>
+template <int SH>
 struct ShapeKernels {
+    static_assert(SH * (kBlock / kWave) > 0, "bands per tile");
     static void begin(dim3 g, hipStream_t s, const BatchArgs &a)
     {
-        batch_begin_kernel<TH, TW><<<g, dim3(kBlock), 0, s>>>(a);
+        batch_begin_kernel<SH><<<g, dim3(kBlock), 0, s>>>(a);
     }
     static void substep(dim3 g, hipStream_t s, const BatchArgs &a, int k)
     {
-        batch_substep_kernel<TH, TW><<<g, dim3(kBlock), 0, s>>>(a, k);
+        batch_substep_kernel<SH><<<g, dim3(kBlock), 0, s>>>(a, k);
     }
     static void end(dim3 g, hipStream_t s, const BatchArgs &a, int k)
     {
-        batch_end_kernel<TH, TW><<<g, dim3(kBlock), 0, s>>>(a, k);
+        batch_end_kernel<SH><<<g, dim3(kBlock), 0, s>>>(a, k);
     }
 };
 
 static void launch_begin(int shape, dim3 g, hipStream_t s, const BatchArgs &a)
 {
     switch (shape) {
-    case 0: ShapeKernels<16, 32>::begin(g, s, a); break;
-    case 1: ShapeKernels<8, 64>::begin(g, s, a); break;
-    case 2: ShapeKernels<32, 16>::begin(g, s, a); break;
-    default: ShapeKernels<16, 64>::begin(g, s, a); break;
+    case 0: ShapeKernels<2>::begin(g, s, a); break;
+    case 1: ShapeKernels<4>::begin(g, s, a); break;
+    case 2: ShapeKernels<8>::begin(g, s, a); break;
+    default: ShapeKernels<16>::begin(g, s, a); break;
     }
 }
 static void launch_substep(int shape, dim3 g, hipStream_t s, const BatchArgs &a, int k)
 {
     switch (shape) {
-    case 0: ShapeKernels<16, 32>::substep(g, s, a, k); break;
-    case 1: ShapeKernels<8, 64>::substep(g, s, a, k); break;
-    case 2: ShapeKernels<32, 16>::substep(g, s, a, k); break;
-    default: ShapeKernels<16, 64>::substep(g, s, a, k); break;
+    case 0: ShapeKernels<2>::substep(g, s, a, k); break;
+    case 1: ShapeKernels<4>::substep(g, s, a, k); break;
+    case 2: ShapeKernels<8>::substep(g, s, a, k); break;
+    default: ShapeKernels<16>::substep(g, s, a, k); break;
     }
 }
 static void launch_end(int shape, dim3 g, hipStream_t s, const BatchArgs &a, int k)
 {
     switch (shape) {
-    case 0: ShapeKernels<16, 32>::end(g, s, a, k); break;
-    case 1: ShapeKernels<8, 64>::end(g, s, a, k); break;
-    case 2: ShapeKernels<32, 16>::end(g, s, a, k); break;
-    default: ShapeKernels<16, 64>::end(g, s, a, k); break;
+    case 0: ShapeKernels<2>::end(g, s, a, k); break;
+    case 1: ShapeKernels<4>::end(g, s, a, k); break;
+    case 2: ShapeKernels<8>::end(g, s, a, k); break;
+    default: ShapeKernels<16>::end(g, s, a, k); break;
     }
 }
 

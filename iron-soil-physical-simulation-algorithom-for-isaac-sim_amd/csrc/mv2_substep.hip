@@ -28,7 +28,7 @@
 //              quanta (never active in exact arithmetic for CFL <= 0.25), so
 //              thickness stays non-negative without a mass-creating clamp.
 #include "mv2_internal.hpp"
-#include "mv2_tile32.hpp"
+#include "mv2_strip32.hpp"
 
 #include <cmath>
 
@@ -172,62 +172,6 @@ __device__ inline Face64 face64(double bl, double br, double hl, double hr,
 }
 
 // ---------------------------------------------------------------------------
-// FP32 face: same equations in float, velocities from q/h via the hardware
-// reciprocal, pressure correction through the shared p - ps form, mass flux
-// rounded once to whole quanta.  Deterministic: both cells that share the
-// face evaluate it from identical operands with identical instructions.
-// ---------------------------------------------------------------------------
-template <int NORMAL>
-__device__ inline Face32 face32(float bl, float br, int hql, int hqr, float hl, float hr,
-                                float qxl, float qyl, float qxr, float qyr,
-                                const Consts32 &k, float factor, int clamp_quarter)
-{
-    float head_l = bl + k.K * hl;
-    float head_r = br + k.K * hr;
-    float b_face = fmaxf(bl, br);
-    float hs_l = fmaxf((head_l - b_face) * k.invK, 0.0f);
-    float hs_r = fmaxf((head_r - b_face) * k.invK, 0.0f);
-    // reconstructed velocity equals the cell velocity; zero when either
-    // the cell or its reconstruction is dry (reference: scale=0 / v=0)
-    float il = rcpf(hl), ir = rcpf(hr);
-    bool wl = (hl > k.dry) && (hs_l > k.dry);
-    bool wr = (hr > k.dry) && (hs_r > k.dry);
-    float vxl = wl ? qxl * il : 0.0f, vyl = wl ? qyl * il : 0.0f;
-    float vxr = wr ? qxr * ir : 0.0f, vyr = wr ? qyr * ir : 0.0f;
-    float vnl = NORMAL == 0 ? vxl : vyl, vtl = NORMAL == 0 ? vyl : vxl;
-    float vnr = NORMAL == 0 ? vxr : vyr, vtr = NORMAL == 0 ? vyr : vxr;
-    float qnl = hs_l * vnl, qtl = hs_l * vtl;
-    float qnr = hs_r * vnr, qtr = hs_r * vtr;
-    float psl = (k.c * hs_l) * hs_l, psr = (k.c * hs_r) * hs_r;
-    float fln = qnl * vnl + psl, frn = qnr * vnr + psr;
-    float flt = qnl * vtl, frt = qnr * vtr;
-    float speed = fmaxf(fabsf(vnl) + __builtin_sqrtf(k.Kg * hs_l),
-                        fabsf(vnr) + __builtin_sqrtf(k.Kg * hs_r));
-    float hsp = 0.5f * speed;
-    float fh = 0.5f * (qnl + qnr) - hsp * (hs_r - hs_l);
-    float fn = 0.5f * (fln + frn) - hsp * (qnr - qnl);
-    float ft = 0.5f * (flt + frt) - hsp * (qtr - qtl);
-    float pl = (k.c * hl) * hl, pr = (k.c * hr) * hr;
-    Face32 o;
-    float mq = rintf(fh * factor * (float)kInvQuantum);
-    int q = (int)mq;
-    if (clamp_quarter) {
-        if (q > 0) q = min(q, hql >> 2);
-        else q = max(q, -(hqr >> 2));
-    }
-    o.h = q;
-    float nl = (fn + (pl - psl)) * factor;
-    float nr = (fn + (pr - psr)) * factor;
-    float tt = ft * factor;
-    if (NORMAL == 0) {
-        o.qxl = nl; o.qyl = tt; o.qxr = nr; o.qyr = tt;
-    } else {
-        o.qxl = tt; o.qyl = nl; o.qxr = tt; o.qyr = nr;
-    }
-    return o;
-}
-
-// ---------------------------------------------------------------------------
 // per-cell work of one substep
 // ---------------------------------------------------------------------------
 struct CellOut {
@@ -314,95 +258,10 @@ __device__ inline CellOut cell64(const V2Args<F64P> &a, int src, int r, int c, d
     return out;
 }
 
-__device__ inline CellOut cell32(const V2Args<F32P> &a, int src, int r, int c, double sub_dt,
-                                 Acc<F32P> &acc)
-{
-    const int nx = a.nx, ny = a.ny;
-    const int64_t i = (int64_t)r * nx + c;
-    Consts32 k;
-    k.K = (float)a.cfg.earth_pressure;
-    k.invK = (float)(1.0 / a.cfg.earth_pressure);
-    k.Kg = (float)(a.cfg.earth_pressure * a.cfg.gravity);
-    k.c = (float)(0.5 * a.cfg.earth_pressure * a.cfg.gravity);
-    k.dry = (float)a.cfg.dry_tolerance;
-    const float *b = a.b;
-    const int32_t *h = a.h[src];
-    const float *qx = a.qx[src];
-    const float *qy = a.qy[src];
-    const float fxf = (float)(sub_dt / a.dx), fyf = (float)(sub_dt / a.dy);
-    const float Qf = (float)kQuantum;
-    float bi = b[i];
-    int hqi = h[i];
-    float hi = (float)hqi * Qf;
-    float qxi = qx[i], qyi = qy[i];
-    long long dq = 0;
-    float dqx = 0.0f, dqy = 0.0f;
-    const int cq = a.clamp_quarter;
-    if (c + 1 < nx) {
-        int hq2 = h[i + 1];
-        Face32 f = face32<0>(bi, b[i + 1], hqi, hq2, hi, (float)hq2 * Qf, qxi, qyi, qx[i + 1], qy[i + 1], k, fxf, cq);
-        dq -= f.h; dqx = dqx - f.qxl; dqy = dqy - f.qyl;
-    }
-    if (c > 0) {
-        int hq2 = h[i - 1];
-        Face32 f = face32<0>(b[i - 1], bi, hq2, hqi, (float)hq2 * Qf, hi, qx[i - 1], qy[i - 1], qxi, qyi, k, fxf, cq);
-        dq += f.h; dqx = dqx + f.qxr; dqy = dqy + f.qyr;
-    }
-    if (r + 1 < ny) {
-        int hq2 = h[i + nx];
-        Face32 f = face32<1>(bi, b[i + nx], hqi, hq2, hi, (float)hq2 * Qf, qxi, qyi, qx[i + nx], qy[i + nx], k, fyf, cq);
-        dq -= f.h; dqx = dqx - f.qxl; dqy = dqy - f.qyl;
-    }
-    if (r > 0) {
-        int hq2 = h[i - nx];
-        Face32 f = face32<1>(b[i - nx], bi, hq2, hqi, (float)hq2 * Qf, hi, qx[i - nx], qy[i - nx], qxi, qyi, k, fyf, cq);
-        dq += f.h; dqx = dqx + f.qxr; dqy = dqy + f.qyr;
-    }
-    long long hn = (long long)hqi + dq;
-    CellOut out;
-    out.overflow = hn > 2147483647LL;
-    out.hmin = (double)hn * kQuantum;
-    int hq = (int)(hn < 0 ? 0 : (hn > 2147483647LL ? 2147483647LL : hn));
-    float qxn = qxi + dqx, qyn = qyi + dqy;
-    acc.diag[0] += (double)(qxn - qxi);
-    acc.diag[1] += (double)(qyn - qyi);
-    float depth = (float)hq * Qf;
-    float qax = 0.0f, qay = 0.0f, wave_v = 0.0f;
-    if (depth > k.dry) {
-        const float dt = (float)sub_dt;
-        const float ax = (float)a.cfg.external_ax, ay = (float)a.cfg.external_ay;
-        float inv = rcpf(depth);
-        float vbx = qxn * inv, vby = qyn * inv;
-        float vex = vbx + ax * dt, vey = vby + ay * dt;
-        float speed = __builtin_sqrtf(vex * vex + vey * vey);
-        float decrement = (float)(a.cfg.basal_friction * a.cfg.gravity) * dt;
-        float factor = speed > 1.0e-15f ? fmaxf(0.0f, 1.0f - decrement * rcpf(speed)) : 0.0f;
-        float vax = vex * factor, vay = vey * factor;
-        qax = depth * vax;
-        qay = depth * vay;
-        wave_v = speed * factor;
-        double dd = (double)depth;
-        acc.diag[2] += dd * (double)(vex - vbx);
-        acc.diag[3] += dd * (double)(vey - vby);
-        acc.diag[6] += dd * (0.5 * (double)(vbx + vex) * a.cfg.external_ax +
-                             0.5 * (double)(vby + vey) * a.cfg.external_ay) * sub_dt;
-        acc.diag[4] += (double)qax - dd * (double)vex;
-        acc.diag[5] += (double)qay - dd * (double)vey;
-        acc.diag[7] += dd * ((double)speed * speed - ((double)vax * vax + (double)vay * vay));
-    }
-    acc.diag[8] += (double)((depth > k.dry ? qxn : 0.0f) - qxi);
-    acc.diag[9] += (double)((depth > k.dry ? qyn : 0.0f) - qyi);
-    const int dst = src ^ 1;
-    a.h[dst][i] = hq;
-    a.qx[dst][i] = qax;
-    a.qy[dst][i] = qay;
-    out.wave = (double)(wave_v + __builtin_sqrtf(k.Kg * depth));
-    return out;
-}
-
-// FP32 tile of the single-terrain step (faces in mv2_tile32.hpp)
-constexpr int kTH = 16, kTW = 32;
-using Tile32 = TileT<kTH, kTW>;
+// FP32 tiles of the single-terrain step: kStripW columns by kTileRows rows,
+// one band of kSH rows per wave (faces in mv2_strip32.hpp)
+constexpr int kSH = 4;
+constexpr int kTileRows = kSH * (kBlock / kWave);
 
 // Workgroups are dispatched round-robin over the 8 XCDs.  Renumber them so
 // that the workgroups of one XCD own one contiguous band of tiles: halos
@@ -416,18 +275,6 @@ __device__ inline void xcd_tile_range(int tiles, int &first, int &last)
     last = (int)((long long)(pb + 1) * tiles / G);
 }
 
-template <class P>
-__device__ inline CellOut cell_update(const V2Args<P> &a, int src, int r, int c, double sub_dt, Acc<P> &acc);
-template <>
-__device__ inline CellOut cell_update<F64P>(const V2Args<F64P> &a, int src, int r, int c, double sub_dt, Acc<F64P> &acc)
-{
-    return cell64(a, src, r, c, sub_dt, acc);
-}
-template <>
-__device__ inline CellOut cell_update<F32P>(const V2Args<F32P> &a, int src, int r, int c, double sub_dt, Acc<F32P> &acc)
-{
-    return cell32(a, src, r, c, sub_dt, acc);
-}
 
 // ---------------------------------------------------------------------------
 // Ledger passes.  Each thread takes kLedgerU cells per grid stride and loads
@@ -470,7 +317,7 @@ __device__ inline double cell_wave(const V2Args<P> &a, const CellVals &v)
             vx = (float)v.qx * inv;
             vy = (float)v.qy * inv;
         }
-        return (double)(__builtin_sqrtf(vx * vx + vy * vy) + __builtin_sqrtf((float)(K * g) * d));
+        return (double)(sqrtf_hw(vx * vx + vy * vy) + sqrtf_hw((float)(K * g) * d));
     } else {
         const double d = v.depth;
         double vx = d > dry ? v.qx / d : 0.0;
@@ -652,10 +499,6 @@ __global__ void __launch_bounds__(kBlock) v2_begin_kernel(V2Args<P> a)
 // its tile from that buffer before the loop state and the CFL maximum arrive,
 // so all three loads share one memory round trip; launches expected to find
 // the loop ended (eager = 0) read the loop state first and skip the staging.
-// LDS of the substep launch: the FP32 tile, nothing for the FP64 gather path
-template <class P> struct TileStore { int unused; };
-template <> struct TileStore<F32P> { Tile32 tile; };
-
 template <class P>
 __global__ void __launch_bounds__(kBlock) v2_substep_kernel(V2Args<P> a, int k, int eager)
 {
@@ -665,13 +508,16 @@ __global__ void __launch_bounds__(kBlock) v2_substep_kernel(V2Args<P> a, int k, 
     const bool lead = blockIdx.x == 0 && threadIdx.x == 0;
     const int src = k & 1;
     const int ny = a.ny, nx = a.nx;
-    const int tiles_x = (nx + kTW - 1) / kTW;
-    const int tiles = tiles_x * ((ny + kTH - 1) / kTH);
+    const int tiles_x = (nx + kStripW - 1) / kStripW;
+    const int tiles = tiles_x * ((ny + kTileRows - 1) / kTileRows);
     int t = 0;
     if constexpr (P::kQuant) {
         int last;
         xcd_tile_range(tiles, t, last);
     }
+    // this wave's band of the tile
+    const int r0 = (t / tiles_x) * kTileRows + wave_index() * kSH;
+    const int c0 = (t % tiles_x) * kStripW;
     // requests that do not depend on the loop state
     double slot = 0.0;
     if (threadIdx.x < kWave)
@@ -679,9 +525,9 @@ __global__ void __launch_bounds__(kBlock) v2_substep_kernel(V2Args<P> a, int k, 
     double dprev[kDiag];
     if (threadIdx.x == 0)
         for (int d = 0; d < kDiag; d++) dprev[d] = a.diag_part[(int64_t)d * a.diag_groups + blockIdx.x];
-    __shared__ TileStore<P> store;
+    StripRows<kSH> rows;
     if constexpr (P::kQuant) {
-        if (eager) tile_stage<kTH, kTW>(a, store.tile, src, (t / tiles_x) * kTH, (t % tiles_x) * kTW);
+        if (eager) strip_load<kSH>(a, src, r0, c0, rows);
     }
     Control C = resolve_control(s, k);
     if (!C.done && k == 0 && cfg.static_exit) {
@@ -704,7 +550,7 @@ __global__ void __launch_bounds__(kBlock) v2_substep_kernel(V2Args<P> a, int k, 
         return;
     }
     if constexpr (P::kQuant) {
-        if (!eager) tile_stage<kTH, kTW>(a, store.tile, src, (t / tiles_x) * kTH, (t % tiles_x) * kTW);
+        if (!eager) strip_load<kSH>(a, src, r0, c0, rows);
     }
     // CFL maximum of the current state (mobile_v2_reference.py:239-251)
     if (threadIdx.x < kWave) {
@@ -745,13 +591,21 @@ __global__ void __launch_bounds__(kBlock) v2_substep_kernel(V2Args<P> a, int k, 
     double hmin = 0.0;
     int ovf = 0;
     if constexpr (P::kQuant) {
-        tile_update<kTH, kTW>(a, store.tile, src, (t / tiles_x) * kTH, (t % tiles_x) * kTW, sub_dt, acc.diag, wmax[0],
-                              hmin, ovf);
+        float dg[kDiag];
+#pragma unroll
+        for (int d = 0; d < kDiag; d++) dg[d] = 0.0f;
+        float wv = 0.0f;
+        long long hn_min = 0;
+        strip_update<kSH>(a, rows, src, r0, c0, sub_dt, dg, wv, hn_min, ovf);
+#pragma unroll
+        for (int d = 0; d < kDiag; d++) acc.diag[d] += (double)dg[d];
+        hmin = (double)hn_min * kQuantum;
+        wmax[0] = (double)wv;
     } else {
         const int n = ny * nx;
         for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
             const int r = i / nx, c = i - r * nx;
-            CellOut o = cell_update<P>(a, src, r, c, sub_dt, acc);
+            CellOut o = cell64(a, src, r, c, sub_dt, acc);
             wmax[0] = npmax(wmax[0], o.wave);
             hmin = npmin(hmin, o.hmin);
             ovf |= o.overflow;
@@ -906,7 +760,7 @@ static int launch_step(mv2_context *ctx, const mv2_step_config *cfg, mv2_step_re
 {
     const int64_t n = (int64_t)ctx->ny * ctx->nx;
     const int gs = grid_for(n, kBlock * kLedgerU, kFinalizeBlock);   // begin / end passes
-    const int gu = P::kQuant ? ((ctx->nx + kTW - 1) / kTW) * ((ctx->ny + kTH - 1) / kTH)
+    const int gu = P::kQuant ? ((ctx->nx + kStripW - 1) / kStripW) * ((ctx->ny + kTileRows - 1) / kTileRows)
                              : grid_for(n);                              // substep launches
     if (int rc = ensure_scratch(ctx, ctx->partials, (sizeof(double) * kLedgerF + sizeof(long long) * kLedgerQ) * (size_t)gs))
         return rc;

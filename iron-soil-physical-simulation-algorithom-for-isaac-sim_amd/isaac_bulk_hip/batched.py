@@ -29,7 +29,7 @@ from .config import MobileV2Config
 from .mobile_v2 import _step_config
 from .runtime import HipRuntime
 
-TILE_SHAPES = {0: (16, 32), 1: (8, 64), 2: (32, 16), 3: (16, 64)}
+TILE_SHAPES = {0: (8, 62), 1: (16, 62), 2: (32, 62), 3: (64, 62)}    # rows x columns
 
 
 def _torch():
@@ -51,6 +51,45 @@ class BatchStepRecord:
     momentum_after: tuple
     kinetic_before: float
     kinetic_after: float
+
+
+# numpy view of mv2_batch_record (include/mv2.h)
+_RECORD_DTYPE = np.dtype({
+    "names": ["substeps", "numeric_error", "static_exit", "complete", "maximum_cfl", "remaining",
+              "mass_quanta_before", "mass_quanta_after", "momentum_before", "momentum_after",
+              "kinetic_before", "kinetic_after"],
+    "formats": ["<i4", "<i4", "<i4", "<i4", "<f8", "<f8", "<f8", "<f8", ("<f8", 2), ("<f8", 2), "<f8", "<f8"],
+    "offsets": [getattr(_lib.BatchRecord, f).offset for f, _ in _lib.BatchRecord._fields_],
+    "itemsize": ctypes.sizeof(_lib.BatchRecord),
+})
+
+
+class BatchRecords:
+    """The records of one batched step: a sequence of BatchStepRecord, built on
+    access, over a numpy array whose columns (``substeps``, ``mass_quanta_before``
+    ...) are available as attributes without building the records."""
+
+    def __init__(self, table: np.ndarray) -> None:
+        self.table = table
+
+    def __len__(self) -> int:
+        return int(self.table.shape[0])
+
+    def __getitem__(self, i: int) -> BatchStepRecord:
+        r = self.table[i]
+        return BatchStepRecord(int(r["substeps"]), int(r["numeric_error"]), bool(r["static_exit"]),
+                               float(r["maximum_cfl"]), float(r["remaining"]), int(r["mass_quanta_before"]),
+                               int(r["mass_quanta_after"]), tuple(float(v) for v in r["momentum_before"]),
+                               tuple(float(v) for v in r["momentum_after"]), float(r["kinetic_before"]),
+                               float(r["kinetic_after"]))
+
+    def __iter__(self):
+        return (self[i] for i in range(len(self)))
+
+    def __getattr__(self, name: str) -> np.ndarray:
+        if name in _RECORD_DTYPE.names:
+            return self.table[name]
+        raise AttributeError(name)
 
 
 def to_quanta(h: np.ndarray) -> np.ndarray:
@@ -108,22 +147,19 @@ class BatchedMobileV2:
         self.tile_shape = int(shape)
         self.rt.check(self.rt.lib.mv2_batch_set_tile_shape(self.rt.ctx, self.tile_shape))
 
-    def step(self, dt: float, *, limit_mode: int = 0) -> list:
+    def step(self, dt: float, *, limit_mode: int = 0) -> BatchRecords:
         """Advance every instance by dt; one record per instance."""
         rt = self.rt
         rt.bind_stream()
         cfg = _step_config(self.config, dt, density=1.0, limit_mode=limit_mode, weight_mode=0)
-        recs = (_lib.BatchRecord * self.count)()
+        table = np.zeros(self.count, _RECORD_DTYPE)
         ms = (ctypes.c_double * 3)()
-        rc = rt.lib.mv2_batch_step(rt.ctx, ctypes.byref(cfg), int(self.use_graph), recs, ms)
+        rc = rt.lib.mv2_batch_step(rt.ctx, ctypes.byref(cfg), int(self.use_graph), table.ctypes.data, ms)
         self.last_kernel_ms, self.last_substep_ms, self.last_launches = ms[0], ms[1], int(ms[2])
         if rc == _lib.MV2_ENUMERIC:
             raise RuntimeError(_lib.last_error(rt.ctx))
         rt.check(rc)
-        return [BatchStepRecord(r.substeps, r.numeric_error, bool(r.static_exit), r.maximum_cfl, r.remaining,
-                                int(r.mass_quanta_before), int(r.mass_quanta_after),
-                                tuple(r.momentum_before), tuple(r.momentum_after), r.kinetic_before,
-                                r.kinetic_after) for r in recs]
+        return BatchRecords(table)
 
     def fields(self, i: int):
         """(b_eff, h_mobile [m], qx, qy) of instance i as numpy arrays."""

@@ -178,3 +178,31 @@ def test_committed_warm_start_weights_load():
     # zero-sum taps on the b_eff channel (constant-shift invariance)
     assert np.abs(params["w1"][:, :, 0].sum(axis=0)).max() < 1e-5
     assert all(np.isfinite(v).all() for v in params.values())
+
+
+def test_batched_tile_shapes_match_library():
+    from isaac_bulk_hip.batched import TILE_SHAPES
+
+    lib = _lib.load()
+    for shape, (th, tw) in TILE_SHAPES.items():
+        a, b = ctypes.c_int(), ctypes.c_int()
+        assert lib.mv2_batch_tile_shape(shape, ctypes.byref(a), ctypes.byref(b)) == 0
+        assert (a.value, b.value) == (th, tw)
+    assert lib.mv2_batch_tile_shape(len(TILE_SHAPES), ctypes.byref(a), ctypes.byref(b)) != 0
+
+
+def test_batch_records_view_matches_ctypes_layout():
+    from isaac_bulk_hip.batched import _RECORD_DTYPE, BatchRecords
+
+    raw = (_lib.BatchRecord * 2)()
+    raw[1].substeps = 7
+    raw[1].static_exit = 1
+    raw[1].mass_quanta_after = 12345.0
+    raw[1].momentum_after[1] = -2.5
+    raw[1].kinetic_before = 0.25
+    table = np.frombuffer(bytes(raw), dtype=_RECORD_DTYPE)
+    recs = BatchRecords(table)
+    assert len(recs) == 2 and recs.substeps.tolist() == [0, 7]
+    r = recs[1]
+    assert r.static_exit and r.mass_quanta_after == 12345 and r.momentum_after == (0.0, -2.5)
+    assert r.kinetic_before == 0.25 and [x.substeps for x in recs] == [0, 7]
