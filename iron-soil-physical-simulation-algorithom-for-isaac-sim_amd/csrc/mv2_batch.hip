@@ -144,20 +144,6 @@ __global__ void batch_reset_kernel(BatchArgs a)
     for (int64_t i = tid; i < 2 * (int64_t)a.count * kInstSlots; i += nth) a.stat[i * kInstStride] = 0ull;
 }
 
-// double-precision value of lane l + 1 / lane l - 1 (two 32-bit DPP moves)
-__device__ inline double from_next_lane_d(double v)
-{
-    const unsigned long long u = __builtin_bit_cast(unsigned long long, v);
-    const int lo = from_next_lane((int)(unsigned)u), hi = from_next_lane((int)(unsigned)(u >> 32));
-    return __builtin_bit_cast(double, ((unsigned long long)(unsigned)hi << 32) | (unsigned)lo);
-}
-__device__ inline double from_prev_lane_d(double v)
-{
-    const unsigned long long u = __builtin_bit_cast(unsigned long long, v);
-    const int lo = from_prev_lane((int)(unsigned)u), hi = from_prev_lane((int)(unsigned)(u >> 32));
-    return __builtin_bit_cast(double, ((unsigned long long)(unsigned)hi << 32) | (unsigned)lo);
-}
-
 // One tile: ledger before, static diagnostic, CFL maximum of the input.  The
 // tile's waves use the strip layout of the substep: lanes 1..62 own the
 // columns, a lane holds its column's rows, and the static gradient takes its
@@ -165,7 +151,7 @@ __device__ inline double from_prev_lane_d(double v)
 template <int SH>
 __global__ void __launch_bounds__(kBlock) batch_begin_kernel(BatchArgs a)
 {
-    __shared__ double smem[64 * 4];
+    __shared__ double lds[6 * kBlock];
     const int g = blockIdx.x;
     const int i = a.tile_inst[g];
     const InstView v(a, i);
@@ -184,15 +170,7 @@ __global__ void __launch_bounds__(kBlock) batch_begin_kernel(BatchArgs a)
     double s[kBLedger] = {0.0, 0.0, 0.0};
     long long q[1] = {0};
     double m[3] = {0.0, 0.0, 0.0};                // max |grad H|^2, max |q|, max wave speed
-    // The static test only asks whether every |grad H| <= tan_start + 1e-14.
-    // |grad H| is taken with reciprocal multiplies (a few ulps from the
-    // quotients); a cell within a relative 1e-12 of the bound is redone with
-    // the quotients, so the test decides exactly as with quotients everywhere.
-    const double bound = a.cfg.tan_start + 1.0e-14;
-    const double band_lo = bound * bound * (1.0 - 1.0e-12), band_hi = bound * bound * (1.0 + 1.0e-12);
-    const double denx = (gc > 0 && gc + 1 < nx) ? 2.0 * a.dx : a.dx;
-    const double inv_denx = 1.0 / denx;
-    const double inv_dy2 = 1.0 / (2.0 * a.dy), inv_dy1 = 1.0 / a.dy;
+    const StaticGrad sg(a.cfg.tan_start, a.dx, a.dy, gc > 0 && gc + 1 < nx);
     const float Kg = uniform((float)(K * gr)), dryf = uniform((float)dry);
 #pragma unroll
     for (int j = 1; j <= SH; j++) {
@@ -217,30 +195,23 @@ __global__ void __launch_bounds__(kBlock) batch_begin_kernel(BatchArgs a)
             // central differences inside, one-sided at the terrain edge
             const double hl = gc > 0 ? HL : H[j], hr = gc + 1 < nx ? HR : H[j];
             const double hd = r > 0 ? H[j - 1] : H[j], hu = r + 1 < ny ? H[j + 1] : H[j];
-            const bool inner_y = r > 0 && r + 1 < ny;
-            const double gx = (hr - hl) * inv_denx, gy = (hu - hd) * (inner_y ? inv_dy2 : inv_dy1);
-            double g2 = gx * gx + gy * gy;
-            if (g2 >= band_lo && g2 <= band_hi) {
-                const double deny = inner_y ? 2.0 * a.dy : a.dy;
-                const double ex = (hr - hl) / denx, ey = (hu - hd) / deny;
-                g2 = ex * ex + ey * ey;
-            }
+            const double g2 = sg.grad2(hl, hr, hd, hu, r > 0 && r + 1 < ny);
             m[0] = npmax(m[0], g2);
             m[1] = npmax(m[1], fabs((double)qxv));
             m[1] = npmax(m[1], fabs((double)qyv));
         }
     }
-    block_reduce<double, kBLedger, 0>(s, smem);
-    block_reduce<long long, 1, 0>(q, reinterpret_cast<long long *>(smem));
-    block_reduce<double, 3, 1>(m, smem);
+    double f6[6] = {s[0], s[1], s[2], m[0], m[1], m[2]};
+    block_fold<double, 6, 0x38u>(f6, lds);
+    block_fold<long long, 1>(q, reinterpret_cast<long long *>(lds));
     if (threadIdx.x == 0) {
-        for (int f = 0; f < kBLedger; f++) a.ledger[(int64_t)f * a.tiles + g] = s[f];
+        for (int f = 0; f < kBLedger; f++) a.ledger[(int64_t)f * a.tiles + g] = f6[f];
         a.quanta[g] = q[0];
-        inst_max(inst_set(a.wave, 0, a.count, i), m[2]);
+        inst_max(inst_set(a.wave, 0, a.count, i), f6[5]);
         if (a.cfg.static_exit) {
             // the square root is monotonic: the root of the largest square is the largest root
-            inst_max(inst_set(a.stat, 0, a.count, i), sqrt(m[0]));
-            inst_max(inst_set(a.stat, 1, a.count, i), m[1]);
+            inst_max(inst_set(a.stat, 0, a.count, i), sqrt(f6[3]));
+            inst_max(inst_set(a.stat, 1, a.count, i), f6[4]);
         }
     }
 }
@@ -332,7 +303,7 @@ __global__ void __launch_bounds__(kBlock) batch_substep_kernel(BatchArgs a, int 
 template <int SH>
 __global__ void __launch_bounds__(kBlock) batch_end_kernel(BatchArgs a, int k)
 {
-    __shared__ double smem[64 * 4];
+    __shared__ double lds[kBLedger * kBlock];
     const int g = blockIdx.x;
     const int i = a.tile_inst[g];
     const BControl C = resolve_bcontrol(a, k, i);
@@ -380,8 +351,8 @@ __global__ void __launch_bounds__(kBlock) batch_end_kernel(BatchArgs a, int k)
             s[2] += (double)(0.5f * (qxv[j] * (qxv[j] * inv) + qyv[j] * (qyv[j] * inv)));
         }
     }
-    block_reduce<double, kBLedger, 0>(s, smem);
-    block_reduce<long long, 1, 0>(q, reinterpret_cast<long long *>(smem));
+    block_fold<double, kBLedger>(s, lds);
+    block_fold<long long, 1>(q, reinterpret_cast<long long *>(lds));
     if (threadIdx.x == 0) {
         for (int f = 0; f < kBLedger; f++) a.ledger[(int64_t)(kBLedger + f) * a.tiles + g] = s[f];
         a.quanta[a.tiles + g] = q[0];
@@ -731,11 +702,18 @@ extern "C" int mv2_batch_step(mv2_context *ctx, const mv2_step_config *cfg, int 
     double sub_ms = 0.0;
     for (;;) {
         MV2_HIP(ctx, hipEventRecord(bs->ev_sub0, st));
-        if (use_graph) {
-            const int reps = (want + 7) / 8;
-            for (int r = 0; r < reps; r++, k += 8) MV2_HIP(ctx, hipGraphLaunch(bs->graph, st));
-        } else {
-            for (int j = 0; j < want; j++, k++) launch_substep(bs->shape, grid, st, a, k % 8);
+        // whole groups of eight from the graph when the phase allows, the rest
+        // as single launches
+        for (int left = want; left > 0;) {
+            if (use_graph && k % 8 == 0 && left >= 8) {
+                MV2_HIP(ctx, hipGraphLaunch(bs->graph, st));
+                k += 8;
+                left -= 8;
+            } else {
+                launch_substep(bs->shape, grid, st, a, k % 8);
+                k++;
+                left--;
+            }
         }
         MV2_HIP(ctx, hipEventRecord(bs->ev_sub1, st));
         launch_end(bs->shape, grid, st, a, k % 8);

@@ -59,6 +59,9 @@ struct mv2_context {
     int timing = 0;                       // bracket each substep launch with events
     std::vector<hipEvent_t> sub_events;   // event pairs of the substep launches
     int v2_predicted = 0;                 // substep launches to enqueue up front
+    int v2_frame = 0;                     // steps taken (the parity picks the loop-state block)
+    int v2_clean = -1;                    // parity whose loop state is known zeroed, -1 none
+    int v2_last_sync = 0;                 // parity of the last completed step
     int relax_predicted = 0;              // relaxation launches to enqueue up front
     mv2_batch_state *batch = nullptr;     // batched terrains bound to this context
 };
@@ -189,6 +192,39 @@ __device__ inline void block_reduce(T (&v)[NV], T *smem)
     __syncthreads();
 }
 
+// Block-wide reduction of NV values, field-parallel: every thread stores its
+// values to LDS, wave w folds fields w, w + NW, ... across the block (lane l
+// takes thread l of every wave) and one DPP reduction per field finishes it.
+// Field f is a maximum if bit f of MAXM is set, a minimum if bit f of MINM is
+// set, a sum otherwise.  `lds` holds NV * NW * 64 values.  Result in thread 0.
+template <class T, int NV, unsigned MAXM = 0u, unsigned MINM = 0u, int NW = kBlock / kWave>
+__device__ inline void block_fold(T (&v)[NV], T *lds)
+{
+    constexpr int NT = NW * kWave;
+    const int lane = threadIdx.x & (kWave - 1);
+    const int wave = threadIdx.x / kWave;
+#pragma unroll
+    for (int f = 0; f < NV; f++) lds[f * NT + threadIdx.x] = v[f];
+    __syncthreads();
+    for (int f = wave; f < NV; f += NW) {
+        const bool mx = (MAXM >> f) & 1u, mn = (MINM >> f) & 1u;
+        T x = lds[f * NT + lane];
+#pragma unroll
+        for (int w = 1; w < NW; w++) {
+            const T y = lds[f * NT + w * kWave + lane];
+            x = mx ? red_op<1>(x, y) : (mn ? red_op<2>(x, y) : red_op<0>(x, y));
+        }
+        x = mx ? wave_reduce<1>(x) : (mn ? wave_reduce<2>(x) : wave_reduce<0>(x));
+        if (lane == 0) lds[f * NT] = x;          // only this wave touches field f
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int f = 0; f < NV; f++) v[f] = lds[f * NT];
+    }
+    __syncthreads();
+}
+
 // Triangle-A-C vertex weight (bulk_state_authority.py:624-634), same
 // accumulation order as the reference's four slice additions.
 __host__ __device__ inline double vertex_weight(int ny, int nx, double f6, int r, int c)
@@ -239,11 +275,13 @@ __device__ inline void atomic_max_nonneg(unsigned long long *slot, double v)
 constexpr int kSpread = 64;
 constexpr int kSpreadStride = 16;                    // u64 words between slots
 constexpr int kSpreadWords = kSpread * kSpreadStride;
-// the step uses sets 0..4, the envelope 5..7, the relaxation 8..10, the labelling 12..14
+// the step uses sets 0..4 (even frames) and 16..20 (odd frames), the envelope
+// 5..7, the relaxation 8..10, the labelling 12..14
 constexpr int kEnvSpreadSet = 5;
 constexpr int kRelaxSpreadSet = 8;
 constexpr int kLabelSpreadSet = 12;
-constexpr size_t kSpreadScratchBytes = (size_t)16 * kSpreadWords * 8;
+constexpr int kStepOddSpreadSet = 16;
+constexpr size_t kSpreadScratchBytes = (size_t)21 * kSpreadWords * 8;
 
 __device__ inline void spread_max(unsigned long long *slots, double v)
 {

@@ -54,6 +54,20 @@ __device__ inline float from_prev_lane(float v)
     return __builtin_bit_cast(float, from_prev_lane(__builtin_bit_cast(int, v)));
 }
 
+// double-precision value of lane l + 1 / lane l - 1 (two 32-bit DPP moves)
+__device__ inline double from_next_lane_d(double v)
+{
+    const unsigned long long u = __builtin_bit_cast(unsigned long long, v);
+    const int lo = from_next_lane((int)(unsigned)u), hi = from_next_lane((int)(unsigned)(u >> 32));
+    return __builtin_bit_cast(double, ((unsigned long long)(unsigned)hi << 32) | (unsigned)lo);
+}
+__device__ inline double from_prev_lane_d(double v)
+{
+    const unsigned long long u = __builtin_bit_cast(unsigned long long, v);
+    const int lo = from_prev_lane((int)(unsigned)u), hi = from_prev_lane((int)(unsigned)(u >> 32));
+    return __builtin_bit_cast(double, ((unsigned long long)(unsigned)hi << 32) | (unsigned)lo);
+}
+
 struct Face32 { int h; float qxl, qyl, qxr, qyr; };
 
 // FP32 constants of one substep, rounded once from the FP64 configuration and
@@ -131,7 +145,8 @@ __device__ inline Side32 side_from_next_lane(const Side32 &s)
 // multiplied by factor = dt / spacing.  Every product-sum is an explicit fused
 // multiply-add, so each evaluation of a face gives the same bits.
 template <int NORMAL>
-__device__ inline Face32 face32v(const Side32 &L, const Side32 &R, const Consts32 &k, float factor, float qfactor)
+__device__ inline Face32 face32v(const Side32 &L, const Side32 &R, const Consts32 &k, float factor, float hfactor,
+                                 float qhalf)
 {
     const float b_face = fmaxf(L.b, R.b);
     const float hs_l = fmaxf((L.head - b_face) * k.invK, 0.0f);
@@ -145,20 +160,21 @@ __device__ inline Face32 face32v(const Side32 &L, const Side32 &R, const Consts3
     const float fln = fmaf(qnl, vnl, psl), frn = fmaf(qnr, vnr, psr);
     const float flt = qnl * vtl, frt = qnr * vtr;
     const float speed = fmaxf(fabsf(vnl) + sqrtf_hw(k.Kg * hs_l), fabsf(vnr) + sqrtf_hw(k.Kg * hs_r));
-    const float nhsp = -0.5f * speed;
-    const float fh = fmaf(nhsp, hs_r - hs_l, 0.5f * (qnl + qnr));
-    const float fn = fmaf(nhsp, qnr - qnl, 0.5f * (fln + frn));
-    const float ft = fmaf(nhsp, qtr - qtl, 0.5f * (flt + frt));
+    // twice the Rusanov fluxes: F = (f_l + f_r) - speed (u_r - u_l); the
+    // halving is exact and is folded into the factors below
+    const float Fh = fmaf(-speed, hs_r - hs_l, qnl + qnr);
+    const float Fn = fmaf(-speed, qnr - qnl, fln + frn);
+    const float Ft = fmaf(-speed, qtr - qtl, flt + frt);
     Face32 o;
-    int q = (int)rintf(fh * qfactor);
+    int q = (int)rintf(Fh * qhalf);
     // outflow of a side is capped at a quarter of its quanta (q > 0 leaves
     // the left cell, q < 0 the right one); both caps are >= 0, so applying
     // both is the same as applying the one of the flow's direction
     if (k.clamp_quarter) q = max(min(q, L.q4), -R.q4);
     o.h = q;
-    const float nl = (fn + (L.p - psl)) * factor;
-    const float nr = (fn + (R.p - psr)) * factor;
-    const float tt = ft * factor;
+    const float nl = fmaf(0.5f, Fn, L.p - psl) * factor;
+    const float nr = fmaf(0.5f, Fn, R.p - psr) * factor;
+    const float tt = Ft * hfactor;
     if (NORMAL == 0) {
         o.qxl = nl; o.qyl = tt; o.qxr = nr; o.qyr = tt;
     } else {
@@ -166,6 +182,37 @@ __device__ inline Face32 face32v(const Side32 &L, const Side32 &R, const Consts3
     }
     return o;
 }
+
+// Squared gradient of H = b + h at one cell for the static test
+// max |grad H| <= tan_start + 1e-14 (warp_mobile_v2.py:56-85: centred
+// differences inside, one-sided at the terrain edge).  Taken with reciprocal
+// multiplies (a few ulps from the quotients); a cell within a relative 1e-12
+// of the bound is redone with the quotients, so the test decides exactly as
+// with quotients everywhere.
+struct StaticGrad {
+    double denx, inv_denx, dy, inv_dy1, inv_dy2, band_lo, band_hi;
+    __device__ StaticGrad(double tan_start, double dx, double dy_, bool inner_x)
+    {
+        const double bound = tan_start + 1.0e-14;
+        band_lo = bound * bound * (1.0 - 1.0e-12);
+        band_hi = bound * bound * (1.0 + 1.0e-12);
+        denx = inner_x ? 2.0 * dx : dx;
+        inv_denx = 1.0 / denx;
+        dy = dy_;
+        inv_dy1 = 1.0 / dy_;
+        inv_dy2 = 1.0 / (2.0 * dy_);
+    }
+    __device__ double grad2(double hl, double hr, double hd, double hu, bool inner_y) const
+    {
+        const double gx = (hr - hl) * inv_denx, gy = (hu - hd) * (inner_y ? inv_dy2 : inv_dy1);
+        double g2 = gx * gx + gy * gy;
+        if (g2 >= band_lo && g2 <= band_hi) {
+            const double ex = (hr - hl) / denx, ey = (hu - hd) / (inner_y ? 2.0 * dy : dy);
+            g2 = ex * ex + ey * ey;
+        }
+        return g2;
+    }
+};
 
 // the rows of one lane's column: band rows 1..SH, halo rows 0 and SH + 1
 template <int SH>
@@ -225,15 +272,17 @@ __device__ inline void strip_update(const A &a, const StripRows<SH> &R, int src,
     Side32 sd[SH + 2];
 #pragma unroll
     for (int j = 0; j < SH + 2; j++) sd[j] = side32(R.b[j], R.hq[j], R.qx[j], R.qy[j], k);
-    const float qfx = uniform(k.fxf * (float)kInvQuantum), qfy = uniform(k.fyf * (float)kInvQuantum);
+    // per direction: dt / spacing, half of it, half of it in quanta per metre
+    const float hfx = 0.5f * k.fxf, hfy = 0.5f * k.fyf;
+    const float qfx = uniform(hfx * (float)kInvQuantum), qfy = uniform(hfy * (float)kInvQuantum);
     // y-face between halo row 0 and band row 1
-    Face32 fdown = face32v<1>(sd[0], sd[1], k, k.fyf, qfy);
+    Face32 fdown = face32v<1>(sd[0], sd[1], k, k.fyf, hfy, qfy);
 #pragma unroll
     for (int j = 1; j <= SH; j++) {
         const int gr = r0 - 1 + j;
-        const Face32 fup = face32v<1>(sd[j], sd[j + 1], k, k.fyf, qfy);
+        const Face32 fup = face32v<1>(sd[j], sd[j + 1], k, k.fyf, hfy, qfy);
         // x-face between this lane's column and the next
-        const Face32 fx = face32v<0>(sd[j], side_from_next_lane(sd[j]), k, k.fxf, qfx);
+        const Face32 fx = face32v<0>(sd[j], side_from_next_lane(sd[j]), k, k.fxf, hfx, qfx);
         // the x-face on the left of this lane's column, from the lane before
         const int lq = from_prev_lane(fx.h);
         const float lqx = from_prev_lane(fx.qxr), lqy = from_prev_lane(fx.qyr);

@@ -31,6 +31,7 @@
 #include "mv2_strip32.hpp"
 
 #include <cmath>
+#include <cstdlib>
 
 namespace mv2 {
 
@@ -56,6 +57,11 @@ struct SyncBlock {
     int flags[2];                        // launch k ORs into [(k+1)&1]: bit 0 positivity, bit 1 range
     Control ctl[2];
 };
+// Steps alternate between two loop-state blocks (and two groups of spread
+// sets) by the parity of the step count; the finalize pass of a step zeroes
+// the other parity's block, so the next step needs no reset launch.
+constexpr size_t kSyncParityBytes = 256;
+static_assert(sizeof(SyncBlock) <= kSyncParityBytes, "two loop-state blocks fit below the relaxation's");
 
 // spread-maximum sets of one step (kSpreadWords u64 each): the CFL maximum
 // rotates over three sets (launch k reads k % 3, fills (k+1) % 3 and clears
@@ -97,6 +103,8 @@ struct V2Args {
     typename P::real *qy[2];
     mv2_step_config cfg;
     SyncBlock *sync;
+    SyncBlock *sync_next;                // the other parity's block, zeroed by the finalize pass
+    unsigned long long *spread_next;     // the other parity's spread sets, zeroed likewise
     double *ledger_f;                    // kLedgerF x groups of the begin / end passes
     long long *ledger_q;                 // kLedgerQ x groups
     int groups;                          // workgroups of the begin / end passes
@@ -283,7 +291,7 @@ __device__ inline void xcd_tile_range(int tiles, int &first, int &last)
 // trip instead of one per cell.
 // ---------------------------------------------------------------------------
 constexpr int kLedgerU = 4;
-constexpr int kFinalizeBlock = 512;
+constexpr int kFinalizeBlock = 256;
 
 struct CellVals {
     double b, depth, qx, qy;
@@ -416,17 +424,12 @@ __device__ inline Control resolve_control(const SyncBlock *s, int k)
     return C;
 }
 
-__global__ void v2_reset_kernel(SyncBlock *s, mv2_step_record *rec, double *diag_part, int ndiag,
-                                unsigned long long *spread)
+// zero one parity's loop state and spread sets (first step, or after an error)
+__global__ void v2_reset_kernel(SyncBlock *s, unsigned long long *spread)
 {
-    const int tid = blockIdx.x * blockDim.x + threadIdx.x;
-    const int nthreads = gridDim.x * blockDim.x;
     unsigned *ps = reinterpret_cast<unsigned *>(s);
-    for (int i = tid; i < (int)(sizeof(SyncBlock) / 4); i += nthreads) ps[i] = 0u;
-    unsigned *pr = reinterpret_cast<unsigned *>(rec);
-    for (int i = tid; i < (int)(sizeof(mv2_step_record) / 4); i += nthreads) pr[i] = 0u;
-    for (int i = tid; i < ndiag * kDiag; i += nthreads) diag_part[i] = 0.0;
-    for (int i = tid; i < kStepSpreadSets * kSpread; i += nthreads) spread[(int64_t)i * kSpreadStride] = 0ull;
+    for (int i = threadIdx.x; i < (int)(sizeof(SyncBlock) / 4); i += blockDim.x) ps[i] = 0u;
+    for (int i = threadIdx.x; i < kStepSpreadSets * kSpread; i += blockDim.x) spread[(int64_t)i * kSpreadStride] = 0ull;
 }
 
 // One pass over the input state: ledger before, static diagnostic
@@ -499,9 +502,10 @@ __global__ void __launch_bounds__(kBlock) v2_begin_kernel(V2Args<P> a)
 // its tile from that buffer before the loop state and the CFL maximum arrive,
 // so all three loads share one memory round trip; launches expected to find
 // the loop ended (eager = 0) read the loop state first and skip the staging.
-template <class P>
+template <class P, int SH = kSH>
 __global__ void __launch_bounds__(kBlock) v2_substep_kernel(V2Args<P> a, int k, int eager)
 {
+    constexpr int kRows = SH * (kBlock / kWave);     // rows of a tile
     __shared__ double smem[64 * 4];
     SyncBlock *s = a.sync;
     const mv2_step_config &cfg = a.cfg;
@@ -509,25 +513,27 @@ __global__ void __launch_bounds__(kBlock) v2_substep_kernel(V2Args<P> a, int k, 
     const int src = k & 1;
     const int ny = a.ny, nx = a.nx;
     const int tiles_x = (nx + kStripW - 1) / kStripW;
-    const int tiles = tiles_x * ((ny + kTileRows - 1) / kTileRows);
+    const int tiles = tiles_x * ((ny + kRows - 1) / kRows);
     int t = 0;
     if constexpr (P::kQuant) {
         int last;
         xcd_tile_range(tiles, t, last);
     }
     // this wave's band of the tile
-    const int r0 = (t / tiles_x) * kTileRows + wave_index() * kSH;
+    const int r0 = (t / tiles_x) * kRows + wave_index() * SH;
     const int c0 = (t % tiles_x) * kStripW;
     // requests that do not depend on the loop state
     double slot = 0.0;
     if (threadIdx.x < kWave)
         slot = __longlong_as_double((long long)a.spread[(k % kWaveSets) * kSpreadWords + threadIdx.x * kSpreadStride]);
+    // the workgroup's diagnostic slot accumulates over the working launches;
+    // launch 0 is the first of them and starts it from zero
     double dprev[kDiag];
     if (threadIdx.x == 0)
-        for (int d = 0; d < kDiag; d++) dprev[d] = a.diag_part[(int64_t)d * a.diag_groups + blockIdx.x];
-    StripRows<kSH> rows;
+        for (int d = 0; d < kDiag; d++) dprev[d] = k == 0 ? 0.0 : a.diag_part[(int64_t)d * a.diag_groups + blockIdx.x];
+    StripRows<SH> rows;
     if constexpr (P::kQuant) {
-        if (eager) strip_load<kSH>(a, src, r0, c0, rows);
+        if (eager) strip_load<SH>(a, src, r0, c0, rows);
     }
     Control C = resolve_control(s, k);
     if (!C.done && k == 0 && cfg.static_exit) {
@@ -550,7 +556,7 @@ __global__ void __launch_bounds__(kBlock) v2_substep_kernel(V2Args<P> a, int k, 
         return;
     }
     if constexpr (P::kQuant) {
-        if (!eager) strip_load<kSH>(a, src, r0, c0, rows);
+        if (!eager) strip_load<SH>(a, src, r0, c0, rows);
     }
     // CFL maximum of the current state (mobile_v2_reference.py:239-251)
     if (threadIdx.x < kWave) {
@@ -596,7 +602,7 @@ __global__ void __launch_bounds__(kBlock) v2_substep_kernel(V2Args<P> a, int k, 
         for (int d = 0; d < kDiag; d++) dg[d] = 0.0f;
         float wv = 0.0f;
         long long hn_min = 0;
-        strip_update<kSH>(a, rows, src, r0, c0, sub_dt, dg, wv, hn_min, ovf);
+        strip_update<SH>(a, rows, src, r0, c0, sub_dt, dg, wv, hn_min, ovf);
 #pragma unroll
         for (int d = 0; d < kDiag; d++) acc.diag[d] += (double)dg[d];
         hmin = (double)hn_min * kQuantum;
@@ -673,58 +679,265 @@ __global__ void __launch_bounds__(kBlock) v2_end_kernel(V2Args<P> a, int k)
     store_ledger<P>(a, L, false, smem);
 }
 
+// ---------------------------------------------------------------------------
+// FP32 ledger passes in the strip layout of the substep (one tile of the
+// substep grid per workgroup, one band of kSH rows per wave): x-neighbours
+// by DPP, y-neighbours from the rows in registers, no integer division.  Per
+// cell the ledger terms of warp_mobile_v2.py:41-54 are formed in float and
+// summed per lane in float over the band, then in double; the mobile volume
+// itself comes exactly from the quanta per weight class.
+// ---------------------------------------------------------------------------
+constexpr int kLedger32D = 7;      // FP64 fields per workgroup: ledger 1..7 of warp_mobile_v2.py:41-54
+
+struct Ledger32 {
+    float qx, qy, ke, pot, prs, vmax, hmin;   // sums weighted by the cell weight; max speed, min depth
+    // quanta of the band's cells by row class: first row, inner rows, last row
+    // (three scalars, not an array: a select between array slots would be
+    // turned into a dynamically indexed private array)
+    long long q_first, q_inner, q_last;
+    __device__ void init()
+    {
+        qx = qy = ke = pot = prs = vmax = 0.0f;
+        hmin = 3.0e38f;
+        q_first = q_inner = q_last = 0;
+    }
+    // one cell at row r (uniform over the wave), weight w, terrain b
+    __device__ void add(int r, int ny, int hq, float qxv, float qyv, float b, float w, float dry)
+    {
+        const float d = (float)hq * (float)kQuantum;
+        const long long u = (unsigned)hq;
+        q_first += r == 0 ? u : 0;
+        q_last += r == ny - 1 ? u : 0;
+        q_inner += (r != 0 && r != ny - 1) ? u : 0;
+        qx += qxv * w;
+        qy += qyv * w;
+        if (d > dry) {
+            const float inv = rcpf(d);
+            const float vx = qxv * inv, vy = qyv * inv;
+            const float s2 = vx * vx + vy * vy;
+            vmax = fmaxf(vmax, sqrtf_hw(s2));
+            ke += d * s2 * w;
+        }
+        pot += d * b * w;
+        prs += d * d * w;
+        hmin = fminf(hmin, d);
+    }
+};
+
+// weight of cell (r, c): the cell area, or the Triangle-A-C vertex weight
+// (bulk_state_authority.py:624-634) as a multiple of area / 6
+__device__ inline float cell_weight32(int mode, int ny, int nx, float area, float f6, int r, int c)
+{
+    if (!mode) return area;
+    const int n = (r < ny - 1 && c < nx - 1 ? 2 : 0) + (r < ny - 1 && c > 0 ? 1 : 0) + (r > 0 && c > 0 ? 2 : 0) +
+                  (r > 0 && c < nx - 1 ? 1 : 0);
+    return (float)n * f6;
+}
+
+// fold the workgroup's ledger and store it as partial g (before or after)
+__device__ inline void store_ledger32(const V2Args<F32P> &a, const Ledger32 &L, int gc, bool owner, bool before,
+                                      double *lds)
+{
+    const double rho = a.cfg.density, g = a.cfg.gravity, K = a.cfg.earth_pressure;
+    double f[kLedger32D] = {(double)L.qx, (double)L.qy, (double)L.vmax, (double)L.ke, (double)L.pot, (double)L.prs,
+                            (double)L.hmin};
+    block_fold<double, kLedger32D, 1u << 2, 1u << 6>(f, lds);
+    // quanta per weight class: the lane's column class, the band's row class
+    const int cc = gc == 0 ? 0 : (gc == a.nx - 1 ? 2 : 1);
+    long long mq[kClasses];
+#pragma unroll
+    for (int q = 0; q < kClasses; q++)
+        mq[q] = (owner && q % 3 == cc) ? (q / 3 == 0 ? L.q_first : (q / 3 == 1 ? L.q_inner : L.q_last)) : 0;
+    block_fold<long long, kClasses>(mq, reinterpret_cast<long long *>(lds));
+    if (threadIdx.x == 0) {
+        const int G = a.groups, gi = blockIdx.x;
+        double *dst = a.ledger_f + (before ? 0 : kSum) * G + gi;
+        const double vals[kSum] = {0.0, rho * f[0], rho * f[1], f[2], 0.5 * rho * f[3], rho * g * f[4],
+                                   rho * 0.5 * K * g * f[5], f[6]};
+#pragma unroll
+        for (int q = 0; q < kSum; q++) dst[q * G] = vals[q];
+        long long *qd = a.ledger_q + (before ? 0 : kClasses) * G + gi;
+#pragma unroll
+        for (int q = 0; q < kClasses; q++) qd[q * G] = mq[q];
+    }
+}
+
+// One pass over the input state: ledger before, static diagnostic and the
+// CFL maximum of the first substep (as v2_begin_kernel, FP32 strip layout).
+__global__ void __launch_bounds__(kBlock) v2_begin32_kernel(V2Args<F32P> a)
+{
+    __shared__ double lds[kLedger32D * kBlock];
+    const int ny = a.ny, nx = a.nx;
+    const int tiles_x = (nx + kStripW - 1) / kStripW;
+    const int t = blockIdx.x;
+    const int r0 = (t / tiles_x) * kTileRows + wave_index() * kSH, c0 = (t % tiles_x) * kStripW;
+    const int lane = threadIdx.x & (kWave - 1);
+    const int gc = c0 - 1 + lane;
+    const bool owner = lane >= 1 && lane <= kStripW && gc < nx;
+    const bool stat = a.cfg.static_exit != 0;
+    StripRows<kSH> R;
+    strip_load<kSH>(a, 0, r0, c0, R);
+    double H[kSH + 2];
+#pragma unroll
+    for (int j = 0; j < kSH + 2; j++) H[j] = (double)R.b[j] + (double)((float)R.hq[j] * (float)kQuantum);
+    const float area = uniform((float)(a.dx * a.dy)), f6 = uniform((float)(a.dx * a.dy / 6.0));
+    const float dry = uniform((float)a.cfg.dry_tolerance);
+    const float Kg = uniform((float)(a.cfg.earth_pressure * a.cfg.gravity));
+    const StaticGrad sg(a.cfg.tan_start, a.dx, a.dy, gc > 0 && gc + 1 < nx);
+    Ledger32 L;
+    L.init();
+    double m[3] = {0.0, 0.0, 0.0};              // max |grad H|^2, max |q|, max wave speed
+#pragma unroll
+    for (int j = 1; j <= kSH; j++) {
+        const int r = r0 - 1 + j;
+        const double HR = from_next_lane_d(H[j]), HL = from_prev_lane_d(H[j]);
+        if (!owner || r >= ny) continue;
+        const int hq = R.hq[j];
+        const float qxv = R.qx[j], qyv = R.qy[j];
+        L.add(r, ny, hq, qxv, qyv, R.b[j], cell_weight32(a.cfg.weight_mode, ny, nx, area, f6, r, gc), dry);
+        // CFL wave speed of the state, in the arithmetic of the substep
+        const float d = (float)hq * (float)kQuantum;
+        float vx = 0.0f, vy = 0.0f;
+        if (d > dry) {
+            const float inv = rcpf(d);
+            vx = qxv * inv;
+            vy = qyv * inv;
+        }
+        m[2] = npmax(m[2], (double)(sqrtf_hw(vx * vx + vy * vy) + sqrtf_hw(Kg * d)));
+        if (stat) {
+            const double hl = gc > 0 ? HL : H[j], hr = gc + 1 < nx ? HR : H[j];
+            const double hd = r > 0 ? H[j - 1] : H[j], hu = r + 1 < ny ? H[j + 1] : H[j];
+            m[0] = npmax(m[0], sg.grad2(hl, hr, hd, hu, r > 0 && r + 1 < ny));
+            m[1] = npmax(m[1], (double)fmaxf(fabsf(qxv), fabsf(qyv)));
+        }
+    }
+    store_ledger32(a, L, gc, owner, true, lds);
+    block_fold<double, 3, 0x7u>(m, lds);
+    if (threadIdx.x == 0) {
+        if (stat) {
+            // the square root is monotonic: the root of the largest square is the largest root
+            spread_max(a.spread + kSetStaticGrad * kSpreadWords, sqrt(m[0]));
+            spread_max(a.spread + kSetStaticQ * kSpreadWords, m[1]);
+        }
+        spread_max(a.spread, m[2]);
+        if (blockIdx.x == 0) a.sync->ctl[0].remaining = a.cfg.dt;
+    }
+}
+
+// copy back when the final state sits in scratch; ledger after (FP32 strips)
+__global__ void __launch_bounds__(kBlock) v2_end32_kernel(V2Args<F32P> a, int k)
+{
+    __shared__ double lds[kLedger32D * kBlock];
+    const Control C = resolve_control(a.sync, k);
+    if (C.static_exit) return;
+    const int fsrc = C.src;
+    const int ny = a.ny, nx = a.nx;
+    const int tiles_x = (nx + kStripW - 1) / kStripW;
+    const int t = blockIdx.x;
+    const int r0 = (t / tiles_x) * kTileRows + wave_index() * kSH, c0 = (t % tiles_x) * kStripW;
+    const int lane = threadIdx.x & (kWave - 1);
+    const int gc = c0 - 1 + lane;
+    const bool owner = lane >= 1 && lane <= kStripW && gc < nx;
+    const int32_t *gh = fsrc ? a.h[1] : a.h[0];
+    const float *gqx = fsrc ? a.qx[1] : a.qx[0], *gqy = fsrc ? a.qy[1] : a.qy[0];
+    int hq[kSH];
+    float qxv[kSH], qyv[kSH], bv[kSH];
+#pragma unroll
+    for (int j = 0; j < kSH; j++) {
+        const int r = r0 + j;
+        hq[j] = 0; qxv[j] = 0.0f; qyv[j] = 0.0f; bv[j] = 0.0f;
+        if (owner && r < ny) {
+            const int i = r * nx + gc;
+            bv[j] = a.b[i];
+            hq[j] = gh[i];
+            qxv[j] = gqx[i];
+            qyv[j] = gqy[i];
+        }
+    }
+    const float area = uniform((float)(a.dx * a.dy)), f6 = uniform((float)(a.dx * a.dy / 6.0));
+    const float dry = uniform((float)a.cfg.dry_tolerance);
+    Ledger32 L;
+    L.init();
+#pragma unroll
+    for (int j = 0; j < kSH; j++) {
+        const int r = r0 + j;
+        if (!owner || r >= ny) continue;
+        if (fsrc == 1) {
+            const int i = r * nx + gc;
+            a.h[0][i] = hq[j];
+            a.qx[0][i] = qxv[j];
+            a.qy[0][i] = qyv[j];
+        }
+        L.add(r, ny, hq[j], qxv[j], qyv[j], bv[j], cell_weight32(a.cfg.weight_mode, ny, nx, area, f6, r, gc), dry);
+    }
+    store_ledger32(a, L, gc, owner, false, lds);
+}
+
 // Fold the partial ledgers and diagnostics in fixed order into the record.
-// One workgroup of kFinalizeBlock threads; thread p folds workgroups p,
-// p + kFinalizeBlock, ... of each field (coalesced across the threads).
+// One workgroup of kFinalizeBlock threads.  Thread p accumulates parts p,
+// p + kFinalizeBlock, ... of every field, issuing all loads of a part at once
+// (one memory round trip per part); a field-parallel LDS fold then reduces
+// the threads, each field by one wave.
+constexpr int kFoldF = 2 * kSum + kDiag + 2;   // double fields: ledgers, diagnostics, static maxima
+constexpr int kFoldQ = 2 * kClasses;           // int64 fields: quanta per weight class
+// field 3 and kSum + 3 are maxima (speed), 7 and kSum + 7 minima (depth), the
+// last two (static-gradient and momentum spread maxima) maxima
+constexpr unsigned kFoldMax = (1u << 3) | (1u << (kSum + 3)) | (3u << (2 * kSum + kDiag));
+constexpr unsigned kFoldMin = (1u << 7) | (1u << (kSum + 7));
+
 template <class P>
 __global__ void __launch_bounds__(kFinalizeBlock) v2_finalize_kernel(V2Args<P> a, int k, int nparts, int ndiag)
 {
-    __shared__ double smem[kFinalizeBlock / kWave * 40];
+    __shared__ double lds[kFoldF * kFinalizeBlock];
+    constexpr int kNW = kFinalizeBlock / kWave;
     const Control C = resolve_control(a.sync, k);
     const bool after_valid = !C.static_exit;
-    double v[kSum * 2 + kDiag];
-    for (int q = 0; q < kSum * 2 + kDiag; q++) v[q] = 0.0;
-    double mx[6] = {0.0, 0.0, -1e300, -1e300, 0.0, 0.0};   // max speed b/a; -min h b/a; static grad, q
-    long long mq[2 * kClasses];
-    for (int q = 0; q < 2 * kClasses; q++) mq[q] = 0;
-    for (int p = threadIdx.x; p < ndiag; p += blockDim.x)
-        for (int d = 0; d < kDiag; d++) v[2 * kSum + d] += a.diag_part[(int64_t)d * ndiag + p];
+    double fsum[kFoldF];
+#pragma unroll
+    for (int f = 0; f < kFoldF; f++) fsum[f] = ((kFoldMin >> f) & 1u) ? 1e300 : 0.0;
+    long long qsum[kFoldQ];
+#pragma unroll
+    for (int f = 0; f < kFoldQ; f++) qsum[f] = 0;
     for (int p = threadIdx.x; p < nparts; p += blockDim.x) {
-        double f[kLedgerF];
-        long long q8[kLedgerQ];
+        double x[2 * kSum];
+        long long q[kFoldQ];
 #pragma unroll
-        for (int q = 0; q < kLedgerF; q++) f[q] = a.ledger_f[(int64_t)q * nparts + p];
+        for (int f = 0; f < 2 * kSum; f++) x[f] = a.ledger_f[(int64_t)f * nparts + p];
 #pragma unroll
-        for (int q = 0; q < kLedgerQ; q++) q8[q] = a.ledger_q[(int64_t)q * nparts + p];
-        for (int q = 0; q < kSum; q++) {
-            v[q] += f[q];
-            v[kSum + q] += f[kSum + q];
-        }
-        mx[0] = npmax(mx[0], f[3]);
-        mx[1] = npmax(mx[1], f[kSum + 3]);
-        mx[2] = npmax(mx[2], -f[7]);
-        mx[3] = npmax(mx[3], -f[kSum + 7]);
-        for (int q = 0; q < kLedgerQ; q++) mq[q] += q8[q];
+        for (int f = 0; f < kFoldQ; f++) q[f] = a.ledger_q[(int64_t)f * nparts + p];
+#pragma unroll
+        for (int f = 0; f < 2 * kSum; f++)
+            fsum[f] = ((kFoldMax >> f) & 1u) ? npmax(fsum[f], x[f]) : (((kFoldMin >> f) & 1u) ? npmin(fsum[f], x[f]) : fsum[f] + x[f]);
+#pragma unroll
+        for (int f = 0; f < kFoldQ; f++) qsum[f] += q[f];
+    }
+    for (int p = threadIdx.x; p < ndiag; p += blockDim.x) {
+        double x[kDiag];
+#pragma unroll
+        for (int d = 0; d < kDiag; d++) x[d] = a.diag_part[(int64_t)d * ndiag + p];
+#pragma unroll
+        for (int d = 0; d < kDiag; d++) fsum[2 * kSum + d] += x[d];
     }
     if (threadIdx.x < kWave) {
-        mx[4] = __longlong_as_double((long long)a.spread[kSetStaticGrad * kSpreadWords + threadIdx.x * kSpreadStride]);
-        mx[5] = __longlong_as_double((long long)a.spread[kSetStaticQ * kSpreadWords + threadIdx.x * kSpreadStride]);
+        fsum[2 * kSum + kDiag] = __longlong_as_double((long long)a.spread[kSetStaticGrad * kSpreadWords + threadIdx.x * kSpreadStride]);
+        fsum[2 * kSum + kDiag + 1] = __longlong_as_double((long long)a.spread[kSetStaticQ * kSpreadWords + threadIdx.x * kSpreadStride]);
     }
-    constexpr int kNW = kFinalizeBlock / kWave;
-    block_reduce<double, kSum * 2 + kDiag, 0, kNW>(v, smem);
-    block_reduce<double, 6, 1, kNW>(mx, smem);
-    block_reduce<long long, 2 * kClasses, 0, kNW>(mq, reinterpret_cast<long long *>(smem));
+    // the next step's loop state (other parity): unused during this step
+    {
+        unsigned *ps = reinterpret_cast<unsigned *>(a.sync_next);
+        for (int i = threadIdx.x; i < (int)(sizeof(SyncBlock) / 4); i += blockDim.x) ps[i] = 0u;
+        for (int i = threadIdx.x; i < kStepSpreadSets * kSpread; i += blockDim.x)
+            a.spread_next[(int64_t)i * kSpreadStride] = 0ull;
+    }
+    block_fold<double, kFoldF, kFoldMax, kFoldMin, kNW>(fsum, lds);
+    block_fold<long long, kFoldQ, 0u, 0u, kNW>(qsum, reinterpret_cast<long long *>(lds));
     if (threadIdx.x == 0) {
         mv2_step_record *rec = a.rec;
         for (int q = 0; q < kSum; q++) {
-            rec->summary_before[q] = v[q];
-            rec->summary_after[q] = after_valid ? v[kSum + q] : v[q];
+            rec->summary_before[q] = fsum[q];
+            rec->summary_after[q] = after_valid ? fsum[kSum + q] : fsum[q];
         }
-        rec->summary_before[3] = mx[0];
-        rec->summary_after[3] = after_valid ? mx[1] : mx[0];
-        rec->summary_before[7] = -mx[2];
-        rec->summary_after[7] = after_valid ? -mx[3] : -mx[2];
-        for (int d = 0; d < kDiag; d++) rec->diag[d] = after_valid ? v[2 * kSum + d] : 0.0;
+        for (int d = 0; d < kDiag; d++) rec->diag[d] = (after_valid && C.substeps > 0) ? fsum[2 * kSum + d] : 0.0;
         if (P::kQuant) {
             // exact mobile volume: integer quanta per weight class times its weight
             const double area = a.dx * a.dy, f6 = a.dx * a.dy / 6.0;
@@ -732,18 +945,18 @@ __global__ void __launch_bounds__(kFinalizeBlock) v2_finalize_kernel(V2Args<P> a
             double mb = 0.0, ma = 0.0, qb = 0.0, qa = 0.0;
             for (int q = 0; q < kClasses; q++) {
                 double w = a.cfg.weight_mode ? vertex_weight(a.ny, a.nx, f6, rep_r[q / 3], rep_c[q % 3]) : area;
-                mb += ((double)mq[q] * kQuantum) * w;
-                ma += ((double)mq[kClasses + q] * kQuantum) * w;
-                qb += (double)mq[q];
-                qa += (double)mq[kClasses + q];
+                mb += ((double)qsum[q] * kQuantum) * w;
+                ma += ((double)qsum[kClasses + q] * kQuantum) * w;
+                qb += (double)qsum[q];
+                qa += (double)qsum[kClasses + q];
             }
             rec->summary_before[0] = mb;
             rec->summary_after[0] = after_valid ? ma : mb;
             rec->mass_quanta_before = qb;
             rec->mass_quanta_after = after_valid ? qa : qb;
         }
-        a.sync->static_grad = (unsigned long long)__double_as_longlong(mx[4]);
-        a.sync->static_q = (unsigned long long)__double_as_longlong(mx[5]);
+        a.sync->static_grad = (unsigned long long)__double_as_longlong(fsum[2 * kSum + kDiag]);
+        a.sync->static_q = (unsigned long long)__double_as_longlong(fsum[2 * kSum + kDiag + 1]);
         rec->substeps = C.substeps;
         rec->cfl_limited = C.cfl_limited;
         rec->static_exit = C.static_exit;
@@ -759,9 +972,18 @@ template <class P>
 static int launch_step(mv2_context *ctx, const mv2_step_config *cfg, mv2_step_record *out, int fixed_chunk = 0)
 {
     const int64_t n = (int64_t)ctx->ny * ctx->nx;
-    const int gs = grid_for(n, kBlock * kLedgerU, kFinalizeBlock);   // begin / end passes
-    const int gu = P::kQuant ? ((ctx->nx + kStripW - 1) / kStripW) * ((ctx->ny + kTileRows - 1) / kTileRows)
+    // FP32 band height of the substep: 4 rows (measured faster than 2 on a
+    // 701^2 terrain, where 2-row bands give twice the waves); 2 on request
+    int band = 4;
+    if (P::kQuant) {
+        if (const char *e = getenv("MV2_FRAME_BAND_ROWS")) band = atoi(e) == 2 ? 2 : 4;
+    }
+    const int rows_sub = band * (kBlock / kWave);
+    const int gu = P::kQuant ? ((ctx->nx + kStripW - 1) / kStripW) * ((ctx->ny + rows_sub - 1) / rows_sub)
                              : grid_for(n);                              // substep launches
+    const int gl = P::kQuant ? ((ctx->nx + kStripW - 1) / kStripW) * ((ctx->ny + kTileRows - 1) / kTileRows) : 0;
+    // begin / end passes: the substep tiles (FP32), a grid stride (FP64)
+    const int gs = P::kQuant ? gl : grid_for(n, kBlock * kLedgerU, kFinalizeBlock);
     if (int rc = ensure_scratch(ctx, ctx->partials, (sizeof(double) * kLedgerF + sizeof(long long) * kLedgerQ) * (size_t)gs))
         return rc;
     if (int rc = ensure_scratch(ctx, ctx->step_diag, sizeof(double) * kDiag * (size_t)gu)) return rc;
@@ -779,21 +1001,29 @@ static int launch_step(mv2_context *ctx, const mv2_step_config *cfg, mv2_step_re
     a.qy[0] = (typename P::real *)ctx->qy;
     a.qy[1] = (typename P::real *)ctx->alt_qy.ptr;
     a.cfg = *cfg;
-    a.sync = (SyncBlock *)ctx->sync.ptr;
+    const int parity = ctx->v2_frame & 1;
+    unsigned long long *spread_even = (unsigned long long *)ctx->spread.ptr;
+    unsigned long long *spread_odd = spread_even + (size_t)kStepOddSpreadSet * kSpreadWords;
+    a.sync = (SyncBlock *)((char *)ctx->sync.ptr + parity * kSyncParityBytes);
+    a.sync_next = (SyncBlock *)((char *)ctx->sync.ptr + (parity ^ 1) * kSyncParityBytes);
+    a.spread_next = parity ? spread_even : spread_odd;
     a.ledger_f = (double *)ctx->partials.ptr;
     a.ledger_q = (long long *)(a.ledger_f + (size_t)kLedgerF * gs);
     a.groups = gs;
     a.diag_part = (double *)ctx->step_diag.ptr;
     a.diag_groups = gu;
-    a.spread = (unsigned long long *)ctx->spread.ptr;
+    a.spread = parity ? spread_odd : spread_even;
     a.rec = (mv2_step_record *)ctx->result.ptr;
     a.clamp_quarter = cfg->cfl <= 0.25 ? 1 : 0;
     hipStream_t st = ctx->stream;
     const bool timing = ctx->timing != 0;
     MV2_HIP(ctx, hipEventRecord(ctx->ev_start, st));
-    hipLaunchKernelGGL(v2_reset_kernel, dim3(grid_for((int64_t)gu * kDiag, kBlock, 64)), dim3(kBlock), 0, st,
-                       a.sync, a.rec, a.diag_part, gu, a.spread);
-    hipLaunchKernelGGL(v2_begin_kernel<P>, dim3(gs), dim3(kBlock), 0, st, a);
+    // this parity's loop state is zero if the previous step completed; an
+    // error anywhere below leaves v2_clean at -1 and the next step resets
+    if (ctx->v2_clean != parity) hipLaunchKernelGGL(v2_reset_kernel, dim3(1), dim3(kBlock), 0, st, a.sync, a.spread);
+    ctx->v2_clean = -1;
+    if constexpr (P::kQuant) hipLaunchKernelGGL(v2_begin32_kernel, dim3(gs), dim3(kBlock), 0, st, a);
+    else hipLaunchKernelGGL(v2_begin_kernel<P>, dim3(gs), dim3(kBlock), 0, st, a);
     MV2_HIP(ctx, hipGetLastError());
     int k = 0;
     int chunk = fixed_chunk > 0 ? fixed_chunk : (ctx->v2_predicted > 0 ? ctx->v2_predicted + 1 : 4);
@@ -810,10 +1040,17 @@ static int launch_step(mv2_context *ctx, const mv2_step_config *cfg, mv2_step_re
         }
         for (int j = 0; j < chunk; j++, k++) {
             if (timing) MV2_HIP(ctx, hipEventRecord(ctx->sub_events[2 * k], st));
-            hipLaunchKernelGGL(v2_substep_kernel<P>, dim3(gu), dim3(kBlock), 0, st, a, k, k < eager_until ? 1 : 0);
+            const int eager = k < eager_until ? 1 : 0;
+            if constexpr (P::kQuant) {
+                if (band == 2) hipLaunchKernelGGL((v2_substep_kernel<P, 2>), dim3(gu), dim3(kBlock), 0, st, a, k, eager);
+                else hipLaunchKernelGGL((v2_substep_kernel<P, 4>), dim3(gu), dim3(kBlock), 0, st, a, k, eager);
+            } else {
+                hipLaunchKernelGGL(v2_substep_kernel<P>, dim3(gu), dim3(kBlock), 0, st, a, k, eager);
+            }
             if (timing) MV2_HIP(ctx, hipEventRecord(ctx->sub_events[2 * k + 1], st));
         }
-        hipLaunchKernelGGL(v2_end_kernel<P>, dim3(gs), dim3(kBlock), 0, st, a, k);
+        if constexpr (P::kQuant) hipLaunchKernelGGL(v2_end32_kernel, dim3(gs), dim3(kBlock), 0, st, a, k);
+        else hipLaunchKernelGGL(v2_end_kernel<P>, dim3(gs), dim3(kBlock), 0, st, a, k);
         hipLaunchKernelGGL(v2_finalize_kernel<P>, dim3(1), dim3(kFinalizeBlock), 0, st, a, k, gs, gu);
         MV2_HIP(ctx, hipGetLastError());
         MV2_HIP(ctx, hipEventRecord(ctx->ev_stop, st));
@@ -825,6 +1062,9 @@ static int launch_step(mv2_context *ctx, const mv2_step_config *cfg, mv2_step_re
         eager_until = k + chunk;
     }
     *out = *hrec;
+    ctx->v2_clean = parity ^ 1;          // zeroed by this step's finalize pass
+    ctx->v2_frame++;
+    ctx->v2_last_sync = parity;
     if (fixed_chunk == 0) ctx->v2_predicted = out->substeps;
     out->launches = k;
     float ms = 0.0f;
@@ -919,7 +1159,8 @@ extern "C" int mv2_static_diagnostic(mv2_context *ctx, double out[2])
                                         : launch_step<F32P>(ctx, &c, &rec, 1);
     if (rc) return rc;
     SyncBlock sb;
-    MV2_HIP(ctx, hipMemcpy(&sb, ctx->sync.ptr, sizeof(SyncBlock), hipMemcpyDeviceToHost));
+    MV2_HIP(ctx, hipMemcpy(&sb, (char *)ctx->sync.ptr + ctx->v2_last_sync * kSyncParityBytes, sizeof(SyncBlock),
+                           hipMemcpyDeviceToHost));
     out[0] = __builtin_bit_cast(double, sb.static_grad);
     out[1] = __builtin_bit_cast(double, sb.static_q);
     return MV2_OK;
