@@ -56,10 +56,15 @@ struct BatchArgs {
     const int64_t *inst_off;
     const int32_t *inst_first_tile, *inst_tiles;
     const int32_t *tile_inst, *tile_r0, *tile_c0;
+    // loop state of this step's parity (steps alternate between two copies;
+    // the finalize pass zeroes the other copy for the next step)
     BControl *ctl;                      // [2][count]
     int *flags;                         // [2][count]
     unsigned long long *wave;           // [4][count][kInstWords]
     unsigned long long *stat;           // [2][count][kInstWords]: max |grad H|, max |q|
+    BControl *ctl_next;
+    int *flags_next;
+    unsigned long long *wave_next, *stat_next;
     double *ledger;                     // [2 passes][kBLedger][tiles]
     long long *quanta;                  // [2 passes][tiles]
     mv2_batch_record *rec;              // [count]
@@ -129,19 +134,27 @@ __device__ inline BControl resolve_bcontrol(const BatchArgs &a, int k, int i)
     return C;
 }
 
+// zero the loop state of one instance (words of the four arrays)
+__device__ inline void zero_instance_state(BControl *ctl, int *flags, unsigned long long *wave,
+                                           unsigned long long *stat, int count, int i, int t, int nt)
+{
+    constexpr int kCtlWords = (int)(sizeof(BControl) / 4);
+    for (int w = t; w < 2 * kCtlWords; w += nt)
+        reinterpret_cast<unsigned *>(ctl + (int64_t)(w / kCtlWords) * count + i)[w % kCtlWords] = 0u;
+    if (t < 2) flags[(int64_t)t * count + i] = 0;
+    for (int w = t; w < kWaveSetsB * kInstSlots; w += nt)
+        wave[((int64_t)(w / kInstSlots) * count + i) * kInstWords + (w % kInstSlots) * kInstStride] = 0ull;
+    for (int w = t; w < 2 * kInstSlots; w += nt)
+        stat[((int64_t)(w / kInstSlots) * count + i) * kInstWords + (w % kInstSlots) * kInstStride] = 0ull;
+}
+
+// zero this parity's loop state (first step, or after an error); a zero
+// record is an instance that has not started: its first substep launch
+// takes remaining = dt
 __global__ void batch_reset_kernel(BatchArgs a)
 {
-    const int64_t tid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const int64_t nth = (int64_t)gridDim.x * blockDim.x;
-    for (int64_t i = tid; i < 2 * (int64_t)a.count; i += nth) {
-        BControl c;
-        memset(&c, 0, sizeof(c));
-        c.remaining = a.cfg.dt;
-        a.ctl[i] = c;
-        a.flags[i] = 0;
-    }
-    for (int64_t i = tid; i < (int64_t)kWaveSetsB * a.count * kInstSlots; i += nth) a.wave[i * kInstStride] = 0ull;
-    for (int64_t i = tid; i < 2 * (int64_t)a.count * kInstSlots; i += nth) a.stat[i * kInstStride] = 0ull;
+    for (int i = blockIdx.x; i < a.count; i += gridDim.x)
+        zero_instance_state(a.ctl, a.flags, a.wave, a.stat, a.count, i, threadIdx.x, blockDim.x);
 }
 
 // One tile: ledger before, static diagnostic, CFL maximum of the input.  The
@@ -152,7 +165,7 @@ template <int SH>
 __global__ void __launch_bounds__(kBlock) batch_begin_kernel(BatchArgs a)
 {
     __shared__ double lds[6 * kBlock];
-    const int g = blockIdx.x;
+    const int g = xcd_block();
     const int i = a.tile_inst[g];
     const InstView v(a, i);
     const int lane = threadIdx.x & (kWave - 1);
@@ -221,13 +234,14 @@ template <int SH>
 __global__ void __launch_bounds__(kBlock) batch_substep_kernel(BatchArgs a, int k)
 {
     __shared__ double smem[64 * 4];
-    const int g = blockIdx.x;
+    const int g = xcd_block();
     const int i = a.tile_inst[g];
     const bool lead = g == a.inst_first_tile[i] && threadIdx.x == 0;
     const mv2_step_config &cfg = a.cfg;
     BControl C = resolve_bcontrol(a, k, i);
     if (!C.done && !C.started) {
         C.started = 1;
+        C.remaining = cfg.dt;
         if (cfg.static_exit) {
             const double ext_max = npmax(0.0, npmax(fabs(cfg.external_ax), fabs(cfg.external_ay)));
             const double qmax = inst_read_max(inst_set(a.stat, 1, a.count, i), smem);
@@ -304,7 +318,7 @@ template <int SH>
 __global__ void __launch_bounds__(kBlock) batch_end_kernel(BatchArgs a, int k)
 {
     __shared__ double lds[kBLedger * kBlock];
-    const int g = blockIdx.x;
+    const int g = xcd_block();
     const int i = a.tile_inst[g];
     const BControl C = resolve_bcontrol(a, k, i);
     const InstView v(a, i);
@@ -374,6 +388,8 @@ __global__ void __launch_bounds__(kBlock) batch_finalize_kernel(BatchArgs a, int
         q[0] += a.quanta[t0 + t];
         q[1] += a.quanta[a.tiles + t0 + t];
     }
+    // the next step's loop state of this instance (other parity): unused now
+    zero_instance_state(a.ctl_next, a.flags_next, a.wave_next, a.stat_next, a.count, i, threadIdx.x, blockDim.x);
     block_reduce<double, 2 * kBLedger, 0>(s, smem);
     block_reduce<long long, 2, 0>(q, reinterpret_cast<long long *>(smem));
     if (threadIdx.x == 0) {
@@ -474,11 +490,13 @@ struct mv2_batch_state {
     Scratch alt_h, alt_qx, alt_qy;
     Scratch ctl, wave, stat, ledger, rec;
     int predicted = 0;
-    // graph of eight substep launches
-    hipGraphExec_t graph = nullptr;
+    int frame = 0;                       // steps taken: the parity picks the loop-state copy
+    int clean = -1;                      // parity whose loop state is known zeroed, -1 none
+    // graphs of eight substep launches, one per parity
+    hipGraphExec_t graph[2] = {nullptr, nullptr};
     hipStream_t capture = nullptr;
     hipEvent_t ev_sub0 = nullptr, ev_sub1 = nullptr;   // bracket the substep launches
-    std::vector<unsigned char> graph_key;
+    std::vector<unsigned char> graph_key[2];
 };
 
 static mv2_batch_state *batch_of(mv2_context *ctx)
@@ -491,7 +509,8 @@ void mv2_batch_release(mv2_context *ctx)
 {
     mv2_batch_state *bs = ctx->batch;
     if (!bs) return;
-    if (bs->graph) hipGraphExecDestroy(bs->graph);
+    for (int p = 0; p < 2; p++)
+        if (bs->graph[p]) hipGraphExecDestroy(bs->graph[p]);
     if (bs->capture) hipStreamDestroy(bs->capture);
     if (bs->ev_sub0) hipEventDestroy(bs->ev_sub0);
     if (bs->ev_sub1) hipEventDestroy(bs->ev_sub1);
@@ -534,10 +553,11 @@ static int build_tiles(mv2_context *ctx, mv2_batch_state *bs, int shape)
     bs->shape = shape;
     if (int rc = ensure_scratch(ctx, bs->ledger, (sizeof(double) * 2 * kBLedger + sizeof(long long) * 2) * (size_t)T))
         return rc;
-    if (bs->graph) {
-        hipGraphExecDestroy(bs->graph);
-        bs->graph = nullptr;
-    }
+    for (int p = 0; p < 2; p++)
+        if (bs->graph[p]) {
+            hipGraphExecDestroy(bs->graph[p]);
+            bs->graph[p] = nullptr;
+        }
     return MV2_OK;
 }
 
@@ -566,10 +586,21 @@ static BatchArgs batch_args(mv2_batch_state *bs, const mv2_step_config *cfg)
     a.tile_inst = tab;
     a.tile_r0 = tab + bs->tiles;
     a.tile_c0 = tab + 2 * (size_t)bs->tiles;
-    a.ctl = (BControl *)bs->ctl.ptr;
-    a.flags = (int *)((char *)bs->ctl.ptr + sizeof(BControl) * 2 * n);
-    a.wave = (unsigned long long *)bs->wave.ptr;
-    a.stat = (unsigned long long *)bs->stat.ptr;
+    // two copies of the loop state, by step parity
+    const int p = bs->frame & 1;
+    BControl *ctl0 = (BControl *)bs->ctl.ptr;
+    int *flags0 = (int *)((char *)bs->ctl.ptr + sizeof(BControl) * 4 * n);
+    unsigned long long *wave0 = (unsigned long long *)bs->wave.ptr;
+    unsigned long long *stat0 = (unsigned long long *)bs->stat.ptr;
+    const size_t wave_copy = (size_t)kWaveSetsB * kInstWords * n, stat_copy = (size_t)2 * kInstWords * n;
+    a.ctl = ctl0 + (size_t)p * 2 * n;
+    a.flags = flags0 + (size_t)p * 2 * n;
+    a.wave = wave0 + p * wave_copy;
+    a.stat = stat0 + p * stat_copy;
+    a.ctl_next = ctl0 + (size_t)(p ^ 1) * 2 * n;
+    a.flags_next = flags0 + (size_t)(p ^ 1) * 2 * n;
+    a.wave_next = wave0 + (p ^ 1) * wave_copy;
+    a.stat_next = stat0 + (p ^ 1) * stat_copy;
     a.ledger = (double *)bs->ledger.ptr;
     a.quanta = (long long *)(a.ledger + (size_t)2 * kBLedger * bs->tiles);
     a.rec = (mv2_batch_record *)bs->rec.ptr;
@@ -616,10 +647,12 @@ extern "C" int mv2_batch_bind(mv2_context *ctx, int count, const int32_t *ny, co
     if (int rc = ensure_scratch(ctx, bs->alt_h, sizeof(int32_t) * (size_t)cells)) return rc;
     if (int rc = ensure_scratch(ctx, bs->alt_qx, sizeof(float) * (size_t)cells)) return rc;
     if (int rc = ensure_scratch(ctx, bs->alt_qy, sizeof(float) * (size_t)cells)) return rc;
-    if (int rc = ensure_scratch(ctx, bs->ctl, (sizeof(BControl) + sizeof(int)) * 2 * (size_t)count)) return rc;
-    if (int rc = ensure_scratch(ctx, bs->wave, sizeof(unsigned long long) * kWaveSetsB * kInstWords * (size_t)count))
+    if (int rc = ensure_scratch(ctx, bs->ctl, (sizeof(BControl) + sizeof(int)) * 4 * (size_t)count)) return rc;
+    if (int rc = ensure_scratch(ctx, bs->wave, sizeof(unsigned long long) * 2 * kWaveSetsB * kInstWords * (size_t)count))
         return rc;
-    if (int rc = ensure_scratch(ctx, bs->stat, sizeof(unsigned long long) * 2 * kInstWords * (size_t)count)) return rc;
+    if (int rc = ensure_scratch(ctx, bs->stat, sizeof(unsigned long long) * 2 * 2 * kInstWords * (size_t)count))
+        return rc;
+    bs->clean = -1;                      // new scratch: reset before the next step
     if (int rc = ensure_scratch(ctx, bs->rec, sizeof(mv2_batch_record) * (size_t)count)) return rc;
     return build_tiles(ctx, bs, bs->shape);
 }
@@ -645,26 +678,27 @@ extern "C" int mv2_batch_tile_shape(int shape, int *th, int *tw)
 // capture eight substep launches (k = 0..7) into a graph for this config
 static int ensure_graph(mv2_context *ctx, mv2_batch_state *bs, const BatchArgs &a)
 {
+    const int p = bs->frame & 1;
     std::vector<unsigned char> key(sizeof(BatchArgs) + sizeof(int));
     memcpy(key.data(), &a, sizeof(BatchArgs));
     memcpy(key.data() + sizeof(BatchArgs), &bs->shape, sizeof(int));
-    if (bs->graph && key == bs->graph_key) return MV2_OK;
-    if (bs->graph) {
-        hipGraphExecDestroy(bs->graph);
-        bs->graph = nullptr;
+    if (bs->graph[p] && key == bs->graph_key[p]) return MV2_OK;
+    if (bs->graph[p]) {
+        hipGraphExecDestroy(bs->graph[p]);
+        bs->graph[p] = nullptr;
     }
     if (!bs->capture) MV2_HIP(ctx, hipStreamCreateWithFlags(&bs->capture, hipStreamNonBlocking));
     MV2_HIP(ctx, hipStreamBeginCapture(bs->capture, hipStreamCaptureModeThreadLocal));
     for (int k = 0; k < 8; k++) launch_substep(bs->shape, dim3(bs->tiles), bs->capture, a, k);
     hipGraph_t graph = nullptr;
     MV2_HIP(ctx, hipStreamEndCapture(bs->capture, &graph));
-    hipError_t e = hipGraphInstantiate(&bs->graph, graph, nullptr, nullptr, 0);
+    hipError_t e = hipGraphInstantiate(&bs->graph[p], graph, nullptr, nullptr, 0);
     hipGraphDestroy(graph);
     if (e != hipSuccess) {
-        bs->graph = nullptr;
+        bs->graph[p] = nullptr;
         return check_hip(ctx, e, "hipGraphInstantiate");
     }
-    bs->graph_key = key;
+    bs->graph_key[p] = key;
     return MV2_OK;
 }
 
@@ -686,8 +720,12 @@ extern "C" int mv2_batch_step(mv2_context *ctx, const mv2_step_config *cfg, int 
         if (int rc = ensure_graph(ctx, bs, a)) return rc;
     const dim3 grid(bs->tiles);
     MV2_HIP(ctx, hipEventRecord(ctx->ev_start, st));
-    hipLaunchKernelGGL(batch_reset_kernel, dim3(grid_for(2 * (int64_t)bs->count * kInstWords, kBlock, 256)),
-                       dim3(kBlock), 0, st, a);
+    // this parity's loop state is zero if the previous step completed; an
+    // error anywhere below leaves clean at -1 and the next step resets
+    const int parity = bs->frame & 1;
+    if (bs->clean != parity)
+        hipLaunchKernelGGL(batch_reset_kernel, dim3(bs->count < 1024 ? bs->count : 1024), dim3(kWave), 0, st, a);
+    bs->clean = -1;
     launch_begin(bs->shape, grid, st, a);
     MV2_HIP(ctx, hipGetLastError());
     int k = 0;
@@ -706,7 +744,7 @@ extern "C" int mv2_batch_step(mv2_context *ctx, const mv2_step_config *cfg, int 
         // as single launches
         for (int left = want; left > 0;) {
             if (use_graph && k % 8 == 0 && left >= 8) {
-                MV2_HIP(ctx, hipGraphLaunch(bs->graph, st));
+                MV2_HIP(ctx, hipGraphLaunch(bs->graph[parity], st));
                 k += 8;
                 left -= 8;
             } else {
@@ -740,6 +778,8 @@ extern "C" int mv2_batch_step(mv2_context *ctx, const mv2_step_config *cfg, int 
         (void)most;
     }
     memcpy(out, hrec, rec_bytes);
+    bs->clean = parity ^ 1;              // zeroed by this step's finalize pass
+    bs->frame++;
     if (timing_ms) {
         float ms = 0.0f;
         MV2_HIP(ctx, hipEventElapsedTime(&ms, ctx->ev_start, ctx->ev_stop));

@@ -276,9 +276,8 @@ constexpr int kTileRows = kSH * (kBlock / kWave);
 // then mostly hit that XCD's L2.
 __device__ inline void xcd_tile_range(int tiles, int &first, int &last)
 {
-    const int G = gridDim.x, b = blockIdx.x;
-    const int x = b % 8, j = b / 8;
-    const int pb = x * (G / 8) + min(x, G % 8) + j;
+    const int G = gridDim.x;
+    const int pb = xcd_block();
     first = (int)((long long)pb * tiles / G);
     last = (int)((long long)(pb + 1) * tiles / G);
 }
@@ -769,7 +768,7 @@ __global__ void __launch_bounds__(kBlock) v2_begin32_kernel(V2Args<F32P> a)
     __shared__ double lds[kLedger32D * kBlock];
     const int ny = a.ny, nx = a.nx;
     const int tiles_x = (nx + kStripW - 1) / kStripW;
-    const int t = blockIdx.x;
+    const int t = xcd_block();
     const int r0 = (t / tiles_x) * kTileRows + wave_index() * kSH, c0 = (t % tiles_x) * kStripW;
     const int lane = threadIdx.x & (kWave - 1);
     const int gc = c0 - 1 + lane;
@@ -833,7 +832,7 @@ __global__ void __launch_bounds__(kBlock) v2_end32_kernel(V2Args<F32P> a, int k)
     const int fsrc = C.src;
     const int ny = a.ny, nx = a.nx;
     const int tiles_x = (nx + kStripW - 1) / kStripW;
-    const int t = blockIdx.x;
+    const int t = xcd_block();
     const int r0 = (t / tiles_x) * kTileRows + wave_index() * kSH, c0 = (t % tiles_x) * kStripW;
     const int lane = threadIdx.x & (kWave - 1);
     const int gc = c0 - 1 + lane;
@@ -898,25 +897,42 @@ __global__ void __launch_bounds__(kFinalizeBlock) v2_finalize_kernel(V2Args<P> a
     long long qsum[kFoldQ];
 #pragma unroll
     for (int f = 0; f < kFoldQ; f++) qsum[f] = 0;
-    for (int p = threadIdx.x; p < nparts; p += blockDim.x) {
-        double x[2 * kSum];
-        long long q[kFoldQ];
+    // kFoldU parts per thread and round, all their loads issued together
+    constexpr int kFoldU = 4;
+    for (int p0 = threadIdx.x; p0 < nparts; p0 += kFoldU * kFinalizeBlock) {
+        double x[kFoldU][2 * kSum];
+        long long q[kFoldU][kFoldQ];
 #pragma unroll
-        for (int f = 0; f < 2 * kSum; f++) x[f] = a.ledger_f[(int64_t)f * nparts + p];
+        for (int u = 0; u < kFoldU; u++) {
+            const int p = p0 + u * kFinalizeBlock;
 #pragma unroll
-        for (int f = 0; f < kFoldQ; f++) q[f] = a.ledger_q[(int64_t)f * nparts + p];
+            for (int f = 0; f < 2 * kSum; f++)
+                x[u][f] = p < nparts ? a.ledger_f[(int64_t)f * nparts + p] : (((kFoldMin >> f) & 1u) ? 1e300 : 0.0);
 #pragma unroll
-        for (int f = 0; f < 2 * kSum; f++)
-            fsum[f] = ((kFoldMax >> f) & 1u) ? npmax(fsum[f], x[f]) : (((kFoldMin >> f) & 1u) ? npmin(fsum[f], x[f]) : fsum[f] + x[f]);
+            for (int f = 0; f < kFoldQ; f++) q[u][f] = p < nparts ? a.ledger_q[(int64_t)f * nparts + p] : 0;
+        }
 #pragma unroll
-        for (int f = 0; f < kFoldQ; f++) qsum[f] += q[f];
+        for (int u = 0; u < kFoldU; u++) {
+#pragma unroll
+            for (int f = 0; f < 2 * kSum; f++)
+                fsum[f] = ((kFoldMax >> f) & 1u) ? npmax(fsum[f], x[u][f])
+                                                 : (((kFoldMin >> f) & 1u) ? npmin(fsum[f], x[u][f]) : fsum[f] + x[u][f]);
+#pragma unroll
+            for (int f = 0; f < kFoldQ; f++) qsum[f] += q[u][f];
+        }
     }
-    for (int p = threadIdx.x; p < ndiag; p += blockDim.x) {
-        double x[kDiag];
+    for (int p0 = threadIdx.x; p0 < ndiag; p0 += kFoldU * kFinalizeBlock) {
+        double x[kFoldU][kDiag];
 #pragma unroll
-        for (int d = 0; d < kDiag; d++) x[d] = a.diag_part[(int64_t)d * ndiag + p];
+        for (int u = 0; u < kFoldU; u++) {
+            const int p = p0 + u * kFinalizeBlock;
 #pragma unroll
-        for (int d = 0; d < kDiag; d++) fsum[2 * kSum + d] += x[d];
+            for (int d = 0; d < kDiag; d++) x[u][d] = p < ndiag ? a.diag_part[(int64_t)d * ndiag + p] : 0.0;
+        }
+#pragma unroll
+        for (int u = 0; u < kFoldU; u++)
+#pragma unroll
+            for (int d = 0; d < kDiag; d++) fsum[2 * kSum + d] += x[u][d];
     }
     if (threadIdx.x < kWave) {
         fsum[2 * kSum + kDiag] = __longlong_as_double((long long)a.spread[kSetStaticGrad * kSpreadWords + threadIdx.x * kSpreadStride]);
