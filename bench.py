@@ -39,7 +39,10 @@ HBM_PEAK_GBS = 8000.0
 # from rocprofv3 FETCH_SIZE / WRITE_SIZE scaled by a 64 MiB copy
 # (profiles/v2_substep_701x701_cutface_traffic.md); counters cannot be read
 # from inside the timed run.
-MEASURED_TRAFFIC_BYTES = {"fp32": 15.99e6}
+MEASURED_TRAFFIC_BYTES = {"fp32": 15.22e6}
+# the same for a full launch of the batched substep (256 terrains, 18 274 304
+# cells; profiles/batch_substep_256_terrains.md)
+MEASURED_TRAFFIC_BATCHED_FULL = 515.1e6
 BYTES_PER_CELL_SUBSTEP = {"fp32": 28, "fp64": 56}
 
 
@@ -189,7 +192,9 @@ def run_batched(args) -> int:
                    "cells": int(cells.sum()), "frame_dt_s": FRAME_DT, "tile": TILE_SHAPES[tune["best"]]},
         "autotune_ms": tune["ms"],
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": "batch_substep_kernel",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": MEASURED_TRAFFIC_BATCHED_FULL,
+                     "traffic_note": "HBM bytes of one full launch (every instance active)",
+                     "kernel": "batch_substep_kernel",
                      "bytes_per_cell_substep": bpc, "substep_launches": launches,
                      "substep_launch_ms_total": sub_ms},
     }
