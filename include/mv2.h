@@ -147,7 +147,8 @@ typedef struct mv2_batch_record {
 int mv2_batch_bind(mv2_context *ctx, int count, const int32_t *ny, const int32_t *nx,
                    const int64_t *offset, double dx, double dy, float *b, int32_t *h_quanta,
                    float *qx, float *qy);
-/* tile shape of the batched kernels: 0 16x32, 1 8x64, 2 32x16, 3 16x64 */
+/* tile shape of the batched kernels (rows x columns): 0 8x62, 1 16x62, 2 32x62,
+ * 3 64x62 (four wave bands of 2, 4, 8, 16 rows) */
 int mv2_batch_set_tile_shape(mv2_context *ctx, int shape);
 int mv2_batch_tile_shape(int shape, int *th, int *tw);
 /* one frame of every instance (cfg->dt each); use_graph replays a HIP graph of
@@ -173,6 +174,15 @@ typedef struct mv2_relax_result {
 int mv2_relax(mv2_context *ctx, int precision, void *height, int ny, int nx,
               const double crit_diff[4], int max_iterations, double tolerance,
               mv2_relax_result *out);
+
+/* the plain relaxation of mv2_relax (FP32) applied to every instance of the
+ * bound batch at once, each instance with its own iteration count and
+ * convergence test as if relaxed alone: `field` has the batch's packed layout
+ * (the bound b_eff, or any array laid out like it); out has one result per
+ * instance.  Replaces a loop of relax_critical_slope calls
+ * (slope_model.py:273-343), one per terrain. */
+int mv2_batch_relax(mv2_context *ctx, float *field, const double crit_diff[4], int max_iterations,
+                    double tolerance, mv2_relax_result *out);
 
 /* ---- compact colour-batch sweep: WarpCompactActiveEdgeOperator.relax_phase /
  *      scan_phase (pkg/solvers/warp_compact_frontier.py:57-130, :296-392).
@@ -235,6 +245,20 @@ typedef struct mv2_label_result {
 
 int mv2_avalanche_label(mv2_context *ctx, const mv2_avalanche_config *cfg,
                         const mv2_avalanche_fields *f, mv2_label_result *out);
+/* batched counterparts for the bound batch (FP32 fields, int32 quanta), every
+ * instance as if handled alone:
+ * mv2_batch_entrain: the entrain contract (bulk_state_authority.py:104-127) at
+ *   (instance, instance-local cell) pairs; z_base NULL reads as zero; indices
+ *   unique per call; volume_out[count] = m^3 moved per instance (whole quanta,
+ *   uniform cell area).
+ * mv2_batch_label: failure mask (gpu_large_avalanche.py:59-162 start
+ *   criterion) and connected components (:165-252) per instance; labels is
+ *   the packed layout (instance-local root index or -1); stats_out[4*count] =
+ *   regions, largest, winner (-1 if none), unstable cells per instance. */
+int mv2_batch_entrain(mv2_context *ctx, const float *z_base, const int32_t *inst, const int32_t *cell,
+                      const double *depth, int n, double *volume_out);
+int mv2_batch_label(mv2_context *ctx, const float *z_base, const mv2_avalanche_config *cfg, int32_t *labels,
+                    int32_t *stats_out);
 /* update_previous_and_tiles (gpu_large_avalanche.py:284-306) */
 int mv2_avalanche_previous(mv2_context *ctx, const mv2_avalanche_fields *f, int32_t winner,
                            int keep_candidate, int seed_frontier, int32_t *frontier_reached,

@@ -22,6 +22,8 @@
 // shapes and to the single-terrain step of each instance.
 #include "mv2_internal.hpp"
 #include "mv2_strip32.hpp"
+#include "mv2_relax_tile.hpp"
+#include "mv2_batch.hpp"
 
 #include <cmath>
 #include <cstring>
@@ -415,6 +417,115 @@ __global__ void __launch_bounds__(kBlock) batch_finalize_kernel(BatchArgs a, int
 }
 
 // ---------------------------------------------------------------------------
+// Batched relaxation: the relaxation of every instance of the batch, each with
+// its own iteration count and convergence test (as if relaxed alone), one
+// launch per iteration over the 32 x 48 relaxation tiles of all instances.
+// ---------------------------------------------------------------------------
+struct BRelaxControl {
+    int iterations, src, done, converged;
+    double last;            // maximum excess of the last applied iteration
+};
+
+struct BatchRelaxArgs {
+    int count, tiles;
+    float *h[2];                        // packed field and its ping-pong copy
+    const int32_t *inst_ny, *inst_nx;
+    const int64_t *inst_off;
+    const int32_t *inst_first, *inst_tiles;
+    const int32_t *tile_inst, *tile_r0, *tile_c0;
+    float crit[4];
+    int max_iterations;
+    double tolerance;
+    BRelaxControl *ctl;                 // [2][count]
+    unsigned long long *spread;         // [3][count][kInstWords]: maximum excess
+    mv2_relax_result *out;              // [count]
+};
+
+// loop state of instance i for launch k (every thread of the block calls it)
+__device__ inline BRelaxControl brelax_resolve(const BatchRelaxArgs &a, int k, int i, double *smem1)
+{
+    BRelaxControl C = a.ctl[(k & 1) * a.count + i];
+    if (!C.done && k > 0) {
+        C.last = inst_read_max(inst_set(a.spread, (k - 1) % 3, a.count, i), smem1);
+        if (C.last <= a.tolerance) {
+            C.converged = 1;
+            C.done = 1;
+        } else if (C.iterations >= a.max_iterations) {
+            C.done = 1;
+        }
+    }
+    return C;
+}
+
+__global__ void batch_relax_reset_kernel(BatchRelaxArgs a)
+{
+    for (int i = blockIdx.x; i < a.count; i += gridDim.x) {
+        for (int w = threadIdx.x; w < 2 * (int)(sizeof(BRelaxControl) / 4); w += blockDim.x) {
+            constexpr int kW = (int)(sizeof(BRelaxControl) / 4);
+            reinterpret_cast<unsigned *>(a.ctl + (int64_t)(w / kW) * a.count + i)[w % kW] = 0u;
+        }
+        for (int w = threadIdx.x; w < 3 * kInstSlots; w += blockDim.x)
+            inst_set(a.spread, w / kInstSlots, a.count, i)[(w % kInstSlots) * kInstStride] = 0ull;
+    }
+}
+
+__global__ void __launch_bounds__(kRelaxBlock) batch_relax_kernel(BatchRelaxArgs a, int k)
+{
+    __shared__ float tile[kRelaxLH * kRelaxLW];
+    __shared__ double smem[kRelaxBlock / kWave];
+    const int g = xcd_block();
+    const int i = a.tile_inst[g];
+    const bool first = g == a.inst_first[i];
+    const BRelaxControl C = brelax_resolve(a, k, i, smem);
+    if (C.done) {
+        if (first && threadIdx.x == 0) a.ctl[((k + 1) & 1) * a.count + i] = C;
+        return;
+    }
+    const int src = C.src;
+    if (first && threadIdx.x < kInstSlots)
+        inst_set(a.spread, (k + 1) % 3, a.count, i)[threadIdx.x * kInstStride] = 0ull;
+    if (first && threadIdx.x == 0) {
+        BRelaxControl nc = C;
+        nc.iterations = C.iterations + 1;
+        nc.src = src ^ 1;
+        a.ctl[((k + 1) & 1) * a.count + i] = nc;
+    }
+    const int64_t off = a.inst_off[i];
+    const float tmax = relax_tile_iteration<float>(a.h[src] + off, a.h[src ^ 1] + off, a.inst_ny[i], a.inst_nx[i],
+                                                   a.crit, a.tile_r0[g], a.tile_c0[g], tile);
+    double m[1] = {(double)tmax};
+    block_reduce<double, 1, 1, kRelaxBlock / kWave>(m, smem);
+    if (threadIdx.x == 0) inst_max(inst_set(a.spread, k % 3, a.count, i), m[0]);
+}
+
+// after launch k - 1: copy each tile's interior back when its instance's field
+// sits in scratch; the instance's first tile writes the instance's result
+__global__ void __launch_bounds__(kBlock) batch_relax_end_kernel(BatchRelaxArgs a, int k)
+{
+    __shared__ double smem[1];
+    const int g = xcd_block();
+    const int i = a.tile_inst[g];
+    const BRelaxControl C = brelax_resolve(a, k, i, smem);
+    if (C.src == 1) {
+        const int64_t off = a.inst_off[i];
+        const int ny = a.inst_ny[i], nx = a.inst_nx[i], r0 = a.tile_r0[g], c0 = a.tile_c0[g];
+        for (int e = threadIdx.x; e < kRelaxTH * kRelaxTW; e += blockDim.x) {
+            const int r = r0 + e / kRelaxTW, c = c0 + e % kRelaxTW;
+            if (r < ny && c < nx) a.h[0][off + (int64_t)r * nx + c] = a.h[1][off + (int64_t)r * nx + c];
+        }
+    }
+    if (g == a.inst_first[i] && threadIdx.x == 0) {
+        mv2_relax_result r;
+        r.iterations = C.iterations;
+        r.converged = C.converged;
+        r.max_excess_last = C.last;
+        r.complete = C.done;
+        r.launches = k;
+        a.out[i] = r;
+    }
+}
+
+// ---------------------------------------------------------------------------
 // host side
 // ---------------------------------------------------------------------------
 // tile shapes: kStripW columns by 4 bands of SH rows (one band per wave)
@@ -473,32 +584,6 @@ static void launch_end(int shape, dim3 g, hipStream_t s, const BatchArgs &a, int
 
 using namespace mv2;
 
-// batch state lives in the context (one batch per context)
-struct mv2_batch_state {
-    int count = 0;
-    int64_t cells = 0;
-    double dx = 0.0, dy = 0.0;
-    float *b = nullptr;
-    int32_t *h = nullptr;
-    float *qx = nullptr, *qy = nullptr;
-    std::vector<int32_t> ny, nx;
-    std::vector<int64_t> off;
-    int shape = 0;
-    int tiles = 0;
-    Scratch inst;                        // ny, nx, off, first tile, tile count
-    Scratch tiles_tab;                   // tile -> instance, r0, c0
-    Scratch alt_h, alt_qx, alt_qy;
-    Scratch ctl, wave, stat, ledger, rec;
-    int predicted = 0;
-    int frame = 0;                       // steps taken: the parity picks the loop-state copy
-    int clean = -1;                      // parity whose loop state is known zeroed, -1 none
-    // graphs of eight substep launches, one per parity
-    hipGraphExec_t graph[2] = {nullptr, nullptr};
-    hipStream_t capture = nullptr;
-    hipEvent_t ev_sub0 = nullptr, ev_sub1 = nullptr;   // bracket the substep launches
-    std::vector<unsigned char> graph_key[2];
-};
-
 static mv2_batch_state *batch_of(mv2_context *ctx)
 {
     if (!ctx->batch) ctx->batch = new mv2_batch_state();
@@ -515,7 +600,9 @@ void mv2_batch_release(mv2_context *ctx)
     if (bs->ev_sub0) hipEventDestroy(bs->ev_sub0);
     if (bs->ev_sub1) hipEventDestroy(bs->ev_sub1);
     for (Scratch *s : {&bs->inst, &bs->tiles_tab, &bs->alt_h, &bs->alt_qx, &bs->alt_qy, &bs->ctl, &bs->wave,
-                       &bs->stat, &bs->ledger, &bs->rec})
+                       &bs->stat, &bs->ledger, &bs->rec, &bs->rtab, &bs->relax_alt, &bs->relax_ctl,
+                       &bs->relax_spread, &bs->relax_out, &bs->ctab, &bs->label_counts, &bs->label_stats,
+                       &bs->moved})
         if (s->ptr) hipFree(s->ptr);
     delete bs;
     ctx->batch = nullptr;
@@ -558,6 +645,35 @@ static int build_tiles(mv2_context *ctx, mv2_batch_state *bs, int shape)
             hipGraphExecDestroy(bs->graph[p]);
             bs->graph[p] = nullptr;
         }
+    return MV2_OK;
+}
+
+// relaxation tiles of every instance (32 x 48, row-major per instance)
+static int build_relax_tiles(mv2_context *ctx, mv2_batch_state *bs)
+{
+    std::vector<int32_t> first(bs->count), ntiles(bs->count), ti, tr, tc;
+    for (int i = 0; i < bs->count; i++) {
+        first[i] = (int32_t)ti.size();
+        const int tx = (bs->nx[i] + kRelaxTW - 1) / kRelaxTW, ty = (bs->ny[i] + kRelaxTH - 1) / kRelaxTH;
+        for (int r = 0; r < ty; r++)
+            for (int c = 0; c < tx; c++) {
+                ti.push_back(i);
+                tr.push_back(r * kRelaxTH);
+                tc.push_back(c * kRelaxTW);
+            }
+        ntiles[i] = tx * ty;
+    }
+    const int T = (int)ti.size();
+    const size_t n = (size_t)bs->count;
+    if (int rc = ensure_scratch(ctx, bs->rtab, sizeof(int32_t) * (2 * n + 3 * (size_t)T))) return rc;
+    int32_t *tab = (int32_t *)bs->rtab.ptr;
+    MV2_HIP(ctx, hipMemcpy(tab, first.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice));
+    MV2_HIP(ctx, hipMemcpy(tab + n, ntiles.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice));
+    MV2_HIP(ctx, hipMemcpy(tab + 2 * n, ti.data(), sizeof(int32_t) * T, hipMemcpyHostToDevice));
+    MV2_HIP(ctx, hipMemcpy(tab + 2 * n + T, tr.data(), sizeof(int32_t) * T, hipMemcpyHostToDevice));
+    MV2_HIP(ctx, hipMemcpy(tab + 2 * n + 2 * (size_t)T, tc.data(), sizeof(int32_t) * T, hipMemcpyHostToDevice));
+    bs->rtiles = T;
+    bs->relax_predicted = 0;
     return MV2_OK;
 }
 
@@ -653,7 +769,9 @@ extern "C" int mv2_batch_bind(mv2_context *ctx, int count, const int32_t *ny, co
     if (int rc = ensure_scratch(ctx, bs->stat, sizeof(unsigned long long) * 2 * 2 * kInstWords * (size_t)count))
         return rc;
     bs->clean = -1;                      // new scratch: reset before the next step
+    bs->ctiles = 0;                      // labelling tiles of the new layout: built on first use
     if (int rc = ensure_scratch(ctx, bs->rec, sizeof(mv2_batch_record) * (size_t)count)) return rc;
+    if (int rc = build_relax_tiles(ctx, bs)) return rc;
     return build_tiles(ctx, bs, bs->shape);
 }
 
@@ -793,5 +911,77 @@ extern "C" int mv2_batch_step(mv2_context *ctx, const mv2_step_config *cfg, int 
                              out[i].numeric_error == 1 ? "MobileV2 positivity failure"
                              : out[i].numeric_error == 2 ? "MobileV2 maximum_substeps exceeded"
                                                          : "mobile thickness exceeds the FP32 storage range");
+    return MV2_OK;
+}
+
+extern "C" int mv2_batch_relax(mv2_context *ctx, float *field, const double crit_diff[4], int max_iterations,
+                               double tolerance, mv2_relax_result *out)
+{
+    if (!ctx) return set_error(nullptr, MV2_EINVAL, "[BatchedMiniSlope] null context");
+    mv2_batch_state *bs = ctx->batch;
+    if (!bs || !bs->count) return set_error(ctx, MV2_ESTATE, "[BatchedMiniSlope] bind a batch first");
+    if (!field || !crit_diff || !out) return set_error(ctx, MV2_EINVAL, "[BatchedMiniSlope] null argument");
+    if (max_iterations < 1) return set_error(ctx, MV2_EINVAL, "[BatchedMiniSlope] max_iterations must be >= 1");
+    hipSetDevice(ctx->device);
+    const size_t n = (size_t)bs->count;
+    if (int rc = ensure_scratch(ctx, bs->relax_alt, sizeof(float) * (size_t)bs->cells)) return rc;
+    if (int rc = ensure_scratch(ctx, bs->relax_ctl, sizeof(BRelaxControl) * 2 * n)) return rc;
+    if (int rc = ensure_scratch(ctx, bs->relax_spread, sizeof(unsigned long long) * 3 * kInstWords * n)) return rc;
+    if (int rc = ensure_scratch(ctx, bs->relax_out, sizeof(mv2_relax_result) * n)) return rc;
+    BatchRelaxArgs a;
+    a.count = bs->count;
+    a.tiles = bs->rtiles;
+    a.h[0] = field;
+    a.h[1] = (float *)bs->relax_alt.ptr;
+    const int32_t *inst = (const int32_t *)bs->inst.ptr;
+    a.inst_ny = inst;
+    a.inst_nx = inst + n;
+    a.inst_off = (const int64_t *)(inst + 2 * n);
+    const int32_t *tab = (const int32_t *)bs->rtab.ptr;
+    const size_t T = (size_t)bs->rtiles;
+    a.inst_first = tab;
+    a.inst_tiles = tab + n;
+    a.tile_inst = tab + 2 * n;
+    a.tile_r0 = tab + 2 * n + T;
+    a.tile_c0 = tab + 2 * n + 2 * T;
+    for (int d = 0; d < 4; d++) a.crit[d] = (float)crit_diff[d];
+    a.max_iterations = max_iterations;
+    a.tolerance = tolerance;
+    a.ctl = (BRelaxControl *)bs->relax_ctl.ptr;
+    a.spread = (unsigned long long *)bs->relax_spread.ptr;
+    a.out = (mv2_relax_result *)bs->relax_out.ptr;
+    hipStream_t st = ctx->stream;
+    hipLaunchKernelGGL(batch_relax_reset_kernel, dim3(bs->count < 1024 ? bs->count : 1024), dim3(kWave), 0, st, a);
+    MV2_HIP(ctx, hipGetLastError());
+    const size_t out_bytes = sizeof(mv2_relax_result) * n;
+    if (int rc = ensure_scratch(ctx, ctx->host_result, out_bytes + 4096, true)) return rc;
+    const mv2_relax_result *hres = (const mv2_relax_result *)ctx->host_result.ptr;
+    // as the single-field relaxation: the previous call's count + 1 (2..32),
+    // then geometric growth while some instance has not finished
+    int chunk = bs->relax_predicted + 1;
+    if (chunk < 2) chunk = 2;
+    if (chunk > 32) chunk = 32;
+    int k = 0;
+    for (;;) {
+        if (chunk > max_iterations - k) chunk = max_iterations - k;
+        for (int j = 0; j < chunk; j++, k++)
+            hipLaunchKernelGGL(batch_relax_kernel, dim3(bs->rtiles), dim3(kRelaxBlock), 0, st, a, k);
+        hipLaunchKernelGGL(batch_relax_end_kernel, dim3(bs->rtiles), dim3(kBlock), 0, st, a, k);
+        MV2_HIP(ctx, hipGetLastError());
+        MV2_HIP(ctx, hipMemcpyAsync(ctx->host_result.ptr, a.out, out_bytes, hipMemcpyDeviceToHost, st));
+        MV2_HIP(ctx, hipStreamSynchronize(st));
+        bool all = true;
+        int most = 0;
+        for (int i = 0; i < bs->count; i++) {
+            all = all && hres[i].complete;
+            most = hres[i].iterations > most ? hres[i].iterations : most;
+        }
+        if (all || k >= max_iterations) {
+            bs->relax_predicted = most;
+            break;
+        }
+        chunk = k / 2 > 8 ? k / 2 : 8;
+    }
+    memcpy(out, hres, out_bytes);
     return MV2_OK;
 }

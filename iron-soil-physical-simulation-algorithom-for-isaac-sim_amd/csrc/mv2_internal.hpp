@@ -265,6 +265,16 @@ __device__ inline void atomic_max_nonneg(unsigned long long *slot, double v)
     atomicMax(slot, (unsigned long long)__double_as_longlong(v));
 }
 
+// Workgroups are dispatched round-robin over the 8 XCDs.  The renumbered
+// index gives the workgroups of one XCD one contiguous range of [0, G): tiles
+// numbered row-major then share their halo rows and columns (and the cache
+// lines a 62-column strip straddles) inside one XCD's L2.
+__device__ inline int xcd_block()
+{
+    const int G = gridDim.x, b = blockIdx.x;
+    const int x = b % 8, j = b / 8;
+    return x * (G / 8) + min(x, G % 8) + j;
+}
 // ---- spread maxima -----------------------------------------------------------------
 // Device-scope atomics on one address serialise (~12 ns each on MI355X, so a
 // launch of ~1000 workgroups pays ~12 us for one global maximum).  A spread

@@ -40,16 +40,6 @@ __device__ inline int from_next_lane(int v) { return __builtin_amdgcn_mov_dpp(v,
 __device__ inline int from_prev_lane(int v) { return __builtin_amdgcn_mov_dpp(v, 0x138, 0xf, 0xf, true); }
 // the wave's index in its workgroup, as a scalar
 __device__ inline int wave_index() { return __builtin_amdgcn_readfirstlane((int)(threadIdx.x / kWave)); }
-// Workgroups are dispatched round-robin over the 8 XCDs.  The renumbered
-// index gives the workgroups of one XCD one contiguous range of [0, G): tiles
-// numbered row-major then share their halo rows and columns (and the cache
-// lines a 62-column strip straddles) inside one XCD's L2.
-__device__ inline int xcd_block()
-{
-    const int G = gridDim.x, b = blockIdx.x;
-    const int x = b % 8, j = b / 8;
-    return x * (G / 8) + min(x, G % 8) + j;
-}
 // a wave-uniform value, held in a scalar register
 __device__ inline float uniform(float v)
 {

@@ -103,6 +103,9 @@ SIGNATURES: dict[str, tuple[object, list[object]]] = {
     "mv2_batch_set_tile_shape": (_I, [_P, _I]),
     "mv2_batch_tile_shape": (_I, [_I, _P, _P]),
     "mv2_batch_step": (_I, [_P, ctypes.POINTER(StepConfig), _I, _P, _P]),
+    "mv2_batch_relax": (_I, [_P, _P, ctypes.POINTER(_D), _I, _D, _P]),
+    "mv2_batch_entrain": (_I, [_P, _P, _P, _P, _P, _I, _P]),
+    "mv2_batch_label": (_I, [_P, _P, ctypes.POINTER(AvalancheConfig), _P, _P]),
     "mv2_entrain": (_I, [_P, _P, _P, _I, _I, ctypes.POINTER(_D)]),
     "mv2_deposit": (_I, [_P, _P, _P, _I, _I, _D, _P]),
     "mv2_reservoirs": (_I, [_P, _I, _D, _P]),

@@ -161,6 +161,80 @@ class BatchedMobileV2:
         rt.check(rc)
         return BatchRecords(table)
 
+    # ---- relaxation, exchange and labelling of every instance at once -------
+    def relax(self, critical_angle_deg: float, tolerance: float, *, max_iterations: int = 20000,
+              field=None) -> np.ndarray:
+        """Angle-of-repose relaxation of every instance (of b_eff, or of `field`
+        laid out like it), each to its own convergence as if relaxed alone
+        (relax_critical_slope, slope_model.py:273-343).  Returns a structured
+        array per instance: iterations, converged, max_excess_last."""
+        from .relaxation import legacy_critical_differences
+
+        rt = self.rt
+        rt.bind_stream()
+        target = self.b if field is None else field
+        if target.dtype != _torch().float32 or target.numel() != self.cells or not target.is_contiguous():
+            raise ValueError("[BatchedMiniSlope] field must be a contiguous float32 array of the batch's cells")
+        crit = (ctypes.c_double * 4)(*legacy_critical_differences(self.dx, self.dy, critical_angle_deg))
+        out = (_lib.RelaxResult * self.count)()
+        rt.check(rt.lib.mv2_batch_relax(rt.ctx, target.data_ptr(), crit, int(max_iterations), float(tolerance),
+                                        out))
+        res = np.zeros(self.count, [("iterations", "<i4"), ("converged", "?"), ("max_excess_last", "<f8")])
+        for i, r in enumerate(out):
+            res[i] = (r.iterations, bool(r.converged), r.max_excess_last)
+        return res
+
+    def entrain(self, instance, cell, depth_m, *, z_base=None) -> np.ndarray:
+        """The entrain contract (bulk_state_authority.py:104-127) at (instance,
+        instance-local flat cell) pairs; returns the volume moved per instance
+        (m^3, whole quanta, uniform cell area).  Indices must be unique."""
+        torch = _torch()
+        rt = self.rt
+        rt.bind_stream()
+        dev = rt.device
+        inst = torch.as_tensor(np.asarray(instance, np.int32), device=dev)
+        cells = torch.as_tensor(np.asarray(cell, np.int32), device=dev)
+        depth = torch.as_tensor(np.asarray(depth_m, np.float64), device=dev)
+        n = int(inst.numel())
+        if cells.numel() != n or depth.numel() != n:
+            raise ValueError("[BatchedDeviceBulkState] instance, cell and depth must have the same length")
+        if n and (int(inst.min()) < 0 or int(inst.max()) >= self.count):
+            raise ValueError("[BatchedDeviceBulkState] instance index out of range")
+        if n:
+            sizes = torch.as_tensor((self.ny.astype(np.int64) * self.nx.astype(np.int64)), device=dev)
+            if bool((cells < 0).any()) or bool((cells.to(torch.int64) >= sizes[inst.to(torch.int64)]).any()):
+                raise ValueError("[BatchedDeviceBulkState] cell index out of range")
+        zb = 0 if z_base is None else z_base.data_ptr()
+        vol = np.zeros(self.count, np.float64)
+        rt.check(rt.lib.mv2_batch_entrain(rt.ctx, zb, inst.data_ptr(), cells.data_ptr(), depth.data_ptr(), n,
+                                          vol.ctypes.data))
+        return vol
+
+    def label(self, *, start_angle_deg: float, stop_angle_deg: float, density: float, cohesion: float,
+              mobilization_depth: float, dry_tolerance: float = 1e-8, connectivity: int = 8,
+              gravity: float = 9.81, z_base=None, labels=None):
+        """Failure mask (gpu_large_avalanche.py:59-162) and connected components
+        (:165-252) of every instance.  Returns (labels, stats): labels is the
+        packed int32 array (instance-local root index, -1 outside the mask),
+        stats an int32 [count, 4] array of region count, largest component,
+        winner root (-1 if none) and unstable cells per instance."""
+        torch = _torch()
+        rt = self.rt
+        rt.bind_stream()
+        if labels is None:
+            labels = torch.empty(self.cells, dtype=torch.int32, device=rt.device)
+        start, stop = float(np.deg2rad(start_angle_deg)), float(np.deg2rad(stop_angle_deg))
+        cfg = _lib.AvalancheConfig(tan_start=float(np.tan(start)), tan_stop=float(np.tan(stop)), start_rad=start,
+                                   stop_rad=stop, density=float(density), gravity=float(gravity),
+                                   cohesion=float(cohesion), mobilization_depth=float(mobilization_depth),
+                                   dry_tolerance=float(dry_tolerance), activity_speed=0.0,
+                                   connectivity=int(connectivity), release_settled_latches=0, weight_mode=0,
+                                   reserved=0)
+        stats = np.zeros((self.count, 4), np.int32)
+        zb = 0 if z_base is None else z_base.data_ptr()
+        rt.check(rt.lib.mv2_batch_label(rt.ctx, zb, ctypes.byref(cfg), labels.data_ptr(), stats.ctypes.data))
+        return labels, stats
+
     def fields(self, i: int):
         """(b_eff, h_mobile [m], qx, qy) of instance i as numpy arrays."""
         o, n = int(self.offset[i]), int(self.ny[i]) * int(self.nx[i])

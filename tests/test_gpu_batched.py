@@ -105,3 +105,99 @@ def test_autotune_picks_a_shape_and_keeps_the_state():
     assert result["best"] in TILE_SHAPES and set(result["ms"]) == set(TILE_SHAPES)
     for i in range(3):
         np.testing.assert_array_equal(batch.fields(i)[1], before[i])
+
+
+def _relax_terrains():
+    rng = np.random.default_rng(7)
+    out = []
+    for ny, nx in [(40, 37), (65, 130), (97, 33), (2, 2), (128, 96), (17, 50)]:
+        b = rng.uniform(0.0, 1.5, size=(ny, nx)).astype(np.float32)
+        out.append((b, np.zeros((ny, nx))))
+    return out
+
+
+def test_batched_relax_matches_single_field_relaxation():
+    """Every instance relaxed in the batch equals the same field relaxed alone
+    (FP32), bit for bit, with the same iteration count."""
+    import torch
+    from isaac_bulk_hip import relax_device
+    from isaac_bulk_hip.batched import BatchedMobileV2
+    from isaac_bulk_hip.relaxation import _runtime
+
+    terrains = _relax_terrains()
+    batch = BatchedMobileV2(terrains, 0.05, runtime=_runtime("cuda:0"))
+    res = batch.relax(34.0, 1e-6, max_iterations=5000)
+    for i, (b, _) in enumerate(terrains):
+        t = torch.from_numpy(b.copy()).to("cuda:0")
+        it, conv, _ = relax_device(t, 0.05, 0.05, 34.0, 5000, 1e-6)
+        assert (int(res["iterations"][i]), bool(res["converged"][i])) == (it, conv), i
+        np.testing.assert_array_equal(batch.fields(i)[0], t.cpu().numpy())
+    # a second call starts from rest: every instance converges at once
+    again = batch.relax(34.0, 1e-6)
+    assert again["converged"].all() and (again["iterations"] <= 1).all()
+
+
+def test_batched_entrain_contract():
+    """entrain moves whole quanta, limited by the donor layer, per instance."""
+    from isaac_bulk_hip.batched import BatchedMobileV2
+    from isaac_bulk_hip.relaxation import _runtime
+
+    terrains = _relax_terrains()[:3]
+    batch = BatchedMobileV2(terrains, 0.05, runtime=_runtime("cuda:0"))
+    before = [batch.fields(i) for i in range(3)]
+    inst = np.array([0, 0, 1, 2, 2], np.int32)
+    cell = np.array([0, 5, 17, 3, 40], np.int32)
+    depth = np.array([0.01, 5.0, 0.0123456, -1.0, 0.2])
+    vol = batch.entrain(inst, cell, depth)
+    expect = np.zeros(3)
+    for k in range(len(inst)):
+        i, c = int(inst[k]), int(cell[k])
+        b0 = float(before[i][0].ravel()[c])
+        d = min(max(depth[k], 0.0), max(b0, 0.0))
+        dq = int(np.floor(d * QUANTUM ** -1))
+        b1, h1 = batch.fields(i)[0].ravel()[c], batch.fields(i)[1].ravel()[c]
+        assert b1 == np.float32(b0 - dq * QUANTUM)
+        assert np.rint(h1 / QUANTUM) == dq
+        expect[i] += dq * QUANTUM * 0.05 * 0.05
+    np.testing.assert_allclose(vol, expect, rtol=0, atol=1e-18)
+    # cells outside the lists are untouched
+    assert np.array_equal(batch.fields(1)[0].ravel()[:17], before[1][0].ravel()[:17])
+
+
+@pytest.mark.parametrize("connectivity", [4, 8])
+def test_batched_label_matches_oracle(connectivity):
+    """Failure masks, labels and component statistics of every instance equal
+    the oracle's on that terrain alone (FP32 storage read into FP64)."""
+    import oracle
+    from isaac_bulk_hip.batched import BatchedMobileV2
+    from isaac_bulk_hip.relaxation import _runtime
+    from isaac_bulk_hip.workloads import cut_face_terrain
+
+    terrains = []
+    for k, n in enumerate((96, 64, 50, 33)):
+        b, h, _ = cut_face_terrain(n, seed=k, face_deg=45.0)
+        terrains.append((b.astype(np.float32), h))
+    batch = BatchedMobileV2(terrains, 0.05, runtime=_runtime("cuda:0"))
+    labels, stats = batch.label(start_angle_deg=38.0, stop_angle_deg=30.0, density=2200.0, cohesion=0.0,
+                                mobilization_depth=0.05, dry_tolerance=1e-8, connectivity=connectivity)
+    labels = labels.cpu().numpy()
+    some = 0
+    for i, (b, h) in enumerate(terrains):
+        ny, nx = b.shape
+        bf, hf = batch.fields(i)[0].astype(np.float64), batch.fields(i)[1]
+        z = np.zeros((ny, nx))
+        o = oracle.avalanche_initialize(
+            z, bf, hf, z, z, np.ones((ny, nx)), z, z, np.zeros((ny, nx), np.int32), dx=0.05, dy=0.05,
+            tan_start=float(np.tan(np.deg2rad(38.0))), tan_stop=float(np.tan(np.deg2rad(30.0))),
+            density=2200.0, gravity=9.81, cohesion=0.0, mobilization_depth=0.05, dry=1e-8,
+            activity_speed=0.0, release=False)
+        lab = oracle.label(o["unstable"], connectivity)
+        off = int(batch.offset[i])
+        np.testing.assert_array_equal(labels[off:off + ny * nx].reshape(ny, nx), lab)
+        roots, counts = np.unique(lab[lab >= 0], return_counts=True)
+        regions = len(roots)
+        largest = int(counts.max()) if regions else 0
+        winner = int(roots[counts == largest].min()) if regions else -1
+        assert stats[i].tolist() == [regions, largest, winner, int((lab >= 0).sum())], i
+        some += regions
+    assert some > 0
