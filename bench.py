@@ -58,8 +58,12 @@ def parse() -> argparse.Namespace:
     p.add_argument("--no-cpu-baseline", action="store_true")
     p.add_argument("--skip-relax-to-rest", action="store_true",
                    help="omit the relax-to-rest measurement (short profiling runs)")
-    p.add_argument("--workload", choices=("frame", "batched"), default="frame",
-                   help="frame: the 701x701 frame (headline); batched: 256 uneven terrains")
+    p.add_argument("--workload", choices=("frame", "batched", "digging"), default="frame",
+                   help="frame: the 701x701 frame (headline); batched: 256 uneven terrains; "
+                        "digging: 64 terrains of 256^2 with a bucket sweep")
+    p.add_argument("--dig-count", type=int, default=64)
+    p.add_argument("--dig-relax", choices=("learned", "plain"), default="learned",
+                   help="digging: learned warm start + exact relaxation, or the plain relaxation")
     p.add_argument("--batch-count", type=int, default=256)
     p.add_argument("--calibrate-copy", action="store_true",
                    help="only copy a 64 MiB float32 tensor 10 times (scale of the HBM byte counters)")
@@ -203,6 +207,107 @@ def run_batched(args) -> int:
     return 0
 
 
+def run_digging(args) -> int:
+    """Digging workload: 64 terrains of 256^2 (Gaussian piles on a 0.2 m floor,
+    relaxed to rest at 34 degrees before timing), each with a 1.5 x 0.8 m
+    bucket swept 0.02 m per frame along a seeded line;
+    per 1/60 s frame and for every terrain at once: entrainment of 0.01 m under
+    the bucket (entrain contract), the Mobile V2 step, relaxation of b_eff at
+    34 degrees to 1e-6 m (learned warm start + exact sweeps, or plain), failure
+    labelling.  Run it with --steps 180 for the 3 s sweep."""
+    import torch
+    from isaac_bulk_hip import MobileV2Config
+    from isaac_bulk_hip.batched import BatchedMobileV2
+    from isaac_bulk_hip.workloads import bucket_cells, digging_sites
+
+    n = 256
+    terrains, lines = digging_sites(args.dig_count, n, DX, seed=args.seed)
+    batch = BatchedMobileV2([(b.astype(np.float32), h) for b, h in terrains], DX, config=MobileV2Config(cfl=0.18))
+    frames = args.warmup + args.steps
+    # bucket index lists of every frame, uploaded before timing
+    plan = []
+    for f in range(frames):
+        cells = [bucket_cells(line, f * 0.02, n, DX) for line in lines]
+        inst = np.concatenate([np.full(len(c), i, np.int32) for i, c in enumerate(cells)])
+        plan.append((torch.from_numpy(inst).to("cuda:0"), torch.from_numpy(np.concatenate(cells)).to("cuda:0"),
+                     torch.full((len(inst),), 0.01, dtype=torch.float64, device="cuda:0")))
+    # the piles start at rest (relaxed to 34 degrees), then the bucket digs
+    rest = batch.relax(34.0, 1e-6, max_iterations=20000)
+    net = None
+    if args.dig_relax == "learned":
+        from isaac_bulk_hip.warm_start_training import load_net
+
+        net, _ = load_net(batch.rt)
+    env_launches = 0
+    batch.entrain(*plan[0], check=True)            # validates the index layout once (and digs frame 0)
+
+    def total_volume():
+        return float(batch.b.double().sum().item() + batch.h.double().sum().item() * 2.0 ** -24) * DX * DX
+
+    cells_total = int(batch.cells)
+    relax_it, relax_max, cell_sub, dug = 0, 0, 0, 0.0
+    v2_s = 0.0
+    t_parts = {"entrain": 0.0, "v2": 0.0, "relax": 0.0, "label": 0.0}
+    for f in range(frames):
+        timed = f >= args.warmup
+        if f == args.warmup:
+            torch.cuda.synchronize()
+            volume0 = total_volume()
+            t0 = time.perf_counter()
+        ta = time.perf_counter()
+        if f > 0:
+            vol = batch.entrain(*plan[f], check=False)
+            if timed:
+                dug += float(vol.sum())
+        tb = time.perf_counter()
+        recs = batch.step(FRAME_DT)
+        tc = time.perf_counter()
+        if net is not None:
+            ws = batch.warm_start_relax(net, 34.0, 1e-6, max_iterations=args.relax_max_iterations)
+            rr = ws["relax"]
+            if timed:
+                env_launches += ws["envelope_launches"]
+        else:
+            rr = batch.relax(34.0, 1e-6, max_iterations=args.relax_max_iterations)
+        td = time.perf_counter()
+        _, stats = batch.label(start_angle_deg=38.0, stop_angle_deg=30.0, density=1370.0, cohesion=0.0,
+                               mobilization_depth=0.05, dry_tolerance=1e-8, connectivity=8)
+        te = time.perf_counter()
+        if timed:
+            cell_sub += int(np.dot(batch.ny.astype(np.int64) * batch.nx.astype(np.int64),
+                                   recs.substeps.astype(np.int64)))
+            relax_it += int(rr["iterations"].sum())
+            relax_max = max(relax_max, int(rr["iterations"].max()))
+            for k, v in zip(t_parts, (tb - ta, tc - tb, td - tc, te - td)):
+                t_parts[k] += v
+            v2_s += tc - tb
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    volume1 = total_volume()
+    out = {
+        "metric": "digging frames/s (entrain + Mobile V2 + relaxation + labelling, 64 terrains)",
+        "value": args.steps / wall, "unit": "frames/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": wall * 1e3 / max(args.steps, 1), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32", "data": "synthetic (seeded Gaussian piles, seeded bucket lines)",
+        "config": {"workload": f"{args.dig_count} terrains of 256^2, bucket 1.5 x 0.8 m at 0.02 m/frame, "
+                               "0.01 m entrained per frame, relaxation 34 deg to 1e-6 m, 1/60 s frames",
+                   "cells": cells_total, "frame_dt_s": FRAME_DT},
+        "v2_cell_substeps_per_s": cell_sub / v2_s if v2_s > 0 else None,
+        "ms_per_frame_parts": {k: v * 1e3 / max(args.steps, 1) for k, v in t_parts.items()},
+        "relax_iterations_per_terrain_frame": relax_it / max(args.steps * args.dig_count, 1),
+        "relax_iterations_max": relax_max,
+        "relaxation": args.dig_relax,
+        "envelope_launches_per_frame": env_launches / max(args.steps, 1),
+        "setup_relax_to_rest_iterations_max": int(rest["iterations"].max()),
+        "entrained_m3": dug,
+        "volume_rel_change": (volume1 - volume0) / volume0,
+        "unstable_cells_last_frame": int(stats[:, 3].sum()),
+    }
+    print(json.dumps(out), flush=True)
+    batch.rt.close()
+    return 0
+
+
 def main() -> int:
     args = parse()
     if args.gpus != 1:
@@ -224,6 +329,8 @@ def main() -> int:
         return 0
     if args.workload == "batched":
         return run_batched(args)
+    if args.workload == "digging":
+        return run_digging(args)
     state, solver, bridge, b0, h0 = build_state(args.precision, args.seed)
 
     # relax-to-rest of the initial terrain (plain relaxation, same tolerance as the frames)

@@ -309,6 +309,13 @@ int mv2_nn_commit_surface(mv2_context *ctx, float *b, const float *z, int64_t n,
  * directions; device float32 H[y, x] fields.  launches_out[0] = launches used. */
 int mv2_envelope(mv2_context *ctx, const float *b, const float *delta, float *z, int ny, int nx,
                  const double crit_diff[4], int max_launches, int *launches_out);
+/* batched learned warm start for `count` fields of equal size stacked back to
+ * back ([count][ny][nx]): the envelope of every field (no edge crosses from
+ * one field to the next) and the mass projection per field.  No reference
+ * counterpart (the learned warm start is this project's). */
+int mv2_batch_envelope(mv2_context *ctx, const float *b, const float *delta, float *z, int count, int ny, int nx,
+                       const double crit_diff[4], int max_launches, int *launches_out);
+int mv2_batch_commit_surface(mv2_context *ctx, float *b, const float *z, int count, int64_t n, double eps);
 /* batched relaxation of independent FP32 patches (<= 4096 cells each), one
  * workgroup per patch, same schedule as mv2_relax; iterations per patch out */
 int mv2_relax_patches(mv2_context *ctx, float *h, int npatch, int H, int W,

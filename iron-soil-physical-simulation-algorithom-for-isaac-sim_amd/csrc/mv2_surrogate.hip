@@ -475,8 +475,12 @@ __global__ void __launch_bounds__(kBlock) excess_loss_kernel(const float *pred, 
 }
 
 // Mass projection of a change delta of b (delta = given array, or z - b for a
-// candidate surface z): remove the mean of the change over the cells it moves
-// (|delta| > eps) so the change sums to zero there, then keep heights >= 0.
+// candidate surface z): remove a shift s from the change over the cells it
+// moves (|delta| > eps) and keep heights >= 0, with s such that the moved cells
+// keep their total: first s = the mean change, then Newton steps on the mass
+// the moved cells gain, s += gain(s) / (moved cells above zero).  gain is
+// convex and decreasing in s, so the steps approach its zero from one side;
+// without cells held at zero the first s already zeroes it (to rounding).
 // Field-major partials: sum at [g], count at [gridDim.x + g].
 template <bool FROM_SURFACE>
 __device__ inline float change_at(const float *b, const float *src, int64_t p)
@@ -484,11 +488,17 @@ __device__ inline float change_at(const float *b, const float *src, int64_t p)
     return FROM_SURFACE ? src[p] - b[p] : src[p];
 }
 
+// Segments: the field is `gridDim.y` independent segments of n cells each
+// (segment s at offset s * n), each with its own shift.
 template <bool FROM_SURFACE>
 __global__ void __launch_bounds__(kBlock) delta_stats_kernel(const float *b, const float *src, int64_t n, float eps,
                                                              double *partials)
 {
     __shared__ double smem[8];
+    const int64_t base = (int64_t)blockIdx.y * n;
+    b += base;
+    src += base;
+    partials += (int64_t)blockIdx.y * 2 * gridDim.x;
     double s[2] = {0.0, 0.0};
     for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < n; p += (int64_t)gridDim.x * blockDim.x) {
         const float d = change_at<FROM_SURFACE>(b, src, p);
@@ -504,24 +514,71 @@ __global__ void __launch_bounds__(kBlock) delta_stats_kernel(const float *b, con
     }
 }
 
-// fold the partials (one workgroup, fixed order) into the shift
+// fold the partials of segment blockIdx.x (one workgroup, fixed order) into its shift
 __global__ void __launch_bounds__(kBlock) delta_shift_kernel(const double *partials, int nparts, float *shift)
 {
     __shared__ double smem[8];
+    partials += (int64_t)blockIdx.x * 2 * nparts;
     double s[2] = {0.0, 0.0};
     for (int k = threadIdx.x; k < nparts; k += blockDim.x) {
         s[0] += partials[k];
         s[1] += partials[nparts + k];
     }
     block_reduce<double, 2, 0>(s, smem);
-    if (threadIdx.x == 0) *shift = s[1] > 0.0 ? (float)(s[0] / s[1]) : 0.0f;
+    if (threadIdx.x == 0) shift[blockIdx.x] = s[1] > 0.0 ? (float)(s[0] / s[1]) : 0.0f;
+}
+
+// mass the moved cells gain at the current shift (clamp at zero included),
+// and the moved cells that stay above zero (same arithmetic as apply_delta_kernel)
+template <bool FROM_SURFACE>
+__global__ void __launch_bounds__(kBlock) clamp_stats_kernel(const float *b, const float *src, int64_t n, float eps,
+                                                             const float *shift_ptr, double *partials)
+{
+    __shared__ double smem[8];
+    const float shift = shift_ptr[blockIdx.y];
+    const int64_t base = (int64_t)blockIdx.y * n;
+    b += base;
+    src += base;
+    partials += (int64_t)blockIdx.y * 2 * gridDim.x;
+    double s[2] = {0.0, 0.0};
+    for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < n; p += (int64_t)gridDim.x * blockDim.x) {
+        const float d = change_at<FROM_SURFACE>(b, src, p);
+        if (fabsf(d) > eps) {
+            float v = b[p] + (d - shift);
+            v = v > 0.0f ? v : 0.0f;
+            s[0] += (double)v - (double)b[p];     // mass the cell gains at this shift
+            if (v > 0.0f) s[1] += 1.0;
+        }
+    }
+    block_reduce<double, 2, 0>(s, smem);
+    if (threadIdx.x == 0) {
+        partials[blockIdx.x] = s[0];
+        partials[gridDim.x + blockIdx.x] = s[1];
+    }
+}
+
+// Newton step on the gained mass of segment blockIdx.x: shift += gain / cells above zero
+__global__ void __launch_bounds__(kBlock) clamp_shift_kernel(const double *partials, int nparts, float *shift)
+{
+    __shared__ double smem[8];
+    partials += (int64_t)blockIdx.x * 2 * nparts;
+    double s[2] = {0.0, 0.0};
+    for (int k = threadIdx.x; k < nparts; k += blockDim.x) {
+        s[0] += partials[k];
+        s[1] += partials[nparts + k];
+    }
+    block_reduce<double, 2, 0>(s, smem);
+    if (threadIdx.x == 0 && s[1] > 0.0) shift[blockIdx.x] = (float)((double)shift[blockIdx.x] + s[0] / s[1]);
 }
 
 template <bool FROM_SURFACE>
 __global__ void __launch_bounds__(kBlock) apply_delta_kernel(float *b, const float *src, int64_t n, float eps,
                                                              const float *shift_ptr)
 {
-    const float shift = *shift_ptr;
+    const float shift = shift_ptr[blockIdx.y];
+    const int64_t base = (int64_t)blockIdx.y * n;
+    b += base;
+    src += base;
     for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < n; p += (int64_t)gridDim.x * blockDim.x) {
         const float d = change_at<FROM_SURFACE>(b, src, p);
         if (fabsf(d) > eps) {
@@ -562,7 +619,7 @@ __global__ void add_surface_kernel(const float *b, const float *delta, float *z,
 
 __global__ void __launch_bounds__(kBlock) envelope_kernel(float *z, int ny, int nx, float c0, float c1, float c2,
                                                           float c3, int tiles_x, EnvControl *ctl,
-                                                          unsigned long long *spread, int k)
+                                                          unsigned long long *spread, int k, int seam)
 {
     __shared__ float t[kEnvLH * kEnvLW];
     __shared__ double smem[1];
@@ -604,11 +661,16 @@ __global__ void __launch_bounds__(kBlock) envelope_kernel(float *z, int ny, int 
             // LDS accesses, any value read is a valid upper bound
             const float v = __hip_atomic_load(&t[e], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
             if (v == __builtin_huge_valf()) continue;        // outside the grid
+            // stacked independent fields: no edge crosses a seam row boundary
+            const int grow = R0 + lr;
+            const bool seam_below = seam > 0 && (grow + 1) % seam == 0;
+            const bool seam_above = seam > 0 && grow % seam == 0;
             float m = v;
 #pragma unroll
             for (int q = 0; q < 8; q++) {
                 const int rr = lr + nr[q], cc = lc + ncol[q];
                 if (rr < 0 || rr >= kEnvLH || cc < 0 || cc >= kEnvLW) continue;
+                if ((nr[q] > 0 && seam_below) || (nr[q] < 0 && seam_above)) continue;
                 const float cand = __hip_atomic_load(&t[rr * kEnvLW + cc], __ATOMIC_RELAXED,
                                                      __HIP_MEMORY_SCOPE_WORKGROUP) + lim[q];
                 m = cand < m ? cand : m;
@@ -819,18 +881,26 @@ extern "C" int mv2_relax_patches(mv2_context *ctx, float *h, int npatch, int H, 
     return MV2_OK;
 }
 
+constexpr int kClampCorrections = 4;
+
 template <bool FROM_SURFACE>
-static int project_change(mv2_context *ctx, float *b, const float *src, int64_t n, double eps)
+static int project_change(mv2_context *ctx, float *b, const float *src, int64_t n, double eps, int segments = 1)
 {
-    const int grid = grid_for(n, kBlock, 1024);
-    if (int rc = ensure_scratch(ctx, ctx->nn_loss, sizeof(double) * 2 * 1024 + 64)) return rc;
+    const int grid = grid_for(n, kBlock, 1024);     // per segment: the same partials as one field
+    const size_t parts_n = (size_t)2 * grid * segments;
+    if (int rc = ensure_scratch(ctx, ctx->nn_loss, sizeof(double) * parts_n + sizeof(float) * segments + 64)) return rc;
     double *parts = (double *)ctx->nn_loss.ptr;
-    float *shift = (float *)(parts + 2 * 1024);
-    hipLaunchKernelGGL(delta_stats_kernel<FROM_SURFACE>, dim3(grid), dim3(kBlock), 0, ctx->stream, b, src, n,
-                       (float)eps, parts);
-    hipLaunchKernelGGL(delta_shift_kernel, dim3(1), dim3(kBlock), 0, ctx->stream, parts, grid, shift);
-    hipLaunchKernelGGL(apply_delta_kernel<FROM_SURFACE>, dim3(grid), dim3(kBlock), 0, ctx->stream, b, src, n,
-                       (float)eps, shift);
+    float *shift = (float *)(parts + parts_n);
+    hipLaunchKernelGGL(delta_stats_kernel<FROM_SURFACE>, dim3(grid, segments), dim3(kBlock), 0, ctx->stream, b, src,
+                       n, (float)eps, parts);
+    hipLaunchKernelGGL(delta_shift_kernel, dim3(segments), dim3(kBlock), 0, ctx->stream, parts, grid, shift);
+    for (int c = 0; c < kClampCorrections; c++) {
+        hipLaunchKernelGGL(clamp_stats_kernel<FROM_SURFACE>, dim3(grid, segments), dim3(kBlock), 0, ctx->stream, b,
+                           src, n, (float)eps, shift, parts);
+        hipLaunchKernelGGL(clamp_shift_kernel, dim3(segments), dim3(kBlock), 0, ctx->stream, parts, grid, shift);
+    }
+    hipLaunchKernelGGL(apply_delta_kernel<FROM_SURFACE>, dim3(grid, segments), dim3(kBlock), 0, ctx->stream, b, src,
+                       n, (float)eps, shift);
     MV2_HIP(ctx, hipGetLastError());
     return MV2_OK;
 }
@@ -849,8 +919,33 @@ extern "C" int mv2_nn_commit_surface(mv2_context *ctx, float *b, const float *z,
     return project_change<true>(ctx, b, z, n, eps);
 }
 
+static int envelope_run(mv2_context *ctx, const float *b, const float *delta, float *z, int ny, int nx,
+                        const double crit_diff[4], int max_launches, int *launches_out, int seam);
+
 extern "C" int mv2_envelope(mv2_context *ctx, const float *b, const float *delta, float *z, int ny, int nx,
                             const double crit_diff[4], int max_launches, int *launches_out)
+{
+    return envelope_run(ctx, b, delta, z, ny, nx, crit_diff, max_launches, launches_out, 0);
+}
+
+extern "C" int mv2_batch_envelope(mv2_context *ctx, const float *b, const float *delta, float *z, int count,
+                                  int ny, int nx, const double crit_diff[4], int max_launches, int *launches_out)
+{
+    if (count < 1 || (int64_t)count * ny >= ((int64_t)1 << 31))
+        return set_error(ctx, MV2_EINVAL, "[WarmStart] invalid batch of envelopes");
+    return envelope_run(ctx, b, delta, z, count * ny, nx, crit_diff, max_launches, launches_out, ny);
+}
+
+extern "C" int mv2_batch_commit_surface(mv2_context *ctx, float *b, const float *z, int count, int64_t n, double eps)
+{
+    if (int rc = check_plain(ctx)) return rc;
+    if (!b || !z || n < 1 || count < 1 || count > 65535)
+        return set_error(ctx, MV2_EINVAL, "[WarmStart] invalid commit arguments");
+    return project_change<true>(ctx, b, z, n, eps, count);
+}
+
+static int envelope_run(mv2_context *ctx, const float *b, const float *delta, float *z, int ny, int nx,
+                        const double crit_diff[4], int max_launches, int *launches_out, int seam)
 {
     if (int rc = check_plain(ctx)) return rc;
     if (!z || !crit_diff || ny < 1 || nx < 1 || max_launches < 1 || (delta && !b))
@@ -874,7 +969,8 @@ extern "C" int mv2_envelope(mv2_context *ctx, const float *b, const float *delta
         if (chunk > max_launches - k) chunk = max_launches - k;
         for (int j = 0; j < chunk; j++, k++)
             hipLaunchKernelGGL(envelope_kernel, dim3(tiles), dim3(kBlock), 0, st, z, ny, nx, (float)crit_diff[0],
-                               (float)crit_diff[1], (float)crit_diff[2], (float)crit_diff[3], tiles_x, ctl, spread, k);
+                               (float)crit_diff[1], (float)crit_diff[2], (float)crit_diff[3], tiles_x, ctl, spread, k,
+                               seam);
         hipLaunchKernelGGL(envelope_result_kernel, dim3(1), dim3(kWave), 0, st, ctl, spread, k, dres);
         MV2_HIP(ctx, hipGetLastError());
         MV2_HIP(ctx, hipMemcpyAsync(ctx->host_result.ptr, dres, 2 * sizeof(int), hipMemcpyDeviceToHost, st));

@@ -184,23 +184,59 @@ class BatchedMobileV2:
             res[i] = (r.iterations, bool(r.converged), r.max_excess_last)
         return res
 
-    def entrain(self, instance, cell, depth_m, *, z_base=None) -> np.ndarray:
+    def warm_start_relax(self, net, critical_angle_deg: float, tolerance: float, *, max_iterations: int = 20000,
+                         eps: float = 1e-5, envelope_angle_deg: float | None = None) -> dict:
+        """Learned warm start of the relaxation of b_eff for a batch of
+        equal-size terrains: the network on all of them at once, the Lipschitz
+        envelope of every predicted surface, the mass projection per terrain,
+        then the exact relaxation of every terrain to `tolerance` (each to its
+        own convergence, as ``relax``)."""
+        from .relaxation import legacy_critical_differences
+
+        torch = _torch()
+        if len(set(zip(self.ny.tolist(), self.nx.tolist()))) != 1:
+            raise ValueError("[BatchedWarmStart] the terrains of the batch must have equal sizes")
+        rt = self.rt
+        rt.bind_stream()
+        B, H, W = self.count, int(self.ny[0]), int(self.nx[0])
+        b3 = self.b.view(B, H, W)
+        h3 = (self.h.to(torch.float32) * MOBILE_QUANTUM_M).view(B, H, W)
+        pred = net.forward(b3, h3, self.dx, self.dy)
+        crit = legacy_critical_differences(self.dx, self.dy,
+                                           critical_angle_deg if envelope_angle_deg is None else envelope_angle_deg)
+        launches = (ctypes.c_int * 1)()
+        rt.check(rt.lib.mv2_batch_envelope(rt.ctx, self.b.data_ptr(), pred.data_ptr(), pred.data_ptr(), B, H, W,
+                                           crit.ctypes.data, 256, launches))
+        rt.check(rt.lib.mv2_batch_commit_surface(rt.ctx, self.b.data_ptr(), pred.data_ptr(), B, H * W, eps))
+        res = self.relax(critical_angle_deg, tolerance, max_iterations=max_iterations)
+        return {"relax": res, "envelope_launches": int(launches[0])}
+
+    def entrain(self, instance, cell, depth_m, *, z_base=None, check: bool = True) -> np.ndarray:
         """The entrain contract (bulk_state_authority.py:104-127) at (instance,
         instance-local flat cell) pairs; returns the volume moved per instance
-        (m^3, whole quanta, uniform cell area).  Indices must be unique."""
+        (m^3, whole quanta, uniform cell area).  Indices must be unique.
+        Inputs may be host arrays or device tensors; `check=False` skips the
+        index range checks (which read the indices back) for inputs already
+        validated."""
         torch = _torch()
         rt = self.rt
         rt.bind_stream()
         dev = rt.device
-        inst = torch.as_tensor(np.asarray(instance, np.int32), device=dev)
-        cells = torch.as_tensor(np.asarray(cell, np.int32), device=dev)
-        depth = torch.as_tensor(np.asarray(depth_m, np.float64), device=dev)
+
+        def dev_tensor(x, dtype):
+            if isinstance(x, torch.Tensor):
+                return x.to(device=dev, dtype=dtype).contiguous()
+            return torch.as_tensor(np.asarray(x), dtype=dtype, device=dev)
+
+        inst = dev_tensor(instance, torch.int32)
+        cells = dev_tensor(cell, torch.int32)
+        depth = dev_tensor(depth_m, torch.float64)
         n = int(inst.numel())
         if cells.numel() != n or depth.numel() != n:
             raise ValueError("[BatchedDeviceBulkState] instance, cell and depth must have the same length")
-        if n and (int(inst.min()) < 0 or int(inst.max()) >= self.count):
+        if check and n and (int(inst.min()) < 0 or int(inst.max()) >= self.count):
             raise ValueError("[BatchedDeviceBulkState] instance index out of range")
-        if n:
+        if check and n:
             sizes = torch.as_tensor((self.ny.astype(np.int64) * self.nx.astype(np.int64)), device=dev)
             if bool((cells < 0).any()) or bool((cells.to(torch.int64) >= sizes[inst.to(torch.int64)]).any()):
                 raise ValueError("[BatchedDeviceBulkState] cell index out of range")

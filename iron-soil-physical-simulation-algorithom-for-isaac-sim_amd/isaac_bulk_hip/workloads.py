@@ -98,3 +98,41 @@ def batched_piles(count: int = 256, dx: float = 0.05, seed: int = 0, sizes=BATCH
         h = mobile_patch(n, n, rng, size=min(32, n))
         out.append((b, h))
     return out
+
+
+# bucket footprint of the digging workload: 1.5 m along the sweep x 0.8 m across
+BUCKET_ALONG_M, BUCKET_ACROSS_M = 1.5, 0.8
+
+
+def digging_sites(count: int = 64, n: int = 256, dx: float = 0.05, seed: int = 0, sweep_m: float = 3.6):
+    """Digging workload: `count` terrains of n^2 (Gaussian piles on a 0.2 m
+    floor, no mobile layer) and per terrain a seeded straight bucket line.
+    Returns (terrains, lines); a line is (start_y_m, start_x_m, heading_rad),
+    chosen so that the bucket stays inside the terrain over `sweep_m`."""
+    rng = np.random.default_rng(seed)
+    terrains, lines = [], []
+    size = n * dx
+    margin = sweep_m + 0.5 * BUCKET_ALONG_M + dx
+    for _ in range(count):
+        b = 0.2 + gaussian_pile(n, n, dx, rng)
+        terrains.append((b, np.zeros((n, n))))
+        heading = float(rng.uniform(0.0, 2.0 * np.pi))
+        y0 = float(rng.uniform(margin * abs(np.sin(heading)) + 1.0, size - margin * abs(np.sin(heading)) - 1.0))
+        x0 = float(rng.uniform(margin * abs(np.cos(heading)) + 1.0, size - margin * abs(np.cos(heading)) - 1.0))
+        lines.append((y0, x0, heading))
+    return terrains, lines
+
+
+def bucket_cells(line, travelled_m: float, n: int, dx: float = 0.05) -> np.ndarray:
+    """Flat indices of the cells under the bucket after `travelled_m` along
+    `line`: an axis-aligned 1.5 m x 0.8 m rectangle (the long side on the axis
+    nearer the heading) centred on the bucket position, clipped to the terrain."""
+    y0, x0, heading = line
+    cy, cx = y0 + travelled_m * np.sin(heading), x0 + travelled_m * np.cos(heading)
+    along, across = int(round(BUCKET_ALONG_M / dx)), int(round(BUCKET_ACROSS_M / dx))
+    rows, cols = (across, along) if abs(np.cos(heading)) >= abs(np.sin(heading)) else (along, across)
+    r0 = int(np.floor(cy / dx)) - rows // 2
+    c0 = int(np.floor(cx / dx)) - cols // 2
+    rr = np.clip(np.arange(r0, r0 + rows), 0, n - 1)
+    cc = np.clip(np.arange(c0, c0 + cols), 0, n - 1)
+    return np.unique((rr[:, None] * n + cc[None, :]).ravel()).astype(np.int32)

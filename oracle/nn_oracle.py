@@ -191,12 +191,21 @@ def envelope(z: np.ndarray, crit, *, max_sweeps: int = 100000) -> np.ndarray:
     raise RuntimeError("envelope did not converge")
 
 
-def commit_surface(b: np.ndarray, z: np.ndarray, eps: float) -> np.ndarray:
-    """b + (z - b) with the change shifted to zero mean over |z - b| > eps (float32), heights >= 0."""
+def commit_surface(b: np.ndarray, z: np.ndarray, eps: float, corrections: int = 4) -> np.ndarray:
+    """b + (z - b) over the moved cells |z - b| > eps, minus a shift s, heights
+    clamped at 0 (float32).  s starts as the mean change over the moved cells;
+    each correction adds (mass the clamp creates) / (moved cells above zero),
+    so the moved cells keep their total when some are held at zero."""
     b = np.asarray(b, np.float32)
     d = np.asarray(z, np.float32) - b
     m = np.abs(d) > np.float32(eps)
     shift = np.float32(d[m].astype(np.float64).sum() / m.sum()) if m.any() else np.float32(0.0)
+    for _ in range(corrections):
+        v = np.maximum(b[m] + (d[m] - shift), np.float32(0.0))
+        gain = float((v.astype(np.float64) - b[m].astype(np.float64)).sum())   # mass gained at this shift
+        above = int((v > 0).sum())
+        if above > 0:
+            shift = np.float32(np.float64(shift) + gain / above)
     out = b.copy()
     out[m] = np.maximum(b[m] + (d[m] - shift), np.float32(0.0))
     return out

@@ -150,3 +150,25 @@ def test_commit_surface_against_oracle():
     rt.bind_stream()
     rt.check(rt.lib.mv2_nn_commit_surface(rt.ctx, bt.data_ptr(), zt.data_ptr(), b.size, 1e-5))
     np.testing.assert_allclose(bt.cpu().numpy(), nn_oracle.commit_surface(b, z, 1e-5), rtol=0, atol=2e-7)
+
+
+def test_commit_surface_with_floor_clamp_against_oracle():
+    """Cells pushed below zero: the GPU's shift corrections match the oracle's
+    and the moved cells keep their total."""
+    import torch
+    from isaac_bulk_hip.relaxation import _runtime
+
+    rt = _runtime("cuda:0")
+    rng = np.random.default_rng(23)
+    b = rng.uniform(0.0, 0.3, size=(64, 96)).astype(np.float32)
+    z = b - (rng.uniform(0.0, 0.4, size=b.shape) * (rng.random(b.shape) < 0.5)).astype(np.float32)
+    z[:8] += 0.5
+    bt, zt = _t(b, torch.float32), _t(z, torch.float32)
+    rt.bind_stream()
+    rt.check(rt.lib.mv2_nn_commit_surface(rt.ctx, bt.data_ptr(), zt.data_ptr(), b.size, 1e-5))
+    got = bt.cpu().numpy()
+    np.testing.assert_allclose(got, nn_oracle.commit_surface(b, z, 1e-5), rtol=0, atol=2e-7)
+    moved = np.abs(z - b) > 1e-5
+    assert got.min() >= 0.0 and (got == 0.0).any()
+    total = float(b[moved].astype(np.float64).sum())
+    assert abs(float(got[moved].astype(np.float64).sum()) - total) <= 1e-6 * total

@@ -141,3 +141,21 @@ def test_commit_surface_conserves_the_moved_mass():
     out = nn_oracle.commit_surface(b, z, 1e-5)
     assert abs(float(out.astype(np.float64).sum() - b.astype(np.float64).sum())) < 1e-5
     assert np.array_equal(out[0:4, :], b[0:4, :])
+
+
+def test_commit_surface_keeps_mass_when_the_floor_clamps():
+    """Cells pushed below zero are held at zero and the shift makes up for it:
+    the moved cells keep their total (to float rounding)."""
+    rng = np.random.default_rng(5)
+    b = rng.uniform(0.0, 0.3, size=(32, 32)).astype(np.float32)
+    z = b - rng.uniform(0.0, 0.4, size=b.shape).astype(np.float32) * (rng.random(b.shape) < 0.5)
+    z[:4] += 0.5                                  # some mass moves up elsewhere
+    out = nn_oracle.commit_surface(b, z, 1e-5)
+    assert out.min() >= 0.0 and (out == 0.0).any()
+    moved = np.abs(z - b) > 1e-5
+    total_in = float(b[moved].astype(np.float64).sum())
+    total_out = float(out[moved].astype(np.float64).sum())
+    assert abs(total_out - total_in) <= 1e-6 * total_in
+    # the plain mean shift alone would have created mass
+    plain = nn_oracle.commit_surface(b, z, 1e-5, corrections=0)
+    assert float(plain[moved].astype(np.float64).sum()) - total_in > 1e-3
