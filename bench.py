@@ -136,26 +136,36 @@ def learned_relax_to_rest(state, plain_rest) -> dict:
 
 
 def cpu_baseline(b, h, frames: int) -> dict:
-    """FP64 CPU oracle (C, one thread) on the first frames of the same workload."""
+    """FP64 CPU oracle (the C restatement) on the first frames of the same
+    workload, on every host core the process may use (OpenMP build; fields
+    identical to the one-thread build) and on one core."""
     sys.path.insert(0, str(ROOT / "oracle"))
     import oracle
 
-    qx = np.zeros_like(h)
-    qy = np.zeros_like(h)
-    cur_h = h.copy()
-    substeps = 0
-    t0 = time.perf_counter()
-    for _ in range(frames):
-        cur_h, qx, qy, rec = oracle.v2_step(b, cur_h, qx, qy, FRAME_DT, limit_mode=1)
-        substeps += rec["substeps"]
-        if time.perf_counter() - t0 > 30.0:
-            frames = _ + 1
-            break
-    elapsed = time.perf_counter() - t0
-    return {"value": N * N * substeps / elapsed, "unit": "cell-substeps/s", "cores": 1,
-            "kind": "port",
-            "sample": f"oracle FP64 Mobile V2 step, first {frames} frames (1/60 s each) of the "
-                      f"701x701 cut-face terrain: {substeps} substeps in {elapsed:.2f} s"}
+    def run(threads: bool, budget_s: float):
+        qx = np.zeros_like(h)
+        qy = np.zeros_like(h)
+        cur_h = h.copy()
+        substeps, done = 0, 0
+        t0 = time.perf_counter()
+        for _ in range(frames):
+            cur_h, qx, qy, rec = oracle.v2_step(b, cur_h, qx, qy, FRAME_DT, limit_mode=1, threads=threads)
+            substeps += rec["substeps"]
+            done += 1
+            if time.perf_counter() - t0 > budget_s:
+                break
+        elapsed = time.perf_counter() - t0
+        return N * N * substeps / elapsed, substeps, done, elapsed
+
+    cores = int(os.environ.get("OMP_NUM_THREADS", "0")) or len(os.sched_getaffinity(0))
+    os.environ["OMP_NUM_THREADS"] = str(cores)
+    v_all, s_all, f_all, e_all = run(True, 15.0)
+    v_one, s_one, f_one, e_one = run(False, 15.0)
+    return {"value": v_all, "unit": "cell-substeps/s", "cores": cores, "kind": "port",
+            "sample": f"oracle FP64 Mobile V2 step (C restatement, OpenMP over cells), first {f_all} frames "
+                      f"(1/60 s each) of the 701x701 cut-face terrain: {s_all} substeps in {e_all:.2f} s",
+            "one_core": {"value": v_one, "cores": 1,
+                         "sample": f"same, one thread: {f_one} frames, {s_one} substeps in {e_one:.2f} s"}}
 
 
 def run_batched(args) -> int:

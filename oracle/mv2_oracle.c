@@ -185,6 +185,9 @@ int orc_v2_step(int ny, int nx, const orc_v2_params *p,
     double *fix = (double *)malloc(sizeof(double) * n);
     double *fiy = (double *)malloc(sizeof(double) * n);
     double *fdis = (double *)malloc(sizeof(double) * n);
+#ifdef _OPENMP
+    double *diag_terms = (double *)malloc(sizeof(double) * 8 * n);
+#endif
     const double mind = dx <= dy ? dx : dy;
     const double Kg = K * g;
     const double mug = p->basal_friction * g;
@@ -196,6 +199,8 @@ int orc_v2_step(int ny, int nx, const orc_v2_params *p,
     while (remaining > 1.0e-14) {
         if (p->limit_mode == 0 && substeps >= p->maximum_substeps) { status = 2; break; }
         double maxw = 0.0;
+        /* OMP: the maximum does not depend on the order of the cells */
+#pragma omp parallel for reduction(max : maxw)
         for (int64_t i = 0; i < n; i++) {
             double vx = h[i] > dry ? qx[i] / h[i] : 0.0;
             double vy = h[i] > dry ? qy[i] / h[i] : 0.0;
@@ -210,19 +215,22 @@ int orc_v2_step(int ny, int nx, const orc_v2_params *p,
         double before_y = seq_sum(qy, n);
         /* faces: x then y (mobile_v2_reference.py:261-262) */
         const double fxf = sub_dt / dx, fyf = sub_dt / dy;
+#pragma omp parallel for
         for (int r = 0; r < ny; r++)
             for (int c = 0; c + 1 < nx; c++) {
                 int64_t L = (int64_t)r * nx + c, R = L + 1;
                 fx[(int64_t)r * (nx - 1) + c] = v2_face(b[L], b[R], h[L], h[R], qx[L], qy[L],
                                                        qx[R], qy[R], 0, K, g, dry, fxf);
             }
-        for (int r = 0; r + 1 < ny; r++)
+#pragma omp parallel for
+        for (int r = 0; r < ny - 1; r++)
             for (int c = 0; c < nx; c++) {
                 int64_t L = (int64_t)r * nx + c, R = L + nx;
                 fy[(int64_t)r * nx + c] = v2_face(b[L], b[R], h[L], h[R], qx[L], qy[L],
                                                  qx[R], qy[R], 1, K, g, dry, fyf);
             }
         double hmin = 0.0;
+#pragma omp parallel for reduction(min : hmin)
         for (int r = 0; r < ny; r++)
             for (int c = 0; c < nx; c++) {
                 int64_t i = (int64_t)r * nx + c;
@@ -250,6 +258,34 @@ int orc_v2_step(int ny, int nx, const orc_v2_params *p,
                 hmin = npmin(hmin, hn[i]);
             }
         if (hmin < -2.0e-13) { status = 1; break; }
+#ifdef _OPENMP
+        /* production-kernel style sums (mobile_v2_warp.py:193-231): the terms
+         * of every cell first (independent), then the sums in cell order */
+#pragma omp parallel for
+        for (int64_t i = 0; i < n; i++) {
+            double *t = diag_terms + (size_t)8 * i;
+            double depth = npmax(hn[i], 0.0);
+            t[0] = qxn[i] - qx[i];
+            t[1] = qyn[i] - qy[i];
+            for (int k = 2; k < 8; k++) t[k] = 0.0;
+            if (depth <= dry) continue;
+            double vx0 = qxn[i] / depth, vy0 = qyn[i] / depth;
+            double vx1 = vx0 + ax * sub_dt, vy1 = vy0 + ay * sub_dt;
+            t[2] = depth * (vx1 - vx0);
+            t[3] = depth * (vy1 - vy0);
+            t[6] = 0.5 * depth * (vx1 * vx1 + vy1 * vy1 - vx0 * vx0 - vy0 * vy0);
+            double speed = sqrt(vx1 * vx1 + vy1 * vy1);
+            double factor = 0.0;
+            if (speed > 1.0e-15) factor = npmax(0.0, 1.0 - mug * sub_dt / speed);
+            double vx2 = vx1 * factor, vy2 = vy1 * factor;
+            t[4] = depth * (vx2 - vx1);
+            t[5] = depth * (vy2 - vy1);
+            t[7] = 0.5 * depth * (vx1 * vx1 + vy1 * vy1 - vx2 * vx2 - vy2 * vy2);
+        }
+        /* adding the +0 terms of dry cells leaves every sum unchanged */
+        for (int64_t i = 0; i < n; i++)
+            for (int k = 0; k < 8; k++) out->diag[k] += diag_terms[(size_t)8 * i + k];
+#else
         /* production-kernel style sums (mobile_v2_warp.py:193-231) */
         for (int64_t i = 0; i < n; i++) {
             double depth = npmax(hn[i], 0.0);
@@ -269,6 +305,8 @@ int orc_v2_step(int ny, int nx, const orc_v2_params *p,
             out->diag[5] += depth * (vy2 - vy1);
             out->diag[7] += 0.5 * depth * (vx1 * vx1 + vy1 * vy1 - vx2 * vx2 - vy2 * vy2);
         }
+#endif
+#pragma omp parallel for
         for (int64_t i = 0; i < n; i++) {
             hn[i] = npmax(hn[i], 0.0);
             if (hn[i] <= dry) { qxn[i] = 0.0; qyn[i] = 0.0; }
@@ -278,6 +316,7 @@ int orc_v2_step(int ny, int nx, const orc_v2_params *p,
         out->conservative_impulse[0] += (after_x - before_x) * dx * dy;
         out->conservative_impulse[1] += (after_y - before_y) * dx * dy;
         /* external source (midpoint work) and Coulomb friction (:274-304) */
+#pragma omp parallel for
         for (int64_t i = 0; i < n; i++) {
             int wet = hn[i] > dry;
             double vbx = wet ? qxn[i] / hn[i] : 0.0, vby = wet ? qyn[i] / hn[i] : 0.0;
@@ -311,6 +350,9 @@ int orc_v2_step(int ny, int nx, const orc_v2_params *p,
     }
     free(fx); free(fy); free(hn); free(qxn); free(qyn);
     free(ext_work); free(eix); free(eiy); free(fix); free(fiy); free(fdis);
+#ifdef _OPENMP
+    free(diag_terms);
+#endif
     out->status = status;
     out->substeps = substeps;
     out->maximum_cfl = max_cfl;

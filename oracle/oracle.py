@@ -98,6 +98,24 @@ def lib() -> ctypes.CDLL:
     return _lib
 
 
+_lib_omp: ctypes.CDLL | None = None
+
+
+def lib_threads() -> ctypes.CDLL:
+    """liboracle_omp.so: the same restatement with its per-cell loops over
+    OpenMP threads (OMP_NUM_THREADS); fields identical to lib()'s."""
+    global _lib_omp
+    if _lib_omp is None:
+        path = _HERE / "liboracle_omp.so"
+        if not path.exists():
+            build()
+        _lib_omp = ctypes.CDLL(str(path))
+        _lib_omp.orc_v2_step.argtypes = [_I, _I, ctypes.POINTER(V2Params), _P, _P, _P, _P, _D,
+                                         ctypes.POINTER(V2Result)]
+        _lib_omp.orc_v2_step.restype = _I
+    return _lib_omp
+
+
 def _ptr(a: np.ndarray) -> int:
     return a.ctypes.data
 
@@ -117,8 +135,9 @@ def pairwise_sum(a: np.ndarray) -> float:
 
 def v2_step(b, h, qx, qy, dt, *, dx=0.05, dy=0.05, gravity=9.81, K=0.45, cfl=0.18,
             dry=1.0e-10, mu=0.55, tan_start=None, start_angle_deg=34.0,
-            external=(0.0, 0.0), maximum_substeps=4096, limit_mode=0, static_exit=True):
-    """Restated ``MobileV2ReferenceSolver.step`` (FP64).  Returns new fields + record."""
+            external=(0.0, 0.0), maximum_substeps=4096, limit_mode=0, static_exit=True, threads=False):
+    """Restated ``MobileV2ReferenceSolver.step`` (FP64).  Returns new fields + record.
+    threads=True runs the OpenMP build of the same restatement."""
     b = _f64(b)
     h = _f64(h).copy()
     qx = _f64(qx).copy()
@@ -129,8 +148,8 @@ def v2_step(b, h, qx, qy, dt, *, dx=0.05, dy=0.05, gravity=9.81, K=0.45, cfl=0.1
     p = V2Params(dx, dy, gravity, K, cfl, dry, mu, tan_start, float(external[0]),
                  float(external[1]), int(maximum_substeps), int(limit_mode), int(bool(static_exit)))
     r = V2Result()
-    lib().orc_v2_step(ny, nx, ctypes.byref(p), _ptr(b), _ptr(h), _ptr(qx), _ptr(qy),
-                      float(dt), ctypes.byref(r))
+    (lib_threads() if threads else lib()).orc_v2_step(ny, nx, ctypes.byref(p), _ptr(b), _ptr(h), _ptr(qx),
+                                                      _ptr(qy), float(dt), ctypes.byref(r))
     rec = {
         "status": r.status, "substeps": r.substeps, "cfl_limited": bool(r.cfl_limited),
         "static_exit": bool(r.static_exit), "maximum_cfl": r.maximum_cfl,

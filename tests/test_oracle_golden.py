@@ -277,3 +277,20 @@ def test_disconnected_spikes_give_four_regions():
                                float(np.deg2rad(38.0)), float(np.deg2rad(30.0)))
     assert di[0] >= 4
     assert di[3] == int(out["unstable"].sum())
+
+
+def test_threaded_oracle_gives_the_same_fields():
+    """The OpenMP build of the restatement (CPU baseline on every core) gives
+    bit-identical fields and the same substep count."""
+    import os
+
+    from isaac_bulk_hip.workloads import pile_with_patch
+
+    b, h, qx, qy = pile_with_patch(96, seed=3)
+    os.environ.setdefault("OMP_NUM_THREADS", "4")
+    h1, qx1, qy1, r1 = oracle.v2_step(b, h, qx, qy, 0.5)
+    h2, qx2, qy2, r2 = oracle.v2_step(b, h, qx, qy, 0.5, threads=True)
+    assert r1["substeps"] == r2["substeps"] > 3
+    np.testing.assert_array_equal(h1, h2)
+    np.testing.assert_array_equal(qx1, qx2)
+    np.testing.assert_array_equal(qy1, qy2)
