@@ -54,7 +54,8 @@ def parse() -> argparse.Namespace:
     p.add_argument("--precision", choices=("fp32", "fp64"), default="fp32")
     p.add_argument("--seed", type=int, default=0)
     p.add_argument("--relax-max-iterations", type=int, default=1000)
-    p.add_argument("--cpu-sample-frames", type=int, default=6)
+    p.add_argument("--cpu-sample-frames", type=int, default=100000,
+                   help="CPU baseline: frames to run at most (each mode stops after ~12 s)")
     p.add_argument("--no-cpu-baseline", action="store_true")
     p.add_argument("--skip-relax-to-rest", action="store_true",
                    help="omit the relax-to-rest measurement (short profiling runs)")
@@ -159,8 +160,8 @@ def cpu_baseline(b, h, frames: int) -> dict:
 
     cores = int(os.environ.get("OMP_NUM_THREADS", "0")) or len(os.sched_getaffinity(0))
     os.environ["OMP_NUM_THREADS"] = str(cores)
-    v_all, s_all, f_all, e_all = run(True, 15.0)
-    v_one, s_one, f_one, e_one = run(False, 15.0)
+    v_all, s_all, f_all, e_all = run(True, 12.0)
+    v_one, s_one, f_one, e_one = run(False, 12.0)
     return {"value": v_all, "unit": "cell-substeps/s", "cores": cores, "kind": "port",
             "sample": f"oracle FP64 Mobile V2 step (C restatement, OpenMP over cells), first {f_all} frames "
                       f"(1/60 s each) of the 701x701 cut-face terrain: {s_all} substeps in {e_all:.2f} s",
