@@ -59,9 +59,10 @@ def parse() -> argparse.Namespace:
     p.add_argument("--no-cpu-baseline", action="store_true")
     p.add_argument("--skip-relax-to-rest", action="store_true",
                    help="omit the relax-to-rest measurement (short profiling runs)")
-    p.add_argument("--workload", choices=("frame", "batched", "digging"), default="frame",
+    p.add_argument("--workload", choices=("frame", "batched", "digging", "deposition"), default="frame",
                    help="frame: the 701x701 frame (headline); batched: 256 uneven terrains; "
-                        "digging: 64 terrains of 256^2 with a bucket sweep")
+                        "digging: 64 terrains of 256^2 with a bucket sweep; deposition: the "
+                        "Mobile -> Resting deposition step on the 701x701 frame state")
     p.add_argument("--dig-count", type=int, default=64)
     p.add_argument("--dig-relax", choices=("learned", "plain"), default="learned",
                    help="digging: learned warm start + exact relaxation, or the plain relaxation")
@@ -319,6 +320,104 @@ def run_digging(args) -> int:
     return 0
 
 
+def run_deposition(args) -> int:
+    """Mobile -> Resting deposition step (WarpDepositionOperator.step_resident)
+    on the 701x701 cut-face state after 30 Mobile V2 frames (developed depth
+    and momentum).  Every call starts from that snapshot, restored outside
+    the timed region; value = cells / device time of one call (HIP events on
+    the library's stream around the two kernels, the result copy and sync)."""
+    import torch
+    from isaac_bulk_hip import MaterialScenario, WarpDepositionOperator
+
+    state, solver, bridge, b0, h0 = build_state(args.precision, args.seed)
+    for _ in range(30):
+        solver.step_resident(FRAME_DT)
+    material = MaterialScenario("iron_ore_fines_uncalibrated", 1370.0, 29.8, 0.0, 0.5, 34.0, 30.0, 0.55)
+    rt = state.runtime
+    op = WarpDepositionOperator(runtime=rt)
+    op.bind_device_state(state)
+    names = ("b_eff", "mobile", "momentum_x", "momentum_y", "avalanche_m2r_cumulative")
+    snap = {k: rt.arrays[k].clone() for k in names}
+
+    def restore():
+        for k in names:
+            rt.arrays[k].copy_(snap[k])
+
+    for _ in range(args.warmup):
+        restore()
+        op.step_resident(material, FRAME_DT)
+    # cells that the call touches (constant: every call starts from the snapshot)
+    restore()
+    op.step_resident(material, FRAME_DT)
+    dep = rt.arrays["deposition_work"]
+    n_dep = int((dep > 0).sum().item())
+    m_snap, f = snap["mobile"], rt.arrays["material_mask"]
+    n_wet = int(((m_snap > 0) & (f == 0)).sum().item())
+    stream = torch.cuda.current_stream()
+    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+          for _ in range(args.steps)]
+    deposited = 0.0
+    for k in range(args.steps):
+        restore()
+        ev[k][0].record(stream)
+        step = op.step_resident(material, FRAME_DT)
+        ev[k][1].record(stream)
+        deposited = step.deposited_volume_m3
+    torch.cuda.synchronize()
+    ms = [a.elapsed_time(b) for a, b in ev]
+    avg_ms = float(np.mean(ms))
+    cells = N * N
+    # algorithmic bytes per call: pass 1 reads b, mobile, forcing and (wet,
+    # unforced cells) momentum, writes deposited and removed momentum; pass 2
+    # reads deposited and, where material settles, reads and writes b, mobile,
+    # momentum and the cumulative field
+    f_b = 4 if args.precision == "fp32" else 8
+    per_cell = 2 * f_b + 4 + 3 * f_b + f_b
+    per_wet = 2 * f_b
+    per_dep = 2 * (4 * f_b + 8)
+    algo = cells * per_cell + n_wet * per_wet + n_dep * per_dep
+    achieved = algo / (avg_ms * 1e-3) / 1e9
+    out = {
+        "metric": "Mobile -> Resting deposition cells/s",
+        "value": cells / (avg_ms * 1e-3), "unit": "cells/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": avg_ms, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32" if args.precision == "fp32" else "f64",
+        "data": "synthetic (701x701 cut-face terrain after 30 Mobile V2 frames)",
+        "config": {"workload": "701x701 deposition step, 1/60 s, cohesion 0, stop 30 deg",
+                   "cells": cells, "wet_unforced_cells": n_wet, "settling_cells": n_dep,
+                   "deposited_volume_m3": deposited, "precision": args.precision},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": "deposition_decide_kernel + deposition_apply_kernel (one call)",
+                     "algorithmic_bytes_per_call": algo, "avg_call_ms": avg_ms,
+                     "min_call_ms": float(np.min(ms))},
+    }
+    if not args.no_cpu_baseline and args.precision == "fp32":
+        sys.path.insert(0, str(ROOT / "oracle"))
+        import oracle
+        from isaac_bulk_hip.config import triangle_vertex_weights
+
+        host = {k: v.cpu().numpy().reshape(N, N) for k, v in snap.items()}
+        forcing = f.cpu().numpy().reshape(N, N)
+        w = triangle_vertex_weights(N, N, DX, DX)
+        calls, t0 = 0, time.perf_counter()
+        while time.perf_counter() - t0 < 10.0:
+            oracle.deposition(host["b_eff"], host["mobile"], host["momentum_x"], host["momentum_y"],
+                              host["avalanche_m2r_cumulative"], w, dx=DX, dy=DX, dt=FRAME_DT,
+                              density=1370.0, cohesion=0.0, tan_stop=float(np.tan(np.deg2rad(30.0))),
+                              forcing=forcing, fp32=True)
+            calls += 1
+        elapsed = time.perf_counter() - t0
+        out["cpu_baseline"] = {"value": cells * calls / elapsed, "unit": "cells/s", "cores": 1,
+                               "kind": "port",
+                               "sample": f"oracle FP32-storage deposition step (C restatement, one "
+                                         f"thread) on the same snapshot: {calls} calls in {elapsed:.2f} s"}
+    print(json.dumps(out), flush=True)
+    torch.cuda.synchronize()
+    rt.close()
+    return 0
+
+
 def main() -> int:
     args = parse()
     if args.gpus != 1:
@@ -342,6 +441,8 @@ def main() -> int:
         return run_batched(args)
     if args.workload == "digging":
         return run_digging(args)
+    if args.workload == "deposition":
+        return run_deposition(args)
     state, solver, bridge, b0, h0 = build_state(args.precision, args.seed)
 
     # relax-to-rest of the initial terrain (plain relaxation, same tolerance as the frames)

@@ -124,6 +124,36 @@ int mv2_deposit(mv2_context *ctx, const int32_t *idx, const double *depth, int n
  * out = resting volume, mobile volume, density*px, density*py */
 int mv2_reservoirs(mv2_context *ctx, int weight_mode, double density, double out[4]);
 
+/* ---- Mobile -> Resting deposition: WarpDepositionOperator.step_resident
+ *      (pkg/bulk_interaction/warp_deposition.py:180-235) with its kernels
+ *      deposit (:23-115) and weighted_sum (:117-129).  A cell with mobile
+ *      material, forcing[i] == 0 (material_mask; forcing may be NULL), speed
+ *      at or below speed_threshold and drive <= cohesion + stop resistance on
+ *      the free surface b_eff + mobile moves min(h, settling_rate dt) into
+ *      b_eff (all of h when settle_subcell_tail != 0) and the same fraction of
+ *      its momentum out.  Every cell is judged on the pre-call fields.
+ *      Per-cell outputs (DEVICE arrays of ny*nx): deposited = deposition_work
+ *      (FP64 double m, FP32 int32 quanta), removed_x/y =
+ *      deposition_removed_momentum_x/y (momentum precision);
+ *      m2r_cumulative (double, avalanche_m2r_cumulative) += deposited * w.
+ *      out = sum(deposited w), density sum(removed_x w), density sum(removed_y w).
+ *      FP32 storage deposits whole quanta, min(h, floor(rate dt / quantum)). ---- */
+typedef struct mv2_deposition_config {
+    double dt;                        /* dt_s (> 0) */
+    double settling_rate;             /* settling_rate_m_s (> 0) */
+    double speed_threshold;           /* speed_threshold_m_s (>= 0) */
+    double density;                   /* assumed_bulk_density_kg_m3 */
+    double gravity;                   /* 9.81 in the reference */
+    double cohesion;                  /* cohesion_proxy_pa */
+    double tan_stop;                  /* tan(stop_angle_deg) */
+    int settle_subcell_tail;
+    int weight_mode;                  /* 0: uniform dx*dy, 1: Triangle-A-C vertex weights */
+} mv2_deposition_config;
+
+int mv2_deposition_step(mv2_context *ctx, const mv2_deposition_config *cfg, const int32_t *forcing,
+                        void *deposited, void *removed_x, void *removed_y, double *m2r_cumulative,
+                        double out[3]);
+
 /* ---- batched Mobile V2 (no single counterpart in the reference: the
  *      reference steps one terrain per solver, warp_mobile_v2.py:148-245;
  *      this advances many independent terrains, each with its own CFL loop,

@@ -23,6 +23,7 @@ from .config import (
     TerrainGrid,
     TerrainVolumeIntegrator,
 )
+from .deposition import WarpDepositionOperator, WarpDepositionStep
 from .large_avalanche import DeviceLargeAvalancheBridge, DeviceLargeAvalancheResult, label_mask
 from .mobile_v2 import (
     HipMobileV2ReferenceSolver,

@@ -78,6 +78,14 @@ class LabelResult(ctypes.Structure):
     _fields_ = [("diag_i", _I32 * 8), ("diag_f", _D * 8), ("union_passes", _I), ("reserved", _I)]
 
 
+class DepositionConfig(ctypes.Structure):
+    _fields_ = [
+        ("dt", _D), ("settling_rate", _D), ("speed_threshold", _D), ("density", _D),
+        ("gravity", _D), ("cohesion", _D), ("tan_stop", _D),
+        ("settle_subcell_tail", _I), ("weight_mode", _I),
+    ]
+
+
 class BatchDesc(ctypes.Structure):
     _fields_ = [
         ("ny", _I), ("nx", _I), ("offset", ctypes.c_int64),
@@ -109,6 +117,7 @@ SIGNATURES: dict[str, tuple[object, list[object]]] = {
     "mv2_entrain": (_I, [_P, _P, _P, _I, _I, ctypes.POINTER(_D)]),
     "mv2_deposit": (_I, [_P, _P, _P, _I, _I, _D, _P]),
     "mv2_reservoirs": (_I, [_P, _I, _D, _P]),
+    "mv2_deposition_step": (_I, [_P, ctypes.POINTER(DepositionConfig), _P, _P, _P, _P, _P, _P]),
     "mv2_relax": (_I, [_P, _I, _P, _I, _I, _P, _I, _D, ctypes.POINTER(RelaxResult)]),
     "mv2_frontier_relax": (_I, [_P, _P, _P, _P, _P, _P, _P, _P, _I, _I, _I, _I, _I, _I, _D, _P,
                                 ctypes.POINTER(_I32)]),

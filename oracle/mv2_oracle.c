@@ -482,6 +482,148 @@ void orc_reservoirs(int n_cells, const double *z_base, const double *b_eff,
 }
 
 /* ------------------------------------------------------------------------ */
+/* Mobile -> Resting deposition: warp_deposition.py:23-115 (deposit) and     */
+/* :117-129 (weighted_sum), one step_resident call (:180-235).  Every cell   */
+/* is judged on the field as it was before the call (the CPU operator        */
+/* deposition.py:102-153 also works on a copy); the neighbour free surface   */
+/* comes from copies of b_eff and mobile taken before any cell is changed.   */
+/* ------------------------------------------------------------------------ */
+
+/* warp_deposition.py:69-102: 1 when the slow cell is within the cohesive
+ * stop yield of its free surface; bl/ml are the pre-call b_eff and mobile */
+static int deposition_settles(int ny, int nx, int r, int c, const double *bl, const double *ml,
+                              double h, double qx, double qy, const orc_deposition_params *p)
+{
+    double vx = qx / h, vy = qy / h;
+    if (sqrt(vx * vx + vy * vy) > p->speed_threshold) return 0;
+    int i = r * nx + c;
+    int left = c > 0 ? i - 1 : i, right = c + 1 < nx ? i + 1 : i;
+    int down = r > 0 ? i - nx : i, up = r + 1 < ny ? i + nx : i;
+    double hx = (c == 0 || c + 1 == nx) ? p->dx : 2.0 * p->dx;
+    double hy = (r == 0 || r + 1 == ny) ? p->dy : 2.0 * p->dy;
+    double sx = (bl[right] + ml[right] - bl[left] - ml[left]) / hx;
+    double sy = (bl[up] + ml[up] - bl[down] - ml[down]) / hy;
+    double slope_tan = sqrt(sx * sx + sy * sy);
+    double normalization = sqrt(1.0 + slope_tan * slope_tan);
+    double drive = p->density * p->gravity * h * slope_tan / normalization;
+    double resistance = p->cohesion + p->density * p->gravity * h * p->tan_stop / normalization;
+    return !(drive > resistance);
+}
+
+void orc_deposition(int ny, int nx, const orc_deposition_params *p, const int32_t *forcing,
+                    double *b_eff, double *mobile, double *qx, double *qy,
+                    double *deposited, double *removed_x, double *removed_y,
+                    double *m2r_cumulative, const double *weights, double out[3])
+{
+    const size_t n = (size_t)ny * nx;
+    double *b0 = malloc(n * sizeof(double)), *m0 = malloc(n * sizeof(double));
+    memcpy(b0, b_eff, n * sizeof(double));
+    memcpy(m0, mobile, n * sizeof(double));
+    for (int r = 0; r < ny; r++)
+        for (int c = 0; c < nx; c++) {
+            size_t i = (size_t)r * nx + c;
+            double h = m0[i];
+            deposited[i] = 0.0;
+            removed_x[i] = 0.0;
+            removed_y[i] = 0.0;
+            if (h <= 0.0 || (forcing && forcing[i] != 0)) continue;
+            if (p->settle_subcell_tail) {
+                removed_x[i] = qx[i];
+                removed_y[i] = qy[i];
+                b_eff[i] = b_eff[i] + h;
+                mobile[i] = 0.0;
+                qx[i] = 0.0;
+                qy[i] = 0.0;
+                deposited[i] = h;
+                m2r_cumulative[i] = m2r_cumulative[i] + h * weights[i];
+                continue;
+            }
+            if (!deposition_settles(ny, nx, r, c, b0, m0, h, qx[i], qy[i], p)) continue;
+            double amount = npmin(h, p->settling_rate * p->dt);
+            double fraction_removed = amount / h;
+            removed_x[i] = qx[i] * fraction_removed;
+            removed_y[i] = qy[i] * fraction_removed;
+            b_eff[i] = b_eff[i] + amount;
+            mobile[i] = h - amount;
+            double fraction = (h - amount) / h;
+            qx[i] = qx[i] * fraction;
+            qy[i] = qy[i] * fraction;
+            deposited[i] = amount;
+            m2r_cumulative[i] = m2r_cumulative[i] + amount * weights[i];
+        }
+    out[0] = out[1] = out[2] = 0.0;
+    for (size_t i = 0; i < n; i++) {
+        out[0] += deposited[i] * weights[i];
+        out[1] += p->density * removed_x[i] * weights[i];
+        out[2] += p->density * removed_y[i] * weights[i];
+    }
+    free(b0);
+    free(m0);
+}
+
+/* The same step on FP32 storage.  The decision is evaluated in double on the
+ * widened values; the amount is whole quanta, min(h, floor(rate dt / q)),
+ * so b_eff + mobile moves exactly what the ledger reports. */
+void orc_deposition32(int ny, int nx, const orc_deposition_params *p, const int32_t *forcing,
+                      float *b_eff, int32_t *mobile, float *qx, float *qy,
+                      int32_t *deposited, float *removed_x, float *removed_y,
+                      double *m2r_cumulative, const double *weights, double out[3])
+{
+    const double Q = 1.0 / 16777216.0;
+    const size_t n = (size_t)ny * nx;
+    double *b0 = malloc(n * sizeof(double)), *m0 = malloc(n * sizeof(double));
+    for (size_t i = 0; i < n; i++) {
+        b0[i] = (double)b_eff[i];
+        m0[i] = (double)mobile[i] * Q;
+    }
+    double cap_d = floor(p->settling_rate * p->dt * 16777216.0);
+    int32_t cap = cap_d >= 2147483647.0 ? 2147483647 : (int32_t)cap_d;
+    for (int r = 0; r < ny; r++)
+        for (int c = 0; c < nx; c++) {
+            size_t i = (size_t)r * nx + c;
+            int32_t hq = mobile[i];
+            deposited[i] = 0;
+            removed_x[i] = 0.0f;
+            removed_y[i] = 0.0f;
+            if (hq <= 0 || (forcing && forcing[i] != 0)) continue;
+            double h = m0[i];
+            if (p->settle_subcell_tail) {
+                removed_x[i] = qx[i];
+                removed_y[i] = qy[i];
+                b_eff[i] = (float)((double)b_eff[i] + h);
+                mobile[i] = 0;
+                qx[i] = 0.0f;
+                qy[i] = 0.0f;
+                deposited[i] = hq;
+                m2r_cumulative[i] = m2r_cumulative[i] + h * weights[i];
+                continue;
+            }
+            if (!deposition_settles(ny, nx, r, c, b0, m0, h, (double)qx[i], (double)qy[i], p)) continue;
+            int32_t aq = hq < cap ? hq : cap;
+            if (aq == 0) continue;
+            double amount = (double)aq * Q;
+            double fraction_removed = (double)aq / (double)hq;
+            double fraction = (double)(hq - aq) / (double)hq;
+            removed_x[i] = (float)((double)qx[i] * fraction_removed);
+            removed_y[i] = (float)((double)qy[i] * fraction_removed);
+            b_eff[i] = (float)((double)b_eff[i] + amount);
+            mobile[i] = hq - aq;
+            qx[i] = (float)((double)qx[i] * fraction);
+            qy[i] = (float)((double)qy[i] * fraction);
+            deposited[i] = aq;
+            m2r_cumulative[i] = m2r_cumulative[i] + amount * weights[i];
+        }
+    out[0] = out[1] = out[2] = 0.0;
+    for (size_t i = 0; i < n; i++) {
+        out[0] += (double)deposited[i] * Q * weights[i];
+        out[1] += p->density * (double)removed_x[i] * weights[i];
+        out[2] += p->density * (double)removed_y[i] * weights[i];
+    }
+    free(b0);
+    free(m0);
+}
+
+/* ------------------------------------------------------------------------ */
 /* relax_critical_slope (slope_model.py:273-343) on a legacy-layout array    */
 /* A[n0][n1]; directions (1,0),(0,1),(1,1),(1,-1) of _neighbor_pairs         */
 /* (:744-750) and the views of _paired_views (:753-764).  The caller passes  */

@@ -83,6 +83,25 @@ void orc_reservoirs(int n_cells, const double *z_base, const double *b_eff,
                     const double *mobile, const double *qx, const double *qy,
                     const double *weights, double density, double out[4]);
 
+/* Mobile -> Resting deposition step (one step_resident call) */
+typedef struct orc_deposition_params {
+    double dx, dy, dt;
+    double settling_rate, speed_threshold;
+    double density, gravity, cohesion, tan_stop;
+    int settle_subcell_tail;
+} orc_deposition_params;
+
+/* FP64 storage; forcing may be NULL.  out = sum(dep w), sum(rho rx w), sum(rho ry w) */
+void orc_deposition(int ny, int nx, const orc_deposition_params *p, const int32_t *forcing,
+                    double *b_eff, double *mobile, double *qx, double *qy,
+                    double *deposited, double *removed_x, double *removed_y,
+                    double *m2r_cumulative, const double *weights, double out[3]);
+/* FP32 storage: b/momentum float, mobile and deposited in int32 quanta of 2^-24 m */
+void orc_deposition32(int ny, int nx, const orc_deposition_params *p, const int32_t *forcing,
+                      float *b_eff, int32_t *mobile, float *qx, float *qy,
+                      int32_t *deposited, float *removed_x, float *removed_y,
+                      double *m2r_cumulative, const double *weights, double out[3]);
+
 /* relax_critical_slope on an array in the legacy [axis0][axis1] layout */
 int orc_relax_legacy_f64(int n0, int n1, double d0, double d1, double crit_slope,
                          int max_iterations, double tolerance, double *h,

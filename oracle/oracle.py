@@ -33,6 +33,14 @@ class V2Params(ctypes.Structure):
     ]
 
 
+class DepositionParams(ctypes.Structure):
+    _fields_ = [
+        ("dx", _D), ("dy", _D), ("dt", _D), ("settling_rate", _D), ("speed_threshold", _D),
+        ("density", _D), ("gravity", _D), ("cohesion", _D), ("tan_stop", _D),
+        ("settle_subcell_tail", _I),
+    ]
+
+
 class V2Result(ctypes.Structure):
     _fields_ = [
         ("status", _I), ("substeps", _I), ("cfl_limited", _I), ("static_exit", _I),
@@ -78,6 +86,8 @@ def lib() -> ctypes.CDLL:
         _lib.orc_entrain.argtypes = [_I, _P, _P, _P, _P, _P, _P, _I]
         _lib.orc_deposit.argtypes = [_I, _P, _P, _P, _P, _P, _P, _P, _I, _D, _P]
         _lib.orc_reservoirs.argtypes = [_I, _P, _P, _P, _P, _P, _P, _D, _P]
+        for name in ("orc_deposition", "orc_deposition32"):
+            getattr(_lib, name).argtypes = [_I, _I, ctypes.POINTER(DepositionParams), _P] + [_P] * 10
         for name in ("orc_relax_legacy_f64", "orc_relax_legacy_f32"):
             fn = getattr(_lib, name)
             fn.restype = _I
@@ -212,6 +222,36 @@ def deposit(b, m, qx, qy, w, idx, req, density):
     lib().orc_deposit(b.size, _ptr(b), _ptr(m), _ptr(qx), _ptr(qy), _ptr(w), _ptr(idx),
                       _ptr(req), idx.size, density, _ptr(out))
     return b, m, qx, qy, out
+
+
+def deposition(b, m, qx, qy, cumulative, weights, *, dx, dy, dt, settling_rate=0.30,
+               speed_threshold=0.18, density=1800.0, gravity=9.81, cohesion=0.0, tan_stop=0.0,
+               settle_subcell_tail=False, forcing=None, fp32=False):
+    """One WarpDepositionOperator.step_resident call on copies of the fields.
+
+    fp32=False: float64 fields.  fp32=True: b/qx/qy float32, m int32 quanta.
+    Returns (b, m, qx, qy, deposited, removed_x, removed_y, cumulative, sums[3]).
+    """
+    ny, nx = np.shape(b)
+    if fp32:
+        fd, md = np.float32, np.int32
+    else:
+        fd, md = np.float64, np.float64
+    b, qx, qy = (np.ascontiguousarray(v, dtype=fd).copy() for v in (b, qx, qy))
+    m = np.ascontiguousarray(m, dtype=md).copy()
+    cum = _f64(cumulative).copy()
+    w = _f64(weights)
+    dep = np.zeros((ny, nx), dtype=md)
+    rx = np.zeros((ny, nx), dtype=fd)
+    ry = np.zeros((ny, nx), dtype=fd)
+    f = None if forcing is None else _i32(forcing)
+    p = DepositionParams(dx, dy, dt, settling_rate, speed_threshold, density, gravity, cohesion,
+                         tan_stop, int(bool(settle_subcell_tail)))
+    out = np.zeros(3)
+    fn = lib().orc_deposition32 if fp32 else lib().orc_deposition
+    fn(ny, nx, ctypes.byref(p), None if f is None else _ptr(f), _ptr(b), _ptr(m), _ptr(qx),
+       _ptr(qy), _ptr(dep), _ptr(rx), _ptr(ry), _ptr(cum), _ptr(w), _ptr(out))
+    return b, m, qx, qy, dep, rx, ry, cum, out
 
 
 def reservoirs(z, b, m, qx, qy, w, density):
