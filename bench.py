@@ -59,10 +59,12 @@ def parse() -> argparse.Namespace:
     p.add_argument("--no-cpu-baseline", action="store_true")
     p.add_argument("--skip-relax-to-rest", action="store_true",
                    help="omit the relax-to-rest measurement (short profiling runs)")
-    p.add_argument("--workload", choices=("frame", "batched", "digging", "deposition"), default="frame",
+    p.add_argument("--workload", choices=("frame", "batched", "digging", "deposition", "frontier"),
+                   default="frame",
                    help="frame: the 701x701 frame (headline); batched: 256 uneven terrains; "
                         "digging: 64 terrains of 256^2 with a bucket sweep; deposition: the "
-                        "Mobile -> Resting deposition step on the 701x701 frame state")
+                        "Mobile -> Resting deposition step on the 701x701 frame state; frontier: "
+                        "residual projection rounds after landings on the 701x701 terrain")
     p.add_argument("--dig-count", type=int, default=64)
     p.add_argument("--dig-relax", choices=("learned", "plain"), default="learned",
                    help="digging: learned warm start + exact relaxation, or the plain relaxation")
@@ -418,6 +420,105 @@ def run_deposition(args) -> int:
     return 0
 
 
+def run_frontier(args) -> int:
+    """Residual MiniSlope projection (the frame scheduler's compact-frontier
+    rounds, v2_physics_core.py:1172-1254) on the 701x701 cut-face terrain
+    relaxed to rest at 30 degrees, after 24 seeded landings of 0.3 m; tile 64,
+    FP64, vertex weights.  Timed: the device round loop (one call), and the
+    host-driven loop of the reference (per colour batch one relax call and
+    one scan call, each with a host read of the tile flags) on the same
+    input; both end on identical fields."""
+    import torch
+    from isaac_bulk_hip import (CompactTileFrontier, HipCompactActiveEdgeOperator, neighbor_pairs,
+                                relax_device)
+    from isaac_bulk_hip.config import triangle_vertex_weights
+    from isaac_bulk_hip.workloads import cut_face_terrain
+
+    ts, stop_deg = 64, 30.0
+    b, _, _ = cut_face_terrain(N, DX, seed=args.seed)
+    rest = torch.from_numpy(np.ascontiguousarray(b)).to("cuda:0")
+    relax_device(rest, DX, DX, stop_deg, 20000, 1e-8)
+    base = rest.cpu().numpy()
+    rng = np.random.default_rng(args.seed + 1)
+    cells = rng.choice(N * N, size=24, replace=False)
+    disturbed = base.copy().ravel()
+    disturbed[cells] += 0.3
+    disturbed = disturbed.reshape(N, N)
+    reached = np.zeros(N * N, dtype=np.int32)
+    reached[cells] = 1
+    reached = reached.reshape(N, N)
+    w = triangle_vertex_weights(N, N, DX, DX)
+    frontier = CompactTileFrontier((N, N), ts)
+    seeds = frontier.tiles_around_cells(*np.divmod(cells, N))
+    nb = neighbor_pairs(DX, DX)
+    tan = float(np.tan(np.deg2rad(stop_deg)))
+    op = HipCompactActiveEdgeOperator((N, N), tile_size=ts)
+
+    def device_run():
+        op.initialize(disturbed, reached, w)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        res = op.advance_rounds(seeds, neighbors=nb, critical_tangent=tan, tolerance_m=1e-8, max_rounds=4096)
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t0) * 1e3, res
+
+    def host_run():
+        op.initialize(disturbed, reached, w)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        active, rounds = seeds.astype(np.int32), 0
+        while active.size:
+            rounds += 1
+            phase = active
+            for d, (di, dj, dist) in enumerate(nb):
+                for p in (0, 1):
+                    r = op.relax_phase(frontier.edge_batch(phase, direction_index=d, di=di, dj=dj, phase=p),
+                                       critical_difference_m=tan * dist)
+                    if r.triggered_tile_ids.size:
+                        phase = np.union1d(phase, r.triggered_tile_ids).astype(np.int32)
+            owners = []
+            for d, (di, dj, dist) in enumerate(nb):
+                for p in (0, 1):
+                    r = op.scan_phase(frontier.edge_batch(phase, direction_index=d, di=di, dj=dj, phase=p),
+                                      critical_difference_m=tan * dist, tolerance_m=1e-8)
+                    if r.unstable_owner_tile_ids.size:
+                        owners.append(r.unstable_owner_tile_ids)
+            active = np.unique(np.concatenate(owners)).astype(np.int32) if owners else np.empty(0, np.int32)
+        op.runtime.arrays["reached"].zero_()
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t0) * 1e3, rounds
+
+    for _ in range(max(args.warmup, 1)):
+        device_run()
+        host_run()
+    dev_ms, host_ms = [], []
+    for _ in range(args.steps):
+        ms, res = device_run()
+        dev_ms.append(ms)
+    h_dev = op.download_state()[0]
+    for _ in range(max(1, min(args.steps, 5))):
+        ms, host_rounds = host_run()
+        host_ms.append(ms)
+    h_host = op.download_state()[0]
+    out = {
+        "metric": "residual projection rounds to rest (device loop vs host-driven loop)",
+        "value": float(np.median(dev_ms)), "unit": "ms", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": float(np.median(dev_ms)), "higher_is_better": False,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic (701x701 cut-face terrain at rest at 30 deg, 24 landings of 0.3 m)",
+        "config": {"workload": "701x701 residual projection, tile 64, 4 directions, tol 1e-8",
+                   "rounds": res.rounds, "moved_edges": res.moved_edge_count,
+                   "seed_tiles": int(seeds.size)},
+        "host_driven_loop_ms": float(np.median(host_ms)), "host_driven_rounds": host_rounds,
+        "fields_identical": bool(np.array_equal(h_dev, h_host)),
+        "speedup_vs_host_loop": float(np.median(host_ms) / np.median(dev_ms)),
+    }
+    print(json.dumps(out), flush=True)
+    torch.cuda.synchronize()
+    op.runtime.close()
+    return 0
+
+
 def main() -> int:
     args = parse()
     if args.gpus != 1:
@@ -443,6 +544,8 @@ def main() -> int:
         return run_digging(args)
     if args.workload == "deposition":
         return run_deposition(args)
+    if args.workload == "frontier":
+        return run_frontier(args)
     state, solver, bridge, b0, h0 = build_state(args.precision, args.seed)
 
     # relax-to-rest of the initial terrain (plain relaxation, same tolerance as the frames)

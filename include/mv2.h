@@ -229,6 +229,40 @@ int mv2_frontier_scan(mv2_context *ctx, const double *height, const int32_t *rea
                       int tile_count, double critical_difference, double tolerance,
                       int32_t *unstable_tile_flags);
 
+/* ---- residual projection rounds on the device: the loop of
+ *      V2PhysicsCore._advance_gpu_residual_frontier
+ *      (pkg/runtime/v2_physics_core.py:1172-1254) -- per round the relax
+ *      phases of every direction and parity over the active tiles (tiles a
+ *      transfer touches join for the later phases), then the scan of the
+ *      round's tiles; its unstable owner tiles are the next round's active
+ *      set.  Ends when the set is empty (reached is then zeroed, :1246-1249)
+ *      or after max_rounds rounds.  The edge batches are those of
+ *      CompactTileFrontier.edge_batch (pkg/solvers/sparse_tile_slope.py:116-230).
+ *      active_tile_flags (device, tile_count int32): in, the active tiles
+ *      (nonzero); out, the tiles still active.  One host sync per chunk of
+ *      rounds instead of two per colour batch. ---- */
+typedef struct mv2_frontier_round_config {
+    int rows, cols, tile_size, tiles_x, tile_count;
+    int n_directions;                 /* 1..4, e.g. slope_model._neighbor_pairs */
+    int di[4], dj[4];                 /* neighbour offsets (each in -1..1) */
+    double critical_difference[4];    /* critical tangent x neighbour distance */
+    double tolerance;                 /* scan tolerance (m) */
+    int max_rounds;                   /* residual_projection_max_rounds (>= 0) */
+    int reserved;
+} mv2_frontier_round_config;
+
+typedef struct mv2_frontier_round_result {
+    int rounds;                       /* rounds run */
+    int converged;                    /* 1: the active set is empty */
+    int active_tiles;                 /* tiles still active */
+    int cleared_reached;              /* 1: a round ended empty and reached was zeroed */
+    long long moved_edges;            /* transfers over all rounds */
+} mv2_frontier_round_result;
+
+int mv2_frontier_rounds(mv2_context *ctx, const mv2_frontier_round_config *cfg, double *height,
+                        const double *weights, int32_t *reached, int32_t *active_tile_flags,
+                        mv2_frontier_round_result *out);
+
 /* ---- LargeAvalanche failure mask + connected components
  *      DeviceLargeAvalancheBridge.observe_and_maybe_mobilize
  *      (pkg/runtime/gpu_large_avalanche.py:541-767) device part. ---- */
@@ -293,6 +327,20 @@ int mv2_batch_label(mv2_context *ctx, const float *z_base, const mv2_avalanche_c
 int mv2_avalanche_previous(mv2_context *ctx, const mv2_avalanche_fields *f, int32_t winner,
                            int keep_candidate, int seed_frontier, int32_t *frontier_reached,
                            int32_t *tile_flags, int tile_size, int tiles_x, int tile_count);
+/* reduce_persistence (gpu_large_avalanche.py:378-451) and its host summary
+ * (:769-839): lifecycle scalars of the resident avalanche history over
+ * every cell.  diag_i: 0 cells ever activated, 1 activated more than once,
+ * 2 wet cells (h > dry_tolerance), 3 retired candidates; diag_f: the
+ * reference's avalanche_persistence_float slots 0-13 (5 and 9 maxima).
+ * tile_flags (device, tile_count int32) are rewritten: 1 where a cell is
+ * unstable or carries mobile material faster than activity_speed;
+ * active_tile_count = their number.  initial_b_eff has the state's
+ * precision; m2r_cumulative is avalanche_m2r_cumulative (double). */
+int mv2_avalanche_persistence(mv2_context *ctx, const mv2_avalanche_config *cfg,
+                              const mv2_avalanche_fields *f, const void *initial_b_eff,
+                              const double *m2r_cumulative, int tile_size, int tiles_x,
+                              int tile_count, int32_t *tile_flags, int32_t diag_i[4],
+                              double diag_f[14], int32_t *active_tile_count);
 /* mobilize (gpu_large_avalanche.py:309-376); diag_f[16] as there */
 int mv2_avalanche_mobilize(mv2_context *ctx, const mv2_avalanche_config *cfg,
                            const mv2_avalanche_fields *f, int32_t winner,

@@ -55,6 +55,7 @@ struct mv2_context {
     mv2::Scratch nn_loss;                 // loss / projection partials
     mv2::Scratch step_diag;               // per-workgroup substep diagnostics
     mv2::Scratch spread;                  // spread-maximum slots of the step and the relaxation
+    mv2::Scratch frontier;                // round scheduler: control records, tile stamps, owner flags
     hipEvent_t ev_start = nullptr, ev_stop = nullptr;   // device timing of step calls
     int timing = 0;                       // bracket each substep launch with events
     std::vector<hipEvent_t> sub_events;   // event pairs of the substep launches
@@ -63,6 +64,7 @@ struct mv2_context {
     int v2_clean = -1;                    // parity whose loop state is known zeroed, -1 none
     int v2_last_sync = 0;                 // parity of the last completed step
     int relax_predicted = 0;              // relaxation launches to enqueue up front
+    int frontier_predicted = 0;           // frontier rounds to enqueue up front
     mv2_batch_state *batch = nullptr;     // batched terrains bound to this context
 };
 

@@ -435,6 +435,87 @@ __global__ void __launch_bounds__(kBlock) mobilize_final_kernel(const double *pa
         for (int k = 0; k < kMobF; k++) out[k] = s[k];
 }
 
+// reduce_persistence (:378-451): lifecycle scalars over every cell.  Per
+// thread: 14 float sums/maxima (the reference's avalanche_persistence_float
+// slots, maxima in slots 5 and 9 from zero) and 4 counts (exact in double);
+// folded per workgroup field-parallel, then across workgroups in order.
+constexpr int kPersF = 14, kPersV = 18;
+constexpr unsigned kPersMax = (1u << 5) | (1u << 9);
+__global__ void __launch_bounds__(kBlock) persistence_kernel(
+    const void *initial, const void *resting, const void *mobile, const void *qx_, const void *qy_, int f32,
+    int ny, int nx, double area, double f6, int weight_mode, mv2_avalanche_fields f, const double *m2r,
+    double start_rad, double stop_rad, double dry, double activity_speed, int tile_size, int tiles_x,
+    int32_t *tile_flags, double *partials)
+{
+    __shared__ double lds[kPersV * kBlock];
+    double v[kPersV];
+#pragma unroll
+    for (int k = 0; k < kPersV; k++) v[k] = 0.0;
+    const int64_t n = (int64_t)ny * nx;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        const int r = (int)i / nx, c = (int)i - r * nx;
+        const double w = cell_weight(weight_mode, ny, nx, area, f6, r, c);
+        const int32_t count = f.activation_count[i];
+        const double h = ld_mobile(mobile, f32, i);
+        double speed = 0.0;
+        if (h > dry) {
+            const double vx = ld_field(qx_, f32, i) / h, vy = ld_field(qy_, f32, i) / h;
+            speed = sqrt(vx * vx + vy * vy);
+            v[kPersF + 2] += 1.0;
+            v[4] += h * w;
+            v[5] = fmax(v[5], speed);
+            if (speed > activity_speed) v[13] += h * w;
+        }
+        if (f.unstable[i] != 0 || (h > dry && speed > activity_speed))
+            tile_flags[(r / tile_size) * tiles_x + c / tile_size] = 1;
+        const double delta = ld_field(resting, f32, i) + h - ld_field(initial, f32, i);
+        v[6] += delta * w;
+        v[7] += fabs(delta) * w * 0.5;
+        v[0] += f.r2m_cumulative[i];
+        v[1] += m2r[i];
+        if (count <= 0) continue;
+        const double s = f.slope[i];
+        v[kPersF + 0] += 1.0;
+        v[2] += w;
+        v[8] += s;
+        v[9] = fmax(v[9], s);
+        v[10] += s - stop_rad;
+        v[11] += start_rad - s;
+        if (count > 1) {
+            v[kPersF + 1] += 1.0;
+            v[3] += w;
+        }
+        if (h <= dry && s <= stop_rad && f.latch[i] == 0) {
+            v[kPersF + 3] += 1.0;
+            v[12] += w;
+        }
+    }
+    block_fold<double, kPersV, kPersMax>(v, lds);
+    if (threadIdx.x == 0)
+        for (int k = 0; k < kPersV; k++) partials[(size_t)k * gridDim.x + blockIdx.x] = v[k];
+}
+
+// fold of the workgroup partials (field-parallel, workgroup order) and the
+// count of flagged tiles
+__global__ void __launch_bounds__(kBlock) persistence_final_kernel(const double *partials, int nparts,
+                                                                   const int32_t *tile_flags, int tile_count,
+                                                                   double *out)
+{
+    __shared__ double lds[(kPersV + 1) * kBlock];
+    double v[kPersV + 1];
+#pragma unroll
+    for (int k = 0; k <= kPersV; k++) v[k] = 0.0;
+    for (int p = threadIdx.x; p < nparts; p += kBlock)
+        for (int k = 0; k < kPersV; k++) {
+            const double x = partials[(size_t)k * nparts + p];
+            v[k] = ((kPersMax >> k) & 1u) ? fmax(v[k], x) : v[k] + x;
+        }
+    for (int t = threadIdx.x; t < tile_count; t += kBlock) v[kPersV] += tile_flags[t] != 0 ? 1.0 : 0.0;
+    block_fold<double, kPersV + 1, kPersMax>(v, lds);
+    if (threadIdx.x == 0)
+        for (int k = 0; k <= kPersV; k++) out[k] = v[k];
+}
+
 static int check_ctx(mv2_context *ctx)
 {
     if (!ctx) return set_error(nullptr, MV2_EINVAL, "[GpuLargeAvalanche] null context");
@@ -565,5 +646,47 @@ extern "C" int mv2_avalanche_mobilize(mv2_context *ctx, const mv2_avalanche_conf
     diag_f[11] = s[2];
     diag_f[12] = s[3];
     diag_f[13] = s[4];
+    return MV2_OK;
+}
+
+extern "C" int mv2_avalanche_persistence(mv2_context *ctx, const mv2_avalanche_config *cfg,
+                                         const mv2_avalanche_fields *f, const void *initial_b_eff,
+                                         const double *m2r_cumulative, int tile_size, int tiles_x,
+                                         int tile_count, int32_t *tile_flags, int32_t diag_i[4],
+                                         double diag_f[14], int32_t *active_tile_count)
+{
+    if (int rc = check_ctx(ctx)) return rc;
+    if (!cfg || !f || !initial_b_eff || !m2r_cumulative || !tile_flags || !diag_i || !diag_f ||
+        !active_tile_count)
+        return set_error(ctx, MV2_EINVAL, "[GpuLargeAvalanche] null argument");
+    const int ny = ctx->ny, nx = ctx->nx;
+    if (tile_size < 1 || tiles_x != (nx + tile_size - 1) / tile_size ||
+        tile_count != tiles_x * ((ny + tile_size - 1) / tile_size))
+        return set_error(ctx, MV2_EINVAL, "[GpuLargeAvalanche] tile grid does not match the state");
+    const int64_t n = (int64_t)ny * nx;
+    const int f32 = ctx->precision == MV2_FP32;
+    const double area = ctx->dx * ctx->dy, f6 = area / 6.0;
+    const size_t need = sizeof(LabelAux) + sizeof(double) * kPersV * kLabelParts;
+    if (int rc = ensure_scratch(ctx, ctx->label_aux, need)) return rc;
+    double *partials = (double *)((char *)ctx->label_aux.ptr + sizeof(LabelAux));
+    double *dres = (double *)((char *)ctx->result.ptr + 1024);
+    const int sgrid = grid_for(n, kBlock, kLabelParts);
+    hipLaunchKernelGGL(fill32, dim3(grid_for(tile_count)), dim3(kBlock), 0, ctx->stream, tile_flags,
+                       (int64_t)tile_count, 0);
+    hipLaunchKernelGGL(persistence_kernel, dim3(sgrid), dim3(kBlock), 0, ctx->stream, initial_b_eff, ctx->b_eff,
+                       ctx->mobile, ctx->qx, ctx->qy, f32, ny, nx, area, f6, cfg->weight_mode, *f, m2r_cumulative,
+                       cfg->start_rad, cfg->stop_rad, cfg->dry_tolerance, cfg->activity_speed, tile_size, tiles_x,
+                       tile_flags, partials);
+    hipLaunchKernelGGL(persistence_final_kernel, dim3(1), dim3(kBlock), 0, ctx->stream, partials, sgrid,
+                       tile_flags, tile_count, dres);
+    MV2_HIP(ctx, hipGetLastError());
+    MV2_HIP(ctx, hipMemcpyAsync(ctx->host_result.ptr, dres, sizeof(double) * (kPersV + 1), hipMemcpyDeviceToHost,
+                                ctx->stream));
+    MV2_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    double s[kPersV + 1];
+    memcpy(s, ctx->host_result.ptr, sizeof(s));
+    for (int k = 0; k < kPersF; k++) diag_f[k] = s[k];
+    for (int k = 0; k < 4; k++) diag_i[k] = (int32_t)s[kPersF + k];
+    *active_tile_count = (int32_t)s[kPersV];
     return MV2_OK;
 }

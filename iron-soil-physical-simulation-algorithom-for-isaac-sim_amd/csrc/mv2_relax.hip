@@ -14,6 +14,7 @@
 // An iteration costs one read + one write of the field instead of eight
 // full-array passes.  Directions and parities: mv2_relax_tile.hpp.
 #include "mv2_internal.hpp"
+#include "mv2_frontier_edge.hpp"
 #include "mv2_relax_tile.hpp"
 
 #include <cmath>
@@ -193,19 +194,7 @@ __global__ void __launch_bounds__(kBlock) frontier_relax_kernel(
     for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < n_edges; e += gridDim.x * blockDim.x) {
         const int64_t a = (int64_t)a_i[e] * cols + a_j[e];
         const int64_t b = (int64_t)b_i[e] * cols + b_j[e];
-        if (reached[a] == 0 && reached[b] == 0) continue;
-        double difference = height[a] - height[b];
-        double excess = fabs(difference) - critical_difference;
-        if (excess > 0.0) {
-            double signed_excess = difference < 0.0 ? -excess : excess;
-            double wa = weights[a], wb = weights[b];
-            double ws = wa + wb;
-            double ta = signed_excess * wb / ws;
-            double tb = signed_excess * wa / ws;
-            height[a] = height[a] - ta;
-            height[b] = height[b] + tb;
-            reached[a] = 1;
-            reached[b] = 1;
+        if (frontier_edge_transfer(height, weights, reached, a, b, critical_difference)) {
             moved++;
             mark_tile_halo(a_i[e], a_j[e], rows, cols, tile_size, tiles_x, moved_flags);
             mark_tile_halo(b_i[e], b_j[e], rows, cols, tile_size, tiles_x, moved_flags);
@@ -224,10 +213,8 @@ __global__ void __launch_bounds__(kBlock) frontier_scan_kernel(
     for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < n_edges; e += gridDim.x * blockDim.x) {
         const int64_t a = (int64_t)a_i[e] * cols + a_j[e];
         const int64_t b = (int64_t)b_i[e] * cols + b_j[e];
-        if (reached[a] != 0 || reached[b] != 0) {
-            double excess = fabs(height[a] - height[b]) - critical_difference;
-            if (excess > tolerance) atomicMax(&flags[tile_ids[e]], 1);
-        }
+        if (frontier_edge_unstable(height, reached, a, b, critical_difference, tolerance))
+            atomicMax(&flags[tile_ids[e]], 1);
     }
 }
 

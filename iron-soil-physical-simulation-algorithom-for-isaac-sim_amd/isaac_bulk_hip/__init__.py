@@ -37,11 +37,14 @@ from .mobile_v2 import (
 from .relaxation import (
     CompactActiveEdgeBatch,
     CompactTileFrontier,
+    FrontierRoundsResult,
+    GpuResidualFrontier,
     HipCompactActiveEdgeOperator,
     HipMinimumSlopeAdapter,
     RelaxationStats,
     WarpCompactActiveEdgeOperator,
     legacy_critical_differences,
+    neighbor_pairs,
     relax_critical_slope,
     relax_device,
 )

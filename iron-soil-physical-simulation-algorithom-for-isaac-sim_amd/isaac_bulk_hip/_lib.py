@@ -86,6 +86,19 @@ class DepositionConfig(ctypes.Structure):
     ]
 
 
+class FrontierRoundConfig(ctypes.Structure):
+    _fields_ = [
+        ("rows", _I), ("cols", _I), ("tile_size", _I), ("tiles_x", _I), ("tile_count", _I),
+        ("n_directions", _I), ("di", _I * 4), ("dj", _I * 4), ("critical_difference", _D * 4),
+        ("tolerance", _D), ("max_rounds", _I), ("reserved", _I),
+    ]
+
+
+class FrontierRoundResult(ctypes.Structure):
+    _fields_ = [("rounds", _I), ("converged", _I), ("active_tiles", _I), ("cleared_reached", _I),
+                ("moved_edges", ctypes.c_longlong)]
+
+
 class BatchDesc(ctypes.Structure):
     _fields_ = [
         ("ny", _I), ("nx", _I), ("offset", ctypes.c_int64),
@@ -122,10 +135,14 @@ SIGNATURES: dict[str, tuple[object, list[object]]] = {
     "mv2_frontier_relax": (_I, [_P, _P, _P, _P, _P, _P, _P, _P, _I, _I, _I, _I, _I, _I, _D, _P,
                                 ctypes.POINTER(_I32)]),
     "mv2_frontier_scan": (_I, [_P, _P, _P, _P, _P, _P, _P, _P, _I, _I, _I, _D, _D, _P]),
+    "mv2_frontier_rounds": (_I, [_P, ctypes.POINTER(FrontierRoundConfig), _P, _P, _P, _P,
+                                 ctypes.POINTER(FrontierRoundResult)]),
     "mv2_avalanche_label": (_I, [_P, ctypes.POINTER(AvalancheConfig), ctypes.POINTER(AvalancheFields),
                                  ctypes.POINTER(LabelResult)]),
     "mv2_avalanche_previous": (_I, [_P, ctypes.POINTER(AvalancheFields), _I32, _I, _I, _P, _P, _I, _I,
                                     _I]),
+    "mv2_avalanche_persistence": (_I, [_P, ctypes.POINTER(AvalancheConfig), ctypes.POINTER(AvalancheFields),
+                                       _P, _P, _I, _I, _I, _P, _P, _P, ctypes.POINTER(_I32)]),
     "mv2_avalanche_mobilize": (_I, [_P, ctypes.POINTER(AvalancheConfig),
                                     ctypes.POINTER(AvalancheFields), _I32, _D, _P]),
     "mv2_label_mask": (_I, [_P, _P, _P, _I, _I, _I]),

@@ -134,6 +134,7 @@ class DeviceBulkState:
         for name in self._F64_FIELDS:
             rt.zeros(name, self.size, torch.float64)
         rt.zeros("dirty_tile_flags", self.tile_count, torch.int32)
+        rt.zeros("avalanche_activity_tile_flags", self.tile_count, torch.int32)
         # per-cell amounts of the last deposition step (metres; FP32: mobile quanta)
         rt.zeros("deposition_work", self.size, self._mdtype)
         self._bind()
@@ -351,7 +352,7 @@ class DeviceBulkState:
         rt = self.runtime
         rt.arrays["b_eff"].copy_(rt.arrays["initial_b_eff"])
         for name in ("z_base", "mobile", "momentum_x", "momentum_y", "dirty_tile_flags",
-                     "deposition_work", *self._INT_FIELDS, *self._F64_FIELDS):
+                     "avalanche_activity_tile_flags", "deposition_work", *self._INT_FIELDS, *self._F64_FIELDS):
             rt.arrays[name].zero_()
         rt.synchronize()
         self._timestamp_device_s = 0.0

@@ -190,52 +190,44 @@ class DeviceLargeAvalancheBridge:
             newly_activated_area_m2=float(mob[12]), reactivated_area_step_m2=float(mob[13]), **p)
 
     def _persistence_summary(self) -> dict:
-        """Lifecycle scalars of reduce_persistence (gpu_large_avalanche.py:379-451, :769-839)."""
-        torch = __import__("torch")
+        """Lifecycle scalars of reduce_persistence (gpu_large_avalanche.py:378-451,
+        :769-839), reduced on the device in one pass (mv2_avalanche_persistence)."""
         st = self.state
-        a = st.runtime.arrays
-        cfg = self.config
-        w = a["weights"]
-        h = st.mobile_m().double()
-        qx, qy = a["momentum_x"].double(), a["momentum_y"].double()
-        wet = h > cfg.dry_tolerance_m
-        safe = torch.where(wet, h, torch.ones_like(h))
-        speed = torch.where(wet, torch.sqrt((qx / safe) ** 2 + (qy / safe) ** 2), torch.zeros_like(h))
-        count = a["avalanche_activation_count"]
-        activated = count > 0
-        slope = a["avalanche_slope"]
-        start_rad = float(np.deg2rad(self.material.start_angle_deg))
-        stop_rad = float(np.deg2rad(self.material.stop_angle_deg))
-        unique = int(activated.sum().item())
-        delta = a["b_eff"].double() + h - a["initial_b_eff"].double()
-        moving = wet & (speed > cfg.mobile_activity_speed_m_s)
-        retired = activated & (~wet) & (slope <= stop_rad) & (a["avalanche_latch"] == 0)
-        active_cells = (a["avalanche_unstable"] != 0) | moving
-        rows = torch.arange(st.size, device=w.device) // st.shape[1]
-        cols = torch.arange(st.size, device=w.device) % st.shape[1]
-        tiles = (rows // st.tile_size) * st.tile_shape[1] + cols // st.tile_size
-        active_tiles = int(torch.unique(tiles[active_cells]).numel())
+        rt = st.runtime
+        cfg = avalanche_config(self.material, self.config, release_settled_latches=True)
+        fields = avalanche_fields(st)
+        di = np.zeros(4, dtype=np.int32)
+        df = np.zeros(14)
+        tiles = ctypes.c_int32(0)
+        rt.bind_stream()
+        rt.check(rt.lib.mv2_avalanche_persistence(
+            rt.ctx, ctypes.byref(cfg), ctypes.byref(fields), rt.ptr("initial_b_eff"),
+            rt.ptr("avalanche_m2r_cumulative"), st.tile_size, st.tile_shape[1], st.tile_count,
+            rt.ptr("avalanche_activity_tile_flags"), di.ctypes.data, df.ctypes.data, ctypes.byref(tiles)))
+        rt.telemetry.record_d2h(di)
+        rt.telemetry.record_d2h(df)
+        unique = int(di[0])
 
         def deg(x):
             return float(np.rad2deg(x))
         return {
-            "cumulative_resting_to_mobile_m3": float(a["avalanche_r2m_cumulative"].sum().item()),
-            "cumulative_mobile_to_resting_m3": float(a["avalanche_m2r_cumulative"].sum().item()),
+            "cumulative_resting_to_mobile_m3": float(df[0]),
+            "cumulative_mobile_to_resting_m3": float(df[1]),
             "unique_cells_ever_activated": unique,
-            "cells_activated_more_than_once": int((count > 1).sum().item()),
-            "unique_activated_area_m2": float(w[activated].sum().item()),
-            "reactivated_area_m2": float(w[count > 1].sum().item()),
-            "retired_candidate_area_m2": float(w[retired].sum().item()),
-            "current_mobile_volume_m3": float((h * w)[wet].sum().item()),
-            "moving_mobile_volume_m3": float((h * w)[moving].sum().item()),
-            "maximum_mobile_speed_m_s": float(speed.max().item()) if st.size else 0.0,
-            "active_tile_count": active_tiles,
-            "net_terrain_volume_change_m3": float((delta * w).sum().item()),
-            "net_spatial_transfer_m3": float((delta.abs() * w * 0.5).sum().item()),
-            "mean_activated_slope_deg": deg(slope[activated].sum().item() / unique) if unique else 0.0,
-            "maximum_activated_slope_deg": deg(slope[activated].max().item()) if unique else 0.0,
-            "mean_slope_minus_theta_stop_deg": deg((slope[activated] - stop_rad).sum().item() / unique) if unique else 0.0,
-            "mean_theta_start_minus_slope_deg": deg((start_rad - slope[activated]).sum().item() / unique) if unique else 0.0,
+            "cells_activated_more_than_once": int(di[1]),
+            "unique_activated_area_m2": float(df[2]),
+            "reactivated_area_m2": float(df[3]),
+            "retired_candidate_area_m2": float(df[12]),
+            "current_mobile_volume_m3": float(df[4]),
+            "moving_mobile_volume_m3": float(df[13]),
+            "maximum_mobile_speed_m_s": float(df[5]),
+            "active_tile_count": int(tiles.value),
+            "net_terrain_volume_change_m3": float(df[6]),
+            "net_spatial_transfer_m3": float(df[7]),
+            "mean_activated_slope_deg": deg(df[8] / unique) if unique else 0.0,
+            "maximum_activated_slope_deg": deg(df[9]),
+            "mean_slope_minus_theta_stop_deg": deg(df[10] / unique) if unique else 0.0,
+            "mean_theta_start_minus_slope_deg": deg(df[11] / unique) if unique else 0.0,
         }
 
     def diagnostics(self) -> dict:

@@ -2,15 +2,19 @@
 
 * ``relax_critical_slope`` / ``relax_device``: slope_model.relax_critical_slope
   (slope_model.py:273-343) -- the plain relaxation and the error judge of the
-  learned warm start -- as one cooperative HIP launch per solve (LDS tiles,
-  all 8 red-black sub-phases of an iteration per tile, one grid barrier per
-  iteration, device-side convergence test).
+  learned warm start -- one HIP launch per iteration (LDS tiles, all 8
+  red-black sub-phases of an iteration per tile, device-side convergence
+  test, launches enqueued in batches with one host read per batch).
 * ``HipMinimumSlopeAdapter``: MinimumSlopeAdapter (pkg/solvers/
   minimum_slope_adapter.py:14-320) on canonical H[y, x], closed boundary.
 * ``CompactTileFrontier`` / ``CompactActiveEdgeBatch``: the host-side edge
   batch builder (pkg/solvers/sparse_tile_slope.py:20-230), and
   ``HipCompactActiveEdgeOperator``: WarpCompactActiveEdgeOperator
-  (pkg/solvers/warp_compact_frontier.py:144-405) with HIP kernels.
+  (pkg/solvers/warp_compact_frontier.py:144-405) with HIP kernels, plus
+  ``advance_rounds``: the residual projection loop of the frame scheduler
+  (pkg/runtime/v2_physics_core.py:1172-1254) run on the device.
+* ``GpuResidualFrontier``: that loop's seeds, active tile set and failure
+  modes (v2_physics_core.py:1142-1254) around ``advance_rounds``.
 """
 
 from __future__ import annotations
@@ -292,6 +296,21 @@ class CompactTileFrontier:
         return batch
 
 
+def neighbor_pairs(dx: float, dy: float) -> list[tuple[int, int, float]]:
+    """slope_model._neighbor_pairs (slope_model.py:744-750)."""
+    return [(1, 0, float(dx)), (0, 1, float(dy)), (1, 1, float(np.hypot(dx, dy))),
+            (1, -1, float(np.hypot(dx, dy)))]
+
+
+@dataclass(frozen=True)
+class FrontierRoundsResult:
+    rounds: int
+    converged: bool
+    active_tile_ids: np.ndarray
+    moved_edge_count: int
+    cleared_reached: bool
+
+
 @dataclass(frozen=True)
 class FrontierPhaseResult:
     moved_edge_count: int
@@ -411,6 +430,58 @@ class HipCompactActiveEdgeOperator:
         return FrontierPhaseResult(moved.moved_edge_count, moved.triggered_tile_ids,
                                    unstable.unstable_owner_tile_ids)
 
+    def advance_rounds(self, active_tiles, *, neighbors, critical_tangent: float, tolerance_m: float,
+                       max_rounds: int) -> FrontierRoundsResult:
+        """Residual projection rounds on the device (v2_physics_core.py:1183-1254).
+
+        ``active_tiles``: tile ids (host) or a device int32 tensor of
+        ``tile_count`` flags, which is then updated in place to the tiles
+        still active.  ``neighbors``: (di, dj, distance) per direction, the
+        critical difference of a direction being ``critical_tangent *
+        distance``.  Stops when the active set is empty (``reached`` is then
+        zeroed) or after ``max_rounds`` rounds."""
+        torch = __import__("torch")
+        if not self._initialized:
+            raise RuntimeError("[WarpMiniSlope] initialize resident state first")
+        pairs = list(neighbors)
+        if not 1 <= len(pairs) <= 4:
+            raise ValueError("[WarpMiniSlope] 1 to 4 neighbour directions")
+        if int(max_rounds) < 0:
+            raise ValueError("[WarpMiniSlope] max_rounds must be non-negative")
+        rt = self.runtime
+        if isinstance(active_tiles, torch.Tensor):
+            flags = active_tiles
+            if (flags.dtype != torch.int32 or flags.numel() != self.tile_count
+                    or flags.device != rt.device or not flags.is_contiguous()):
+                raise ValueError("[WarpMiniSlope] tile flags must be a contiguous device int32 "
+                                 "tensor of tile_count entries")
+        else:
+            ids = np.unique(np.asarray(active_tiles, dtype=np.int64).ravel())
+            if ids.size and (ids[0] < 0 or ids[-1] >= self.tile_count):
+                raise ValueError("[WarpMiniSlope] tile ID outside grid")
+            host = np.zeros(self.tile_count, dtype=np.int32)
+            host[ids] = 1
+            flags = torch.from_numpy(host).to(rt.device)
+        cfg = _lib.FrontierRoundConfig()
+        cfg.rows, cfg.cols = self.shape
+        cfg.tile_size, cfg.tiles_x, cfg.tile_count = self.tile_size, self.tile_shape[1], self.tile_count
+        cfg.n_directions = len(pairs)
+        for d, (di, dj, distance) in enumerate(pairs):
+            cfg.di[d], cfg.dj[d] = int(di), int(dj)
+            cfg.critical_difference[d] = float(critical_tangent) * float(distance)
+        cfg.tolerance = float(tolerance_m)
+        cfg.max_rounds = int(max_rounds)
+        out = _lib.FrontierRoundResult()
+        rt.bind_stream()
+        rc = rt.lib.mv2_frontier_rounds(rt.ctx, ctypes.byref(cfg), rt.ptr(self._height_key),
+                                        rt.ptr(self._weights_key), rt.ptr(self._reached_key),
+                                        flags.data_ptr(), ctypes.byref(out))
+        rt.check(rc)
+        active = (np.empty(0, dtype=np.int32) if out.converged
+                  else np.flatnonzero(flags.cpu().numpy()).astype(np.int32))
+        return FrontierRoundsResult(int(out.rounds), bool(out.converged), active, int(out.moved_edges),
+                                    bool(out.cleared_reached))
+
     def download_state(self):
         h = self.runtime.download(self._height_key).reshape(self.shape)
         r = self.runtime.download(self._reached_key).reshape(self.shape).astype(bool)
@@ -421,3 +492,76 @@ class HipCompactActiveEdgeOperator:
 
 
 WarpCompactActiveEdgeOperator = HipCompactActiveEdgeOperator
+
+
+class GpuResidualFrontier:
+    """Residual MiniSlope projection of the frame scheduler on the device.
+
+    Mirrors V2PhysicsCore's GPU frontier state (pkg/runtime/v2_physics_core.py:
+    338-343, 966-973, 1142-1254): cell seeds mark ``frontier_reached`` and
+    activate the tiles around them, residual seed tiles join the active set,
+    and ``advance()`` runs the bounded rounds -- here all on the device --
+    raising the reference's NUMERICAL_NONCONVERGENCE errors."""
+
+    def __init__(self, operator: HipCompactActiveEdgeOperator, *, stop_angle_deg: float, dx: float,
+                 dy: float, tolerance_m: float = 1.0e-8, max_rounds: int = 4096,
+                 numerical_safety_max_iterations: int = 1_000_000) -> None:
+        torch = __import__("torch")
+        self.operator = operator
+        self.critical_tangent = float(np.tan(np.deg2rad(stop_angle_deg)))
+        self.neighbors = neighbor_pairs(dx, dy)
+        self.tolerance_m = float(tolerance_m)
+        self.max_rounds = int(max_rounds)
+        self.numerical_safety_max_iterations = int(numerical_safety_max_iterations)
+        self.iterations = 0
+        self.frontier = CompactTileFrontier(operator.shape, operator.tile_size)
+        self._flags = torch.zeros(operator.tile_count, dtype=torch.int32, device=operator.runtime.device)
+
+    @property
+    def active_tile_ids(self) -> np.ndarray:
+        return np.flatnonzero(self._flags.cpu().numpy()).astype(np.int32)
+
+    def add_tiles(self, tile_ids) -> None:
+        """Union of residual seed tiles into the active set (v2_physics_core.py:966-973)."""
+        ids = np.unique(np.asarray(tile_ids, dtype=np.int64).ravel())
+        if ids.size == 0:
+            return
+        if ids[0] < 0 or ids[-1] >= self.operator.tile_count:
+            raise ValueError("[WarpMiniSlope] tile ID outside grid")
+        torch = __import__("torch")
+        self._flags[torch.from_numpy(ids).to(self._flags.device)] = 1
+
+    def seed_cells(self, flat_indices) -> None:
+        """_seed_gpu_residual_frontier (v2_physics_core.py:1152-1170)."""
+        torch = __import__("torch")
+        indices = np.unique(np.asarray(flat_indices, dtype=np.int64).ravel())
+        if indices.size == 0:
+            return
+        rows_total, cols_total = self.operator.shape
+        if indices[0] < 0 or indices[-1] >= rows_total * cols_total:
+            raise ValueError("[WarpMiniSlope] seed index outside grid")
+        reached = self.operator.runtime.arrays[self.operator._reached_key]
+        reached[torch.from_numpy(indices).to(reached.device)] = 1
+        rows, cols = np.divmod(indices, cols_total)
+        self.add_tiles(self.frontier.tiles_around_cells(rows, cols))
+
+    def clear(self) -> None:
+        """Discard the candidate seeds and the reach mask (v2_physics_core.py:991-1000)."""
+        self._flags.zero_()
+        self.operator.runtime.arrays[self.operator._reached_key].zero_()
+
+    def advance(self) -> FrontierRoundsResult:
+        """_advance_gpu_residual_frontier (v2_physics_core.py:1172-1254)."""
+        safety_left = self.numerical_safety_max_iterations - self.iterations
+        budget = max(0, min(self.max_rounds, safety_left))
+        result = self.operator.advance_rounds(self._flags, neighbors=self.neighbors,
+                                              critical_tangent=self.critical_tangent,
+                                              tolerance_m=self.tolerance_m, max_rounds=budget)
+        self.iterations += result.rounds
+        if not result.converged:
+            if safety_left < self.max_rounds:
+                raise RuntimeError("NUMERICAL_NONCONVERGENCE: GPU residual MiniSlope exceeded "
+                                   "the numerical safety limit")
+            raise RuntimeError("NUMERICAL_NONCONVERGENCE: V3 GPU residual projection exceeded "
+                               f"{result.rounds} bounded numerical rounds")
+        return result

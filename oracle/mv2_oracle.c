@@ -894,6 +894,60 @@ void orc_components(int ny, int nx, const int32_t *labels,
     free(counts);
 }
 
+/* reduce_persistence (gpu_large_avalanche.py:378-451) in cell order; the
+ * maxima start at zero like the reference's zeroed accumulators */
+void orc_persistence(int ny, int nx, int tile_size, int tiles_x, const double *initial_resting,
+                     const double *resting, const double *mobile, const double *qx, const double *qy,
+                     const double *weights, const double *slope, const int32_t *unstable,
+                     const int32_t *latch, const int32_t *activation_count, const double *r2m,
+                     const double *m2r, double start_rad, double stop_rad, double dry,
+                     double activity_speed, int32_t diag_i[4], double diag_f[14], int32_t *tile_flags)
+{
+    for (int k = 0; k < 4; k++) diag_i[k] = 0;
+    for (int k = 0; k < 14; k++) diag_f[k] = 0.0;
+    int tiles_y = (ny + tile_size - 1) / tile_size;
+    for (int t = 0; t < tiles_y * tiles_x; t++) tile_flags[t] = 0;
+    for (int row = 0; row < ny; row++)
+        for (int col = 0; col < nx; col++) {
+            size_t i = (size_t)row * nx + col;
+            int count = activation_count[i];
+            double h = mobile[i];
+            double speed = 0.0;
+            double w = weights[i];
+            if (h > dry) {
+                double vx = qx[i] / h, vy = qy[i] / h;
+                speed = sqrt(vx * vx + vy * vy);
+                diag_i[2] += 1;
+                diag_f[4] += h * w;
+                diag_f[5] = npmax(diag_f[5], speed);
+                if (speed > activity_speed) diag_f[13] += h * w;
+            }
+            if (unstable[i] != 0 || (h > dry && speed > activity_speed))
+                tile_flags[(row / tile_size) * tiles_x + col / tile_size] = 1;
+            double delta = resting[i] + h - initial_resting[i];
+            diag_f[6] += delta * w;
+            diag_f[7] += fabs(delta) * w * 0.5;
+            diag_f[0] += r2m[i];
+            diag_f[1] += m2r[i];
+            if (count <= 0) continue;
+            double s = slope[i];
+            diag_i[0] += 1;
+            diag_f[2] += w;
+            diag_f[8] += s;
+            diag_f[9] = npmax(diag_f[9], s);
+            diag_f[10] += s - stop_rad;
+            diag_f[11] += start_rad - s;
+            if (count > 1) {
+                diag_i[1] += 1;
+                diag_f[3] += w;
+            }
+            if (h <= dry && s <= stop_rad && latch[i] == 0) {
+                diag_i[3] += 1;
+                diag_f[12] += w;
+            }
+        }
+}
+
 /* mobilize (gpu_large_avalanche.py:309-376); diag_f indices as there */
 void orc_mobilize(int n_cells, const int32_t *labels, int32_t winner,
                   const double *z_base, double *resting, double *mobile,

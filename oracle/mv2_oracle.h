@@ -152,6 +152,15 @@ void orc_components(int ny, int nx, const int32_t *labels,
                     double start_rad, double stop_rad,
                     int32_t diag_i[8], double diag_f[8]);
 
+/* reduce_persistence over every cell: diag_i[4], diag_f[14] as the reference's
+ * avalanche_persistence_int/float; tile_flags[tile_count] set to 1 where a cell
+ * is unstable or carries moving mobile material */
+void orc_persistence(int ny, int nx, int tile_size, int tiles_x, const double *initial_resting,
+                     const double *resting, const double *mobile, const double *qx, const double *qy,
+                     const double *weights, const double *slope, const int32_t *unstable,
+                     const int32_t *latch, const int32_t *activation_count, const double *r2m,
+                     const double *m2r, double start_rad, double stop_rad, double dry,
+                     double activity_speed, int32_t diag_i[4], double diag_f[14], int32_t *tile_flags);
 void orc_mobilize(int n_cells, const int32_t *labels, int32_t winner,
                   const double *z_base, double *resting, double *mobile,
                   const double *weights, int32_t *latch, double *owned_surface,

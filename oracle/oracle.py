@@ -101,6 +101,7 @@ def lib() -> ctypes.CDLL:
             _I, _I, ctypes.POINTER(AvalancheParams), _P, _P, _P, _P, _P, _P, _P, _P,
             _P, _P, _P, _P, _P, _P]
         _lib.orc_label.argtypes = [_I, _I, _I, _P, _P]
+        _lib.orc_persistence.argtypes = [_I, _I, _I, _I] + [_P] * 12 + [_D] * 4 + [_P] * 3
         _lib.orc_components.argtypes = [_I, _I, _P, _P, _P, _P, _P, _P, _P, _P, _D, _D, _D,
                                         _P, _P]
         _lib.orc_mobilize.argtypes = [_I, _P, ctypes.c_int32, _P, _P, _P, _P, _P, _P, _P, _P,
@@ -308,6 +309,93 @@ def frontier_scan(height, reached, tile_ids, a_i, a_j, b_i, b_j, critical_differ
                             _ptr(a_i), _ptr(a_j), _ptr(b_i), _ptr(b_j), a_i.size,
                             critical_difference, tolerance, _ptr(flags))
     return flags
+
+
+def edge_batch(shape, tile_size, active_tiles, di, dj, phase):
+    """CompactTileFrontier.edge_batch (sparse_tile_slope.py:116-230): the edges
+    (a, a + (di, dj)) owned by every cell a of the active tiles whose
+    coordinate along the direction (row if di else column) has parity
+    `phase`.  Returns (owner tile, a_i, a_j, b_i, b_j)."""
+    rows, cols = shape
+    tiles_x = (cols + tile_size - 1) // tile_size
+    tiles = np.unique(np.asarray(active_tiles, dtype=np.int64))
+    li, lj = np.indices((tile_size, tile_size))
+    li, lj = li.ravel(), lj.ravel()
+    a_i = ((tiles // tiles_x)[:, None] * tile_size + li[None, :]).ravel()
+    a_j = ((tiles % tiles_x)[:, None] * tile_size + lj[None, :]).ravel()
+    owner = np.repeat(tiles, li.size)
+    b_i, b_j = a_i + di, a_j + dj
+    coord = a_i if di else a_j
+    keep = ((a_i < rows) & (a_j < cols) & (b_i >= 0) & (b_i < rows) & (b_j >= 0) & (b_j < cols)
+            & ((coord & 1) == phase))
+    return tuple(v[keep].astype(np.int32) for v in (owner, a_i, a_j, b_i, b_j))
+
+
+def frontier_rounds(height, weights, reached, active_tiles, *, tile_size, neighbors, critical_tangent,
+                    tolerance, max_rounds):
+    """The residual projection loop of V2PhysicsCore._advance_gpu_residual_frontier
+    (pkg/runtime/v2_physics_core.py:1172-1254) over the oracle's phase
+    functions: per round the relax phases of every direction and parity over
+    the active tiles (tiles a phase moves join for the later phases), then
+    the scan of the round's tiles; its unstable owner tiles are the next
+    active set.  `neighbors` is a list of (di, dj, distance).
+
+    Returns (height, reached, rounds, active_tiles, moved_edges, cleared)."""
+    height = _f64(height).copy()
+    reached = _i32(reached).copy()
+    rows, cols = height.shape
+    tiles_x = (cols + tile_size - 1) // tile_size
+    tile_count = ((rows + tile_size - 1) // tile_size) * tiles_x
+    active = np.unique(np.asarray(active_tiles, dtype=np.int32))
+    rounds, moved_total, cleared = 0, 0, False
+    for _ in range(max_rounds):
+        if active.size == 0:
+            break
+        rounds += 1
+        phase_tiles = active
+        for di, dj, distance in neighbors:
+            for phase in (0, 1):
+                _, a_i, a_j, b_i, b_j = edge_batch((rows, cols), tile_size, phase_tiles, di, dj, phase)
+                if a_i.size == 0:
+                    continue
+                height, reached, moved, flags = frontier_relax(
+                    height, weights, reached, a_i, a_j, b_i, b_j, critical_tangent * distance,
+                    tile_size, tiles_x)
+                moved_total += moved
+                triggered = np.flatnonzero(flags).astype(np.int32)
+                if triggered.size:
+                    phase_tiles = np.union1d(phase_tiles, triggered).astype(np.int32)
+        owners = np.zeros(tile_count, dtype=np.int32)
+        for di, dj, distance in neighbors:
+            for phase in (0, 1):
+                owner, a_i, a_j, b_i, b_j = edge_batch((rows, cols), tile_size, phase_tiles, di, dj, phase)
+                if a_i.size == 0:
+                    continue
+                owners |= frontier_scan(height, reached, owner, a_i, a_j, b_i, b_j,
+                                        critical_tangent * distance, tolerance, tile_count)
+        active = np.flatnonzero(owners).astype(np.int32)
+        if active.size == 0:
+            reached[:] = 0
+            cleared = True
+            break
+    return height, reached, rounds, active, moved_total, cleared
+
+
+def persistence(initial, resting, mobile, qx, qy, weights, slope, unstable, latch, activation_count,
+                r2m, m2r, *, tile_size, start_rad, stop_rad, dry, activity_speed):
+    """reduce_persistence (gpu_large_avalanche.py:378-451): (diag_i[4], diag_f[14], tile_flags)."""
+    ny, nx = np.shape(resting)
+    f = [_f64(v) for v in (initial, resting, mobile, qx, qy, weights, slope)]
+    ints = [_i32(v) for v in (unstable, latch, activation_count)]
+    r2m, m2r = _f64(r2m), _f64(m2r)
+    tiles_x = (nx + tile_size - 1) // tile_size
+    flags = np.zeros(((ny + tile_size - 1) // tile_size) * tiles_x, dtype=np.int32)
+    di = np.zeros(4, dtype=np.int32)
+    df = np.zeros(14)
+    lib().orc_persistence(ny, nx, tile_size, tiles_x, *(_ptr(v) for v in f), *(_ptr(v) for v in ints),
+                          _ptr(r2m), _ptr(m2r), start_rad, stop_rad, dry, activity_speed, _ptr(di),
+                          _ptr(df), _ptr(flags))
+    return di, df, flags
 
 
 def avalanche_initialize(z, resting, mobile, qx, qy, weights, owned_baseline, export_cum, latch,

@@ -294,3 +294,47 @@ def test_threaded_oracle_gives_the_same_fields():
     np.testing.assert_array_equal(h1, h2)
     np.testing.assert_array_equal(qx1, qx2)
     np.testing.assert_array_equal(qy1, qy2)
+
+
+def test_persistence_reduction_hand_case():
+    """reduce_persistence (gpu_large_avalanche.py:378-451) on a 6 x 8 grid
+    whose every scalar is known by hand: one dry cell activated twice at a
+    slope below the stop angle (a retired candidate), one wet cell moving at
+    0.5 m/s, one unstable cell, and a mass-conserving move of 0.02 m."""
+    ny, nx, ts = 6, 8, 4
+    w = np.full((ny, nx), 0.0025)
+    initial = np.ones((ny, nx))
+    resting = initial.copy()
+    resting[0, 0] += 0.02
+    resting[5, 7] -= 0.02
+    mobile = np.zeros((ny, nx))
+    qx = np.zeros((ny, nx))
+    qy = np.zeros((ny, nx))
+    mobile[1, 6] = 0.1
+    qx[1, 6], qy[1, 6] = 0.03, 0.04           # |v| = 0.5 m/s
+    resting[1, 6] -= 0.1                       # the mobile layer came from below
+    slope = np.zeros((ny, nx))
+    count = np.zeros((ny, nx), dtype=np.int32)
+    count[4, 2], slope[4, 2] = 2, 0.4
+    unstable = np.zeros((ny, nx), dtype=np.int32)
+    unstable[4, 5] = 1
+    latch = np.zeros((ny, nx), dtype=np.int32)
+    r2m = np.zeros((ny, nx))
+    r2m[4, 2] = 3e-4
+    m2r = np.zeros((ny, nx))
+    m2r[1, 6] = 1e-4
+    stop, start = np.deg2rad(30.0), np.deg2rad(38.0)
+    di, df, flags = oracle.persistence(initial, resting, mobile, qx, qy, w, slope, unstable, latch,
+                                       count, r2m, m2r, tile_size=ts, start_rad=start, stop_rad=stop,
+                                       dry=1e-9, activity_speed=0.05)
+    assert list(di) == [1, 1, 1, 1]
+    assert df[0] == 3e-4 and df[1] == 1e-4
+    assert df[2] == 0.0025 and df[3] == 0.0025 and df[12] == 0.0025
+    assert df[4] == pytest.approx(0.1 * 0.0025) and df[13] == df[4]
+    assert df[5] == pytest.approx(0.5, rel=1e-15)
+    assert df[6] == pytest.approx(0.0, abs=1e-18)          # the moves conserve the surface
+    assert df[7] == pytest.approx(0.5 * 0.04 * 0.0025, rel=1e-12)
+    assert df[8] == 0.4 and df[9] == 0.4
+    assert df[10] == pytest.approx(0.4 - stop) and df[11] == pytest.approx(start - 0.4)
+    # tiles (2 x 2 of 4 x 4 cells): the moving cell (1, 6) -> tile 1; the unstable (4, 5) -> tile 3
+    assert list(flags) == [0, 1, 0, 1]
