@@ -23,6 +23,8 @@ from .config import (
     TerrainGrid,
     TerrainVolumeIntegrator,
 )
+from .checkpoint import (load_frozen_boundary, restore_device_checkpoint, sha256_array,
+                         write_device_checkpoint)
 from .deposition import WarpDepositionOperator, WarpDepositionStep
 from .large_avalanche import DeviceLargeAvalancheBridge, DeviceLargeAvalancheResult, label_mask
 from .mobile_v2 import (

@@ -94,7 +94,9 @@ class DeviceBulkState:
                    "avalanche_severity", "avalanche_first_activation_time",
                    "avalanche_last_activation_time", "avalanche_owned_surface",
                    "avalanche_owned_export_baseline", "mobile_export_cumulative",
-                   "avalanche_r2m_cumulative", "avalanche_m2r_cumulative")
+                   "avalanche_r2m_cumulative", "avalanche_m2r_cumulative", "track_rut",
+                   "mobile_flux_export_cumulative")
+    _STATE_FIELDS = ("z_base", "b_eff", "mobile", "momentum_x", "momentum_y")
 
     def __init__(self, grid: TerrainGrid, initial_resting_m, *, device: str = "cuda:0",
                  tile_size: int = 64, precision: str = "fp64",
@@ -304,45 +306,93 @@ class DeviceBulkState:
             raise BulkStateAuthorityError(
                 "[DeviceBulkState] full host views are restricted to explicit boundaries, "
                 f"got source={source!r}")
-        rt = self.runtime
-        rt.synchronize()
-        resting = rt.download("b_eff").astype(np.float64).reshape(self.shape)
-        z = rt.download("z_base").astype(np.float64).reshape(self.shape)
-        mobile = rt.download("mobile").astype(np.float64).reshape(self.shape)
-        if self.precision == "fp32":
-            mobile = mobile * MOBILE_QUANTUM_M
-        q = np.stack((rt.download("momentum_x").astype(np.float64).reshape(self.shape),
-                      rt.download("momentum_y").astype(np.float64).reshape(self.shape)), axis=-1)
+        f = self.download_fields(self._STATE_FIELDS)
+        resting, z, mobile = f["b_eff"], f["z_base"], f["mobile"]
+        q = np.stack((f["momentum_x"], f["momentum_y"]), axis=-1)
         self._timestamp_host_view_s = self._timestamp_device_s
         return HostBulkStateView(resting, mobile, q, self._timestamp_device_s,
                                  self._reset_generation, source, self.consume_dirty_tile_ids(), z)
 
+    # -- packed full-field transfers (explicit boundaries only) ---------------------
+    def _is_float_field(self, name: str) -> bool:
+        return name in self._STATE_FIELDS or name in self._F64_FIELDS
+
+    def download_fields(self, names) -> dict[str, np.ndarray]:
+        """Named fields as host arrays -- float fields in float64 metres (FP32
+        mobile quanta times 2^-24, exact), int fields int32 -- packed on the
+        device and copied in one pinned transfer per dtype."""
+        torch = __import__("torch")
+        rt = self.runtime
+        names = list(names)
+        floats = [n for n in names if self._is_float_field(n)]
+        ints = [n for n in names if n in self._INT_FIELDS]
+        unknown = sorted(set(names) - set(floats) - set(ints))
+        if unknown:
+            raise BulkStateAuthorityError(f"[DeviceBulkState] unknown fields: {unknown}")
+        out: dict[str, np.ndarray] = {}
+        for group, dtype in ((floats, torch.float64), (ints, torch.int32)):
+            if not group:
+                continue
+            parts = []
+            for name in group:
+                t = rt.arrays[name]
+                if name == "mobile" and self.precision == "fp32":
+                    t = t.double() * MOBILE_QUANTUM_M
+                parts.append(t.reshape(-1).to(dtype))
+            packed = torch.cat(parts)
+            host = torch.empty(packed.shape, dtype=dtype, pin_memory=True)
+            host.copy_(packed)
+            arr = host.numpy()
+            rt.telemetry.record_d2h(arr)
+            for k, name in enumerate(group):
+                out[name] = arr[k * self.size:(k + 1) * self.size].reshape(self.shape).copy()
+        return out
+
+    def _upload_fields(self, fields: dict) -> None:
+        torch = __import__("torch")
+        rt = self.runtime
+        floats = [n for n in fields if self._is_float_field(n)]
+        ints = [n for n in fields if n in self._INT_FIELDS]
+        for group, dtype in ((floats, np.float64), (ints, np.int32)):
+            if not group:
+                continue
+            packed = np.concatenate([np.ascontiguousarray(np.asarray(fields[n], dtype=dtype)).ravel()
+                                     for n in group])
+            rt.telemetry.record_h2d(packed)
+            dev = torch.from_numpy(packed).pin_memory().to(rt.device, non_blocking=True)
+            for k, name in enumerate(group):
+                view = dev[k * self.size:(k + 1) * self.size]
+                target = rt.arrays[name]
+                if name == "mobile" and self.precision == "fp32":
+                    q = torch.round(view * (1.0 / MOBILE_QUANTUM_M))
+                    if bool(((q > np.iinfo(np.int32).max) | (q < np.iinfo(np.int32).min)).any()):
+                        raise ValueError("[DeviceBulkState] mobile thickness exceeds the FP32 storage range")
+                    target.copy_(q.to(torch.int32))
+                else:
+                    target.copy_(view.to(target.dtype))
+        rt.synchronize()
+
     def restore_checkpoint_fields(self, fields: dict, *, timestamp_s: float, source: str) -> None:
+        """Explicit checkpoint boundary (bulk_state_authority.py:1219-1278):
+        validated fields uploaded packed, one transfer per dtype."""
         if source not in {"acceptance", "checkpoint"}:
             raise BulkStateAuthorityError(
                 "[DeviceBulkState] checkpoint restore requires an explicit boundary")
-        torch = __import__("torch")
-        float_state = {"z_base", "b_eff", "resting", "mobile", "momentum_x", "momentum_y"}
-        known = float_state | set(self._F64_FIELDS) | set(self._INT_FIELDS)
+        known = {"resting"} | set(self._STATE_FIELDS) | set(self._F64_FIELDS) | set(self._INT_FIELDS)
         unknown = sorted(set(fields) - known)
         if unknown:
             raise BulkStateAuthorityError(f"[DeviceBulkState] unsupported checkpoint fields: {unknown}")
-        state_values = {}
+        values = {}
         for name, value in fields.items():
             key = "b_eff" if name == "resting" else name
             arr = np.asarray(value)
             if arr.shape != self.shape or not np.all(np.isfinite(arr)):
                 raise BulkStateAuthorityError(f"[DeviceBulkState] invalid checkpoint field {key!r}")
-            if key in float_state:
-                state_values[key] = arr
-            else:
-                dtype = torch.float64 if key in self._F64_FIELDS else torch.int32
-                self.runtime.arrays[key].copy_(torch.from_numpy(np.ascontiguousarray(arr).ravel()).to(
-                    self.runtime.device, dtype))
-        self.set_fields(**state_values)
+            values[key] = arr
         t = float(timestamp_s)
         if not np.isfinite(t) or t < 0.0:
             raise BulkStateAuthorityError("[DeviceBulkState] checkpoint timestamp must be finite/non-negative")
+        self._upload_fields(values)
         self._timestamp_device_s = t
         self._timestamp_host_view_s = -1.0
         self.mark_dirty_tiles(range(self.tile_count))
