@@ -124,6 +124,27 @@ int mv2_deposit(mv2_context *ctx, const int32_t *idx, const double *depth, int n
  * out = resting volume, mobile volume, density*px, density*py */
 int mv2_reservoirs(mv2_context *ctx, int weight_mode, double density, double out[4]);
 
+/* ---- surface publication: DeviceBulkState.capture_surface_for_dirty_tracking
+ *      / collect_surface_dirty_tiles (pkg/runtime/bulk_state_authority.py:733-781,
+ *      kernels :156-189), sample_surface_bilinear (:1138-1181, kernel :196-228)
+ *      and download_dirty_tiles (:1280-1328).  Surface = b_eff + mobile. ---- */
+/* surface_before (device, ny*nx double) = the current free surface */
+int mv2_surface_capture(mv2_context *ctx, double *surface_before);
+/* tiles whose surface moved by more than tolerance since the capture:
+ * tile_flags (device, tile_count) rewritten; tile_ids_out (HOST, tile_count
+ * entries) receives the ascending ids, count_out their number */
+int mv2_surface_dirty_tiles(mv2_context *ctx, const double *surface_before, double tolerance,
+                            int tile_size, int tiles_x, int tile_count, int32_t *tile_flags,
+                            int32_t *tile_ids_out, int *count_out);
+/* bilinear free-surface samples at n fractional (row, col) pairs (device
+ * double[n][2], clamped to the grid) into out (device double[n]) */
+int mv2_surface_sample(mv2_context *ctx, const double *rows_cols, int n, double *out);
+/* free surface of n_tiles tiles (device ids) with their one-cell overlap:
+ * tile j fills out[j*(tile_size+1)^2 ...] row-major over
+ * (min(r0+tile_size+1, ny) - r0) x (min(c0+tile_size+1, nx) - c0) cells */
+int mv2_surface_gather_tiles(mv2_context *ctx, const int32_t *tile_ids, int n_tiles, int tile_size,
+                             int tiles_x, int tile_count, double *out);
+
 /* ---- Mobile -> Resting deposition: WarpDepositionOperator.step_resident
  *      (pkg/bulk_interaction/warp_deposition.py:180-235) with its kernels
  *      deposit (:23-115) and weighted_sum (:117-129).  A cell with mobile

@@ -130,6 +130,10 @@ SIGNATURES: dict[str, tuple[object, list[object]]] = {
     "mv2_entrain": (_I, [_P, _P, _P, _I, _I, ctypes.POINTER(_D)]),
     "mv2_deposit": (_I, [_P, _P, _P, _I, _I, _D, _P]),
     "mv2_reservoirs": (_I, [_P, _I, _D, _P]),
+    "mv2_surface_capture": (_I, [_P, _P]),
+    "mv2_surface_dirty_tiles": (_I, [_P, _P, _D, _I, _I, _I, _P, _P, ctypes.POINTER(_I)]),
+    "mv2_surface_sample": (_I, [_P, _P, _I, _P]),
+    "mv2_surface_gather_tiles": (_I, [_P, _P, _I, _I, _I, _I, _P]),
     "mv2_deposition_step": (_I, [_P, ctypes.POINTER(DepositionConfig), _P, _P, _P, _P, _P, _P]),
     "mv2_relax": (_I, [_P, _I, _P, _I, _I, _P, _I, _D, ctypes.POINTER(RelaxResult)]),
     "mv2_frontier_relax": (_I, [_P, _P, _P, _P, _P, _P, _P, _P, _I, _I, _I, _I, _I, _I, _D, _P,

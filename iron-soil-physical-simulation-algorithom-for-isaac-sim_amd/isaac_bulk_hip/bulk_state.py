@@ -137,6 +137,7 @@ class DeviceBulkState:
             rt.zeros(name, self.size, torch.float64)
         rt.zeros("dirty_tile_flags", self.tile_count, torch.int32)
         rt.zeros("avalanche_activity_tile_flags", self.tile_count, torch.int32)
+        rt.zeros("surface_before", self.size, torch.float64)
         # per-cell amounts of the last deposition step (metres; FP32: mobile quanta)
         rt.zeros("deposition_work", self.size, self._mdtype)
         self._bind()
@@ -293,6 +294,83 @@ class DeviceBulkState:
         fy, ly = max(0, row0 // self.tile_size), min(ty - 1, (row1 - 1) // self.tile_size)
         fx, lx = max(0, col0 // self.tile_size), min(tx - 1, (col1 - 1) // self.tile_size)
         self.mark_dirty_tiles(r * tx + c for r in range(fy, ly + 1) for c in range(fx, lx + 1))
+
+    def capture_surface_for_dirty_tracking(self) -> None:
+        """Snapshot b_eff + mobile before a mutating operator (bulk_state_authority.py:733-745)."""
+        rt = self.runtime
+        rt.bind_stream()
+        rt.check(rt.lib.mv2_surface_capture(rt.ctx, rt.ptr("surface_before")))
+
+    def collect_surface_dirty_tiles(self, *, tolerance_m: float = 1.0e-10) -> np.ndarray:
+        """Tiles whose surface moved by more than the tolerance since the capture
+        (:747-781): flagged and compacted to ascending ids on the device."""
+        tolerance = float(tolerance_m)
+        if not np.isfinite(tolerance) or tolerance < 0.0:
+            raise ValueError("[DeviceBulkState] dirty tolerance must be finite/non-negative")
+        rt = self.runtime
+        rt.bind_stream()
+        ids = np.zeros(self.tile_count, dtype=np.int32)
+        count = ctypes.c_int(0)
+        rt.check(rt.lib.mv2_surface_dirty_tiles(
+            rt.ctx, rt.ptr("surface_before"), tolerance, self.tile_size, self.tile_shape[1],
+            self.tile_count, rt.ptr("dirty_tile_flags"), ids.ctypes.data, ctypes.byref(count)))
+        tile_ids = ids[:count.value].copy()
+        rt.telemetry.record_d2h(tile_ids)
+        self.mark_dirty_tiles(tile_ids)
+        return tile_ids
+
+    def sample_surface_bilinear(self, row_col, *, source: str) -> np.ndarray:
+        """Bilinear free-surface heights at fractional (row, col) queries (:1138-1181)."""
+        torch = __import__("torch")
+        if not source:
+            raise ValueError("[DeviceBulkState] terrain query source must be non-empty")
+        queries = np.asarray(row_col, dtype=np.float64)
+        if queries.ndim != 2 or queries.shape[1] != 2 or not np.all(np.isfinite(queries)):
+            raise ValueError("[DeviceBulkState] terrain queries must be finite [N,2]")
+        count = int(queries.shape[0])
+        if count == 0:
+            return np.empty(0, dtype=np.float64)
+        rt = self.runtime
+        q = torch.from_numpy(np.ascontiguousarray(queries)).to(rt.device)
+        out = torch.empty(count, dtype=torch.float64, device=rt.device)
+        rt.bind_stream()
+        rt.check(rt.lib.mv2_surface_sample(rt.ctx, q.data_ptr(), count, out.data_ptr()))
+        values = out.cpu().numpy()
+        rt.telemetry.record_h2d(queries)
+        rt.telemetry.record_d2h(values)
+        return values
+
+    def download_dirty_tiles(self, tile_ids=None) -> dict[int, np.ndarray]:
+        """Current surface of the given (default: the recorded dirty) tiles with
+        their one-cell overlap (:1280-1328), gathered by one launch."""
+        torch = __import__("torch")
+        selected = (np.asarray(tuple(tile_ids), dtype=np.int64).ravel() if tile_ids is not None
+                    else self.consume_dirty_tile_ids().astype(np.int64))
+        if selected.size == 0:
+            return {}
+        if np.any(selected < 0) or np.any(selected >= self.tile_count):
+            raise ValueError("[DeviceBulkState] dirty tile ID outside grid")
+        rt = self.runtime
+        ts = self.tile_size
+        stride = (ts + 1) * (ts + 1)
+        result: dict[int, np.ndarray] = {}
+        tiles_x = self.tile_shape[1]
+        for start in range(0, selected.size, 65535):
+            chunk = selected[start:start + 65535]
+            ids = torch.from_numpy(chunk.astype(np.int32)).to(rt.device)
+            out = torch.empty(chunk.size * stride, dtype=torch.float64, device=rt.device)
+            rt.bind_stream()
+            rt.check(rt.lib.mv2_surface_gather_tiles(rt.ctx, ids.data_ptr(), int(chunk.size), ts, tiles_x,
+                                                     self.tile_count, out.data_ptr()))
+            host = out.cpu().numpy()
+            rt.telemetry.record_d2h(host)
+            for j, tile in enumerate(chunk.tolist()):
+                r0, c0 = (tile // tiles_x) * ts, (tile % tiles_x) * ts
+                h = min(r0 + ts + 1, self.shape[0]) - r0
+                w = min(c0 + ts + 1, self.shape[1]) - c0
+                result[int(tile)] = host[j * stride:j * stride + h * w].reshape(h, w).copy()
+        self._timestamp_host_view_s = self._timestamp_device_s
+        return result
 
     def consume_dirty_tile_ids(self) -> np.ndarray:
         out = np.asarray(sorted(self._dirty_tile_ids), dtype=np.int32)

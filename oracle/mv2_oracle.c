@@ -481,6 +481,26 @@ void orc_reservoirs(int n_cells, const double *z_base, const double *b_eff,
     }
 }
 
+/* bilinear_surface (bulk_state_authority.py:196-228) */
+void orc_surface_sample(int ny, int nx, const double *b, const double *m, const double *rows_cols,
+                        int n, double *out)
+{
+    for (int k = 0; k < n; k++) {
+        double row = npmin(npmax(rows_cols[2 * k], 0.0), (double)(ny - 1));
+        double col = npmin(npmax(rows_cols[2 * k + 1], 0.0), (double)(nx - 1));
+        int r0 = (int)floor(row), c0 = (int)floor(col);
+        int r1 = r0 + 1 < ny - 1 ? r0 + 1 : ny - 1;
+        int c1 = c0 + 1 < nx - 1 ? c0 + 1 : nx - 1;
+        double fr = row - (double)r0, fc = col - (double)c0;
+        double v00 = b[(size_t)r0 * nx + c0] + m[(size_t)r0 * nx + c0];
+        double v01 = b[(size_t)r0 * nx + c1] + m[(size_t)r0 * nx + c1];
+        double v10 = b[(size_t)r1 * nx + c0] + m[(size_t)r1 * nx + c0];
+        double v11 = b[(size_t)r1 * nx + c1] + m[(size_t)r1 * nx + c1];
+        out[k] = (1.0 - fr) * (1.0 - fc) * v00 + (1.0 - fr) * fc * v01 + fr * (1.0 - fc) * v10
+                 + fr * fc * v11;
+    }
+}
+
 /* ------------------------------------------------------------------------ */
 /* Mobile -> Resting deposition: warp_deposition.py:23-115 (deposit) and     */
 /* :117-129 (weighted_sum), one step_resident call (:180-235).  Every cell   */

@@ -102,6 +102,10 @@ void orc_deposition32(int ny, int nx, const orc_deposition_params *p, const int3
                       int32_t *deposited, float *removed_x, float *removed_y,
                       double *m2r_cumulative, const double *weights, double out[3]);
 
+/* bilinear free-surface samples (b + m) at n fractional (row, col) pairs */
+void orc_surface_sample(int ny, int nx, const double *b, const double *m, const double *rows_cols,
+                        int n, double *out);
+
 /* relax_critical_slope on an array in the legacy [axis0][axis1] layout */
 int orc_relax_legacy_f64(int n0, int n1, double d0, double d1, double crit_slope,
                          int max_iterations, double tolerance, double *h,

@@ -101,6 +101,7 @@ def lib() -> ctypes.CDLL:
             _I, _I, ctypes.POINTER(AvalancheParams), _P, _P, _P, _P, _P, _P, _P, _P,
             _P, _P, _P, _P, _P, _P]
         _lib.orc_label.argtypes = [_I, _I, _I, _P, _P]
+        _lib.orc_surface_sample.argtypes = [_I, _I, _P, _P, _P, _I, _P]
         _lib.orc_persistence.argtypes = [_I, _I, _I, _I] + [_P] * 12 + [_D] * 4 + [_P] * 3
         _lib.orc_components.argtypes = [_I, _I, _P, _P, _P, _P, _P, _P, _P, _P, _D, _D, _D,
                                         _P, _P]
@@ -253,6 +254,15 @@ def deposition(b, m, qx, qy, cumulative, weights, *, dx, dy, dt, settling_rate=0
     fn(ny, nx, ctypes.byref(p), None if f is None else _ptr(f), _ptr(b), _ptr(m), _ptr(qx),
        _ptr(qy), _ptr(dep), _ptr(rx), _ptr(ry), _ptr(cum), _ptr(w), _ptr(out))
     return b, m, qx, qy, dep, rx, ry, cum, out
+
+
+def surface_sample(b, m, rows_cols):
+    """bilinear_surface (bulk_state_authority.py:196-228) of b + m at (row, col) pairs."""
+    b, m = _f64(b), _f64(m)
+    q = _f64(rows_cols).reshape(-1, 2)
+    out = np.zeros(q.shape[0])
+    lib().orc_surface_sample(b.shape[0], b.shape[1], _ptr(b), _ptr(m), _ptr(q), q.shape[0], _ptr(out))
+    return out
 
 
 def reservoirs(z, b, m, qx, qy, w, density):

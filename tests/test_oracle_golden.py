@@ -338,3 +338,27 @@ def test_persistence_reduction_hand_case():
     assert df[10] == pytest.approx(0.4 - stop) and df[11] == pytest.approx(start - 0.4)
     # tiles (2 x 2 of 4 x 4 cells): the moving cell (1, 6) -> tile 1; the unstable (4, 5) -> tile 3
     assert list(flags) == [0, 1, 0, 1]
+
+
+def test_surface_sample_reproduces_planes_and_grid_values():
+    """bilinear_surface (bulk_state_authority.py:196-228): a bilinear
+    interpolant reproduces a plane everywhere inside the grid, returns the
+    grid value at integer points, and clamps queries to the grid."""
+    ny, nx = 9, 13
+    rr, cc = np.meshgrid(np.arange(ny, dtype=float), np.arange(nx, dtype=float), indexing="ij")
+    plane = 0.7 + 0.03 * rr - 0.011 * cc
+    m = np.zeros((ny, nx))
+    m[2, 3] = 0.25
+    rng = np.random.default_rng(2)
+    q = np.column_stack((rng.uniform(0, ny - 1, 200), rng.uniform(0, nx - 1, 200)))
+    got = oracle.surface_sample(plane, np.zeros((ny, nx)), q)
+    np.testing.assert_allclose(got, 0.7 + 0.03 * q[:, 0] - 0.011 * q[:, 1], rtol=0, atol=1e-15)
+    pts = np.array([[2.0, 3.0], [0.0, 0.0], [ny - 1.0, nx - 1.0], [4.0, 12.0]])
+    np.testing.assert_array_equal(oracle.surface_sample(plane, m, pts),
+                                  [plane[2, 3] + 0.25, plane[0, 0], plane[-1, -1], plane[4, 12]])
+    far = np.array([[-5.0, -3.0], [100.0, 200.0], [2.0, -1.0]])
+    np.testing.assert_array_equal(oracle.surface_sample(plane, m, far),
+                                  [plane[0, 0], plane[-1, -1], plane[2, 0]])
+    # halfway between the bump cell and its neighbour: the mean of the two
+    assert oracle.surface_sample(plane, m, [[2.0, 3.5]])[0] == pytest.approx(
+        0.5 * (plane[2, 3] + 0.25 + plane[2, 4]), rel=1e-15)
