@@ -501,6 +501,8 @@ __global__ void __launch_bounds__(kBlock) v2_begin_kernel(V2Args<P> a)
 // its tile from that buffer before the loop state and the CFL maximum arrive,
 // so all three loads share one memory round trip; launches expected to find
 // the loop ended (eager = 0) read the loop state first and skip the staging.
+constexpr int kSubFold = 3 + kDiag;   // values folded per workgroup at the end of a substep
+
 template <class P, int SH = kSH>
 __global__ void __launch_bounds__(kBlock) v2_substep_kernel(V2Args<P> a, int k, int eager)
 {
@@ -616,20 +618,25 @@ __global__ void __launch_bounds__(kBlock) v2_substep_kernel(V2Args<P> a, int k, 
             ovf |= o.overflow;
         }
     }
-    block_reduce<double, 1, 1>(wmax, smem);
-    double hm[1] = {hmin};
-    block_reduce<double, 1, 2>(hm, smem);
-    const int anyovf = __syncthreads_or(ovf);
-    block_reduce<double, kDiag, 0>(acc.diag, smem);
+    // one field-parallel fold of the workgroup's values: 0 wave-speed maximum,
+    // 1 overflow flag (maximum), 2 minimum new depth, 3.. diagnostic sums
+    __shared__ double fold_lds[kSubFold * kBlock];
+    double fv[kSubFold];
+    fv[0] = wmax[0];
+    fv[1] = ovf ? 1.0 : 0.0;
+    fv[2] = hmin;
+#pragma unroll
+    for (int d = 0; d < kDiag; d++) fv[3 + d] = acc.diag[d];
+    block_fold<double, kSubFold, 0x3u, 0x4u>(fv, fold_lds);
     if (threadIdx.x == 0) {
-        spread_max(a.spread + ((k + 1) % kWaveSets) * kSpreadWords, wmax[0]);
+        spread_max(a.spread + ((k + 1) % kWaveSets) * kSpreadWords, fv[0]);
         const double tol = P::kQuant ? -1.0e-300 : -2.0e-13;
         int f = 0;
-        if (hm[0] < tol) f |= 1;
-        if (anyovf) f |= 2;
+        if (fv[2] < tol) f |= 1;
+        if (fv[1] != 0.0) f |= 2;
         if (f) atomicOr(&s->flags[(k + 1) & 1], f);
         // each workgroup owns its diagnostic slot across the substep launches
-        for (int d = 0; d < kDiag; d++) a.diag_part[(int64_t)d * a.diag_groups + blockIdx.x] = dprev[d] + acc.diag[d];
+        for (int d = 0; d < kDiag; d++) a.diag_part[(int64_t)d * a.diag_groups + blockIdx.x] = dprev[d] + fv[3 + d];
     }
 }
 
