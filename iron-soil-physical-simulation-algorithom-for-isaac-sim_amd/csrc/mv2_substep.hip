@@ -290,7 +290,7 @@ __device__ inline void xcd_tile_range(int tiles, int &first, int &last)
 // trip instead of one per cell.
 // ---------------------------------------------------------------------------
 constexpr int kLedgerU = 4;
-constexpr int kFinalizeBlock = 256;
+constexpr int kFinalizeBlock = 512;
 
 struct CellVals {
     double b, depth, qx, qy;
@@ -880,8 +880,8 @@ __global__ void __launch_bounds__(kBlock) v2_end32_kernel(V2Args<F32P> a, int k)
 }
 
 // Fold the partial ledgers and diagnostics in fixed order into the record.
-// One workgroup of kFinalizeBlock threads.  Thread p accumulates parts p,
-// p + kFinalizeBlock, ... of every field, issuing all loads of a part at once
+// One workgroup of kFinalizeBlock (512) threads.  Thread p accumulates parts p,
+// p + kFinalizeBlock, ... of every field, issuing all loads of two parts at once
 // (one memory round trip per part); a field-parallel LDS fold then reduces
 // the threads, each field by one wave.
 constexpr int kFoldF = 2 * kSum + kDiag + 2;   // double fields: ledgers, diagnostics, static maxima
@@ -896,8 +896,6 @@ __global__ void __launch_bounds__(kFinalizeBlock) v2_finalize_kernel(V2Args<P> a
 {
     __shared__ double lds[kFoldF * kFinalizeBlock];
     constexpr int kNW = kFinalizeBlock / kWave;
-    const Control C = resolve_control(a.sync, k);
-    const bool after_valid = !C.static_exit;
     double fsum[kFoldF];
 #pragma unroll
     for (int f = 0; f < kFoldF; f++) fsum[f] = ((kFoldMin >> f) & 1u) ? 1e300 : 0.0;
@@ -905,7 +903,7 @@ __global__ void __launch_bounds__(kFinalizeBlock) v2_finalize_kernel(V2Args<P> a
 #pragma unroll
     for (int f = 0; f < kFoldQ; f++) qsum[f] = 0;
     // kFoldU parts per thread and round, all their loads issued together
-    constexpr int kFoldU = 4;
+    constexpr int kFoldU = 2;
     for (int p0 = threadIdx.x; p0 < nparts; p0 += kFoldU * kFinalizeBlock) {
         double x[kFoldU][2 * kSum];
         long long q[kFoldU][kFoldQ];
@@ -955,6 +953,9 @@ __global__ void __launch_bounds__(kFinalizeBlock) v2_finalize_kernel(V2Args<P> a
     block_fold<double, kFoldF, kFoldMax, kFoldMin, kNW>(fsum, lds);
     block_fold<long long, kFoldQ, 0u, 0u, kNW>(qsum, reinterpret_cast<long long *>(lds));
     if (threadIdx.x == 0) {
+        // the loop state is read only here: the partial loads above do not wait for it
+        const Control C = resolve_control(a.sync, k);
+        const bool after_valid = !C.static_exit;
         mv2_step_record *rec = a.rec;
         for (int q = 0; q < kSum; q++) {
             rec->summary_before[q] = fsum[q];
@@ -1036,7 +1037,9 @@ static int launch_step(mv2_context *ctx, const mv2_step_config *cfg, mv2_step_re
     a.diag_part = (double *)ctx->step_diag.ptr;
     a.diag_groups = gu;
     a.spread = parity ? spread_odd : spread_even;
-    a.rec = (mv2_step_record *)ctx->result.ptr;
+    // the finalize pass writes the record straight into pinned host memory
+    // (device-visible): no copy launch before the stream sync
+    a.rec = (mv2_step_record *)ctx->host_result.ptr;
     a.clamp_quarter = cfg->cfl <= 0.25 ? 1 : 0;
     hipStream_t st = ctx->stream;
     const bool timing = ctx->timing != 0;
@@ -1077,7 +1080,6 @@ static int launch_step(mv2_context *ctx, const mv2_step_config *cfg, mv2_step_re
         hipLaunchKernelGGL(v2_finalize_kernel<P>, dim3(1), dim3(kFinalizeBlock), 0, st, a, k, gs, gu);
         MV2_HIP(ctx, hipGetLastError());
         MV2_HIP(ctx, hipEventRecord(ctx->ev_stop, st));
-        MV2_HIP(ctx, hipMemcpyAsync(ctx->host_result.ptr, a.rec, sizeof(mv2_step_record), hipMemcpyDeviceToHost, st));
         MV2_HIP(ctx, hipStreamSynchronize(st));
         if (hrec->complete) break;
         // more substeps than enqueued: the state so far is valid, enqueue more
