@@ -996,11 +996,30 @@ template <class P>
 static int launch_step(mv2_context *ctx, const mv2_step_config *cfg, mv2_step_record *out, int fixed_chunk = 0)
 {
     const int64_t n = (int64_t)ctx->ny * ctx->nx;
-    // FP32 band height of the substep: 4 rows (measured faster than 2 on a
-    // 701^2 terrain, where 2-row bands give twice the waves); 2 on request
+    // FP32 band height of the substep (rows per wave): the workgroups of a
+    // launch spread over the CUs, and the CUs that receive one more
+    // workgroup than the rest set the launch time.  The band minimises that
+    // busiest CU's work, ceil(workgroups / CUs) x (band + 1.5) -- a wave's
+    // rows plus its halo rows and fixed costs (701^2: 5 rows, 432
+    // workgroups, at most 2 per CU, against 3 per CU with 4 rows).
+    // MV2_FRAME_BAND_ROWS forces a band of 2 to 6.
     int band = 4;
     if (P::kQuant) {
-        if (const char *e = getenv("MV2_FRAME_BAND_ROWS")) band = atoi(e) == 2 ? 2 : 4;
+        const int strips = (ctx->nx + kStripW - 1) / kStripW;
+        const int cus = ctx->cu_count > 0 ? ctx->cu_count : 256;
+        double best = 1e300;
+        for (int cand = 3; cand <= 6; cand++) {
+            const int wgs = strips * ((ctx->ny + cand * (kBlock / kWave) - 1) / (cand * (kBlock / kWave)));
+            const double cost = (double)((wgs + cus - 1) / cus) * (cand + 1.5);
+            if (cost < best - 1e-9) {
+                best = cost;
+                band = cand;
+            }
+        }
+        if (const char *e = getenv("MV2_FRAME_BAND_ROWS")) {
+            const int forced = atoi(e);
+            if (forced >= 2 && forced <= 6) band = forced;
+        }
     }
     const int rows_sub = band * (kBlock / kWave);
     const int gu = P::kQuant ? ((ctx->nx + kStripW - 1) / kStripW) * ((ctx->ny + rows_sub - 1) / rows_sub)
@@ -1068,8 +1087,13 @@ static int launch_step(mv2_context *ctx, const mv2_step_config *cfg, mv2_step_re
             if (timing) MV2_HIP(ctx, hipEventRecord(ctx->sub_events[2 * k], st));
             const int eager = k < eager_until ? 1 : 0;
             if constexpr (P::kQuant) {
-                if (band == 2) hipLaunchKernelGGL((v2_substep_kernel<P, 2>), dim3(gu), dim3(kBlock), 0, st, a, k, eager);
-                else hipLaunchKernelGGL((v2_substep_kernel<P, 4>), dim3(gu), dim3(kBlock), 0, st, a, k, eager);
+                switch (band) {
+                case 2: hipLaunchKernelGGL((v2_substep_kernel<P, 2>), dim3(gu), dim3(kBlock), 0, st, a, k, eager); break;
+                case 3: hipLaunchKernelGGL((v2_substep_kernel<P, 3>), dim3(gu), dim3(kBlock), 0, st, a, k, eager); break;
+                case 5: hipLaunchKernelGGL((v2_substep_kernel<P, 5>), dim3(gu), dim3(kBlock), 0, st, a, k, eager); break;
+                case 6: hipLaunchKernelGGL((v2_substep_kernel<P, 6>), dim3(gu), dim3(kBlock), 0, st, a, k, eager); break;
+                default: hipLaunchKernelGGL((v2_substep_kernel<P, 4>), dim3(gu), dim3(kBlock), 0, st, a, k, eager); break;
+                }
             } else {
                 hipLaunchKernelGGL(v2_substep_kernel<P>, dim3(gu), dim3(kBlock), 0, st, a, k, eager);
             }

@@ -80,6 +80,28 @@ def test_fp32_pile_128_within_tolerance_and_mass_exact():
     assert abs(step.substeps - rec["substeps"]) <= max(2, rec["substeps"] // 20)
 
 
+def test_fp32_band_heights_give_identical_fields(monkeypatch):
+    """The FP32 substep's band height (rows per wave, chosen per grid) only
+    changes how cells are grouped into waves: every band gives the same bits."""
+    from isaac_bulk_hip import MobileV2State
+    from isaac_bulk_hip.workloads import pile_with_patch
+
+    b, h, _, _ = pile_with_patch(150, seed=3)
+    state = MobileV2State(b.astype(np.float32).astype(np.float64), np.rint(h * 2.0 ** 24) * 2.0 ** -24,
+                          np.zeros(h.shape + (2,)))
+    results = {}
+    for band in (2, 3, 4, 5, 6):
+        monkeypatch.setenv("MV2_FRAME_BAND_ROWS", str(band))
+        step = _solver("fp32", cfl=0.25).step(state, 0.25)
+        results[band] = (step.state.h_m, step.state.q_m2_s, step.substeps)
+    monkeypatch.delenv("MV2_FRAME_BAND_ROWS")
+    auto = _solver("fp32", cfl=0.25).step(state, 0.25)
+    for band, (hh, qq, n) in results.items():
+        np.testing.assert_array_equal(hh, auto.state.h_m, err_msg=str(band))
+        np.testing.assert_array_equal(qq, auto.state.q_m2_s, err_msg=str(band))
+        assert n == auto.substeps > 3
+
+
 def test_production_step_resident_matches_oracle_and_ledgers():
     from isaac_bulk_hip import (DeviceBulkState, MaterialScenario, MobileV2Config, TerrainGrid,
                                 TerrainVolumeIntegrator, WarpProductionMobileV2Solver)
