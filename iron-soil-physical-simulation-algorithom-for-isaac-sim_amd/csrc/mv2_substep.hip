@@ -266,10 +266,10 @@ __device__ inline CellOut cell64(const V2Args<F64P> &a, int src, int r, int c, d
     return out;
 }
 
-// FP32 tiles of the single-terrain step: kStripW columns by kTileRows rows,
-// one band of kSH rows per wave (faces in mv2_strip32.hpp)
+// FP32 tiles of the single-terrain step: kStripW columns by 4 x SH rows, one
+// band of SH rows per wave (faces in mv2_strip32.hpp); SH is chosen per grid
+// at launch (kSH when nothing else is known)
 constexpr int kSH = 4;
-constexpr int kTileRows = kSH * (kBlock / kWave);
 
 // Workgroups are dispatched round-robin over the 8 XCDs.  Renumber them so
 // that the workgroups of one XCD own one contiguous band of tiles: halos
@@ -687,7 +687,7 @@ __global__ void __launch_bounds__(kBlock) v2_end_kernel(V2Args<P> a, int k)
 
 // ---------------------------------------------------------------------------
 // FP32 ledger passes in the strip layout of the substep (one tile of the
-// substep grid per workgroup, one band of kSH rows per wave): x-neighbours
+// substep grid per workgroup, one band of SH rows per wave): x-neighbours
 // by DPP, y-neighbours from the rows in registers, no integer division.  Per
 // cell the ledger terms of warp_mobile_v2.py:41-54 are formed in float and
 // summed per lane in float over the band, then in double; the mobile volume
@@ -770,22 +770,23 @@ __device__ inline void store_ledger32(const V2Args<F32P> &a, const Ledger32 &L, 
 
 // One pass over the input state: ledger before, static diagnostic and the
 // CFL maximum of the first substep (as v2_begin_kernel, FP32 strip layout).
+template <int SH>
 __global__ void __launch_bounds__(kBlock) v2_begin32_kernel(V2Args<F32P> a)
 {
     __shared__ double lds[kLedger32D * kBlock];
     const int ny = a.ny, nx = a.nx;
     const int tiles_x = (nx + kStripW - 1) / kStripW;
     const int t = xcd_block();
-    const int r0 = (t / tiles_x) * kTileRows + wave_index() * kSH, c0 = (t % tiles_x) * kStripW;
+    const int r0 = (t / tiles_x) * (SH * (kBlock / kWave)) + wave_index() * SH, c0 = (t % tiles_x) * kStripW;
     const int lane = threadIdx.x & (kWave - 1);
     const int gc = c0 - 1 + lane;
     const bool owner = lane >= 1 && lane <= kStripW && gc < nx;
     const bool stat = a.cfg.static_exit != 0;
-    StripRows<kSH> R;
-    strip_load<kSH>(a, 0, r0, c0, R);
-    double H[kSH + 2];
+    StripRows<SH> R;
+    strip_load<SH>(a, 0, r0, c0, R);
+    double H[SH + 2];
 #pragma unroll
-    for (int j = 0; j < kSH + 2; j++) H[j] = (double)R.b[j] + (double)((float)R.hq[j] * (float)kQuantum);
+    for (int j = 0; j < SH + 2; j++) H[j] = (double)R.b[j] + (double)((float)R.hq[j] * (float)kQuantum);
     const float area = uniform((float)(a.dx * a.dy)), f6 = uniform((float)(a.dx * a.dy / 6.0));
     const float dry = uniform((float)a.cfg.dry_tolerance);
     const float Kg = uniform((float)(a.cfg.earth_pressure * a.cfg.gravity));
@@ -794,7 +795,7 @@ __global__ void __launch_bounds__(kBlock) v2_begin32_kernel(V2Args<F32P> a)
     L.init();
     double m[3] = {0.0, 0.0, 0.0};              // max |grad H|^2, max |q|, max wave speed
 #pragma unroll
-    for (int j = 1; j <= kSH; j++) {
+    for (int j = 1; j <= SH; j++) {
         const int r = r0 - 1 + j;
         const double HR = from_next_lane_d(H[j]), HL = from_prev_lane_d(H[j]);
         if (!owner || r >= ny) continue;
@@ -831,6 +832,7 @@ __global__ void __launch_bounds__(kBlock) v2_begin32_kernel(V2Args<F32P> a)
 }
 
 // copy back when the final state sits in scratch; ledger after (FP32 strips)
+template <int SH>
 __global__ void __launch_bounds__(kBlock) v2_end32_kernel(V2Args<F32P> a, int k)
 {
     __shared__ double lds[kLedger32D * kBlock];
@@ -840,16 +842,16 @@ __global__ void __launch_bounds__(kBlock) v2_end32_kernel(V2Args<F32P> a, int k)
     const int ny = a.ny, nx = a.nx;
     const int tiles_x = (nx + kStripW - 1) / kStripW;
     const int t = xcd_block();
-    const int r0 = (t / tiles_x) * kTileRows + wave_index() * kSH, c0 = (t % tiles_x) * kStripW;
+    const int r0 = (t / tiles_x) * (SH * (kBlock / kWave)) + wave_index() * SH, c0 = (t % tiles_x) * kStripW;
     const int lane = threadIdx.x & (kWave - 1);
     const int gc = c0 - 1 + lane;
     const bool owner = lane >= 1 && lane <= kStripW && gc < nx;
     const int32_t *gh = fsrc ? a.h[1] : a.h[0];
     const float *gqx = fsrc ? a.qx[1] : a.qx[0], *gqy = fsrc ? a.qy[1] : a.qy[0];
-    int hq[kSH];
-    float qxv[kSH], qyv[kSH], bv[kSH];
+    int hq[SH];
+    float qxv[SH], qyv[SH], bv[SH];
 #pragma unroll
-    for (int j = 0; j < kSH; j++) {
+    for (int j = 0; j < SH; j++) {
         const int r = r0 + j;
         hq[j] = 0; qxv[j] = 0.0f; qyv[j] = 0.0f; bv[j] = 0.0f;
         if (owner && r < ny) {
@@ -865,7 +867,7 @@ __global__ void __launch_bounds__(kBlock) v2_end32_kernel(V2Args<F32P> a, int k)
     Ledger32 L;
     L.init();
 #pragma unroll
-    for (int j = 0; j < kSH; j++) {
+    for (int j = 0; j < SH; j++) {
         const int r = r0 + j;
         if (!owner || r >= ny) continue;
         if (fsrc == 1) {
@@ -1024,7 +1026,8 @@ static int launch_step(mv2_context *ctx, const mv2_step_config *cfg, mv2_step_re
     const int rows_sub = band * (kBlock / kWave);
     const int gu = P::kQuant ? ((ctx->nx + kStripW - 1) / kStripW) * ((ctx->ny + rows_sub - 1) / rows_sub)
                              : grid_for(n);                              // substep launches
-    const int gl = P::kQuant ? ((ctx->nx + kStripW - 1) / kStripW) * ((ctx->ny + kTileRows - 1) / kTileRows) : 0;
+    // the begin / end passes use the substep's tiles (same band)
+    const int gl = P::kQuant ? ((ctx->nx + kStripW - 1) / kStripW) * ((ctx->ny + rows_sub - 1) / rows_sub) : 0;
     // begin / end passes: the substep tiles (FP32), a grid stride (FP64)
     const int gs = P::kQuant ? gl : grid_for(n, kBlock * kLedgerU, kFinalizeBlock);
     if (int rc = ensure_scratch(ctx, ctx->partials, (sizeof(double) * kLedgerF + sizeof(long long) * kLedgerQ) * (size_t)gs))
@@ -1067,7 +1070,15 @@ static int launch_step(mv2_context *ctx, const mv2_step_config *cfg, mv2_step_re
     // error anywhere below leaves v2_clean at -1 and the next step resets
     if (ctx->v2_clean != parity) hipLaunchKernelGGL(v2_reset_kernel, dim3(1), dim3(kBlock), 0, st, a.sync, a.spread);
     ctx->v2_clean = -1;
-    if constexpr (P::kQuant) hipLaunchKernelGGL(v2_begin32_kernel, dim3(gs), dim3(kBlock), 0, st, a);
+    if constexpr (P::kQuant) {
+        switch (band) {
+        case 2: hipLaunchKernelGGL(v2_begin32_kernel<2>, dim3(gs), dim3(kBlock), 0, st, a); break;
+        case 3: hipLaunchKernelGGL(v2_begin32_kernel<3>, dim3(gs), dim3(kBlock), 0, st, a); break;
+        case 5: hipLaunchKernelGGL(v2_begin32_kernel<5>, dim3(gs), dim3(kBlock), 0, st, a); break;
+        case 6: hipLaunchKernelGGL(v2_begin32_kernel<6>, dim3(gs), dim3(kBlock), 0, st, a); break;
+        default: hipLaunchKernelGGL(v2_begin32_kernel<4>, dim3(gs), dim3(kBlock), 0, st, a); break;
+        }
+    }
     else hipLaunchKernelGGL(v2_begin_kernel<P>, dim3(gs), dim3(kBlock), 0, st, a);
     MV2_HIP(ctx, hipGetLastError());
     int k = 0;
@@ -1099,7 +1110,15 @@ static int launch_step(mv2_context *ctx, const mv2_step_config *cfg, mv2_step_re
             }
             if (timing) MV2_HIP(ctx, hipEventRecord(ctx->sub_events[2 * k + 1], st));
         }
-        if constexpr (P::kQuant) hipLaunchKernelGGL(v2_end32_kernel, dim3(gs), dim3(kBlock), 0, st, a, k);
+        if constexpr (P::kQuant) {
+            switch (band) {
+            case 2: hipLaunchKernelGGL(v2_end32_kernel<2>, dim3(gs), dim3(kBlock), 0, st, a, k); break;
+            case 3: hipLaunchKernelGGL(v2_end32_kernel<3>, dim3(gs), dim3(kBlock), 0, st, a, k); break;
+            case 5: hipLaunchKernelGGL(v2_end32_kernel<5>, dim3(gs), dim3(kBlock), 0, st, a, k); break;
+            case 6: hipLaunchKernelGGL(v2_end32_kernel<6>, dim3(gs), dim3(kBlock), 0, st, a, k); break;
+            default: hipLaunchKernelGGL(v2_end32_kernel<4>, dim3(gs), dim3(kBlock), 0, st, a, k); break;
+            }
+        }
         else hipLaunchKernelGGL(v2_end_kernel<P>, dim3(gs), dim3(kBlock), 0, st, a, k);
         hipLaunchKernelGGL(v2_finalize_kernel<P>, dim3(1), dim3(kFinalizeBlock), 0, st, a, k, gs, gu);
         MV2_HIP(ctx, hipGetLastError());
