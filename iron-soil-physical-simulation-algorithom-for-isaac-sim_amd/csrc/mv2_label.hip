@@ -30,14 +30,24 @@ struct LabelAux {
 };
 
 // initialize (gpu_large_avalanche.py:59-162)
+// The same pass zeroes the per-root counts, the statistics record and the
+// label spread sets for the labelling launches that follow.
 __global__ void __launch_bounds__(kBlock) avalanche_init_kernel(
     const void *z_base, const void *resting, const void *mobile, const void *qx, const void *qy, int f32,
     int ny, int nx, double dx, double dy, mv2_avalanche_config cfg, double area, double f6,
-    mv2_avalanche_fields f)
+    mv2_avalanche_fields f, LabelAux *aux, int32_t n_plus_one, unsigned long long *spread)
 {
     const int64_t n = (int64_t)ny * nx;
+    if (blockIdx.x == 0) {
+        for (int i = threadIdx.x; i < 3 * kSpread; i += blockDim.x) spread[(int64_t)i * kSpreadStride] = 0ull;
+        if (threadIdx.x == 0) {
+            for (int k = 0; k < 8; k++) { aux->diag_i[k] = 0; aux->diag_f[k] = 0.0; }
+            aux->diag_i[2] = n_plus_one;
+        }
+    }
     for (int64_t index = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; index < n;
          index += (int64_t)gridDim.x * blockDim.x) {
+        f.counts[index] = 0;
         const int row = (int)index / nx, col = (int)index - row * nx;
         int64_t left = index, right = index, down = index, up = index;
         double denom_x = dx, denom_y = dy;
@@ -219,81 +229,68 @@ __global__ void fill32(int32_t *p, int64_t n, int32_t v)
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) p[i] = v;
 }
 
-// zero the per-root counts, the statistics record and the spread sets
-__global__ void label_reset_kernel(int32_t *counts, int64_t n, LabelAux *aux, int32_t n_plus_one,
-                                   unsigned long long *spread)
-{
-    const int64_t tid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const int64_t nthreads = (int64_t)gridDim.x * blockDim.x;
-    for (int64_t i = tid; i < n; i += nthreads) counts[i] = 0;
-    for (int64_t i = tid; i < 3 * kSpread; i += nthreads) spread[i * kSpreadStride] = 0ull;
-    if (tid == 0) {
-        for (int k = 0; k < 8; k++) { aux->diag_i[k] = 0; aux->diag_f[k] = 0.0; }
-        aux->diag_i[2] = n_plus_one;
-    }
-}
-
-// region count / largest / unstable (reduce_components :234-243) into spread
-// sets 0 (regions), 1 (unstable cells) and 2 (largest); integer sums, so the
-// order of the atomics does not change the result
+// reduce_components + select_winner (:234-252) in one pass over the per-root
+// counts: region count, unstable cells and the key (count << 32) |
+// (INT32_MAX - root), whose maximum is the largest component with the
+// smallest root (the reference's tie rule), into spread sets 0, 1 and 2.
+// Integer sums and maxima: the order of the atomics does not change them.
 __global__ void __launch_bounds__(kBlock) count_kernel(const int32_t *counts, int64_t n, unsigned long long *spread)
 {
-    __shared__ long long smem[16];
-    long long v[2] = {0, 0};
-    long long mx[1] = {0};
+    __shared__ long long lds[3 * kBlock];
+    long long v[3] = {0, 0, 0};
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
         const int32_t c = counts[i];
         if (c > 0) {
             v[0] += 1;
             v[1] += c;
-            mx[0] = mx[0] >= c ? mx[0] : c;
+            const long long key = ((long long)c << 32) | (long long)(0x7fffffff - (int32_t)i);
+            v[2] = v[2] >= key ? v[2] : key;
         }
     }
-    block_reduce<long long, 2, 0>(v, smem);
-    block_reduce<long long, 1, 1>(mx, smem);
+    block_fold<long long, 3, 0x4u>(v, lds);
     if (threadIdx.x == 0) {
         const int slot = (blockIdx.x % kSpread) * kSpreadStride;
         if (v[0]) atomicAdd(&spread[slot], (unsigned long long)v[0]);
         if (v[1]) atomicAdd(&spread[kSpreadWords + slot], (unsigned long long)v[1]);
-        if (mx[0]) atomicMax(&spread[2 * kSpreadWords + slot], (unsigned long long)mx[0]);
+        if (v[2]) atomicMax(&spread[2 * kSpreadWords + slot], (unsigned long long)v[2]);
     }
 }
 
-// select_winner (:246-252): smallest root among the largest components; the
-// first workgroup also folds the spread sets into the statistics record
-__global__ void __launch_bounds__(kBlock) winner_kernel(const int32_t *counts, int64_t n,
-                                                         const unsigned long long *spread, LabelAux *aux)
+// the counts pass's result: regions, unstable cells, largest count and the
+// winner root (n + 1 when there is no component); workgroup 0 records them
+__device__ inline int32_t resolve_winner(const unsigned long long *spread, int64_t n, LabelAux *aux)
 {
-    __shared__ long long smem[4];
+    __shared__ long long win;
     if (threadIdx.x < kWave) {
         const int slot = threadIdx.x * kSpreadStride;
         const long long regions = wave_sum((long long)spread[slot]);
         const long long unstable = wave_sum((long long)spread[kSpreadWords + slot]);
-        const long long largest = wave_max((long long)spread[2 * kSpreadWords + slot]);
+        const long long key = wave_max((long long)spread[2 * kSpreadWords + slot]);
         if (threadIdx.x == 0) {
-            smem[0] = largest;
+            const int32_t largest = (int32_t)(key >> 32);
+            const int32_t root = largest > 0 ? 0x7fffffff - (int32_t)(key & 0xffffffffll) : (int32_t)(n + 1);
+            win = root;
             if (blockIdx.x == 0) {
                 aux->diag_i[0] = (int32_t)regions;
-                aux->diag_i[1] = (int32_t)largest;
+                aux->diag_i[1] = largest;
+                aux->diag_i[2] = root;
                 aux->diag_i[3] = (int32_t)unstable;
             }
         }
     }
     __syncthreads();
-    const int32_t largest = (int32_t)smem[0];
-    if (largest <= 0) return;
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
-        if (counts[i] == largest) atomicMin(&aux->diag_i[2], (int32_t)i);
+    return (int32_t)win;
 }
 
 // aggregate + winner_diagnostics (:201-231, :255-281) restricted to the winner
 __global__ void __launch_bounds__(kBlock) winner_stats_kernel(
     const int32_t *labels, const void *z_base, const void *resting, int f32, const int32_t *latch,
     const int32_t *previous, const double *slope, const double *severity, int ny, int nx,
-    mv2_avalanche_config cfg, double area, double f6, LabelAux *aux, double *partials)
+    mv2_avalanche_config cfg, double area, double f6, LabelAux *aux, const unsigned long long *spread,
+    double *partials)
 {
     __shared__ double smem[kWinF * 4];
-    const int32_t winner = aux->diag_i[2];
+    const int32_t winner = resolve_winner(spread, (int64_t)ny * nx, aux);
     double s[5] = {0.0, 0.0, 0.0, 0.0, 0.0};   // sum es, sum et, area, volume, root excess
     double m[2] = {0.0, 0.0};                   // max es, max et
     int prev = 0;
@@ -575,19 +572,17 @@ extern "C" int mv2_avalanche_label(mv2_context *ctx, const mv2_avalanche_config 
     LabelAux *aux = (LabelAux *)ctx->label_aux.ptr;
     double *partials = (double *)((char *)ctx->label_aux.ptr + sizeof(LabelAux));
     const int grid = grid_for(n, kBlock, 4096);
-    hipLaunchKernelGGL(avalanche_init_kernel, dim3(grid), dim3(kBlock), 0, ctx->stream, ctx->z_base, ctx->b_eff,
-                       ctx->mobile, ctx->qx, ctx->qy, f32, ny, nx, ctx->dx, ctx->dy, *cfg, area, f6, *f);
     if (int rc = ensure_scratch(ctx, ctx->spread, kSpreadScratchBytes)) return rc;
     unsigned long long *spread = (unsigned long long *)ctx->spread.ptr + (size_t)kLabelSpreadSet * kSpreadWords;
-    hipLaunchKernelGGL(label_reset_kernel, dim3(grid_for(n, kBlock, 1024)), dim3(kBlock), 0, ctx->stream, f->counts,
-                       n, aux, (int32_t)(n + 1), spread);
+    hipLaunchKernelGGL(avalanche_init_kernel, dim3(grid), dim3(kBlock), 0, ctx->stream, ctx->z_base, ctx->b_eff,
+                       ctx->mobile, ctx->qx, ctx->qy, f32, ny, nx, ctx->dx, ctx->dy, *cfg, area, f6, *f, aux,
+                       (int32_t)(n + 1), spread);
     if (int rc = label_core(ctx, f->labels, ny, nx, cfg->connectivity, f->counts)) return rc;
     const int sgrid = grid_for(n, kBlock, kLabelParts);
     hipLaunchKernelGGL(count_kernel, dim3(sgrid), dim3(kBlock), 0, ctx->stream, f->counts, n, spread);
-    hipLaunchKernelGGL(winner_kernel, dim3(sgrid), dim3(kBlock), 0, ctx->stream, f->counts, n, spread, aux);
     hipLaunchKernelGGL(winner_stats_kernel, dim3(sgrid), dim3(kBlock), 0, ctx->stream, f->labels, ctx->z_base,
                        ctx->b_eff, f32, f->latch, f->previous, f->slope, f->severity, ny, nx, *cfg, area, f6, aux,
-                       partials);
+                       spread, partials);
     hipLaunchKernelGGL(winner_final_kernel, dim3(1), dim3(kBlock), 0, ctx->stream, partials, sgrid, aux);
     MV2_HIP(ctx, hipGetLastError());
     MV2_HIP(ctx, hipMemcpyAsync(ctx->host_result.ptr, aux, sizeof(LabelAux), hipMemcpyDeviceToHost, ctx->stream));
