@@ -362,3 +362,109 @@ def test_surface_sample_reproduces_planes_and_grid_values():
     # halfway between the bump cell and its neighbour: the mean of the two
     assert oracle.surface_sample(plane, m, [[2.0, 3.5]])[0] == pytest.approx(
         0.5 * (plane[2, 3] + 0.25 + plane[2, 4]), rel=1e-15)
+
+
+def _components_by_scipy(mask, connectivity):
+    """Components of a mask with scipy.ndimage.label: (sizes, min linear index) per component."""
+    structure = np.ones((3, 3)) if connectivity == 8 else None
+    lab, nfeat = scipy.ndimage.label(mask, structure=structure)
+    flat = lab.ravel()
+    sizes = np.bincount(flat)[1:]
+    first = np.full(nfeat, flat.size)
+    idx = np.arange(flat.size)
+    np.minimum.at(first, flat[flat > 0] - 1, idx[flat > 0])
+    return sizes, first
+
+
+@pytest.mark.parametrize("connectivity", [4, 8])
+def test_component_statistics_against_scipy_and_closed_forms(connectivity):
+    """reduce_components / select_winner / winner_diagnostics
+    (gpu_large_avalanche.py:201-281): region count, unstable cells and the
+    largest size from scipy's components; the winner is the largest component
+    whose first cell comes first (ties included: two equal largest blocks);
+    on the winner every cell has slope start + 0.1, stop + 0.3, transfer
+    min(resting - z, depth x severity) = 0.2 unless latched, so every sum is a
+    closed form."""
+    ny, nx = 20, 24
+    mask = np.zeros((ny, nx), np.int32)
+    mask[2:6, 3:9] = 1                   # 24 cells, first index 2*24+3
+    mask[12:16, 10:16] = 1               # 24 cells (tie), later first index
+    mask[8, 20] = mask[9, 21] = 1        # diagonal pair: one region under 8-connectivity only
+    mask[18, 1:4] = 1
+    rng = np.random.default_rng(1)
+    start, stop = np.deg2rad(38.0), np.deg2rad(30.0)
+    slope = np.where(mask > 0, start + 0.1, rng.uniform(0.0, 0.3, (ny, nx)))
+    z = np.zeros((ny, nx))
+    resting = np.full((ny, nx), 0.5)
+    severity = np.ones((ny, nx))
+    latch = np.zeros((ny, nx), np.int32)
+    latch[2, 3] = 1                      # the winner's first cell is latched: no transfer there
+    previous = np.zeros((ny, nx), np.int32)
+    previous[3, 4] = previous[13, 11] = 1
+    weights = oracle.vertex_weights(ny, nx, 0.05, 0.05)
+    lab = oracle.label(mask, connectivity)
+    di, df = oracle.components(lab, z, resting, latch, weights, slope, severity, previous, 0.2, start, stop)
+    sizes, first = _components_by_scipy(mask, connectivity)
+    assert di[0] == sizes.size
+    assert di[1] == sizes.max() == 24
+    assert di[3] == int(mask.sum())
+    winners = np.flatnonzero(sizes == sizes.max())
+    assert di[2] == first[winners].min() == 2 * nx + 3
+    win = np.zeros((ny, nx), bool)
+    win[2:6, 3:9] = True
+    assert di[4] == 1                                            # the winner's previous overlap
+    assert df[0] == pytest.approx(24 * 0.1, rel=1e-12) and df[6] == df[0]
+    assert df[1] == pytest.approx(0.1, rel=1e-12)
+    assert df[2] == pytest.approx(24 * (start + 0.1 - stop), rel=1e-12)
+    assert df[3] == pytest.approx(start + 0.1 - stop, rel=1e-12)
+    assert df[4] == pytest.approx(float(weights[win].sum()), rel=1e-14)
+    moved = win.copy()
+    moved[2, 3] = False
+    assert df[5] == pytest.approx(0.2 * float(weights[moved].sum()), rel=1e-12)
+
+
+def test_mobilize_closed_forms():
+    """mobilize (gpu_large_avalanche.py:309-376) on a constructed winner:
+    t = min(max(b - z, 0), depth x severity) moves from b to h on every
+    unlatched winner cell above the dry tolerance, the free surface is kept,
+    the cell latches, its activation count and times advance, and the four
+    weighted sums split first activations from re-activations."""
+    ny, nx = 10, 12
+    labels = np.full((ny, nx), -1, np.int32)
+    labels[2:5, 2:7] = 2 * nx + 2                     # the winner (root = first cell)
+    labels[7:9, 1:4] = 7 * nx + 1                     # another component
+    z = np.zeros((ny, nx))
+    resting = np.full((ny, nx), 0.8)
+    resting[2, 2:7] = 0.05                            # shallow: depth-limited transfer
+    resting[3, 5] = 1e-12                             # below the dry tolerance: no move
+    mobile = np.full((ny, nx), 0.01)
+    severity = np.full((ny, nx), 0.5)
+    latch = np.zeros((ny, nx), np.int32)
+    latch[4, 6] = 1
+    count = np.zeros((ny, nx), np.int32)
+    count[4, 2:6] = 2                                 # re-activations
+    weights = oracle.vertex_weights(ny, nx, 0.05, 0.05)
+    zeros = np.zeros((ny, nx))
+    out = oracle.mobilize(labels, 2 * nx + 2, z, resting, mobile, weights, latch, zeros, zeros, zeros,
+                          severity, count, zeros, zeros, zeros, 7.5, 0.3, 1e-9)
+    expect_t = np.zeros((ny, nx))
+    win = labels == 2 * nx + 2
+    expect_t[win] = np.minimum(resting[win], 0.3 * 0.5)
+    expect_t[4, 6] = 0.0
+    expect_t[3, 5] = 0.0
+    moved = expect_t > 0
+    np.testing.assert_array_equal(out["resting"][moved], resting[moved] - expect_t[moved])
+    np.testing.assert_array_equal(out["mobile"][moved], mobile[moved] + expect_t[moved])
+    np.testing.assert_array_equal(out["resting"][~moved], resting[~moved])
+    np.testing.assert_allclose(out["resting"] + out["mobile"], resting + mobile, rtol=0, atol=2e-16)
+    np.testing.assert_array_equal(out["latch"], np.where(moved, 1, latch))
+    np.testing.assert_array_equal(out["activation_count"], count + moved)
+    first = moved & (count == 0)
+    np.testing.assert_array_equal(out["first_time"], np.where(first, 7.5, 0.0))
+    np.testing.assert_array_equal(out["last_time"], np.where(moved, 7.5, 0.0))
+    d = out["diag"]
+    assert d[7] == pytest.approx(float(np.sum(expect_t * weights)), rel=1e-13)
+    assert d[10] == pytest.approx(float(np.sum((expect_t * weights)[first])), rel=1e-13)
+    assert d[11] == pytest.approx(float(np.sum((expect_t * weights)[moved & ~first])), rel=1e-13)
+    assert d[12] == pytest.approx(float(weights[first].sum()), rel=1e-14)
+    assert d[13] == pytest.approx(float(weights[moved & ~first].sum()), rel=1e-14)
