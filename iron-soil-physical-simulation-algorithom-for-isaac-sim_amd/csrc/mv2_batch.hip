@@ -301,16 +301,15 @@ __global__ void __launch_bounds__(kBlock) batch_substep_kernel(BatchArgs a, int 
     long long hn_min = 0;
     int ovf = 0;
     strip_update<SH>(v, rows, src, r0, c0, sub_dt, dg, wv, hn_min, ovf);
-    double wmax[1] = {(double)wv};
-    block_reduce<double, 1, 1>(wmax, smem);
-    double hm[1] = {(double)hn_min * kQuantum};
-    block_reduce<double, 1, 2>(hm, smem);
-    const int anyovf = __syncthreads_or(ovf);
+    // one field-parallel fold: wave-speed maximum, overflow flag, minimum depth
+    __shared__ double fold_lds[3 * kBlock];
+    double fv[3] = {(double)wv, ovf ? 1.0 : 0.0, (double)hn_min * kQuantum};
+    block_fold<double, 3, 0x3u, 0x4u>(fv, fold_lds);
     if (threadIdx.x == 0) {
-        inst_max(inst_set(a.wave, (k + 1) % kWaveSetsB, a.count, i), wmax[0]);
+        inst_max(inst_set(a.wave, (k + 1) % kWaveSetsB, a.count, i), fv[0]);
         int f = 0;
-        if (hm[0] < -1.0e-300) f |= 1;
-        if (anyovf) f |= 2;
+        if (fv[2] < -1.0e-300) f |= 1;
+        if (fv[1] != 0.0) f |= 2;
         if (f) atomicOr(&a.flags[((k + 1) & 1) * a.count + i], f);
     }
 }
