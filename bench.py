@@ -260,6 +260,7 @@ def run_digging(args) -> int:
 
     cells_total = int(batch.cells)
     relax_it, relax_max, cell_sub, dug = 0, 0, 0, 0.0
+    frame_max = []                     # per frame: the most iterations any terrain needed
     v2_s = 0.0
     t_parts = {"entrain": 0.0, "v2": 0.0, "relax": 0.0, "label": 0.0}
     for f in range(frames):
@@ -292,6 +293,7 @@ def run_digging(args) -> int:
                                    recs.substeps.astype(np.int64)))
             relax_it += int(rr["iterations"].sum())
             relax_max = max(relax_max, int(rr["iterations"].max()))
+            frame_max.append(int(rr["iterations"].max()))
             for k, v in zip(t_parts, (tb - ta, tc - tb, td - tc, te - td)):
                 t_parts[k] += v
             v2_s += tc - tb
@@ -310,6 +312,9 @@ def run_digging(args) -> int:
         "ms_per_frame_parts": {k: v * 1e3 / max(args.steps, 1) for k, v in t_parts.items()},
         "relax_iterations_per_terrain_frame": relax_it / max(args.steps * args.dig_count, 1),
         "relax_iterations_max": relax_max,
+        "relax_iterations_frame_max": {"mean": float(np.mean(frame_max)) if frame_max else None,
+                                       "p50": float(np.percentile(frame_max, 50)) if frame_max else None,
+                                       "p90": float(np.percentile(frame_max, 90)) if frame_max else None},
         "relaxation": args.dig_relax,
         "envelope_launches_per_frame": env_launches / max(args.steps, 1),
         "setup_relax_to_rest_iterations_max": int(rest["iterations"].max()),
