@@ -4,10 +4,11 @@
 // plus the kernels that train it and the batched patch relaxation that makes
 // its training pairs.
 //
-// * Layer 1 (3 input channels) runs in FP32 on the VALU: terrain heights of a
-//   few metres need FP32 to resolve millimetre differences, and its taps on
-//   the b_eff channel are constrained to sum to zero, so the network's output
-//   is exactly invariant to adding a constant to the terrain.
+// * Layer 1 (3 input channels) runs in FP32 on v_mfma_f32_32x32x2_f32:
+//   terrain heights of a few metres need FP32 to resolve millimetre
+//   differences, and its taps on the b_eff channel are constrained to sum to
+//   zero, so the network's output is invariant (in exact arithmetic) to
+//   adding a constant to the terrain.
 // * Layers 2-5 run as implicit GEMMs on the matrix cores:
 //   v_mfma_f32_32x32x16_bf16, BF16 activations/weights, FP32 accumulation.
 //   One wave computes 32 consecutive pixels of one row x 32 output channels;
@@ -58,40 +59,87 @@ __global__ void __launch_bounds__(kBlock) features_kernel(const float *b, const 
     }
 }
 
-// layer 1: FP32 3x3 conv over 3 feature channels, bias, ReLU -> BF16 NHWC32
-// w1: [9][32][4] FP32 (channel 3 unused)
-__global__ void __launch_bounds__(kBlock) conv1_fp32_kernel(const float *feat, const float *w1, const float *b1,
+// ---------------------------------------------------------------------------
+// Tiling shared by the conv layers: one wave computes 32 output channels x 32
+// consecutive pixels of one row per tile (D = W x X: the weights are the MFMA
+// A operand, the pixels the B operand), so each lane's accumulators hold one
+// pixel and the channels 8i + 4*half + j (i, j < 4) -- four runs of four
+// contiguous channels, stored as four 8-byte writes.  A wave keeps its weights
+// in registers and walks kConvRowsPerWave rows; a workgroup (4 waves) covers
+// kConvRows rows x 32 pixels of one field.
+// Grid: (ceil(W/32), ceil(H/kConvRows), B).
+// ---------------------------------------------------------------------------
+constexpr int kConvRowsPerWave = 8;
+constexpr int kConvRows = 4 * kConvRowsPerWave;
+
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
+
+// layer 1: FP32 3x3 conv over 3 feature channels, bias, ReLU -> BF16 NHWC32,
+// on v_mfma_f32_32x32x2_f32 (K = 9 taps x 3 channels, padded to 28).
+// w1: [9][32][4] FP32 (channel 3 unused).  Zero padding of the b_eff channel
+// would break the constant-shift invariance at the domain edge: outside the
+// grid it takes the centre value, the other channels 0.
+__global__ void __launch_bounds__(kBlock) conv1_mfma_kernel(const float *feat, const float *w1, const float *b1,
                                                             int B, int H, int W, int dil, __bf16 *y)
 {
-    __shared__ float ws[9 * 32 * 4];
-    __shared__ float bs[32];
-    for (int i = threadIdx.x; i < 9 * 32 * 4; i += blockDim.x) ws[i] = w1[i];
-    if (threadIdx.x < 32) bs[threadIdx.x] = b1[threadIdx.x];
-    __syncthreads();
-    const int64_t total = (int64_t)B * H * W;
-    for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < total;
-         p += (int64_t)gridDim.x * blockDim.x) {
-        const int64_t n = p / ((int64_t)H * W);
-        const int rem = (int)(p - n * H * W);
-        const int yc = rem / W, xc = rem - yc * W;
-        float f[9][3];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int m = lane & 31, half = lane >> 5;
+    const int x = blockIdx.x * 32 + m;
+    const int n = blockIdx.z;
+    const int ybeg = blockIdx.y * kConvRows + wave * kConvRowsPerWave;
+    if (ybeg >= H) return;
+    const int yend = min(ybeg + kConvRowsPerWave, H);
+    // A operand of step j: w[t][co = m][c] with t*3 + c = 2j + half
+    float wa[14];
+#pragma unroll
+    for (int j = 0; j < 14; j++) {
+        const int kk = 2 * j + half;
+        wa[j] = kk < 27 ? w1[((kk / 3) * 32 + m) * 4 + kk % 3] : 0.0f;
+    }
+    float bias[16];
+#pragma unroll
+    for (int r = 0; r < 16; r++) bias[r] = b1[(r / 4) * 8 + half * 4 + (r % 4)];
+    const f32x4 *f4 = reinterpret_cast<const f32x4 *>(feat);
+    const int64_t plane = (int64_t)H * W;
+    for (int yr = ybeg; yr < yend; yr++) {
+        f32x4 f[9];
+        const f32x4 zero = {0.0f, 0.0f, 0.0f, 0.0f};
+        const f32x4 centre = x < W ? f4[n * plane + (int64_t)yr * W + x] : zero;
+#pragma unroll
         for (int t = 0; t < 9; t++) {
-            const int yy = yc + (t / 3 - 1) * dil, xx = xc + (t % 3 - 1) * dil;
-            const bool in = yy >= 0 && yy < H && xx >= 0 && xx < W;
-            const int64_t q = n * H * W + (int64_t)yy * W + xx;
-            for (int c = 0; c < 3; c++) f[t][c] = in ? feat[q * 4 + c] : 0.0f;
+            const int yy = yr + (t / 3 - 1) * dil, xx = x + (t % 3 - 1) * dil;
+            const bool in = x < W && yy >= 0 && yy < H && xx >= 0 && xx < W;
+            if (t == 4) {
+                f[t] = centre;
+            } else if (in) {
+                f[t] = f4[n * plane + (int64_t)yy * W + xx];
+            } else {
+                f[t] = zero;
+                f[t][0] = centre[0];
+            }
         }
-        // zero padding of the b_eff channel would break the constant-shift
-        // invariance at the domain edge: replicate the centre value instead
-        for (int t = 0; t < 9; t++) {
-            const int yy = yc + (t / 3 - 1) * dil, xx = xc + (t % 3 - 1) * dil;
-            if (!(yy >= 0 && yy < H && xx >= 0 && xx < W)) f[t][0] = feat[p * 4 + 0];
+        f32x16 acc;
+#pragma unroll
+        for (int r = 0; r < 16; r++) acc[r] = 0.0f;
+#pragma unroll
+        for (int j = 0; j < 14; j++) {
+            const int ke = 2 * j, ko = 2 * j + 1;
+            const float ve = f[ke / 3][ke % 3];
+            const float vo = ko < 27 ? f[ko / 3][ko % 3] : 0.0f;
+            acc = __builtin_amdgcn_mfma_f32_32x32x2f32(wa[j], half ? vo : ve, acc, 0, 0, 0);
         }
-        for (int co = 0; co < 32; co++) {
-            float acc = bs[co];
-            for (int t = 0; t < 9; t++)
-                for (int c = 0; c < 3; c++) acc += ws[(t * 32 + co) * 4 + c] * f[t][c];
-            y[p * 32 + co] = to_bf16(acc > 0.0f ? acc : 0.0f);
+        if (x >= W) continue;
+        __bf16 *yp = y + (n * plane + (int64_t)yr * W + x) * 32 + half * 4;
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            bf16x4 o;
+#pragma unroll
+            for (int j = 0; j < 4; j++) {
+                const float v = acc[4 * i + j] + bias[4 * i + j];
+                o[j] = to_bf16(v > 0.0f ? v : 0.0f);
+            }
+            *reinterpret_cast<bf16x4 *>(yp + 8 * i) = o;
         }
     }
 }
@@ -102,7 +150,8 @@ __global__ void __launch_bounds__(kBlock) conv1_fp32_kernel(const float *feat, c
 // mode 0: BF16 NHWC32 output (+bias, optional ReLU, optional multiply by
 //         (mask > 0) where mask is BF16 NHWC32)
 // mode 1: FP32 single-channel output = channel 0 (+bias[0])
-// Grid: (ceil(W/32), ceil(H/4), B); 4 waves, one row each.
+// v_mfma_f32_32x32x16_bf16, K = 9 taps x CX, 16 per step; the wave's 9*KS
+// weight operands stay in registers for all its rows.
 // ---------------------------------------------------------------------------
 struct ConvArgs {
     const __bf16 *x;
@@ -119,48 +168,63 @@ __global__ void __launch_bounds__(kBlock) conv_mfma_kernel(ConvArgs a)
 {
     constexpr int CX = 16 * KS;
     constexpr int VPR = CX / 8;                    // bf16x8 vectors per (tap, co) row
-    __shared__ bf16x8 wl[9 * 32 * VPR];
-    const bf16x8 *wg = reinterpret_cast<const bf16x8 *>(a.w);
-    for (int i = threadIdx.x; i < 9 * 32 * VPR; i += blockDim.x) wl[i] = wg[i];
-    __syncthreads();
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int x0 = blockIdx.x * 32;
-    const int yrow = blockIdx.y * 4 + wave;
-    const int n = blockIdx.z;
-    if (yrow >= a.H) return;
     const int m = lane & 31, half = lane >> 5;
-    const int x = x0 + m;
-    f32x16 acc;
-    for (int r = 0; r < 16; r++) acc[r] = 0.0f;
+    const int x = blockIdx.x * 32 + m;
+    const int n = blockIdx.z;
+    const int ybeg = blockIdx.y * kConvRows + wave * kConvRowsPerWave;
+    if (ybeg >= a.H) return;
+    const int yend = min(ybeg + kConvRowsPerWave, a.H);
+    const bf16x8 *wg = reinterpret_cast<const bf16x8 *>(a.w);
+    bf16x8 wr[9][KS];
+#pragma unroll
+    for (int t = 0; t < 9; t++)
+#pragma unroll
+        for (int ks = 0; ks < KS; ks++) wr[t][ks] = wg[(t * 32 + m) * VPR + ks * 2 + half];
+    float bias[16];
+#pragma unroll
+    for (int r = 0; r < 16; r++) bias[r] = a.bias ? a.bias[(r / 4) * 8 + half * 4 + (r % 4)] : 0.0f;
+    const int64_t plane = (int64_t)a.H * a.W;
     const bf16x8 zero = {};
-    for (int tap = 0; tap < 9; tap++) {
-        const int yy = yrow + (tap / 3 - 1) * a.dil, xx = x + (tap % 3 - 1) * a.dil;
-        const bool in = x < a.W && yy >= 0 && yy < a.H && xx >= 0 && xx < a.W;
-        const bf16x8 *xp = reinterpret_cast<const bf16x8 *>(
-            a.x + (((int64_t)n * a.H + (in ? yy : 0)) * a.W + (in ? xx : 0)) * CX);
+    for (int yr = ybeg; yr < yend; yr++) {
+        f32x16 acc;
 #pragma unroll
-        for (int ks = 0; ks < KS; ks++) {
-            const bf16x8 av = in ? xp[ks * 2 + half] : zero;
-            const bf16x8 bv = wl[(tap * 32 + m) * VPR + ks * 2 + half];
-            acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(av, bv, acc, 0, 0, 0);
+        for (int r = 0; r < 16; r++) acc[r] = 0.0f;
+#pragma unroll
+        for (int t = 0; t < 9; t++) {
+            const int yy = yr + (t / 3 - 1) * a.dil, xx = x + (t % 3 - 1) * a.dil;
+            const bool in = x < a.W && yy >= 0 && yy < a.H && xx >= 0 && xx < a.W;
+            const bf16x8 *xp = reinterpret_cast<const bf16x8 *>(a.x + (n * plane + (int64_t)(in ? yy : 0) * a.W +
+                                                                       (in ? xx : 0)) * CX);
+#pragma unroll
+            for (int ks = 0; ks < KS; ks++) {
+                const bf16x8 xv = in ? xp[ks * 2 + half] : zero;
+                acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(wr[t][ks], xv, acc, 0, 0, 0);
+            }
         }
-    }
-    const int co = lane & 31;
-    const float bias = a.bias ? a.bias[co] : 0.0f;
-#pragma unroll
-    for (int r = 0; r < 16; r++) {
-        const int mm = (r / 4) * 8 + half * 4 + (r % 4);
-        const int px = x0 + mm;
-        if (px >= a.W) continue;
-        const int64_t p = ((int64_t)n * a.H + yrow) * a.W + px;
-        float v = acc[r] + bias;
+        if (x >= a.W) continue;
+        const int64_t p = n * plane + (int64_t)yr * a.W + x;
         if (a.mode == 1) {
-            if (co == 0) reinterpret_cast<float *>(a.y)[p] = v;
+            // channel 0 is accumulator 0 of the lanes with half == 0
+            if (half == 0) reinterpret_cast<float *>(a.y)[p] = acc[0] + bias[0];
             continue;
         }
-        if (a.relu) v = v > 0.0f ? v : 0.0f;
-        if (a.mask && !(from_bf16(a.mask[p * 32 + co]) > 0.0f)) v = 0.0f;
-        reinterpret_cast<__bf16 *>(a.y)[p * 32 + co] = to_bf16(v);
+        __bf16 *yp = reinterpret_cast<__bf16 *>(a.y) + p * 32 + half * 4;
+        const __bf16 *mp = a.mask ? a.mask + p * 32 + half * 4 : nullptr;
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            bf16x4 mk = {};
+            if (mp) mk = *reinterpret_cast<const bf16x4 *>(mp + 8 * i);
+            bf16x4 o;
+#pragma unroll
+            for (int j = 0; j < 4; j++) {
+                float v = acc[4 * i + j] + bias[4 * i + j];
+                if (a.relu) v = v > 0.0f ? v : 0.0f;
+                if (mp && !(from_bf16(mk[j]) > 0.0f)) v = 0.0f;
+                o[j] = to_bf16(v);
+            }
+            *reinterpret_cast<bf16x4 *>(yp + 8 * i) = o;
+        }
     }
 }
 
@@ -231,7 +295,7 @@ __global__ void __launch_bounds__(kBlock) weight_grad_kernel(const void *dz, int
                     if (xmode == 0) xv = from_bf16(reinterpret_cast<const __bf16 *>(xin)[q * 32 + c]);
                     else xv = c < 3 ? reinterpret_cast<const float *>(xin)[q * 4 + c] : 0.0f;
                 } else if (xmode == 1 && c == 0) {
-                    // replicate-padding of the b_eff channel (conv1_fp32_kernel)
+                    // replicate-padding of the b_eff channel (conv1_mfma_kernel)
                     const int64_t q = n * H * W + (int64_t)(rem / W) * W + rem % W;
                     xv = reinterpret_cast<const float *>(xin)[q * 4];
                 }
@@ -593,19 +657,31 @@ __global__ void __launch_bounds__(kBlock) apply_delta_kernel(float *b, const flo
 // every edge of the four legacy directions obey |z_a - z_b| <= crit[d], i.e.
 // z(p) = min over q of (z(q) + shortest edge-weighted path length q -> p).
 // It only lowers cells, by exactly the amount their slopes exceed the limit,
-// so after it the exact relaxation finds no excess.  One launch per batch of
-// kEnvSweeps in-place sweeps over a tile plus a kEnvSweeps-cell halo in LDS:
-// values only ever decrease and always stay above the envelope, so
-// concurrent updates are benign, and after s sweeps a cell is at least as low
-// as after s Jacobi steps, which the halo makes exact for the interior.  The
-// next launch stops the loop when no interior cell of any tile changed.  The
-// fixed point is the minimum over paths of the path sums evaluated from the
-// source outwards; float rounding is monotone, so every update order reaches
-// the same bits.
+// so after it the exact relaxation finds no excess.
+//
+// Every update z(p) <- min(z(p), z(q) + crit) keeps z a valid upper bound of
+// the envelope, and the fixed point is the minimum over paths of the path sums
+// evaluated from the source outwards (float rounding is monotone), so any
+// update order that reaches a fixed point gives the same bits.  The kernel
+// uses that freedom for directional Gauss-Seidel passes: one wave owns a
+// 64 x 64 region (a kEnvTile^2 tile plus a kEnvHalo ring) in LDS; a down pass
+// walks the rows top to bottom with lane = column, carrying the updated row
+// above in registers and taking the neighbouring columns by DPP, so one pass
+// moves information along any downward path of the region; up, right (lane =
+// row) and left passes follow.  An optimal path between two cells uses at most
+// two adjacent move directions, and every such pair lies inside one pass's
+// three moves, so one cycle of the four passes settles a region up to
+// rounding; cycles repeat while anything in the region moves (at most
+// kEnvMaxCycles).  Tiles exchange through HBM between launches; the next
+// launch stops the loop when no interior cell of any tile changed (a halo of
+// one cell already makes that a global fixed point).
 // ---------------------------------------------------------------------------
-constexpr int kEnvSweeps = 8;
-constexpr int kEnvTH = 32, kEnvTW = 64;
-constexpr int kEnvLH = kEnvTH + 2 * kEnvSweeps, kEnvLW = kEnvTW + 2 * kEnvSweeps;
+constexpr int kEnvRegion = 64;                          // region side = wave width
+constexpr int kEnvHalo = 2;
+constexpr int kEnvTile = kEnvRegion - 2 * kEnvHalo;     // interior cells per side
+constexpr int kEnvStride = kEnvRegion + 1;              // LDS row stride: column walks are conflict-free
+constexpr int kEnvWaves = 2;                            // regions per workgroup
+constexpr int kEnvMaxCycles = 6;
 
 struct EnvControl {
     int launches, done, pad[2];
@@ -617,11 +693,96 @@ __global__ void add_surface_kernel(const float *b, const float *delta, float *z,
         z[p] = b[p] + delta[p];
 }
 
-__global__ void __launch_bounds__(kBlock) envelope_kernel(float *z, int ny, int nx, float c0, float c1, float c2,
-                                                          float c3, int tiles_x, EnvControl *ctl,
-                                                          unsigned long long *spread, int k, int seam)
+__device__ inline float env_from_prev_lane(float v)     // lane l gets lane l - 1 (lane 0: +inf)
 {
-    __shared__ float t[kEnvLH * kEnvLW];
+    return __int_as_float(__builtin_amdgcn_update_dpp(0x7f800000, __float_as_int(v), 0x138, 0xf, 0xf, false));
+}
+__device__ inline float env_from_next_lane(float v)     // lane l gets lane l + 1 (lane 63: +inf)
+{
+    return __int_as_float(__builtin_amdgcn_update_dpp(0x7f800000, __float_as_int(v), 0x130, 0xf, 0xf, false));
+}
+
+// one directional pass over the wave's region.  HORZ: lane = region row, the
+// walk goes over columns (FWD: left to right); otherwise lane = region column
+// and the walk goes over rows (FWD: top to bottom).  valid_* are bit masks of
+// the region rows/columns inside the grid; cut_above / cut_below mark region
+// rows whose edge to the row above / below crosses a seam between stacked
+// fields.  Returns whether this lane lowered a cell; *moved_interior is set
+// when that cell is in the tile's interior.
+template <bool HORZ, bool FWD>
+__device__ inline bool env_pass(float *t, int lane, uint64_t valid_rows, uint64_t valid_cols, uint64_t cut_above,
+                                uint64_t cut_below, float c0, float c1, float c2, float c3, bool *moved_interior)
+{
+    constexpr float INF = __builtin_huge_valf();
+    auto at = [&](int s) -> float * { return HORZ ? &t[lane * kEnvStride + s] : &t[s * kEnvStride + lane]; };
+    const bool lane_valid = ((HORZ ? valid_rows : valid_cols) >> lane) & 1;
+    const bool lane_inner = lane >= kEnvHalo && lane < kEnvHalo + kEnvTile;
+    // seams cut the edges to the lanes above / below (HORZ: lane = row)
+    const bool lane_cut_above = HORZ && ((cut_above >> lane) & 1);
+    const bool lane_cut_below = HORZ && ((cut_below >> lane) & 1);
+    bool changed = false;
+    float prev = INF;                                   // updated value one step back
+    float cur = *at(FWD ? 0 : kEnvRegion - 1);
+    for (int k = 0; k < kEnvRegion; k++) {
+        const int s = FWD ? k : kEnvRegion - 1 - k;
+        const float next = k + 1 < kEnvRegion ? *at(FWD ? s + 1 : s - 1) : INF;
+        const float before = FWD ? prev : next;         // step s - 1 (row above / column left)
+        const float after = FWD ? next : prev;          // step s + 1 (row below / column right)
+        const float cur_p = env_from_prev_lane(cur), cur_n = env_from_next_lane(cur);
+        const float bef_p = env_from_prev_lane(before), bef_n = env_from_next_lane(before);
+        const float aft_p = env_from_prev_lane(after), aft_n = env_from_next_lane(after);
+        float m = cur;
+        if (HORZ) {
+            // lane = row r, step = column c: before = (r, c-1), after = (r, c+1)
+            m = fminf(m, before + c0);
+            m = fminf(m, after + c0);
+            if (!lane_cut_above) {
+                m = fminf(m, cur_p + c1);               // (r-1, c)
+                m = fminf(m, bef_p + c2);               // (r-1, c-1)
+                m = fminf(m, aft_p + c3);               // (r-1, c+1)
+            }
+            if (!lane_cut_below) {
+                m = fminf(m, cur_n + c1);               // (r+1, c)
+                m = fminf(m, bef_n + c3);               // (r+1, c-1)
+                m = fminf(m, aft_n + c2);               // (r+1, c+1)
+            }
+        } else {
+            // lane = column c, step = row r: before = (r-1, c), after = (r+1, c)
+            m = fminf(m, cur_p + c0);                   // (r, c-1)
+            m = fminf(m, cur_n + c0);                   // (r, c+1)
+            if (!((cut_above >> s) & 1)) {
+                m = fminf(m, before + c1);
+                m = fminf(m, bef_p + c2);               // (r-1, c-1)
+                m = fminf(m, bef_n + c3);               // (r-1, c+1)
+            }
+            if (!((cut_below >> s) & 1)) {
+                m = fminf(m, after + c1);
+                m = fminf(m, aft_p + c3);               // (r+1, c-1)
+                m = fminf(m, aft_n + c2);               // (r+1, c+1)
+            }
+        }
+        const bool step_valid = ((HORZ ? valid_cols : valid_rows) >> s) & 1;
+        if (lane_valid && step_valid && m < cur) {
+            *at(s) = m;
+            changed = true;
+            if (lane_inner && s >= kEnvHalo && s < kEnvHalo + kEnvTile) *moved_interior = true;
+            cur = m;
+        }
+        prev = cur;
+        cur = next;
+    }
+    // the next pass reads what other lanes wrote
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    return changed;
+}
+
+__global__ void __launch_bounds__(kEnvWaves * kWave) envelope_kernel(float *z, int ny, int nx, float c0, float c1,
+                                                                      float c2, float c3, int tiles_x, int tiles,
+                                                                      EnvControl *ctl, unsigned long long *spread,
+                                                                      int k, int seam)
+{
+    __shared__ float lds[kEnvWaves][kEnvRegion * kEnvStride];
     __shared__ double smem[1];
     __shared__ int changed;
     // launch k - 1 recorded whether any interior cell changed in spread set (k-1) % 3
@@ -640,53 +801,40 @@ __global__ void __launch_bounds__(kBlock) envelope_kernel(float *z, int ny, int 
         nc.launches = C.launches + 1;
         ctl[(k + 1) & 1] = nc;
     }
-    const int r0 = (blockIdx.x / tiles_x) * kEnvTH, c0i = (blockIdx.x % tiles_x) * kEnvTW;
-    const int R0 = r0 - kEnvSweeps, C0 = c0i - kEnvSweeps;
     if (threadIdx.x == 0) changed = 0;
-    for (int e = threadIdx.x; e < kEnvLH * kEnvLW; e += blockDim.x) {
-        const int gr = R0 + e / kEnvLW, gc = C0 + e % kEnvLW;
-        t[e] = (gr >= 0 && gr < ny && gc >= 0 && gc < nx) ? z[gr * nx + gc] : __builtin_huge_valf();
-    }
     __syncthreads();
-    // neighbours (dr, dc) and their limits: both ends of d0 (0, +-1), d1 (+-1, 0),
-    // d2 (+-1, +-1 same sign), d3 (-+1, +-1 opposite sign)
-    const int nr[8] = {0, 0, 1, -1, 1, -1, -1, 1};
-    const int ncol[8] = {1, -1, 0, 0, 1, -1, 1, -1};
-    const float lim[8] = {c0, c0, c1, c1, c2, c2, c3, c3};
-    int moved = 0;
-    for (int sweep = 0; sweep < kEnvSweeps; sweep++) {
-        for (int e = threadIdx.x; e < kEnvLH * kEnvLW; e += blockDim.x) {
-            const int lr = e / kEnvLW, lc = e % kEnvLW;
-            // other threads lower their cells concurrently: word-sized relaxed
-            // LDS accesses, any value read is a valid upper bound
-            const float v = __hip_atomic_load(&t[e], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-            if (v == __builtin_huge_valf()) continue;        // outside the grid
-            // stacked independent fields: no edge crosses a seam row boundary
-            const int grow = R0 + lr;
-            const bool seam_below = seam > 0 && (grow + 1) % seam == 0;
-            const bool seam_above = seam > 0 && grow % seam == 0;
-            float m = v;
-#pragma unroll
-            for (int q = 0; q < 8; q++) {
-                const int rr = lr + nr[q], cc = lc + ncol[q];
-                if (rr < 0 || rr >= kEnvLH || cc < 0 || cc >= kEnvLW) continue;
-                if ((nr[q] > 0 && seam_below) || (nr[q] < 0 && seam_above)) continue;
-                const float cand = __hip_atomic_load(&t[rr * kEnvLW + cc], __ATOMIC_RELAXED,
-                                                     __HIP_MEMORY_SCOPE_WORKGROUP) + lim[q];
-                m = cand < m ? cand : m;
-            }
-            if (m < v) {
-                __hip_atomic_store(&t[e], m, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-                if (lr >= kEnvSweeps && lr < kEnvSweeps + kEnvTH && lc >= kEnvSweeps && lc < kEnvSweeps + kEnvTW)
-                    moved = 1;
-            }
+    const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
+    const int tile = blockIdx.x * kEnvWaves + wave;
+    bool moved = false;
+    if (tile < tiles) {
+        float *t = lds[wave];
+        const int R0 = (tile / tiles_x) * kEnvTile - kEnvHalo, C0 = (tile % tiles_x) * kEnvTile - kEnvHalo;
+        // this lane as a region row and as a region column
+        const int grow = R0 + lane, gcol = C0 + lane;
+        const bool row_in = grow >= 0 && grow < ny, col_in = gcol >= 0 && gcol < nx;
+        const uint64_t valid_rows = __ballot(row_in), valid_cols = __ballot(col_in);
+        const uint64_t cut_above = __ballot(row_in && seam > 0 && grow % seam == 0);
+        const uint64_t cut_below = __ballot(row_in && seam > 0 && (grow + 1) % seam == 0);
+        for (int r = 0; r < kEnvRegion; r++)
+            t[r * kEnvStride + lane] = ((valid_rows >> r) & 1) && col_in ? z[(int64_t)(R0 + r) * nx + gcol]
+                                                                         : __builtin_huge_valf();
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        for (int cycle = 0; cycle < kEnvMaxCycles; cycle++) {
+            bool ch = env_pass<false, true>(t, lane, valid_rows, valid_cols, cut_above, cut_below, c0, c1, c2, c3,
+                                            &moved);
+            ch |= env_pass<false, false>(t, lane, valid_rows, valid_cols, cut_above, cut_below, c0, c1, c2, c3,
+                                         &moved);
+            ch |= env_pass<true, true>(t, lane, valid_rows, valid_cols, cut_above, cut_below, c0, c1, c2, c3, &moved);
+            ch |= env_pass<true, false>(t, lane, valid_rows, valid_cols, cut_above, cut_below, c0, c1, c2, c3,
+                                        &moved);
+            if (!__any(ch)) break;
         }
-        __syncthreads();
-    }
-    for (int e = threadIdx.x; e < kEnvTH * kEnvTW; e += blockDim.x) {
-        const int ir = e / kEnvTW, ic = e % kEnvTW;
-        const int gr = r0 + ir, gc = c0i + ic;
-        if (gr < ny && gc < nx) z[gr * nx + gc] = t[(ir + kEnvSweeps) * kEnvLW + ic + kEnvSweeps];
+        if (__any(moved)) {
+            for (int r = kEnvHalo; r < kEnvHalo + kEnvTile; r++)
+                if (((valid_rows >> r) & 1) && col_in && lane >= kEnvHalo && lane < kEnvHalo + kEnvTile)
+                    z[(int64_t)(R0 + r) * nx + gcol] = t[r * kEnvStride + lane];
+        }
     }
     if (moved) changed = 1;     // benign race: every writer stores 1
     __syncthreads();
@@ -736,9 +884,11 @@ extern "C" int mv2_nn_conv1(mv2_context *ctx, const float *feat, const float *w1
                             int W, int dil, void *y)
 {
     if (int rc = check_plain(ctx)) return rc;
-    const int64_t n = (int64_t)B * H * W;
-    hipLaunchKernelGGL(conv1_fp32_kernel, dim3(grid_for(n, kBlock, 8192)), dim3(kBlock), 0, ctx->stream, feat, w1,
-                       b1, B, H, W, dil, (__bf16 *)y);
+    if (B < 1 || H < 1 || W < 1 || B > 65535 || (H + kConvRows - 1) / kConvRows > 65535)
+        return set_error(ctx, MV2_EINVAL, "[Surrogate] invalid activation shape");
+    const dim3 grid((W + 31) / 32, (H + kConvRows - 1) / kConvRows, B);
+    hipLaunchKernelGGL(conv1_mfma_kernel, grid, dim3(kBlock), 0, ctx->stream, feat, w1, b1, B, H, W, dil,
+                       (__bf16 *)y);
     MV2_HIP(ctx, hipGetLastError());
     return MV2_OK;
 }
@@ -748,10 +898,10 @@ extern "C" int mv2_nn_conv_mfma(mv2_context *ctx, const void *x, int cx, const v
 {
     if (int rc = check_plain(ctx)) return rc;
     if (cx != 16 && cx != 32) return set_error(ctx, MV2_EINVAL, "[Surrogate] input channels must be 16 or 32");
-    if (B < 1 || H < 1 || W < 1 || B > 65535 || (H + 3) / 4 > 65535)
+    if (B < 1 || H < 1 || W < 1 || B > 65535 || (H + kConvRows - 1) / kConvRows > 65535)
         return set_error(ctx, MV2_EINVAL, "[Surrogate] invalid activation shape");
     ConvArgs a{(const __bf16 *)x, (const __bf16 *)w, bias, (const __bf16 *)mask, y, B, H, W, dil, relu, mode};
-    dim3 grid((W + 31) / 32, (H + 3) / 4, B);
+    const dim3 grid((W + 31) / 32, (H + kConvRows - 1) / kConvRows, B);
     if (cx == 16) hipLaunchKernelGGL(conv_mfma_kernel<1>, grid, dim3(kBlock), 0, ctx->stream, a);
     else hipLaunchKernelGGL(conv_mfma_kernel<2>, grid, dim3(kBlock), 0, ctx->stream, a);
     MV2_HIP(ctx, hipGetLastError());
@@ -961,16 +1111,17 @@ static int envelope_run(mv2_context *ctx, const float *b, const float *delta, fl
     int *dres = (int *)((char *)ctx->result.ptr + 3800);
     if (delta) hipLaunchKernelGGL(add_surface_kernel, dim3(grid_for(n, kBlock, 2048)), dim3(kBlock), 0, st, b, delta, z, n);
     hipLaunchKernelGGL(envelope_reset_kernel, dim3(1), dim3(kBlock), 0, st, ctl, spread);
-    const int tiles_x = (nx + kEnvTW - 1) / kEnvTW;
-    const int tiles = tiles_x * ((ny + kEnvTH - 1) / kEnvTH);
+    const int tiles_x = (nx + kEnvTile - 1) / kEnvTile;
+    const int tiles = tiles_x * ((ny + kEnvTile - 1) / kEnvTile);
+    const int blocks = (tiles + kEnvWaves - 1) / kEnvWaves;
     int k = 0, chunk = 4;
     const int *hres = (const int *)ctx->host_result.ptr;
     for (;;) {
         if (chunk > max_launches - k) chunk = max_launches - k;
         for (int j = 0; j < chunk; j++, k++)
-            hipLaunchKernelGGL(envelope_kernel, dim3(tiles), dim3(kBlock), 0, st, z, ny, nx, (float)crit_diff[0],
-                               (float)crit_diff[1], (float)crit_diff[2], (float)crit_diff[3], tiles_x, ctl, spread, k,
-                               seam);
+            hipLaunchKernelGGL(envelope_kernel, dim3(blocks), dim3(kEnvWaves * kWave), 0, st, z, ny, nx,
+                               (float)crit_diff[0], (float)crit_diff[1], (float)crit_diff[2], (float)crit_diff[3],
+                               tiles_x, tiles, ctl, spread, k, seam);
         hipLaunchKernelGGL(envelope_result_kernel, dim3(1), dim3(kWave), 0, st, ctl, spread, k, dres);
         MV2_HIP(ctx, hipGetLastError());
         MV2_HIP(ctx, hipMemcpyAsync(ctx->host_result.ptr, dres, 2 * sizeof(int), hipMemcpyDeviceToHost, st));
