@@ -64,9 +64,9 @@ __global__ void __launch_bounds__(kBlock) features_kernel(const float *b, const 
 // consecutive pixels of one row per tile (D = W x X: the weights are the MFMA
 // A operand, the pixels the B operand), so each lane's accumulators hold one
 // pixel and the channels 8i + 4*half + j (i, j < 4) -- four runs of four
-// contiguous channels, stored as four 8-byte writes.  A wave keeps its weights
-// in registers and walks kConvRowsPerWave rows; a workgroup (4 waves) covers
-// kConvRows rows x 32 pixels of one field.
+// contiguous channels, stored as four 8-byte writes.  A wave walks
+// kConvRowsPerWave rows; a workgroup (4 waves) covers kConvRows rows x 32
+// pixels of one field.
 // Grid: (ceil(W/32), ceil(H/kConvRows), B).
 // ---------------------------------------------------------------------------
 constexpr int kConvRowsPerWave = 8;
@@ -102,22 +102,23 @@ __global__ void __launch_bounds__(kBlock) conv1_mfma_kernel(const float *feat, c
     for (int r = 0; r < 16; r++) bias[r] = b1[(r / 4) * 8 + half * 4 + (r % 4)];
     const f32x4 *f4 = reinterpret_cast<const f32x4 *>(feat);
     const int64_t plane = (int64_t)H * W;
+    const int xc = x < W ? x : W - 1;                 // lanes past the edge read a valid pixel
     for (int yr = ybeg; yr < yend; yr++) {
+        // every load is issued unconditionally (clamped address) and masked
+        // afterwards, so the nine loads of a row are in flight together
         f32x4 f[9];
-        const f32x4 zero = {0.0f, 0.0f, 0.0f, 0.0f};
-        const f32x4 centre = x < W ? f4[n * plane + (int64_t)yr * W + x] : zero;
 #pragma unroll
         for (int t = 0; t < 9; t++) {
-            const int yy = yr + (t / 3 - 1) * dil, xx = x + (t % 3 - 1) * dil;
-            const bool in = x < W && yy >= 0 && yy < H && xx >= 0 && xx < W;
-            if (t == 4) {
-                f[t] = centre;
-            } else if (in) {
-                f[t] = f4[n * plane + (int64_t)yy * W + xx];
-            } else {
-                f[t] = zero;
-                f[t][0] = centre[0];
-            }
+            const int yy = yr + (t / 3 - 1) * dil, xx = xc + (t % 3 - 1) * dil;
+            const bool in = yy >= 0 && yy < H && xx >= 0 && xx < W;
+            f[t] = f4[n * plane + (int64_t)(in ? yy : yr) * W + (in ? xx : xc)];
+        }
+#pragma unroll
+        for (int t = 0; t < 9; t++) {
+            if (t == 4) continue;
+            const int yy = yr + (t / 3 - 1) * dil, xx = xc + (t % 3 - 1) * dil;
+            const bool in = yy >= 0 && yy < H && xx >= 0 && xx < W;
+            if (!in) f[t] = f32x4{f[4][0], 0.0f, 0.0f, 0.0f};
         }
         f32x16 acc;
 #pragma unroll
@@ -150,9 +151,20 @@ __global__ void __launch_bounds__(kBlock) conv1_mfma_kernel(const float *feat, c
 // mode 0: BF16 NHWC32 output (+bias, optional ReLU, optional multiply by
 //         (mask > 0) where mask is BF16 NHWC32)
 // mode 1: FP32 single-channel output = channel 0 (+bias[0])
-// v_mfma_f32_32x32x16_bf16, K = 9 taps x CX, 16 per step; the wave's 9*KS
-// weight operands stay in registers for all its rows.
+// v_mfma_f32_32x32x16_bf16, K = 9 taps x CX, 16 per step.
+//
+// A wave walks a chain of kChain output rows spaced by the dilation D (rows
+// base, base + D, ...): output row y reads input rows y - D, y, y + D, so
+// consecutive rows of a chain share two of their three input rows, kept in a
+// rotating three-row register window; each step loads one new input row (the
+// 3 horizontal taps x KS vectors), issued after the MFMAs of the window row it
+// replaces, a step and a half ahead of its use.  Rows of a field split into
+// blocks of kChain * D rows, each holding D chains.  The weights (padded
+// stride per (tap, channel) row: conflict-free operand reads) and the bias
+// sit in LDS.  Grid: (ceil(W/32), ceil(chains/4), B).
 // ---------------------------------------------------------------------------
+constexpr int kChain = 8;
+
 struct ConvArgs {
     const __bf16 *x;
     const __bf16 *w;
@@ -163,65 +175,119 @@ struct ConvArgs {
     int relu, mode;
 };
 
-template <int KS>
+inline int conv_chains(int H, int dil) { return ((H + kChain * dil - 1) / (kChain * dil)) * dil; }
+
+template <int KS, bool MASK>
 __global__ void __launch_bounds__(kBlock) conv_mfma_kernel(ConvArgs a)
 {
     constexpr int CX = 16 * KS;
     constexpr int VPR = CX / 8;                    // bf16x8 vectors per (tap, co) row
+    constexpr int WS = VPR + 1;                    // padded LDS stride of a (tap, co) row
+    __shared__ bf16x8 wl[9 * 32 * WS];
+    __shared__ float bl[32];
+    const bf16x8 *wg = reinterpret_cast<const bf16x8 *>(a.w);
+    for (int i = threadIdx.x; i < 9 * 32 * VPR; i += blockDim.x) wl[(i / VPR) * WS + i % VPR] = wg[i];
+    if (threadIdx.x < 32) bl[threadIdx.x] = a.bias ? a.bias[threadIdx.x] : 0.0f;
+    __syncthreads();
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int m = lane & 31, half = lane >> 5;
     const int x = blockIdx.x * 32 + m;
     const int n = blockIdx.z;
-    const int ybeg = blockIdx.y * kConvRows + wave * kConvRowsPerWave;
-    if (ybeg >= a.H) return;
-    const int yend = min(ybeg + kConvRowsPerWave, a.H);
-    const bf16x8 *wg = reinterpret_cast<const bf16x8 *>(a.w);
-    bf16x8 wr[9][KS];
-#pragma unroll
-    for (int t = 0; t < 9; t++)
-#pragma unroll
-        for (int ks = 0; ks < KS; ks++) wr[t][ks] = wg[(t * 32 + m) * VPR + ks * 2 + half];
-    float bias[16];
-#pragma unroll
-    for (int r = 0; r < 16; r++) bias[r] = a.bias ? a.bias[(r / 4) * 8 + half * 4 + (r % 4)] : 0.0f;
+    const int D = a.dil;
+    const int g = blockIdx.y * 4 + wave;
+    const int base = (g / D) * (kChain * D) + g % D;
+    if (base >= a.H) return;
     const int64_t plane = (int64_t)a.H * a.W;
+    const int xc = x < a.W ? x : a.W - 1;             // lanes past the edge read a valid pixel
+    bool col_in[3];
+    int col[3];
+#pragma unroll
+    for (int tx = 0; tx < 3; tx++) {
+        const int xx = xc + (tx - 1) * D;
+        col_in[tx] = xx >= 0 && xx < a.W;
+        col[tx] = col_in[tx] ? xx : xc;
+    }
+    const __bf16 *img = a.x + n * plane * CX;
     const bf16x8 zero = {};
-    for (int yr = ybeg; yr < yend; yr++) {
+    // input row yy into a window slot: loads at clamped addresses; the zero
+    // padding is applied where the slot is used (row_ok), so nothing waits
+    // for a load before its MFMA needs it
+    bf16x8 win[3][3][KS];
+    bool row_ok[3];
+    auto load_row = [&](int yy, int slot) {
+        row_ok[slot] = yy >= 0 && yy < a.H;
+        const int yc = row_ok[slot] ? yy : base;
+#pragma unroll
+        for (int tx = 0; tx < 3; tx++) {
+            const bf16x8 *xp = reinterpret_cast<const bf16x8 *>(img + ((int64_t)yc * a.W + col[tx]) * CX);
+#pragma unroll
+            for (int ks = 0; ks < KS; ks++) win[slot][tx][ks] = xp[ks * 2 + half];
+        }
+    };
+    load_row(base - D, 0);
+    load_row(base, 1);
+    load_row(base + D, 2);
+    __builtin_amdgcn_sched_barrier(0);             // keep the window loads ahead of the first prefetch
+#pragma unroll
+    for (int j = 0; j < kChain; j++) {
+        const int y = base + j * D;
+        if (y >= a.H) break;
         f32x16 acc;
 #pragma unroll
         for (int r = 0; r < 16; r++) acc[r] = 0.0f;
 #pragma unroll
-        for (int t = 0; t < 9; t++) {
-            const int yy = yr + (t / 3 - 1) * a.dil, xx = x + (t % 3 - 1) * a.dil;
-            const bool in = x < a.W && yy >= 0 && yy < a.H && xx >= 0 && xx < a.W;
-            const bf16x8 *xp = reinterpret_cast<const bf16x8 *>(a.x + (n * plane + (int64_t)(in ? yy : 0) * a.W +
-                                                                       (in ? xx : 0)) * CX);
+        for (int ty = 0; ty < 3; ty++) {
+            const int slot = (j + ty) % 3;
 #pragma unroll
-            for (int ks = 0; ks < KS; ks++) {
-                const bf16x8 xv = in ? xp[ks * 2 + half] : zero;
-                acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(wr[t][ks], xv, acc, 0, 0, 0);
+            for (int tx = 0; tx < 3; tx++)
+#pragma unroll
+                for (int ks = 0; ks < KS; ks++)
+                    acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(
+                        wl[((ty * 3 + tx) * 32 + m) * WS + ks * 2 + half],
+                        row_ok[slot] && col_in[tx] ? win[slot][tx][ks] : zero, acc, 0, 0, 0);
+            if (ty == 0 && j + 1 < kChain && y + D < a.H) {
+                // the slot of input row y - D is free: next step's row y + 2D.
+                // Loads may complete out of order, so a use of any pending
+                // load waits for all of them: the previous prefetch is waited
+                // for here, before the new one is issued, and the rest of the
+                // step uses complete registers only
+                __builtin_amdgcn_s_waitcnt(0x0f70);            // vmcnt(0)
+                __builtin_amdgcn_sched_barrier(0);
+                load_row(y + 2 * D, j % 3);
+                __builtin_amdgcn_sched_barrier(0);
             }
         }
+        // weight reads two MFMAs ahead of their use rather than all hoisted
+        // (registers): lead of 2 KS reads, then one read per MFMA
+        __builtin_amdgcn_sched_group_barrier(0x100, 2 * KS, 0);
+#pragma unroll
+        for (int i = 0; i < 7 * KS; i++) {
+            __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);
+            __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);
+        }
+        __builtin_amdgcn_sched_group_barrier(0x008, 2 * KS, 0);
         if (x >= a.W) continue;
-        const int64_t p = n * plane + (int64_t)yr * a.W + x;
+        const int64_t p = n * plane + (int64_t)y * a.W + x;
         if (a.mode == 1) {
             // channel 0 is accumulator 0 of the lanes with half == 0
-            if (half == 0) reinterpret_cast<float *>(a.y)[p] = acc[0] + bias[0];
+            if (half == 0) reinterpret_cast<float *>(a.y)[p] = acc[0] + bl[0];
             continue;
         }
         __bf16 *yp = reinterpret_cast<__bf16 *>(a.y) + p * 32 + half * 4;
-        const __bf16 *mp = a.mask ? a.mask + p * 32 + half * 4 : nullptr;
+        // the backward's mask loads are a separate instantiation: their wait
+        // would also wait for the prefetched row
+        const __bf16 *mp = a.mask + p * 32 + half * 4;
 #pragma unroll
         for (int i = 0; i < 4; i++) {
             bf16x4 mk = {};
-            if (mp) mk = *reinterpret_cast<const bf16x4 *>(mp + 8 * i);
+            if (MASK) mk = *reinterpret_cast<const bf16x4 *>(mp + 8 * i);
             bf16x4 o;
 #pragma unroll
-            for (int j = 0; j < 4; j++) {
-                float v = acc[4 * i + j] + bias[4 * i + j];
+            for (int jj = 0; jj < 4; jj++) {
+                float v = acc[4 * i + jj] + bl[8 * i + 4 * half + jj];
                 if (a.relu) v = v > 0.0f ? v : 0.0f;
-                if (mp && !(from_bf16(mk[j]) > 0.0f)) v = 0.0f;
-                o[j] = to_bf16(v);
+                if (MASK && !(from_bf16(mk[jj]) > 0.0f)) v = 0.0f;
+                o[jj] = to_bf16(v);
             }
             *reinterpret_cast<bf16x4 *>(yp + 8 * i) = o;
         }
@@ -898,12 +964,14 @@ extern "C" int mv2_nn_conv_mfma(mv2_context *ctx, const void *x, int cx, const v
 {
     if (int rc = check_plain(ctx)) return rc;
     if (cx != 16 && cx != 32) return set_error(ctx, MV2_EINVAL, "[Surrogate] input channels must be 16 or 32");
-    if (B < 1 || H < 1 || W < 1 || B > 65535 || (H + kConvRows - 1) / kConvRows > 65535)
+    if (B < 1 || H < 1 || W < 1 || B > 65535 || dil < 1 || dil > 4096 || (conv_chains(H, dil) + 3) / 4 > 65535)
         return set_error(ctx, MV2_EINVAL, "[Surrogate] invalid activation shape");
     ConvArgs a{(const __bf16 *)x, (const __bf16 *)w, bias, (const __bf16 *)mask, y, B, H, W, dil, relu, mode};
-    const dim3 grid((W + 31) / 32, (H + kConvRows - 1) / kConvRows, B);
-    if (cx == 16) hipLaunchKernelGGL(conv_mfma_kernel<1>, grid, dim3(kBlock), 0, ctx->stream, a);
-    else hipLaunchKernelGGL(conv_mfma_kernel<2>, grid, dim3(kBlock), 0, ctx->stream, a);
+    const dim3 grid((W + 31) / 32, (conv_chains(H, dil) + 3) / 4, B);
+    if (cx == 16 && mask) hipLaunchKernelGGL((conv_mfma_kernel<1, true>), grid, dim3(kBlock), 0, ctx->stream, a);
+    else if (cx == 16) hipLaunchKernelGGL((conv_mfma_kernel<1, false>), grid, dim3(kBlock), 0, ctx->stream, a);
+    else if (mask) hipLaunchKernelGGL((conv_mfma_kernel<2, true>), grid, dim3(kBlock), 0, ctx->stream, a);
+    else hipLaunchKernelGGL((conv_mfma_kernel<2, false>), grid, dim3(kBlock), 0, ctx->stream, a);
     MV2_HIP(ctx, hipGetLastError());
     return MV2_OK;
 }
