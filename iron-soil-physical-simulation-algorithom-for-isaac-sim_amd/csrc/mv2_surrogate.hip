@@ -75,6 +75,34 @@ constexpr int kConvRows = 4 * kConvRowsPerWave;
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
 
+// Output tile store: lane (n, half) holds channels 8i + 4*half + (0..3) of
+// pixel n as o[i].  Written as-is, each 8-byte store instruction would touch
+// 16 bytes of every 64-byte pixel; instead the wave stages the 32 x 32 tile in
+// its LDS slice (pixel stride 40 channels: conflict-free 8-byte writes, 16-byte
+// aligned reads) and writes it back as 16 contiguous pixels (1 KB) per store
+// instruction.  dst: the row's first pixel of the segment; valid: pixels of
+// the segment inside the field.  Every lane of the wave takes part.
+constexpr int kStageStride = 40;
+constexpr int kStageElems = 32 * kStageStride;
+
+__device__ inline void store_tile_bf16(__bf16 *stage, const bf16x4 (&o)[4], int lane, __bf16 *dst, int valid)
+{
+    const int n = lane & 31, h = lane >> 5;
+#pragma unroll
+    for (int i = 0; i < 4; i++) *reinterpret_cast<bf16x4 *>(stage + n * kStageStride + 8 * i + 4 * h) = o[i];
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+#pragma unroll
+    for (int k = 0; k < 2; k++) {
+        const int q = (lane >> 2) + 16 * k, c = lane & 3;
+        const bf16x8 v = *reinterpret_cast<const bf16x8 *>(stage + q * kStageStride + 8 * c);
+        if (q < valid) *reinterpret_cast<bf16x8 *>(dst + q * 32 + 8 * c) = v;
+    }
+    // the slice is written again by the wave's next row
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+}
+
 // layer 1: FP32 3x3 conv over 3 feature channels, bias, ReLU -> BF16 NHWC32,
 // on v_mfma_f32_32x32x2_f32 (K = 9 taps x 3 channels, padded to 28).
 // w1: [9][32][4] FP32 (channel 3 unused).  Zero padding of the b_eff channel
@@ -83,9 +111,10 @@ typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
 __global__ void __launch_bounds__(kBlock) conv1_mfma_kernel(const float *feat, const float *w1, const float *b1,
                                                             int B, int H, int W, int dil, __bf16 *y)
 {
+    __shared__ __bf16 stage[4][kStageElems];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int m = lane & 31, half = lane >> 5;
-    const int x = blockIdx.x * 32 + m;
+    const int x0 = blockIdx.x * 32, x = x0 + m;
     const int n = blockIdx.z;
     const int ybeg = blockIdx.y * kConvRows + wave * kConvRowsPerWave;
     if (ybeg >= H) return;
@@ -130,18 +159,15 @@ __global__ void __launch_bounds__(kBlock) conv1_mfma_kernel(const float *feat, c
             const float vo = ko < 27 ? f[ko / 3][ko % 3] : 0.0f;
             acc = __builtin_amdgcn_mfma_f32_32x32x2f32(wa[j], half ? vo : ve, acc, 0, 0, 0);
         }
-        if (x >= W) continue;
-        __bf16 *yp = y + (n * plane + (int64_t)yr * W + x) * 32 + half * 4;
+        bf16x4 o[4];
 #pragma unroll
-        for (int i = 0; i < 4; i++) {
-            bf16x4 o;
+        for (int i = 0; i < 4; i++)
 #pragma unroll
             for (int j = 0; j < 4; j++) {
                 const float v = acc[4 * i + j] + bias[4 * i + j];
-                o[j] = to_bf16(v > 0.0f ? v : 0.0f);
+                o[i][j] = to_bf16(v > 0.0f ? v : 0.0f);
             }
-            *reinterpret_cast<bf16x4 *>(yp + 8 * i) = o;
-        }
+        store_tile_bf16(stage[wave], o, lane, y + (n * plane + (int64_t)yr * W + x0) * 32, W - x0);
     }
 }
 
@@ -185,13 +211,14 @@ __global__ void __launch_bounds__(kBlock) conv_mfma_kernel(ConvArgs a)
     constexpr int WS = VPR + 1;                    // padded LDS stride of a (tap, co) row
     __shared__ bf16x8 wl[9 * 32 * WS];
     __shared__ float bl[32];
+    __shared__ __bf16 stage[4][kStageElems];
     const bf16x8 *wg = reinterpret_cast<const bf16x8 *>(a.w);
     for (int i = threadIdx.x; i < 9 * 32 * VPR; i += blockDim.x) wl[(i / VPR) * WS + i % VPR] = wg[i];
     if (threadIdx.x < 32) bl[threadIdx.x] = a.bias ? a.bias[threadIdx.x] : 0.0f;
     __syncthreads();
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int m = lane & 31, half = lane >> 5;
-    const int x = blockIdx.x * 32 + m;
+    const int x0 = blockIdx.x * 32, x = x0 + m;
     const int n = blockIdx.z;
     const int D = a.dil;
     const int g = blockIdx.y * 4 + wave;
@@ -266,31 +293,29 @@ __global__ void __launch_bounds__(kBlock) conv_mfma_kernel(ConvArgs a)
             __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);
         }
         __builtin_amdgcn_sched_group_barrier(0x008, 2 * KS, 0);
-        if (x >= a.W) continue;
-        const int64_t p = n * plane + (int64_t)y * a.W + x;
         if (a.mode == 1) {
             // channel 0 is accumulator 0 of the lanes with half == 0
-            if (half == 0) reinterpret_cast<float *>(a.y)[p] = acc[0] + bl[0];
+            if (half == 0 && x < a.W) reinterpret_cast<float *>(a.y)[n * plane + (int64_t)y * a.W + x] = acc[0] + bl[0];
             continue;
         }
-        __bf16 *yp = reinterpret_cast<__bf16 *>(a.y) + p * 32 + half * 4;
         // the backward's mask loads are a separate instantiation: their wait
         // would also wait for the prefetched row
-        const __bf16 *mp = a.mask + p * 32 + half * 4;
+        const __bf16 *mp = a.mask + (n * plane + (int64_t)y * a.W + xc) * 32 + half * 4;
+        bf16x4 o[4];
 #pragma unroll
         for (int i = 0; i < 4; i++) {
             bf16x4 mk = {};
             if (MASK) mk = *reinterpret_cast<const bf16x4 *>(mp + 8 * i);
-            bf16x4 o;
 #pragma unroll
             for (int jj = 0; jj < 4; jj++) {
                 float v = acc[4 * i + jj] + bl[8 * i + 4 * half + jj];
                 if (a.relu) v = v > 0.0f ? v : 0.0f;
                 if (MASK && !(from_bf16(mk[jj]) > 0.0f)) v = 0.0f;
-                o[jj] = to_bf16(v);
+                o[i][jj] = to_bf16(v);
             }
-            *reinterpret_cast<bf16x4 *>(yp + 8 * i) = o;
         }
+        store_tile_bf16(stage[wave], o, lane,
+                        reinterpret_cast<__bf16 *>(a.y) + (n * plane + (int64_t)y * a.W + x0) * 32, a.W - x0);
     }
 }
 
