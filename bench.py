@@ -129,7 +129,10 @@ def learned_relax_to_rest(state, plain_rest) -> dict:
     check_it, check_conv, check_last = relax_device(b.clone(), DX, DX, 34.0, 1, relax_tol("fp32"),
                                                     runtime=state.runtime)
     mass0 = float(b0.double().sum().item())
+    # the same inputs as the warm start above (b_eff only)
+    roof = surrogate_roofline(net, b0.view(1, N, N).contiguous(), None)
     return {"ms": res.elapsed_ms, "ms_runs": [r.elapsed_ms for r in runs],
+            "surrogate_roofline": roof,
             "exact_iterations": res.iterations_exact + res.fallback_iterations,
             "envelope_launches": res.envelope_launches, "converged": bool(res.converged),
             "judge_max_excess": check_last, "judge_converged": bool(check_conv),
@@ -137,6 +140,47 @@ def learned_relax_to_rest(state, plain_rest) -> dict:
             "max_abs_vs_plain_m": float((b - plain_rest).abs().max().item()),
             "weights": {"margin_deg": meta.get("margin_deg"), "pairs": meta.get("pairs"),
                         "steps": meta.get("steps")}}
+
+
+# learned warm start: algorithmic work per pixel of one forward pass
+NN_FLOP_LAYER1 = 9 * 3 * 32 * 2            # FP32 MFMA (v_mfma_f32_32x32x2_f32)
+NN_FLOP_MID = 9 * 32 * 32 * 2              # each of layers 2-4, BF16 MFMA
+NN_FLOP_LAST = 9 * 32 * 1 * 2              # layer 5 (one output channel)
+NN_BYTES_MID = 32 * 2 * 2                  # BF16 NHWC32 activation read + written
+MFMA_PEAK_BF16_TFLOPS = 2500.0
+MFMA_PEAK_FP32_TFLOPS = 157.3
+
+
+def surrogate_roofline(net, b, h, reps: int = 20) -> dict:
+    """Per-layer device time of the warm-start network's forward pass (HIP
+    events on the library's stream) and the matrix-core fraction of the 32 ->
+    32 BF16 layers (conv_mfma_kernel), the network's dominant kernel."""
+    import torch
+
+    B, H, W = b.shape
+    px = B * H * W
+    net.forward(b, h, DX, DX)                                  # sizes the buffers
+    torch.cuda.synchronize()
+    tot = np.zeros(5)
+    for _ in range(reps):
+        ev = [torch.cuda.Event(enable_timing=True) for _ in range(6)]
+        net.forward(b, h, DX, DX, events=ev)
+        torch.cuda.synchronize()
+        tot += [ev[i].elapsed_time(ev[i + 1]) for i in range(5)]
+    layer_us = tot / reps * 1e3                               # layer 1, 2, 3, 4, 5
+    mid_us = float(np.mean(layer_us[1:4]))
+    achieved = NN_FLOP_MID * px / (mid_us * 1e-6) / 1e12
+    fwd_us = float(layer_us.sum())
+    return {"kernel": "conv_mfma_kernel<2> (layers 2-4, 32 -> 32 channels, 3x3 dilated, BF16 MFMA)",
+            "bound": "mfma", "achieved": achieved, "peak": MFMA_PEAK_BF16_TFLOPS, "unit": "TFLOP/s",
+            "frac": achieved / MFMA_PEAK_BF16_TFLOPS,
+            "hbm_GBps_algorithmic": NN_BYTES_MID * px / (mid_us * 1e-6) / 1e9,
+            "layer1_fp32_mfma_frac": NN_FLOP_LAYER1 * px / (layer_us[0] * 1e-6) / 1e12 / MFMA_PEAK_FP32_TFLOPS,
+            "per_layer_us": [round(float(v), 2) for v in layer_us],
+            "forward_us": fwd_us,
+            "forward_tflops": (NN_FLOP_LAYER1 + 3 * NN_FLOP_MID + NN_FLOP_LAST) * px / (fwd_us * 1e-6) / 1e12,
+            "pixels": px, "shape": [B, H, W],
+            "flop_per_pixel": {"layer1": NN_FLOP_LAYER1, "layers2_4_each": NN_FLOP_MID, "layer5": NN_FLOP_LAST}}
 
 
 def cpu_baseline(b, h, frames: int) -> dict:
@@ -322,6 +366,11 @@ def run_digging(args) -> int:
         "volume_rel_change": (volume1 - volume0) / volume0,
         "unstable_cells_last_frame": int(stats[:, 3].sum()),
     }
+    if net is not None:
+        # the network on the final fields of all terrains, after the timed frames
+        b3 = batch.b.view(args.dig_count, n, n)
+        h3 = (batch.h.to(torch.float32) * 2.0 ** -24).view(args.dig_count, n, n)
+        out["surrogate_roofline"] = surrogate_roofline(net, b3, h3)
     print(json.dumps(out), flush=True)
     batch.rt.close()
     return 0

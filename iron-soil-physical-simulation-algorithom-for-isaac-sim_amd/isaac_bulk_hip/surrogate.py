@@ -112,24 +112,34 @@ class WarmStartNet:
         return self._bufs[key]
 
     # ---- forward / backward --------------------------------------------------------
-    def forward(self, b, h, dx: float, dy: float):
-        """b, h: device float32 [B, H, W] -> device float32 prediction [B, H, W]."""
+    def forward(self, b, h, dx: float, dy: float, events: list | None = None):
+        """b, h: device float32 [B, H, W] -> device float32 prediction [B, H, W].
+
+        ``events``: optional list of 6 torch.cuda.Event recorded on the
+        library's stream around the launches (features | layer 1 | layers
+        2, 3, 4 | layer 5), for per-layer timing.
+        """
         rt = self.rt
         rt.bind_stream()
         B, H, W = b.shape
         buf = self._buffers(B, H, W)
         lib, ctx = rt.lib, rt.ctx
+        mark = (lambda i: events[i].record()) if events is not None else (lambda i: None)
         hp = h.data_ptr() if h is not None else None
         rt.check(lib.mv2_nn_features(ctx, b.data_ptr(), hp, B, H, W, dx, dy, buf["feat"].data_ptr()))
+        mark(0)
         rt.check(lib.mv2_nn_conv1(ctx, buf["feat"].data_ptr(), self.p["w1"].data_ptr(),
                                   self.p["b1"].data_ptr(), B, H, W, DILATIONS[0], buf["y"][1].data_ptr()))
+        mark(1)
         for k in (2, 3, 4):
             rt.check(lib.mv2_nn_conv_mfma(ctx, buf["y"][k - 1].data_ptr(), 32, self.wf[k].data_ptr(),
                                           self.p[f"b{k}"].data_ptr(), None, buf["y"][k].data_ptr(),
                                           B, H, W, DILATIONS[k - 1], 1, 0))
+            mark(k)
         rt.check(lib.mv2_nn_conv_mfma(ctx, buf["y"][4].data_ptr(), 32, self.wf[5].data_ptr(),
                                       self.p["b5"].data_ptr(), None, buf["out"].data_ptr(),
                                       B, H, W, DILATIONS[4], 0, 1))
+        mark(5)
         return buf["out"].view(B, H, W)
 
     def backward(self, grad_out, shape) -> dict:
