@@ -438,7 +438,15 @@ struct BatchRelaxArgs {
     BRelaxControl *ctl;                 // [2][count]
     unsigned long long *spread;         // [3][count][kInstWords]: maximum excess
     mv2_relax_result *out;              // [count]
+    const int32_t *tile_list;           // tiles to run (the instances still relaxing), null: every tile
 };
+
+// tile of this workgroup: XCD-aware order over every tile or over the list
+__device__ inline int brelax_tile(const BatchRelaxArgs &a)
+{
+    const int g = xcd_block();
+    return a.tile_list ? a.tile_list[g] : g;
+}
 
 // loop state of instance i for launch k (every thread of the block calls it)
 __device__ inline BRelaxControl brelax_resolve(const BatchRelaxArgs &a, int k, int i, double *smem1)
@@ -472,7 +480,7 @@ __global__ void __launch_bounds__(kRelaxBlock) batch_relax_kernel(BatchRelaxArgs
 {
     __shared__ float tile[kRelaxLH * kRelaxLW];
     __shared__ double smem[kRelaxBlock / kWave];
-    const int g = xcd_block();
+    const int g = brelax_tile(a);
     const int i = a.tile_inst[g];
     const bool first = g == a.inst_first[i];
     const BRelaxControl C = brelax_resolve(a, k, i, smem);
@@ -502,7 +510,7 @@ __global__ void __launch_bounds__(kRelaxBlock) batch_relax_kernel(BatchRelaxArgs
 __global__ void __launch_bounds__(kBlock) batch_relax_end_kernel(BatchRelaxArgs a, int k)
 {
     __shared__ double smem[1];
-    const int g = xcd_block();
+    const int g = brelax_tile(a);
     const int i = a.tile_inst[g];
     const BRelaxControl C = brelax_resolve(a, k, i, smem);
     if (C.src == 1) {
@@ -601,8 +609,9 @@ void mv2_batch_release(mv2_context *ctx)
     for (Scratch *s : {&bs->inst, &bs->tiles_tab, &bs->alt_h, &bs->alt_qx, &bs->alt_qy, &bs->ctl, &bs->wave,
                        &bs->stat, &bs->ledger, &bs->rec, &bs->rtab, &bs->relax_alt, &bs->relax_ctl,
                        &bs->relax_spread, &bs->relax_out, &bs->ctab, &bs->label_counts, &bs->label_stats,
-                       &bs->moved})
+                       &bs->moved, &bs->relax_list})
         if (s->ptr) hipFree(s->ptr);
+    if (bs->relax_list_host.ptr) hipHostFree(bs->relax_list_host.ptr);
     delete bs;
     ctx->batch = nullptr;
 }
@@ -673,6 +682,8 @@ static int build_relax_tiles(mv2_context *ctx, mv2_batch_state *bs)
     MV2_HIP(ctx, hipMemcpy(tab + 2 * n + 2 * (size_t)T, tc.data(), sizeof(int32_t) * T, hipMemcpyHostToDevice));
     bs->rtiles = T;
     bs->relax_predicted = 0;
+    bs->rfirst = first;
+    bs->rcount = ntiles;
     return MV2_OK;
 }
 
@@ -949,6 +960,10 @@ extern "C" int mv2_batch_relax(mv2_context *ctx, float *field, const double crit
     a.ctl = (BRelaxControl *)bs->relax_ctl.ptr;
     a.spread = (unsigned long long *)bs->relax_spread.ptr;
     a.out = (mv2_relax_result *)bs->relax_out.ptr;
+    a.tile_list = nullptr;
+    if (int rc = ensure_scratch(ctx, bs->relax_list, sizeof(int32_t) * T)) return rc;
+    if (int rc = ensure_scratch(ctx, bs->relax_list_host, sizeof(int32_t) * T, true)) return rc;
+    int grid = bs->rtiles;
     hipStream_t st = ctx->stream;
     hipLaunchKernelGGL(batch_relax_reset_kernel, dim3(bs->count < 1024 ? bs->count : 1024), dim3(kWave), 0, st, a);
     MV2_HIP(ctx, hipGetLastError());
@@ -964,8 +979,8 @@ extern "C" int mv2_batch_relax(mv2_context *ctx, float *field, const double crit
     for (;;) {
         if (chunk > max_iterations - k) chunk = max_iterations - k;
         for (int j = 0; j < chunk; j++, k++)
-            hipLaunchKernelGGL(batch_relax_kernel, dim3(bs->rtiles), dim3(kRelaxBlock), 0, st, a, k);
-        hipLaunchKernelGGL(batch_relax_end_kernel, dim3(bs->rtiles), dim3(kBlock), 0, st, a, k);
+            hipLaunchKernelGGL(batch_relax_kernel, dim3(grid), dim3(kRelaxBlock), 0, st, a, k);
+        hipLaunchKernelGGL(batch_relax_end_kernel, dim3(grid), dim3(kBlock), 0, st, a, k);
         MV2_HIP(ctx, hipGetLastError());
         MV2_HIP(ctx, hipMemcpyAsync(ctx->host_result.ptr, a.out, out_bytes, hipMemcpyDeviceToHost, st));
         MV2_HIP(ctx, hipStreamSynchronize(st));
@@ -980,6 +995,21 @@ extern "C" int mv2_batch_relax(mv2_context *ctx, float *field, const double crit
             break;
         }
         chunk = k / 2 > 8 ? k / 2 : 8;
+        // later launches run only the tiles of the instances still relaxing: a
+        // finished instance's field, result and loop state were settled by the
+        // end pass above and nothing reads them again in this call.  The
+        // staging buffer is rewritten only after the next synchronisation.
+        int32_t *list = (int32_t *)bs->relax_list_host.ptr;
+        int live = 0;
+        for (int i = 0; i < bs->count; i++)
+            if (!hres[i].complete)
+                for (int t = 0; t < bs->rcount[i]; t++) list[live++] = bs->rfirst[i] + t;
+        if (live < grid) {
+            MV2_HIP(ctx, hipMemcpyAsync(bs->relax_list.ptr, list, sizeof(int32_t) * (size_t)live,
+                                        hipMemcpyHostToDevice, st));
+            a.tile_list = (const int32_t *)bs->relax_list.ptr;
+            grid = live;
+        }
     }
     memcpy(out, hres, out_bytes);
     return MV2_OK;

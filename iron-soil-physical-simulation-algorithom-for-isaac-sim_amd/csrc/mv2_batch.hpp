@@ -30,6 +30,8 @@ struct mv2_batch_state {
     int rtiles = 0;
     Scratch relax_alt, relax_ctl, relax_spread, relax_out;
     int relax_predicted = 0;
+    std::vector<int32_t> rfirst, rcount;  // host copy: first relaxation tile and tile count per instance
+    Scratch relax_list, relax_list_host;  // tiles of the instances still relaxing (device / pinned staging)
     // labelling: 32 x 32 tiles of every instance (built on first use), per-root
     // counts, per-instance statistics; entrainment: quanta moved per instance
     Scratch ctab;
