@@ -378,7 +378,8 @@ class HipCompactActiveEdgeOperator:
             self._cache.move_to_end(key)
             return cached
         dev = self.runtime.device
-        arrays = tuple(torch.from_numpy(np.ascontiguousarray(v)).to(dev)
+        # the batch's index arrays may be read-only views: upload copies
+        arrays = tuple(torch.from_numpy(np.array(v, order="C", copy=True)).to(dev)
                        for v in (batch.tile_ids, batch.a_i, batch.a_j, batch.b_i, batch.b_j))
         self._cache[key] = arrays
         while len(self._cache) > self.batch_cache_limit:
