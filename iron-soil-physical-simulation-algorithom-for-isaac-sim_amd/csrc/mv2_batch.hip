@@ -439,6 +439,8 @@ struct BatchRelaxArgs {
     unsigned long long *spread;         // [3][count][kInstWords]: maximum excess
     mv2_relax_result *out;              // [count]
     const int32_t *tile_list;           // tiles to run (the instances still relaxing), null: every tile
+    int32_t *tchanged;                  // [2][tiles]: the tile's interior changed in launch k (parity k & 1)
+    double *texcess;                    // [tiles]: the tile's maximum excess in its last computed iteration
 };
 
 // tile of this workgroup: XCD-aware order over every tile or over the list
@@ -498,11 +500,41 @@ __global__ void __launch_bounds__(kRelaxBlock) batch_relax_kernel(BatchRelaxArgs
         a.ctl[((k + 1) & 1) * a.count + i] = nc;
     }
     const int64_t off = a.inst_off[i];
+    // Quiet tiles: an iteration of a tile reads its interior and an 8-cell
+    // halo, i.e. its 3 x 3 tile neighbourhood.  When none of those tiles
+    // changed in launch k - 1, the tile's input equals the input of its last
+    // computed iteration, which changed nothing, so this iteration would
+    // reproduce the same field (already in both ping-pong buffers) and the same
+    // excess: the stored excess stands in for it.  Launch 0 runs every tile.
+    if (k > 0) {
+        const int nx = a.inst_nx[i], ny = a.inst_ny[i];
+        const int tx = (nx + kRelaxTW - 1) / kRelaxTW, ty = (ny + kRelaxTH - 1) / kRelaxTH;
+        const int trow = a.tile_r0[g] / kRelaxTH, tcol = a.tile_c0[g] / kRelaxTW;
+        int flag = 0;
+        if (threadIdx.x < 9) {
+            const int rr = trow + (int)threadIdx.x / 3 - 1, cc = tcol + (int)threadIdx.x % 3 - 1;
+            if (rr >= 0 && rr < ty && cc >= 0 && cc < tx)
+                flag = a.tchanged[((k - 1) & 1) * a.tiles + a.inst_first[i] + rr * tx + cc];
+        }
+        if (!__syncthreads_or(flag)) {
+            if (threadIdx.x == 0) {
+                a.tchanged[(k & 1) * a.tiles + g] = 0;
+                inst_max(inst_set(a.spread, k % 3, a.count, i), a.texcess[g]);
+            }
+            return;
+        }
+    }
+    bool changed = false;
     const float tmax = relax_tile_iteration<float>(a.h[src] + off, a.h[src ^ 1] + off, a.inst_ny[i], a.inst_nx[i],
-                                                   a.crit, a.tile_r0[g], a.tile_c0[g], tile);
+                                                   a.crit, a.tile_r0[g], a.tile_c0[g], tile, &changed);
     double m[1] = {(double)tmax};
     block_reduce<double, 1, 1, kRelaxBlock / kWave>(m, smem);
-    if (threadIdx.x == 0) inst_max(inst_set(a.spread, k % 3, a.count, i), m[0]);
+    const int any = __syncthreads_or(changed ? 1 : 0);
+    if (threadIdx.x == 0) {
+        inst_max(inst_set(a.spread, k % 3, a.count, i), m[0]);
+        a.texcess[g] = m[0];
+        a.tchanged[(k & 1) * a.tiles + g] = any;
+    }
 }
 
 // after launch k - 1: copy each tile's interior back when its instance's field
@@ -609,7 +641,7 @@ void mv2_batch_release(mv2_context *ctx)
     for (Scratch *s : {&bs->inst, &bs->tiles_tab, &bs->alt_h, &bs->alt_qx, &bs->alt_qy, &bs->ctl, &bs->wave,
                        &bs->stat, &bs->ledger, &bs->rec, &bs->rtab, &bs->relax_alt, &bs->relax_ctl,
                        &bs->relax_spread, &bs->relax_out, &bs->ctab, &bs->label_counts, &bs->label_stats,
-                       &bs->moved, &bs->relax_list})
+                       &bs->moved, &bs->relax_list, &bs->relax_quiet})
         if (s->ptr) hipFree(s->ptr);
     if (bs->relax_list_host.ptr) hipHostFree(bs->relax_list_host.ptr);
     delete bs;
@@ -961,6 +993,9 @@ extern "C" int mv2_batch_relax(mv2_context *ctx, float *field, const double crit
     a.spread = (unsigned long long *)bs->relax_spread.ptr;
     a.out = (mv2_relax_result *)bs->relax_out.ptr;
     a.tile_list = nullptr;
+    if (int rc = ensure_scratch(ctx, bs->relax_quiet, (2 * sizeof(int32_t) + sizeof(double)) * T)) return rc;
+    a.texcess = (double *)bs->relax_quiet.ptr;
+    a.tchanged = (int32_t *)(a.texcess + T);
     if (int rc = ensure_scratch(ctx, bs->relax_list, sizeof(int32_t) * T)) return rc;
     if (int rc = ensure_scratch(ctx, bs->relax_list_host, sizeof(int32_t) * T, true)) return rc;
     int grid = bs->rtiles;

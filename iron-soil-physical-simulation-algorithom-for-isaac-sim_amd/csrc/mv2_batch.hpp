@@ -32,6 +32,7 @@ struct mv2_batch_state {
     int relax_predicted = 0;
     std::vector<int32_t> rfirst, rcount;  // host copy: first relaxation tile and tile count per instance
     Scratch relax_list, relax_list_host;  // tiles of the instances still relaxing (device / pinned staging)
+    Scratch relax_quiet;                  // per tile: last excess (double), changed flags [2] (int32)
     // labelling: 32 x 32 tiles of every instance (built on first use), per-root
     // counts, per-instance statistics; entrainment: quanta moved per instance
     Scratch ctab;

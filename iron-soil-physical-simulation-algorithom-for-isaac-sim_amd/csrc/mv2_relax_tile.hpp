@@ -62,10 +62,11 @@ static_assert(kRelaxHalo % 2 == 0 && kRelaxTH % 2 == 0 && kRelaxTW % 2 == 0,
 // the tile owns, taken before each sub-phase.  Every phase first loads all the
 // operands a thread needs, then computes, then stores: the edges of one parity
 // are disjoint, so the loads of one thread's edges never depend on its own
-// stores.  All threads of the block must call it.
+// stores.  All threads of the block must call it.  With `changed`, the thread
+// also reports whether any interior cell it writes differs from its value in hs.
 template <class R>
 __device__ inline R relax_tile_iteration(const R *hs, R *hd, int ny, int nx, const R (&crits)[4], int r0, int c0,
-                                         R *tile)
+                                         R *tile, bool *changed = nullptr)
 {
     const int R0 = r0 - kRelaxHalo, C0 = c0 - kRelaxHalo;
     // stage tile + halo in one round trip (cells outside the grid are never touched)
@@ -162,7 +163,11 @@ __device__ inline R relax_tile_iteration(const R *hs, R *hd, int ny, int nx, con
         const int e = threadIdx.x + j * kRelaxBlock;
         const int ir = e / kRelaxTW, ic = e % kRelaxTW;
         const int r = r0 + ir, c = c0 + ic;
-        if (r < ny && c < nx) hd[r * nx + c] = tile[(ir + kRelaxHalo) * kRelaxLW + ic + kRelaxHalo];
+        if (r < ny && c < nx) {
+            const R v = tile[(ir + kRelaxHalo) * kRelaxLW + ic + kRelaxHalo];
+            hd[r * nx + c] = v;
+            if (changed && v != hs[r * nx + c]) *changed = true;
+        }
     }
     return tmax;
 }
