@@ -118,6 +118,7 @@ extern "C" void mv2_destroy(mv2_context *ctx)
     release(ctx->partials);
     release(ctx->result);
     release(ctx->relax_alt);
+    release(ctx->relax_quiet);
     release(ctx->label_aux);
     release(ctx->nn_partials);
     release(ctx->nn_loss);

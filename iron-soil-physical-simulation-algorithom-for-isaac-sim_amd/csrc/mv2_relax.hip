@@ -12,7 +12,9 @@
 // it against the tolerance on the device and returns at once once converged,
 // so the host enqueues batches of launches and reads one result per batch.
 // An iteration costs one read + one write of the field instead of eight
-// full-array passes.  Directions and parities: mv2_relax_tile.hpp.
+// full-array passes; a tile whose 3 x 3 tile neighbourhood did not change in
+// the previous launch is skipped (its field and excess are already known).
+// Directions and parities: mv2_relax_tile.hpp.
 #include "mv2_internal.hpp"
 #include "mv2_frontier_edge.hpp"
 #include "mv2_relax_tile.hpp"
@@ -47,6 +49,8 @@ struct RelaxArgs {
     unsigned long long *spread;   // three spread-maximum sets
     mv2_relax_result *out;
     int tiles_x, tiles;
+    int32_t *tchanged;            // [2][tiles]: the tile's interior changed in launch k (parity k & 1)
+    double *texcess;              // [tiles]: the tile's maximum excess in its last computed iteration
 };
 
 // loop state for launch k: did iteration k (applied by launch k - 1) converge?
@@ -89,11 +93,40 @@ __global__ void __launch_bounds__(kRelaxBlock) relax_kernel(RelaxArgs<R> a, int 
         s->ctl[(k + 1) & 1] = nc;
     }
     const int t = blockIdx.x;
-    const int r0 = (t / a.tiles_x) * kRelaxTH, c0 = (t % a.tiles_x) * kRelaxTW;
-    const R tmax = relax_tile_iteration<R>(a.h[src], a.h[src ^ 1], a.ny, a.nx, a.crit, r0, c0, tile);
+    const int trow = t / a.tiles_x, tcol = t % a.tiles_x;
+    const int r0 = trow * kRelaxTH, c0 = tcol * kRelaxTW;
+    // Quiet tiles: an iteration of a tile reads its interior and an 8-cell
+    // halo, i.e. its 3 x 3 tile neighbourhood.  When none of those tiles
+    // changed in launch k - 1, the tile's input equals the input of its last
+    // computed iteration, which changed nothing, so this iteration would
+    // reproduce the same field (already in both ping-pong buffers) and the same
+    // excess: the stored excess stands in for it.  Launch 0 runs every tile.
+    if (k > 0) {
+        const int ty = a.tiles / a.tiles_x;
+        int flag = 0;
+        if (threadIdx.x < 9) {
+            const int rr = trow + (int)threadIdx.x / 3 - 1, cc = tcol + (int)threadIdx.x % 3 - 1;
+            if (rr >= 0 && rr < ty && cc >= 0 && cc < a.tiles_x)
+                flag = a.tchanged[((k - 1) & 1) * a.tiles + rr * a.tiles_x + cc];
+        }
+        if (!__syncthreads_or(flag)) {
+            if (threadIdx.x == 0) {
+                a.tchanged[(k & 1) * a.tiles + t] = 0;
+                spread_max(a.spread + (k % 3) * kSpreadWords, a.texcess[t]);
+            }
+            return;
+        }
+    }
+    bool changed = false;
+    const R tmax = relax_tile_iteration<R>(a.h[src], a.h[src ^ 1], a.ny, a.nx, a.crit, r0, c0, tile, &changed);
     double m[1] = {(double)tmax};
     block_reduce<double, 1, 1, kRelaxBlock / kWave>(m, smem);
-    if (threadIdx.x == 0) spread_max(a.spread + (k % 3) * kSpreadWords, m[0]);
+    const int any = __syncthreads_or(changed ? 1 : 0);
+    if (threadIdx.x == 0) {
+        spread_max(a.spread + (k % 3) * kSpreadWords, m[0]);
+        a.texcess[t] = m[0];
+        a.tchanged[(k & 1) * a.tiles + t] = any;
+    }
 }
 
 // after launch k - 1: copy back when the field sits in scratch, write the result
@@ -144,6 +177,10 @@ static int launch_relax(mv2_context *ctx, void *height, int ny, int nx, const do
     a.out = (mv2_relax_result *)((char *)ctx->result.ptr + 2048);
     a.tiles_x = (nx + kRelaxTW - 1) / kRelaxTW;
     a.tiles = a.tiles_x * ((ny + kRelaxTH - 1) / kRelaxTH);
+    if (int rc = ensure_scratch(ctx, ctx->relax_quiet, (2 * sizeof(int32_t) + sizeof(double)) * (size_t)a.tiles))
+        return rc;
+    a.texcess = (double *)ctx->relax_quiet.ptr;
+    a.tchanged = (int32_t *)(a.texcess + a.tiles);
     hipStream_t st = ctx->stream;
     const int gend = grid_for((int64_t)n, kBlock, 4 * ctx->cu_count);
     hipLaunchKernelGGL(zero_relax_sync, dim3(1), dim3(kBlock), 0, st, a.sync, a.spread);
