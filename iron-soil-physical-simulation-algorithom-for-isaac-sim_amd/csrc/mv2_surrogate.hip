@@ -20,6 +20,7 @@
 #include "mv2_internal.hpp"
 
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <vector>
 
@@ -762,8 +763,10 @@ __global__ void __launch_bounds__(kBlock) apply_delta_kernel(float *b, const flo
 // row) and left passes follow.  An optimal path between two cells uses at most
 // two adjacent move directions, and every such pair lies inside one pass's
 // three moves, so one cycle of the four passes settles a region up to
-// rounding; cycles repeat while anything in the region moves (at most
-// kEnvMaxCycles).  Tiles exchange through HBM between launches; the next
+// rounding.  One cycle per launch (kEnvCycles): further cycles rarely moved
+// anything, and a launch lasts as long as its busiest region (digging, 64 x
+// 256^2: 245 frames/s with one cycle, 232 with up to 2, 3 or 6, the same 4
+// launches per frame).  Tiles exchange through HBM between launches; the next
 // launch stops the loop when no interior cell of any tile changed (a halo of
 // one cell already makes that a global fixed point).
 // ---------------------------------------------------------------------------
@@ -772,7 +775,8 @@ constexpr int kEnvHalo = 2;
 constexpr int kEnvTile = kEnvRegion - 2 * kEnvHalo;     // interior cells per side
 constexpr int kEnvStride = kEnvRegion + 1;              // LDS row stride: column walks are conflict-free
 constexpr int kEnvWaves = 2;                            // regions per workgroup
-constexpr int kEnvMaxCycles = 6;
+constexpr int kEnvMaxCycles = 6;                        // upper limit of MV2_ENV_CYCLES
+constexpr int kEnvCycles = 1;                           // default cycles per region and launch
 
 struct EnvControl {
     int launches, done, pad[2];
@@ -871,7 +875,8 @@ __device__ inline bool env_pass(float *t, int lane, uint64_t valid_rows, uint64_
 __global__ void __launch_bounds__(kEnvWaves * kWave) envelope_kernel(float *z, int ny, int nx, float c0, float c1,
                                                                       float c2, float c3, int tiles_x, int tiles,
                                                                       EnvControl *ctl, unsigned long long *spread,
-                                                                      int k, int seam)
+                                                                      int32_t *tile_moved, int k, int seam,
+                                                                      int max_cycles)
 {
     __shared__ float lds[kEnvWaves][kEnvRegion * kEnvStride];
     __shared__ double smem[1];
@@ -897,7 +902,21 @@ __global__ void __launch_bounds__(kEnvWaves * kWave) envelope_kernel(float *z, i
     const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
     const int tile = blockIdx.x * kEnvWaves + wave;
     bool moved = false;
-    if (tile < tiles) {
+    // Quiet tiles: a region reads its tile's 3 x 3 tile neighbourhood.  When
+    // no interior of those tiles changed in launch k - 1, nothing wrote into
+    // the region during it, so its input equals that of the last launch that
+    // processed it, which changed nothing: it is skipped.  Launch 0 runs all.
+    bool quiet = false;
+    if (tile < tiles && k > 0) {
+        const int ty = tiles / tiles_x, trow = tile / tiles_x, tcol = tile % tiles_x;
+        int flag = 0;
+        if (lane < 9) {
+            const int rr = trow + lane / 3 - 1, cc = tcol + lane % 3 - 1;
+            if (rr >= 0 && rr < ty && cc >= 0 && cc < tiles_x) flag = tile_moved[((k - 1) & 1) * tiles + rr * tiles_x + cc];
+        }
+        quiet = !__any(flag);
+    }
+    if (tile < tiles && !quiet) {
         float *t = lds[wave];
         const int R0 = (tile / tiles_x) * kEnvTile - kEnvHalo, C0 = (tile % tiles_x) * kEnvTile - kEnvHalo;
         // this lane as a region row and as a region column
@@ -911,7 +930,7 @@ __global__ void __launch_bounds__(kEnvWaves * kWave) envelope_kernel(float *z, i
                                                                          : __builtin_huge_valf();
         __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
         __builtin_amdgcn_wave_barrier();
-        for (int cycle = 0; cycle < kEnvMaxCycles; cycle++) {
+        for (int cycle = 0; cycle < max_cycles; cycle++) {
             bool ch = env_pass<false, true>(t, lane, valid_rows, valid_cols, cut_above, cut_below, c0, c1, c2, c3,
                                             &moved);
             ch |= env_pass<false, false>(t, lane, valid_rows, valid_cols, cut_above, cut_below, c0, c1, c2, c3,
@@ -926,6 +945,10 @@ __global__ void __launch_bounds__(kEnvWaves * kWave) envelope_kernel(float *z, i
                 if (((valid_rows >> r) & 1) && col_in && lane >= kEnvHalo && lane < kEnvHalo + kEnvTile)
                     z[(int64_t)(R0 + r) * nx + gcol] = t[r * kEnvStride + lane];
         }
+    }
+    if (tile < tiles) {                              // wave-uniform
+        const bool wave_moved = __any(moved);
+        if (lane == 0) tile_moved[(k & 1) * tiles + tile] = wave_moved ? 1 : 0;
     }
     if (moved) changed = 1;     // benign race: every writer stores 1
     __syncthreads();
@@ -1207,6 +1230,16 @@ static int envelope_run(mv2_context *ctx, const float *b, const float *delta, fl
     const int tiles_x = (nx + kEnvTile - 1) / kEnvTile;
     const int tiles = tiles_x * ((ny + kEnvTile - 1) / kEnvTile);
     const int blocks = (tiles + kEnvWaves - 1) / kEnvWaves;
+    if (int rc = ensure_scratch(ctx, ctx->env_moved, 2 * sizeof(int32_t) * (size_t)tiles)) return rc;
+    int32_t *tile_moved = (int32_t *)ctx->env_moved.ptr;
+    // cycles of the four passes per region and launch: more settle a region
+    // in one launch, fewer bound the launch by its slowest region (a launch
+    // lasts as long as its busiest region); MV2_ENV_CYCLES = 1..6 overrides
+    int cycles = kEnvCycles;
+    if (const char *e = getenv("MV2_ENV_CYCLES")) {
+        const int forced = atoi(e);
+        if (forced >= 1 && forced <= kEnvMaxCycles) cycles = forced;
+    }
     int k = 0, chunk = 4;
     const int *hres = (const int *)ctx->host_result.ptr;
     for (;;) {
@@ -1214,7 +1247,7 @@ static int envelope_run(mv2_context *ctx, const float *b, const float *delta, fl
         for (int j = 0; j < chunk; j++, k++)
             hipLaunchKernelGGL(envelope_kernel, dim3(blocks), dim3(kEnvWaves * kWave), 0, st, z, ny, nx,
                                (float)crit_diff[0], (float)crit_diff[1], (float)crit_diff[2], (float)crit_diff[3],
-                               tiles_x, tiles, ctl, spread, k, seam);
+                               tiles_x, tiles, ctl, spread, tile_moved, k, seam, cycles);
         hipLaunchKernelGGL(envelope_result_kernel, dim3(1), dim3(kWave), 0, st, ctl, spread, k, dres);
         MV2_HIP(ctx, hipGetLastError());
         MV2_HIP(ctx, hipMemcpyAsync(ctx->host_result.ptr, dres, 2 * sizeof(int), hipMemcpyDeviceToHost, st));
