@@ -415,6 +415,18 @@ int mv2_envelope(mv2_context *ctx, const float *b, const float *delta, float *z,
 int mv2_batch_envelope(mv2_context *ctx, const float *b, const float *delta, float *z, int count, int ny, int nx,
                        const double crit_diff[4], int max_launches, int *launches_out);
 int mv2_batch_commit_surface(mv2_context *ctx, float *b, const float *z, int count, int64_t n, double eps);
+/* Settled commit (no counterpart in the reference; learned warm start): the
+ * commit of mv2_nn_commit_surface, then (rounds - 1) settle rounds, each the
+ * Lipschitz envelope of the committed field followed by a projection whose
+ * moved cells are those of the first commit and whose shift balances the mass
+ * of the whole field; a round whose envelope moves nothing ends them.  z is
+ * the enveloped surface on entry and scratch afterwards.  The batched form
+ * works on `count` stacked fields of ny x nx, each on its own.
+ * launches_out[0] = envelope launches of the settle rounds. */
+int mv2_nn_commit_settled(mv2_context *ctx, float *b, float *z, int ny, int nx, const double crit_diff[4], double eps,
+                          int rounds, int max_launches, int *launches_out);
+int mv2_batch_commit_settled(mv2_context *ctx, float *b, float *z, int count, int ny, int nx,
+                             const double crit_diff[4], double eps, int rounds, int max_launches, int *launches_out);
 /* batched relaxation of independent FP32 patches (<= 4096 cells each), one
  * workgroup per patch, same schedule as mv2_relax; iterations per patch out */
 int mv2_relax_patches(mv2_context *ctx, float *h, int npatch, int H, int W,

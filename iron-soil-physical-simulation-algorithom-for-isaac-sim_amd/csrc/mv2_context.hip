@@ -120,6 +120,7 @@ extern "C" void mv2_destroy(mv2_context *ctx)
     release(ctx->relax_alt);
     release(ctx->relax_quiet);
     release(ctx->env_moved);
+    release(ctx->nn_mask);
     release(ctx->label_aux);
     release(ctx->nn_partials);
     release(ctx->nn_loss);

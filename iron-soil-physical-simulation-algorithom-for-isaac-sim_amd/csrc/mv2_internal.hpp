@@ -51,6 +51,7 @@ struct mv2_context {
     mv2::Scratch relax_alt;               // ping-pong height of the relaxation
     mv2::Scratch relax_quiet;             // relaxation tiles: last excess (double), changed flags [2] (int32)
     mv2::Scratch env_moved;               // envelope tiles: interior changed in launch k, [2][tiles] (int32)
+    mv2::Scratch nn_mask;                 // warm start: cells moved by the first mass projection (uint8)
     mv2::Scratch host_result;             // pinned host staging
     mv2::Scratch label_aux;               // labelling partials
     mv2::Scratch nn_partials;             // weight-gradient partials of the surrogate

@@ -637,6 +637,14 @@ __global__ void __launch_bounds__(kBlock) excess_loss_kernel(const float *pred, 
 // the moved cells gain, s += gain(s) / (moved cells above zero).  gain is
 // convex and decreasing in s, so the steps approach its zero from one side;
 // without cells held at zero the first s already zeroes it (to rounding).
+//
+// Masked form (MASKED, from a surface z): the moved cells are a given set M
+// (the cells the first projection moved) and every other cell takes z as it
+// is; the shift over M balances the mass of the whole field, so the change
+// z - b outside M is paid for by M.  The settle rounds use it: the envelope of
+// the committed surface cuts the steps the first shift left at the rim of M,
+// and the masked projection returns that mass to M, which leaves a step of a
+// much smaller shift.
 // Field-major partials: sum at [g], count at [gridDim.x + g].
 template <bool FROM_SURFACE>
 __device__ inline float change_at(const float *b, const float *src, int64_t p)
@@ -644,21 +652,38 @@ __device__ inline float change_at(const float *b, const float *src, int64_t p)
     return FROM_SURFACE ? src[p] - b[p] : src[p];
 }
 
+struct ProjectArgs {
+    const float *src;
+    int64_t n;
+    float eps;
+    const uint8_t *mask_in;     // MASKED: the moved cells
+    uint8_t *mask_out;          // optional (unmasked): records the moved cells
+};
+
+template <bool FROM_SURFACE, bool MASKED>
+__device__ inline bool moved_at(const ProjectArgs &a, int64_t base, int64_t p, float d)
+{
+    return MASKED ? a.mask_in[base + p] != 0 : fabsf(d) > a.eps;
+}
+
 // Segments: the field is `gridDim.y` independent segments of n cells each
 // (segment s at offset s * n), each with its own shift.
-template <bool FROM_SURFACE>
-__global__ void __launch_bounds__(kBlock) delta_stats_kernel(const float *b, const float *src, int64_t n, float eps,
-                                                             double *partials)
+template <bool FROM_SURFACE, bool MASKED>
+__global__ void __launch_bounds__(kBlock) delta_stats_kernel(const float *b, ProjectArgs a, double *partials)
 {
     __shared__ double smem[8];
-    const int64_t base = (int64_t)blockIdx.y * n;
+    const int64_t base = (int64_t)blockIdx.y * a.n;
     b += base;
-    src += base;
+    const float *src = a.src + base;
     partials += (int64_t)blockIdx.y * 2 * gridDim.x;
     double s[2] = {0.0, 0.0};
-    for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < n; p += (int64_t)gridDim.x * blockDim.x) {
+    for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < a.n; p += (int64_t)gridDim.x * blockDim.x) {
         const float d = change_at<FROM_SURFACE>(b, src, p);
-        if (fabsf(d) > eps) {
+        const bool mv = moved_at<FROM_SURFACE, MASKED>(a, base, p, d);
+        if (MASKED) {
+            s[0] += (double)src[p] - (double)b[p];
+            if (mv) s[1] += 1.0;
+        } else if (mv) {
             s[0] += (double)d;
             s[1] += 1.0;
         }
@@ -684,26 +709,29 @@ __global__ void __launch_bounds__(kBlock) delta_shift_kernel(const double *parti
     if (threadIdx.x == 0) shift[blockIdx.x] = s[1] > 0.0 ? (float)(s[0] / s[1]) : 0.0f;
 }
 
-// mass the moved cells gain at the current shift (clamp at zero included),
-// and the moved cells that stay above zero (same arithmetic as apply_delta_kernel)
-template <bool FROM_SURFACE>
-__global__ void __launch_bounds__(kBlock) clamp_stats_kernel(const float *b, const float *src, int64_t n, float eps,
-                                                             const float *shift_ptr, double *partials)
+// mass the field gains at the current shift (clamp at zero included; masked:
+// with the fixed change outside M), and the moved cells that stay above zero
+// (same arithmetic as apply_delta_kernel)
+template <bool FROM_SURFACE, bool MASKED>
+__global__ void __launch_bounds__(kBlock) clamp_stats_kernel(const float *b, ProjectArgs a, const float *shift_ptr,
+                                                             double *partials)
 {
     __shared__ double smem[8];
     const float shift = shift_ptr[blockIdx.y];
-    const int64_t base = (int64_t)blockIdx.y * n;
+    const int64_t base = (int64_t)blockIdx.y * a.n;
     b += base;
-    src += base;
+    const float *src = a.src + base;
     partials += (int64_t)blockIdx.y * 2 * gridDim.x;
     double s[2] = {0.0, 0.0};
-    for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < n; p += (int64_t)gridDim.x * blockDim.x) {
+    for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < a.n; p += (int64_t)gridDim.x * blockDim.x) {
         const float d = change_at<FROM_SURFACE>(b, src, p);
-        if (fabsf(d) > eps) {
+        if (moved_at<FROM_SURFACE, MASKED>(a, base, p, d)) {
             float v = b[p] + (d - shift);
             v = v > 0.0f ? v : 0.0f;
             s[0] += (double)v - (double)b[p];     // mass the cell gains at this shift
             if (v > 0.0f) s[1] += 1.0;
+        } else if (MASKED) {
+            s[0] += (double)src[p] - (double)b[p];
         }
     }
     block_reduce<double, 2, 0>(s, smem);
@@ -727,20 +755,23 @@ __global__ void __launch_bounds__(kBlock) clamp_shift_kernel(const double *parti
     if (threadIdx.x == 0 && s[1] > 0.0) shift[blockIdx.x] = (float)((double)shift[blockIdx.x] + s[0] / s[1]);
 }
 
-template <bool FROM_SURFACE>
-__global__ void __launch_bounds__(kBlock) apply_delta_kernel(float *b, const float *src, int64_t n, float eps,
-                                                             const float *shift_ptr)
+template <bool FROM_SURFACE, bool MASKED>
+__global__ void __launch_bounds__(kBlock) apply_delta_kernel(float *b, ProjectArgs a, const float *shift_ptr)
 {
     const float shift = shift_ptr[blockIdx.y];
-    const int64_t base = (int64_t)blockIdx.y * n;
+    const int64_t base = (int64_t)blockIdx.y * a.n;
     b += base;
-    src += base;
-    for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < n; p += (int64_t)gridDim.x * blockDim.x) {
+    const float *src = a.src + base;
+    for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < a.n; p += (int64_t)gridDim.x * blockDim.x) {
         const float d = change_at<FROM_SURFACE>(b, src, p);
-        if (fabsf(d) > eps) {
+        const bool mv = moved_at<FROM_SURFACE, MASKED>(a, base, p, d);
+        if (mv) {
             const float v = b[p] + (d - shift);
             b[p] = v > 0.0f ? v : 0.0f;
+        } else if (MASKED) {
+            b[p] = src[p];
         }
+        if (!MASKED && a.mask_out) a.mask_out[base + p] = mv ? 1 : 0;
     }
 }
 
@@ -1149,24 +1180,26 @@ extern "C" int mv2_relax_patches(mv2_context *ctx, float *h, int npatch, int H, 
 
 constexpr int kClampCorrections = 4;
 
-template <bool FROM_SURFACE>
-static int project_change(mv2_context *ctx, float *b, const float *src, int64_t n, double eps, int segments = 1)
+template <bool FROM_SURFACE, bool MASKED = false>
+static int project_change(mv2_context *ctx, float *b, const float *src, int64_t n, double eps, int segments = 1,
+                          const uint8_t *mask_in = nullptr, uint8_t *mask_out = nullptr)
 {
     const int grid = grid_for(n, kBlock, 1024);     // per segment: the same partials as one field
     const size_t parts_n = (size_t)2 * grid * segments;
     if (int rc = ensure_scratch(ctx, ctx->nn_loss, sizeof(double) * parts_n + sizeof(float) * segments + 64)) return rc;
     double *parts = (double *)ctx->nn_loss.ptr;
     float *shift = (float *)(parts + parts_n);
-    hipLaunchKernelGGL(delta_stats_kernel<FROM_SURFACE>, dim3(grid, segments), dim3(kBlock), 0, ctx->stream, b, src,
-                       n, (float)eps, parts);
+    const ProjectArgs a{src, n, (float)eps, mask_in, mask_out};
+    hipLaunchKernelGGL((delta_stats_kernel<FROM_SURFACE, MASKED>), dim3(grid, segments), dim3(kBlock), 0, ctx->stream,
+                       b, a, parts);
     hipLaunchKernelGGL(delta_shift_kernel, dim3(segments), dim3(kBlock), 0, ctx->stream, parts, grid, shift);
     for (int c = 0; c < kClampCorrections; c++) {
-        hipLaunchKernelGGL(clamp_stats_kernel<FROM_SURFACE>, dim3(grid, segments), dim3(kBlock), 0, ctx->stream, b,
-                           src, n, (float)eps, shift, parts);
+        hipLaunchKernelGGL((clamp_stats_kernel<FROM_SURFACE, MASKED>), dim3(grid, segments), dim3(kBlock), 0,
+                           ctx->stream, b, a, shift, parts);
         hipLaunchKernelGGL(clamp_shift_kernel, dim3(segments), dim3(kBlock), 0, ctx->stream, parts, grid, shift);
     }
-    hipLaunchKernelGGL(apply_delta_kernel<FROM_SURFACE>, dim3(grid, segments), dim3(kBlock), 0, ctx->stream, b, src,
-                       n, (float)eps, shift);
+    hipLaunchKernelGGL((apply_delta_kernel<FROM_SURFACE, MASKED>), dim3(grid, segments), dim3(kBlock), 0, ctx->stream,
+                       b, a, shift);
     MV2_HIP(ctx, hipGetLastError());
     return MV2_OK;
 }
@@ -1208,6 +1241,48 @@ extern "C" int mv2_batch_commit_surface(mv2_context *ctx, float *b, const float 
     if (!b || !z || n < 1 || count < 1 || count > 65535)
         return set_error(ctx, MV2_EINVAL, "[WarmStart] invalid commit arguments");
     return project_change<true>(ctx, b, z, n, eps, count);
+}
+
+// Settle rounds: the commit, then (rounds - 1) times the Lipschitz envelope of
+// the committed field and the masked projection onto the first commit's moved
+// cells (stacked fields: the envelope does not cross their seams, every field
+// keeps its own shift).  A round whose envelope changes nothing ends them.
+static int commit_settled(mv2_context *ctx, float *b, float *z, int count, int ny, int nx, const double crit_diff[4],
+                          double eps, int rounds, int max_launches, int *launches_out)
+{
+    if (int rc = check_plain(ctx)) return rc;
+    if (!b || !z || !crit_diff || count < 1 || count > 65535 || ny < 1 || nx < 1 || rounds < 1 || max_launches < 1)
+        return set_error(ctx, MV2_EINVAL, "[WarmStart] invalid settle arguments");
+    const int64_t n = (int64_t)ny * nx;
+    if (n * count >= ((int64_t)1 << 31)) return set_error(ctx, MV2_EINVAL, "[WarmStart] grid too large");
+    if (int rc = ensure_scratch(ctx, ctx->nn_mask, (size_t)(n * count))) return rc;
+    uint8_t *mask = (uint8_t *)ctx->nn_mask.ptr;
+    if (int rc = project_change<true>(ctx, b, z, n, eps, count, nullptr, mask)) return rc;
+    int total = 0;
+    for (int r = 1; r < rounds; r++) {
+        MV2_HIP(ctx, hipMemcpyAsync(z, b, sizeof(float) * (size_t)(n * count), hipMemcpyDeviceToDevice, ctx->stream));
+        int l = 0;
+        if (int rc = envelope_run(ctx, nullptr, nullptr, z, count * ny, nx, crit_diff, max_launches, &l, ny))
+            return rc;
+        total += l;
+        if (l <= 1) break;               // nothing moved: the field is its own envelope
+        if (int rc = project_change<true, true>(ctx, b, z, n, eps, count, mask)) return rc;
+    }
+    if (launches_out) launches_out[0] = total;
+    return MV2_OK;
+}
+
+extern "C" int mv2_nn_commit_settled(mv2_context *ctx, float *b, float *z, int ny, int nx, const double crit_diff[4],
+                                     double eps, int rounds, int max_launches, int *launches_out)
+{
+    return commit_settled(ctx, b, z, 1, ny, nx, crit_diff, eps, rounds, max_launches, launches_out);
+}
+
+extern "C" int mv2_batch_commit_settled(mv2_context *ctx, float *b, float *z, int count, int ny, int nx,
+                                        const double crit_diff[4], double eps, int rounds, int max_launches,
+                                        int *launches_out)
+{
+    return commit_settled(ctx, b, z, count, ny, nx, crit_diff, eps, rounds, max_launches, launches_out);
 }
 
 static int envelope_run(mv2_context *ctx, const float *b, const float *delta, float *z, int ny, int nx,

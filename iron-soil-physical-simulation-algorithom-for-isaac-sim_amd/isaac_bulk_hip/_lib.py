@@ -163,6 +163,8 @@ SIGNATURES: dict[str, tuple[object, list[object]]] = {
     "mv2_nn_commit_surface": (_I, [_P, _P, _P, ctypes.c_int64, _D]),
     "mv2_envelope": (_I, [_P, _P, _P, _P, _I, _I, _P, _I, _P]),
     "mv2_batch_envelope": (_I, [_P, _P, _P, _P, _I, _I, _I, _P, _I, _P]),
+    "mv2_nn_commit_settled": (_I, [_P, _P, _P, _I, _I, _P, _D, _I, _I, _P]),
+    "mv2_batch_commit_settled": (_I, [_P, _P, _P, _I, _I, _I, _P, _D, _I, _I, _P]),
     "mv2_batch_commit_surface": (_I, [_P, _P, _P, _I, ctypes.c_int64, _D]),
     "mv2_nn_excess_loss": (_I, [_P, _P, _P, _I, _I, _I, _P, _D, _P, ctypes.POINTER(_D)]),
     "mv2_relax_patches": (_I, [_P, _P, _I, _I, _I, _P, _I, _D, _P]),

@@ -260,18 +260,25 @@ class WarmStartResult:
     residual_after_sweeps: float
     elapsed_ms: float
     envelope_launches: int = 0
+    settle_launches: int = 0
 
 
 def warm_start_relax(net: WarmStartNet, b_field, h_field, dx: float, critical_angle_deg: float,
                      tolerance: float, *, max_iterations: int = 20000, sweeps: int = 4,
                      fallback_threshold: float = 1e-6, eps: float = 1e-5,
-                     envelope_angle_deg: float | None = None) -> WarmStartResult:
+                     envelope_angle_deg: float | None = None, settle_rounds: int = 1) -> WarmStartResult:
     """In-place accelerated relaxation of a device float32 [H, W] b_eff field.
 
     1. the network predicts the relaxed change of b_eff;
     2. the Lipschitz envelope (slopes <= envelope angle, default the critical
        angle) removes the local over-steep spots of the predicted surface;
     3. mass projection: the change is shifted to zero sum over the moved cells;
+       then ``settle_rounds - 1`` rounds of the envelope of the committed field
+       and a projection onto the same moved cells that balances the whole
+       field's mass (each round shrinks the step the shift leaves at the rim of
+       the moved cells; one field of 701^2: 0.52 ms to rest with no settle
+       round, 11 exact iterations, against 0.61 ms with two rounds and 1
+       iteration -- each round costs an envelope and a host check);
     4. at most ``sweeps`` exact relaxation iterations; if the maximum excess
        still exceeds the tolerance, the plain relaxation continues to the same
        tolerance (fallback; ``used_fallback`` when the residual after the sweeps
@@ -289,7 +296,9 @@ def warm_start_relax(net: WarmStartNet, b_field, h_field, dx: float, critical_an
     zbuf = pred.view(H, W)                      # the surface overwrites the prediction in place
     rt.check(rt.lib.mv2_envelope(rt.ctx, b_field.data_ptr(), pred.data_ptr(), zbuf.data_ptr(), H, W,
                                  crit.ctypes.data, 256, launches))
-    rt.check(rt.lib.mv2_nn_commit_surface(rt.ctx, b_field.data_ptr(), zbuf.data_ptr(), H * W, eps))
+    settle = (ctypes.c_int * 1)()
+    rt.check(rt.lib.mv2_nn_commit_settled(rt.ctx, b_field.data_ptr(), zbuf.data_ptr(), H, W, crit.ctypes.data, eps,
+                                          int(settle_rounds), 256, settle))
     it, conv, last = relax_device(b_field, dx, dx, critical_angle_deg, sweeps, tolerance, runtime=rt)
     fb_it = 0
     used = False
@@ -299,4 +308,5 @@ def warm_start_relax(net: WarmStartNet, b_field, h_field, dx: float, critical_an
         fb_it, conv, last2 = relax_device(b_field, dx, dx, critical_angle_deg, max_iterations, tolerance,
                                           runtime=rt)
     torch.cuda.synchronize(rt.device)
-    return WarmStartResult(it, fb_it, conv, used, last, (time.perf_counter() - t0) * 1e3, int(launches[0]))
+    return WarmStartResult(it, fb_it, conv, used, last, (time.perf_counter() - t0) * 1e3, int(launches[0]),
+                           int(settle[0]))

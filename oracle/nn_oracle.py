@@ -191,7 +191,8 @@ def envelope(z: np.ndarray, crit, *, max_sweeps: int = 100000) -> np.ndarray:
     raise RuntimeError("envelope did not converge")
 
 
-def commit_surface(b: np.ndarray, z: np.ndarray, eps: float, corrections: int = 4) -> np.ndarray:
+def commit_surface(b: np.ndarray, z: np.ndarray, eps: float, corrections: int = 4,
+                   return_mask: bool = False):
     """b + (z - b) over the moved cells |z - b| > eps, minus a shift s, heights
     clamped at 0 (float32).  s starts as the mean change over the moved cells;
     each correction adds (mass the clamp creates) / (moved cells above zero),
@@ -208,4 +209,50 @@ def commit_surface(b: np.ndarray, z: np.ndarray, eps: float, corrections: int = 
             shift = np.float32(np.float64(shift) + gain / above)
     out = b.copy()
     out[m] = np.maximum(b[m] + (d[m] - shift), np.float32(0.0))
+    return (out, m) if return_mask else out
+
+
+def commit_masked(b: np.ndarray, z: np.ndarray, mask: np.ndarray, corrections: int = 4) -> np.ndarray:
+    """The masked projection (float32): cells outside ``mask`` take z, the
+    cells of ``mask`` take b + (z - b) - s, clamped at 0, with s balancing the
+    mass of the whole field: s starts as (sum of z - b over all cells) / |mask|
+    and each correction adds (mass the field gains at s) / (mask cells above
+    zero), as in commit_surface."""
+    b = np.asarray(b, np.float32)
+    z = np.asarray(z, np.float32)
+    m = np.asarray(mask, bool)
+    d = z - b
+    outside = float((z[~m].astype(np.float64) - b[~m].astype(np.float64)).sum())
+    total = float((z.astype(np.float64) - b.astype(np.float64)).sum())
+    shift = np.float32(total / m.sum()) if m.any() else np.float32(0.0)
+    for _ in range(corrections):
+        v = np.maximum(b[m] + (d[m] - shift), np.float32(0.0))
+        gain = float((v.astype(np.float64) - b[m].astype(np.float64)).sum()) + outside
+        above = int((v > 0).sum())
+        if above > 0:
+            shift = np.float32(np.float64(shift) + gain / above)
+    out = z.copy()
+    out[m] = np.maximum(b[m] + (d[m] - shift), np.float32(0.0))
+    return out
+
+
+def commit_settled(b: np.ndarray, z: np.ndarray, crit, eps: float, rounds: int, corrections: int = 4,
+                   seam: int = 0) -> np.ndarray:
+    """commit_surface, then (rounds - 1) settle rounds: the envelope of the
+    committed field (fields of ``seam`` rows stacked: each on its own) and the
+    masked projection onto the first commit's moved cells; a round whose
+    envelope changes nothing ends them.  With seam, every field keeps its own
+    shifts."""
+    b = np.asarray(b, np.float32)
+    z = np.asarray(z, np.float32)
+    rows = seam if seam else b.shape[0]
+    out = np.empty_like(b)
+    for r0 in range(0, b.shape[0], rows):
+        cur, mask = commit_surface(b[r0:r0 + rows], z[r0:r0 + rows], eps, corrections, return_mask=True)
+        for _ in range(rounds - 1):
+            env = envelope(cur, np.asarray(crit, np.float32))
+            if np.array_equal(env, cur):
+                break
+            cur = commit_masked(cur, env, mask, corrections)
+        out[r0:r0 + rows] = cur
     return out

@@ -172,3 +172,44 @@ def test_commit_surface_with_floor_clamp_against_oracle():
     assert got.min() >= 0.0 and (got == 0.0).any()
     total = float(b[moved].astype(np.float64).sum())
     assert abs(float(got[moved].astype(np.float64).sum()) - total) <= 1e-6 * total
+
+
+def test_commit_settled_against_oracle():
+    """mv2_nn_commit_settled / mv2_batch_commit_settled against the oracle's
+    settle rounds (float32 shifts from double sums in a different order: the
+    same 2e-7 m bound as the plain commit), single field and stacked fields."""
+    import ctypes
+
+    import torch
+    from isaac_bulk_hip.relaxation import _runtime, legacy_critical_differences
+
+    rt = _runtime("cuda:0")
+    crit = legacy_critical_differences(0.05, 0.05, 34.0)
+    c32 = crit.astype(np.float32)
+    rng = np.random.default_rng(23)
+    fields = []
+    for k in range(3):
+        yy, xx = np.mgrid[0:48, 0:72]
+        cone = (1.2 + 0.1 * k - 0.05 * np.hypot(yy - 24, xx - 30 - 4 * k) * np.tan(np.deg2rad(30)))
+        b = nn_oracle.envelope(cone.astype(np.float32), c32)
+        b[10 + k:30, 20:24 + k] -= 0.25
+        pred = (nn_oracle.envelope(b, c32) - b) * 0.9 + rng.normal(0, 0.003, b.shape).astype(np.float32)
+        z = nn_oracle.envelope((b + pred).astype(np.float32), c32)
+        fields.append((b, z))
+    launches = (ctypes.c_int * 1)()
+    rt.bind_stream()
+    for rounds in (1, 3):
+        b, z = fields[0]
+        bt, zt = _t(b, torch.float32), _t(z, torch.float32)
+        rt.check(rt.lib.mv2_nn_commit_settled(rt.ctx, bt.data_ptr(), zt.data_ptr(), 48, 72, crit.ctypes.data, 1e-5,
+                                              rounds, 256, launches))
+        ref = nn_oracle.commit_settled(b, z, c32, 1e-5, rounds)
+        np.testing.assert_allclose(bt.cpu().numpy(), ref, rtol=0, atol=2e-7)
+        assert (launches[0] == 0) == (rounds == 1)
+    B = np.concatenate([f[0] for f in fields])
+    Z = np.concatenate([f[1] for f in fields])
+    bt, zt = _t(B, torch.float32), _t(Z, torch.float32)
+    rt.check(rt.lib.mv2_batch_commit_settled(rt.ctx, bt.data_ptr(), zt.data_ptr(), 3, 48, 72, crit.ctypes.data, 1e-5,
+                                             3, 256, launches))
+    ref = nn_oracle.commit_settled(B, Z, c32, 1e-5, 3, seam=48)
+    np.testing.assert_allclose(bt.cpu().numpy(), ref, rtol=0, atol=2e-7)

@@ -159,3 +159,59 @@ def test_commit_surface_keeps_mass_when_the_floor_clamps():
     # the plain mean shift alone would have created mass
     plain = nn_oracle.commit_surface(b, z, 1e-5, corrections=0)
     assert float(plain[moved].astype(np.float64).sum()) - total_in > 1e-3
+
+
+def _trench_case():
+    from isaac_bulk_hip.relaxation import legacy_critical_differences
+
+    crit = legacy_critical_differences(0.05, 0.05, 34.0).astype(np.float32)
+    rng = np.random.default_rng(3)
+    yy, xx = np.mgrid[0:64, 0:80]
+    cone = (1.5 - 0.05 * np.hypot(yy - 32, xx - 40) * np.tan(np.deg2rad(30))).astype(np.float32)
+    b = nn_oracle.envelope(cone, crit)
+    b[20:40, 30:34] -= 0.3                                 # a dug trench with over-steep walls
+    pred = (nn_oracle.envelope(b, crit) - b) * 0.9 + rng.normal(0, 0.003, b.shape).astype(np.float32)
+    z = nn_oracle.envelope((b + pred).astype(np.float32), crit)
+    return b, z, crit
+
+
+def _max_excess(f, crit):
+    f = f.astype(np.float64)
+    worst = 0.0
+    for dr, dc, d in nn_oracle.NEIGHBOURS:
+        if dr < 0 or (dr == 0 and dc < 0):
+            continue
+        r1, c1 = f.shape[0] - dr, f.shape[1] - abs(dc)
+        a = f[0:r1, max(0, -dc):max(0, -dc) + c1]
+        o = f[dr:dr + r1, max(0, dc):max(0, dc) + c1]
+        worst = max(worst, float((np.abs(a - o) - crit[d]).max()))
+    return worst
+
+
+def test_masked_commit_balances_the_whole_field():
+    """Outside the mask the field takes z exactly; the shift over the mask pays
+    for every change, so the total is kept to float32 rounding of the shift;
+    with nothing moved outside the mask it reduces to commit_surface."""
+    rng = np.random.default_rng(11)
+    b = rng.uniform(0.5, 1.0, (40, 50)).astype(np.float32)
+    z = (b - np.where(rng.random(b.shape) < 0.2, rng.uniform(0, 0.01, b.shape), 0.0)).astype(np.float32)
+    mask = rng.random(b.shape) < 0.5
+    out = nn_oracle.commit_masked(b, z, mask)
+    np.testing.assert_array_equal(out[~mask], z[~mask])
+    assert abs(out.astype(np.float64).sum() - b.astype(np.float64).sum()) <= 2e-7 * b.size
+    moved = np.abs(z - b) > np.float32(1e-5)
+    np.testing.assert_array_equal(nn_oracle.commit_masked(b, np.where(moved, z, b), moved),
+                                  nn_oracle.commit_surface(b, np.where(moved, z, b), 1e-5))
+
+
+def test_settle_rounds_remove_the_rim_step_and_keep_mass():
+    """The uniform shift of the first commit leaves a step at the rim of the
+    moved cells; each settle round cuts it by orders of magnitude (closed loop:
+    the envelope takes the step, the masked projection returns the mass)."""
+    b, z, crit = _trench_case()
+    ex = [_max_excess(nn_oracle.commit_settled(b, z, crit, 1e-5, r), crit) for r in (1, 2, 3)]
+    assert ex[0] > 1e-3 and ex[1] < 1e-2 * ex[0] and ex[2] < 1e-6
+    np.testing.assert_array_equal(nn_oracle.commit_settled(b, z, crit, 1e-5, 1), nn_oracle.commit_surface(b, z, 1e-5))
+    mass = b.astype(np.float64).sum()
+    for r in (1, 2, 3):
+        assert abs(nn_oracle.commit_settled(b, z, crit, 1e-5, r).astype(np.float64).sum() - mass) <= 1e-7 * mass
