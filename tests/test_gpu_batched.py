@@ -203,9 +203,10 @@ def test_batched_label_matches_oracle(connectivity):
     assert some > 0
 
 
-def test_batched_warm_start_matches_single_field():
+@pytest.mark.parametrize("settle_rounds", [1, 3])
+def test_batched_warm_start_matches_single_field(settle_rounds):
     """The learned warm start of a batch of equal-size terrains equals the
-    single-field warm start of each terrain, bit for bit."""
+    single-field warm start of each terrain, bit for bit (same settle rounds)."""
     import torch
     from isaac_bulk_hip.batched import BatchedMobileV2
     from isaac_bulk_hip.relaxation import _runtime
@@ -220,11 +221,11 @@ def test_batched_warm_start_matches_single_field():
         b, h, _ = cut_face_terrain(96, seed=10 + k, face_deg=45.0)
         terrains.append((b.astype(np.float32), h))
     batch = BatchedMobileV2(terrains, 0.05, runtime=rt)
-    out = batch.warm_start_relax(net, 34.0, 1e-6, max_iterations=5000)
+    out = batch.warm_start_relax(net, 34.0, 1e-6, max_iterations=5000, settle_rounds=settle_rounds)
     assert out["relax"]["converged"].all()
     for i, (b, _) in enumerate(terrains):
         bt = torch.from_numpy(b.copy()).to("cuda:0")
         ht = torch.from_numpy(batch.fields(i)[1].astype(np.float32)).to("cuda:0")
-        single = warm_start_relax(net, bt, ht, 0.05, 34.0, 1e-6, max_iterations=5000)
+        single = warm_start_relax(net, bt, ht, 0.05, 34.0, 1e-6, max_iterations=5000, settle_rounds=settle_rounds)
         assert single.converged
         np.testing.assert_array_equal(batch.fields(i)[0], bt.cpu().numpy())
