@@ -936,9 +936,12 @@ __global__ void __launch_bounds__(kEnvWaves * kWave) envelope_kernel(float *z, i
     // Quiet tiles: a region reads its tile's 3 x 3 tile neighbourhood.  When
     // no interior of those tiles changed in launch k - 1, nothing wrote into
     // the region during it, so its input equals that of the last launch that
-    // processed it, which changed nothing: it is skipped.  Launch 0 runs all.
+    // processed it, which changed nothing: it is skipped.  Launch 0 reads the
+    // flags of envelope_flags_kernel instead (parity 1): a region with no cell
+    // above a neighbour plus its limit in its tile neighbourhood changes
+    // nothing.
     bool quiet = false;
-    if (tile < tiles && k > 0) {
+    if (tile < tiles) {
         const int ty = tiles / tiles_x, trow = tile / tiles_x, tcol = tile % tiles_x;
         int flag = 0;
         if (lane < 9) {
@@ -984,6 +987,41 @@ __global__ void __launch_bounds__(kEnvWaves * kWave) envelope_kernel(float *z, i
     if (moved) changed = 1;     // benign race: every writer stores 1
     __syncthreads();
     if (threadIdx.x == 0 && changed) spread_max(spread + (k % 3) * kSpreadWords, 1.0);
+}
+
+// launch 0's flags: tile t is flagged when one of its cells would be lowered by
+// an update (z(q) + lim < z(p) for a neighbour q across an edge the envelope
+// uses; same float arithmetic); flags of parity 1, cleared first
+__global__ void __launch_bounds__(kBlock) envelope_flags_clear_kernel(int32_t *flags, int tiles)
+{
+    for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < tiles; t += gridDim.x * blockDim.x) flags[tiles + t] = 0;
+}
+
+__global__ void __launch_bounds__(kBlock) envelope_flags_kernel(const float *z, int ny, int nx, float c0, float c1,
+                                                                float c2, float c3, int tiles_x, int tiles,
+                                                                int seam, int32_t *flags)
+{
+    const int64_t n = (int64_t)ny * nx;
+    for (int64_t p = (int64_t)xcd_block() * kBlock + threadIdx.x; p < n; p += (int64_t)gridDim.x * kBlock) {
+        const int r = (int)(p / nx), c = (int)(p - (int64_t)r * nx);
+        const float v = z[p];
+        const bool up_ok = r > 0 && !(seam > 0 && r % seam == 0);
+        const bool dn_ok = r + 1 < ny && !(seam > 0 && (r + 1) % seam == 0);
+        bool low = false;
+        if (c > 0) low |= z[p - 1] + c0 < v;
+        if (c + 1 < nx) low |= z[p + 1] + c0 < v;
+        if (up_ok) {
+            low |= z[p - nx] + c1 < v;
+            if (c > 0) low |= z[p - nx - 1] + c2 < v;
+            if (c + 1 < nx) low |= z[p - nx + 1] + c3 < v;
+        }
+        if (dn_ok) {
+            low |= z[p + nx] + c1 < v;
+            if (c + 1 < nx) low |= z[p + nx + 1] + c2 < v;
+            if (c > 0) low |= z[p + nx - 1] + c3 < v;
+        }
+        if (low) flags[tiles + (r / kEnvTile) * tiles_x + c / kEnvTile] = 1;   // every writer stores 1
+    }
 }
 
 __global__ void envelope_reset_kernel(EnvControl *ctl, unsigned long long *spread)
@@ -1307,6 +1345,11 @@ static int envelope_run(mv2_context *ctx, const float *b, const float *delta, fl
     const int blocks = (tiles + kEnvWaves - 1) / kEnvWaves;
     if (int rc = ensure_scratch(ctx, ctx->env_moved, 2 * sizeof(int32_t) * (size_t)tiles)) return rc;
     int32_t *tile_moved = (int32_t *)ctx->env_moved.ptr;
+    hipLaunchKernelGGL(envelope_flags_clear_kernel, dim3((tiles + kBlock - 1) / kBlock), dim3(kBlock), 0, st,
+                       tile_moved, tiles);
+    hipLaunchKernelGGL(envelope_flags_kernel, dim3(grid_for(n, kBlock, 4096)), dim3(kBlock), 0, st, z, ny, nx,
+                       (float)crit_diff[0], (float)crit_diff[1], (float)crit_diff[2], (float)crit_diff[3], tiles_x,
+                       tiles, seam, tile_moved);
     // cycles of the four passes per region and launch: more settle a region
     // in one launch, fewer bound the launch by its slowest region (a launch
     // lasts as long as its busiest region); MV2_ENV_CYCLES = 1..6 overrides
