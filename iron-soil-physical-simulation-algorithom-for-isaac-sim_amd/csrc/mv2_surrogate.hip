@@ -989,12 +989,14 @@ __global__ void __launch_bounds__(kEnvWaves * kWave) envelope_kernel(float *z, i
     if (threadIdx.x == 0 && changed) spread_max(spread + (k % 3) * kSpreadWords, 1.0);
 }
 
-// launch 0's flags: tile t is flagged when one of its cells would be lowered by
-// an update (z(q) + lim < z(p) for a neighbour q across an edge the envelope
-// uses; same float arithmetic); flags of parity 1, cleared first
-__global__ void __launch_bounds__(kBlock) envelope_flags_clear_kernel(int32_t *flags, int tiles)
+// launch 0's flags (parity 1): cleared to 0 and then set for every tile with a
+// cell that an update would lower (z(q) + lim < z(p) for a neighbour q across
+// an edge the envelope uses; same float arithmetic), or all set to 1 (every
+// region runs)
+__global__ void __launch_bounds__(kBlock) envelope_flags_clear_kernel(int32_t *flags, int tiles, int value)
 {
-    for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < tiles; t += gridDim.x * blockDim.x) flags[tiles + t] = 0;
+    for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < tiles; t += gridDim.x * blockDim.x)
+        flags[tiles + t] = value;
 }
 
 __global__ void __launch_bounds__(kBlock) envelope_flags_kernel(const float *z, int ny, int nx, float c0, float c1,
@@ -1257,12 +1259,12 @@ extern "C" int mv2_nn_commit_surface(mv2_context *ctx, float *b, const float *z,
 }
 
 static int envelope_run(mv2_context *ctx, const float *b, const float *delta, float *z, int ny, int nx,
-                        const double crit_diff[4], int max_launches, int *launches_out, int seam);
+                        const double crit_diff[4], int max_launches, int *launches_out, int seam, bool prior);
 
 extern "C" int mv2_envelope(mv2_context *ctx, const float *b, const float *delta, float *z, int ny, int nx,
                             const double crit_diff[4], int max_launches, int *launches_out)
 {
-    return envelope_run(ctx, b, delta, z, ny, nx, crit_diff, max_launches, launches_out, 0);
+    return envelope_run(ctx, b, delta, z, ny, nx, crit_diff, max_launches, launches_out, 0, true);
 }
 
 extern "C" int mv2_batch_envelope(mv2_context *ctx, const float *b, const float *delta, float *z, int count,
@@ -1270,7 +1272,7 @@ extern "C" int mv2_batch_envelope(mv2_context *ctx, const float *b, const float 
 {
     if (count < 1 || (int64_t)count * ny >= ((int64_t)1 << 31))
         return set_error(ctx, MV2_EINVAL, "[WarmStart] invalid batch of envelopes");
-    return envelope_run(ctx, b, delta, z, count * ny, nx, crit_diff, max_launches, launches_out, ny);
+    return envelope_run(ctx, b, delta, z, count * ny, nx, crit_diff, max_launches, launches_out, ny, false);
 }
 
 extern "C" int mv2_batch_commit_surface(mv2_context *ctx, float *b, const float *z, int count, int64_t n, double eps)
@@ -1300,7 +1302,8 @@ static int commit_settled(mv2_context *ctx, float *b, float *z, int count, int n
     for (int r = 1; r < rounds; r++) {
         MV2_HIP(ctx, hipMemcpyAsync(z, b, sizeof(float) * (size_t)(n * count), hipMemcpyDeviceToDevice, ctx->stream));
         int l = 0;
-        if (int rc = envelope_run(ctx, nullptr, nullptr, z, count * ny, nx, crit_diff, max_launches, &l, ny))
+        if (int rc = envelope_run(ctx, nullptr, nullptr, z, count * ny, nx, crit_diff, max_launches, &l, ny,
+                                  count == 1))
             return rc;
         total += l;
         if (l <= 1) break;               // nothing moved: the field is its own envelope
@@ -1324,7 +1327,7 @@ extern "C" int mv2_batch_commit_settled(mv2_context *ctx, float *b, float *z, in
 }
 
 static int envelope_run(mv2_context *ctx, const float *b, const float *delta, float *z, int ny, int nx,
-                        const double crit_diff[4], int max_launches, int *launches_out, int seam)
+                        const double crit_diff[4], int max_launches, int *launches_out, int seam, bool prior)
 {
     if (int rc = check_plain(ctx)) return rc;
     if (!z || !crit_diff || ny < 1 || nx < 1 || max_launches < 1 || (delta && !b))
@@ -1345,11 +1348,17 @@ static int envelope_run(mv2_context *ctx, const float *b, const float *delta, fl
     const int blocks = (tiles + kEnvWaves - 1) / kEnvWaves;
     if (int rc = ensure_scratch(ctx, ctx->env_moved, 2 * sizeof(int32_t) * (size_t)tiles)) return rc;
     int32_t *tile_moved = (int32_t *)ctx->env_moved.ptr;
+    // launch 0 of one field runs only the regions next to a cell that an update
+    // would lower (a flag pass: 701^2 learned relax-to-rest 0.50 against
+    // 0.52 ms); for a batch of fields, whose predictions leave such cells in
+    // most regions, the pass costs more than it skips (64 x 256^2: 22 us per
+    // pass against no shorter first launch), so every region runs
     hipLaunchKernelGGL(envelope_flags_clear_kernel, dim3((tiles + kBlock - 1) / kBlock), dim3(kBlock), 0, st,
-                       tile_moved, tiles);
-    hipLaunchKernelGGL(envelope_flags_kernel, dim3(grid_for(n, kBlock, 4096)), dim3(kBlock), 0, st, z, ny, nx,
-                       (float)crit_diff[0], (float)crit_diff[1], (float)crit_diff[2], (float)crit_diff[3], tiles_x,
-                       tiles, seam, tile_moved);
+                       tile_moved, tiles, prior ? 0 : 1);
+    if (prior)
+        hipLaunchKernelGGL(envelope_flags_kernel, dim3(grid_for(n, kBlock, 4096)), dim3(kBlock), 0, st, z, ny, nx,
+                           (float)crit_diff[0], (float)crit_diff[1], (float)crit_diff[2], (float)crit_diff[3],
+                           tiles_x, tiles, seam, tile_moved);
     // cycles of the four passes per region and launch: more settle a region
     // in one launch, fewer bound the launch by its slowest region (a launch
     // lasts as long as its busiest region); MV2_ENV_CYCLES = 1..6 overrides
