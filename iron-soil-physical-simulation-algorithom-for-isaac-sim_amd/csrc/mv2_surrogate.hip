@@ -830,11 +830,16 @@ __device__ inline float env_from_next_lane(float v)     // lane l gets lane l + 
 
 // one directional pass over the wave's region.  HORZ: lane = region row, the
 // walk goes over columns (FWD: left to right); otherwise lane = region column
-// and the walk goes over rows (FWD: top to bottom).  valid_* are bit masks of
-// the region rows/columns inside the grid; cut_above / cut_below mark region
-// rows whose edge to the row above / below crosses a seam between stacked
-// fields.  Returns whether this lane lowered a cell; *moved_interior is set
-// when that cell is in the tile's interior.
+// and the walk goes over rows (FWD: top to bottom).  Each cell is tested
+// against the three cells one step back along the walk, already updated in
+// this pass (the lane's own previous value and its two neighbour lanes' by
+// DPP): a pass carries information along its three forward moves, and the four
+// passes of a cycle test every cell against all eight neighbours, so a cycle
+// that changes nothing leaves a fixed point of the region.  valid_* are bit
+// masks of the region rows/columns inside the grid; cut_above / cut_below mark
+// region rows whose edge to the row above / below crosses a seam between
+// stacked fields.  Returns whether this lane lowered a cell; *moved_interior is
+// set when that cell is in the tile's interior.
 template <bool HORZ, bool FWD>
 __device__ inline bool env_pass(float *t, int lane, uint64_t valid_rows, uint64_t valid_cols, uint64_t cut_above,
                                 uint64_t cut_below, float c0, float c1, float c2, float c3, bool *moved_interior)
@@ -843,49 +848,30 @@ __device__ inline bool env_pass(float *t, int lane, uint64_t valid_rows, uint64_
     auto at = [&](int s) -> float * { return HORZ ? &t[lane * kEnvStride + s] : &t[s * kEnvStride + lane]; };
     const bool lane_valid = ((HORZ ? valid_rows : valid_cols) >> lane) & 1;
     const bool lane_inner = lane >= kEnvHalo && lane < kEnvHalo + kEnvTile;
-    // seams cut the edges to the lanes above / below (HORZ: lane = row)
+    // HORZ: seams cut the edges to the lanes above / below (lane = row)
     const bool lane_cut_above = HORZ && ((cut_above >> lane) & 1);
     const bool lane_cut_below = HORZ && ((cut_below >> lane) & 1);
+    // limits of the moves from the previous step's lane l - 1 / l + 1:
+    // vertical down (r-1, c-1) d2, (r-1, c+1) d3; up (r+1, c-1) d3, (r+1, c+1) d2;
+    // horizontal right (r-1, c-1) d2, (r+1, c-1) d3; left (r-1, c+1) d3, (r+1, c+1) d2
+    const float lim_back = HORZ ? c0 : c1;
+    const float lim_p = FWD ? c2 : c3, lim_n = FWD ? c3 : c2;
     bool changed = false;
     float prev = INF;                                   // updated value one step back
     float cur = *at(FWD ? 0 : kEnvRegion - 1);
     for (int k = 0; k < kEnvRegion; k++) {
         const int s = FWD ? k : kEnvRegion - 1 - k;
         const float next = k + 1 < kEnvRegion ? *at(FWD ? s + 1 : s - 1) : INF;
-        const float before = FWD ? prev : next;         // step s - 1 (row above / column left)
-        const float after = FWD ? next : prev;          // step s + 1 (row below / column right)
-        const float cur_p = env_from_prev_lane(cur), cur_n = env_from_next_lane(cur);
-        const float bef_p = env_from_prev_lane(before), bef_n = env_from_next_lane(before);
-        const float aft_p = env_from_prev_lane(after), aft_n = env_from_next_lane(after);
+        const float prev_p = env_from_prev_lane(prev), prev_n = env_from_next_lane(prev);
         float m = cur;
         if (HORZ) {
-            // lane = row r, step = column c: before = (r, c-1), after = (r, c+1)
-            m = fminf(m, before + c0);
-            m = fminf(m, after + c0);
-            if (!lane_cut_above) {
-                m = fminf(m, cur_p + c1);               // (r-1, c)
-                m = fminf(m, bef_p + c2);               // (r-1, c-1)
-                m = fminf(m, aft_p + c3);               // (r-1, c+1)
-            }
-            if (!lane_cut_below) {
-                m = fminf(m, cur_n + c1);               // (r+1, c)
-                m = fminf(m, bef_n + c3);               // (r+1, c-1)
-                m = fminf(m, aft_n + c2);               // (r+1, c+1)
-            }
-        } else {
-            // lane = column c, step = row r: before = (r-1, c), after = (r+1, c)
-            m = fminf(m, cur_p + c0);                   // (r, c-1)
-            m = fminf(m, cur_n + c0);                   // (r, c+1)
-            if (!((cut_above >> s) & 1)) {
-                m = fminf(m, before + c1);
-                m = fminf(m, bef_p + c2);               // (r-1, c-1)
-                m = fminf(m, bef_n + c3);               // (r-1, c+1)
-            }
-            if (!((cut_below >> s) & 1)) {
-                m = fminf(m, after + c1);
-                m = fminf(m, aft_p + c3);               // (r+1, c-1)
-                m = fminf(m, aft_n + c2);               // (r+1, c+1)
-            }
+            m = fminf(m, prev + lim_back);
+            if (!lane_cut_above) m = fminf(m, prev_p + lim_p);
+            if (!lane_cut_below) m = fminf(m, prev_n + lim_n);
+        } else if (!(((FWD ? cut_above : cut_below) >> s) & 1)) {
+            m = fminf(m, prev + lim_back);
+            m = fminf(m, prev_p + lim_p);
+            m = fminf(m, prev_n + lim_n);
         }
         const bool step_valid = ((HORZ ? valid_cols : valid_rows) >> s) & 1;
         if (lane_valid && step_valid && m < cur) {
