@@ -26,23 +26,48 @@
 namespace mv2 {
 
 // ---- entrainment ------------------------------------------------------------
+// One thread per (instance, cell) request.  The quanta moved are summed per
+// instance: the lanes of a wave that name the same instance (request lists are
+// grouped by instance, so a wave usually holds one or two) add theirs with a
+// wave reduction and one atomic, instead of one atomic per cell on the few
+// counters of the batch.
 __global__ void __launch_bounds__(kBlock) batch_entrain_kernel(const float *z_base, float *b, int32_t *h,
                                                                const int64_t *inst_off, const int32_t *inst,
                                                                const int32_t *cell, const double *req, int n,
                                                                unsigned long long *moved)
 {
-    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
-        const int64_t i = inst_off[inst[k]] + cell[k];
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    int my_inst = -1;
+    long long dq = 0;
+    if (k < n) {
+        my_inst = inst[k];
+        const int64_t i = inst_off[my_inst] + cell[k];
         const double zb = z_base ? (double)z_base[i] : 0.0;
         const double available = fmax((double)b[i] - zb, 0.0);
         const double depth = fmin(fmax(req[k], 0.0), available);
-        long long dq = (long long)floor(depth * kInvQuantum);
+        dq = (long long)floor(depth * kInvQuantum);
         const long long room = 2147483647LL - (long long)h[i];
         if (dq > room) dq = room;
-        if (dq <= 0) continue;
-        b[i] = (float)((double)b[i] - (double)dq * kQuantum);
-        h[i] = (int32_t)((long long)h[i] + dq);
-        atomicAdd(&moved[inst[k]], (unsigned long long)dq);
+        if (dq > 0) {
+            b[i] = (float)((double)b[i] - (double)dq * kQuantum);
+            h[i] = (int32_t)((long long)h[i] + dq);
+        } else {
+            dq = 0;
+        }
+    }
+    // per distinct instance of the wave: sum its lanes' quanta, one atomic
+    const int lane = threadIdx.x & (kWave - 1);
+    bool pending = my_inst >= 0;
+    for (;;) {
+        const unsigned long long active = __ballot(pending);
+        if (!active) break;
+        const int leader = __ffsll((long long)active) - 1;
+        const int target = __shfl(my_inst, leader);
+        long long v = pending && my_inst == target ? dq : 0;
+#pragma unroll
+        for (int off = kWave / 2; off > 0; off >>= 1) v += __shfl_xor(v, off);
+        if (lane == leader && v > 0) atomicAdd(&moved[target], (unsigned long long)v);
+        if (my_inst == target) pending = false;
     }
 }
 
@@ -359,7 +384,7 @@ extern "C" int mv2_batch_entrain(mv2_context *ctx, const float *z_base, const in
     hipStream_t st = ctx->stream;
     hipLaunchKernelGGL(zero_u64, dim3(grid_for((int64_t)cnt, kBlock, 64)), dim3(kBlock), 0, st, moved, bs->count);
     if (n > 0)
-        hipLaunchKernelGGL(batch_entrain_kernel, dim3(grid_for(n, kBlock, 1024)), dim3(kBlock), 0, st, z_base, bs->b,
+        hipLaunchKernelGGL(batch_entrain_kernel, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, st, z_base, bs->b,
                            bs->h, batch_inst_tables(bs).off, inst, cell, depth, n, moved);
     MV2_HIP(ctx, hipGetLastError());
     if (int rc = ensure_scratch(ctx, ctx->host_result, sizeof(unsigned long long) * cnt + 4096, true)) return rc;
