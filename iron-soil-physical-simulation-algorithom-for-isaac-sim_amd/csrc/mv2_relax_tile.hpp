@@ -63,7 +63,8 @@ static_assert(kRelaxHalo % 2 == 0 && kRelaxTH % 2 == 0 && kRelaxTW % 2 == 0,
 // operands a thread needs, then computes, then stores: the edges of one parity
 // are disjoint, so the loads of one thread's edges never depend on its own
 // stores.  All threads of the block must call it.  With `changed`, the thread
-// also reports whether any interior cell it writes differs from its value in hs.
+// also reports whether any interior cell it writes differs from its value in hs
+// (compared with the staged copy, not re-read).
 template <class R>
 __device__ inline R relax_tile_iteration(const R *hs, R *hd, int ny, int nx, const R (&crits)[4], int r0, int c0,
                                          R *tile, bool *changed = nullptr)
@@ -83,6 +84,15 @@ __device__ inline R relax_tile_iteration(const R *hs, R *hd, int ny, int nx, con
         for (int j = 0; j < kRelaxStage; j++) tile[threadIdx.x + j * kRelaxBlock] = v[j];
     }
     __syncthreads();
+    // the interior cells this thread writes back, as staged (change tracking)
+    R orig[kRelaxOwned];
+    if (changed) {
+#pragma unroll
+        for (int j = 0; j < kRelaxOwned; j++) {
+            const int e = threadIdx.x + j * kRelaxBlock;
+            orig[j] = tile[(e / kRelaxTW + kRelaxHalo) * kRelaxLW + e % kRelaxTW + kRelaxHalo];
+        }
+    }
     R tmax = (R)0;
     // the 8 sub-phases unrolled: the direction and parity are compile-time
     // constants in each, so the edge geometry folds into the addressing
@@ -166,7 +176,7 @@ __device__ inline R relax_tile_iteration(const R *hs, R *hd, int ny, int nx, con
         if (r < ny && c < nx) {
             const R v = tile[(ir + kRelaxHalo) * kRelaxLW + ic + kRelaxHalo];
             hd[r * nx + c] = v;
-            if (changed && v != hs[r * nx + c]) *changed = true;
+            if (changed && v != orig[j]) *changed = true;
         }
     }
     return tmax;
