@@ -698,11 +698,9 @@ def main() -> int:
         out["cpu_baseline"] = cpu_baseline(b0, h0, args.cpu_sample_frames)
     print(json.dumps(out), flush=True)
     # release device resources before interpreter teardown (profilers hook exit)
-    from isaac_bulk_hip import relaxation as _relaxation
+    from isaac_bulk_hip.runtime import close_all_runtimes
 
-    torch.cuda.synchronize()
-    for rt in list(_relaxation._RUNTIMES.values()) + [state.runtime]:
-        rt.close()
+    close_all_runtimes()
     return 0
 
 

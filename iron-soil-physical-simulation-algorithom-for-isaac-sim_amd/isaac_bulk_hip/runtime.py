@@ -7,7 +7,10 @@ transfers, and hands raw pointers plus the current HIP stream to the C-ABI.
 
 from __future__ import annotations
 
+import atexit
 import ctypes
+import sys
+import weakref
 from dataclasses import dataclass
 
 import numpy as np
@@ -51,6 +54,44 @@ def _torch():
     return torch
 
 
+# Live runtimes in creation order.  At interpreter exit they are closed by an
+# atexit hook registered when the first runtime is created -- after torch was
+# imported, so it runs before torch's own exit handlers and while the HIP
+# runtime and the borrowed torch streams still exist.  Closing from __del__
+# during interpreter finalisation instead (module globals torn down in
+# arbitrary order, after torch/HIP teardown) synchronised a stream handle that
+# no longer existed and freed device memory after the runtime's own teardown;
+# that is what crashed profiled runs in exit/__cxa_finalize.
+_LIVE: list = []
+_EXIT_HOOK = False
+
+
+def close_all_runtimes() -> None:
+    """Close every live runtime, newest first, after draining the device."""
+    live = [r() for r in reversed(_LIVE)]
+    _LIVE.clear()
+    live = [r for r in live if r is not None and getattr(r, "ctx", None)]
+    if not live:
+        return
+    try:
+        _torch().cuda.synchronize()
+    except Exception:
+        pass
+    for rt in live:
+        try:
+            rt.close()
+        except Exception:
+            pass
+
+
+def _register(rt) -> None:
+    global _EXIT_HOOK
+    if not _EXIT_HOOK:
+        atexit.register(close_all_runtimes)
+        _EXIT_HOOK = True
+    _LIVE.append(weakref.ref(rt))
+
+
 class HipRuntime:
     """Named device arrays on one MI355X plus the libmv2 context that computes on them."""
 
@@ -70,6 +111,7 @@ class HipRuntime:
         self.ctx = handle.value
         self.arrays: dict[str, object] = {}
         self.telemetry = TransferTelemetry()
+        _register(self)
 
     # -- stream / sync ---------------------------------------------------------
     def bind_stream(self) -> None:
@@ -116,11 +158,19 @@ class HipRuntime:
         return int(t.data_ptr())
 
     def close(self) -> None:
+        """Release the library context: the stream handle is dropped first (the
+        library then drains the whole device), then its scratch is freed."""
         if getattr(self, "ctx", None):
-            self.lib.mv2_destroy(self.ctx)
-            self.ctx = None
+            ctx, self.ctx = self.ctx, None
+            self.lib.mv2_set_stream(ctx, None)
+            self.lib.mv2_destroy(ctx)
 
     def __del__(self) -> None:  # pragma: no cover - interpreter shutdown order
+        # Once the interpreter is finalising, HIP and torch may already be torn
+        # down: make no device call (the driver reclaims the memory at exit).
+        if sys.is_finalizing():
+            self.ctx = None
+            return
         try:
             self.close()
         except Exception:

@@ -200,6 +200,19 @@ def vertex_weights(ny, nx, dx, dy) -> np.ndarray:
     return w
 
 
+def vertex_weights_fast(ny, nx, dx, dy) -> np.ndarray:
+    """Triangle-A-C vertex weights as whole-array sums
+    (DeviceBulkState._vertex_weights, bulk_state_authority.py:624-634);
+    equal to ``vertex_weights`` cell by cell."""
+    w = np.zeros((ny, nx))
+    f = dx * dy / 6.0
+    w[:-1, :-1] += 2.0 * f
+    w[:-1, 1:] += f
+    w[1:, 1:] += 2.0 * f
+    w[1:, :-1] += f
+    return w
+
+
 def v2_summary(b, h, qx, qy, *, dx, dy, weight_mode, density, K=0.45, g=9.81, dry=1.0e-10):
     b, h, qx, qy = (_f64(v) for v in (b, h, qx, qy))
     out = np.zeros(7)

@@ -468,3 +468,42 @@ def test_mobilize_closed_forms():
     assert d[11] == pytest.approx(float(np.sum((expect_t * weights)[moved & ~first])), rel=1e-13)
     assert d[12] == pytest.approx(float(weights[first].sum()), rel=1e-14)
     assert d[13] == pytest.approx(float(weights[moved & ~first].sum()), rel=1e-14)
+
+
+def test_vertex_weights_fast_equals_the_per_cell_restatement():
+    for ny, nx, dx, dy in ((7, 9, 0.05, 0.05), (12, 5, 0.1, 0.04), (2, 2, 0.05, 0.07)):
+        np.testing.assert_array_equal(oracle.vertex_weights_fast(ny, nx, dx, dy),
+                                      oracle.vertex_weights(ny, nx, dx, dy))
+        # the weights tile the rectangle: their sum is its area (ny-1)(nx-1) dx dy
+        assert float(oracle.vertex_weights_fast(ny, nx, dx, dy).sum()) == pytest.approx(
+            (ny - 1) * (nx - 1) * dx * dy, rel=1e-14)
+
+
+def test_frame_sequence_keeps_mass_and_ends_each_frame_relaxed():
+    """The FP64 frame of the full-size workload at a small size: the V2 layer
+    keeps its mass, b_eff + h is kept by the relaxation, and every frame ends
+    with no legacy edge above the critical difference by more than the
+    tolerance (or at the iteration cap)."""
+    from frame_sequence import OracleFrameSequence
+    from isaac_bulk_hip.workloads import cut_face_terrain
+
+    b, h, _ = cut_face_terrain(64, 0.05, seed=2)
+    seq = OracleFrameSequence(b, h, dx=0.05, dt=1.0 / 60.0, v2={"limit_mode": 1}, relax_angle_deg=34.0,
+                              relax_max_iterations=1000, relax_tolerance=1.0e-6,
+                              avalanche={"tan_start": float(np.tan(np.deg2rad(34.0))),
+                                         "tan_stop": float(np.tan(np.deg2rad(30.0))), "density": 1370.0,
+                                         "gravity": 9.81, "cohesion": 0.0, "mobilization_depth": 0.08,
+                                         "dry": 1.0e-8, "activity_speed": 0.05, "release": True})
+    crit = float(np.tan(np.deg2rad(34.0)))
+    diag = float(np.hypot(0.05, 0.05))
+    for _ in range(4):
+        rec = seq.step()
+        assert rec["substeps"] > 0
+        assert seq.mass_error() <= 1e-14
+        assert seq.total_error() <= 1e-13
+        if rec["relax_converged"]:
+            bb = seq.b
+            for a, c, d in ((bb[:-1, :], bb[1:, :], 0.05), (bb[:, :-1], bb[:, 1:], 0.05),
+                            (bb[:-1, :-1], bb[1:, 1:], diag), (bb[:-1, 1:], bb[1:, :-1], diag)):
+                assert float(np.max(np.abs(a - c) - crit * d)) <= 1e-6 + 1e-12
+        np.testing.assert_array_equal(rec["labels"], oracle.label(rec["unstable"], 8))
