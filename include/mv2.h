@@ -123,6 +123,16 @@ int mv2_deposit(mv2_context *ctx, const int32_t *idx, const double *depth, int n
 /* DeviceBulkState.reservoir_reduction (bulk_state_authority.py:60-75, :1004-1034):
  * out = resting volume, mobile volume, density*px, density*py */
 int mv2_reservoirs(mv2_context *ctx, int weight_mode, double density, double out[4]);
+/* Allocation-free variant of mv2_reservoirs.  No counterpart in the reference:
+ * it is for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  Same
+ * kernels and sums as mv2_reservoirs, bit for bit; out_device is a DEVICE
+ * double[4] (same order) written asynchronously on the context's stream (no
+ * sync, no host copy).  Its workgroup partials live in a file-scope device
+ * array of 256 x 4 doubles (8 KiB per GPU, held until the process ends) that
+ * every grid size fits; the library allocates nothing in this call.  Calls
+ * that overlap on one GPU share that array and race. */
+int mv2_reservoirs_noalloc(mv2_context *ctx, int weight_mode, double density, double *out_device);
 
 /* ---- surface publication: DeviceBulkState.capture_surface_for_dirty_tracking
  *      / collect_surface_dirty_tiles (pkg/runtime/bulk_state_authority.py:733-781,

@@ -97,6 +97,11 @@ extern "C" int mv2_create(int device, mv2_context **out)
     if (int rc = ensure_scratch(ctx, ctx->sync, 4096)) { delete ctx; return rc; }
     if (int rc = ensure_scratch(ctx, ctx->result, 4096)) { delete ctx; return rc; }
     if (int rc = ensure_scratch(ctx, ctx->host_result, 4096, true)) { delete ctx; return rc; }
+    // with `device` current: the per-device address of the allocation-free scratch
+    if (int rc = noalloc_scratch_address(ctx, &ctx->noalloc_scratch)) {
+        mv2_destroy(ctx);
+        return rc;
+    }
     if (hipEventCreate(&ctx->ev_start) != hipSuccess || hipEventCreate(&ctx->ev_stop) != hipSuccess) {
         delete ctx;
         return set_error(nullptr, MV2_EDEVICE, "hipEventCreate failed");

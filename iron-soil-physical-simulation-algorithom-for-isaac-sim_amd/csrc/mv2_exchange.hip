@@ -19,6 +19,30 @@ namespace mv2 {
 
 constexpr int kExGrid = 256;
 
+}  // namespace mv2
+
+// Workgroup partials of the allocation-free ledger (mv2_reservoirs_noalloc):
+// kExGrid partials of 4 values, every grid size folds into at most kExGrid
+// workgroups.  Each GPU holds its own copy for the life of the process; host
+// code never uses this name as an address -- mv2_create resolves the device
+// address once per context (noalloc_scratch_address) and the entry point
+// passes that pointer to the kernels.
+static __device__ __align__(256) double scratch[4 * mv2::kExGrid];
+
+namespace mv2 {
+
+int noalloc_scratch_address(mv2_context *ctx, double **out)
+{
+    void *p = nullptr;
+    hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) {
+        *out = nullptr;
+        return check_hip(ctx, e, "hipGetSymbolAddress(scratch)");
+    }
+    *out = static_cast<double *>(p);
+    return MV2_OK;
+}
+
 __device__ inline double dmaxd(double a, double b) { return a >= b ? a : b; }
 __device__ inline double dmind(double a, double b) { return a <= b ? a : b; }
 
@@ -259,4 +283,35 @@ extern "C" int mv2_reservoirs(mv2_context *ctx, int weight_mode, double density,
                            (double *)ctx->partials.ptr);
     MV2_HIP(ctx, hipGetLastError());
     return finish(ctx, 4, grid, out);
+}
+
+// Allocation-free variant of mv2_reservoirs: the same kernels and grid (so the
+// same sums, bit for bit), the workgroup partials in the file-scope array and
+// the folded result written straight to the caller's device buffer.  No
+// allocation, no host copy, no stream sync; every partial that the fold reads
+// is written first by its workgroup, so nothing needs clearing.
+extern "C" int mv2_reservoirs_noalloc(mv2_context *ctx, int weight_mode, double density, double *out_device)
+{
+    if (int rc = check_bound(ctx, "[DeviceBulkState]")) return rc;
+    if (!out_device || !(density > 0.0))
+        return set_error(ctx, MV2_EINVAL, "[DeviceBulkState] density must be finite/positive");
+    if (!ctx->noalloc_scratch)
+        return set_error(ctx, MV2_EDEVICE, "[DeviceBulkState] allocation-free scratch unavailable on device %d",
+                         ctx->device);
+    const int64_t n = (int64_t)ctx->ny * ctx->nx;
+    const int grid = grid_for(n, kBlock, kExGrid);
+    if (grid > kExGrid)
+        return set_error(ctx, MV2_EINVAL, "[DeviceBulkState] %d ledger partials exceed the allocation-free "
+                         "scratch (%d)", grid, kExGrid);
+    double *partials = ctx->noalloc_scratch;
+    const double area = ctx->dx * ctx->dy, f6 = area / 6.0;
+    if (ctx->precision == MV2_FP64)
+        hipLaunchKernelGGL(reservoir_kernel<false>, dim3(grid), dim3(kBlock), 0, ctx->stream, ctx->z_base, ctx->b_eff,
+                           ctx->mobile, ctx->qx, ctx->qy, ctx->ny, ctx->nx, area, f6, weight_mode, density, partials);
+    else
+        hipLaunchKernelGGL(reservoir_kernel<true>, dim3(grid), dim3(kBlock), 0, ctx->stream, ctx->z_base, ctx->b_eff,
+                           ctx->mobile, ctx->qx, ctx->qy, ctx->ny, ctx->nx, area, f6, weight_mode, density, partials);
+    hipLaunchKernelGGL(reduce_partials<4>, dim3(1), dim3(kBlock), 0, ctx->stream, partials, grid, out_device);
+    MV2_HIP(ctx, hipGetLastError());
+    return MV2_OK;
 }

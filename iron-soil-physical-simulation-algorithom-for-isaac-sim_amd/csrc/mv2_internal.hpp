@@ -69,6 +69,7 @@ struct mv2_context {
     int relax_predicted = 0;              // relaxation launches to enqueue up front
     int frontier_predicted = 0;           // frontier rounds to enqueue up front
     mv2_batch_state *batch = nullptr;     // batched terrains bound to this context
+    double *noalloc_scratch = nullptr;    // device address of the allocation-free ledger's partials
 };
 
 void mv2_batch_release(mv2_context *ctx);
@@ -78,6 +79,8 @@ namespace mv2 {
 int set_error(mv2_context *ctx, int code, const char *fmt, ...);
 int check_hip(mv2_context *ctx, hipError_t err, const char *what);
 int ensure_scratch(mv2_context *ctx, Scratch &s, size_t bytes, bool host_pinned = false);
+// device address of the allocation-free ledger's file-scope partials (mv2_exchange.hip)
+int noalloc_scratch_address(mv2_context *ctx, double **out);
 
 #define MV2_HIP(ctx, expr)                                     \
     do {                                                       \

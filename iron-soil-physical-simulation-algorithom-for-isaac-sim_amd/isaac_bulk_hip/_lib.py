@@ -130,6 +130,7 @@ SIGNATURES: dict[str, tuple[object, list[object]]] = {
     "mv2_entrain": (_I, [_P, _P, _P, _I, _I, ctypes.POINTER(_D)]),
     "mv2_deposit": (_I, [_P, _P, _P, _I, _I, _D, _P]),
     "mv2_reservoirs": (_I, [_P, _I, _D, _P]),
+    "mv2_reservoirs_noalloc": (_I, [_P, _I, _D, _P]),
     "mv2_surface_capture": (_I, [_P, _P]),
     "mv2_surface_dirty_tiles": (_I, [_P, _P, _D, _I, _I, _I, _P, _P, ctypes.POINTER(_I)]),
     "mv2_surface_sample": (_I, [_P, _P, _I, _P]),

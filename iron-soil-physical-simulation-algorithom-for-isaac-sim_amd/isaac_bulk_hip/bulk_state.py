@@ -279,6 +279,22 @@ class DeviceBulkState:
         return {"resting_volume_m3": float(out[0]), "mobile_volume_m3": float(out[1]),
                 "mobile_momentum_kg_m_s": np.asarray([out[2], out[3], 0.0])}
 
+    def reservoir_reduction_into(self, out, density_kg_m3: float) -> None:
+        """The sums of ``reservoir_reduction`` written asynchronously into the
+        device float64 tensor ``out[4]`` (resting, mobile, density * px,
+        density * py) without allocating device memory or synchronising --
+        usable under HIP graph capture.  No counterpart in the reference."""
+        torch = __import__("torch")
+        density = float(density_kg_m3)
+        if not np.isfinite(density) or density <= 0.0:
+            raise ValueError("[DeviceBulkState] density must be finite/positive")
+        if (out.dtype != torch.float64 or out.device != self.runtime.device or out.numel() < 4
+                or not out.is_contiguous()):
+            raise ValueError("[DeviceBulkState] out must be a contiguous float64 device tensor of >= 4 values")
+        rt = self.runtime
+        rt.bind_stream()
+        rt.check(rt.lib.mv2_reservoirs_noalloc(rt.ctx, 1, density, out.data_ptr()))
+
     # -- dirty tiles -----------------------------------------------------------------
     def mark_dirty_tiles(self, tile_ids: Iterable[int]) -> None:
         for tile_id in np.asarray(tuple(tile_ids), dtype=np.int64).ravel():
