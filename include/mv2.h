@@ -155,6 +155,43 @@ int mv2_surface_sample(mv2_context *ctx, const double *rows_cols, int n, double 
 int mv2_surface_gather_tiles(mv2_context *ctx, const int32_t *tile_ids, int n_tiles, int tile_size,
                              int tiles_x, int tile_count, double *out);
 
+/* ---- compact state bridges: DeviceBulkState.apply_host_patch / apply_host_indices /
+ *      add_host_indices / download_patch / download_region
+ *      (pkg/runtime/bulk_state_authority.py:804-918, :1036-1137).
+ *      field: 0 b_eff, 1 mobile, 2 momentum_x, 3 momentum_y; values in metres
+ *      (FP32 mobile stored as rint(value / quantum)).  All arrays are DEVICE
+ *      arrays; the calls are asynchronous on the context's stream. ---- */
+/* values: row-major (row1-row0) x (col1-col0) doubles for rows [row0,row1) x cols [col0,col1) */
+int mv2_patch_scatter(mv2_context *ctx, int field, int row0, int row1, int col0, int col1,
+                      const double *values);
+/* target[idx[k]] = values ? values[k] : scalar, for an int32 field of ny*nx cells
+ * (material_mask, active_mask, deposition_mask, frontier_reached); indices
+ * outside the grid are skipped */
+int mv2_index_scatter_int(mv2_context *ctx, int32_t *target, const int32_t *idx, const int32_t *values,
+                          int32_t scalar, int n);
+/* field[idx[k]] += values[k] (atomic; indices may repeat) */
+int mv2_index_add(mv2_context *ctx, int field, const int32_t *idx, const double *values, int n);
+/* out (device double[5][h*w]): b_eff, z_base, mobile (m), momentum_x, momentum_y of the region */
+int mv2_region_gather(mv2_context *ctx, int row0, int row1, int col0, int col1, double *out);
+
+/* ---- FailureZone activation: DeviceFailureZoneBridge.execute after the CAD
+ *      geometry (pkg/runtime/gpu_failure_bridge.py:114-136).  activated and
+ *      affected are DEVICE arrays over the patch rows [row0,row1) x cols
+ *      [col0,col1), row-major.  Entrains min(max(activated, 0), b_eff - z_base)
+ *      per cell (Triangle-A-C ledger), then sets material_mask = 1 on every
+ *      cell with mobile_before + activated > 0 that is, or is a 4-neighbour
+ *      inside the patch of, an affected cell with mobile_before + activated > 0;
+ *      forcing_out (device, capacity = patch cells) receives those flat indices
+ *      in row-major order.  One stream sync. ---- */
+typedef struct mv2_activation_result {
+    double activated_volume;          /* m^3 moved resting -> mobile */
+    int forcing_count;                /* entries written to forcing_out */
+    int reserved;
+} mv2_activation_result;
+int mv2_failure_activate(mv2_context *ctx, int row0, int row1, int col0, int col1, const double *activated,
+                         const int32_t *affected, int32_t *material_mask, int32_t *forcing_out,
+                         mv2_activation_result *out);
+
 /* ---- Mobile -> Resting deposition: WarpDepositionOperator.step_resident
  *      (pkg/bulk_interaction/warp_deposition.py:180-235) with its kernels
  *      deposit (:23-115) and weighted_sum (:117-129).  A cell with mobile

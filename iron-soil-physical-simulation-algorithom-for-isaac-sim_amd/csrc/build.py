@@ -25,7 +25,7 @@ LIB = OUT_DIR / "libmv2.so"
 BUILD = CSRC / "_build"
 SOURCES = ["mv2_context.hip", "mv2_substep.hip", "mv2_exchange.hip", "mv2_relax.hip",
            "mv2_label.hip", "mv2_batch.hip", "mv2_batch_ops.hip", "mv2_surrogate.hip", "mv2_deposition.hip",
-           "mv2_frontier.hip", "mv2_surface.hip"]
+           "mv2_frontier.hip", "mv2_surface.hip", "mv2_patch.hip"]
 ARCH = os.environ.get("MV2_OFFLOAD_ARCH", "gfx950")
 
 CFLAGS = [

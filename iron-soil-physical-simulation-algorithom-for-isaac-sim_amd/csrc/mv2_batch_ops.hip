@@ -45,11 +45,9 @@ __global__ void __launch_bounds__(kBlock) batch_entrain_kernel(const float *z_ba
         const double zb = z_base ? (double)z_base[i] : 0.0;
         const double available = fmax((double)b[i] - zb, 0.0);
         const double depth = fmin(fmax(req[k], 0.0), available);
-        dq = (long long)floor(depth * kInvQuantum);
-        const long long room = 2147483647LL - (long long)h[i];
-        if (dq > room) dq = room;
+        dq = entrain_quanta_f32(b[i], depth, 2147483647LL - (long long)h[i]);
         if (dq > 0) {
-            b[i] = (float)((double)b[i] - (double)dq * kQuantum);
+            b[i] = (float)((double)b[i] - (double)dq * kQuantum);   // exact
             h[i] = (int32_t)((long long)h[i] + dq);
         } else {
             dq = 0;

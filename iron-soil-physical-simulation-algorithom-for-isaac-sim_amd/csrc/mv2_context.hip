@@ -133,6 +133,7 @@ extern "C" void mv2_destroy(mv2_context *ctx)
     release(ctx->step_diag);
     release(ctx->spread);
     release(ctx->frontier);
+    release(ctx->patch);
     release(ctx->host_result, true);
     if (ctx->ev_start) hipEventDestroy(ctx->ev_start);
     if (ctx->ev_stop) hipEventDestroy(ctx->ev_stop);

@@ -8,8 +8,10 @@
 // reproducible run to run (the reference uses FP64 atomics).
 //
 // FP32 storage: b_eff/z_base/momentum are float, mobile is int32 quanta.  A
-// requested depth is converted to whole quanta (rounded down, so the donor
-// limit still holds); the ledger reports exactly the quanta moved.
+// requested depth is rounded down to the largest amount that is whole quanta
+// and leaves b_eff exactly on the float32 grid (entrain_quanta_f32 /
+// deposit_quanta_f32), so the donor limit holds, H_free is bit-invariant per
+// cell and the ledger reports exactly what both reservoirs changed by.
 #include "mv2_internal.hpp"
 
 #include <cmath>
@@ -86,11 +88,9 @@ __global__ void __launch_bounds__(kBlock) entrain_kernel(const void *z_base, voi
             int32_t *m = (int32_t *)mobile;
             double available = dmaxd((double)b[i] - (double)z[i], 0.0);
             double depth = dmind(dmaxd(req[k], 0.0), available);
-            long long dq = (long long)floor(depth * kInvQuantum);
-            long long room = 2147483647LL - (long long)m[i];
-            if (dq > room) dq = room;
+            const long long dq = entrain_quanta_f32(b[i], depth, 2147483647LL - (long long)m[i]);
             double moved = (double)dq * kQuantum;
-            b[i] = (float)((double)b[i] - moved);
+            b[i] = (float)((double)b[i] - moved);            // exact
             m[i] = (int32_t)((long long)m[i] + dq);
             v[0] += moved * w;
         }
@@ -135,10 +135,12 @@ __global__ void __launch_bounds__(kBlock) deposit_kernel(void *b_eff, void *mobi
             long long before = m[i] > 0 ? m[i] : 0;
             long long dq = (long long)floor(dmaxd(req[k], 0.0) * kInvQuantum);
             if (dq > before) dq = before;
+            float b_new;
+            dq = deposit_quanta_f32(b[i], dq, &b_new);
             float fraction = before > 0 ? (float)((double)dq / (double)before) : 0.0f;
             float rx = qx[i] * fraction, ry = qy[i] * fraction;
             double moved = (double)dq * kQuantum;
-            b[i] = (float)((double)b[i] + moved);
+            b[i] = b_new;
             m[i] = (int32_t)(before - dq);
             qx[i] = qx[i] - rx;
             qy[i] = qy[i] - ry;

@@ -70,6 +70,7 @@ struct mv2_context {
     int frontier_predicted = 0;           // frontier rounds to enqueue up front
     mv2_batch_state *batch = nullptr;     // batched terrains bound to this context
     double *noalloc_scratch = nullptr;    // device address of the allocation-free ledger's partials
+    mv2::Scratch patch;                   // FailureZone activation: per-cell flags, partials, result
 };
 
 void mv2_batch_release(mv2_context *ctx);
@@ -248,6 +249,80 @@ __host__ __device__ inline double vertex_weight(int ny, int nx, double f6, int r
 __device__ inline double cell_weight(int mode, int ny, int nx, double area, double f6, int r, int c)
 {
     return mode ? vertex_weight(ny, nx, f6, r, c) : area;
+}
+
+// ---- FP32-storage exchange amounts -----------------------------------------------
+// b_eff is float32 and mobile holds whole quanta Q = 2^-24 m.  Moving d metres
+// between them keeps H_free = b_eff + mobile bit-invariant per cell only if d
+// is a whole number of quanta AND b_eff -/+ d is exactly a float32.  These give
+// the largest such d (in quanta) not above the (already donor-limited) request;
+// the ledger then reports exactly the change of both reservoirs.
+
+// spacing of float32 numbers in the binade of v
+__host__ __device__ inline double f32_spacing(float v)
+{
+    const unsigned e = (__builtin_bit_cast(unsigned, v) >> 23) & 0xffu;
+    return e == 0 ? ldexp(1.0, -149) : ldexp(1.0, (int)e - 150);
+}
+
+__host__ __device__ inline bool f32_exact(double r) { return (double)(float)r == r; }
+
+// Resting -> Mobile: depth = min(max(request, 0), available) metres, room =
+// quanta the cell's int32 can still take.  A multiple of max(Q, spacing(b))
+// below b stays on the float grid (it only moves towards smaller binades).
+__host__ __device__ inline long long entrain_quanta_f32(float b, double depth, long long room)
+{
+    const double g = fmax(kQuantum, f32_spacing(b));
+    double d = floor(depth / g) * g;
+    const double cap = floor((double)room * kQuantum / g) * g;
+    if (d > cap) d = cap;
+    if (!(d > 0.0) || !f32_exact((double)b - d)) return 0;
+    return (long long)(d * kInvQuantum);
+}
+
+// Mobile -> Resting: up to `quanta` = min(floor(max(request, 0) / Q), mobile
+// quanta).  b_eff + d is kept exact when it can be: the sum itself, or the
+// largest float32 below it when that still differs from b_eff by whole quanta
+// (always so for b_eff >= 0.5 m, whose float32 spacing is >= Q; a request
+// below that spacing moves nothing).  Otherwise b_eff lies below 0.5 m off
+// the quantum grid and the deposit would carry it into a binade whose floats
+// are all on the grid: then as many quanta as keep b_eff in its own binade
+// move exactly, and when that is none the whole amount moves with b_eff
+// rounded to the nearest float32 (a change of H_free below half that binade's
+// spacing, once per such crossing, after which b_eff is on the grid).
+// Returns the quanta moved; *b_new receives the new b_eff.
+__host__ __device__ inline long long deposit_quanta_f32(float b, long long quanta, float *b_new)
+{
+    *b_new = b;
+    if (quanta <= 0) return 0;
+    const double d0 = (double)quanta * kQuantum;
+    const double r = (double)b + d0;
+    if (f32_exact(r)) {
+        *b_new = (float)r;
+        return quanta;
+    }
+    float rf = (float)r;
+    if ((double)rf > r) rf = nextafterf(rf, -__builtin_inff());
+    const double d = (double)rf - (double)b;
+    if (d > 0.0 && floor(d * kInvQuantum) == d * kInvQuantum) {
+        *b_new = rf;
+        return (long long)(d * kInvQuantum);
+    }
+    // b_eff on the quantum grid: no whole number of quanta up to the request
+    // lands on the float grid (the request is below b's spacing)
+    if (floor((double)b * kInvQuantum) == (double)b * kInvQuantum) return 0;
+    // b_eff off the quantum grid (below 0.5 m): stay inside its binade (its
+    // floats differ from b_eff by multiples of its spacing, which divides Q)
+    const double sb = f32_spacing(b);
+    const unsigned e = (__builtin_bit_cast(unsigned, b) >> 23) & 0xffu;
+    const double top = ldexp(1.0, (int)e - 126) - sb;            // largest float of b's binade
+    const double k = floor((top - (double)b) * kInvQuantum);
+    if (k >= 1.0 && f32_exact((double)b + k * kQuantum)) {
+        *b_new = (float)((double)b + k * kQuantum);
+        return (long long)k;
+    }
+    *b_new = (float)r;                                           // the rare rounded crossing
+    return quanta;
 }
 
 // Load a small record written by an earlier launch, word by word with

@@ -13,8 +13,11 @@ from .bulk_state import (
     DeviceBulkState,
     DeviceMaterialLedger,
     DeviceMaterialLedgerSnapshot,
+    DeviceTransferSnapshot,
+    HostBulkStatePatch,
     HostBulkStateView,
 )
+from .chain import GpuBulkOperatorChain, GpuBulkOperatorStep
 from .config import (
     LargeAvalancheTransitionConfig,
     MaterialScenario,
@@ -26,6 +29,7 @@ from .config import (
 from .checkpoint import (load_frozen_boundary, restore_device_checkpoint, sha256_array,
                          write_device_checkpoint)
 from .deposition import WarpDepositionOperator, WarpDepositionStep
+from .failure_zone import DeviceFailureZoneBridge, DeviceFailureZoneResult, FailureGeometry
 from .large_avalanche import DeviceLargeAvalancheBridge, DeviceLargeAvalancheResult, label_mask
 from .mobile_v2 import (
     HipMobileV2ReferenceSolver,

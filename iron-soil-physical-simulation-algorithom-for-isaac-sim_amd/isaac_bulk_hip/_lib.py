@@ -99,6 +99,10 @@ class FrontierRoundResult(ctypes.Structure):
                 ("moved_edges", ctypes.c_longlong)]
 
 
+class ActivationResult(ctypes.Structure):
+    _fields_ = [("activated_volume", _D), ("forcing_count", _I), ("reserved", _I)]
+
+
 class BatchDesc(ctypes.Structure):
     _fields_ = [
         ("ny", _I), ("nx", _I), ("offset", ctypes.c_int64),
@@ -131,6 +135,11 @@ SIGNATURES: dict[str, tuple[object, list[object]]] = {
     "mv2_deposit": (_I, [_P, _P, _P, _I, _I, _D, _P]),
     "mv2_reservoirs": (_I, [_P, _I, _D, _P]),
     "mv2_reservoirs_noalloc": (_I, [_P, _I, _D, _P]),
+    "mv2_patch_scatter": (_I, [_P, _I, _I, _I, _I, _I, _P]),
+    "mv2_index_scatter_int": (_I, [_P, _P, _P, _P, _I32, _I]),
+    "mv2_index_add": (_I, [_P, _I, _P, _P, _I]),
+    "mv2_region_gather": (_I, [_P, _I, _I, _I, _I, _P]),
+    "mv2_failure_activate": (_I, [_P, _I, _I, _I, _I, _P, _P, _P, _P, ctypes.POINTER(ActivationResult)]),
     "mv2_surface_capture": (_I, [_P, _P]),
     "mv2_surface_dirty_tiles": (_I, [_P, _P, _D, _I, _I, _I, _P, _P, ctypes.POINTER(_I)]),
     "mv2_surface_sample": (_I, [_P, _P, _I, _P]),

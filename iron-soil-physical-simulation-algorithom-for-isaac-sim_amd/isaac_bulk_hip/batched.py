@@ -220,10 +220,10 @@ class BatchedMobileV2:
     def entrain(self, instance, cell, depth_m, *, z_base=None, check: bool = True) -> np.ndarray:
         """The entrain contract (bulk_state_authority.py:104-127) at (instance,
         instance-local flat cell) pairs; returns the volume moved per instance
-        (m^3, whole quanta, uniform cell area).  Indices must be unique.
-        Inputs may be host arrays or device tensors; `check=False` skips the
-        index range checks (which read the indices back) for inputs already
-        validated."""
+        (m^3, whole quanta, uniform cell area).  (instance, cell) pairs must be
+        unique.  Inputs may be host arrays or device tensors; `check=False`
+        skips the range and uniqueness checks (which read the indices back)
+        for inputs already validated."""
         torch = _torch()
         rt = self.rt
         rt.bind_stream()
@@ -246,6 +246,10 @@ class BatchedMobileV2:
             sizes = torch.as_tensor((self.ny.astype(np.int64) * self.nx.astype(np.int64)), device=dev)
             if bool((cells < 0).any()) or bool((cells.to(torch.int64) >= sizes[inst.to(torch.int64)]).any()):
                 raise ValueError("[BatchedDeviceBulkState] cell index out of range")
+            # the kernel updates each (instance, cell) with a plain read-modify-write
+            key = inst.to(torch.int64) * int(sizes.max().item()) + cells.to(torch.int64)
+            if int(torch.unique(key).numel()) != n:
+                raise ValueError("[BatchedDeviceBulkState] duplicate (instance, cell) entrainment pairs")
         zb = 0 if z_base is None else z_base.data_ptr()
         vol = np.zeros(self.count, np.float64)
         rt.check(rt.lib.mv2_batch_entrain(rt.ctx, zb, inst.data_ptr(), cells.data_ptr(), depth.data_ptr(), n,

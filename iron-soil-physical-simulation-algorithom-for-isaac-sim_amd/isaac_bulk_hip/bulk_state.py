@@ -20,6 +20,7 @@ from __future__ import annotations
 import ctypes
 from dataclasses import dataclass
 from enum import Enum
+from time import perf_counter
 from typing import Iterable
 
 import numpy as np
@@ -50,6 +51,69 @@ class DeviceMaterialLedgerSnapshot:
     outflow_m3: float
     absolute_volume_error_m3: float
     relative_volume_error: float
+
+
+@dataclass(frozen=True)
+class DeviceTransferSnapshot:
+    """Per-step transfer accounting with explicit full-field detection
+    (bulk_state_authority.py:281-306)."""
+
+    h2d_bytes: int
+    d2h_bytes: int
+    h2d_transfer_count: int
+    d2h_transfer_count: int
+    synchronization_count: int
+    kernel_launch_count: int
+    h2d_blocking_ms: float
+    d2h_blocking_ms: float
+    full_field_h2d_count: int
+    full_field_d2h_count: int
+
+    def to_dict(self) -> dict[str, int | float]:
+        return {name: getattr(self, name) for name in self.__dataclass_fields__}
+
+
+@dataclass(frozen=True)
+class HostBulkStatePatch:
+    """Compact non-authoritative terrain patch (bulk_state_authority.py:373-413)."""
+
+    bbox_yx: tuple[int, int, int, int]
+    H_resting_m: np.ndarray
+    H_mobile_m: np.ndarray
+    mobile_momentum_m2_s: np.ndarray
+    timestamp_device_s: float
+    source: str
+    z_base_m: np.ndarray | None = None
+
+    def __post_init__(self) -> None:
+        row0, row1, col0, col1 = (int(v) for v in self.bbox_yx)
+        expected = (row1 - row0, col1 - col0)
+        for name in ("H_resting_m", "H_mobile_m", "mobile_momentum_m2_s"):
+            arr = np.ascontiguousarray(np.asarray(getattr(self, name), dtype=np.float64))
+            arr.setflags(write=False)
+            object.__setattr__(self, name, arr)
+        if self.H_resting_m.shape != expected or self.H_mobile_m.shape != expected:
+            raise ValueError("[HostBulkStatePatch] scalar field shape mismatch")
+        if self.mobile_momentum_m2_s.shape != expected + (2,):
+            raise ValueError("[HostBulkStatePatch] momentum field shape mismatch")
+        base = (np.zeros(expected) if self.z_base_m is None
+                else np.ascontiguousarray(np.asarray(self.z_base_m, dtype=np.float64)))
+        if base.shape != expected:
+            raise ValueError("[HostBulkStatePatch] z_base shape mismatch")
+        base.setflags(write=False)
+        object.__setattr__(self, "z_base_m", base)
+
+    @property
+    def b_eff_m(self) -> np.ndarray:
+        return self.H_resting_m
+
+    @property
+    def h_resting_derived_m(self) -> np.ndarray:
+        return self.b_eff_m - self.z_base_m
+
+    @property
+    def H_free_m(self) -> np.ndarray:
+        return self.b_eff_m + self.H_mobile_m
 
 
 @dataclass(frozen=True)
@@ -97,6 +161,11 @@ class DeviceBulkState:
                    "avalanche_r2m_cumulative", "avalanche_m2r_cumulative", "track_rut",
                    "mobile_flux_export_cumulative")
     _STATE_FIELDS = ("z_base", "b_eff", "mobile", "momentum_x", "momentum_y")
+    # transfers of these count as full-field transfers (bulk_state_authority.py:504-516)
+    _FULL_FIELD_NAMES = frozenset({"z_base", "b_eff", "mobile", "momentum_x", "momentum_y",
+                                   "initial_b_eff", "weights"})
+    _MUTABLE_FLOAT = {"b_eff": 0, "mobile": 1, "momentum_x": 2, "momentum_y": 3}
+    _COMPACT_INT = ("material_mask", "active_mask", "deposition_mask", "frontier_reached")
 
     def __init__(self, grid: TerrainGrid, initial_resting_m, *, device: str = "cuda:0",
                  tile_size: int = 64, precision: str = "fp64",
@@ -116,6 +185,10 @@ class DeviceBulkState:
         self._timestamp_host_view_s = -1.0
         self._reset_generation = 0
         self._dirty_tile_ids: set[int] = set()
+        self._full_field_h2d_count = 0
+        self._full_field_d2h_count = 0
+        self._h2d_blocking_ms = 0.0
+        self._d2h_blocking_ms = 0.0
         torch = __import__("torch")
         self._fdtype = torch.float64 if precision == "fp64" else torch.float32
         self._mdtype = torch.float64 if precision == "fp64" else torch.int32
@@ -171,7 +244,30 @@ class DeviceBulkState:
         return int(ty * tx)
 
     def begin_physics_step(self) -> None:
+        """Reset the per-step transfer counters (initialisation/reset excluded)."""
         self.runtime.telemetry.reset()
+        self._full_field_h2d_count = 0
+        self._full_field_d2h_count = 0
+        self._h2d_blocking_ms = 0.0
+        self._d2h_blocking_ms = 0.0
+
+    def transfer_snapshot(self) -> DeviceTransferSnapshot:
+        """Transfers since begin_physics_step (bulk_state_authority.py:656-681)."""
+        t = self.runtime.telemetry
+        return DeviceTransferSnapshot(
+            h2d_bytes=int(t.h2d_bytes), d2h_bytes=int(t.d2h_bytes),
+            h2d_transfer_count=int(t.h2d_transfer_count), d2h_transfer_count=int(t.d2h_transfer_count),
+            synchronization_count=int(t.synchronization_count), kernel_launch_count=int(t.kernel_launch_count),
+            h2d_blocking_ms=self._h2d_blocking_ms, d2h_blocking_ms=self._d2h_blocking_ms,
+            full_field_h2d_count=self._full_field_h2d_count, full_field_d2h_count=self._full_field_d2h_count)
+
+    def assert_normal_step_transfer_budget(self) -> None:
+        """A normal physics step may move no complete terrain field (:683-689)."""
+        snap = self.transfer_snapshot()
+        if snap.full_field_h2d_count or snap.full_field_d2h_count:
+            raise BulkStateAuthorityError(
+                "[DeviceBulkState] normal GPU physics performed a full-field transfer: "
+                f"H2D={snap.full_field_h2d_count}, D2H={snap.full_field_d2h_count}")
 
     def advance_time(self, dt_s: float) -> None:
         dt = float(dt_s)
@@ -196,6 +292,8 @@ class DeviceBulkState:
                 continue
             arr = np.asarray(value, dtype=np.float64).reshape(self.shape)
             rt.arrays[name].copy_(torch.from_numpy(arr.ravel()).to(rt.device, self._fdtype))
+            rt.telemetry.record_h2d(arr)
+            self._full_field_h2d_count += 1
         if mobile is not None:
             arr = np.asarray(mobile, dtype=np.float64).reshape(self.shape)
             if np.any(arr < 0.0):
@@ -208,6 +306,8 @@ class DeviceBulkState:
             else:
                 t = torch.from_numpy(arr.ravel())
             rt.arrays["mobile"].copy_(t.to(rt.device))
+            rt.telemetry.record_h2d(arr)
+            self._full_field_h2d_count += 1
 
     # -- contracts -------------------------------------------------------------------
     def _compact(self, flat_indices, values, what: str):
@@ -266,6 +366,141 @@ class DeviceBulkState:
         self.mark_dirty_bbox((int(rows.min()), int(rows.max()) + 1, int(cols.min()), int(cols.max()) + 1))
         return {"deposited_volume_m3": float(out[0]),
                 "removed_mobile_momentum_kg_m_s": np.asarray([out[1], out[2], 0.0])}
+
+    # -- compact host bridges (bulk_state_authority.py:804-918, :1036-1137) ------------
+    def _upload_compact(self, values, dtype):
+        """A compact command/patch buffer (never a terrain field) to the device."""
+        torch = __import__("torch")
+        start = perf_counter()
+        host = np.ascontiguousarray(values)
+        t = torch.from_numpy(host).to(self.runtime.device, non_blocking=False)
+        self.runtime.telemetry.record_h2d(host)
+        self._h2d_blocking_ms += (perf_counter() - start) * 1e3
+        return t if t.dtype == dtype else t.to(dtype)
+
+    def apply_host_patch(self, field: str, bbox_yx, values, *, reason: str) -> None:
+        """Commit a compact host-generated patch of one float field (:804-845)."""
+        torch = __import__("torch")
+        if field == "resting":
+            field = "b_eff"
+        if field not in self._MUTABLE_FLOAT:
+            raise ValueError("[DeviceBulkState] unsupported mutable field")
+        if not reason:
+            raise ValueError("[DeviceBulkState] patch reason must be non-empty")
+        row0, row1, col0, col1 = (int(v) for v in bbox_yx)
+        if not (0 <= row0 <= row1 <= self.shape[0] and 0 <= col0 <= col1 <= self.shape[1]):
+            raise ValueError("[DeviceBulkState] patch bbox outside state")
+        patch = np.asarray(values, dtype=np.float64)
+        if patch.shape != (row1 - row0, col1 - col0) or not np.all(np.isfinite(patch)):
+            raise ValueError("[DeviceBulkState] patch values shape/finite check failed")
+        if patch.size == 0:
+            return
+        if field == "mobile" and self.precision == "fp32":
+            q = np.rint(patch / MOBILE_QUANTUM_M)
+            if np.any(q < np.iinfo(np.int32).min) or np.any(q > np.iinfo(np.int32).max):
+                raise ValueError("[DeviceBulkState] mobile thickness exceeds the FP32 storage range")
+        dev = self._upload_compact(patch.ravel(), torch.float64)
+        rt = self.runtime
+        rt.bind_stream()
+        rt.check(rt.lib.mv2_patch_scatter(rt.ctx, self._MUTABLE_FLOAT[field], row0, row1, col0, col1,
+                                          dev.data_ptr()))
+        rt.telemetry.record_call(syncs=0)
+        self.mark_dirty_bbox((row0, row1, col0, col1))
+
+    def apply_host_indices(self, field: str, flat_indices, values, *, reason: str) -> None:
+        """Scatter compact activation/mask data into an int32 field (:847-885)."""
+        torch = __import__("torch")
+        if field not in self._COMPACT_INT:
+            raise ValueError("[DeviceBulkState] unsupported compact integer field")
+        if not reason:
+            raise ValueError("[DeviceBulkState] compact scatter reason must be non-empty")
+        indices = np.asarray(flat_indices, dtype=np.int64).reshape(-1)
+        if np.any(indices < 0) or np.any(indices >= self.size):
+            raise ValueError("[DeviceBulkState] compact scatter index outside state")
+        scalar = np.isscalar(values)
+        if not scalar:
+            data = np.asarray(values, dtype=np.int32).reshape(-1)
+            if data.shape != indices.shape:
+                raise ValueError("[DeviceBulkState] compact scatter values shape mismatch")
+        if not indices.size:
+            return
+        rt = self.runtime
+        idx = self._upload_compact(indices.astype(np.int32), torch.int32)
+        vals = None if scalar else self._upload_compact(data, torch.int32)
+        rt.bind_stream()
+        rt.check(rt.lib.mv2_index_scatter_int(rt.ctx, rt.ptr(field), idx.data_ptr(),
+                                              None if vals is None else vals.data_ptr(),
+                                              int(values) if scalar else 0, int(indices.size)))
+        rt.telemetry.record_call(syncs=0)
+
+    def add_host_indices(self, field: str, flat_indices, values, *, reason: str) -> None:
+        """Conservatively add compact source terms to a float field (:887-918);
+        repeated indices accumulate (atomic adds)."""
+        torch = __import__("torch")
+        if field == "resting":
+            field = "b_eff"
+        if field not in self._MUTABLE_FLOAT:
+            raise ValueError("[DeviceBulkState] unsupported compact float field")
+        if not reason:
+            raise ValueError("[DeviceBulkState] compact add reason must be non-empty")
+        indices = np.asarray(flat_indices, dtype=np.int64).reshape(-1)
+        data = np.asarray(values, dtype=np.float64).reshape(-1)
+        if data.shape != indices.shape or not np.all(np.isfinite(data)):
+            raise ValueError("[DeviceBulkState] compact add values shape/finite check failed")
+        if np.any(indices < 0) or np.any(indices >= self.size):
+            raise ValueError("[DeviceBulkState] compact add index outside state")
+        if not indices.size:
+            return
+        if field == "mobile" and self.precision == "fp32" and np.any(np.abs(data) / MOBILE_QUANTUM_M > 2.0 ** 31):
+            raise ValueError("[DeviceBulkState] mobile thickness exceeds the FP32 storage range")
+        rt = self.runtime
+        idx = self._upload_compact(indices.astype(np.int32), torch.int32)
+        vals = self._upload_compact(data, torch.float64)
+        rt.bind_stream()
+        rt.check(rt.lib.mv2_index_add(rt.ctx, self._MUTABLE_FLOAT[field], idx.data_ptr(), vals.data_ptr(),
+                                      int(indices.size)))
+        rt.telemetry.record_call(syncs=0)
+
+    def _gather_region(self, row0, row1, col0, col1, source: str) -> HostBulkStatePatch:
+        torch = __import__("torch")
+        rt = self.runtime
+        h, w = row1 - row0, col1 - col0
+        start = perf_counter()
+        dev = torch.empty(5 * h * w, dtype=torch.float64, device=rt.device)
+        rt.bind_stream()
+        rt.check(rt.lib.mv2_region_gather(rt.ctx, row0, row1, col0, col1, dev.data_ptr()))
+        host = dev.cpu().numpy().reshape(5, h, w)
+        rt.telemetry.record_call(syncs=1)
+        rt.telemetry.record_d2h(host)
+        self._d2h_blocking_ms += (perf_counter() - start) * 1e3
+        self._timestamp_host_view_s = self._timestamp_device_s
+        return HostBulkStatePatch((row0, row1, col0, col1), host[0], host[2], np.stack((host[3], host[4]), -1),
+                                  self._timestamp_device_s, source, host[1])
+
+    def download_patch(self, bbox_yx, *, source: str) -> HostBulkStatePatch:
+        """A bounded patch (at most one tile of cells) for a host geometry operator (:1036-1085)."""
+        if not source:
+            raise ValueError("[DeviceBulkState] patch source must be non-empty")
+        row0, row1, col0, col1 = (int(v) for v in bbox_yx)
+        if not (0 <= row0 < row1 <= self.shape[0] and 0 <= col0 < col1 <= self.shape[1]):
+            raise ValueError("[DeviceBulkState] patch bbox outside state")
+        if (row1 - row0) * (col1 - col0) > self.tile_size * self.tile_size:
+            raise ValueError("[DeviceBulkState] patch exceeds compact bridge capacity; split into tiles")
+        return self._gather_region(row0, row1, col0, col1, source)
+
+    def download_region(self, bbox_yx, *, source: str) -> HostBulkStatePatch:
+        """A geometry-driven compact region of any extent short of the whole
+        terrain (:1087-1137), gathered by one launch and one transfer (the
+        reference stitches one download per tile and field)."""
+        row0, row1, col0, col1 = (int(v) for v in bbox_yx)
+        if not (0 <= row0 < row1 <= self.shape[0] and 0 <= col0 < col1 <= self.shape[1]):
+            raise ValueError("[DeviceBulkState] region bbox outside state")
+        if (row1 - row0) * (col1 - col0) >= self.size:
+            raise BulkStateAuthorityError(
+                "GPU_RUNTIME_FULL_FIELD_TERRAIN_REQUEST: compact bridge requested whole terrain")
+        if not source:
+            raise ValueError("[DeviceBulkState] patch source must be non-empty")
+        return self._gather_region(row0, row1, col0, col1, source)
 
     def reservoir_reduction(self, density_kg_m3: float) -> dict:
         density = float(density_kg_m3)
@@ -424,6 +659,8 @@ class DeviceBulkState:
         if unknown:
             raise BulkStateAuthorityError(f"[DeviceBulkState] unknown fields: {unknown}")
         out: dict[str, np.ndarray] = {}
+        start = perf_counter()
+        self._full_field_d2h_count += sum(1 for n in names if n in self._FULL_FIELD_NAMES)
         for group, dtype in ((floats, torch.float64), (ints, torch.int32)):
             if not group:
                 continue
@@ -440,6 +677,7 @@ class DeviceBulkState:
             rt.telemetry.record_d2h(arr)
             for k, name in enumerate(group):
                 out[name] = arr[k * self.size:(k + 1) * self.size].reshape(self.shape).copy()
+        self._d2h_blocking_ms += (perf_counter() - start) * 1e3
         return out
 
     def _upload_fields(self, fields: dict) -> None:
@@ -447,6 +685,8 @@ class DeviceBulkState:
         rt = self.runtime
         floats = [n for n in fields if self._is_float_field(n)]
         ints = [n for n in fields if n in self._INT_FIELDS]
+        start = perf_counter()
+        self._full_field_h2d_count += sum(1 for n in fields if n in self._FULL_FIELD_NAMES)
         for group, dtype in ((floats, np.float64), (ints, np.int32)):
             if not group:
                 continue
@@ -465,6 +705,7 @@ class DeviceBulkState:
                 else:
                     target.copy_(view.to(target.dtype))
         rt.synchronize()
+        self._h2d_blocking_ms += (perf_counter() - start) * 1e3
 
     def restore_checkpoint_fields(self, fields: dict, *, timestamp_s: float, source: str) -> None:
         """Explicit checkpoint boundary (bulk_state_authority.py:1219-1278):

@@ -498,3 +498,104 @@ if os.environ.get("MV2_ORACLE_AUTOBUILD", "1") == "1" and not _LIB_PATH.exists()
         build()
     except Exception:  # pragma: no cover - reported on first lib() call
         pass
+
+
+# ---- FP32-storage exchange amounts (restating mv2_internal.hpp
+#      entrain_quanta_f32 / deposit_quanta_f32; the contract they serve is
+#      bulk_state_authority.py:104-150 with H_free = b_eff + mobile invariant
+#      per cell, here with b_eff float32 and mobile whole quanta of 2^-24 m) ----
+QUANTUM = 2.0 ** -24
+
+
+def f32_spacing(b) -> float:
+    """Spacing of float32 numbers in the binade of b."""
+    e = (int(np.float32(b).view(np.uint32)) >> 23) & 0xFF
+    return 2.0 ** -149 if e == 0 else 2.0 ** (e - 150)
+
+
+def _f32_exact(r: float) -> bool:
+    return float(np.float32(r)) == r
+
+
+def entrain_quanta_f32(b, depth, room=2 ** 31 - 1) -> int:
+    """Quanta moved Resting -> Mobile for a donor-limited depth (metres)."""
+    b = np.float32(b)
+    g = max(QUANTUM, f32_spacing(b))
+    d = np.floor(depth / g) * g
+    cap = np.floor(room * QUANTUM / g) * g
+    d = min(d, cap)
+    if not d > 0.0 or not _f32_exact(float(b) - d):
+        return 0
+    return int(d / QUANTUM)
+
+
+def deposit_quanta_f32(b, quanta):
+    """(quanta moved Mobile -> Resting, new float32 b_eff) for a request of
+    `quanta` whole quanta (already limited by the mobile layer)."""
+    b = np.float32(b)
+    if quanta <= 0:
+        return 0, b
+    d0 = quanta * QUANTUM
+    r = float(b) + d0
+    if _f32_exact(r):
+        return int(quanta), np.float32(r)
+    rf = np.float32(r)
+    if float(rf) > r:
+        rf = np.nextafter(rf, np.float32(-np.inf))
+    d = float(rf) - float(b)
+    if d > 0.0 and np.floor(d / QUANTUM) == d / QUANTUM:
+        return int(d / QUANTUM), rf
+    if np.floor(float(b) / QUANTUM) == float(b) / QUANTUM:
+        return 0, b
+    sb = f32_spacing(b)
+    e = (int(b.view(np.uint32)) >> 23) & 0xFF
+    top = 2.0 ** (e - 126) - sb
+    k = np.floor((top - float(b)) / QUANTUM)
+    if k >= 1.0 and _f32_exact(float(b) + k * QUANTUM):
+        return int(k), np.float32(float(b) + k * QUANTUM)
+    return int(quanta), np.float32(r)
+
+
+def failure_activation(z, b, m, weights, bbox_yx, activated, affected, *, fp32=False):
+    """FailureZone activation commit (DeviceFailureZoneBridge.execute after the
+    CAD geometry, gpu_failure_bridge.py:114-136) on full fields: the
+    entrainment contract over the patch (FP64: orc_entrain; FP32 storage:
+    whole quanta by entrain_quanta_f32, m given in quanta), then the forcing
+    cells -- wet (patch mobile + activated > 0) cells that are, or are a
+    4-neighbour inside the patch of, an affected wet cell -- as ascending flat
+    indices.  Returns (b, m, volume, forcing)."""
+    row0, row1, col0, col1 = (int(v) for v in bbox_yx)
+    ny, nx = np.shape(b)
+    act = np.asarray(activated, dtype=np.float64)
+    aff = np.asarray(affected, dtype=bool)
+    rows, cols = np.indices(act.shape)
+    flat = ((rows + row0) * nx + (cols + col0)).ravel()
+    if not fp32:
+        mob_before = np.asarray(m, dtype=np.float64)[row0:row1, col0:col1]
+        b, m, volume = entrain(z, b, m, weights, flat, act.ravel())
+        b, m = b.reshape(ny, nx), m.reshape(ny, nx)
+    else:
+        b = np.asarray(b, dtype=np.float32).copy().ravel()
+        m = np.asarray(m, dtype=np.int32).copy().ravel()
+        zf = np.asarray(z, dtype=np.float32).ravel()
+        w = np.asarray(weights, dtype=np.float64).ravel()
+        mob_before = m.reshape(ny, nx)[row0:row1, col0:col1].astype(np.float64) * QUANTUM
+        volume = 0.0
+        for i, req in zip(flat, act.ravel()):
+            depth = min(max(req, 0.0), max(float(b[i]) - float(zf[i]), 0.0))
+            dq = entrain_quanta_f32(b[i], depth, 2 ** 31 - 1 - int(m[i]))
+            b[i] = np.float32(float(b[i]) - dq * QUANTUM)
+            m[i] += dq
+            volume += dq * QUANTUM * w[i]
+        b, m = b.reshape(ny, nx), m.reshape(ny, nx)
+    wet = mob_before + act > 0.0
+    contact = aff & wet
+    near = contact.copy()
+    near[1:, :] |= contact[:-1, :]
+    near[:-1, :] |= contact[1:, :]
+    near[:, 1:] |= contact[:, :-1]
+    near[:, :-1] |= contact[:, 1:]
+    near &= wet
+    r, c = np.nonzero(near)
+    forcing = ((r + row0) * nx + (c + col0)).astype(np.int32)
+    return b, m, float(volume), forcing

@@ -7,6 +7,8 @@ from __future__ import annotations
 import numpy as np
 import pytest
 
+import oracle
+
 pytestmark = pytest.mark.gpu
 
 QUANTUM = 2.0 ** -24
@@ -138,7 +140,9 @@ def test_batched_relax_matches_single_field_relaxation():
 
 
 def test_batched_entrain_contract():
-    """entrain moves whole quanta, limited by the donor layer, per instance."""
+    """entrain moves whole quanta, limited by the donor layer, per instance,
+    keeping b_eff + mobile bit-invariant (the oracle's FP32 amount rule);
+    duplicate (instance, cell) pairs are rejected."""
     from isaac_bulk_hip.batched import BatchedMobileV2
     from isaac_bulk_hip.relaxation import _runtime
 
@@ -154,14 +158,17 @@ def test_batched_entrain_contract():
         i, c = int(inst[k]), int(cell[k])
         b0 = float(before[i][0].ravel()[c])
         d = min(max(depth[k], 0.0), max(b0, 0.0))
-        dq = int(np.floor(d * QUANTUM ** -1))
+        h0 = int(np.rint(before[i][1].ravel()[c] / QUANTUM))
+        dq = oracle.entrain_quanta_f32(np.float32(b0), d, 2 ** 31 - 1 - h0)
         b1, h1 = batch.fields(i)[0].ravel()[c], batch.fields(i)[1].ravel()[c]
-        assert b1 == np.float32(b0 - dq * QUANTUM)
-        assert np.rint(h1 / QUANTUM) == dq
+        assert float(b1) == b0 - dq * QUANTUM                      # exact: H_free unchanged
+        assert np.rint(h1 / QUANTUM) - h0 == dq
         expect[i] += dq * QUANTUM * 0.05 * 0.05
     np.testing.assert_allclose(vol, expect, rtol=0, atol=1e-18)
     # cells outside the lists are untouched
     assert np.array_equal(batch.fields(1)[0].ravel()[:17], before[1][0].ravel()[:17])
+    with pytest.raises(ValueError, match="duplicate"):
+        batch.entrain(np.array([0, 1, 0], np.int32), np.array([3, 3, 3], np.int32), np.full(3, 0.01))
 
 
 @pytest.mark.parametrize("connectivity", [4, 8])

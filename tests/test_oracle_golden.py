@@ -507,3 +507,96 @@ def test_frame_sequence_keeps_mass_and_ends_each_frame_relaxed():
                             (bb[:-1, :-1], bb[1:, 1:], diag), (bb[:-1, 1:], bb[1:, :-1], diag)):
                 assert float(np.max(np.abs(a - c) - crit * d)) <= 1e-6 + 1e-12
         np.testing.assert_array_equal(rec["labels"], oracle.label(rec["unstable"], 8))
+
+
+def _exchange_samples(rng, n):
+    """float32 heights across binades (0.01 .. 3.9 m, some exactly at binade
+    edges and one float below them) and requests of 0 .. 4000 quanta."""
+    b = np.concatenate([rng.uniform(0.01, 3.9, n).astype(np.float32),
+                        np.float32([0.25, 0.5, 1.0, 2.0]),
+                        np.nextafter(np.float32([0.25, 0.5, 1.0, 2.0]), np.float32(0))])
+    q = rng.integers(0, 4000, b.size)
+    return b, q
+
+
+def test_fp32_entrain_amount_exact_bounded_and_near_maximal():
+    """H_free = b_eff + mobile stays bit-invariant: b - moved is a float32 and
+    moved is whole quanta; moved never exceeds the donor-limited depth, and it
+    is within one step max(quantum, spacing(b)) of it; brute force confirms no
+    exact amount lies above it within that step... except via lower binades."""
+    rng = np.random.default_rng(5)
+    b, q = _exchange_samples(rng, 400)
+    Q = oracle.QUANTUM
+    for bi, qi in zip(b, q):
+        depth = qi * Q * 1.37                      # not a whole number of quanta
+        dq = oracle.entrain_quanta_f32(bi, depth)
+        moved = dq * Q
+        assert moved <= depth
+        assert float(np.float32(float(bi) - moved)) == float(bi) - moved       # exactly on the float grid
+        g = max(Q, oracle.f32_spacing(bi))
+        assert moved > depth - g - 1e-18
+        # brute force: every larger whole-quanta amount up to the depth either
+        # leaves the float grid or stays within one step of the chosen one
+        for k in range(dq + 1, int(np.floor(depth / Q)) + 1):
+            if float(np.float32(float(bi) - k * Q)) == float(bi) - k * Q:
+                assert (k - dq) * Q < g
+
+
+def test_fp32_deposit_amount_exact_unless_crossing_off_grid():
+    rng = np.random.default_rng(6)
+    b, q = _exchange_samples(rng, 400)
+    Q = oracle.QUANTUM
+    for bi, qi in zip(b, q):
+        dq, b_new = oracle.deposit_quanta_f32(bi, int(qi))
+        assert 0 <= dq <= qi
+        on_grid = np.floor(float(bi) / Q) == float(bi) / Q
+        exact = float(b_new) == float(bi) + dq * Q
+        if on_grid:
+            # on the quantum grid: always exact, and within one float spacing
+            # of the request (brute force finds nothing exact in between)
+            assert exact
+            for k in range(dq + 1, int(qi) + 1):
+                assert float(np.float32(float(bi) + k * Q)) != float(bi) + k * Q
+        elif not exact:
+            # the rounded crossing: everything moved, b_eff left its binade,
+            # the error is below half the new spacing, and b_eff is on the grid after
+            assert dq == qi
+            err = abs(float(b_new) - (float(bi) + dq * Q))
+            assert err <= 0.5 * oracle.f32_spacing(b_new)
+            assert oracle.f32_spacing(b_new) > oracle.f32_spacing(bi)
+            assert np.floor(float(b_new) / Q) == float(b_new) / Q or oracle.f32_spacing(b_new) < Q
+    # planted mistakes: the pre-existing rounding rule breaks exactness
+    bad = [float(np.float32(float(bi) + int(qi) * Q)) == float(bi) + int(qi) * Q for bi, qi in zip(b, q) if qi]
+    assert not all(bad)
+
+
+def test_failure_activation_hand_case():
+    """A 5 x 6 patch at (2, 3) of a 10 x 12 field: activation 0.01 m on a 3 x 3
+    block, one affected cell in its centre and one affected dry cell on the
+    patch edge.  Forced: the centre and its four neighbours (all wet), not the
+    diagonals, nothing around the dry affected cell; the flat indices come in
+    row-major order; the moved volume is 9 x 0.01 x w; H_free is unchanged."""
+    ny, nx = 10, 12
+    z = np.zeros((ny, nx))
+    b = np.full((ny, nx), 1.0)
+    m = np.zeros((ny, nx))
+    w = oracle.vertex_weights_fast(ny, nx, 0.05, 0.05)
+    bbox = (2, 7, 3, 9)
+    act = np.zeros((5, 6))
+    act[1:4, 1:4] = 0.01
+    aff = np.zeros((5, 6), bool)
+    aff[2, 2] = True
+    aff[0, 5] = True                                   # dry: forces nothing
+    b1, m1, vol, forcing = oracle.failure_activation(z, b, m, w, bbox, act, aff)
+    centre = (2 + 2) * nx + (3 + 2)
+    assert forcing.tolist() == [centre - nx, centre - 1, centre, centre + 1, centre + nx]
+    assert vol == pytest.approx(9 * 0.01 * 0.05 * 0.05, rel=1e-14)
+    np.testing.assert_array_equal(b1 + m1, b + m)
+    # FP32 storage: the same forcing set, whole quanta, H_free bit-invariant
+    b32, m32, vol32, forcing32 = oracle.failure_activation(z, b.astype(np.float32), np.zeros((ny, nx), np.int32),
+                                                          w, bbox, act, aff, fp32=True)
+    assert forcing32.tolist() == forcing.tolist()
+    np.testing.assert_array_equal(b32.astype(np.float64) + m32 * oracle.QUANTUM, b)
+    assert vol32 == pytest.approx(vol, rel=1e-6)
+    # planted mistake: 8-neighbourhood forcing would add the diagonals
+    assert (centre - nx - 1) not in forcing.tolist()
