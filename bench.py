@@ -55,8 +55,10 @@ def parse() -> argparse.Namespace:
     p.add_argument("--seed", type=int, default=0)
     p.add_argument("--relax-max-iterations", type=int, default=1000)
     p.add_argument("--cpu-sample-frames", type=int, default=100000,
-                   help="CPU baseline: frames to run at most (each mode stops after ~12 s)")
+                   help="CPU baseline: frames to run at most (each mode stops after ~10 s)")
     p.add_argument("--no-cpu-baseline", action="store_true")
+    p.add_argument("--parity-frames", type=int, default=30,
+                   help="frames of the FP32-vs-FP64-oracle parity check (cpu_baseline leg)")
     p.add_argument("--skip-relax-to-rest", action="store_true",
                    help="omit the relax-to-rest measurement (short profiling runs)")
     p.add_argument("--workload", choices=("frame", "batched", "digging", "deposition", "frontier"),
@@ -207,13 +209,78 @@ def cpu_baseline(b, h, frames: int) -> dict:
 
     cores = int(os.environ.get("OMP_NUM_THREADS", "0")) or len(os.sched_getaffinity(0))
     os.environ["OMP_NUM_THREADS"] = str(cores)
-    v_all, s_all, f_all, e_all = run(True, 12.0)
-    v_one, s_one, f_one, e_one = run(False, 12.0)
+    v_all, s_all, f_all, e_all = run(True, 10.0)
+    v_one, s_one, f_one, e_one = run(False, 10.0)
     return {"value": v_all, "unit": "cell-substeps/s", "cores": cores, "kind": "port",
             "sample": f"oracle FP64 Mobile V2 step (C restatement, OpenMP over cells), first {f_all} frames "
                       f"(1/60 s each) of the 701x701 cut-face terrain: {s_all} substeps in {e_all:.2f} s",
             "one_core": {"value": v_one, "cores": 1,
                          "sample": f"same, one thread: {f_one} frames, {s_one} substeps in {e_one:.2f} s"}}
+
+
+PARITY_TOL = {"linf_m": 1.0e-3, "rmse_m": 1.0e-5, "v2_mass_rel": 1.0e-10}
+
+
+def frame_parity(seed: int, frames: int) -> dict:
+    """FP32 device frames against the FP64 oracle's frames from the same
+    (FP32-rounded, quantised) initial fields: per-frame L-inf / RMSE of h,
+    qx, qy, b_eff and H_free, the Mobile V2 mass error |M_t - M_0| / M_0 of
+    both, the failure mask against the oracle on the device's own fields
+    (must be bit-exact) and against the FP64 sequence, and the substep counts.
+    Tolerances: L-inf <= 1e-3 m, RMSE <= 1e-5 m (the reference's terrain
+    acceptance, tools/run_terrain_hierarchy_acceptance.py:42-43), mass <= 1e-10."""
+    sys.path.insert(0, str(ROOT / "oracle"))
+    import oracle
+    from frame_sequence import OracleFrameSequence, field_errors
+    from isaac_bulk_hip import relax_device
+
+    state, solver, bridge, b, h = build_state("fp32", seed)
+    q = 2.0 ** -24
+    b0 = b.astype(np.float32).astype(np.float64)
+    h0 = np.rint(h / q) * q
+    aval = {"tan_start": float(np.tan(np.deg2rad(34.0))), "tan_stop": float(np.tan(np.deg2rad(30.0))),
+            "density": 1370.0, "gravity": 9.81, "cohesion": 0.0, "mobilization_depth": 0.08,
+            "dry": 1.0e-8, "activity_speed": 0.05, "release": True}
+    seq = OracleFrameSequence(b0, h0, dx=DX, dt=FRAME_DT, v2={"limit_mode": 1}, relax_angle_deg=34.0,
+                              relax_max_iterations=1000, relax_tolerance=1.0e-6, avalanche=aval)
+    quanta0 = int(np.sum(np.rint(h0 / q).astype(np.int64)))
+    worst = {}
+    mass_dev = mass_orc = 0.0
+    mask_fp64 = 0
+    mask_own_exact = labels_exact = True
+    sub_diff = 0
+    for _ in range(frames):
+        rec = solver.step_resident(FRAME_DT)
+        relax_device(state.runtime.arrays["b_eff"].view(N, N), DX, DX, 34.0, 1000, 1.0e-6, runtime=state.runtime)
+        bridge.label()
+        ref = seq.step()
+        view = state.explicit_host_view(source="acceptance")
+        dev = {"h": view.H_mobile_m, "qx": view.mobile_momentum_m2_s[..., 0],
+               "qy": view.mobile_momentum_m2_s[..., 1], "b_eff": view.b_eff_m, "H_free": view.H_free_m}
+        err = field_errors(dev, {"h": seq.h, "qx": seq.qx, "qy": seq.qy, "b_eff": seq.b, "H_free": seq.b + seq.h})
+        for k, v in err.items():
+            w = worst.setdefault(k, {"linf": 0.0, "rmse": 0.0})
+            w["linf"], w["rmse"] = max(w["linf"], v["linf"]), max(w["rmse"], v["rmse"])
+        unstable = state.runtime.arrays["avalanche_unstable"].cpu().numpy().reshape(N, N)
+        labels = state.runtime.arrays["avalanche_parent"].cpu().numpy().reshape(N, N)
+        own = seq.mask(dev["b_eff"], dev["h"], dev["qx"], dev["qy"])
+        mask_own_exact &= bool(np.array_equal(unstable, own["unstable"]))
+        labels_exact &= bool(np.array_equal(labels, oracle.label(unstable, 8)))
+        mask_fp64 = max(mask_fp64, int(np.count_nonzero(unstable != ref["unstable"])))
+        quanta = int(np.sum(np.rint(dev["h"] / q).astype(np.int64)))
+        mass_dev = max(mass_dev, abs(quanta - quanta0) / quanta0)
+        mass_orc = max(mass_orc, seq.mass_error())
+        sub_diff = max(sub_diff, abs(rec.substeps - ref["substeps"]))
+    passed = (all(v["linf"] <= PARITY_TOL["linf_m"] and v["rmse"] <= PARITY_TOL["rmse_m"] for v in worst.values())
+              and mass_dev <= PARITY_TOL["v2_mass_rel"] and mask_own_exact and labels_exact and sub_diff <= 1)
+    state.runtime.close()
+    return {"frames": frames, "reference": "FP64 oracle frames (C restatement: V2 step, relaxation, mask, labels)",
+            "tolerance": PARITY_TOL, "worst_per_field": worst,
+            "v2_mass_error_rel": {"device_fp32": mass_dev, "oracle_fp64": mass_orc},
+            "total_volume_error_rel_oracle_fp64": seq.total_error(),
+            "mask_bit_exact_on_device_fields": mask_own_exact, "labels_bit_exact": labels_exact,
+            "mask_cells_differing_from_fp64_sequence_max": mask_fp64, "substep_count_diff_max": sub_diff,
+            "passed": bool(passed)}
 
 
 def run_batched(args) -> int:
@@ -696,6 +763,8 @@ def main() -> int:
     }
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(b0, h0, args.cpu_sample_frames)
+        if args.precision == "fp32" and args.parity_frames > 0:
+            out["parity"] = frame_parity(args.seed, args.parity_frames)
     print(json.dumps(out), flush=True)
     # release device resources before interpreter teardown (profilers hook exit)
     from isaac_bulk_hip.runtime import close_all_runtimes
