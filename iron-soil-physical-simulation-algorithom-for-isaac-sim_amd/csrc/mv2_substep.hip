@@ -7,7 +7,11 @@
 //             (Control below), so launches past the end of the loop return
 //             at once and the host never reads a per-substep value
 //   end       copy back when the final state sits in scratch, ledger after
-//   finalize  one workgroup folds the partials in fixed order into the record
+//             (FP32: done by the first substep launch that finds the loop
+//             ended, so it costs no launch of its own)
+//   fold      one wave per ledger / diagnostic field folds the workgroup
+//             partials in fixed order into pinned host memory; the host
+//             assembles the record after its one stream sync
 // The host enqueues as many substep launches as the previous call needed
 // (plus one) and reads the record once; only a call that needs more substeps
 // than enqueued pays a second round trip.  A kernel boundary costs a few
@@ -48,12 +52,12 @@ struct Control {
     int done;                  // loop finished (normally or by an error)
     int numeric;               // 0 ok, 1 positivity, 2 substep limit, 3 int32 range
     int cfl_limited, static_exit;
-    int rolled, pad;           // 1 once a failed substep was rolled back
+    int rolled;                // 1 once a failed substep was rolled back
+    int end_done;              // FP32: a launch after the loop's end copied back and stored the ledger after
 };
 
 struct SyncBlock {
-    unsigned long long static_grad;      // double bits, folded by the finalize pass
-    unsigned long long static_q;         // double bits
+    unsigned long long pad[2];
     int flags[2];                        // launch k ORs into [(k+1)&1]: bit 0 positivity, bit 1 range
     Control ctl[2];
 };
@@ -79,6 +83,7 @@ constexpr int kClasses = 9;              // weight classes of the exact FP32 mas
 // consecutive workgroups with coalesced loads.
 constexpr int kLedgerF = 2 * kSum;        // doubles per workgroup: before[8], after[8]
 constexpr int kLedgerQ = 2 * kClasses;    // int64 per workgroup: quanta before[9], after[9]
+constexpr int kLedger32D = 7;             // FP32 strips: ledger 1..7 of warp_mobile_v2.py:41-54 per workgroup
 
 struct F64P {
     using real = double;
@@ -111,7 +116,6 @@ struct V2Args {
     double *diag_part;                   // kDiag x (workgroups of the substep launches)
     int diag_groups;
     unsigned long long *spread;          // kStepSpreadSets spread-maximum sets
-    mv2_step_record *rec;
     int clamp_quarter;                   // FP32: cap outflow per face at h/4
 };
 
@@ -290,7 +294,7 @@ __device__ inline void xcd_tile_range(int tiles, int &first, int &last)
 // trip instead of one per cell.
 // ---------------------------------------------------------------------------
 constexpr int kLedgerU = 4;
-constexpr int kFinalizeBlock = 512;
+constexpr int kLedgerGroupsMax = 512;      // FP64 ledger passes: at most this many workgroups
 
 struct CellVals {
     double b, depth, qx, qy;
@@ -503,11 +507,15 @@ __global__ void __launch_bounds__(kBlock) v2_begin_kernel(V2Args<P> a)
 // the loop ended (eager = 0) read the loop state first and skip the staging.
 constexpr int kSubFold = 3 + kDiag;   // values folded per workgroup at the end of a substep
 
+template <int SH>
+__device__ inline void end32_body(const V2Args<F32P> &a, int fsrc, double *lds);
+
 template <class P, int SH = kSH>
 __global__ void __launch_bounds__(kBlock) v2_substep_kernel(V2Args<P> a, int k, int eager)
 {
     constexpr int kRows = SH * (kBlock / kWave);     // rows of a tile
     __shared__ double smem[64 * 4];
+    __shared__ double fold_lds[kSubFold * kBlock];
     SyncBlock *s = a.sync;
     const mv2_step_config &cfg = a.cfg;
     const bool lead = blockIdx.x == 0 && threadIdx.x == 0;
@@ -553,6 +561,12 @@ __global__ void __launch_bounds__(kBlock) v2_substep_kernel(V2Args<P> a, int k, 
         C.done = 1;
     }
     if (C.done) {
+        if constexpr (P::kQuant) {
+            // the first launch past the loop's end: copy back + ledger after
+            static_assert(kLedger32D <= kSubFold, "the ledger fold fits the substep's LDS");
+            if (!C.end_done && !C.static_exit) end32_body<SH>(a, C.src, fold_lds);
+            C.end_done = 1;
+        }
         if (lead) s->ctl[(k + 1) & 1] = C;
         return;
     }
@@ -620,7 +634,6 @@ __global__ void __launch_bounds__(kBlock) v2_substep_kernel(V2Args<P> a, int k, 
     }
     // one field-parallel fold of the workgroup's values: 0 wave-speed maximum,
     // 1 overflow flag (maximum), 2 minimum new depth, 3.. diagnostic sums
-    __shared__ double fold_lds[kSubFold * kBlock];
     double fv[kSubFold];
     fv[0] = wmax[0];
     fv[1] = ovf ? 1.0 : 0.0;
@@ -693,7 +706,6 @@ __global__ void __launch_bounds__(kBlock) v2_end_kernel(V2Args<P> a, int k)
 // summed per lane in float over the band, then in double; the mobile volume
 // itself comes exactly from the quanta per weight class.
 // ---------------------------------------------------------------------------
-constexpr int kLedger32D = 7;      // FP64 fields per workgroup: ledger 1..7 of warp_mobile_v2.py:41-54
 
 struct Ledger32 {
     float qx, qy, ke, pot, prs, vmax, hmin;   // sums weighted by the cell weight; max speed, min depth
@@ -831,14 +843,12 @@ __global__ void __launch_bounds__(kBlock) v2_begin32_kernel(V2Args<F32P> a)
     }
 }
 
-// copy back when the final state sits in scratch; ledger after (FP32 strips)
+// copy back when the final state sits in scratch; ledger after (FP32 strips).
+// The body runs in the first substep launch that finds the loop ended (same
+// tile grid as the substeps); lds holds kLedger32D x kBlock doubles.
 template <int SH>
-__global__ void __launch_bounds__(kBlock) v2_end32_kernel(V2Args<F32P> a, int k)
+__device__ inline void end32_body(const V2Args<F32P> &a, int fsrc, double *lds)
 {
-    __shared__ double lds[kLedger32D * kBlock];
-    const Control C = resolve_control(a.sync, k);
-    if (C.static_exit) return;
-    const int fsrc = C.src;
     const int ny = a.ny, nx = a.nx;
     const int tiles_x = (nx + kStripW - 1) / kStripW;
     const int t = xcd_block();
@@ -881,117 +891,128 @@ __global__ void __launch_bounds__(kBlock) v2_end32_kernel(V2Args<F32P> a, int k)
     store_ledger32(a, L, gc, owner, false, lds);
 }
 
-// Fold the partial ledgers and diagnostics in fixed order into the record.
-// One workgroup of kFinalizeBlock (512) threads.  Thread p accumulates parts p,
-// p + kFinalizeBlock, ... of every field, issuing all loads of two parts at once
-// (one memory round trip per part); a field-parallel LDS fold then reduces
-// the threads, each field by one wave.
+template <int SH>
+__global__ void __launch_bounds__(kBlock) v2_end32_kernel(V2Args<F32P> a, int k)
+{
+    __shared__ double lds[kLedger32D * kBlock];
+    const Control C = resolve_control(a.sync, k);
+    if (C.static_exit) return;
+    end32_body<SH>(a, C.src, lds);
+}
+
+// Fold the partial ledgers and diagnostics in fixed order.  One wave per
+// field: lane l sums the partials l, l + 64, ... (their loads issued
+// together), then a DPP tree combines the lanes -- no cross-wave or
+// cross-workgroup step, so the workgroups of the launch run independently.
+// The results go straight to pinned host memory (FoldOut); the last wave
+// copies out the resolved loop state and zeroes the next step's loop state
+// (the other parity's block and spread sets, unused during this step).
 constexpr int kFoldF = 2 * kSum + kDiag + 2;   // double fields: ledgers, diagnostics, static maxima
 constexpr int kFoldQ = 2 * kClasses;           // int64 fields: quanta per weight class
 // field 3 and kSum + 3 are maxima (speed), 7 and kSum + 7 minima (depth), the
 // last two (static-gradient and momentum spread maxima) maxima
 constexpr unsigned kFoldMax = (1u << 3) | (1u << (kSum + 3)) | (3u << (2 * kSum + kDiag));
 constexpr unsigned kFoldMin = (1u << 7) | (1u << (kSum + 7));
+constexpr int kFoldWaves = kFoldF + kFoldQ + 1;
+constexpr int kFoldUnroll = 8;
+
+struct FoldOut {
+    double f[kFoldF];
+    long long q[kFoldQ];
+    Control ctl;
+};
+static_assert(sizeof(FoldOut) <= 1024, "the fold output fits the pinned result block");
+
+template <int OP, class T>
+__device__ inline T lane_fold(const T *src, int n, T ident)
+{
+    const int lane = threadIdx.x & (kWave - 1);
+    T acc = ident;
+    for (int p0 = lane; p0 < n; p0 += kWave * kFoldUnroll) {
+        T x[kFoldUnroll];
+#pragma unroll
+        for (int u = 0; u < kFoldUnroll; u++) {
+            const int p = p0 + u * kWave;
+            x[u] = p < n ? src[p] : ident;
+        }
+#pragma unroll
+        for (int u = 0; u < kFoldUnroll; u++) acc = red_op<OP>(acc, x[u]);
+    }
+    return wave_reduce<OP>(acc);
+}
 
 template <class P>
-__global__ void __launch_bounds__(kFinalizeBlock) v2_finalize_kernel(V2Args<P> a, int k, int nparts, int ndiag)
+__global__ void __launch_bounds__(kBlock) v2_fold_kernel(V2Args<P> a, int k, int nparts, int ndiag, FoldOut *out)
 {
-    __shared__ double lds[kFoldF * kFinalizeBlock];
-    constexpr int kNW = kFinalizeBlock / kWave;
-    double fsum[kFoldF];
-#pragma unroll
-    for (int f = 0; f < kFoldF; f++) fsum[f] = ((kFoldMin >> f) & 1u) ? 1e300 : 0.0;
-    long long qsum[kFoldQ];
-#pragma unroll
-    for (int f = 0; f < kFoldQ; f++) qsum[f] = 0;
-    // kFoldU parts per thread and round, all their loads issued together
-    constexpr int kFoldU = 2;
-    for (int p0 = threadIdx.x; p0 < nparts; p0 += kFoldU * kFinalizeBlock) {
-        double x[kFoldU][2 * kSum];
-        long long q[kFoldU][kFoldQ];
-#pragma unroll
-        for (int u = 0; u < kFoldU; u++) {
-            const int p = p0 + u * kFinalizeBlock;
-#pragma unroll
-            for (int f = 0; f < 2 * kSum; f++)
-                x[u][f] = p < nparts ? a.ledger_f[(int64_t)f * nparts + p] : (((kFoldMin >> f) & 1u) ? 1e300 : 0.0);
-#pragma unroll
-            for (int f = 0; f < kFoldQ; f++) q[u][f] = p < nparts ? a.ledger_q[(int64_t)f * nparts + p] : 0;
-        }
-#pragma unroll
-        for (int u = 0; u < kFoldU; u++) {
-#pragma unroll
-            for (int f = 0; f < 2 * kSum; f++)
-                fsum[f] = ((kFoldMax >> f) & 1u) ? npmax(fsum[f], x[u][f])
-                                                 : (((kFoldMin >> f) & 1u) ? npmin(fsum[f], x[u][f]) : fsum[f] + x[u][f]);
-#pragma unroll
-            for (int f = 0; f < kFoldQ; f++) qsum[f] += q[u][f];
-        }
-    }
-    for (int p0 = threadIdx.x; p0 < ndiag; p0 += kFoldU * kFinalizeBlock) {
-        double x[kFoldU][kDiag];
-#pragma unroll
-        for (int u = 0; u < kFoldU; u++) {
-            const int p = p0 + u * kFinalizeBlock;
-#pragma unroll
-            for (int d = 0; d < kDiag; d++) x[u][d] = p < ndiag ? a.diag_part[(int64_t)d * ndiag + p] : 0.0;
-        }
-#pragma unroll
-        for (int u = 0; u < kFoldU; u++)
-#pragma unroll
-            for (int d = 0; d < kDiag; d++) fsum[2 * kSum + d] += x[u][d];
-    }
-    if (threadIdx.x < kWave) {
-        fsum[2 * kSum + kDiag] = __longlong_as_double((long long)a.spread[kSetStaticGrad * kSpreadWords + threadIdx.x * kSpreadStride]);
-        fsum[2 * kSum + kDiag + 1] = __longlong_as_double((long long)a.spread[kSetStaticQ * kSpreadWords + threadIdx.x * kSpreadStride]);
-    }
-    // the next step's loop state (other parity): unused during this step
-    {
+    const int w = blockIdx.x * (kBlock / kWave) + wave_index();
+    const int lane = threadIdx.x & (kWave - 1);
+    if (w < 2 * kSum) {
+        const double *src = a.ledger_f + (int64_t)w * nparts;
+        double v;
+        if ((kFoldMax >> w) & 1u) v = lane_fold<1>(src, nparts, 0.0);
+        else if ((kFoldMin >> w) & 1u) v = lane_fold<2>(src, nparts, 1e300);
+        else v = lane_fold<0>(src, nparts, 0.0);
+        if (lane == 0) out->f[w] = v;
+    } else if (w < 2 * kSum + kDiag) {
+        const int d = w - 2 * kSum;
+        const double v = lane_fold<0>(a.diag_part + (int64_t)d * ndiag, ndiag, 0.0);
+        if (lane == 0) out->f[w] = v;
+    } else if (w < kFoldF) {
+        const int set = w == 2 * kSum + kDiag ? kSetStaticGrad : kSetStaticQ;
+        const double x = __longlong_as_double((long long)a.spread[set * kSpreadWords + lane * kSpreadStride]);
+        const double v = wave_reduce<1>(x);
+        if (lane == 0) out->f[w] = v;
+    } else if (w < kFoldF + kFoldQ) {
+        const int q = w - kFoldF;
+        const long long v = lane_fold<0>(a.ledger_q + (int64_t)q * nparts, nparts, 0ll);
+        if (lane == 0) out->q[q] = v;
+    } else if (w == kFoldWaves - 1) {
+        if (lane == 0) out->ctl = resolve_control(a.sync, k);
         unsigned *ps = reinterpret_cast<unsigned *>(a.sync_next);
-        for (int i = threadIdx.x; i < (int)(sizeof(SyncBlock) / 4); i += blockDim.x) ps[i] = 0u;
-        for (int i = threadIdx.x; i < kStepSpreadSets * kSpread; i += blockDim.x)
-            a.spread_next[(int64_t)i * kSpreadStride] = 0ull;
+        for (int i = lane; i < (int)(sizeof(SyncBlock) / 4); i += kWave) ps[i] = 0u;
+        for (int i = lane; i < kStepSpreadSets * kSpread; i += kWave) a.spread_next[(int64_t)i * kSpreadStride] = 0ull;
     }
-    block_fold<double, kFoldF, kFoldMax, kFoldMin, kNW>(fsum, lds);
-    block_fold<long long, kFoldQ, 0u, 0u, kNW>(qsum, reinterpret_cast<long long *>(lds));
-    if (threadIdx.x == 0) {
-        // the loop state is read only here: the partial loads above do not wait for it
-        const Control C = resolve_control(a.sync, k);
-        const bool after_valid = !C.static_exit;
-        mv2_step_record *rec = a.rec;
-        for (int q = 0; q < kSum; q++) {
-            rec->summary_before[q] = fsum[q];
-            rec->summary_after[q] = after_valid ? fsum[kSum + q] : fsum[q];
-        }
-        for (int d = 0; d < kDiag; d++) rec->diag[d] = (after_valid && C.substeps > 0) ? fsum[2 * kSum + d] : 0.0;
-        if (P::kQuant) {
-            // exact mobile volume: integer quanta per weight class times its weight
-            const double area = a.dx * a.dy, f6 = a.dx * a.dy / 6.0;
-            const int rep_r[3] = {0, 1, a.ny - 1}, rep_c[3] = {0, 1, a.nx - 1};
-            double mb = 0.0, ma = 0.0, qb = 0.0, qa = 0.0;
-            for (int q = 0; q < kClasses; q++) {
-                double w = a.cfg.weight_mode ? vertex_weight(a.ny, a.nx, f6, rep_r[q / 3], rep_c[q % 3]) : area;
-                mb += ((double)qsum[q] * kQuantum) * w;
-                ma += ((double)qsum[kClasses + q] * kQuantum) * w;
-                qb += (double)qsum[q];
-                qa += (double)qsum[kClasses + q];
-            }
-            rec->summary_before[0] = mb;
-            rec->summary_after[0] = after_valid ? ma : mb;
-            rec->mass_quanta_before = qb;
-            rec->mass_quanta_after = after_valid ? qa : qb;
-        }
-        a.sync->static_grad = (unsigned long long)__double_as_longlong(fsum[2 * kSum + kDiag]);
-        a.sync->static_q = (unsigned long long)__double_as_longlong(fsum[2 * kSum + kDiag + 1]);
-        rec->substeps = C.substeps;
-        rec->cfl_limited = C.cfl_limited;
-        rec->static_exit = C.static_exit;
-        rec->maximum_cfl = C.max_cfl;
-        rec->remaining = C.remaining;
-        rec->numeric_error = C.numeric;
-        rec->status = C.numeric ? MV2_ENUMERIC : MV2_OK;
-        rec->complete = C.done;
+}
+
+// the step record from the folded values (host side, after the stream sync)
+template <class P>
+static void assemble_record(const FoldOut &fo, const V2Args<P> &a, mv2_step_record *rec, double static_out[2])
+{
+    const Control &C = fo.ctl;
+    const bool after_valid = !C.static_exit;
+    for (int q = 0; q < kSum; q++) {
+        rec->summary_before[q] = fo.f[q];
+        rec->summary_after[q] = after_valid ? fo.f[kSum + q] : fo.f[q];
     }
+    for (int d = 0; d < kDiag; d++) rec->diag[d] = (after_valid && C.substeps > 0) ? fo.f[2 * kSum + d] : 0.0;
+    rec->mass_quanta_before = rec->mass_quanta_after = 0.0;
+    if (P::kQuant) {
+        // exact mobile volume: integer quanta per weight class times its weight
+        const double area = a.dx * a.dy, f6 = a.dx * a.dy / 6.0;
+        const int rep_r[3] = {0, 1, a.ny - 1}, rep_c[3] = {0, 1, a.nx - 1};
+        double mb = 0.0, ma = 0.0, qb = 0.0, qa = 0.0;
+        for (int q = 0; q < kClasses; q++) {
+            const double w = a.cfg.weight_mode ? vertex_weight(a.ny, a.nx, f6, rep_r[q / 3], rep_c[q % 3]) : area;
+            mb += ((double)fo.q[q] * kQuantum) * w;
+            ma += ((double)fo.q[kClasses + q] * kQuantum) * w;
+            qb += (double)fo.q[q];
+            qa += (double)fo.q[kClasses + q];
+        }
+        rec->summary_before[0] = mb;
+        rec->summary_after[0] = after_valid ? ma : mb;
+        rec->mass_quanta_before = qb;
+        rec->mass_quanta_after = after_valid ? qa : qb;
+    }
+    static_out[0] = fo.f[2 * kSum + kDiag];
+    static_out[1] = fo.f[2 * kSum + kDiag + 1];
+    rec->substeps = C.substeps;
+    rec->cfl_limited = C.cfl_limited;
+    rec->static_exit = C.static_exit;
+    rec->maximum_cfl = C.max_cfl;
+    rec->remaining = C.remaining;
+    rec->numeric_error = C.numeric;
+    rec->status = C.numeric ? MV2_ENUMERIC : MV2_OK;
+    rec->complete = C.done && (!P::kQuant || C.end_done);
 }
 
 template <class P>
@@ -1029,7 +1050,7 @@ static int launch_step(mv2_context *ctx, const mv2_step_config *cfg, mv2_step_re
     // the begin / end passes use the substep's tiles (same band)
     const int gl = P::kQuant ? ((ctx->nx + kStripW - 1) / kStripW) * ((ctx->ny + rows_sub - 1) / rows_sub) : 0;
     // begin / end passes: the substep tiles (FP32), a grid stride (FP64)
-    const int gs = P::kQuant ? gl : grid_for(n, kBlock * kLedgerU, kFinalizeBlock);
+    const int gs = P::kQuant ? gl : grid_for(n, kBlock * kLedgerU, kLedgerGroupsMax);
     if (int rc = ensure_scratch(ctx, ctx->partials, (sizeof(double) * kLedgerF + sizeof(long long) * kLedgerQ) * (size_t)gs))
         return rc;
     if (int rc = ensure_scratch(ctx, ctx->step_diag, sizeof(double) * kDiag * (size_t)gu)) return rc;
@@ -1059,9 +1080,9 @@ static int launch_step(mv2_context *ctx, const mv2_step_config *cfg, mv2_step_re
     a.diag_part = (double *)ctx->step_diag.ptr;
     a.diag_groups = gu;
     a.spread = parity ? spread_odd : spread_even;
-    // the finalize pass writes the record straight into pinned host memory
+    // the fold pass writes its results straight into pinned host memory
     // (device-visible): no copy launch before the stream sync
-    a.rec = (mv2_step_record *)ctx->host_result.ptr;
+    FoldOut *fold = (FoldOut *)ctx->host_result.ptr;
     a.clamp_quarter = cfg->cfl <= 0.25 ? 1 : 0;
     hipStream_t st = ctx->stream;
     const bool timing = ctx->timing != 0;
@@ -1085,7 +1106,7 @@ static int launch_step(mv2_context *ctx, const mv2_step_config *cfg, mv2_step_re
     int chunk = fixed_chunk > 0 ? fixed_chunk : (ctx->v2_predicted > 0 ? ctx->v2_predicted + 1 : 4);
     // launches expected to work stage their tile before the loop state arrives
     int eager_until = fixed_chunk > 0 ? 0 : (ctx->v2_predicted > 0 ? ctx->v2_predicted : 4);
-    const mv2_step_record *hrec = (const mv2_step_record *)ctx->host_result.ptr;
+    const int gfold = (kFoldWaves + kBlock / kWave - 1) / (kBlock / kWave);
     for (;;) {
         if (timing && (int)ctx->sub_events.size() < 2 * (k + chunk)) {
             while ((int)ctx->sub_events.size() < 2 * (k + chunk)) {
@@ -1110,27 +1131,21 @@ static int launch_step(mv2_context *ctx, const mv2_step_config *cfg, mv2_step_re
             }
             if (timing) MV2_HIP(ctx, hipEventRecord(ctx->sub_events[2 * k + 1], st));
         }
-        if constexpr (P::kQuant) {
-            switch (band) {
-            case 2: hipLaunchKernelGGL(v2_end32_kernel<2>, dim3(gs), dim3(kBlock), 0, st, a, k); break;
-            case 3: hipLaunchKernelGGL(v2_end32_kernel<3>, dim3(gs), dim3(kBlock), 0, st, a, k); break;
-            case 5: hipLaunchKernelGGL(v2_end32_kernel<5>, dim3(gs), dim3(kBlock), 0, st, a, k); break;
-            case 6: hipLaunchKernelGGL(v2_end32_kernel<6>, dim3(gs), dim3(kBlock), 0, st, a, k); break;
-            default: hipLaunchKernelGGL(v2_end32_kernel<4>, dim3(gs), dim3(kBlock), 0, st, a, k); break;
-            }
-        }
-        else hipLaunchKernelGGL(v2_end_kernel<P>, dim3(gs), dim3(kBlock), 0, st, a, k);
-        hipLaunchKernelGGL(v2_finalize_kernel<P>, dim3(1), dim3(kFinalizeBlock), 0, st, a, k, gs, gu);
+        // FP32: the first launch past the loop's end did the end pass; FP64
+        // runs it as a launch of its own
+        if constexpr (!P::kQuant) hipLaunchKernelGGL(v2_end_kernel<P>, dim3(gs), dim3(kBlock), 0, st, a, k);
+        hipLaunchKernelGGL(v2_fold_kernel<P>, dim3(gfold), dim3(kBlock), 0, st, a, k, gs, gu, fold);
         MV2_HIP(ctx, hipGetLastError());
         MV2_HIP(ctx, hipEventRecord(ctx->ev_stop, st));
         MV2_HIP(ctx, hipStreamSynchronize(st));
-        if (hrec->complete) break;
+        if (fold->ctl.done && (!P::kQuant || fold->ctl.end_done)) break;
         // more substeps than enqueued: the state so far is valid, enqueue more
         chunk = chunk * 2 > 4 ? chunk * 2 : 4;
         eager_until = k + chunk;
     }
-    *out = *hrec;
-    ctx->v2_clean = parity ^ 1;          // zeroed by this step's finalize pass
+    *out = mv2_step_record{};
+    assemble_record<P>(*fold, a, out, ctx->v2_static);
+    ctx->v2_clean = parity ^ 1;          // zeroed by this step's fold pass
     ctx->v2_frame++;
     ctx->v2_last_sync = parity;
     if (fixed_chunk == 0) ctx->v2_predicted = out->substeps;
@@ -1226,11 +1241,8 @@ extern "C" int mv2_static_diagnostic(mv2_context *ctx, double out[2])
     int rc = ctx->precision == MV2_FP64 ? launch_step<F64P>(ctx, &c, &rec, 1)
                                         : launch_step<F32P>(ctx, &c, &rec, 1);
     if (rc) return rc;
-    SyncBlock sb;
-    MV2_HIP(ctx, hipMemcpy(&sb, (char *)ctx->sync.ptr + ctx->v2_last_sync * kSyncParityBytes, sizeof(SyncBlock),
-                           hipMemcpyDeviceToHost));
-    out[0] = __builtin_bit_cast(double, sb.static_grad);
-    out[1] = __builtin_bit_cast(double, sb.static_q);
+    out[0] = ctx->v2_static[0];
+    out[1] = ctx->v2_static[1];
     return MV2_OK;
 }
 

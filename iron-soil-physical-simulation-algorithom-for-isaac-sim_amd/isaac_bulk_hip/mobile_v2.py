@@ -111,18 +111,33 @@ def _step_config(config: MobileV2Config, dt: float, *, density: float, limit_mod
         weight_mode=int(weight_mode), static_exit=1)
 
 
+_RECORD_BYTES = ctypes.sizeof(_lib.StepRecord)
+_CONFIG_CACHE: dict = {}
+
+
 def run_step(state: DeviceBulkState, config: MobileV2Config, dt: float, *, density: float,
              limit_mode: int, weight_mode: int, external=(0.0, 0.0),
              error_prefix: str = "") -> _lib.StepRecord:
     """Run one device step on ``state`` and return the raw record."""
     rt = state.runtime
     rt.bind_stream()
-    cfg = _step_config(config, dt, density=density, limit_mode=limit_mode,
-                       weight_mode=weight_mode, external=external)
+    # configs are frozen dataclasses: the C struct of a (config, dt, ...) key is reused
+    key = (config, float(dt), float(density), int(limit_mode), int(weight_mode), tuple(external))
+    try:
+        cfg = _CONFIG_CACHE.get(key)
+    except TypeError:                      # an unhashable caller config: no caching
+        key, cfg = None, None
+    if cfg is None:
+        if len(_CONFIG_CACHE) > 64:
+            _CONFIG_CACHE.clear()
+        cfg = _step_config(config, dt, density=density, limit_mode=limit_mode,
+                           weight_mode=weight_mode, external=external)
+        if key is not None:
+            _CONFIG_CACHE[key] = cfg
     rec = _lib.StepRecord()
     rc = rt.lib.mv2_step(rt.ctx, ctypes.byref(cfg), ctypes.byref(rec))
     rt.telemetry.record_call(launches=int(rec.launches) + 4, syncs=1)
-    rt.telemetry.record_d2h(np.zeros(ctypes.sizeof(rec) // 8))
+    rt.telemetry.record_d2h_bytes(_RECORD_BYTES)
     if rc != _lib.MV2_OK:
         msg = _lib.last_error(rt.ctx)
         if rc == _lib.MV2_EINVAL:
@@ -171,9 +186,10 @@ class HipProductionMobileV2Solver:
         rec = run_step(state, self.config, float(dt_s), density=density, limit_mode=1,
                        weight_mode=1, error_prefix="[MobileV2Production] ")
         self.last_record = rec
-        before = np.asarray(rec.summary_before[:])
-        after = np.asarray(rec.summary_after[:])
-        diag = np.asarray(rec.diag[:])
+        # zero-copy views of the record (a new record per call, never mutated)
+        before = np.frombuffer(rec.summary_before, dtype=np.float64)
+        after = np.frombuffer(rec.summary_after, dtype=np.float64)
+        diag = np.frombuffer(rec.diag, dtype=np.float64)
         area = state.grid.dx * state.grid.dy
         before_p = np.asarray([before[1], before[2], 0.0])
         if rec.static_exit:

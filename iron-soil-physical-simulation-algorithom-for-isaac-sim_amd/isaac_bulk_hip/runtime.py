@@ -38,6 +38,10 @@ class TransferTelemetry:
         self.d2h_bytes += int(np.asarray(value).nbytes)
         self.d2h_transfer_count += 1
 
+    def record_d2h_bytes(self, nbytes: int) -> None:
+        self.d2h_bytes += int(nbytes)
+        self.d2h_transfer_count += 1
+
     def record_call(self, launches: int = 1, syncs: int = 1) -> None:
         self.kernel_launch_count += launches
         self.synchronization_count += syncs
@@ -52,6 +56,16 @@ def _torch():
     import torch  # local import: the CPU test suite never needs a device
 
     return torch
+
+
+def _raw_stream(device_index: int) -> int:
+    """Handle of torch's current HIP stream on a device (the raw query avoids
+    building a Stream object on every call)."""
+    torch = _torch()
+    raw = getattr(torch._C, "_cuda_getCurrentRawStream", None)
+    if raw is not None:
+        return int(raw(device_index))
+    return int(torch.cuda.current_stream(device_index).cuda_stream)
 
 
 # Live runtimes in creation order.  At interpreter exit they are closed by an
@@ -109,15 +123,20 @@ class HipRuntime:
         if rc != _lib.MV2_OK:
             raise HipBackendUnavailable(_lib.last_error(None))
         self.ctx = handle.value
+        self._device_index = int(index)
+        self._bound_stream = None
         self.arrays: dict[str, object] = {}
         self.telemetry = TransferTelemetry()
         _register(self)
 
     # -- stream / sync ---------------------------------------------------------
     def bind_stream(self) -> None:
-        torch = _torch()
-        stream = torch.cuda.current_stream(self.device).cuda_stream
-        self.lib.mv2_set_stream(self.ctx, ctypes.c_void_p(stream))
+        """Hand torch's current stream on this device to the library (only
+        when it changed since the last call)."""
+        stream = _raw_stream(self._device_index)
+        if stream != self._bound_stream:
+            self.lib.mv2_set_stream(self.ctx, ctypes.c_void_p(stream))
+            self._bound_stream = stream
 
     def synchronize(self) -> None:
         _torch().cuda.synchronize(self.device)
@@ -162,6 +181,7 @@ class HipRuntime:
         library then drains the whole device), then its scratch is freed."""
         if getattr(self, "ctx", None):
             ctx, self.ctx = self.ctx, None
+            self._bound_stream = None
             self.lib.mv2_set_stream(ctx, None)
             self.lib.mv2_destroy(ctx)
 
