@@ -62,7 +62,7 @@ typedef struct mv2_step_config {
     double dry_tolerance;
     double basal_friction;            /* mu */
     double tan_start;                 /* tan(start angle) for the static early exit */
-    double external_ax, external_ay;  /* uniform external acceleration */
+    double external_ax, external_ay;  /* uniform external acceleration (see mv2_set_external) */
     double density;                   /* ledger density (1.0 = per-density) */
     int maximum_substeps;
     int limit_mode;                   /* 0: mobile_v2_reference.py:237, 1: warp_mobile_v2.py:215 */
@@ -100,6 +100,16 @@ typedef struct mv2_step_record {
  * (pkg/experimental/mobile_v2_warp.py:251-344): static check, device-side CFL
  * loop with no host round trip per substep, ledgers before/after. */
 int mv2_step(mv2_context *ctx, const mv2_step_config *cfg, mv2_step_record *out);
+
+/* Per-cell external acceleration of the following mv2_step calls: ax / ay are
+ * DEVICE arrays of ny*nx values in the state's precision (double for
+ * MV2_FP64, float for MV2_FP32), used instead of cfg->external_ax/ay; NULL
+ * (both) returns to the uniform values.  max_abs = max |external| over the
+ * cells (the static early exit needs it <= 1e-14).  Replaces the
+ * external_acceleration_m_s2 array of MobileV2ReferenceSolver.step
+ * (pkg/experimental/mobile_v2_reference.py:191-210, applied :276-304).  The
+ * arrays are borrowed: keep them alive while set. */
+int mv2_set_external(mv2_context *ctx, const void *ax, const void *ay, double max_abs);
 
 /* Timing mode (no counterpart in the reference; a measurement aid): when
  * enabled, mv2_step brackets every substep launch with HIP events and reports

@@ -84,6 +84,7 @@ struct InstView {
     float *qy[2];
     mv2_step_config cfg;                // a copy: no address of the kernel arguments is taken
     int clamp_quarter;
+    const float *ext_x = nullptr, *ext_y = nullptr;   // batches take the uniform cfg acceleration
     __device__ InstView(const BatchArgs &a, int i)
         : ny(a.inst_ny[i]), nx(a.inst_nx[i]), dx(a.dx), dy(a.dy), cfg(a.cfg), clamp_quarter(a.clamp_quarter)
     {

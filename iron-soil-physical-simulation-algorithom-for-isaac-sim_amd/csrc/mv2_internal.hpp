@@ -67,6 +67,9 @@ struct mv2_context {
     int v2_clean = -1;                    // parity whose loop state is known zeroed, -1 none
     int v2_last_sync = 0;                 // parity of the last completed step
     double v2_static[2] = {0.0, 0.0};     // static diagnostic of the last step's input: max |grad H|, max |q|
+    const void *ext_x = nullptr;          // per-cell external acceleration (state precision), or null
+    const void *ext_y = nullptr;
+    double ext_max = 0.0;                 // max |external| over the cells (static exit test)
     int relax_predicted = 0;              // relaxation launches to enqueue up front
     int frontier_predicted = 0;           // frontier rounds to enqueue up front
     mv2_batch_state *batch = nullptr;     // batched terrains bound to this context

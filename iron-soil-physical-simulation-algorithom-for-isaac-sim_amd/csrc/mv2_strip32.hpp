@@ -262,7 +262,7 @@ __device__ inline void strip_update(const A &a, const StripRows<SH> &R, int src,
     const int gc = c0 - 1 + lane;
     const bool owner = lane >= 1 && lane <= kStripW && gc < nx;
     const Consts32 k = consts32(a.cfg, sub_dt, a.dx, a.dy, a.clamp_quarter);
-    const float dt = k.dt, ax = k.ax, ay = k.ay, decrement = k.decrement;
+    const float dt = k.dt, decrement = k.decrement;
     const int dst = src ^ 1;
     int32_t *oh = (dst ? a.h[1] : a.h[0]);
     float *oqx = (dst ? a.qx[1] : a.qx[0]);
@@ -334,6 +334,11 @@ __device__ inline void strip_update(const A &a, const StripRows<SH> &R, int src,
             float qax = 0.0f, qay = 0.0f, wave_v = 0.0f;
             const bool wet = depth > k.dry;
             if (wet) {
+                float ax = k.ax, ay = k.ay;
+                if (a.ext_x) {                          // per-cell external acceleration
+                    ax = a.ext_x[gr * nx + gc];
+                    ay = a.ext_y[gr * nx + gc];
+                }
                 const float inv = rcpf(depth);
                 const float vbx = qxn * inv, vby = qyn * inv;
                 const float vex = vbx + ax * dt, vey = vby + ay * dt;

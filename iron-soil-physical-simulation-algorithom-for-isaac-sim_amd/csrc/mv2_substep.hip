@@ -107,6 +107,10 @@ struct V2Args {
     typename P::real *qx[2];
     typename P::real *qy[2];
     mv2_step_config cfg;
+    // per-cell external acceleration (state precision; null: the uniform
+    // cfg.external_ax / external_ay) and max |external| for the static exit
+    const typename P::real *ext_x, *ext_y;
+    double ext_max;
     SyncBlock *sync;
     SyncBlock *sync_next;                // the other parity's block, zeroed by the finalize pass
     unsigned long long *spread_next;     // the other parity's spread sets, zeroed likewise
@@ -235,7 +239,8 @@ __device__ inline CellOut cell64(const V2Args<F64P> &a, int src, int r, int c, d
     acc.diag[0] += qxn - qxi;
     acc.diag[1] += qyn - qyi;
     double depth = npmax(hn, 0.0);
-    const double ax = a.cfg.external_ax, ay = a.cfg.external_ay;
+    const double ax = a.ext_x ? a.ext_x[i] : a.cfg.external_ax;
+    const double ay = a.ext_y ? a.ext_y[i] : a.cfg.external_ay;
     const double mug = a.cfg.basal_friction * g;
     double qax = 0.0, qay = 0.0;
     if (depth > dry) {
@@ -547,7 +552,7 @@ __global__ void __launch_bounds__(kBlock) v2_substep_kernel(V2Args<P> a, int k, 
     Control C = resolve_control(s, k);
     if (!C.done && k == 0 && cfg.static_exit) {
         // warp_mobile_v2.py:148-160 - globally static state below the start angle
-        const double ext_max = npmax(0.0, npmax(fabs(cfg.external_ax), fabs(cfg.external_ay)));
+        const double ext_max = a.ext_x ? a.ext_max : npmax(0.0, npmax(fabs(cfg.external_ax), fabs(cfg.external_ay)));
         const double qmax = spread_read_max(a.spread + kSetStaticQ * kSpreadWords, smem);
         const double gmax = spread_read_max(a.spread + kSetStaticGrad * kSpreadWords, smem);
         if (ext_max <= 1.0e-14 && qmax <= 1.0e-14 && gmax <= cfg.tan_start + 1.0e-14) {
@@ -1068,6 +1073,9 @@ static int launch_step(mv2_context *ctx, const mv2_step_config *cfg, mv2_step_re
     a.qy[0] = (typename P::real *)ctx->qy;
     a.qy[1] = (typename P::real *)ctx->alt_qy.ptr;
     a.cfg = *cfg;
+    a.ext_x = (const typename P::real *)ctx->ext_x;
+    a.ext_y = (const typename P::real *)ctx->ext_y;
+    a.ext_max = ctx->ext_max;
     const int parity = ctx->v2_frame & 1;
     unsigned long long *spread_even = (unsigned long long *)ctx->spread.ptr;
     unsigned long long *spread_odd = spread_even + (size_t)kStepOddSpreadSet * kSpreadWords;
@@ -1243,6 +1251,19 @@ extern "C" int mv2_static_diagnostic(mv2_context *ctx, double out[2])
     if (rc) return rc;
     out[0] = ctx->v2_static[0];
     out[1] = ctx->v2_static[1];
+    return MV2_OK;
+}
+
+extern "C" int mv2_set_external(mv2_context *ctx, const void *ax, const void *ay, double max_abs)
+{
+    if (!ctx) return set_error(nullptr, MV2_EINVAL, "[MobileV2] null context");
+    if ((ax == nullptr) != (ay == nullptr))
+        return set_error(ctx, MV2_EINVAL, "MobileV2 external acceleration needs both components");
+    if (ax && !(max_abs >= 0.0 && std::isfinite(max_abs)))
+        return set_error(ctx, MV2_EINVAL, "MobileV2 external acceleration must be finite");
+    ctx->ext_x = ax;
+    ctx->ext_y = ay;
+    ctx->ext_max = ax ? max_abs : 0.0;
     return MV2_OK;
 }
 

@@ -266,17 +266,35 @@ class HipMobileV2ReferenceSolver:
         if not np.isfinite(dt) or dt <= 0.0:
             raise ValueError("dt must be finite/positive")
         ext = (0.0, 0.0)
+        per_cell = None
         if external_acceleration_m_s2 is not None:
+            # broadcast to q.shape like the reference (mobile_v2_reference.py:204-210)
             e = np.asarray(external_acceleration_m_s2, dtype=np.float64)
-            if e.shape != (2,):
-                raise ValueError("device MobileV2 supports a uniform (2,) external acceleration")
-            ext = (float(e[0]), float(e[1]))
+            if e.shape == (2,):
+                ext = (float(e[0]), float(e[1]))
+            else:
+                per_cell = np.broadcast_to(e, np.shape(state.q_m2_s))
+                if not np.all(np.isfinite(per_cell)):
+                    raise ValueError("MobileV2 external acceleration must be finite")
         dev = self._device_state(state.h_m.shape)
         q = np.asarray(state.q_m2_s, dtype=np.float64)
         dev.set_fields(z_base=np.zeros(state.h_m.shape), b_eff=state.b_eff_m, mobile=state.h_m,
                        momentum_x=q[..., 0], momentum_y=q[..., 1])
-        rec = run_step(dev, self.config, dt, density=1.0, limit_mode=0, weight_mode=0,
-                       external=ext, error_prefix="Warp ")
+        rt = dev.runtime
+        if per_cell is not None:
+            fdt = rt.arrays["momentum_x"].dtype
+            ex = rt.upload("v2_external_x", np.ascontiguousarray(per_cell[..., 0]).ravel(), fdt)
+            ey = rt.upload("v2_external_y", np.ascontiguousarray(per_cell[..., 1]).ravel(), fdt)
+            rt.check(rt.lib.mv2_set_external(rt.ctx, ex.data_ptr(), ey.data_ptr(),
+                                             float(np.max(np.abs(per_cell), initial=0.0))))
+        try:
+            rec = run_step(dev, self.config, dt, density=1.0, limit_mode=0, weight_mode=0,
+                           external=ext, error_prefix="Warp ")
+        finally:
+            if per_cell is not None:
+                rt.check(rt.lib.mv2_set_external(rt.ctx, None, None, 0.0))
+                rt.release("v2_external_x")
+                rt.release("v2_external_y")
         view = dev.explicit_host_view(source="acceptance")
         final = MobileV2State(np.array(state.b_eff_m, dtype=np.float64, copy=True),
                               view.H_mobile_m, view.mobile_momentum_m2_s)

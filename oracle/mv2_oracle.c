@@ -161,12 +161,29 @@ int orc_v2_step(int ny, int nx, const orc_v2_params *p,
                 const double *b, double *h, double *qx, double *qy,
                 double dt, orc_v2_result *out)
 {
+    return orc_v2_step_ext(ny, nx, p, b, h, qx, qy, dt, NULL, NULL, out);
+}
+
+/* The step with a per-cell external acceleration (ext_x/ext_y of ny*nx, or
+ * NULL for the uniform p->external_ax/ay): the reference broadcasts
+ * external_acceleration_m_s2 to q.shape (mobile_v2_reference.py:204-210) and
+ * skips the static exit unless max |external| <= 1e-14 (:226-229). */
+int orc_v2_step_ext(int ny, int nx, const orc_v2_params *p,
+                    const double *b, double *h, double *qx, double *qy,
+                    double dt, const double *ext_x, const double *ext_y, orc_v2_result *out)
+{
     memset(out, 0, sizeof(*out));
     const int64_t n = (int64_t)ny * nx;
     const double dx = p->dx, dy = p->dy;
     const double K = p->earth_pressure, g = p->gravity, dry = p->dry_tolerance;
-    const double ax = p->external_ax, ay = p->external_ay;
-    double ext_max = npmax(0.0, npmax(fabs(ax), fabs(ay)));
+#define AX(i) (ext_x ? ext_x[i] : p->external_ax)
+#define AY(i) (ext_y ? ext_y[i] : p->external_ay)
+    double ext_max = 0.0;
+    if (ext_x || ext_y) {
+        for (int64_t i = 0; i < n; i++) ext_max = npmax(ext_max, npmax(fabs(AX(i)), fabs(AY(i))));
+    } else {
+        ext_max = npmax(0.0, npmax(fabs(p->external_ax), fabs(p->external_ay)));
+    }
     if (p->static_exit && ext_max <= 1.0e-14 &&
         static_below_start(ny, nx, dx, dy, b, h, qx, qy, p->tan_start)) {
         out->static_exit = 1;
@@ -270,7 +287,7 @@ int orc_v2_step(int ny, int nx, const orc_v2_params *p,
             for (int k = 2; k < 8; k++) t[k] = 0.0;
             if (depth <= dry) continue;
             double vx0 = qxn[i] / depth, vy0 = qyn[i] / depth;
-            double vx1 = vx0 + ax * sub_dt, vy1 = vy0 + ay * sub_dt;
+            double vx1 = vx0 + AX(i) * sub_dt, vy1 = vy0 + AY(i) * sub_dt;
             t[2] = depth * (vx1 - vx0);
             t[3] = depth * (vy1 - vy0);
             t[6] = 0.5 * depth * (vx1 * vx1 + vy1 * vy1 - vx0 * vx0 - vy0 * vy0);
@@ -293,7 +310,7 @@ int orc_v2_step(int ny, int nx, const orc_v2_params *p,
             out->diag[1] += qyn[i] - qy[i];
             if (depth <= dry) continue;
             double vx0 = qxn[i] / depth, vy0 = qyn[i] / depth;
-            double vx1 = vx0 + ax * sub_dt, vy1 = vy0 + ay * sub_dt;
+            double vx1 = vx0 + AX(i) * sub_dt, vy1 = vy0 + AY(i) * sub_dt;
             out->diag[2] += depth * (vx1 - vx0);
             out->diag[3] += depth * (vy1 - vy0);
             out->diag[6] += 0.5 * depth * (vx1 * vx1 + vy1 * vy1 - vx0 * vx0 - vy0 * vy0);
@@ -320,6 +337,7 @@ int orc_v2_step(int ny, int nx, const orc_v2_params *p,
         for (int64_t i = 0; i < n; i++) {
             int wet = hn[i] > dry;
             double vbx = wet ? qxn[i] / hn[i] : 0.0, vby = wet ? qyn[i] / hn[i] : 0.0;
+            const double ax = AX(i), ay = AY(i);
             double qex = qxn[i] + hn[i] * ax * sub_dt;
             double qey = qyn[i] + hn[i] * ay * sub_dt;
             double vex = wet ? qex / hn[i] : 0.0, vey = wet ? qey / hn[i] : 0.0;
@@ -358,6 +376,8 @@ int orc_v2_step(int ny, int nx, const orc_v2_params *p,
     out->maximum_cfl = max_cfl;
     out->remaining = remaining;
     return status;
+#undef AX
+#undef AY
 }
 
 void orc_v2_static(int ny, int nx, double dx, double dy,

@@ -56,6 +56,9 @@ void orc_v2_face(double bl, double br, double hl, double hr,
 /* numpy-compatible pairwise sum of a contiguous double array */
 double orc_pairwise_sum(const double *a, int64_t n, int64_t stride);
 
+int orc_v2_step_ext(int ny, int nx, const orc_v2_params *p,
+                    const double *b, double *h, double *qx, double *qy,
+                    double dt, const double *ext_x, const double *ext_y, orc_v2_result *out);
 int orc_v2_step(int ny, int nx, const orc_v2_params *p,
                 const double *b, double *h, double *qx, double *qy,
                 double dt, orc_v2_result *out);

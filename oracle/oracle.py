@@ -77,6 +77,9 @@ def lib() -> ctypes.CDLL:
         _lib.orc_v2_step.argtypes = [_I, _I, ctypes.POINTER(V2Params), _P, _P, _P, _P, _D,
                                      ctypes.POINTER(V2Result)]
         _lib.orc_v2_step.restype = _I
+        _lib.orc_v2_step_ext.argtypes = [_I, _I, ctypes.POINTER(V2Params), _P, _P, _P, _P, _D, _P, _P,
+                                         ctypes.POINTER(V2Result)]
+        _lib.orc_v2_step_ext.restype = _I
         _lib.orc_v2_face.argtypes = [_D] * 8 + [_I, _D, _D, _D, _D, _P]
         _lib.orc_v2_static.argtypes = [_I, _I, _D, _D, _P, _P, _P, _P, _P]
         _lib.orc_vertex_weight.restype = _D
@@ -149,7 +152,9 @@ def v2_step(b, h, qx, qy, dt, *, dx=0.05, dy=0.05, gravity=9.81, K=0.45, cfl=0.1
             dry=1.0e-10, mu=0.55, tan_start=None, start_angle_deg=34.0,
             external=(0.0, 0.0), maximum_substeps=4096, limit_mode=0, static_exit=True, threads=False):
     """Restated ``MobileV2ReferenceSolver.step`` (FP64).  Returns new fields + record.
-    threads=True runs the OpenMP build of the same restatement."""
+    ``external`` is a uniform (ax, ay) pair or any array broadcastable to
+    (ny, nx, 2), per cell as the reference takes it.  threads=True runs the
+    OpenMP build of the same restatement."""
     b = _f64(b)
     h = _f64(h).copy()
     qx = _f64(qx).copy()
@@ -157,11 +162,21 @@ def v2_step(b, h, qx, qy, dt, *, dx=0.05, dy=0.05, gravity=9.81, K=0.45, cfl=0.1
     ny, nx = b.shape
     if tan_start is None:
         tan_start = float(np.tan(np.deg2rad(start_angle_deg)))
-    p = V2Params(dx, dy, gravity, K, cfl, dry, mu, tan_start, float(external[0]),
-                 float(external[1]), int(maximum_substeps), int(limit_mode), int(bool(static_exit)))
+    ext = np.asarray(external, dtype=np.float64)
+    per_cell = ext.shape != (2,)
+    ux, uy = (0.0, 0.0) if per_cell else (float(ext[0]), float(ext[1]))
+    p = V2Params(dx, dy, gravity, K, cfl, dry, mu, tan_start, ux, uy,
+                 int(maximum_substeps), int(limit_mode), int(bool(static_exit)))
     r = V2Result()
-    (lib_threads() if threads else lib()).orc_v2_step(ny, nx, ctypes.byref(p), _ptr(b), _ptr(h), _ptr(qx),
-                                                      _ptr(qy), float(dt), ctypes.byref(r))
+    if per_cell:
+        # mobile_v2_reference.py:204-210: broadcast to q.shape = (ny, nx, 2)
+        e = np.ascontiguousarray(np.broadcast_to(ext, (ny, nx, 2)))
+        ex, ey = np.ascontiguousarray(e[..., 0]), np.ascontiguousarray(e[..., 1])
+        lib().orc_v2_step_ext(ny, nx, ctypes.byref(p), _ptr(b), _ptr(h), _ptr(qx), _ptr(qy), float(dt),
+                              _ptr(ex), _ptr(ey), ctypes.byref(r))
+    else:
+        (lib_threads() if threads else lib()).orc_v2_step(ny, nx, ctypes.byref(p), _ptr(b), _ptr(h), _ptr(qx),
+                                                          _ptr(qy), float(dt), ctypes.byref(r))
     rec = {
         "status": r.status, "substeps": r.substeps, "cfl_limited": bool(r.cfl_limited),
         "static_exit": bool(r.static_exit), "maximum_cfl": r.maximum_cfl,

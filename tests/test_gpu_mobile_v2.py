@@ -172,3 +172,69 @@ def test_gate_a_contracts_on_device():
     snap = ledger.snapshot(state, payload_m3=0.0, airborne_m3=0.0, outflow_m3=0.0)
     assert abs(snap.absolute_volume_error_m3) <= 2e-12
     assert state.runtime.arrays["resting"] is state.runtime.arrays["b_eff"]
+
+
+def test_c1_cfl_0_4_through_the_opt_in_bit_exact():
+    """C1 (128^2 pile, mu = tan 35 deg, 1.0 s) at CFL 0.4: the reference
+    rejects CFL > 0.25 (mobile_v2_reference.py:37-38), so it runs through the
+    explicit opt-in; the FP64 fields equal the oracle's at the same CFL bit for
+    bit (or both stop with the same positivity error)."""
+    from isaac_bulk_hip import MobileV2Config, MobileV2State
+    from isaac_bulk_hip.workloads import pile_with_patch
+
+    with pytest.raises(ValueError, match="0.25"):
+        MobileV2Config(cfl=0.4)
+    b, h, qx, qy = pile_with_patch(128, seed=7)
+    mu = float(np.tan(np.deg2rad(35.0)))
+    solver = _solver(cfl=0.4, basal_friction_coefficient=mu, allow_cfl_above_quarter=True)
+    h1, qx1, qy1, rec = oracle.v2_step(b, h, qx, qy, 1.0, cfl=0.4, mu=mu)
+    if rec["status"] == 1:
+        with pytest.raises(RuntimeError, match="positivity"):
+            solver.step(MobileV2State(b, h, np.stack((qx, qy), -1)), 1.0)
+        return
+    step = solver.step(MobileV2State(b, h, np.stack((qx, qy), -1)), 1.0)
+    assert rec["status"] == 0 and step.substeps == rec["substeps"] > 5
+    assert step.maximum_cfl == rec["maximum_cfl"] and rec["maximum_cfl"] > 0.25
+    np.testing.assert_array_equal(step.state.h_m, h1)
+    np.testing.assert_array_equal(step.state.q_m2_s[..., 0], qx1)
+    np.testing.assert_array_equal(step.state.q_m2_s[..., 1], qy1)
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_per_cell_external_acceleration_matches_oracle(precision):
+    """A per-cell external acceleration (the reference's array broadcast to
+    q.shape, mobile_v2_reference.py:204-210): FP64 fields bit-exact against
+    the oracle, FP32 within the frame tolerances; the impulse and work of the
+    push agree with the oracle's ledgers."""
+    from isaac_bulk_hip import MobileV2State
+    from isaac_bulk_hip.workloads import pile_with_patch
+
+    b, h, qx, qy = pile_with_patch(96, seed=5)
+    rng = np.random.default_rng(3)
+    ext = np.zeros((96, 96, 2))
+    ext[..., 0] = rng.uniform(-8.0, 8.0, (96, 96))
+    ext[40:, :, 1] = 6.0
+    qm = 2.0 ** -24
+    if precision == "fp32":
+        b = b.astype(np.float32).astype(np.float64)
+        h = np.rint(h / qm) * qm
+    step = _solver(precision).step(MobileV2State(b, h, np.zeros(h.shape + (2,))), 0.2,
+                                   external_acceleration_m_s2=ext)
+    h1, qx1, qy1, rec = oracle.v2_step(b, h, qx, qy, 0.2, external=ext)
+    assert rec["status"] == 0 and rec["substeps"] > 3
+    if precision == "fp64":
+        assert step.substeps == rec["substeps"]
+        np.testing.assert_array_equal(step.state.h_m, h1)
+        np.testing.assert_array_equal(step.state.q_m2_s[..., 0], qx1)
+        np.testing.assert_array_equal(step.state.q_m2_s[..., 1], qy1)
+        assert step.external_work_j_per_density == pytest.approx(rec["external_work"], rel=1e-9)
+    else:
+        assert abs(step.substeps - rec["substeps"]) <= 1
+        for got, ref in ((step.state.h_m, h1), (step.state.q_m2_s[..., 0], qx1), (step.state.q_m2_s[..., 1], qy1)):
+            assert np.max(np.abs(got - ref)) <= 1e-3 and np.sqrt(np.mean((got - ref) ** 2)) <= 1e-5
+    # the device record counts wet cells (the Warp kernel's convention); the
+    # reference's ledger adds h a dt of cells at or below the dry tolerance
+    np.testing.assert_allclose(step.external_impulse_m4_s, rec["external_impulse"], rtol=1e-5, atol=1e-12)
+    # with no external acceleration the same state moves differently
+    plain = _solver(precision).step(MobileV2State(b, h, np.zeros(h.shape + (2,))), 0.2)
+    assert not np.array_equal(plain.state.q_m2_s, step.state.q_m2_s)

@@ -600,3 +600,30 @@ def test_failure_activation_hand_case():
     assert vol32 == pytest.approx(vol, rel=1e-6)
     # planted mistake: 8-neighbourhood forcing would add the diagonals
     assert (centre - nx - 1) not in forcing.tolist()
+
+
+def test_per_cell_external_acceleration_closed_form():
+    """A uniform 0.1 m layer at rest on a flat bed, pushed for one substep by
+    a per-cell acceleration (20 m/s^2 along x in the left half, none
+    elsewhere): the faces carry nothing, so each pushed cell ends with
+    q = h (a dt)(1 - mu g / a) and every other cell stays at rest; the static
+    exit is skipped because max |a| > 1e-14 (mobile_v2_reference.py:226-229)."""
+    ny, nx, dt, mu = 6, 10, 1.0e-3, 0.55
+    b = np.zeros((ny, nx))
+    h = np.full((ny, nx), 0.1)
+    z = np.zeros((ny, nx))
+    ext = np.zeros((ny, nx, 2))
+    ext[:, : nx // 2, 0] = 20.0
+    h1, qx1, qy1, rec = oracle.v2_step(b, h, z, z, dt, mu=mu, external=ext)
+    assert rec["substeps"] == 1 and not rec["static_exit"]
+    # interior cells: both faces equal, so the flux terms cancel exactly (the
+    # wall cells have no boundary face and take a one-sided pressure push)
+    expect = 0.1 * (20.0 * dt) * (1.0 - mu * 9.81 * dt / (20.0 * dt))
+    np.testing.assert_allclose(qx1[1:-1, 1: nx // 2], expect, rtol=1e-12)
+    assert np.all(qx1[1:-1, nx // 2:-1] == 0.0) and np.all(qy1[1:-1, 1:-1] == 0.0)
+    np.testing.assert_array_equal(h1, h)
+    # the impulse of the push over the pushed cells, per the reference's ledger
+    assert rec["external_impulse"][0] == pytest.approx(ny * (nx // 2) * 0.1 * 20.0 * dt * 0.05 * 0.05, rel=1e-12)
+    # planted mistake: the uniform field of the first cell's value pushes every cell
+    _, qx_u, _, _ = oracle.v2_step(b, h, z, z, dt, mu=mu, external=(20.0, 0.0))
+    assert np.all(qx_u[1:-1, 1:-1] > 0.0)
