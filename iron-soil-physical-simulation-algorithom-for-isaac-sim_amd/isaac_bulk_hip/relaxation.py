@@ -148,7 +148,14 @@ def relax_critical_slope(height: np.ndarray, grid_spacing: tuple[float, float],
 
 
 class HipMinimumSlopeAdapter:
-    """MinimumSlopeAdapter on canonical ``H[y, x]`` backed by the HIP relaxation."""
+    """MinimumSlopeAdapter on canonical ``H[y, x]`` backed by the HIP relaxation.
+
+    Closed boundaries: one device relaxation to convergence.  Open boundaries
+    (minimum_slope_adapter.py:160-181, :185-212): the field padded with a
+    one-cell ring at ``boundary_height_m`` stays on the device; every
+    iteration the ring is reset to that height and one relaxation iteration
+    runs on the padded field, until an iteration converges; material that
+    left the interior is the boundary outflow."""
 
     backend_name = "GPU_HIP_FULL_DOMAIN_MINIMUM_SLOPE"
 
@@ -156,8 +163,8 @@ class HipMinimumSlopeAdapter:
                  precision: str = "fp64") -> None:
         if grid.valid_mask is not None and not np.all(grid.valid_mask):
             raise ValueError("[MinimumSlopeAdapter] legacy solver does not support holes in valid_mask")
-        if config.boundary_condition != "closed":
-            raise ValueError("[MinimumSlopeAdapter] the device adapter implements the closed boundary")
+        if config.boundary_condition not in ("closed", "open"):
+            raise ValueError("[Config:Solver] boundary_condition must be 'closed' or 'open'")
         self.grid = grid
         self.config = config
         self.device = device
@@ -177,25 +184,55 @@ class HipMinimumSlopeAdapter:
         if np.any(initial < 0.0):
             raise ValueError("soil heights must be non-negative")
         tdtype = torch.float64 if self.precision == "fp64" else torch.float32
-        yx = torch.from_numpy(initial).to(device=self.device, dtype=tdtype).contiguous()
-        iterations, converged, last = relax_device(
-            yx, self.grid.dx, self.grid.dy, self.config.critical_angle_deg, max_iterations,
-            self.config.tolerance)
-        stable = yx.cpu().numpy().astype(np.float64)
+        cfg = self.config
+        if cfg.boundary_condition == "closed":
+            yx = torch.from_numpy(initial).to(device=self.device, dtype=tdtype).contiguous()
+            iterations, converged, last = relax_device(
+                yx, self.grid.dx, self.grid.dy, cfg.critical_angle_deg, max_iterations, cfg.tolerance)
+            stable = yx.cpu().numpy().astype(np.float64)
+        else:
+            # the padded field lives on the device; the ring is reset before
+            # every single-iteration call, as the reference pads every call
+            ring = float(cfg.boundary_height_m)
+            if ring < 0.0:
+                raise ValueError("soil heights must be non-negative")
+            padded = np.pad(initial, 1, mode="constant", constant_values=ring)
+            yx = torch.from_numpy(padded).to(device=self.device, dtype=tdtype).contiguous()
+            runtime = _runtime(str(yx.device))
+            iterations, converged, last = 0, False, 0.0
+            for iterations in range(1, max_iterations + 1):
+                yx[0, :] = ring
+                yx[-1, :] = ring
+                yx[:, 0] = ring
+                yx[:, -1] = ring
+                _, conv, last = relax_device(yx, self.grid.dx, self.grid.dy, cfg.critical_angle_deg, 1,
+                                             cfg.tolerance, runtime=runtime)
+                if conv:
+                    converged = True
+                    break
+            stable = yx[1:-1, 1:-1].cpu().numpy().astype(np.float64)
         before = self.grid.compute_volume(initial)
         after = self.grid.compute_volume(stable)
-        balance = before - after
-        if abs(balance) > self.config.conservation_tolerance_m3:
+        if cfg.boundary_condition == "open":
+            if after > before + cfg.conservation_tolerance_m3:
+                raise RuntimeError(
+                    "[MinimumSlopeAdapter] open boundary created terrain volume; "
+                    f"before={before}, after={after}, shape={initial.shape}")
+            outflow = max(before - after, 0.0)
+        else:
+            outflow = 0.0
+        balance = before - after - outflow
+        if cfg.boundary_condition == "closed" and abs(balance) > cfg.conservation_tolerance_m3:
             raise RuntimeError(
                 "[MinimumSlopeAdapter] closed-boundary volume conservation failed; "
                 f"shape={initial.shape}, before={before}, after={after}, error={balance}, "
-                f"tolerance={self.config.conservation_tolerance_m3}")
+                f"tolerance={cfg.conservation_tolerance_m3}")
         result = RelaxationResult(
             stable, (initial.astype(np.float32), stable.astype(np.float32)), iterations, before, after,
-            0.0, converged,
+            outflow, converged,
             {"adapter": "HipMinimumSlopeAdapter", "legacy_entrypoint": "slope_model.relax_critical_slope",
              "canonical_axis_order": "yx", "legacy_axis_order": "xy",
-             "boundary_condition": "closed", "critical_angle_deg": self.config.critical_angle_deg,
+             "boundary_condition": cfg.boundary_condition, "critical_angle_deg": cfg.critical_angle_deg,
              "maximum_iterations": max_iterations, "volume_balance_error_m3": balance,
              "max_excess_last_iteration_m": last, "precision": self.precision})
         self._last_result = result

@@ -67,3 +67,25 @@ def test_frozen_boundary_adapter(tmp_path):
     np.testing.assert_array_equal(q[..., 0], qx)
     np.testing.assert_array_equal(hh, h)
     assert np.max(np.abs(b - resting)) <= 4e-16
+
+
+def test_independently_written_fixture_loads_as_native_v2():
+    """tests/golden/device_checkpoint_13x17.npz was written by plain NumPy in
+    the reference layout (tests/golden/make_checkpoint_fixture.py): the loader
+    reads every field unchanged, the record's terrain hash matches, and the
+    field sums equal the writer's independent summary."""
+    import json
+    from pathlib import Path
+
+    golden = Path(__file__).parent / "golden"
+    summary = json.loads((golden / "device_checkpoint_13x17.json").read_text())
+    with np.load(golden / "device_checkpoint_13x17.npz", allow_pickle=False) as archive:
+        record = json.loads(str(archive["checkpoint_record_json"]))
+        fields, migration = ck.checkpoint_fields(archive)
+    assert migration == "NONE_V2_NATIVE"
+    assert record["schema"] == "390F_PRODUCTION_DEVICE_CHECKPOINT/v2" and record["shape_yx"] == [13, 17]
+    assert ck.sha256_array(fields["b_eff"]) == record["terrain_hash_sha256"]
+    for name in ck.FLOAT_FIELDS:
+        assert fields[name].dtype == np.float64 and float(np.sum(fields[name])) == summary[name]
+    for name in ck.INT_FIELDS:
+        assert fields[name].dtype == np.int32 and int(np.sum(fields[name])) == summary[name]

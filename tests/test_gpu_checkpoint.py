@@ -127,3 +127,32 @@ def test_restore_checks(tmp_path):
     np.savez(tmp_path / "res.npz", **arrays)
     with pytest.raises(RuntimeError, match="DEVICE_CHECKPOINT_RESOLUTION_MISMATCH"):
         restore_device_checkpoint(_state("fp64"), tmp_path / "res.npz")
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_restore_independently_written_fixture(precision):
+    """A checkpoint written by plain NumPy in the reference layout
+    (tests/golden/make_checkpoint_fixture.py) restores onto the device: FP64
+    fields bit for bit; FP32 fields as their float32 rounding, mobile as whole
+    quanta (the fixture's depths are whole quanta, so exactly); record state
+    (timestamp, persistence, dirty tiles) carried over."""
+    from pathlib import Path
+
+    from isaac_bulk_hip import DeviceBulkState, TerrainGrid, restore_device_checkpoint
+    from isaac_bulk_hip.checkpoint import FLOAT_FIELDS, INT_FIELDS
+
+    path = Path(__file__).parent / "golden" / "device_checkpoint_13x17.npz"
+    grid = TerrainGrid(17, 13, 0.05, 0.04)
+    state = DeviceBulkState(grid, np.ones((13, 17)), tile_size=8, precision=precision)
+    bridge = _bridge(state)
+    rec = restore_device_checkpoint(state, path, large_avalanche=bridge)
+    assert rec["legacy_checkpoint_migration"] == "NONE_V2_NATIVE"
+    assert state.timestamp_device_s == 3.5 and bridge._persistence_s == 0.25 and bridge._has_previous
+    got = state.download_fields((*FLOAT_FIELDS, *INT_FIELDS))
+    with np.load(path, allow_pickle=False) as archive:
+        for name in (*FLOAT_FIELDS, *INT_FIELDS):
+            want = np.asarray(archive[name])
+            if precision == "fp32" and name in ("z_base", "b_eff", "momentum_x", "momentum_y"):
+                want = want.astype(np.float32).astype(np.float64)
+            np.testing.assert_array_equal(got[name], want, err_msg=name)
+    assert state.consume_dirty_tile_ids().tolist() == [1, 2]

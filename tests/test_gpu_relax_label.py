@@ -169,3 +169,34 @@ def test_frontier_phase_matches_oracle():
     hd, rd = op.download_state()
     np.testing.assert_array_equal(hd, ho)
     np.testing.assert_array_equal(rd, ro.astype(bool))
+
+
+@pytest.mark.parametrize("ring", [0.0, 0.4])
+def test_open_boundary_adapter_matches_oracle(ring):
+    """Open boundary (minimum_slope_adapter.py:160-181, :185-212): per
+    iteration the field is padded with a ring at the boundary height, one
+    relaxation iteration runs, the interior is kept; the device loop equals
+    the oracle's repetition of that bit for bit, and the outflow is the
+    interior volume lost."""
+    from isaac_bulk_hip import HipMinimumSlopeAdapter, SolverConfig, TerrainGrid
+
+    rng = np.random.default_rng(11)
+    ny, nx, dx = 30, 41, 0.05
+    h = 0.5 + rng.uniform(0.0, 0.3, (ny, nx))
+    h[:6, :8] += 1.2                                  # a steep block against the corner
+    grid = TerrainGrid(nx, ny, dx, dx)
+    cfg = SolverConfig(critical_angle_deg=33.0, max_iterations=4000, tolerance=1e-8, boundary_condition="open",
+                       boundary_height_m=ring)
+    res = HipMinimumSlopeAdapter(grid, cfg).solve(h)
+    crit = float(np.tan(np.deg2rad(33.0)))
+    cur, it, conv = h.copy(), 0, False
+    while it < 4000 and not conv:
+        it += 1
+        padded = np.pad(cur, 1, mode="constant", constant_values=ring)
+        out, _, conv, _ = oracle.relax_yx(padded, dx, dx, crit, 1, 1e-8)
+        cur = out[1:-1, 1:-1]
+    assert res.converged and conv and res.iteration_count == it
+    np.testing.assert_array_equal(res.heightmap_stable, cur)
+    lost = (h.sum() - cur.sum()) * dx * dx
+    assert res.boundary_outflow_m3 == pytest.approx(max(lost, 0.0), rel=1e-9)
+    assert res.boundary_outflow_m3 > 0.0
