@@ -59,6 +59,8 @@ def parse() -> argparse.Namespace:
     p.add_argument("--no-cpu-baseline", action="store_true")
     p.add_argument("--parity-frames", type=int, default=30,
                    help="frames of the FP32-vs-FP64-oracle parity check (cpu_baseline leg)")
+    p.add_argument("--skip-configs", action="store_true",
+                   help="omit the batched (256 terrains, 2 s) and digging (64 x 256^2, 3 s) legs of the default line")
     p.add_argument("--skip-relax-to-rest", action="store_true",
                    help="omit the relax-to-rest measurement (short profiling runs)")
     p.add_argument("--workload", choices=("frame", "batched", "digging", "deposition", "frontier"),
@@ -136,6 +138,8 @@ def learned_relax_to_rest(state, plain_rest) -> dict:
     return {"ms": res.elapsed_ms, "ms_runs": [r.elapsed_ms for r in runs],
             "surrogate_roofline": roof,
             "exact_iterations": res.iterations_exact + res.fallback_iterations,
+            "exact_sweeps": res.iterations_exact, "fallback_iterations": res.fallback_iterations,
+            "used_fallback": bool(res.used_fallback),
             "envelope_launches": res.envelope_launches, "converged": bool(res.converged),
             "judge_max_excess": check_last, "judge_converged": bool(check_conv),
             "mass_rel_change": abs(float(b.double().sum().item()) - mass0) / mass0,
@@ -283,7 +287,7 @@ def frame_parity(seed: int, frames: int) -> dict:
             "passed": bool(passed)}
 
 
-def run_batched(args) -> int:
+def batched_result(count: int, steps: int, warmup: int, seed: int) -> dict:
     """Uneven batch (sizes uniform over {64..512}^2, own pile and mobile patch each),
     1/60 s frames, FP32; value = sum of cells x substeps over the wall time."""
     import torch
@@ -291,48 +295,74 @@ def run_batched(args) -> int:
     from isaac_bulk_hip.batched import TILE_SHAPES, BatchedMobileV2
     from isaac_bulk_hip.workloads import batched_piles
 
-    terrains = [(b.astype(np.float32), h) for b, h in batched_piles(args.batch_count, DX, seed=args.seed)]
+    terrains = [(b.astype(np.float32), h) for b, h in batched_piles(count, DX, seed=seed)]
     batch = BatchedMobileV2(terrains, DX, config=MobileV2Config(cfl=0.18))
     tune = batch.autotune(FRAME_DT, frames=2)
-    for _ in range(args.warmup):
+    for _ in range(warmup):
         batch.step(FRAME_DT)
     cells = (batch.ny.astype(np.int64) * batch.nx.astype(np.int64))
     cell_sub = 0
     sub_ms = 0.0
     launches = 0
+    exact_mass = True
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    for _ in range(args.steps):
+    for _ in range(steps):
         recs = batch.step(FRAME_DT)
         cell_sub += int(np.dot(cells, recs.substeps.astype(np.int64)))
         sub_ms += batch.last_substep_ms
         launches += batch.last_launches
+        exact_mass &= bool(np.array_equal(recs.mass_quanta_before, recs.mass_quanta_after))
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     bpc = BYTES_PER_CELL_SUBSTEP["fp32"]
     achieved = bpc * cell_sub / (sub_ms * 1e-3) / 1e9 if sub_ms > 0 else 0.0
+    # a full launch (every tile of every terrain working): a 1 us step takes
+    # exactly one substep everywhere; the second such step enqueues that
+    # launch plus one that finds every loop ended, both inside the timing
+    full = []
+    for _ in range(4):
+        recs = batch.step(1.0e-6)
+        if np.all(recs.substeps == 1):
+            full.append((batch.last_substep_ms, batch.last_launches))
+    full_ms = min(ms for ms, _ in full[1:]) if len(full) > 1 else None
+    full_bytes = bpc * int(cells.sum())
     out = {
-        "metric": "batched Mobile V2 cell-substeps/s (uneven terrains)",
-        "value": cell_sub / wall, "unit": "cell-substeps/s", "n_gpus": 1, "steps": args.steps,
-        "warmup": args.warmup, "ms_per_step": wall * 1e3 / max(args.steps, 1), "higher_is_better": True,
-        "scaling": "weak", "vs_baseline": None, "dtype": "f32",
-        "data": "synthetic (seeded Gaussian piles and mobile patches)",
-        "config": {"workload": f"{args.batch_count} terrains, sizes uniform over {{64..512}}^2, 1/60 s frames",
+        "value": cell_sub / wall, "unit": "cell-substeps/s", "steps": steps, "warmup": warmup,
+        "ms_per_step": wall * 1e3 / max(steps, 1), "simulated_s": steps * FRAME_DT,
+        "config": {"workload": f"{count} terrains, sizes uniform over {{64..512}}^2, 1/60 s frames",
                    "cells": int(cells.sum()), "frame_dt_s": FRAME_DT, "tile": TILE_SHAPES[tune["best"]]},
-        "autotune_ms": tune["ms"],
+        "autotune_ms": tune["ms"], "mass_quanta_exact_every_instance_and_frame": exact_mass,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": MEASURED_TRAFFIC_BATCHED_FULL,
                      "traffic_note": "HBM bytes of one full launch (every instance active)",
                      "kernel": "batch_substep_kernel",
                      "bytes_per_cell_substep": bpc, "substep_launches": launches,
-                     "substep_launch_ms_total": sub_ms},
+                     "substep_launch_ms_total": sub_ms,
+                     "note": "achieved = algorithmic bytes of the instances' substeps / all substep launch time "
+                             "(late launches carry few active terrains)"},
+        "full_launch_roofline": None if full_ms is None else {
+            "bound": "hbm", "achieved": full_bytes / (full_ms * 1e-3) / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": full_bytes / (full_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, "traffic": MEASURED_TRAFFIC_BATCHED_FULL,
+            "kernel": "batch_substep_kernel", "algorithmic_bytes": full_bytes, "ms": full_ms,
+            "note": "one substep of every terrain; the timing also holds the following launch that finds every "
+                    "loop ended (a few microseconds), so the fraction is a lower bound"},
     }
-    print(json.dumps(out), flush=True)
     batch.rt.close()
+    return out
+
+
+def run_batched(args) -> int:
+    res = batched_result(args.batch_count, args.steps, args.warmup, args.seed)
+    out = {"metric": "batched Mobile V2 cell-substeps/s (uneven terrains)", "value": res.pop("value"),
+           "unit": res.pop("unit"), "n_gpus": 1, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+           "dtype": "f32", "data": "synthetic (seeded Gaussian piles and mobile patches)", **res}
+    print(json.dumps(out), flush=True)
     return 0
 
 
-def run_digging(args) -> int:
+def digging_result(count: int, steps: int, warmup: int, seed: int, relax_mode: str,
+                   relax_max_iterations: int) -> dict:
     """Digging workload: 64 terrains of 256^2 (Gaussian piles on a 0.2 m floor,
     relaxed to rest at 34 degrees before timing), each with a 1.5 x 0.8 m
     bucket swept 0.02 m per frame along a seeded line;
@@ -346,9 +376,9 @@ def run_digging(args) -> int:
     from isaac_bulk_hip.workloads import bucket_cells, digging_sites
 
     n = 256
-    terrains, lines = digging_sites(args.dig_count, n, DX, seed=args.seed)
+    terrains, lines = digging_sites(count, n, DX, seed=seed)
     batch = BatchedMobileV2([(b.astype(np.float32), h) for b, h in terrains], DX, config=MobileV2Config(cfl=0.18))
-    frames = args.warmup + args.steps
+    frames = warmup + steps
     # bucket index lists of every frame, uploaded before timing
     plan = []
     for f in range(frames):
@@ -359,7 +389,7 @@ def run_digging(args) -> int:
     # the piles start at rest (relaxed to 34 degrees), then the bucket digs
     rest = batch.relax(34.0, 1e-6, max_iterations=20000)
     net = None
-    if args.dig_relax == "learned":
+    if relax_mode == "learned":
         from isaac_bulk_hip.warm_start_training import load_net
 
         net, _ = load_net(batch.rt)
@@ -375,8 +405,8 @@ def run_digging(args) -> int:
     v2_s = 0.0
     t_parts = {"entrain": 0.0, "v2": 0.0, "relax": 0.0, "label": 0.0}
     for f in range(frames):
-        timed = f >= args.warmup
-        if f == args.warmup:
+        timed = f >= warmup
+        if f == warmup:
             torch.cuda.synchronize()
             volume0 = total_volume()
             t0 = time.perf_counter()
@@ -389,12 +419,12 @@ def run_digging(args) -> int:
         recs = batch.step(FRAME_DT)
         tc = time.perf_counter()
         if net is not None:
-            ws = batch.warm_start_relax(net, 34.0, 1e-6, max_iterations=args.relax_max_iterations)
+            ws = batch.warm_start_relax(net, 34.0, 1e-6, max_iterations=relax_max_iterations)
             rr = ws["relax"]
             if timed:
                 env_launches += ws["envelope_launches"]
         else:
-            rr = batch.relax(34.0, 1e-6, max_iterations=args.relax_max_iterations)
+            rr = batch.relax(34.0, 1e-6, max_iterations=relax_max_iterations)
         td = time.perf_counter()
         _, stats = batch.label(start_angle_deg=38.0, stop_angle_deg=30.0, density=1370.0, cohesion=0.0,
                                mobilization_depth=0.05, dry_tolerance=1e-8, connectivity=8)
@@ -413,21 +443,21 @@ def run_digging(args) -> int:
     volume1 = total_volume()
     out = {
         "metric": "digging frames/s (entrain + Mobile V2 + relaxation + labelling, 64 terrains)",
-        "value": args.steps / wall, "unit": "frames/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
-        "ms_per_step": wall * 1e3 / max(args.steps, 1), "higher_is_better": True, "scaling": "weak",
+        "value": steps / wall, "unit": "frames/s", "n_gpus": 1, "steps": steps, "warmup": warmup,
+        "ms_per_step": wall * 1e3 / max(steps, 1), "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "f32", "data": "synthetic (seeded Gaussian piles, seeded bucket lines)",
-        "config": {"workload": f"{args.dig_count} terrains of 256^2, bucket 1.5 x 0.8 m at 0.02 m/frame, "
+        "config": {"workload": f"{count} terrains of 256^2, bucket 1.5 x 0.8 m at 0.02 m/frame, "
                                "0.01 m entrained per frame, relaxation 34 deg to 1e-6 m, 1/60 s frames",
                    "cells": cells_total, "frame_dt_s": FRAME_DT},
         "v2_cell_substeps_per_s": cell_sub / v2_s if v2_s > 0 else None,
-        "ms_per_frame_parts": {k: v * 1e3 / max(args.steps, 1) for k, v in t_parts.items()},
-        "relax_iterations_per_terrain_frame": relax_it / max(args.steps * args.dig_count, 1),
+        "ms_per_frame_parts": {k: v * 1e3 / max(steps, 1) for k, v in t_parts.items()},
+        "relax_iterations_per_terrain_frame": relax_it / max(steps * count, 1),
         "relax_iterations_max": relax_max,
         "relax_iterations_frame_max": {"mean": float(np.mean(frame_max)) if frame_max else None,
                                        "p50": float(np.percentile(frame_max, 50)) if frame_max else None,
                                        "p90": float(np.percentile(frame_max, 90)) if frame_max else None},
-        "relaxation": args.dig_relax,
-        "envelope_launches_per_frame": env_launches / max(args.steps, 1),
+        "relaxation": relax_mode,
+        "envelope_launches_per_frame": env_launches / max(steps, 1),
         "setup_relax_to_rest_iterations_max": int(rest["iterations"].max()),
         "entrained_m3": dug,
         "volume_rel_change": (volume1 - volume0) / volume0,
@@ -435,11 +465,17 @@ def run_digging(args) -> int:
     }
     if net is not None:
         # the network on the final fields of all terrains, after the timed frames
-        b3 = batch.b.view(args.dig_count, n, n)
-        h3 = (batch.h.to(torch.float32) * 2.0 ** -24).view(args.dig_count, n, n)
+        b3 = batch.b.view(count, n, n)
+        h3 = (batch.h.to(torch.float32) * 2.0 ** -24).view(count, n, n)
         out["surrogate_roofline"] = surrogate_roofline(net, b3, h3)
-    print(json.dumps(out), flush=True)
     batch.rt.close()
+    return out
+
+
+def run_digging(args) -> int:
+    out = digging_result(args.dig_count, args.steps, args.warmup, args.seed, args.dig_relax,
+                         args.relax_max_iterations)
+    print(json.dumps(out), flush=True)
     return 0
 
 
@@ -761,6 +797,21 @@ def main() -> int:
                      "cells_per_launch": cells, "avg_launch_ms": avg_substep_ms,
                      "launches_timed": timed_sub},
     }
+    if args.precision == "fp32" and not args.skip_configs:
+        out["c3_batched"] = batched_result(256, 120, 3, args.seed)          # 2 s simulated
+        dig = {mode: digging_result(64, 180, 3, args.seed, mode, args.relax_max_iterations)
+               for mode in ("learned", "plain")}                            # 3 s simulated each
+        out["c5_digging"] = {
+            "workload": dig["learned"]["config"]["workload"],
+            "frames": 180, "unit": "frames/s",
+            "learned_frames_per_s": dig["learned"]["value"], "plain_frames_per_s": dig["plain"]["value"],
+            "learned_over_plain": dig["learned"]["value"] / dig["plain"]["value"],
+            "volume_rel_change": {m: dig[m]["volume_rel_change"] for m in dig},
+            "relax_iterations_per_terrain_frame": {m: dig[m]["relax_iterations_per_terrain_frame"] for m in dig},
+            "v2_cell_substeps_per_s": dig["learned"]["v2_cell_substeps_per_s"],
+            "ms_per_frame_parts": {m: dig[m]["ms_per_frame_parts"] for m in dig},
+            "surrogate_roofline": dig["learned"].get("surrogate_roofline"),
+        }
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(b0, h0, args.cpu_sample_frames)
         if args.precision == "fp32" and args.parity_frames > 0:

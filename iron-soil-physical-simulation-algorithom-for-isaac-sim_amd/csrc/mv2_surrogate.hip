@@ -452,13 +452,15 @@ __global__ void __launch_bounds__(kBlock) mse_kernel(const float *pred, const fl
 
 // Adam step on FP32 master parameters; `zero_sum_stride` > 0 re-projects the
 // b_eff-channel taps of layer 1 onto zero sum (per output channel).
+// Adam (Kingma & Ba 2015, Algorithm 1); omb1 / omb2 = 1 - beta1 / 1 - beta2
+// formed in double and rounded once
 __global__ void adam_kernel(float *param, const float *grad, float *m, float *v, int n, float lr, float beta1,
-                            float beta2, float eps, float bc1, float bc2)
+                            float beta2, float omb1, float omb2, float eps, float bc1, float bc2)
 {
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
         const float g = grad[i];
-        const float mi = beta1 * m[i] + (1.0f - beta1) * g;
-        const float vi = beta2 * v[i] + (1.0f - beta2) * g * g;
+        const float mi = beta1 * m[i] + omb1 * g;
+        const float vi = beta2 * v[i] + omb2 * g * g;
         m[i] = mi;
         v[i] = vi;
         param[i] -= lr * (mi / bc1) / (__builtin_sqrtf(vi / bc2) + eps);
@@ -1176,7 +1178,7 @@ extern "C" int mv2_nn_adam(mv2_context *ctx, float *param, const float *grad, fl
     if (int rc = check_plain(ctx)) return rc;
     const float bc1 = (float)(1.0 - pow(beta1, step)), bc2 = (float)(1.0 - pow(beta2, step));
     hipLaunchKernelGGL(adam_kernel, dim3(grid_for(n)), dim3(kBlock), 0, ctx->stream, param, grad, m, v, n, (float)lr,
-                       (float)beta1, (float)beta2, (float)eps, bc1, bc2);
+                       (float)beta1, (float)beta2, (float)(1.0 - beta1), (float)(1.0 - beta2), (float)eps, bc1, bc2);
     MV2_HIP(ctx, hipGetLastError());
     return MV2_OK;
 }

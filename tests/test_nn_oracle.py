@@ -1,8 +1,10 @@
-"""Pin the NumPy network oracle: finite differences of its float64 backward pass."""
+"""Pin the NumPy network oracle: finite differences of its float64 backward
+pass and a library convolution for its forward layers."""
 
 from __future__ import annotations
 
 import numpy as np
+import pytest
 
 import nn_oracle
 
@@ -215,3 +217,41 @@ def test_settle_rounds_remove_the_rim_step_and_keep_mass():
     mass = b.astype(np.float64).sum()
     for r in (1, 2, 3):
         assert abs(nn_oracle.commit_settled(b, z, crit, 1e-5, r).astype(np.float64).sum() - mass) <= 1e-7 * mass
+
+
+@pytest.mark.parametrize("dil", [1, 2, 4, 8])
+def test_conv_matches_library_convolution(dil):
+    """nn_oracle.conv with zero padding is the cross-correlation of a library
+    convolution: CPU torch.nn.functional.conv2d (float64, dilation d,
+    padding d) with the taps in row-major order of (dy, dx) in -1..1."""
+    import torch
+
+    rng = np.random.default_rng(dil)
+    x = rng.standard_normal((2, 11, 13, 5))
+    w = rng.standard_normal((9, 3, 5))
+    bias = rng.standard_normal(3)
+    got = nn_oracle.conv(x, w, bias, dil)
+    wt = torch.from_numpy(w.reshape(3, 3, 3, 5).transpose(2, 3, 0, 1).copy())     # [Cout, Cin, ky, kx]
+    ref = torch.nn.functional.conv2d(torch.from_numpy(x.transpose(0, 3, 1, 2).copy()), wt,
+                                     torch.from_numpy(bias), padding=dil, dilation=dil)
+    np.testing.assert_allclose(got, ref.numpy().transpose(0, 2, 3, 1), rtol=1e-12, atol=1e-12)
+
+
+def test_conv_centre_fill_channel_by_brute_force():
+    """Layer 1's b_eff channel takes the centre value for taps outside the
+    terrain (a constant height offset then cancels through the zero-sum taps):
+    checked against a direct loop on a small input."""
+    rng = np.random.default_rng(7)
+    B, H, W, dil = 1, 5, 6, 2
+    x = rng.standard_normal((B, H, W, 3))
+    w = rng.standard_normal((9, 4, 3))
+    got = nn_oracle.conv(x, w, None, dil, replicate_channel0=True)
+    ref = np.zeros((B, H, W, 4))
+    for y in range(H):
+        for xx in range(W):
+            for t, (ty, tx) in enumerate(nn_oracle.TAPS):
+                yy, xs = y + ty * dil, xx + tx * dil
+                inside = 0 <= yy < H and 0 <= xs < W
+                v = x[0, yy, xs] if inside else np.array([x[0, y, xx, 0], 0.0, 0.0])
+                ref[0, y, xx] += w[t] @ v
+    np.testing.assert_allclose(got, ref, rtol=1e-12, atol=1e-12)
