@@ -920,7 +920,7 @@ extern "C" int mv2_batch_step(mv2_context *ctx, const mv2_step_config *cfg, int 
         MV2_HIP(ctx, hipGetLastError());
         MV2_HIP(ctx, hipEventRecord(ctx->ev_stop, st));
         MV2_HIP(ctx, hipMemcpyAsync(ctx->host_result.ptr, a.rec, rec_bytes, hipMemcpyDeviceToHost, st));
-        MV2_HIP(ctx, hipStreamSynchronize(st));
+        if (int rc_w = stream_wait(ctx, st)) return rc_w;
         float sms = 0.0f;
         MV2_HIP(ctx, hipEventElapsedTime(&sms, bs->ev_sub0, bs->ev_sub1));
         sub_ms += sms;
@@ -1019,7 +1019,7 @@ extern "C" int mv2_batch_relax(mv2_context *ctx, float *field, const double crit
         hipLaunchKernelGGL(batch_relax_end_kernel, dim3(grid), dim3(kBlock), 0, st, a, k);
         MV2_HIP(ctx, hipGetLastError());
         MV2_HIP(ctx, hipMemcpyAsync(ctx->host_result.ptr, a.out, out_bytes, hipMemcpyDeviceToHost, st));
-        MV2_HIP(ctx, hipStreamSynchronize(st));
+        if (int rc_w = stream_wait(ctx, st)) return rc_w;
         bool all = true;
         int most = 0;
         for (int i = 0; i < bs->count; i++) {

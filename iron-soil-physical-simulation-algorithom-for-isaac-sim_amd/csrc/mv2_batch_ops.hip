@@ -388,7 +388,7 @@ extern "C" int mv2_batch_entrain(mv2_context *ctx, const float *z_base, const in
     if (int rc = ensure_scratch(ctx, ctx->host_result, sizeof(unsigned long long) * cnt + 4096, true)) return rc;
     MV2_HIP(ctx, hipMemcpyAsync(ctx->host_result.ptr, moved, sizeof(unsigned long long) * cnt, hipMemcpyDeviceToHost,
                                 st));
-    MV2_HIP(ctx, hipStreamSynchronize(st));
+    if (int rc_w = stream_wait(ctx, st)) return rc_w;
     const unsigned long long *hq = (const unsigned long long *)ctx->host_result.ptr;
     const double area = bs->dx * bs->dy;
     for (size_t i = 0; i < cnt; i++) volume_out[i] = ((double)hq[i] * kQuantum) * area;
@@ -441,7 +441,7 @@ extern "C" int mv2_batch_label(mv2_context *ctx, const float *z_base, const mv2_
     MV2_HIP(ctx, hipGetLastError());
     if (int rc = ensure_scratch(ctx, ctx->host_result, sizeof(int32_t) * 4 * cnt + 4096, true)) return rc;
     MV2_HIP(ctx, hipMemcpyAsync(ctx->host_result.ptr, a.stats, sizeof(int32_t) * 4 * cnt, hipMemcpyDeviceToHost, st));
-    MV2_HIP(ctx, hipStreamSynchronize(st));
+    if (int rc_w = stream_wait(ctx, st)) return rc_w;
     const int32_t *hs = (const int32_t *)ctx->host_result.ptr;
     for (size_t i = 0; i < cnt; i++) {
         stats_out[4 * i + 0] = hs[4 * i + 0];

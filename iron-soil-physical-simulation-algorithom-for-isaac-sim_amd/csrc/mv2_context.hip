@@ -8,6 +8,7 @@
 // pkg/runtime/bulk_state_authority.py:543-621).
 #include "mv2_internal.hpp"
 
+#include <chrono>
 #include <cstdarg>
 #include <cstring>
 
@@ -54,6 +55,26 @@ int ensure_scratch(mv2_context *ctx, Scratch &s, size_t bytes, bool host_pinned)
     }
     s.bytes = bytes;
     return MV2_OK;
+}
+
+// Wait for the work queued on a stream by polling it: a step of the frame
+// path is tens of microseconds, less than the wake-up of a blocking
+// synchronize.  After kSpinLimitNs of polling the blocking call takes over;
+// errors of the stream come back either way.
+int stream_wait(mv2_context *ctx, hipStream_t st)
+{
+    constexpr long long kSpinLimitNs = 2000000;
+    const auto t0 = std::chrono::steady_clock::now();
+    for (;;) {
+        const hipError_t e = hipStreamQuery(st);
+        if (e == hipSuccess) return MV2_OK;
+        if (e != hipErrorNotReady) return check_hip(ctx, e, "hipStreamQuery");
+        if (std::chrono::duration_cast<std::chrono::nanoseconds>(std::chrono::steady_clock::now() - t0).count() >
+            kSpinLimitNs)
+            break;
+    }
+    const hipError_t e = hipStreamSynchronize(st);
+    return e == hipSuccess ? MV2_OK : check_hip(ctx, e, "hipStreamSynchronize");
 }
 
 static void release(Scratch &s, bool host_pinned = false)

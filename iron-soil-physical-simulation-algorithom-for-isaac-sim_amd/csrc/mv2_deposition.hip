@@ -270,7 +270,7 @@ extern "C" int mv2_deposition_step(mv2_context *ctx, const mv2_deposition_config
     MV2_HIP(ctx, hipGetLastError());
     MV2_HIP(ctx, hipMemcpyAsync(ctx->host_result.ptr, a.result, sizeof(double) * 3, hipMemcpyDeviceToHost,
                                 ctx->stream));
-    MV2_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    if (int rc_w = stream_wait(ctx, ctx->stream)) return rc_w;
     memcpy(out, ctx->host_result.ptr, sizeof(double) * 3);
     return MV2_OK;
 }

@@ -203,7 +203,7 @@ static int finish(mv2_context *ctx, int nv, int nparts, double *out)
     }
     MV2_HIP(ctx, hipGetLastError());
     MV2_HIP(ctx, hipMemcpyAsync(ctx->host_result.ptr, dres, sizeof(double) * nv, hipMemcpyDeviceToHost, ctx->stream));
-    MV2_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    if (int rc_w = stream_wait(ctx, ctx->stream)) return rc_w;
     memcpy(out, ctx->host_result.ptr, sizeof(double) * nv);
     return MV2_OK;
 }

@@ -266,7 +266,7 @@ extern "C" int mv2_frontier_rounds(mv2_context *ctx, const mv2_frontier_round_co
         enqueued += n;
         MV2_HIP(ctx, hipMemcpyAsync(ctx->host_result.ptr, a.ctl + (enqueued & 1), sizeof(FrontierControl),
                                     hipMemcpyDeviceToHost, ctx->stream));
-        MV2_HIP(ctx, hipStreamSynchronize(ctx->stream));
+        if (int rc_w = stream_wait(ctx, ctx->stream)) return rc_w;
         memcpy(&rec, ctx->host_result.ptr, sizeof(FrontierControl));
         if (rec.done || rec.active == 0 || enqueued >= a.budget) break;
         chunk = chunk * 2 < kFrontierMaxChunk ? chunk * 2 : kFrontierMaxChunk;

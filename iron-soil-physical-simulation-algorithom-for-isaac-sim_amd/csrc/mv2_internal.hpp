@@ -84,6 +84,8 @@ namespace mv2 {
 int set_error(mv2_context *ctx, int code, const char *fmt, ...);
 int check_hip(mv2_context *ctx, hipError_t err, const char *what);
 int ensure_scratch(mv2_context *ctx, Scratch &s, size_t bytes, bool host_pinned = false);
+// poll a stream until its work is done (blocking sync after 2 ms)
+int stream_wait(mv2_context *ctx, hipStream_t st);
 // device address of the allocation-free ledger's file-scope partials (mv2_exchange.hip)
 int noalloc_scratch_address(mv2_context *ctx, double **out);
 

@@ -553,7 +553,7 @@ extern "C" int mv2_label_mask(mv2_context *ctx, const int32_t *mask, int32_t *la
     const int64_t n = (int64_t)ny * nx;
     hipLaunchKernelGGL(init_from_mask, dim3(grid_for(n, kBlock, 4096)), dim3(kBlock), 0, ctx->stream, mask, labels, n);
     if (int rc = label_core(ctx, labels, ny, nx, connectivity, nullptr)) return rc;
-    MV2_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    if (int rc_w = stream_wait(ctx, ctx->stream)) return rc_w;
     return MV2_OK;
 }
 
@@ -586,7 +586,7 @@ extern "C" int mv2_avalanche_label(mv2_context *ctx, const mv2_avalanche_config 
     hipLaunchKernelGGL(winner_final_kernel, dim3(1), dim3(kBlock), 0, ctx->stream, partials, sgrid, aux);
     MV2_HIP(ctx, hipGetLastError());
     MV2_HIP(ctx, hipMemcpyAsync(ctx->host_result.ptr, aux, sizeof(LabelAux), hipMemcpyDeviceToHost, ctx->stream));
-    MV2_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    if (int rc_w = stream_wait(ctx, ctx->stream)) return rc_w;
     LabelAux host;
     memcpy(&host, ctx->host_result.ptr, sizeof(LabelAux));
     memcpy(out->diag_i, host.diag_i, sizeof(host.diag_i));
@@ -632,7 +632,7 @@ extern "C" int mv2_avalanche_mobilize(mv2_context *ctx, const mv2_avalanche_conf
     hipLaunchKernelGGL(mobilize_final_kernel, dim3(1), dim3(kBlock), 0, ctx->stream, partials, sgrid, dres);
     MV2_HIP(ctx, hipGetLastError());
     MV2_HIP(ctx, hipMemcpyAsync(ctx->host_result.ptr, dres, sizeof(double) * kMobF, hipMemcpyDeviceToHost, ctx->stream));
-    MV2_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    if (int rc_w = stream_wait(ctx, ctx->stream)) return rc_w;
     double s[kMobF];
     memcpy(s, ctx->host_result.ptr, sizeof(s));
     for (int k = 0; k < 16; k++) diag_f[k] = 0.0;
@@ -677,7 +677,7 @@ extern "C" int mv2_avalanche_persistence(mv2_context *ctx, const mv2_avalanche_c
     MV2_HIP(ctx, hipGetLastError());
     MV2_HIP(ctx, hipMemcpyAsync(ctx->host_result.ptr, dres, sizeof(double) * (kPersV + 1), hipMemcpyDeviceToHost,
                                 ctx->stream));
-    MV2_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    if (int rc_w = stream_wait(ctx, ctx->stream)) return rc_w;
     double s[kPersV + 1];
     memcpy(s, ctx->host_result.ptr, sizeof(s));
     for (int k = 0; k < kPersF; k++) diag_f[k] = s[k];

@@ -336,7 +336,7 @@ extern "C" int mv2_failure_activate(mv2_context *ctx, int row0, int row1, int co
                        flags, material_mask, forcing_out, partials, grid, aux);
     MV2_HIP(ctx, hipGetLastError());
     MV2_HIP(ctx, hipMemcpyAsync(ctx->host_result.ptr, aux, sizeof(ActivationAux), hipMemcpyDeviceToHost, ctx->stream));
-    MV2_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    if (int rc_w = stream_wait(ctx, ctx->stream)) return rc_w;
     ActivationAux host;
     memcpy(&host, ctx->host_result.ptr, sizeof(host));
     out->activated_volume = host.volume;

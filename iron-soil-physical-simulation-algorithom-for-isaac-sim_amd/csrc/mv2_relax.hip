@@ -200,7 +200,7 @@ static int launch_relax(mv2_context *ctx, void *height, int ny, int nx, const do
         hipLaunchKernelGGL(relax_end_kernel<R>, dim3(gend), dim3(kBlock), 0, st, a, k);
         MV2_HIP(ctx, hipGetLastError());
         MV2_HIP(ctx, hipMemcpyAsync(ctx->host_result.ptr, a.out, sizeof(mv2_relax_result), hipMemcpyDeviceToHost, st));
-        MV2_HIP(ctx, hipStreamSynchronize(st));
+        if (int rc_w = stream_wait(ctx, st)) return rc_w;
         if (hres->complete || k >= max_iterations) break;
         chunk = k / 2 > 8 ? k / 2 : 8;
     }
@@ -301,7 +301,7 @@ extern "C" int mv2_frontier_relax(mv2_context *ctx, double *height, const double
                            critical_difference, moved_tile_flags, dcount);
     MV2_HIP(ctx, hipGetLastError());
     MV2_HIP(ctx, hipMemcpyAsync(ctx->host_result.ptr, dcount, sizeof(int32_t), hipMemcpyDeviceToHost, ctx->stream));
-    MV2_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    if (int rc_w = stream_wait(ctx, ctx->stream)) return rc_w;
     memcpy(moved_count_out, ctx->host_result.ptr, sizeof(int32_t));
     return MV2_OK;
 }

@@ -1145,7 +1145,7 @@ static int launch_step(mv2_context *ctx, const mv2_step_config *cfg, mv2_step_re
         hipLaunchKernelGGL(v2_fold_kernel<P>, dim3(gfold), dim3(kBlock), 0, st, a, k, gs, gu, fold);
         MV2_HIP(ctx, hipGetLastError());
         MV2_HIP(ctx, hipEventRecord(ctx->ev_stop, st));
-        MV2_HIP(ctx, hipStreamSynchronize(st));
+        if (int rc = stream_wait(ctx, st)) return rc;
         if (fold->ctl.done && (!P::kQuant || fold->ctl.end_done)) break;
         // more substeps than enqueued: the state so far is valid, enqueue more
         chunk = chunk * 2 > 4 ? chunk * 2 : 4;

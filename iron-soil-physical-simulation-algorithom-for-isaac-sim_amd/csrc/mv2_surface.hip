@@ -164,7 +164,7 @@ extern "C" int mv2_surface_dirty_tiles(mv2_context *ctx, const double *surface_b
     surface_compact_kernel<<<1, kBlock, 0, ctx->stream>>>(tile_flags, tile_count, dout);
     MV2_HIP(ctx, hipGetLastError());
     MV2_HIP(ctx, hipMemcpyAsync(ctx->host_result.ptr, dout, bytes, hipMemcpyDeviceToHost, ctx->stream));
-    MV2_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    if (int rc_w = stream_wait(ctx, ctx->stream)) return rc_w;
     const int32_t *h = (const int32_t *)ctx->host_result.ptr;
     *count_out = h[0];
     memcpy(tile_ids_out, h + 1, sizeof(int32_t) * (size_t)h[0]);

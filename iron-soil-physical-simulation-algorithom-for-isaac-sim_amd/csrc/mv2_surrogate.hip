@@ -1139,7 +1139,7 @@ extern "C" int mv2_nn_mse(mv2_context *ctx, const float *pred, const float *targ
     if (loss_out) {
         std::vector<double> host(grid);
         MV2_HIP(ctx, hipMemcpyAsync(host.data(), parts, sizeof(double) * grid, hipMemcpyDeviceToHost, ctx->stream));
-        MV2_HIP(ctx, hipStreamSynchronize(ctx->stream));
+        if (int rc_w = stream_wait(ctx, ctx->stream)) return rc_w;
         double s = 0.0;
         for (double v : host) s += v;
         *loss_out = s / (double)n;
@@ -1164,7 +1164,7 @@ extern "C" int mv2_nn_excess_loss(mv2_context *ctx, const float *pred, const flo
     if (loss_out) {
         std::vector<double> host(grid);
         MV2_HIP(ctx, hipMemcpyAsync(host.data(), parts, sizeof(double) * grid, hipMemcpyDeviceToHost, ctx->stream));
-        MV2_HIP(ctx, hipStreamSynchronize(ctx->stream));
+        if (int rc_w = stream_wait(ctx, ctx->stream)) return rc_w;
         double acc = 0.0;
         for (double v : host) acc += v;
         *loss_out = weight * acc / (double)n;
@@ -1366,7 +1366,7 @@ static int envelope_run(mv2_context *ctx, const float *b, const float *delta, fl
         hipLaunchKernelGGL(envelope_result_kernel, dim3(1), dim3(kWave), 0, st, ctl, spread, k, dres);
         MV2_HIP(ctx, hipGetLastError());
         MV2_HIP(ctx, hipMemcpyAsync(ctx->host_result.ptr, dres, 2 * sizeof(int), hipMemcpyDeviceToHost, st));
-        MV2_HIP(ctx, hipStreamSynchronize(st));
+        if (int rc_w = stream_wait(ctx, st)) return rc_w;
         if (hres[1] || k >= max_launches) break;
         chunk = 2 * chunk;
     }

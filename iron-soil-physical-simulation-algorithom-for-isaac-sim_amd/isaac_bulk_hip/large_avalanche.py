@@ -113,12 +113,24 @@ class DeviceLargeAvalancheBridge:
         self._persistence_s = 0.0
         self._has_previous = False
 
+    def _structs(self, release: bool):
+        """The config and field-pointer structs (rebuilt when an array moved)."""
+        a = self.state.runtime.arrays
+        key = (bool(release), id(self.material), id(self.config), a["avalanche_parent"].data_ptr(),
+               a["avalanche_unstable"].data_ptr(),
+               a["avalanche_slope"].data_ptr(), a["avalanche_latch"].data_ptr())
+        cached = getattr(self, "_struct_cache", None)
+        if cached is None or cached[0] != key:
+            cached = (key, avalanche_config(self.material, self.config, release_settled_latches=release),
+                      avalanche_fields(self.state))
+            self._struct_cache = cached
+        return cached[1], cached[2]
+
     def label(self, *, release_settled_latches: bool = True) -> _lib.LabelResult:
         """Failure mask + canonical labels + winner statistics (device part of observe)."""
         rt = self.state.runtime
         rt.bind_stream()
-        cfg = avalanche_config(self.material, self.config, release_settled_latches=release_settled_latches)
-        fields = avalanche_fields(self.state)
+        cfg, fields = self._structs(release_settled_latches)
         out = _lib.LabelResult()
         rt.check(rt.lib.mv2_avalanche_label(rt.ctx, ctypes.byref(cfg), ctypes.byref(fields),
                                             ctypes.byref(out)))

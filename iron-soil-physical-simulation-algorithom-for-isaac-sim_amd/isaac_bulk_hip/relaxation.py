@@ -77,6 +77,9 @@ def _runtime(device: str) -> HipRuntime:
     return rt
 
 
+_CRIT_CACHE: dict = {}
+
+
 def relax_device(height_yx, dx: float, dy: float, critical_angle_deg: float = 34.0,
                  max_iterations: int = 20_000, tolerance: float = 1e-8, *,
                  runtime: HipRuntime | None = None) -> tuple[int, bool, float]:
@@ -101,7 +104,10 @@ def relax_device(height_yx, dx: float, dy: float, critical_angle_deg: float = 34
         raise ValueError("critical_angle_deg must be between 0 and 90")
     rt = runtime or _runtime(str(height_yx.device))
     rt.bind_stream()
-    crit = legacy_critical_differences(dx, dy, critical_angle_deg)
+    key = (float(dx), float(dy), float(critical_angle_deg))
+    crit = _CRIT_CACHE.get(key)
+    if crit is None:
+        crit = _CRIT_CACHE.setdefault(key, legacy_critical_differences(dx, dy, critical_angle_deg))
     res = _lib.RelaxResult()
     ny, nx = height_yx.shape
     rc = rt.lib.mv2_relax(rt.ctx, prec, height_yx.data_ptr(), int(ny), int(nx), crit.ctypes.data,
