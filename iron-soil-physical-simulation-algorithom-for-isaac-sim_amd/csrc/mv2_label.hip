@@ -29,24 +29,26 @@ struct LabelAux {
     double diag_f[8];
 };
 
-// initialize (gpu_large_avalanche.py:59-162)
-// The same pass zeroes the per-root counts, the statistics record and the
-// label spread sets for the labelling launches that follow.
-__global__ void __launch_bounds__(kBlock) avalanche_init_kernel(
-    const void *z_base, const void *resting, const void *mobile, const void *qx, const void *qy, int f32,
-    int ny, int nx, double dx, double dy, mv2_avalanche_config cfg, double area, double f6,
-    mv2_avalanche_fields f, LabelAux *aux, int32_t n_plus_one, unsigned long long *spread)
+struct InitArgs {
+    const void *z_base, *resting, *mobile, *qx, *qy;
+    int f32, ny, nx;
+    double dx, dy;
+    mv2_avalanche_config cfg;
+    double area, f6;
+    mv2_avalanche_fields f;
+};
+
+// initialize (gpu_large_avalanche.py:59-162) of one cell: gradient, slope,
+// cohesive start / stop margins, severity, the unstable flag and the latch
+// release; zeroes the cell's per-root count.  Returns the unstable flag.
+__device__ inline bool init_cell(const InitArgs &A, int64_t index)
 {
-    const int64_t n = (int64_t)ny * nx;
-    if (blockIdx.x == 0) {
-        for (int i = threadIdx.x; i < 3 * kSpread; i += blockDim.x) spread[(int64_t)i * kSpreadStride] = 0ull;
-        if (threadIdx.x == 0) {
-            for (int k = 0; k < 8; k++) { aux->diag_i[k] = 0; aux->diag_f[k] = 0.0; }
-            aux->diag_i[2] = n_plus_one;
-        }
-    }
-    for (int64_t index = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; index < n;
-         index += (int64_t)gridDim.x * blockDim.x) {
+    const void *z_base = A.z_base, *resting = A.resting, *mobile = A.mobile, *qx = A.qx, *qy = A.qy;
+    const int f32 = A.f32, ny = A.ny, nx = A.nx;
+    const double dx = A.dx, dy = A.dy, area = A.area, f6 = A.f6;
+    const mv2_avalanche_config &cfg = A.cfg;
+    const mv2_avalanche_fields &f = A.f;
+    {
         f.counts[index] = 0;
         const int row = (int)index / nx, col = (int)index - row * nx;
         int64_t left = index, right = index, down = index, up = index;
@@ -76,7 +78,6 @@ __global__ void __launch_bounds__(kBlock) avalanche_init_kernel(
         f.severity[index] = clampd(stop_margin / (denom >= 1.0e-12 ? denom : 1.0e-12), 0.0, 1.0);
         const bool active = layer_depth > cfg.dry_tolerance && start_margin > 0.0;
         f.unstable[index] = active ? 1 : 0;
-        f.labels[index] = active ? (int32_t)index : -1;
         double m = ld_mobile(mobile, f32, index);
         double speed = 0.0;
         if (m > cfg.dry_tolerance) {
@@ -88,7 +89,56 @@ __global__ void __launch_bounds__(kBlock) avalanche_init_kernel(
         bool resolved = exported > cfg.dry_tolerance * w;
         if (cfg.release_settled_latches && speed <= cfg.activity_speed && (resolved || stop_margin <= 0.0))
             f.latch[index] = 0;
+        return active;
     }
+}
+
+// The failure mask fused into the tile-local union-find: one workgroup per
+// 32 x 32 tile evaluates initialize for its cells (the same per-cell
+// arithmetic as the reference kernel), unites the unstable ones in LDS and
+// writes every unstable cell's tile-local root; workgroup 0 also zeroes the
+// statistics record and the label spread sets for the launches that follow.
+constexpr int kLocalBlock = kCclCells;   // one thread per cell of the 32 x 32 tile
+
+__global__ void __launch_bounds__(kLocalBlock) avalanche_local_kernel(InitArgs A, int tiles_x, LabelAux *aux,
+                                                                      int32_t n_plus_one, unsigned long long *spread)
+{
+    __shared__ int lp[kCclCells];
+    const int ny = A.ny, nx = A.nx;
+    if (blockIdx.x == 0) {
+        for (int i = threadIdx.x; i < 3 * kSpread; i += blockDim.x) spread[(int64_t)i * kSpreadStride] = 0ull;
+        if (threadIdx.x == 0) {
+            for (int k = 0; k < 8; k++) { aux->diag_i[k] = 0; aux->diag_f[k] = 0.0; }
+            aux->diag_i[2] = n_plus_one;
+        }
+    }
+    const int t = blockIdx.x;
+    const int tr0 = (t / tiles_x) * kCcl, tc0 = (t % tiles_x) * kCcl;
+    const int e = threadIdx.x;
+    const int lr = e / kCcl, lc = e % kCcl;
+    const int gr = tr0 + lr, gc = tc0 + lc;
+    const bool inside = gr < ny && gc < nx;
+    const bool masked = inside && init_cell(A, (int64_t)gr * nx + gc);
+    lp[e] = masked ? e : -1;
+    __syncthreads();
+    if (masked) {
+        if (lc + 1 < kCcl && lds_load(lp, e + 1) >= 0) lds_unite(lp, e, e + 1);
+        if (lr + 1 < kCcl) {
+            if (lds_load(lp, e + kCcl) >= 0) lds_unite(lp, e, e + kCcl);
+            if (A.cfg.connectivity == 8) {
+                if (lc + 1 < kCcl && lds_load(lp, e + kCcl + 1) >= 0) lds_unite(lp, e, e + kCcl + 1);
+                if (lc > 0 && lds_load(lp, e + kCcl - 1) >= 0) lds_unite(lp, e, e + kCcl - 1);
+            }
+        }
+    }
+    __syncthreads();
+    if (!inside) return;
+    int32_t out = -1;
+    if (masked) {
+        const int r = lds_find(lp, e);
+        out = (tr0 + r / kCcl) * nx + tc0 + r % kCcl;
+    }
+    A.f.labels[(int64_t)gr * nx + gc] = out;
 }
 
 __global__ void init_from_mask(const int32_t *mask, int32_t *labels, int64_t n)
@@ -521,6 +571,19 @@ static int check_ctx(mv2_context *ctx)
     return MV2_OK;
 }
 
+// the border unions and the flattening (after the tile-local pass)
+static int label_global(mv2_context *ctx, int32_t *labels, int ny, int nx, int connectivity, int32_t *counts)
+{
+    const int tiles_x = (nx + kCcl - 1) / kCcl;
+    const int tiles = tiles_x * ((ny + kCcl - 1) / kCcl);
+    hipLaunchKernelGGL(ccl_border_kernel, dim3(grid_for((int64_t)tiles * 3 * kCcl)), dim3(kBlock), 0, ctx->stream,
+                       labels, ny, nx, connectivity, tiles_x, tiles);
+    hipLaunchKernelGGL(ccl_flatten_kernel, dim3(tiles), dim3(kBlock), 0, ctx->stream, labels, ny, nx, tiles_x,
+                       counts);
+    MV2_HIP(ctx, hipGetLastError());
+    return MV2_OK;
+}
+
 static int label_core(mv2_context *ctx, int32_t *labels, int ny, int nx, int connectivity, int32_t *counts)
 {
     const int tiles_x = (nx + kCcl - 1) / kCcl;
@@ -571,13 +634,15 @@ extern "C" int mv2_avalanche_label(mv2_context *ctx, const mv2_avalanche_config 
     if (int rc = ensure_scratch(ctx, ctx->label_aux, sizeof(LabelAux) + sizeof(double) * kWinF * kLabelParts)) return rc;
     LabelAux *aux = (LabelAux *)ctx->label_aux.ptr;
     double *partials = (double *)((char *)ctx->label_aux.ptr + sizeof(LabelAux));
-    const int grid = grid_for(n, kBlock, 4096);
     if (int rc = ensure_scratch(ctx, ctx->spread, kSpreadScratchBytes)) return rc;
     unsigned long long *spread = (unsigned long long *)ctx->spread.ptr + (size_t)kLabelSpreadSet * kSpreadWords;
-    hipLaunchKernelGGL(avalanche_init_kernel, dim3(grid), dim3(kBlock), 0, ctx->stream, ctx->z_base, ctx->b_eff,
-                       ctx->mobile, ctx->qx, ctx->qy, f32, ny, nx, ctx->dx, ctx->dy, *cfg, area, f6, *f, aux,
+    const InitArgs init{ctx->z_base, ctx->b_eff, ctx->mobile, ctx->qx, ctx->qy, f32, ny, nx, ctx->dx, ctx->dy,
+                        *cfg, area, f6, *f};
+    const int tiles_x = (nx + kCcl - 1) / kCcl;
+    const int tiles = tiles_x * ((ny + kCcl - 1) / kCcl);
+    hipLaunchKernelGGL(avalanche_local_kernel, dim3(tiles), dim3(kLocalBlock), 0, ctx->stream, init, tiles_x, aux,
                        (int32_t)(n + 1), spread);
-    if (int rc = label_core(ctx, f->labels, ny, nx, cfg->connectivity, f->counts)) return rc;
+    if (int rc = label_global(ctx, f->labels, ny, nx, cfg->connectivity, f->counts)) return rc;
     const int sgrid = grid_for(n, kBlock, kLabelParts);
     hipLaunchKernelGGL(count_kernel, dim3(sgrid), dim3(kBlock), 0, ctx->stream, f->counts, n, spread);
     hipLaunchKernelGGL(winner_stats_kernel, dim3(sgrid), dim3(kBlock), 0, ctx->stream, f->labels, ctx->z_base,
