@@ -159,6 +159,8 @@ extern "C" void mv2_destroy(mv2_context *ctx)
     if (ctx->ev_start) hipEventDestroy(ctx->ev_start);
     if (ctx->ev_stop) hipEventDestroy(ctx->ev_stop);
     for (hipEvent_t e : ctx->sub_events) hipEventDestroy(e);
+    for (mv2::StepGraph &g : ctx->step_graphs) hipGraphExecDestroy(g.exec);
+    if (ctx->capture_stream) hipStreamDestroy(ctx->capture_stream);
     delete ctx;
 }
 

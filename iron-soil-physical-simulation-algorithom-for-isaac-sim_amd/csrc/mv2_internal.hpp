@@ -27,6 +27,15 @@ struct Scratch {
     size_t bytes = 0;
 };
 
+// A captured launch sequence of mv2_step (begin, the predicted substeps, the
+// end / fold passes).  `key` holds the bytes of every launch argument of the
+// sequence, so a replay is valid exactly when a new call's arguments equal it.
+struct StepGraph {
+    std::vector<unsigned char> key;
+    hipGraphExec_t exec = nullptr;
+    unsigned long long last_use = 0;
+};
+
 }  // namespace mv2
 
 struct mv2_batch_state;
@@ -75,6 +84,12 @@ struct mv2_context {
     mv2_batch_state *batch = nullptr;     // batched terrains bound to this context
     double *noalloc_scratch = nullptr;    // device address of the allocation-free ledger's partials
     mv2::Scratch patch;                   // FailureZone activation: per-cell flags, partials, result
+    int graphs = 0;                       // replay cached HIP graphs of mv2_step's launch sequence (opt-in)
+    hipStream_t capture_stream = nullptr; // private stream the sequences are captured on
+    std::vector<mv2::StepGraph> step_graphs;               // instantiated sequences (at most kStepGraphsMax)
+    std::vector<std::vector<unsigned char>> step_seen;     // keys seen once (captured on the second sighting)
+    unsigned long long graph_clock = 0;
+    long long graph_replays = 0;          // mv2_step calls served by a replay
 };
 
 void mv2_batch_release(mv2_context *ctx);

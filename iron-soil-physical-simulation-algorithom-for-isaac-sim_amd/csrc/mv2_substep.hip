@@ -34,8 +34,10 @@
 #include "mv2_internal.hpp"
 #include "mv2_strip32.hpp"
 
+#include <algorithm>
 #include <cmath>
 #include <cstdlib>
+#include <cstring>
 
 namespace mv2 {
 
@@ -1020,6 +1022,136 @@ static void assemble_record(const FoldOut &fo, const V2Args<P> &a, mv2_step_reco
     rec->complete = C.done && (!P::kQuant || C.end_done);
 }
 
+// Launch geometry of one mv2_step call.
+struct StepLaunch {
+    int band;          // FP32 band height (rows per wave)
+    int gs, gu, gfold; // begin / end workgroups, substep workgroups, fold workgroups
+    FoldOut *fold;     // pinned host record the fold pass writes
+};
+
+template <class P>
+static void enqueue_begin(const V2Args<P> &a, const StepLaunch &L, hipStream_t st)
+{
+    if constexpr (P::kQuant) {
+        switch (L.band) {
+        case 2: hipLaunchKernelGGL(v2_begin32_kernel<2>, dim3(L.gs), dim3(kBlock), 0, st, a); break;
+        case 3: hipLaunchKernelGGL(v2_begin32_kernel<3>, dim3(L.gs), dim3(kBlock), 0, st, a); break;
+        case 5: hipLaunchKernelGGL(v2_begin32_kernel<5>, dim3(L.gs), dim3(kBlock), 0, st, a); break;
+        case 6: hipLaunchKernelGGL(v2_begin32_kernel<6>, dim3(L.gs), dim3(kBlock), 0, st, a); break;
+        default: hipLaunchKernelGGL(v2_begin32_kernel<4>, dim3(L.gs), dim3(kBlock), 0, st, a); break;
+        }
+    } else {
+        hipLaunchKernelGGL(v2_begin_kernel<P>, dim3(L.gs), dim3(kBlock), 0, st, a);
+    }
+}
+
+// substep launches k0 .. k0 + count - 1 (event pairs around each in timing mode)
+template <class P>
+static int enqueue_substeps(mv2_context *ctx, const V2Args<P> &a, const StepLaunch &L, hipStream_t st, int k0,
+                            int count, int eager_until, bool timing)
+{
+    for (int k = k0; k < k0 + count; k++) {
+        if (timing) MV2_HIP(ctx, hipEventRecord(ctx->sub_events[2 * k], st));
+        const int eager = k < eager_until ? 1 : 0;
+        if constexpr (P::kQuant) {
+            switch (L.band) {
+            case 2: hipLaunchKernelGGL((v2_substep_kernel<P, 2>), dim3(L.gu), dim3(kBlock), 0, st, a, k, eager); break;
+            case 3: hipLaunchKernelGGL((v2_substep_kernel<P, 3>), dim3(L.gu), dim3(kBlock), 0, st, a, k, eager); break;
+            case 5: hipLaunchKernelGGL((v2_substep_kernel<P, 5>), dim3(L.gu), dim3(kBlock), 0, st, a, k, eager); break;
+            case 6: hipLaunchKernelGGL((v2_substep_kernel<P, 6>), dim3(L.gu), dim3(kBlock), 0, st, a, k, eager); break;
+            default: hipLaunchKernelGGL((v2_substep_kernel<P, 4>), dim3(L.gu), dim3(kBlock), 0, st, a, k, eager); break;
+            }
+        } else {
+            hipLaunchKernelGGL(v2_substep_kernel<P>, dim3(L.gu), dim3(kBlock), 0, st, a, k, eager);
+        }
+        if (timing) MV2_HIP(ctx, hipEventRecord(ctx->sub_events[2 * k + 1], st));
+    }
+    return MV2_OK;
+}
+
+// after launch k - 1: FP32's first launch past the loop's end did the end pass;
+// FP64 runs it as a launch of its own.  Then the fold into the host record.
+template <class P>
+static void enqueue_close(const V2Args<P> &a, const StepLaunch &L, hipStream_t st, int k)
+{
+    if constexpr (!P::kQuant) hipLaunchKernelGGL(v2_end_kernel<P>, dim3(L.gs), dim3(kBlock), 0, st, a, k);
+    hipLaunchKernelGGL(v2_fold_kernel<P>, dim3(L.gfold), dim3(kBlock), 0, st, a, k, L.gs, L.gu, L.fold);
+}
+
+constexpr size_t kStepGraphsMax = 16;   // instantiated sequences kept per context
+constexpr size_t kStepSeenMax = 16;     // keys remembered from a single sighting
+
+// The first round of a step (begin, `chunk` substeps, end / fold) replayed
+// from a cached graph when an earlier call had the same launch arguments.  A
+// key is captured on its second sighting, so callers whose arguments change
+// every call (a varying dt) stay on plain launches.  The graph is captured on
+// a private stream (the bound stream may be the null stream, which cannot be
+// captured) and launched on `st`; it records no memset and no host call.
+template <class P>
+static int replay_step(mv2_context *ctx, const V2Args<P> &a, const StepLaunch &L, int chunk, int eager_until,
+                       hipStream_t st, bool *replayed)
+{
+    *replayed = false;
+    struct Shape {
+        StepLaunch launch;
+        int chunk, eager_until, quantised, pad;
+    } shape;
+    std::memset(&shape, 0, sizeof(shape));
+    shape.launch = L;
+    shape.chunk = chunk;
+    shape.eager_until = eager_until;
+    shape.quantised = P::kQuant ? 1 : 0;
+    std::vector<unsigned char> key(sizeof(Shape) + sizeof(V2Args<P>));
+    std::memcpy(key.data(), &shape, sizeof(Shape));
+    std::memcpy(key.data() + sizeof(Shape), &a, sizeof(V2Args<P>));
+    StepGraph *hit = nullptr;
+    for (StepGraph &g : ctx->step_graphs)
+        if (g.key == key) {
+            hit = &g;
+            break;
+        }
+    if (!hit) {
+        auto seen = std::find(ctx->step_seen.begin(), ctx->step_seen.end(), key);
+        if (seen == ctx->step_seen.end()) {
+            if (ctx->step_seen.size() >= kStepSeenMax) ctx->step_seen.erase(ctx->step_seen.begin());
+            ctx->step_seen.push_back(std::move(key));
+            return MV2_OK;
+        }
+        ctx->step_seen.erase(seen);
+        if (!ctx->capture_stream)
+            MV2_HIP(ctx, hipStreamCreateWithFlags(&ctx->capture_stream, hipStreamNonBlocking));
+        hipStream_t cs = ctx->capture_stream;
+        MV2_HIP(ctx, hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
+        enqueue_begin<P>(a, L, cs);
+        enqueue_substeps<P>(ctx, a, L, cs, 0, chunk, eager_until, false);
+        enqueue_close<P>(a, L, cs, chunk);
+        hipGraph_t graph = nullptr;
+        hipError_t e = hipStreamEndCapture(cs, &graph);
+        if (e == hipSuccess) e = hipGetLastError();
+        if (e != hipSuccess) {
+            if (graph) hipGraphDestroy(graph);
+            return check_hip(ctx, e, "capture of the step's launch sequence");
+        }
+        hipGraphExec_t exec = nullptr;
+        e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+        hipGraphDestroy(graph);
+        if (e != hipSuccess) return check_hip(ctx, e, "hipGraphInstantiate");
+        if (ctx->step_graphs.size() >= kStepGraphsMax) {
+            auto oldest = std::min_element(ctx->step_graphs.begin(), ctx->step_graphs.end(),
+                                           [](const StepGraph &x, const StepGraph &y) { return x.last_use < y.last_use; });
+            hipGraphExecDestroy(oldest->exec);
+            ctx->step_graphs.erase(oldest);
+        }
+        ctx->step_graphs.push_back(StepGraph{std::move(key), exec, 0});
+        hit = &ctx->step_graphs.back();
+    }
+    hit->last_use = ++ctx->graph_clock;
+    MV2_HIP(ctx, hipGraphLaunch(hit->exec, st));
+    ctx->graph_replays++;
+    *replayed = true;
+    return MV2_OK;
+}
+
 template <class P>
 static int launch_step(mv2_context *ctx, const mv2_step_config *cfg, mv2_step_record *out, int fixed_chunk = 0)
 {
@@ -1061,6 +1193,7 @@ static int launch_step(mv2_context *ctx, const mv2_step_config *cfg, mv2_step_re
     if (int rc = ensure_scratch(ctx, ctx->step_diag, sizeof(double) * kDiag * (size_t)gu)) return rc;
     if (int rc = ensure_scratch(ctx, ctx->spread, kSpreadScratchBytes)) return rc;
     V2Args<P> a;
+    std::memset(&a, 0, sizeof(a));      // padding bytes are part of the graph key
     a.ny = ctx->ny;
     a.nx = ctx->nx;
     a.dx = ctx->dx;
@@ -1099,51 +1232,33 @@ static int launch_step(mv2_context *ctx, const mv2_step_config *cfg, mv2_step_re
     // error anywhere below leaves v2_clean at -1 and the next step resets
     if (ctx->v2_clean != parity) hipLaunchKernelGGL(v2_reset_kernel, dim3(1), dim3(kBlock), 0, st, a.sync, a.spread);
     ctx->v2_clean = -1;
-    if constexpr (P::kQuant) {
-        switch (band) {
-        case 2: hipLaunchKernelGGL(v2_begin32_kernel<2>, dim3(gs), dim3(kBlock), 0, st, a); break;
-        case 3: hipLaunchKernelGGL(v2_begin32_kernel<3>, dim3(gs), dim3(kBlock), 0, st, a); break;
-        case 5: hipLaunchKernelGGL(v2_begin32_kernel<5>, dim3(gs), dim3(kBlock), 0, st, a); break;
-        case 6: hipLaunchKernelGGL(v2_begin32_kernel<6>, dim3(gs), dim3(kBlock), 0, st, a); break;
-        default: hipLaunchKernelGGL(v2_begin32_kernel<4>, dim3(gs), dim3(kBlock), 0, st, a); break;
-        }
-    }
-    else hipLaunchKernelGGL(v2_begin_kernel<P>, dim3(gs), dim3(kBlock), 0, st, a);
-    MV2_HIP(ctx, hipGetLastError());
     int k = 0;
     int chunk = fixed_chunk > 0 ? fixed_chunk : (ctx->v2_predicted > 0 ? ctx->v2_predicted + 1 : 4);
     // launches expected to work stage their tile before the loop state arrives
     int eager_until = fixed_chunk > 0 ? 0 : (ctx->v2_predicted > 0 ? ctx->v2_predicted : 4);
-    const int gfold = (kFoldWaves + kBlock / kWave - 1) / (kBlock / kWave);
+    const StepLaunch L{band, gs, gu, (kFoldWaves + kBlock / kWave - 1) / (kBlock / kWave), fold};
+    // the first round (begin, predicted substeps, end / fold) as one graph launch
+    bool replayed = false;
+    if (ctx->graphs && !timing && fixed_chunk == 0)
+        if (int rc = replay_step<P>(ctx, a, L, chunk, eager_until, st, &replayed)) return rc;
+    if (replayed) k = chunk;
+    else enqueue_begin<P>(a, L, st);
+    MV2_HIP(ctx, hipGetLastError());
     for (;;) {
-        if (timing && (int)ctx->sub_events.size() < 2 * (k + chunk)) {
-            while ((int)ctx->sub_events.size() < 2 * (k + chunk)) {
-                hipEvent_t e;
-                MV2_HIP(ctx, hipEventCreate(&e));
-                ctx->sub_events.push_back(e);
-            }
-        }
-        for (int j = 0; j < chunk; j++, k++) {
-            if (timing) MV2_HIP(ctx, hipEventRecord(ctx->sub_events[2 * k], st));
-            const int eager = k < eager_until ? 1 : 0;
-            if constexpr (P::kQuant) {
-                switch (band) {
-                case 2: hipLaunchKernelGGL((v2_substep_kernel<P, 2>), dim3(gu), dim3(kBlock), 0, st, a, k, eager); break;
-                case 3: hipLaunchKernelGGL((v2_substep_kernel<P, 3>), dim3(gu), dim3(kBlock), 0, st, a, k, eager); break;
-                case 5: hipLaunchKernelGGL((v2_substep_kernel<P, 5>), dim3(gu), dim3(kBlock), 0, st, a, k, eager); break;
-                case 6: hipLaunchKernelGGL((v2_substep_kernel<P, 6>), dim3(gu), dim3(kBlock), 0, st, a, k, eager); break;
-                default: hipLaunchKernelGGL((v2_substep_kernel<P, 4>), dim3(gu), dim3(kBlock), 0, st, a, k, eager); break;
+        if (!replayed) {
+            if (timing && (int)ctx->sub_events.size() < 2 * (k + chunk)) {
+                while ((int)ctx->sub_events.size() < 2 * (k + chunk)) {
+                    hipEvent_t e;
+                    MV2_HIP(ctx, hipEventCreate(&e));
+                    ctx->sub_events.push_back(e);
                 }
-            } else {
-                hipLaunchKernelGGL(v2_substep_kernel<P>, dim3(gu), dim3(kBlock), 0, st, a, k, eager);
             }
-            if (timing) MV2_HIP(ctx, hipEventRecord(ctx->sub_events[2 * k + 1], st));
+            if (int rc = enqueue_substeps<P>(ctx, a, L, st, k, chunk, eager_until, timing)) return rc;
+            k += chunk;
+            enqueue_close<P>(a, L, st, k);
+            MV2_HIP(ctx, hipGetLastError());
         }
-        // FP32: the first launch past the loop's end did the end pass; FP64
-        // runs it as a launch of its own
-        if constexpr (!P::kQuant) hipLaunchKernelGGL(v2_end_kernel<P>, dim3(gs), dim3(kBlock), 0, st, a, k);
-        hipLaunchKernelGGL(v2_fold_kernel<P>, dim3(gfold), dim3(kBlock), 0, st, a, k, gs, gu, fold);
-        MV2_HIP(ctx, hipGetLastError());
+        replayed = false;
         MV2_HIP(ctx, hipEventRecord(ctx->ev_stop, st));
         if (int rc = stream_wait(ctx, st)) return rc;
         if (fold->ctl.done && (!P::kQuant || fold->ctl.end_done)) break;
@@ -1273,3 +1388,12 @@ extern "C" int mv2_set_timing(mv2_context *ctx, int enable)
     ctx->timing = enable ? 1 : 0;
     return MV2_OK;
 }
+
+extern "C" int mv2_set_graphs(mv2_context *ctx, int enable)
+{
+    if (!ctx) return set_error(nullptr, MV2_EINVAL, "[MobileV2] null context");
+    ctx->graphs = enable ? 1 : 0;
+    return MV2_OK;
+}
+
+extern "C" long long mv2_graph_replays(const mv2_context *ctx) { return ctx ? ctx->graph_replays : -1; }

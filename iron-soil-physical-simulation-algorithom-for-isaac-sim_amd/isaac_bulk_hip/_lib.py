@@ -124,6 +124,8 @@ SIGNATURES: dict[str, tuple[object, list[object]]] = {
     "mv2_summary": (_I, [_P, ctypes.POINTER(StepConfig), _P]),
     "mv2_static_diagnostic": (_I, [_P, _P]),
     "mv2_set_timing": (_I, [_P, _I]),
+    "mv2_set_graphs": (_I, [_P, _I]),
+    "mv2_graph_replays": (ctypes.c_longlong, [_P]),
     "mv2_set_external": (_I, [_P, _P, _P, _D]),
     "mv2_batch_bind": (_I, [_P, _I, _P, _P, _P, _D, _D, _P, _P, _P, _P]),
     "mv2_batch_set_tile_shape": (_I, [_P, _I]),
