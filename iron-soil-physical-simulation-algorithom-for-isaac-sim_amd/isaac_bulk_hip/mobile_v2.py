@@ -148,6 +148,66 @@ def run_step(state: DeviceBulkState, config: MobileV2Config, dt: float, *, densi
     return rec
 
 
+def step_from_record(rec: _lib.StepRecord, area: float, density: float) -> WarpMobileStep:
+    """WarpMobileStep of a raw step record (mobile_v2_warp.py:300-344 field
+    meanings): ledger sums are per unit density and cell area on the device,
+    impulses scale by density x area.  Scalars come from ctypes slices as
+    Python floats; the seven 3-vectors are rows of one array (distinct rows,
+    so no two fields share elements)."""
+    before = rec.summary_before[:]
+    if rec.static_exit:
+        vec = np.array(((before[1], before[2], 0.0), (before[1], before[2], 0.0)) + ((0.0, 0.0, 0.0),) * 5)
+        fields = dict(
+            volume_before_m3=before[0], volume_after_m3=before[0],
+            momentum_before_terrain_kg_m_s=vec[0], momentum_after_terrain_kg_m_s=vec[1],
+            gravity_pressure_impulse_terrain_ns=vec[2], basal_friction_impulse_terrain_ns=vec[3],
+            tool_impulse_on_mobile_terrain_ns=vec[4], numerical_dissipative_impulse_terrain_ns=vec[5],
+            maximum_speed_m_s=before[3], substeps=0, cfl_limited=False,
+            kinetic_energy_before_j=before[4], kinetic_energy_after_j=before[4],
+            gravity_pressure_work_j=0.0, basal_friction_work_j=0.0, tool_work_j=0.0,
+            transport_numerical_energy_residual_j=0.0, donor_export_m3=0.0,
+            receiver_import_m3=0.0, transport_mass_residual_m3=0.0,
+            advected_momentum_crossings_kg_m_s=vec[6])
+        return _frozen_step(fields)
+    after = rec.summary_after[:]
+    diag = rec.diag[:]
+    gx, gy = diag[0] * density * area, diag[1] * density * area
+    fx, fy = diag[4] * density * area, diag[5] * density * area
+    friction_work = 0.5 * diag[7] * density * area
+    vec = np.array(((before[1], before[2], 0.0), (after[1], after[2], 0.0), (gx, gy, 0.0), (fx, fy, 0.0),
+                    (0.0, 0.0, 0.0), (after[1] - before[1] - gx - fx, after[2] - before[2] - gy - fy, 0.0),
+                    (0.0, 0.0, 0.0)))
+    fields = dict(
+        volume_before_m3=before[0], volume_after_m3=after[0],
+        momentum_before_terrain_kg_m_s=vec[0], momentum_after_terrain_kg_m_s=vec[1],
+        gravity_pressure_impulse_terrain_ns=vec[2], basal_friction_impulse_terrain_ns=vec[3],
+        tool_impulse_on_mobile_terrain_ns=vec[4], numerical_dissipative_impulse_terrain_ns=vec[5],
+        maximum_speed_m_s=after[3], substeps=int(rec.substeps),
+        cfl_limited=bool(rec.cfl_limited), kinetic_energy_before_j=before[4],
+        kinetic_energy_after_j=after[4],
+        gravity_pressure_work_j=(after[5] + after[6]) - (before[5] + before[6]),
+        basal_friction_work_j=friction_work, tool_work_j=0.0,
+        transport_numerical_energy_residual_j=(after[4] + after[5] + after[6])
+        - (before[4] + before[5] + before[6]) + friction_work,
+        donor_export_m3=0.0, receiver_import_m3=0.0,
+        transport_mass_residual_m3=after[0] - before[0],
+        advected_momentum_crossings_kg_m_s=vec[6])
+    return _frozen_step(fields)
+
+
+_STEP_FIELDS = tuple(WarpMobileStep.__dataclass_fields__)
+
+
+def _frozen_step(fields: dict) -> WarpMobileStep:
+    """The frozen record filled through its instance dict: the generated
+    __init__ sets each field with object.__setattr__, which costs more than
+    the rest of a step call's host work.  Same fields, same frozen class."""
+    assert len(fields) == len(_STEP_FIELDS)
+    step = object.__new__(WarpMobileStep)
+    step.__dict__.update(fields)
+    return step
+
+
 class HipProductionMobileV2Solver:
     """Accepted V2 equations consuming the persistent device fields (step_resident)."""
 
@@ -197,46 +257,7 @@ class HipProductionMobileV2Solver:
         rec = run_step(state, self.config, float(dt_s), density=density, limit_mode=1,
                        weight_mode=1, error_prefix="[MobileV2Production] ")
         self.last_record = rec
-        # zero-copy views of the record (a new record per call, never mutated)
-        before = np.frombuffer(rec.summary_before, dtype=np.float64)
-        after = np.frombuffer(rec.summary_after, dtype=np.float64)
-        diag = np.frombuffer(rec.diag, dtype=np.float64)
-        area = state.grid.dx * state.grid.dy
-        before_p = np.asarray([before[1], before[2], 0.0])
-        if rec.static_exit:
-            return WarpMobileStep(
-                volume_before_m3=float(before[0]), volume_after_m3=float(before[0]),
-                momentum_before_terrain_kg_m_s=before_p, momentum_after_terrain_kg_m_s=before_p.copy(),
-                gravity_pressure_impulse_terrain_ns=np.zeros(3),
-                basal_friction_impulse_terrain_ns=np.zeros(3),
-                tool_impulse_on_mobile_terrain_ns=np.zeros(3),
-                numerical_dissipative_impulse_terrain_ns=np.zeros(3),
-                maximum_speed_m_s=float(before[3]), substeps=0, cfl_limited=False,
-                kinetic_energy_before_j=float(before[4]), kinetic_energy_after_j=float(before[4]),
-                gravity_pressure_work_j=0.0, basal_friction_work_j=0.0, tool_work_j=0.0,
-                transport_numerical_energy_residual_j=0.0, donor_export_m3=0.0,
-                receiver_import_m3=0.0, transport_mass_residual_m3=0.0,
-                advected_momentum_crossings_kg_m_s=np.zeros(3))
-        gravity = np.asarray([diag[0], diag[1], 0.0]) * density * area
-        friction = np.asarray([diag[4], diag[5], 0.0]) * density * area
-        after_p = np.asarray([after[1], after[2], 0.0])
-        friction_work = float(0.5 * diag[7] * density * area)
-        return WarpMobileStep(
-            volume_before_m3=float(before[0]), volume_after_m3=float(after[0]),
-            momentum_before_terrain_kg_m_s=before_p, momentum_after_terrain_kg_m_s=after_p,
-            gravity_pressure_impulse_terrain_ns=gravity, basal_friction_impulse_terrain_ns=friction,
-            tool_impulse_on_mobile_terrain_ns=np.zeros(3),
-            numerical_dissipative_impulse_terrain_ns=after_p - before_p - gravity - friction,
-            maximum_speed_m_s=float(after[3]), substeps=int(rec.substeps),
-            cfl_limited=bool(rec.cfl_limited), kinetic_energy_before_j=float(before[4]),
-            kinetic_energy_after_j=float(after[4]),
-            gravity_pressure_work_j=float((after[5] + after[6]) - (before[5] + before[6])),
-            basal_friction_work_j=friction_work, tool_work_j=0.0,
-            transport_numerical_energy_residual_j=float(
-                (after[4] + after[5] + after[6]) - (before[4] + before[5] + before[6]) + friction_work),
-            donor_export_m3=0.0, receiver_import_m3=0.0,
-            transport_mass_residual_m3=float(after[0] - before[0]),
-            advected_momentum_crossings_kg_m_s=np.zeros(3))
+        return step_from_record(rec, state.grid.dx * state.grid.dy, density)
 
     def diagnostics(self) -> dict:
         return {
