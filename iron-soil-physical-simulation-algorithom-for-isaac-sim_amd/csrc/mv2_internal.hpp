@@ -390,12 +390,13 @@ constexpr int kSpread = 64;
 constexpr int kSpreadStride = 16;                    // u64 words between slots
 constexpr int kSpreadWords = kSpread * kSpreadStride;
 // the step uses sets 0..4 (even frames) and 16..20 (odd frames), the envelope
-// 5..7, the relaxation 8..10, the labelling 12..14
+// 5..7, the labelling 12..14, the relaxation 21..26
 constexpr int kEnvSpreadSet = 5;
-constexpr int kRelaxSpreadSet = 8;
 constexpr int kLabelSpreadSet = 12;
 constexpr int kStepOddSpreadSet = 16;
-constexpr size_t kSpreadScratchBytes = (size_t)21 * kSpreadWords * 8;
+constexpr int kRelaxSpreadSet = 21;                  // six sets: three launch groups of two iterations
+constexpr int kRelaxSpreadSets = 6;
+constexpr size_t kSpreadScratchBytes = (size_t)(kRelaxSpreadSet + kRelaxSpreadSets) * kSpreadWords * 8;
 
 __device__ inline void spread_max(unsigned long long *slots, double v)
 {

@@ -2,19 +2,20 @@
 // slope_model.py:273-343) and the compact colour-batch sweep
 // (pkg/solvers/warp_compact_frontier.py:57-130).
 //
-// relax: one launch per iteration.  Each workgroup stages a tile plus an
-// 8-cell halo in LDS and performs all 8 red-black sub-phases of an iteration
-// there.  A sub-phase moves information one cell along its direction, so after
-// the 8 sub-phases (two per direction, directions x, y and both diagonals) at
-// most 6 halo layers are stale; the interior and every edge it owns are exact.
-// Then the interior is written (ping-pong buffers) and the iteration's maximum
-// excess is reduced into a spread maximum (64 slots).  The next launch tests
-// it against the tolerance on the device and returns at once once converged,
-// so the host enqueues batches of launches and reads one result per batch.
-// An iteration costs one read + one write of the field instead of eight
-// full-array passes; a tile whose 3 x 3 tile neighbourhood did not change in
-// the previous launch is skipped (its field and excess are already known).
-// Directions and parities: mv2_relax_tile.hpp.
+// relax: a launch applies up to K iterations to LDS tiles (tile interior
+// 44 x 48, halo 6K + 2; mv2_relax_tile.hpp) and writes the field once
+// (ping-pong buffers).  The first launch of a call applies one iteration (a
+// field at rest converges there), later FP32 launches two.  Each iteration's
+// maximum excess over the owned edges is reduced into its own spread-maximum
+// set; the next launch tests them in order against the tolerance on the
+// device.  The reference returns the field of the first converged iteration:
+// when that is the first of a launch's two, the following launch re-applies
+// exactly that one iteration to the launch's input buffer (still intact) and
+// ends the loop.  Launches after the loop's end return at once, so the host
+// enqueues batches of launches and reads one result per batch.  A tile whose
+// 3 x 3 tile neighbourhood did not change in the previous launch, which
+// applied as many iterations as this one, is skipped: its field and maxima
+// are already known.
 #include "mv2_internal.hpp"
 #include "mv2_frontier_edge.hpp"
 #include "mv2_relax_tile.hpp"
@@ -24,14 +25,29 @@
 
 namespace mv2 {
 
-// Iteration state: launch k reads ctl[k & 1], its workgroup 0 writes
-// ctl[(k + 1) & 1]; launch k reduces its maximum excess into spread set k % 3,
-// tests set (k - 1) % 3 (the previous iteration) against the tolerance and
-// clears set (k + 1) % 3 for the next launch.
+// Iteration state: launch k reads ctl[k & 1] and its workgroup 0 writes
+// ctl[(k + 1) & 1].  Launch k reduces the maxima of its iterations into spread
+// group k % 3 (set 2 g + i for its iteration i), tests group (k - 1) % 3 (the
+// previous launch) against the tolerance and clears group (k + 1) % 3.
+constexpr int kRelaxKMax = 2;                 // iterations per launch at most
+constexpr int kRelaxGroups = 3;
+static_assert(kRelaxGroups * kRelaxKMax <= kRelaxSpreadSets, "the relaxation's spread sets");
+using RelaxFirst = RelaxGeom<44, 48, 640, 1, false>;      // FP32: a call's first launch
+using RelaxPair = RelaxGeom<44, 48, 1024, 2, true>;       // FP32: later launches
+using RelaxSingle64 = RelaxGeom<44, 48, 960, 1, true>;     // FP64: every launch (spill-free at 960)
+static_assert(RelaxFirst::TH == RelaxPair::TH && RelaxFirst::TW == RelaxPair::TW &&
+              RelaxFirst::TH == RelaxSingle64::TH && RelaxFirst::TW == RelaxSingle64::TW,
+              "all launches of a call share one tile grid");
+constexpr int kRelaxTileH = RelaxFirst::TH, kRelaxTileW = RelaxFirst::TW;
+
 struct RelaxControl {
-    int iterations;      // iterations applied to the field
+    int iterations;      // iterations applied to the field in buffer `src`
     int src;             // buffer holding the current field
     int done, converged;
+    int n_prev;          // iterations of the previous launch whose maxima are untested
+    int prev_src;        // input buffer of the previous launch
+    int prev_iterations; // iterations applied before the previous launch
+    int replay;          // > 0: re-apply this many iterations to `src`, then end the loop
     double last;         // maximum excess of the last applied iteration
 };
 struct RelaxSync {
@@ -46,37 +62,61 @@ struct RelaxArgs {
     int max_iterations;
     double tolerance;
     RelaxSync *sync;
-    unsigned long long *spread;   // three spread-maximum sets
+    unsigned long long *spread;   // kRelaxGroups x kRelaxKMax spread-maximum sets
     mv2_relax_result *out;
     int tiles_x, tiles;
     int32_t *tchanged;            // [2][tiles]: the tile's interior changed in launch k (parity k & 1)
-    double *texcess;              // [tiles]: the tile's maximum excess in its last computed iteration
+    int32_t *titers;              // [tiles]: iterations of the tile's last computed launch
+    double *texcess;              // [tiles][kRelaxKMax]: its per-iteration maxima there
 };
 
-// loop state for launch k: did iteration k (applied by launch k - 1) converge?
+__device__ inline unsigned long long *relax_set(unsigned long long *spread, int group, int i)
+{
+    return spread + (size_t)(group * kRelaxKMax + i) * kSpreadWords;
+}
+
+// loop state for launch k: test the previous launch's iterations in order.
 // Every thread of the block calls it.
 template <class R>
 __device__ inline RelaxControl relax_resolve(const RelaxArgs<R> &a, int k, double *smem1)
 {
     RelaxControl C = a.sync->ctl[k & 1];      // written by an earlier launch: plain load
-    if (!C.done && k > 0) {
-        C.last = spread_read_max(a.spread + ((k - 1) % 3) * kSpreadWords, smem1);
-        if (C.last <= a.tolerance) {
+    if (!C.done && C.replay == 0 && C.n_prev > 0) {
+        const int g = (k - 1) % kRelaxGroups;
+        int first = -1;
+        double mfirst = 0.0, mlast = 0.0;
+        for (int i = 0; i < C.n_prev; i++) {
+            const double m = spread_read_max(relax_set(a.spread, g, i), smem1);
+            if (first < 0 && m <= a.tolerance) {
+                first = i;
+                mfirst = m;
+            }
+            mlast = m;
+        }
+        if (first == C.n_prev - 1) {
             C.converged = 1;
             C.done = 1;
-        } else if (C.iterations >= a.max_iterations) {
-            C.done = 1;
+            C.last = mfirst;
+        } else if (first >= 0) {
+            // converged inside the previous launch: redo its first iterations
+            C.src = C.prev_src;
+            C.iterations = C.prev_iterations;
+            C.replay = first + 1;
+            C.last = mfirst;
+        } else {
+            C.last = mlast;
+            if (C.iterations >= a.max_iterations) C.done = 1;
         }
+        C.n_prev = 0;
     }
     return C;
 }
 
-// One relaxation iteration per launch (the tile work is in mv2_relax_tile.hpp).
-template <class R>
-__global__ void __launch_bounds__(kRelaxBlock) relax_kernel(RelaxArgs<R> a, int k)
+template <class R, class G>
+__global__ void __launch_bounds__(G::BLOCK) relax_kernel(RelaxArgs<R> a, int k)
 {
-    __shared__ R tile[kRelaxLH * kRelaxLW];
-    __shared__ double smem[8];
+    __shared__ R tile[G::kCells];
+    __shared__ double smem[kRelaxKMax * (G::BLOCK / kWave)];
     RelaxSync *s = a.sync;
     const bool lead = blockIdx.x == 0 && threadIdx.x == 0;
     const RelaxControl C = relax_resolve(a, k, smem);
@@ -84,24 +124,37 @@ __global__ void __launch_bounds__(kRelaxBlock) relax_kernel(RelaxArgs<R> a, int 
         if (lead) s->ctl[(k + 1) & 1] = C;
         return;
     }
+    const int left = a.max_iterations - C.iterations;
+    const int n = C.replay > 0 ? C.replay : (G::K < left ? G::K : left);
     const int src = C.src;
-    if (blockIdx.x == 0) spread_clear(a.spread + ((k + 1) % 3) * kSpreadWords);
+    if (blockIdx.x == 0)
+        for (int i = 0; i < kRelaxKMax; i++) spread_clear(relax_set(a.spread, (k + 1) % kRelaxGroups, i));
     if (lead) {
         RelaxControl nc = C;
-        nc.iterations = C.iterations + 1;
+        nc.prev_src = src;
+        nc.prev_iterations = C.iterations;
+        nc.iterations = C.iterations + n;
         nc.src = src ^ 1;
+        nc.n_prev = n;
+        if (C.replay > 0) {      // the re-applied iterations end at the converged one
+            nc.replay = 0;
+            nc.n_prev = 0;
+            nc.done = 1;
+            nc.converged = 1;
+        }
         s->ctl[(k + 1) & 1] = nc;
     }
     const int t = blockIdx.x;
     const int trow = t / a.tiles_x, tcol = t % a.tiles_x;
-    const int r0 = trow * kRelaxTH, c0 = tcol * kRelaxTW;
-    // Quiet tiles: an iteration of a tile reads its interior and an 8-cell
-    // halo, i.e. its 3 x 3 tile neighbourhood.  When none of those tiles
-    // changed in launch k - 1, the tile's input equals the input of its last
-    // computed iteration, which changed nothing, so this iteration would
-    // reproduce the same field (already in both ping-pong buffers) and the same
-    // excess: the stored excess stands in for it.  Launch 0 runs every tile.
-    if (k > 0) {
+    const int r0 = trow * G::TH, c0 = tcol * G::TW;
+    unsigned long long *sets = a.spread + (size_t)(k % kRelaxGroups) * kRelaxKMax * kSpreadWords;
+    // Quiet tiles: the iterations of a tile read its interior and a halo of
+    // at most one tile, i.e. its 3 x 3 tile neighbourhood.  When none of
+    // those tiles changed in launch k - 1, the tile's input equals the input
+    // of its last computed launch; with as many iterations, this launch would
+    // reproduce that launch's field (already in both buffers) and maxima.
+    // Launch 0 and a replay compute every tile.
+    if (k > 0 && C.replay == 0) {
         const int ty = a.tiles / a.tiles_x;
         int flag = 0;
         if (threadIdx.x < 9) {
@@ -109,33 +162,44 @@ __global__ void __launch_bounds__(kRelaxBlock) relax_kernel(RelaxArgs<R> a, int 
             if (rr >= 0 && rr < ty && cc >= 0 && cc < a.tiles_x)
                 flag = a.tchanged[((k - 1) & 1) * a.tiles + rr * a.tiles_x + cc];
         }
-        if (!__syncthreads_or(flag)) {
+        if (!__syncthreads_or(flag) && a.titers[t] == n) {
             if (threadIdx.x == 0) {
                 a.tchanged[(k & 1) * a.tiles + t] = 0;
-                spread_max(a.spread + (k % 3) * kSpreadWords, a.texcess[t]);
+                for (int i = 0; i < n; i++)
+                    spread_max(sets + (size_t)i * kSpreadWords, a.texcess[(size_t)t * kRelaxKMax + i]);
             }
             return;
         }
     }
     bool changed = false;
-    const R tmax = relax_tile_iteration<R>(a.h[src], a.h[src ^ 1], a.ny, a.nx, a.crit, r0, c0, tile, &changed);
-    double m[1] = {(double)tmax};
-    block_reduce<double, 1, 1, kRelaxBlock / kWave>(m, smem);
+    R tm[G::K];
+#pragma unroll
+    for (int i = 0; i < G::K; i++) tm[i] = (R)0;
+    relax_tile_iterations<R, G>(a.h[src], a.h[src ^ 1], a.ny, a.nx, a.crit, r0, c0, tile, n, tm, &changed);
+    double m[G::K];
+#pragma unroll
+    for (int i = 0; i < G::K; i++) m[i] = (double)tm[i];
+    block_reduce<double, G::K, 1, G::BLOCK / kWave>(m, smem);
     const int any = __syncthreads_or(changed ? 1 : 0);
     if (threadIdx.x == 0) {
-        spread_max(a.spread + (k % 3) * kSpreadWords, m[0]);
-        a.texcess[t] = m[0];
+        for (int i = 0; i < n; i++) {
+            spread_max(sets + (size_t)i * kSpreadWords, m[i]);
+            a.texcess[(size_t)t * kRelaxKMax + i] = m[i];
+        }
+        a.titers[t] = n;
         a.tchanged[(k & 1) * a.tiles + t] = any;
     }
 }
 
-// after launch k - 1: copy back when the field sits in scratch, write the result
+// after launch k - 1: once the loop has ended, copy back when the field sits
+// in scratch and write the result.  A loop still running (or with a replay
+// pending) leaves both buffers alone: the next launches continue from them.
 template <class R>
 __global__ void __launch_bounds__(kBlock) relax_end_kernel(RelaxArgs<R> a, int k)
 {
     __shared__ double smem[1];
     const RelaxControl C = relax_resolve(a, k, smem);
-    if (C.src == 1) {
+    if (C.done && C.src == 1) {
         const int64_t n = (int64_t)a.ny * a.nx;
         for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
              i += (int64_t)gridDim.x * blockDim.x)
@@ -153,7 +217,8 @@ __global__ void zero_relax_sync(RelaxSync *s, unsigned long long *spread)
 {
     unsigned *p = reinterpret_cast<unsigned *>(s);
     for (int i = threadIdx.x; i < (int)(sizeof(RelaxSync) / 4); i += blockDim.x) p[i] = 0u;
-    for (int i = threadIdx.x; i < 3 * kSpread; i += blockDim.x) spread[(int64_t)i * kSpreadStride] = 0ull;
+    for (int i = threadIdx.x; i < kRelaxGroups * kRelaxKMax * kSpread; i += blockDim.x)
+        spread[(int64_t)i * kSpreadStride] = 0ull;
 }
 
 template <class R>
@@ -172,41 +237,57 @@ static int launch_relax(mv2_context *ctx, void *height, int ny, int nx, const do
     a.tolerance = tolerance;
     // the relaxation keeps its own barrier block after the V2 sync block
     a.sync = (RelaxSync *)((char *)ctx->sync.ptr + 1024);
+    static_assert(sizeof(RelaxSync) <= 1024, "relaxation loop state");
     if (int rc = ensure_scratch(ctx, ctx->spread, kSpreadScratchBytes)) return rc;
     a.spread = (unsigned long long *)ctx->spread.ptr + (size_t)kRelaxSpreadSet * kSpreadWords;
     a.out = (mv2_relax_result *)((char *)ctx->result.ptr + 2048);
-    a.tiles_x = (nx + kRelaxTW - 1) / kRelaxTW;
-    a.tiles = a.tiles_x * ((ny + kRelaxTH - 1) / kRelaxTH);
-    if (int rc = ensure_scratch(ctx, ctx->relax_quiet, (2 * sizeof(int32_t) + sizeof(double)) * (size_t)a.tiles))
-        return rc;
+    a.tiles_x = (nx + kRelaxTileW - 1) / kRelaxTileW;
+    a.tiles = a.tiles_x * ((ny + kRelaxTileH - 1) / kRelaxTileH);
+    const size_t quiet = (sizeof(double) * kRelaxKMax + 3 * sizeof(int32_t)) * (size_t)a.tiles;
+    if (int rc = ensure_scratch(ctx, ctx->relax_quiet, quiet)) return rc;
     a.texcess = (double *)ctx->relax_quiet.ptr;
-    a.tchanged = (int32_t *)(a.texcess + a.tiles);
+    a.tchanged = (int32_t *)(a.texcess + (size_t)kRelaxKMax * a.tiles);
+    a.titers = a.tchanged + 2 * (size_t)a.tiles;
     hipStream_t st = ctx->stream;
     const int gend = grid_for((int64_t)n, kBlock, 4 * ctx->cu_count);
     hipLaunchKernelGGL(zero_relax_sync, dim3(1), dim3(kBlock), 0, st, a.sync, a.spread);
     MV2_HIP(ctx, hipGetLastError());
-    // One launch per iteration; launches after convergence return at once.
-    // Enqueue about as many as the previous call needed (at most 32), then
-    // grow the batch geometrically while the field has not converged.
+    // Launches after the loop's end return at once.  Enqueue about as many as
+    // the previous call needed (at most 32), then grow the batch while the
+    // field has not converged.  Launch 0 applies one iteration; later FP32
+    // launches two (FP64 one: its two-iteration tile would spill registers).
     int chunk = ctx->relax_predicted + 1;
     if (chunk < 2) chunk = 2;
     if (chunk > 32) chunk = 32;
     int k = 0;
     const mv2_relax_result *hres = (const mv2_relax_result *)ctx->host_result.ptr;
     for (;;) {
-        if (chunk > max_iterations - k) chunk = max_iterations - k;
-        for (int j = 0; j < chunk; j++, k++)
-            hipLaunchKernelGGL(relax_kernel<R>, dim3(a.tiles), dim3(kRelaxBlock), 0, st, a, k);
+        for (int j = 0; j < chunk; j++, k++) {
+            if constexpr (sizeof(R) == 8) {
+                hipLaunchKernelGGL((relax_kernel<R, RelaxSingle64>), dim3(a.tiles), dim3(RelaxSingle64::BLOCK), 0, st, a, k);
+            } else {
+                if (k == 0)
+                    hipLaunchKernelGGL((relax_kernel<R, RelaxFirst>), dim3(a.tiles), dim3(RelaxFirst::BLOCK), 0, st, a, k);
+                else
+                    hipLaunchKernelGGL((relax_kernel<R, RelaxPair>), dim3(a.tiles), dim3(RelaxPair::BLOCK), 0, st, a, k);
+            }
+        }
         hipLaunchKernelGGL(relax_end_kernel<R>, dim3(gend), dim3(kBlock), 0, st, a, k);
         MV2_HIP(ctx, hipGetLastError());
         MV2_HIP(ctx, hipMemcpyAsync(ctx->host_result.ptr, a.out, sizeof(mv2_relax_result), hipMemcpyDeviceToHost, st));
         if (int rc_w = stream_wait(ctx, st)) return rc_w;
-        if (hres->complete || k >= max_iterations) break;
+        if (hres->complete) break;
+        // each launch applies at least one iteration: max_iterations + 1
+        // launches always end the loop (the last may be a replay)
+        if (k > max_iterations + 1) return set_error(ctx, MV2_EDEVICE, "[MiniSlope] relaxation loop did not end");
         chunk = k / 2 > 8 ? k / 2 : 8;
     }
     memcpy(out, hres, sizeof(mv2_relax_result));
     out->launches = k;
-    ctx->relax_predicted = out->iterations;
+    // launches that applied iterations: one, then pairs (FP32); the next call
+    // enqueues one more, which covers a replay
+    const int it = out->iterations;
+    ctx->relax_predicted = sizeof(R) == 8 ? it : (it <= 1 ? 1 : 1 + it / 2);
     return MV2_OK;
 }
 

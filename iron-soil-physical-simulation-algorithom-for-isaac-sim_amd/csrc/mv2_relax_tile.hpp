@@ -182,4 +182,186 @@ __device__ inline R relax_tile_iteration(const R *hs, R *hd, int ny, int nx, con
     return tmax;
 }
 
+// ---- several iterations per launch ------------------------------------------
+// The single-field relaxation runs up to K iterations on an LDS tile before
+// writing it back.  A sub-phase moves information one cell along its
+// direction, so K iterations leave at most 6K halo layers stale; with a halo
+// of 6K + 2 the interior and every edge it owns stay exact for all K
+// iterations.  The tile has a NaN border (one ring before, two after the
+// computed region) and cells outside the grid are staged as NaN: an edge with
+// a NaN end has excess max(|NaN| - crit, 0) = +0 and moves nothing (the valid
+// end is written back unchanged), exactly as a skipped edge, so the loops
+// carry no bounds tests.
+//
+// Each thread takes kSlots active edges of a sub-phase and also loads the
+// cell after the edge's second cell: the edge from that second cell onwards
+// has the opposite parity, so one set of loads gives the maximum excess over
+// the owned edges of both parities in the sub-phase's input state.
+//   X lanes (d = 0, 2, 3; parity on the column): anchor (lr, 2q + p)
+//   Y lanes (d = 1; parity on the row):          anchor (2 yr + p, lc)
+// COLS: consecutive lanes of the X mapping walk down a column pair (LDS
+// stride PW, odd: no bank conflicts) rather than along a row (stride 2).
+template <int TH_, int TW_, int BLOCK_, int K_, bool COLS_>
+struct RelaxGeom {
+    static constexpr int TH = TH_, TW = TW_, BLOCK = BLOCK_, K = K_;
+    static constexpr bool COLS = COLS_;
+    static constexpr int H = 6 * K + 2;
+    static constexpr int LH = TH + 2 * H, LW = TW + 2 * H;   // computed region
+    static constexpr int PW = LW + 3, PH = LH + 3;           // NaN border: 1 before, 2 after
+    static constexpr int kCells = PH * PW;
+    static constexpr int kStage = (kCells + BLOCK - 1) / BLOCK;
+    static constexpr int kEdges = LH * LW / 2;               // active edges of one sub-phase
+    static constexpr int kSlots = (kEdges + BLOCK - 1) / BLOCK;
+    static constexpr int kOwned = (TH * TW + BLOCK - 1) / BLOCK;
+    static_assert(TH % 2 == 0 && TW % 2 == 0 && H % 2 == 0, "even tile origins keep local and global parities equal");
+    static_assert(BLOCK % kWave == 0 && BLOCK <= 1024, "whole waves");
+};
+
+template <class R> __device__ inline R relax_fabs(R x);
+template <> __device__ inline float relax_fabs<float>(float x) { return fabsf(x); }
+template <> __device__ inline double relax_fabs<double>(double x) { return fabs(x); }
+// maximum of two non-NaN values, or the non-NaN one
+template <class R> __device__ inline R relax_fmax(R x, R y);
+template <> __device__ inline float relax_fmax<float>(float x, float y) { return fmaxf(x, y); }
+template <> __device__ inline double relax_fmax<double>(double x, double y) { return fmax(x, y); }
+template <class R> __device__ inline R relax_nan();
+template <> __device__ inline float relax_nan<float>() { return __builtin_nanf(""); }
+template <> __device__ inline double relax_nan<double>() { return __builtin_nan(""); }
+
+// n (1 <= n <= G::K) iterations on the tile with interior origin (r0, c0) of
+// the field hs (ny x nx), written to hd; tmaxk[i] = this thread's maximum
+// excess over the owned edges in iteration i, taken before each sub-phase.
+// Per edge the arithmetic is relax_tile_iteration's: diff = a - b, excess =
+// max(|diff| - crit, 0), transfer (excess / 2) sgn(diff) -- written as
+// diff < 0 ? -half : half, equal to it for every diff (half is +0 whenever
+// diff is 0 or NaN).  All threads of the block must call it.  With `changed`,
+// the thread reports whether an interior cell it writes differs from its
+// staged value.
+template <class R, class G>
+__device__ inline void relax_tile_iterations(const R *hs, R *hd, int ny, int nx, const R (&crits)[4], int r0,
+                                             int c0, R *tile, int n, R (&tmaxk)[G::K], bool *changed = nullptr)
+{
+    constexpr int H = G::H, LH = G::LH, LW = G::LW, PW = G::PW, TH = G::TH, TW = G::TW, BLOCK = G::BLOCK;
+    const int R0 = r0 - H - 1, C0 = c0 - H - 1;         // global cell of padded (0, 0)
+    {
+        R v[G::kStage];
+#pragma unroll
+        for (int j = 0; j < G::kStage; j++) {
+            const int e = threadIdx.x + j * BLOCK;
+            const int pr = e / PW, pc = e - pr * PW;
+            const int gr = R0 + pr, gc = C0 + pc;
+            const bool inner = pr >= 1 && pr <= LH && pc >= 1 && pc <= LW;
+            v[j] = relax_nan<R>();
+            if (e < G::kCells && inner && gr >= 0 && gr < ny && gc >= 0 && gc < nx) v[j] = hs[gr * nx + gc];
+        }
+#pragma unroll
+        for (int j = 0; j < G::kStage; j++) {
+            const int e = threadIdx.x + j * BLOCK;
+            if (e < G::kCells) tile[e] = v[j];
+        }
+    }
+    // per-slot anchors and ownership (anchor in the interior) of the active
+    // edge and of the opposite-parity edge continuing from its second cell
+    int bx[G::kSlots], by[G::kSlots];
+    bool valid[G::kSlots];
+    bool xa[G::kSlots][2], x0[G::kSlots][2], x2[G::kSlots][2], x3[G::kSlots][2];
+    bool ya[G::kSlots][2], yi[G::kSlots][2];
+#pragma unroll
+    for (int j = 0; j < G::kSlots; j++) {
+        const int e0 = threadIdx.x + j * BLOCK;
+        valid[j] = G::kEdges % BLOCK == 0 || e0 < G::kEdges;
+        const int e = valid[j] ? e0 : 0;
+        int lr, q;
+        if constexpr (G::COLS) {
+            q = e / LH;
+            lr = e - q * LH;
+        } else {
+            lr = e / (LW / 2);
+            q = e - lr * (LW / 2);
+        }
+        // an unused slot reads inside the tile and stores nothing
+        bx[j] = valid[j] ? (lr + 1) * PW + 2 * q + 1 : PW + 1;
+        const int yr = e / LW, lc = e - yr * LW;
+        by[j] = valid[j] ? (2 * yr + 1) * PW + lc + 1 : PW + 1;
+        auto in_r = [](int r) { return (unsigned)(r - H) < (unsigned)TH; };
+        auto in_c = [](int c) { return (unsigned)(c - H) < (unsigned)TW; };
+#pragma unroll
+        for (int p = 0; p < 2; p++) {
+            const bool ca = in_c(2 * q + p), ci = in_c(2 * q + p + 1);
+            xa[j][p] = valid[j] && in_r(lr) && ca;
+            x0[j][p] = valid[j] && in_r(lr) && ci;
+            x2[j][p] = valid[j] && in_r(lr + 1) && ci;
+            x3[j][p] = valid[j] && in_r(lr - 1) && ci;
+            ya[j][p] = valid[j] && in_c(lc) && in_r(2 * yr + p);
+            yi[j][p] = valid[j] && in_c(lc) && in_r(2 * yr + p + 1);
+        }
+    }
+    __syncthreads();
+    R orig[G::kOwned];
+    if (changed) {
+#pragma unroll
+        for (int j = 0; j < G::kOwned; j++) {
+            const int e = threadIdx.x + j * BLOCK;
+            const int ir = e / TW, ic = e - ir * TW;
+            orig[j] = e < TH * TW ? tile[(ir + H + 1) * PW + ic + H + 1] : (R)0;
+        }
+    }
+#pragma unroll 1
+    for (int it = 0; it < n; it++) {
+        R tmax = (R)0;
+#pragma unroll
+        for (int d = 0; d < 4; d++) {
+            const R crit = crits[d];
+            // a, b and the cell after b along the direction, from the anchor
+            const int oa = d == 3 ? PW : 0;
+            const int ob = d == 0 ? 1 : d == 1 ? PW : d == 2 ? PW + 1 : 1;
+            const int on = d == 0 ? 2 : d == 1 ? 2 * PW : d == 2 ? 2 * PW + 2 : 2 - PW;
+#pragma unroll
+            for (int p = 0; p < 2; p++) {
+                R va[G::kSlots], vb[G::kSlots], vn[G::kSlots];
+#pragma unroll
+                for (int j = 0; j < G::kSlots; j++) {
+                    const int base = d == 1 ? by[j] + p * PW : bx[j] + p;
+                    va[j] = tile[base + oa];
+                    vb[j] = tile[base + ob];
+                    vn[j] = tile[base + on];
+                }
+                __syncthreads();      // every input of the sub-phase read before any transfer
+#pragma unroll
+                for (int j = 0; j < G::kSlots; j++) {
+                    const bool own_a = d == 1 ? ya[j][p] : xa[j][p];
+                    const bool own_i = d == 0 ? x0[j][p] : d == 1 ? yi[j][p] : d == 2 ? x2[j][p] : x3[j][p];
+                    const R diff = va[j] - vb[j];
+                    const R ex = relax_fmax<R>(relax_fabs<R>(diff) - crit, (R)0);
+                    const R exi = relax_fmax<R>(relax_fabs<R>(vb[j] - vn[j]) - crit, (R)0);
+                    tmax = relax_fmax<R>(tmax, own_a ? ex : (R)0);
+                    tmax = relax_fmax<R>(tmax, own_i ? exi : (R)0);
+                    const R half = (R)0.5 * ex;
+                    const R tr = diff < (R)0 ? -half : half;
+                    if (valid[j]) {
+                        const int base = d == 1 ? by[j] + p * PW : bx[j] + p;
+                        tile[base + oa] = va[j] - tr;
+                        tile[base + ob] = vb[j] + tr;
+                    }
+                }
+                __syncthreads();
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < G::K; k++)
+            if (k == it) tmaxk[k] = tmax;
+    }
+#pragma unroll
+    for (int j = 0; j < G::kOwned; j++) {
+        const int e = threadIdx.x + j * BLOCK;
+        const int ir = e / TW, ic = e - ir * TW;
+        const int r = r0 + ir, c = c0 + ic;
+        if (e < TH * TW && r < ny && c < nx) {
+            const R v = tile[(ir + H + 1) * PW + ic + H + 1];
+            hd[r * nx + c] = v;
+            if (changed && v != orig[j]) *changed = true;
+        }
+    }
+}
+
 }  // namespace mv2

@@ -36,6 +36,53 @@ def test_relax_bit_exact(shape, angle, seed, precision):
     np.testing.assert_array_equal(t.cpu().numpy(), o)
 
 
+@pytest.mark.parametrize("precision", ["fp32", "fp64"])
+def test_relax_converges_at_either_iteration_parity(precision):
+    """A launch applies one iteration, then two (FP32).  Convergence on the
+    first of two re-applies that one iteration to the launch's input; on the
+    second it ends there.  Both must return the oracle's field, iteration
+    count and last maximum excess exactly.  The seeds converge after 6, 7, 8
+    and 5 iterations (checked against the oracle below)."""
+    import torch
+    from isaac_bulk_hip import relax_device
+
+    dtype = torch.float64 if precision == "fp64" else torch.float32
+    npdt = np.float64 if precision == "fp64" else np.float32
+    crit = float(np.tan(np.deg2rad(34.0)))
+    counts = set()
+    for seed in (0, 2, 5, 6, 0):                 # the repeat follows a call that needed more launches
+        h = np.random.default_rng(100 + seed).uniform(0.0, 0.3, size=(48, 56)).astype(npdt)
+        t = _device(h, dtype)
+        it, conv, last = relax_device(t, 0.05, 0.05, 34.0, 5000, 1e-3)
+        o, oit, oconv, olast = oracle.relax_yx(h, 0.05, 0.05, crit, 5000, 1e-3, dtype=npdt)
+        assert (it, conv, last) == (oit, oconv, olast), seed
+        np.testing.assert_array_equal(t.cpu().numpy(), o)
+        counts.add(it)
+    assert counts == {5, 6, 7, 8}
+    # a second call on the relaxed field converges in its first iteration
+    it, conv, _ = relax_device(t, 0.05, 0.05, 34.0, 5000, 1e-3)
+    assert (it, conv) == (1, True)
+
+
+@pytest.mark.parametrize("precision", ["fp32", "fp64"])
+def test_relax_iteration_limits(precision):
+    """max_iterations cuts the loop inside a two-iteration launch or at its
+    end: the field after exactly that many iterations, not converged."""
+    import torch
+    from isaac_bulk_hip import relax_device
+
+    dtype = torch.float64 if precision == "fp64" else torch.float32
+    npdt = np.float64 if precision == "fp64" else np.float32
+    crit = float(np.tan(np.deg2rad(30.0)))
+    h = np.random.default_rng(11).uniform(0.0, 1.5, size=(61, 90)).astype(npdt)
+    for limit in (1, 2, 3, 4, 5, 8):
+        t = _device(h, dtype)
+        it, conv, last = relax_device(t, 0.05, 0.05, 30.0, limit, 1e-12)
+        o, oit, oconv, olast = oracle.relax_yx(h, 0.05, 0.05, crit, limit, 1e-12, dtype=npdt)
+        assert (it, conv, last) == (oit, oconv, olast) == (limit, False, olast), limit
+        np.testing.assert_array_equal(t.cpu().numpy(), o)
+
+
 def test_relax_toe_701_iteration_count_and_field():
     import torch
     from isaac_bulk_hip import relax_device
