@@ -79,7 +79,9 @@ struct mv2_context {
     const void *ext_x = nullptr;          // per-cell external acceleration (state precision), or null
     const void *ext_y = nullptr;
     double ext_max = 0.0;                 // max |external| over the cells (static exit test)
-    int relax_predicted = 0;              // relaxation launches to enqueue up front
+    int relax_predicted = 0;              // relaxation launches that worked in the last call
+    int relax_clean = 0;                  // the relaxation's loop state and next first group are zero
+    unsigned relax_calls = 0;             // completed relaxation calls (parity picks the first group)
     int frontier_predicted = 0;           // frontier rounds to enqueue up front
     mv2_batch_state *batch = nullptr;     // batched terrains bound to this context
     double *noalloc_scratch = nullptr;    // device address of the allocation-free ledger's partials
@@ -390,12 +392,12 @@ constexpr int kSpread = 64;
 constexpr int kSpreadStride = 16;                    // u64 words between slots
 constexpr int kSpreadWords = kSpread * kSpreadStride;
 // the step uses sets 0..4 (even frames) and 16..20 (odd frames), the envelope
-// 5..7, the labelling 12..14, the relaxation 21..26
+// 5..7, the labelling 12..14, the relaxation 21..30
 constexpr int kEnvSpreadSet = 5;
 constexpr int kLabelSpreadSet = 12;
 constexpr int kStepOddSpreadSet = 16;
-constexpr int kRelaxSpreadSet = 21;                  // six sets: three launch groups of two iterations
-constexpr int kRelaxSpreadSets = 6;
+constexpr int kRelaxSpreadSet = 21;                  // five launch groups of two iterations
+constexpr int kRelaxSpreadSets = 10;
 constexpr size_t kSpreadScratchBytes = (size_t)(kRelaxSpreadSet + kRelaxSpreadSets) * kSpreadWords * 8;
 
 __device__ inline void spread_max(unsigned long long *slots, double v)

@@ -25,13 +25,17 @@
 
 namespace mv2 {
 
-// Iteration state: launch k reads ctl[k & 1] and its workgroup 0 writes
-// ctl[(k + 1) & 1].  Launch k reduces the maxima of its iterations into spread
-// group k % 3 (set 2 g + i for its iteration i), tests group (k - 1) % 3 (the
-// previous launch) against the tolerance and clears group (k + 1) % 3.
+// Iteration state: launch k reads ctl[k & 1] (launch 0 starts from a fresh
+// state and reads nothing) and its workgroup 0 writes ctl[(k + 1) & 1].
+// Launch k reduces the maxima of its iterations into spread group g(k) (set
+// 2 g + i for its iteration i), tests group g(k - 1) (the previous launch)
+// against the tolerance and clears group g(k + 1).  g(k) = k % 3 for k > 0;
+// launch 0 uses group 3 or 4 by the parity of the call, and every end pass of
+// a call clears the other one for the next call, so a call needs no reset
+// launch (only the first call, or one after an error, zeroes everything).
 constexpr int kRelaxKMax = 2;                 // iterations per launch at most
-constexpr int kRelaxGroups = 3;
-static_assert(kRelaxGroups * kRelaxKMax <= kRelaxSpreadSets, "the relaxation's spread sets");
+constexpr int kRelaxGroups = 3;               // rotating groups of launches k > 0
+static_assert((kRelaxGroups + 2) * kRelaxKMax <= kRelaxSpreadSets, "the relaxation's spread sets");
 using RelaxFirst = RelaxGeom<44, 48, 640, 1, false>;      // FP32: a call's first launch
 using RelaxPair = RelaxGeom<44, 48, 1024, 2, true>;       // FP32: later launches
 using RelaxSingle64 = RelaxGeom<44, 48, 960, 1, true>;     // FP64: every launch (spill-free at 960)
@@ -62,8 +66,9 @@ struct RelaxArgs {
     int max_iterations;
     double tolerance;
     RelaxSync *sync;
-    unsigned long long *spread;   // kRelaxGroups x kRelaxKMax spread-maximum sets
-    mv2_relax_result *out;
+    unsigned long long *spread;   // (kRelaxGroups + 2) x kRelaxKMax spread-maximum sets
+    int first_group;              // group of launch 0 (3 or 4); the end pass clears the other
+    mv2_relax_result *out;        // pinned host memory: the end pass writes the result there
     int tiles_x, tiles;
     int32_t *tchanged;            // [2][tiles]: the tile's interior changed in launch k (parity k & 1)
     int32_t *titers;              // [tiles]: iterations of the tile's last computed launch
@@ -75,14 +80,25 @@ __device__ inline unsigned long long *relax_set(unsigned long long *spread, int 
     return spread + (size_t)(group * kRelaxKMax + i) * kSpreadWords;
 }
 
+template <class R>
+__device__ inline int relax_group(const RelaxArgs<R> &a, int k)
+{
+    return k == 0 ? a.first_group : k % kRelaxGroups;
+}
+
 // loop state for launch k: test the previous launch's iterations in order.
 // Every thread of the block calls it.
 template <class R>
 __device__ inline RelaxControl relax_resolve(const RelaxArgs<R> &a, int k, double *smem1)
 {
-    RelaxControl C = a.sync->ctl[k & 1];      // written by an earlier launch: plain load
+    RelaxControl C;
+    if (k == 0) {
+        C = RelaxControl{};
+        return C;
+    }
+    C = a.sync->ctl[k & 1];      // written by an earlier launch: plain load
     if (!C.done && C.replay == 0 && C.n_prev > 0) {
-        const int g = (k - 1) % kRelaxGroups;
+        const int g = relax_group(a, k - 1);
         int first = -1;
         double mfirst = 0.0, mlast = 0.0;
         for (int i = 0; i < C.n_prev; i++) {
@@ -128,7 +144,7 @@ __global__ void __launch_bounds__(G::BLOCK) relax_kernel(RelaxArgs<R> a, int k)
     const int n = C.replay > 0 ? C.replay : (G::K < left ? G::K : left);
     const int src = C.src;
     if (blockIdx.x == 0)
-        for (int i = 0; i < kRelaxKMax; i++) spread_clear(relax_set(a.spread, (k + 1) % kRelaxGroups, i));
+        for (int i = 0; i < kRelaxKMax; i++) spread_clear(relax_set(a.spread, relax_group(a, k + 1), i));
     if (lead) {
         RelaxControl nc = C;
         nc.prev_src = src;
@@ -147,7 +163,7 @@ __global__ void __launch_bounds__(G::BLOCK) relax_kernel(RelaxArgs<R> a, int k)
     const int t = blockIdx.x;
     const int trow = t / a.tiles_x, tcol = t % a.tiles_x;
     const int r0 = trow * G::TH, c0 = tcol * G::TW;
-    unsigned long long *sets = a.spread + (size_t)(k % kRelaxGroups) * kRelaxKMax * kSpreadWords;
+    unsigned long long *sets = relax_set(a.spread, relax_group(a, k), 0);
     // Quiet tiles: the iterations of a tile read its interior and a halo of
     // at most one tile, i.e. its 3 x 3 tile neighbourhood.  When none of
     // those tiles changed in launch k - 1, the tile's input equals the input
@@ -192,8 +208,10 @@ __global__ void __launch_bounds__(G::BLOCK) relax_kernel(RelaxArgs<R> a, int k)
 }
 
 // after launch k - 1: once the loop has ended, copy back when the field sits
-// in scratch and write the result.  A loop still running (or with a replay
-// pending) leaves both buffers alone: the next launches continue from them.
+// in scratch.  A loop still running (or with a replay pending) leaves both
+// buffers alone: the next launches continue from them.  Writes the result to
+// pinned host memory and clears the next call's first group (read by no
+// launch of this call).
 template <class R>
 __global__ void __launch_bounds__(kBlock) relax_end_kernel(RelaxArgs<R> a, int k)
 {
@@ -205,11 +223,16 @@ __global__ void __launch_bounds__(kBlock) relax_end_kernel(RelaxArgs<R> a, int k
              i += (int64_t)gridDim.x * blockDim.x)
             a.h[0][i] = a.h[1][i];
     }
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
-        a.out->iterations = C.iterations;
-        a.out->converged = C.converged;
-        a.out->max_excess_last = C.last;
-        a.out->complete = C.done;
+    if (blockIdx.x == 0) {
+        for (int i = 0; i < kRelaxKMax; i++) spread_clear(relax_set(a.spread, a.first_group == 3 ? 4 : 3, i));
+        if (threadIdx.x == 0) {
+            mv2_relax_result r{};
+            r.iterations = C.iterations;
+            r.converged = C.converged;
+            r.max_excess_last = C.last;
+            r.complete = C.done;
+            *a.out = r;
+        }
     }
 }
 
@@ -217,7 +240,7 @@ __global__ void zero_relax_sync(RelaxSync *s, unsigned long long *spread)
 {
     unsigned *p = reinterpret_cast<unsigned *>(s);
     for (int i = threadIdx.x; i < (int)(sizeof(RelaxSync) / 4); i += blockDim.x) p[i] = 0u;
-    for (int i = threadIdx.x; i < kRelaxGroups * kRelaxKMax * kSpread; i += blockDim.x)
+    for (int i = threadIdx.x; i < (kRelaxGroups + 2) * kRelaxKMax * kSpread; i += blockDim.x)
         spread[(int64_t)i * kSpreadStride] = 0ull;
 }
 
@@ -240,7 +263,9 @@ static int launch_relax(mv2_context *ctx, void *height, int ny, int nx, const do
     static_assert(sizeof(RelaxSync) <= 1024, "relaxation loop state");
     if (int rc = ensure_scratch(ctx, ctx->spread, kSpreadScratchBytes)) return rc;
     a.spread = (unsigned long long *)ctx->spread.ptr + (size_t)kRelaxSpreadSet * kSpreadWords;
-    a.out = (mv2_relax_result *)((char *)ctx->result.ptr + 2048);
+    a.out = (mv2_relax_result *)((char *)ctx->host_result.ptr + 2048);
+    static_assert(2048 + sizeof(mv2_relax_result) <= 4096, "the relaxation result in the pinned block");
+    a.first_group = 3 + (ctx->relax_calls & 1);
     a.tiles_x = (nx + kRelaxTileW - 1) / kRelaxTileW;
     a.tiles = a.tiles_x * ((ny + kRelaxTileH - 1) / kRelaxTileH);
     const size_t quiet = (sizeof(double) * kRelaxKMax + 3 * sizeof(int32_t)) * (size_t)a.tiles;
@@ -250,17 +275,20 @@ static int launch_relax(mv2_context *ctx, void *height, int ny, int nx, const do
     a.titers = a.tchanged + 2 * (size_t)a.tiles;
     hipStream_t st = ctx->stream;
     const int gend = grid_for((int64_t)n, kBlock, 4 * ctx->cu_count);
-    hipLaunchKernelGGL(zero_relax_sync, dim3(1), dim3(kBlock), 0, st, a.sync, a.spread);
+    if (!ctx->relax_clean) hipLaunchKernelGGL(zero_relax_sync, dim3(1), dim3(kBlock), 0, st, a.sync, a.spread);
+    ctx->relax_clean = 0;          // set again once this call has completed
     MV2_HIP(ctx, hipGetLastError());
     // Launches after the loop's end return at once.  Enqueue about as many as
     // the previous call needed (at most 32), then grow the batch while the
     // field has not converged.  Launch 0 applies one iteration; later FP32
     // launches two (FP64 one: its two-iteration tile would spill registers).
-    int chunk = ctx->relax_predicted + 1;
-    if (chunk < 2) chunk = 2;
+    // A field that converged in its first iteration last time gets one launch
+    // (the end pass detects the convergence); otherwise one more than the
+    // launches that worked, which covers a replay.
+    int chunk = ctx->relax_predicted <= 1 ? 1 : ctx->relax_predicted + 1;
     if (chunk > 32) chunk = 32;
     int k = 0;
-    const mv2_relax_result *hres = (const mv2_relax_result *)ctx->host_result.ptr;
+    const volatile mv2_relax_result *hres = a.out;
     for (;;) {
         for (int j = 0; j < chunk; j++, k++) {
             if constexpr (sizeof(R) == 8) {
@@ -274,7 +302,6 @@ static int launch_relax(mv2_context *ctx, void *height, int ny, int nx, const do
         }
         hipLaunchKernelGGL(relax_end_kernel<R>, dim3(gend), dim3(kBlock), 0, st, a, k);
         MV2_HIP(ctx, hipGetLastError());
-        MV2_HIP(ctx, hipMemcpyAsync(ctx->host_result.ptr, a.out, sizeof(mv2_relax_result), hipMemcpyDeviceToHost, st));
         if (int rc_w = stream_wait(ctx, st)) return rc_w;
         if (hres->complete) break;
         // each launch applies at least one iteration: max_iterations + 1
@@ -282,8 +309,14 @@ static int launch_relax(mv2_context *ctx, void *height, int ny, int nx, const do
         if (k > max_iterations + 1) return set_error(ctx, MV2_EDEVICE, "[MiniSlope] relaxation loop did not end");
         chunk = k / 2 > 8 ? k / 2 : 8;
     }
-    memcpy(out, hres, sizeof(mv2_relax_result));
+    *out = mv2_relax_result{};
+    out->iterations = hres->iterations;
+    out->converged = hres->converged;
+    out->max_excess_last = hres->max_excess_last;
+    out->complete = hres->complete;
     out->launches = k;
+    ctx->relax_clean = 1;
+    ctx->relax_calls++;
     // launches that applied iterations: one, then pairs (FP32); the next call
     // enqueues one more, which covers a replay
     const int it = out->iterations;
