@@ -10,6 +10,11 @@
 // component's final root is its minimum linear index -- the same canonical
 // label as the reference (root = min index, tie -> smallest root).  One
 // flatten pass then writes every cell's root and counts cells per root.
+// The same pass yields the component statistics: region count (cells that
+// are their own root), unstable cells, and the largest component -- a
+// tile's atomicAdd onto a root's count returns the count before it, so the
+// tile that adds last for a root sees that root's final count, and the
+// maximum over every (count seen, root) key is the final maximum.
 //
 // Floating-point sums over the winning component use per-workgroup partials
 // combined in fixed order (the reference uses FP64 atomics).
@@ -21,7 +26,7 @@
 
 namespace mv2 {
 
-constexpr int kLabelParts = 512;   // workgroups of the statistics kernels
+constexpr int kLabelParts = 2048;  // workgroups of the statistics kernels (one cell per thread at 701^2)
 constexpr int kWinF = 7;           // winner float statistics
 
 struct LabelAux {
@@ -216,8 +221,19 @@ __global__ void __launch_bounds__(kBlock) ccl_border_kernel(int32_t *p, int ny, 
     }
 }
 
-// final labels (full path to the root) and cell counts per root
-__global__ void __launch_bounds__(kBlock) ccl_flatten_kernel(int32_t *p, int ny, int nx, int tiles_x, int32_t *counts)
+// final labels (full path to the root) and cell counts per root; with
+// `spread`, the component statistics of reduce_components + select_winner
+// (:234-252) into spread sets 0, 1 and 2: region count, unstable cells and the
+// key (count << 32) | (INT32_MAX - root), whose maximum is the largest
+// component with the smallest root (the reference's tie rule).  Integer sums
+// and maxima: the order of the atomics does not change them.
+__device__ inline long long winner_key(int32_t count, int32_t root)
+{
+    return ((long long)count << 32) | (long long)(0x7fffffff - root);
+}
+
+__global__ void __launch_bounds__(kBlock) ccl_flatten_kernel(int32_t *p, int ny, int nx, int tiles_x, int32_t *counts,
+                                                             unsigned long long *spread)
 {
     __shared__ int keys[kCclHash];
     __shared__ int vals[kCclHash];
@@ -254,10 +270,14 @@ __global__ void __launch_bounds__(kBlock) ccl_flatten_kernel(int32_t *p, int ny,
     }
     if (!counts) return;
     __syncthreads();
+    long long regions = 0, unstable = 0, key = 0;
 #pragma unroll
     for (int j = 0; j < kCclPer; j++) {
         const int r = root[j];
         if (r < 0) continue;
+        const int e = threadIdx.x + j * kBlock;
+        unstable += 1;
+        regions += r == (tr0 + e / kCcl) * nx + tc0 + e % kCcl ? 1 : 0;
         int h = (int)(((unsigned)r * 2654435761u) >> 25) & (kCclHash - 1);
         bool done = false;
         for (int probe = 0; probe < kCclHash && !done; probe++) {
@@ -268,10 +288,27 @@ __global__ void __launch_bounds__(kBlock) ccl_flatten_kernel(int32_t *p, int ny,
             }
             h = (h + 1) & (kCclHash - 1);
         }
-        if (!done) atomicAdd(&counts[r], 1);
+        if (!done) {
+            const int32_t before = atomicAdd(&counts[r], 1);
+            key = key >= winner_key(before + 1, r) ? key : winner_key(before + 1, r);
+        }
     }
     __syncthreads();
-    if (threadIdx.x < kCclHash && keys[threadIdx.x] >= 0) atomicAdd(&counts[keys[threadIdx.x]], vals[threadIdx.x]);
+    if (threadIdx.x < kCclHash && keys[threadIdx.x] >= 0) {
+        const int32_t r = keys[threadIdx.x], add = vals[threadIdx.x];
+        const int32_t before = atomicAdd(&counts[r], add);
+        key = key >= winner_key(before + add, r) ? key : winner_key(before + add, r);
+    }
+    if (!spread) return;
+    regions = wave_sum(regions);
+    unstable = wave_sum(unstable);
+    key = wave_max(key);
+    if ((threadIdx.x & (kWave - 1)) == 0) {
+        const int slot = (blockIdx.x % kSpread) * kSpreadStride;
+        if (regions) atomicAdd(&spread[slot], (unsigned long long)regions);
+        if (unstable) atomicAdd(&spread[kSpreadWords + slot], (unsigned long long)unstable);
+        if (key) atomicMax(&spread[2 * kSpreadWords + slot], (unsigned long long)key);
+    }
 }
 
 __global__ void fill32(int32_t *p, int64_t n, int32_t v)
@@ -279,35 +316,8 @@ __global__ void fill32(int32_t *p, int64_t n, int32_t v)
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) p[i] = v;
 }
 
-// reduce_components + select_winner (:234-252) in one pass over the per-root
-// counts: region count, unstable cells and the key (count << 32) |
-// (INT32_MAX - root), whose maximum is the largest component with the
-// smallest root (the reference's tie rule), into spread sets 0, 1 and 2.
-// Integer sums and maxima: the order of the atomics does not change them.
-__global__ void __launch_bounds__(kBlock) count_kernel(const int32_t *counts, int64_t n, unsigned long long *spread)
-{
-    __shared__ long long lds[3 * kBlock];
-    long long v[3] = {0, 0, 0};
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
-        const int32_t c = counts[i];
-        if (c > 0) {
-            v[0] += 1;
-            v[1] += c;
-            const long long key = ((long long)c << 32) | (long long)(0x7fffffff - (int32_t)i);
-            v[2] = v[2] >= key ? v[2] : key;
-        }
-    }
-    block_fold<long long, 3, 0x4u>(v, lds);
-    if (threadIdx.x == 0) {
-        const int slot = (blockIdx.x % kSpread) * kSpreadStride;
-        if (v[0]) atomicAdd(&spread[slot], (unsigned long long)v[0]);
-        if (v[1]) atomicAdd(&spread[kSpreadWords + slot], (unsigned long long)v[1]);
-        if (v[2]) atomicMax(&spread[2 * kSpreadWords + slot], (unsigned long long)v[2]);
-    }
-}
-
-// the counts pass's result: regions, unstable cells, largest count and the
-// winner root (n + 1 when there is no component); workgroup 0 records them
+// the flatten pass's statistics: regions, unstable cells, largest count and
+// the winner root (n + 1 when there is no component); workgroup 0 records them
 __device__ inline int32_t resolve_winner(const unsigned long long *spread, int64_t n, LabelAux *aux)
 {
     __shared__ long long win;
@@ -375,9 +385,13 @@ __global__ void __launch_bounds__(kBlock) winner_stats_kernel(
     }
 }
 
-__global__ void __launch_bounds__(kBlock) winner_final_kernel(const double *partials, int nparts, LabelAux *aux)
+constexpr int kFinalBlock = 1024;   // the fold of up to kLabelParts partials per field
+
+__global__ void __launch_bounds__(kFinalBlock) winner_final_kernel(const double *partials, int nparts, LabelAux *aux,
+                                                                    LabelAux *host)
 {
-    __shared__ double smem[kWinF * 4];
+    constexpr int kNW = kFinalBlock / kWave;
+    __shared__ double smem[kWinF * kNW];
     double s[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
     double m[2] = {0.0, 0.0};
     for (int p = threadIdx.x; p < nparts; p += blockDim.x) {
@@ -388,8 +402,8 @@ __global__ void __launch_bounds__(kBlock) winner_final_kernel(const double *part
         m[0] = fmax(m[0], pp[1]);
         m[1] = fmax(m[1], pp[3]);
     }
-    block_reduce<double, 5, 0>(s, smem);
-    block_reduce<double, 2, 1>(m, smem);
+    block_reduce<double, 5, 0, kNW>(s, smem);
+    block_reduce<double, 2, 1, kNW>(m, smem);
     if (threadIdx.x == 0) {
         aux->diag_f[0] = s[0];
         aux->diag_f[1] = m[0];
@@ -399,6 +413,11 @@ __global__ void __launch_bounds__(kBlock) winner_final_kernel(const double *part
         aux->diag_f[5] = s[3];
         aux->diag_f[6] = s[4];
         aux->diag_f[7] = 0.0;
+        // the statistics pass's counters (earlier launches) and these sums
+        LabelAux r;
+        for (int k = 0; k < 8; k++) r.diag_i[k] = aux->diag_i[k];
+        for (int k = 0; k < 8; k++) r.diag_f[k] = aux->diag_f[k];
+        *host = r;
     }
 }
 
@@ -572,14 +591,15 @@ static int check_ctx(mv2_context *ctx)
 }
 
 // the border unions and the flattening (after the tile-local pass)
-static int label_global(mv2_context *ctx, int32_t *labels, int ny, int nx, int connectivity, int32_t *counts)
+static int label_global(mv2_context *ctx, int32_t *labels, int ny, int nx, int connectivity, int32_t *counts,
+                        unsigned long long *spread)
 {
     const int tiles_x = (nx + kCcl - 1) / kCcl;
     const int tiles = tiles_x * ((ny + kCcl - 1) / kCcl);
     hipLaunchKernelGGL(ccl_border_kernel, dim3(grid_for((int64_t)tiles * 3 * kCcl)), dim3(kBlock), 0, ctx->stream,
                        labels, ny, nx, connectivity, tiles_x, tiles);
     hipLaunchKernelGGL(ccl_flatten_kernel, dim3(tiles), dim3(kBlock), 0, ctx->stream, labels, ny, nx, tiles_x,
-                       counts);
+                       counts, spread);
     MV2_HIP(ctx, hipGetLastError());
     return MV2_OK;
 }
@@ -593,7 +613,7 @@ static int label_core(mv2_context *ctx, int32_t *labels, int ny, int nx, int con
     hipLaunchKernelGGL(ccl_border_kernel, dim3(grid_for((int64_t)tiles * 3 * kCcl)), dim3(kBlock), 0, ctx->stream,
                        labels, ny, nx, connectivity, tiles_x, tiles);
     hipLaunchKernelGGL(ccl_flatten_kernel, dim3(tiles), dim3(kBlock), 0, ctx->stream, labels, ny, nx, tiles_x,
-                       counts);
+                       counts, (unsigned long long *)nullptr);
     MV2_HIP(ctx, hipGetLastError());
     return MV2_OK;
 }
@@ -642,18 +662,19 @@ extern "C" int mv2_avalanche_label(mv2_context *ctx, const mv2_avalanche_config 
     const int tiles = tiles_x * ((ny + kCcl - 1) / kCcl);
     hipLaunchKernelGGL(avalanche_local_kernel, dim3(tiles), dim3(kLocalBlock), 0, ctx->stream, init, tiles_x, aux,
                        (int32_t)(n + 1), spread);
-    if (int rc = label_global(ctx, f->labels, ny, nx, cfg->connectivity, f->counts)) return rc;
+    if (int rc = label_global(ctx, f->labels, ny, nx, cfg->connectivity, f->counts, spread)) return rc;
     const int sgrid = grid_for(n, kBlock, kLabelParts);
-    hipLaunchKernelGGL(count_kernel, dim3(sgrid), dim3(kBlock), 0, ctx->stream, f->counts, n, spread);
     hipLaunchKernelGGL(winner_stats_kernel, dim3(sgrid), dim3(kBlock), 0, ctx->stream, f->labels, ctx->z_base,
                        ctx->b_eff, f32, f->latch, f->previous, f->slope, f->severity, ny, nx, *cfg, area, f6, aux,
                        spread, partials);
-    hipLaunchKernelGGL(winner_final_kernel, dim3(1), dim3(kBlock), 0, ctx->stream, partials, sgrid, aux);
+    // the result goes straight to pinned host memory (no copy launch)
+    LabelAux *hres = (LabelAux *)((char *)ctx->host_result.ptr + 2048);
+    static_assert(2048 + sizeof(LabelAux) <= 4096, "the labelling result in the pinned block");
+    hipLaunchKernelGGL(winner_final_kernel, dim3(1), dim3(kFinalBlock), 0, ctx->stream, partials, sgrid, aux, hres);
     MV2_HIP(ctx, hipGetLastError());
-    MV2_HIP(ctx, hipMemcpyAsync(ctx->host_result.ptr, aux, sizeof(LabelAux), hipMemcpyDeviceToHost, ctx->stream));
     if (int rc_w = stream_wait(ctx, ctx->stream)) return rc_w;
     LabelAux host;
-    memcpy(&host, ctx->host_result.ptr, sizeof(LabelAux));
+    memcpy(&host, (const void *)hres, sizeof(LabelAux));
     memcpy(out->diag_i, host.diag_i, sizeof(host.diag_i));
     memcpy(out->diag_f, host.diag_f, sizeof(host.diag_f));
     out->union_passes = 1;
