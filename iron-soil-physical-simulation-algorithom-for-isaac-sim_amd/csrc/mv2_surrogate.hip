@@ -411,6 +411,116 @@ __global__ void __launch_bounds__(kBlock) weight_grad_kernel(const void *dz, int
     if (tap == 0 && (threadIdx.x % 8) == 0) bias_partials[(int64_t)blockIdx.x * 32 + co] = bacc;
 }
 
+// The 32 -> 32 channel layers (dZ and X both BF16 NHWC32) on
+// v_mfma_f32_32x32x16_bf16: per tap, dW_t (32 co x 32 ci) = A x B with
+// A = dZ^T (32 co x 16 pixels) and B = X_t (16 pixels x 32 ci), the pixels of
+// the chunk as the contraction.  Each wave stages 64-pixel slabs of dZ and of
+// X shifted by the tap in its own LDS rows (pixel-major, as stored: 16-byte
+// coalesced loads), then gathers its operands channel-wise: lane (m, half)
+// takes the 8 pixels half*8 .. half*8+7 of channel m.  D row i = co, column
+// j = ci: accumulator r of lane (j, half) is co = 8 (r / 4) + 4 half + r % 4.
+// The 4 waves' sums (and the bias sums of tap 0) combine in fixed order into
+// the same per-chunk partials as weight_grad_kernel.
+constexpr int kGradMfmaSlab = 64;                 // pixels per wave and slab
+constexpr int kGradRow = 40;                      // LDS row of a pixel: 32 channels + pad (bf16)
+
+__global__ void __launch_bounds__(kBlock) weight_grad_mfma_kernel(const __bf16 *dz, const __bf16 *xin, int B, int H,
+                                                                  int W, int dil, int64_t chunk, float *partials,
+                                                                  float *bias_partials)
+{
+    constexpr int kWaves = kBlock / kWave;
+    __shared__ __bf16 zs[kWaves][kGradMfmaSlab * kGradRow];
+    __shared__ __bf16 xs[kWaves][kGradMfmaSlab * kGradRow];
+    __shared__ float red[kWaves][kWave * 17];
+    const int tap = blockIdx.y;
+    const int dyo = (tap / 3 - 1) * dil, dxo = (tap % 3 - 1) * dil;
+    const int64_t plane = (int64_t)H * W;
+    const int64_t total = (int64_t)B * plane;
+    const int64_t p0 = (int64_t)blockIdx.x * chunk;
+    const int64_t p1 = p0 + chunk < total ? p0 + chunk : total;
+    const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
+    const int m = lane & 31, half = lane >> 5;
+    __bf16 *zw = zs[wave], *xw = xs[wave];
+    f32x16 acc;
+#pragma unroll
+    for (int r = 0; r < 16; r++) acc[r] = 0.0f;
+    float bacc = 0.0f;
+    const bf16x8 zero = {};
+    // lane l brings pixel s + l of a slab (dZ) and its tap-shifted input pixel
+    // (zeros outside the image or past the chunk)
+    bf16x8 zv[4], xv[4];
+    auto fetch = [&](int64_t s) {
+        const int64_t p = s + lane;
+#pragma unroll
+        for (int c = 0; c < 4; c++) zv[c] = xv[c] = zero;
+        if (p < p1) {
+            const bf16x8 *zp = reinterpret_cast<const bf16x8 *>(dz + p * 32);
+#pragma unroll
+            for (int c = 0; c < 4; c++) zv[c] = zp[c];
+            const int64_t n = p / plane;
+            const int rem = (int)(p - n * plane);
+            const int yy = rem / W + dyo, xx = rem % W + dxo;
+            if (yy >= 0 && yy < H && xx >= 0 && xx < W) {
+                const bf16x8 *xp = reinterpret_cast<const bf16x8 *>(xin + (n * plane + (int64_t)yy * W + xx) * 32);
+#pragma unroll
+                for (int c = 0; c < 4; c++) xv[c] = xp[c];
+            }
+        }
+    };
+    const int64_t step = (int64_t)kWaves * kGradMfmaSlab;
+    int64_t s = p0 + (int64_t)wave * kGradMfmaSlab;
+    if (s < p1) fetch(s);
+    for (; s < p1; s += step) {
+#pragma unroll
+        for (int c = 0; c < 4; c++) {
+            *reinterpret_cast<bf16x8 *>(zw + lane * kGradRow + 8 * c) = zv[c];
+            *reinterpret_cast<bf16x8 *>(xw + lane * kGradRow + 8 * c) = xv[c];
+        }
+        // the next slab's loads fly while this one is multiplied
+        if (s + step < p1) fetch(s + step);
+        // the rows are this wave's own: its LDS operations complete in order
+        __builtin_amdgcn_wave_barrier();
+#pragma unroll
+        for (int k = 0; k < kGradMfmaSlab / 16; k++) {
+            bf16x8 a, b;
+#pragma unroll
+            for (int i = 0; i < 8; i++) {
+                const int row = k * 16 + half * 8 + i;
+                a[i] = zw[row * kGradRow + m];
+                b[i] = xw[row * kGradRow + m];
+            }
+            acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, acc, 0, 0, 0);
+            if (tap == 0) {
+#pragma unroll
+                for (int i = 0; i < 8; i++) bacc += from_bf16(a[i]);
+            }
+        }
+        __builtin_amdgcn_wave_barrier();
+    }
+    // fixed-order combination of the waves: 16 accumulators + the bias sum per lane
+    float *rw = red[wave] + lane * 17;
+#pragma unroll
+    for (int r = 0; r < 16; r++) rw[r] = acc[r];
+    rw[16] = bacc;
+    __syncthreads();
+    if (wave == 0) {
+        float out[17];
+#pragma unroll
+        for (int r = 0; r < 17; r++) out[r] = red[0][lane * 17 + r];
+        for (int w = 1; w < kWaves; w++)
+#pragma unroll
+            for (int r = 0; r < 17; r++) out[r] += red[w][lane * 17 + r];
+        float *dst = partials + ((int64_t)blockIdx.x * 9 + tap) * 1024;
+#pragma unroll
+        for (int r = 0; r < 16; r++) dst[(8 * (r / 4) + 4 * half + r % 4) * 32 + m] = out[r];
+        if (tap == 0) {
+            // lanes m and m + 32 hold the two pixel halves of channel m
+            const float other = __shfl_down(out[16], 32, kWave);
+            if (half == 0) bias_partials[(int64_t)blockIdx.x * 32 + m] = out[16] + other;
+        }
+    }
+}
+
 // fixed-order reduction of the partials into dW [9][cout][cin] and db [cout]
 __global__ void __launch_bounds__(kBlock) reduce_grad_kernel(const float *partials, const float *bias_partials,
                                                              int nchunks, int cout, int cin, float *dw, float *db)
@@ -1119,8 +1229,12 @@ extern "C" int mv2_nn_weight_grad(mv2_context *ctx, const void *dz, int zmode, c
     if (int rc = ensure_scratch(ctx, ctx->nn_partials, need)) return rc;
     float *parts = (float *)ctx->nn_partials.ptr;
     float *bparts = parts + (size_t)nchunks * 9 * 1024;
-    hipLaunchKernelGGL(weight_grad_kernel, dim3(nchunks, 9), dim3(kBlock), 0, ctx->stream, dz, zmode, x, xmode, B, H,
-                       W, dil, chunk, parts, bparts);
+    if (zmode == 0 && xmode == 0)
+        hipLaunchKernelGGL(weight_grad_mfma_kernel, dim3(nchunks, 9), dim3(kBlock), 0, ctx->stream,
+                           (const __bf16 *)dz, (const __bf16 *)x, B, H, W, dil, chunk, parts, bparts);
+    else
+        hipLaunchKernelGGL(weight_grad_kernel, dim3(nchunks, 9), dim3(kBlock), 0, ctx->stream, dz, zmode, x, xmode,
+                           B, H, W, dil, chunk, parts, bparts);
     hipLaunchKernelGGL(reduce_grad_kernel, dim3(37), dim3(kBlock), 0, ctx->stream, parts, bparts, nchunks, cout, cin,
                        dw, db);
     MV2_HIP(ctx, hipGetLastError());

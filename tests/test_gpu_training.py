@@ -62,3 +62,38 @@ def test_training_reduces_the_loss():
     assert last < 0.5 * first, (first, last)
     w1 = net.params_numpy()["w1"]
     assert np.max(np.abs(w1[:, :, 0].sum(axis=0))) < 1e-5
+
+
+@pytest.mark.parametrize("dil", [1, 2])
+def test_mfma_weight_gradient_matches_float64_sum(dil):
+    """The 32 -> 32 layer weight gradient on the matrix cores:
+    dW[t][co][ci] = sum_p dZ[p][co] X[p + off(t)][ci] (zero padding),
+    db[co] = sum_p dZ[p][co], against a float64 sum over the same BF16
+    inputs on an uneven batch (FP32 accumulation: relative error ~1e-6)."""
+    import torch
+    from isaac_bulk_hip.relaxation import _runtime
+
+    rt = _runtime("cuda:0")
+    rt.bind_stream()
+    B, H, W = 3, 37, 45
+    rng = np.random.default_rng(20 + dil)
+    dz = torch.from_numpy(rng.standard_normal((B, H, W, 32)).astype(np.float32)).to("cuda:0", torch.bfloat16)
+    x = torch.from_numpy(rng.uniform(0.0, 1.0, (B, H, W, 32)).astype(np.float32)).to("cuda:0", torch.bfloat16)
+    dw = torch.zeros(9 * 32 * 32, device="cuda:0")
+    db = torch.zeros(32, device="cuda:0")
+    rt.check(rt.lib.mv2_nn_weight_grad(rt.ctx, dz.data_ptr(), 0, x.data_ptr(), 0, B, H, W, dil, 32, 32,
+                                       dw.data_ptr(), db.data_ptr()))
+    torch.cuda.synchronize()
+    z64 = dz.double().cpu().numpy()
+    x64 = x.double().cpu().numpy()
+    ref = np.zeros((9, 32, 32))
+    for t in range(9):
+        dy, dx = (t // 3 - 1) * dil, (t % 3 - 1) * dil
+        xs = np.zeros_like(x64)
+        ys0, ys1 = max(0, -dy), min(H, H - dy)
+        xs0, xs1 = max(0, -dx), min(W, W - dx)
+        xs[:, ys0:ys1, xs0:xs1] = x64[:, ys0 + dy:ys1 + dy, xs0 + dx:xs1 + dx]
+        ref[t] = np.einsum("bhwo,bhwi->oi", z64, xs)
+    got = dw.double().cpu().numpy().reshape(9, 32, 32)
+    assert np.max(np.abs(got - ref)) <= 2e-5 * np.max(np.abs(ref))
+    np.testing.assert_allclose(db.double().cpu().numpy(), z64.sum(axis=(0, 1, 2)), rtol=1e-5, atol=1e-4)
