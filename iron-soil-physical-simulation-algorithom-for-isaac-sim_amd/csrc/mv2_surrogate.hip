@@ -351,64 +351,99 @@ __global__ void expand_grad_kernel(const float *g, int64_t npix, __bf16 *out16)
 // weight gradient: dW[t][co][ci] = sum_p dZ[p][co] * X[p + off(t)][ci],
 // db[co] = sum_p dZ[p][co].  dZ: BF16 NHWC32 (zmode 0) or FP32 channel-0
 // (zmode 1, cout = 1); X: BF16 NHWC32 (xmode 0) or FP32 features stride 4
-// (xmode 1, cin = 3).  One workgroup = one tap x one chunk of pixels; 256
-// threads own 4 (co, ci) pairs each; 64-pixel slabs staged in LDS.
-// partials: [nchunks][9][32][32] (+ [nchunks][32] bias, tap 0 only)
-// ---------------------------------------------------------------------------
-constexpr int kGradSlab = 64;
-
-__global__ void __launch_bounds__(kBlock) weight_grad_kernel(const void *dz, int zmode, const void *xin, int xmode,
-                                                             int B, int H, int W, int dil, int64_t chunk,
-                                                             float *partials, float *bias_partials)
+// (xmode 1, 3 channels and a zero pad slot: cin <= 4).  partials:
+// [nchunks][9][32][32] (+ [nchunks][32] bias, tap 0 only); only the pairs
+// co < cout, ci < cin are written and read.
+//
+// weight_grad_small_kernel: the first and last layers, whose cout x cin is
+// 1 x 32 (zmode 1: FP32 dZ, BF16 NHWC32 X) or 32 x 4 (xmode 1: BF16 NHWC32 dZ,
+// FP32 features: 3 channels and a zero pad slot).  Each thread takes pixels
+// p0 + t, p0 + t + 256, ... of the workgroup's chunk, loads the pixel's
+// 32-channel row as 16-byte vectors and accumulates its 32 (or 3 x 32 + 32
+// bias) sums in registers; DPP wave sums and a fixed-order sum over the 4
+// waves give the chunk's partials.
+template <int XMODE>
+__global__ void __launch_bounds__(kBlock) weight_grad_small_kernel(const void *dz, const void *xin, int B, int H,
+                                                                   int W, int dil, int64_t chunk, float *partials,
+                                                                   float *bias_partials)
 {
-    __shared__ float zs[kGradSlab][33];
-    __shared__ float xs[kGradSlab][33];
+    constexpr int kWaves = kBlock / kWave;
+    constexpr int NV = XMODE ? 32 * 4 : 32 + 1;   // sums per thread: [co][ci 0..2, bias] or [ci], bias
+    __shared__ float red[kWaves][NV];
     const int tap = blockIdx.y;
     const int dyo = (tap / 3 - 1) * dil, dxo = (tap % 3 - 1) * dil;
-    const int64_t total = (int64_t)B * H * W;
+    const int64_t plane = (int64_t)H * W;
+    const int64_t total = (int64_t)B * plane;
     const int64_t p0 = (int64_t)blockIdx.x * chunk;
     const int64_t p1 = p0 + chunk < total ? p0 + chunk : total;
-    const int co = threadIdx.x / 8, ci0 = (threadIdx.x % 8) * 4;
-    float acc[4] = {0.0f, 0.0f, 0.0f, 0.0f};
-    float bacc = 0.0f;
-    for (int64_t s = p0; s < p1; s += kGradSlab) {
-        for (int e = threadIdx.x; e < kGradSlab * 32; e += blockDim.x) {
-            const int pl = e / 32, c = e % 32;
-            const int64_t p = s + pl;
-            float zv = 0.0f, xv = 0.0f;
-            if (p < p1) {
-                if (zmode == 0) zv = from_bf16(reinterpret_cast<const __bf16 *>(dz)[p * 32 + c]);
-                else zv = c == 0 ? reinterpret_cast<const float *>(dz)[p] : 0.0f;
-                const int64_t n = p / ((int64_t)H * W);
-                const int rem = (int)(p - n * H * W);
-                const int yy = rem / W + dyo, xx = rem % W + dxo;
-                if (yy >= 0 && yy < H && xx >= 0 && xx < W) {
-                    const int64_t q = n * H * W + (int64_t)yy * W + xx;
-                    if (xmode == 0) xv = from_bf16(reinterpret_cast<const __bf16 *>(xin)[q * 32 + c]);
-                    else xv = c < 3 ? reinterpret_cast<const float *>(xin)[q * 4 + c] : 0.0f;
-                } else if (xmode == 1 && c == 0) {
-                    // replicate-padding of the b_eff channel (conv1_mfma_kernel)
-                    const int64_t q = n * H * W + (int64_t)(rem / W) * W + rem % W;
-                    xv = reinterpret_cast<const float *>(xin)[q * 4];
+    float acc[NV];
+#pragma unroll
+    for (int v = 0; v < NV; v++) acc[v] = 0.0f;
+    for (int64_t p = p0 + threadIdx.x; p < p1; p += blockDim.x) {
+        const int64_t n = p / plane;
+        const int rem = (int)(p - n * plane);
+        const int yy = rem / W + dyo, xx = rem % W + dxo;
+        const bool inside = yy >= 0 && yy < H && xx >= 0 && xx < W;
+        const int64_t q = n * plane + (int64_t)yy * W + xx;
+        if constexpr (XMODE) {
+            bf16x8 zr[4];
+            const bf16x8 *zp = reinterpret_cast<const bf16x8 *>(reinterpret_cast<const __bf16 *>(dz) + p * 32);
+#pragma unroll
+            for (int c = 0; c < 4; c++) zr[c] = zp[c];
+            const float *feat = reinterpret_cast<const float *>(xin);
+            float f[3] = {0.0f, 0.0f, 0.0f};
+            if (inside) {
+                f[0] = feat[q * 4];
+                f[1] = feat[q * 4 + 1];
+                f[2] = feat[q * 4 + 2];
+            } else {
+                f[0] = feat[p * 4];     // replicate-padding of the b_eff channel (conv1_mfma_kernel)
+            }
+#pragma unroll
+            for (int co = 0; co < 32; co++) {
+                const float z = from_bf16(zr[co / 8][co % 8]);
+                acc[co * 4 + 0] += z * f[0];
+                acc[co * 4 + 1] += z * f[1];
+                acc[co * 4 + 2] += z * f[2];
+                acc[co * 4 + 3] += z;
+            }
+        } else {
+            const float z = reinterpret_cast<const float *>(dz)[p];
+            acc[32] += z;
+            if (inside) {
+                const bf16x8 *xp = reinterpret_cast<const bf16x8 *>(reinterpret_cast<const __bf16 *>(xin) + q * 32);
+#pragma unroll
+                for (int c = 0; c < 4; c++) {
+                    const bf16x8 xr = xp[c];
+#pragma unroll
+                    for (int i = 0; i < 8; i++) acc[8 * c + i] += z * from_bf16(xr[i]);
                 }
             }
-            zs[pl][c] = zv;
-            xs[pl][c] = xv;
         }
-        __syncthreads();
-        for (int pl = 0; pl < kGradSlab; pl++) {
-            const float z = zs[pl][co];
-            acc[0] += z * xs[pl][ci0 + 0];
-            acc[1] += z * xs[pl][ci0 + 1];
-            acc[2] += z * xs[pl][ci0 + 2];
-            acc[3] += z * xs[pl][ci0 + 3];
-            if (tap == 0 && (threadIdx.x % 8) == 0) bacc += z;
-        }
-        __syncthreads();
     }
-    float *dst = partials + ((int64_t)blockIdx.x * 9 + tap) * 1024 + co * 32 + ci0;
-    for (int k = 0; k < 4; k++) dst[k] = acc[k];
-    if (tap == 0 && (threadIdx.x % 8) == 0) bias_partials[(int64_t)blockIdx.x * 32 + co] = bacc;
+    const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
+#pragma unroll
+    for (int v = 0; v < NV; v++) {
+        const float t = wave_reduce<0>(acc[v]);
+        if (lane == 0) red[wave][v] = t;
+    }
+    __syncthreads();
+    float *dst = partials + ((int64_t)blockIdx.x * 9 + tap) * 1024;
+    for (int v = threadIdx.x; v < NV; v += blockDim.x) {
+        float t = red[0][v];
+        for (int w = 1; w < kWaves; w++) t += red[w][v];
+        if constexpr (XMODE) {
+            const int co = v / 4, k = v % 4;
+            if (k < 3) dst[co * 32 + k] = t;
+            else {
+                dst[co * 32 + 3] = 0.0f;          // the features' zero pad slot
+                if (tap == 0) bias_partials[(int64_t)blockIdx.x * 32 + co] = t;
+            }
+        } else {
+            if (v < 32) dst[v] = t;
+            else if (tap == 0) bias_partials[(int64_t)blockIdx.x * 32] = t;
+        }
+    }
 }
 
 // The 32 -> 32 channel layers (dZ and X both BF16 NHWC32) on
@@ -1216,13 +1251,18 @@ extern "C" int mv2_nn_weight_grad(mv2_context *ctx, const void *dz, int zmode, c
                                   int H, int W, int dil, int cout, int cin, float *dw, float *db)
 {
     if (int rc = check_plain(ctx)) return rc;
+    if ((zmode != 0 && zmode != 1) || (xmode != 0 && xmode != 1) || (zmode == 1 && xmode == 1) || cout < 1 ||
+        cout > 32 || cin < 1 || cin > 32 || (zmode == 1 && cout != 1) || (xmode == 1 && cin > 4) || B < 1 || H < 1 ||
+        W < 1 || dil < 1)
+        return set_error(ctx, MV2_EINVAL, "[WarmStartNet] weight gradient: unsupported layer shape");
     const int64_t total = (int64_t)B * H * W;
-    int64_t chunk = 4096;
+    const bool mfma = zmode == 0 && xmode == 0;
+    int64_t chunk = mfma ? 4096 : 1024;         // the narrow layers: 4 pixels per thread
     int nchunks = (int)((total + chunk - 1) / chunk);
     if (nchunks > 2048) {
         nchunks = 2048;
         chunk = (total + nchunks - 1) / nchunks;
-        chunk = (chunk + kGradSlab - 1) / kGradSlab * kGradSlab;
+        chunk = (chunk + kGradMfmaSlab - 1) / kGradMfmaSlab * kGradMfmaSlab;
         nchunks = (int)((total + chunk - 1) / chunk);
     }
     const size_t need = sizeof(float) * ((size_t)nchunks * 9 * 1024 + (size_t)nchunks * 32);
@@ -1232,9 +1272,12 @@ extern "C" int mv2_nn_weight_grad(mv2_context *ctx, const void *dz, int zmode, c
     if (zmode == 0 && xmode == 0)
         hipLaunchKernelGGL(weight_grad_mfma_kernel, dim3(nchunks, 9), dim3(kBlock), 0, ctx->stream,
                            (const __bf16 *)dz, (const __bf16 *)x, B, H, W, dil, chunk, parts, bparts);
+    else if (xmode == 1)
+        hipLaunchKernelGGL(weight_grad_small_kernel<1>, dim3(nchunks, 9), dim3(kBlock), 0, ctx->stream, dz, x, B, H,
+                           W, dil, chunk, parts, bparts);
     else
-        hipLaunchKernelGGL(weight_grad_kernel, dim3(nchunks, 9), dim3(kBlock), 0, ctx->stream, dz, zmode, x, xmode,
-                           B, H, W, dil, chunk, parts, bparts);
+        hipLaunchKernelGGL(weight_grad_small_kernel<0>, dim3(nchunks, 9), dim3(kBlock), 0, ctx->stream, dz, x, B, H,
+                           W, dil, chunk, parts, bparts);
     hipLaunchKernelGGL(reduce_grad_kernel, dim3(37), dim3(kBlock), 0, ctx->stream, parts, bparts, nchunks, cout, cin,
                        dw, db);
     MV2_HIP(ctx, hipGetLastError());
