@@ -419,8 +419,11 @@ __global__ void __launch_bounds__(kBlock) batch_finalize_kernel(BatchArgs a, int
 // ---------------------------------------------------------------------------
 // Batched relaxation: the relaxation of every instance of the batch, each with
 // its own iteration count and convergence test (as if relaxed alone), one
-// launch per iteration over the 32 x 48 relaxation tiles of all instances.
+// launch per iteration over the 44 x 48 relaxation tiles of all instances
+// (the single-field tile iteration of mv2_relax_tile.hpp with one iteration).
 // ---------------------------------------------------------------------------
+using BRelaxGeom = RelaxGeom<44, 48, 640, 1, false>;
+constexpr int kBRelaxTH = BRelaxGeom::TH, kBRelaxTW = BRelaxGeom::TW;
 struct BRelaxControl {
     int iterations, src, done, converged;
     double last;            // maximum excess of the last applied iteration
@@ -479,10 +482,10 @@ __global__ void batch_relax_reset_kernel(BatchRelaxArgs a)
     }
 }
 
-__global__ void __launch_bounds__(kRelaxBlock) batch_relax_kernel(BatchRelaxArgs a, int k)
+__global__ void __launch_bounds__(BRelaxGeom::BLOCK) batch_relax_kernel(BatchRelaxArgs a, int k)
 {
-    __shared__ float tile[kRelaxLH * kRelaxLW];
-    __shared__ double smem[kRelaxBlock / kWave];
+    __shared__ float tile[BRelaxGeom::kCells];
+    __shared__ double smem[BRelaxGeom::BLOCK / kWave];
     const int g = brelax_tile(a);
     const int i = a.tile_inst[g];
     const bool first = g == a.inst_first[i];
@@ -502,15 +505,15 @@ __global__ void __launch_bounds__(kRelaxBlock) batch_relax_kernel(BatchRelaxArgs
     }
     const int64_t off = a.inst_off[i];
     // Quiet tiles: an iteration of a tile reads its interior and an 8-cell
-    // halo, i.e. its 3 x 3 tile neighbourhood.  When none of those tiles
+    // halo (smaller than a tile), i.e. its 3 x 3 tile neighbourhood.  When none of those tiles
     // changed in launch k - 1, the tile's input equals the input of its last
     // computed iteration, which changed nothing, so this iteration would
     // reproduce the same field (already in both ping-pong buffers) and the same
     // excess: the stored excess stands in for it.  Launch 0 runs every tile.
     if (k > 0) {
         const int nx = a.inst_nx[i], ny = a.inst_ny[i];
-        const int tx = (nx + kRelaxTW - 1) / kRelaxTW, ty = (ny + kRelaxTH - 1) / kRelaxTH;
-        const int trow = a.tile_r0[g] / kRelaxTH, tcol = a.tile_c0[g] / kRelaxTW;
+        const int tx = (nx + kBRelaxTW - 1) / kBRelaxTW, ty = (ny + kBRelaxTH - 1) / kBRelaxTH;
+        const int trow = a.tile_r0[g] / kBRelaxTH, tcol = a.tile_c0[g] / kBRelaxTW;
         int flag = 0;
         if (threadIdx.x < 9) {
             const int rr = trow + (int)threadIdx.x / 3 - 1, cc = tcol + (int)threadIdx.x % 3 - 1;
@@ -526,10 +529,11 @@ __global__ void __launch_bounds__(kRelaxBlock) batch_relax_kernel(BatchRelaxArgs
         }
     }
     bool changed = false;
-    const float tmax = relax_tile_iteration<float>(a.h[src] + off, a.h[src ^ 1] + off, a.inst_ny[i], a.inst_nx[i],
-                                                   a.crit, a.tile_r0[g], a.tile_c0[g], tile, &changed);
-    double m[1] = {(double)tmax};
-    block_reduce<double, 1, 1, kRelaxBlock / kWave>(m, smem);
+    float tm[1] = {0.0f};
+    relax_tile_iterations<float, BRelaxGeom>(a.h[src] + off, a.h[src ^ 1] + off, a.inst_ny[i], a.inst_nx[i], a.crit,
+                                             a.tile_r0[g], a.tile_c0[g], tile, 1, tm, &changed);
+    double m[1] = {(double)tm[0]};
+    block_reduce<double, 1, 1, BRelaxGeom::BLOCK / kWave>(m, smem);
     const int any = __syncthreads_or(changed ? 1 : 0);
     if (threadIdx.x == 0) {
         inst_max(inst_set(a.spread, k % 3, a.count, i), m[0]);
@@ -549,8 +553,8 @@ __global__ void __launch_bounds__(kBlock) batch_relax_end_kernel(BatchRelaxArgs 
     if (C.src == 1) {
         const int64_t off = a.inst_off[i];
         const int ny = a.inst_ny[i], nx = a.inst_nx[i], r0 = a.tile_r0[g], c0 = a.tile_c0[g];
-        for (int e = threadIdx.x; e < kRelaxTH * kRelaxTW; e += blockDim.x) {
-            const int r = r0 + e / kRelaxTW, c = c0 + e % kRelaxTW;
+        for (int e = threadIdx.x; e < kBRelaxTH * kBRelaxTW; e += blockDim.x) {
+            const int r = r0 + e / kBRelaxTW, c = c0 + e % kBRelaxTW;
             if (r < ny && c < nx) a.h[0][off + (int64_t)r * nx + c] = a.h[1][off + (int64_t)r * nx + c];
         }
     }
@@ -689,18 +693,18 @@ static int build_tiles(mv2_context *ctx, mv2_batch_state *bs, int shape)
     return MV2_OK;
 }
 
-// relaxation tiles of every instance (32 x 48, row-major per instance)
+// relaxation tiles of every instance (44 x 48, row-major per instance)
 static int build_relax_tiles(mv2_context *ctx, mv2_batch_state *bs)
 {
     std::vector<int32_t> first(bs->count), ntiles(bs->count), ti, tr, tc;
     for (int i = 0; i < bs->count; i++) {
         first[i] = (int32_t)ti.size();
-        const int tx = (bs->nx[i] + kRelaxTW - 1) / kRelaxTW, ty = (bs->ny[i] + kRelaxTH - 1) / kRelaxTH;
+        const int tx = (bs->nx[i] + kBRelaxTW - 1) / kBRelaxTW, ty = (bs->ny[i] + kBRelaxTH - 1) / kBRelaxTH;
         for (int r = 0; r < ty; r++)
             for (int c = 0; c < tx; c++) {
                 ti.push_back(i);
-                tr.push_back(r * kRelaxTH);
-                tc.push_back(c * kRelaxTW);
+                tr.push_back(r * kBRelaxTH);
+                tc.push_back(c * kBRelaxTW);
             }
         ntiles[i] = tx * ty;
     }
@@ -1015,7 +1019,7 @@ extern "C" int mv2_batch_relax(mv2_context *ctx, float *field, const double crit
     for (;;) {
         if (chunk > max_iterations - k) chunk = max_iterations - k;
         for (int j = 0; j < chunk; j++, k++)
-            hipLaunchKernelGGL(batch_relax_kernel, dim3(grid), dim3(kRelaxBlock), 0, st, a, k);
+            hipLaunchKernelGGL(batch_relax_kernel, dim3(grid), dim3(BRelaxGeom::BLOCK), 0, st, a, k);
         hipLaunchKernelGGL(batch_relax_end_kernel, dim3(grid), dim3(kBlock), 0, st, a, k);
         MV2_HIP(ctx, hipGetLastError());
         MV2_HIP(ctx, hipMemcpyAsync(ctx->host_result.ptr, a.out, out_bytes, hipMemcpyDeviceToHost, st));

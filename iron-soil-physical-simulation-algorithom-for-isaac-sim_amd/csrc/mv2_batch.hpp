@@ -25,7 +25,7 @@ struct mv2_batch_state {
     Scratch alt_h, alt_qx, alt_qy;
     Scratch ctl, wave, stat, ledger, rec;
     int predicted = 0;
-    // relaxation: 32 x 48 tiles of every instance, ping-pong field, loop state
+    // relaxation: 44 x 48 tiles of every instance, ping-pong field, loop state
     Scratch rtab;                        // first tile, tile count per instance; tile -> instance, r0, c0
     int rtiles = 0;
     Scratch relax_alt, relax_ctl, relax_spread, relax_out;
