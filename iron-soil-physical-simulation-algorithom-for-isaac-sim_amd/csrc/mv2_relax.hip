@@ -5,7 +5,8 @@
 // relax: a launch applies up to K iterations to LDS tiles (tile interior
 // 44 x 48, halo 6K + 2; mv2_relax_tile.hpp) and writes the field once
 // (ping-pong buffers).  The first launch of a call applies one iteration (a
-// field at rest converges there), later FP32 launches two.  Each iteration's
+// field at rest converges there), later FP32 launches two when the whole grid
+// runs at once (one otherwise).  Each iteration's
 // maximum excess over the owned edges is reduced into its own spread-maximum
 // set; the next launch tests them in order against the tolerance on the
 // device.  The reference returns the field of the first converged iteration:
@@ -289,12 +290,17 @@ static int launch_relax(mv2_context *ctx, void *height, int ny, int nx, const do
     if (chunk > 32) chunk = 32;
     int k = 0;
     const volatile mv2_relax_result *hres = a.out;
+    // two iterations per launch only while every workgroup of a launch runs at
+    // once (at most 2 per CU; 701^2: 240 tiles): a larger grid runs in rounds,
+    // where the two-iteration tile's 1.4x larger LDS area costs more than the
+    // halved launches save
+    const bool pair = a.tiles <= 2 * (ctx->cu_count > 0 ? ctx->cu_count : 256);
     for (;;) {
         for (int j = 0; j < chunk; j++, k++) {
             if constexpr (sizeof(R) == 8) {
                 hipLaunchKernelGGL((relax_kernel<R, RelaxSingle64>), dim3(a.tiles), dim3(RelaxSingle64::BLOCK), 0, st, a, k);
             } else {
-                if (k == 0)
+                if (k == 0 || !pair)
                     hipLaunchKernelGGL((relax_kernel<R, RelaxFirst>), dim3(a.tiles), dim3(RelaxFirst::BLOCK), 0, st, a, k);
                 else
                     hipLaunchKernelGGL((relax_kernel<R, RelaxPair>), dim3(a.tiles), dim3(RelaxPair::BLOCK), 0, st, a, k);
@@ -317,10 +323,10 @@ static int launch_relax(mv2_context *ctx, void *height, int ny, int nx, const do
     out->launches = k;
     ctx->relax_clean = 1;
     ctx->relax_calls++;
-    // launches that applied iterations: one, then pairs (FP32); the next call
-    // enqueues one more, which covers a replay
+    // launches that applied iterations: one, then pairs (FP32, small grids) or
+    // ones; the next call enqueues one more, which covers a replay
     const int it = out->iterations;
-    ctx->relax_predicted = sizeof(R) == 8 ? it : (it <= 1 ? 1 : 1 + it / 2);
+    ctx->relax_predicted = (sizeof(R) == 8 || !pair) ? it : (it <= 1 ? 1 : 1 + it / 2);
     return MV2_OK;
 }
 
