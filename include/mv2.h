@@ -112,8 +112,11 @@ int mv2_step(mv2_context *ctx, const mv2_step_config *cfg, mv2_step_record *out)
 int mv2_set_external(mv2_context *ctx, const void *ax, const void *ay, double max_abs);
 
 /* Timing mode (no counterpart in the reference; a measurement aid): when
- * enabled, mv2_step brackets every substep launch with HIP events and reports
- * their summed duration in substep_kernel_ms. */
+ * enabled, mv2_step records one HIP event before its first substep launch and
+ * one after the last launch expected to work (the previous call's substep
+ * count); substep_kernel_ms is the time between them when the step took
+ * exactly that many substeps, else 0 (events between launches would add
+ * their own latency to each). */
 int mv2_set_timing(mv2_context *ctx, int enable);
 
 /* Launch graphs (no counterpart in the reference; the reference's step is a

@@ -1045,13 +1045,15 @@ static void enqueue_begin(const V2Args<P> &a, const StepLaunch &L, hipStream_t s
     }
 }
 
-// substep launches k0 .. k0 + count - 1 (event pairs around each in timing mode)
+// substep launches k0 .. k0 + count - 1.  Timing mode brackets the launches
+// expected to work (0 .. eager_until - 1 of the first round) with one event
+// pair: events between launches would add their own latency to every launch.
 template <class P>
 static int enqueue_substeps(mv2_context *ctx, const V2Args<P> &a, const StepLaunch &L, hipStream_t st, int k0,
                             int count, int eager_until, bool timing)
 {
     for (int k = k0; k < k0 + count; k++) {
-        if (timing) MV2_HIP(ctx, hipEventRecord(ctx->sub_events[2 * k], st));
+        if (timing && k == 0) MV2_HIP(ctx, hipEventRecord(ctx->sub_events[0], st));
         const int eager = k < eager_until ? 1 : 0;
         if constexpr (P::kQuant) {
             switch (L.band) {
@@ -1064,7 +1066,7 @@ static int enqueue_substeps(mv2_context *ctx, const V2Args<P> &a, const StepLaun
         } else {
             hipLaunchKernelGGL(v2_substep_kernel<P>, dim3(L.gu), dim3(kBlock), 0, st, a, k, eager);
         }
-        if (timing) MV2_HIP(ctx, hipEventRecord(ctx->sub_events[2 * k + 1], st));
+        if (timing && k0 == 0 && k == eager_until - 1) MV2_HIP(ctx, hipEventRecord(ctx->sub_events[1], st));
     }
     return MV2_OK;
 }
@@ -1236,6 +1238,7 @@ static int launch_step(mv2_context *ctx, const mv2_step_config *cfg, mv2_step_re
     int chunk = fixed_chunk > 0 ? fixed_chunk : (ctx->v2_predicted > 0 ? ctx->v2_predicted + 1 : 4);
     // launches expected to work stage their tile before the loop state arrives
     int eager_until = fixed_chunk > 0 ? 0 : (ctx->v2_predicted > 0 ? ctx->v2_predicted : 4);
+    const int bracket = eager_until;        // timing mode: launches 0 .. bracket - 1 between the event pair
     const StepLaunch L{band, gs, gu, (kFoldWaves + kBlock / kWave - 1) / (kBlock / kWave), fold};
     // the first round (begin, predicted substeps, end / fold) as one graph launch
     bool replayed = false;
@@ -1246,8 +1249,8 @@ static int launch_step(mv2_context *ctx, const mv2_step_config *cfg, mv2_step_re
     MV2_HIP(ctx, hipGetLastError());
     for (;;) {
         if (!replayed) {
-            if (timing && (int)ctx->sub_events.size() < 2 * (k + chunk)) {
-                while ((int)ctx->sub_events.size() < 2 * (k + chunk)) {
+            if (timing && (int)ctx->sub_events.size() < 2) {
+                while ((int)ctx->sub_events.size() < 2) {
                     hipEvent_t e;
                     MV2_HIP(ctx, hipEventCreate(&e));
                     ctx->sub_events.push_back(e);
@@ -1277,15 +1280,11 @@ static int launch_step(mv2_context *ctx, const mv2_step_config *cfg, mv2_step_re
     MV2_HIP(ctx, hipEventElapsedTime(&ms, ctx->ev_start, ctx->ev_stop));
     out->kernel_ms = ms;
     out->substep_kernel_ms = 0.0;
-    if (timing) {
-        // substeps are the first launches; the ones after the loop ended return at once
-        double sum = 0.0;
-        for (int j = 0; j < out->substeps && j < k; j++) {
-            float e = 0.0f;
-            MV2_HIP(ctx, hipEventElapsedTime(&e, ctx->sub_events[2 * j], ctx->sub_events[2 * j + 1]));
-            sum += e;
-        }
-        out->substep_kernel_ms = sum;
+    if (timing && bracket > 0 && out->substeps == bracket) {
+        // substeps are the first launches: the bracket held exactly the working ones
+        float e = 0.0f;
+        MV2_HIP(ctx, hipEventElapsedTime(&e, ctx->sub_events[0], ctx->sub_events[1]));
+        out->substep_kernel_ms = e;
     }
     return MV2_OK;
 }
