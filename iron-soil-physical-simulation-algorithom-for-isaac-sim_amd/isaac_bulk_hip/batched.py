@@ -105,7 +105,7 @@ class BatchedMobileV2:
     backend_name = "GPU_HIP_MOBILE_V2_BATCHED"
 
     def __init__(self, terrains, dx: float, dy: float | None = None, *, runtime: HipRuntime | None = None,
-                 config: MobileV2Config | None = None, use_graph: bool = True, tile_shape: int = 0) -> None:
+                 config: MobileV2Config | None = None, use_graph: bool = False, tile_shape: int = 0) -> None:
         torch = _torch()
         self.rt = runtime if runtime is not None else HipRuntime("cuda:0")
         self.dx = float(dx)
@@ -126,6 +126,7 @@ class BatchedMobileV2:
         self.qx = torch.zeros(self.cells, dtype=torch.float32, device=dev)
         self.qy = torch.zeros(self.cells, dtype=torch.float32, device=dev)
         self.tile_shape = int(tile_shape)
+        self._token = object()            # marks the runtime's context as bound to this batch
         self._bind(self.b, self.h, self.qx, self.qy)
         self.last_kernel_ms = 0.0
 
@@ -140,15 +141,28 @@ class BatchedMobileV2:
                                        self.offset.ctypes.data, self.dx, self.dy, b.data_ptr(), h.data_ptr(),
                                        qx.data_ptr(), qy.data_ptr()))
         rt.check(rt.lib.mv2_batch_set_tile_shape(rt.ctx, self.tile_shape))
+        rt.batch_owner = self._token
+
+    def _claim(self) -> None:
+        """A runtime's context holds one bound batch: rebind this one when
+        another batch on the same runtime was bound since (the batches' loop
+        state restarts; fields live in each batch's own tensors)."""
+        if getattr(self.rt, "batch_owner", None) is not self._token:
+            self._bind(self.b, self.h, self.qx, self.qy)
 
     def set_tile_shape(self, shape: int) -> None:
         if shape not in TILE_SHAPES:
             raise ValueError(f"[BatchedMobileV2] unknown tile shape {shape}")
+        self._claim()
         self.tile_shape = int(shape)
         self.rt.check(self.rt.lib.mv2_batch_set_tile_shape(self.rt.ctx, self.tile_shape))
 
     def step(self, dt: float, *, limit_mode: int = 0) -> BatchRecords:
         """Advance every instance by dt; one record per instance."""
+        self._claim()
+        return self._step(dt, limit_mode)
+
+    def _step(self, dt: float, limit_mode: int = 0) -> BatchRecords:
         rt = self.rt
         rt.bind_stream()
         cfg = _step_config(self.config, dt, density=1.0, limit_mode=limit_mode, weight_mode=0)
@@ -170,6 +184,7 @@ class BatchedMobileV2:
         array per instance: iterations, converged, max_excess_last."""
         from .relaxation import legacy_critical_differences
 
+        self._claim()
         rt = self.rt
         rt.bind_stream()
         target = self.b if field is None else field
@@ -200,6 +215,7 @@ class BatchedMobileV2:
         torch = _torch()
         if len(set(zip(self.ny.tolist(), self.nx.tolist()))) != 1:
             raise ValueError("[BatchedWarmStart] the terrains of the batch must have equal sizes")
+        self._claim()
         rt = self.rt
         rt.bind_stream()
         B, H, W = self.count, int(self.ny[0]), int(self.nx[0])
@@ -225,6 +241,7 @@ class BatchedMobileV2:
         skips the range and uniqueness checks (which read the indices back)
         for inputs already validated."""
         torch = _torch()
+        self._claim()
         rt = self.rt
         rt.bind_stream()
         dev = rt.device
@@ -265,6 +282,7 @@ class BatchedMobileV2:
         stats an int32 [count, 4] array of region count, largest component,
         winner root (-1 if none) and unstable cells per instance."""
         torch = _torch()
+        self._claim()
         rt = self.rt
         rt.bind_stream()
         if labels is None:
@@ -307,7 +325,7 @@ class BatchedMobileV2:
                 for _ in range(frames + 1):        # the first frame also captures the graph
                     b.copy_(self.b); h.copy_(self.h); qx.copy_(self.qx); qy.copy_(self.qy)
                     torch.cuda.synchronize()
-                    self.step(dt)
+                    self._step(dt)
                     times.setdefault(s, []).append(self.last_kernel_ms * 1e-3)
         finally:
             best = min(times, key=lambda s: min(times[s][1:] or times[s]))

@@ -236,3 +236,29 @@ def test_batched_warm_start_matches_single_field(settle_rounds):
         single = warm_start_relax(net, bt, ht, 0.05, 34.0, 1e-6, max_iterations=5000, settle_rounds=settle_rounds)
         assert single.converged
         np.testing.assert_array_equal(batch.fields(i)[0], bt.cpu().numpy())
+
+
+def test_two_batches_on_one_runtime():
+    """A runtime's context holds one bound batch: two batches used
+    alternately on the same runtime each give exactly the fields of that batch
+    used alone on its own runtime."""
+    from isaac_bulk_hip import HipRuntime, MobileV2Config
+    from isaac_bulk_hip.batched import BatchedMobileV2
+
+    config = MobileV2Config(cfl=0.18)
+    sets = [_terrains()[:3], _terrains()[3:6]]
+    shared = HipRuntime("cuda:0")
+    together = [BatchedMobileV2(t, 0.05, runtime=shared, config=config) for t in sets]
+    alone = [BatchedMobileV2(t, 0.05, runtime=HipRuntime("cuda:0"), config=config) for t in sets]
+    for frame in range(3):
+        for group in (together, alone):
+            for batch in group:
+                batch.step(0.1)
+                batch.relax(34.0, 1e-6, max_iterations=200)
+                batch.entrain(np.array([0], np.int32), np.array([5], np.int32), np.array([0.001]))
+    for a, b in zip(together, alone):
+        for i in range(a.count):
+            for x, y in zip(a.fields(i), b.fields(i)):
+                np.testing.assert_array_equal(x, y)
+    for rt in {id(b.rt): b.rt for b in together + alone}.values():
+        rt.close()
