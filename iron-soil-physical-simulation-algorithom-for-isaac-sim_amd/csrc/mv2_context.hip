@@ -8,6 +8,7 @@
 // pkg/runtime/bulk_state_authority.py:543-621).
 #include "mv2_internal.hpp"
 
+#include <atomic>
 #include <chrono>
 #include <cstdarg>
 #include <cstring>
@@ -57,24 +58,64 @@ int ensure_scratch(mv2_context *ctx, Scratch &s, size_t bytes, bool host_pinned)
     return MV2_OK;
 }
 
-// Wait for the work queued on a stream by polling it: a step of the frame
-// path is tens of microseconds, less than the wake-up of a blocking
-// synchronize.  After kSpinLimitNs of polling the blocking call takes over;
-// errors of the stream come back either way.
+constexpr long long kSpinLimitNs = 2000000;
+
+static long long elapsed_ns(std::chrono::steady_clock::time_point t0)
+{
+    return std::chrono::duration_cast<std::chrono::nanoseconds>(std::chrono::steady_clock::now() - t0).count();
+}
+
+// Wait for the work queued on a stream by polling a marker recorded behind
+// it.  Measured on MI355X from the launch call of one small kernel to the
+// return: 11.7 us polling the marker, 12.4 us in hipStreamSynchronize and
+// 26 us polling hipStreamQuery.  After kSpinLimitNs the blocking synchronize
+// takes over; errors of the stream come back either way.
 int stream_wait(mv2_context *ctx, hipStream_t st)
 {
-    constexpr long long kSpinLimitNs = 2000000;
-    const auto t0 = std::chrono::steady_clock::now();
-    for (;;) {
-        const hipError_t e = hipStreamQuery(st);
-        if (e == hipSuccess) return MV2_OK;
-        if (e != hipErrorNotReady) return check_hip(ctx, e, "hipStreamQuery");
-        if (std::chrono::duration_cast<std::chrono::nanoseconds>(std::chrono::steady_clock::now() - t0).count() >
-            kSpinLimitNs)
-            break;
+    if (ctx->ev_wait && hipEventRecord(ctx->ev_wait, st) == hipSuccess) {
+        const auto t0 = std::chrono::steady_clock::now();
+        for (;;) {
+            const hipError_t e = hipEventQuery(ctx->ev_wait);
+            if (e == hipSuccess) return MV2_OK;
+            if (e != hipErrorNotReady) break;            // the synchronize below reports it
+            if (elapsed_ns(t0) > kSpinLimitNs) break;
+        }
     }
     const hipError_t e = hipStreamSynchronize(st);
     return e == hipSuccess ? MV2_OK : check_hip(ctx, e, "hipStreamSynchronize");
+}
+
+unsigned next_stamp(mv2_context *ctx, int words)
+{
+    if (!ctx->host_stamps.ptr || words < 1 || words > kStampWords) return 0u;
+    if (++ctx->stamp_seq == 0u) ctx->stamp_seq = 1u;    // 0 means "no stamps"
+    return ctx->stamp_seq;
+}
+
+unsigned *stamp_words(mv2_context *ctx) { return (unsigned *)ctx->host_stamps.ptr; }
+
+// Spin until every stamp word of the launch carries the call's number: the
+// results written to pinned memory before the stamps are then visible
+// (measured on MI355X from the launch call to the return: a pinned flag
+// written by one more kernel 9.9 us, the marker 11.7 us).  A launch that has
+// not stamped within kSpinLimitNs (a long call, or a device error that
+// stopped the stream) is waited for on the stream, which also reports the
+// error.
+int stamp_wait(mv2_context *ctx, hipStream_t st, unsigned stamp, int words)
+{
+    if (!stamp) return stream_wait(ctx, st);
+    const volatile unsigned *w = stamp_words(ctx);
+    const auto t0 = std::chrono::steady_clock::now();
+    int done = 0;
+    for (;;) {
+        while (done < words && w[done] == stamp) done++;
+        if (done == words) {
+            std::atomic_thread_fence(std::memory_order_acquire);
+            return MV2_OK;
+        }
+        if (elapsed_ns(t0) > kSpinLimitNs) break;
+    }
+    return stream_wait(ctx, st);
 }
 
 static void release(Scratch &s, bool host_pinned = false)
@@ -118,13 +159,19 @@ extern "C" int mv2_create(int device, mv2_context **out)
     if (int rc = ensure_scratch(ctx, ctx->sync, 4096)) { delete ctx; return rc; }
     if (int rc = ensure_scratch(ctx, ctx->result, 4096)) { delete ctx; return rc; }
     if (int rc = ensure_scratch(ctx, ctx->host_result, 4096, true)) { delete ctx; return rc; }
+    if (int rc = ensure_scratch(ctx, ctx->host_stamps, sizeof(unsigned) * kStampWords, true)) {
+        mv2_destroy(ctx);
+        return rc;
+    }
+    memset(ctx->host_stamps.ptr, 0, sizeof(unsigned) * kStampWords);
     // with `device` current: the per-device address of the allocation-free scratch
     if (int rc = noalloc_scratch_address(ctx, &ctx->noalloc_scratch)) {
         mv2_destroy(ctx);
         return rc;
     }
-    if (hipEventCreate(&ctx->ev_start) != hipSuccess || hipEventCreate(&ctx->ev_stop) != hipSuccess) {
-        delete ctx;
+    if (hipEventCreate(&ctx->ev_start) != hipSuccess || hipEventCreate(&ctx->ev_stop) != hipSuccess ||
+        hipEventCreateWithFlags(&ctx->ev_wait, hipEventDisableTiming) != hipSuccess) {
+        mv2_destroy(ctx);
         return set_error(nullptr, MV2_EDEVICE, "hipEventCreate failed");
     }
     *out = ctx;
@@ -156,8 +203,10 @@ extern "C" void mv2_destroy(mv2_context *ctx)
     release(ctx->frontier);
     release(ctx->patch);
     release(ctx->host_result, true);
+    release(ctx->host_stamps, true);
     if (ctx->ev_start) hipEventDestroy(ctx->ev_start);
     if (ctx->ev_stop) hipEventDestroy(ctx->ev_stop);
+    if (ctx->ev_wait) hipEventDestroy(ctx->ev_wait);
     for (hipEvent_t e : ctx->sub_events) hipEventDestroy(e);
     for (mv2::StepGraph &g : ctx->step_graphs) hipGraphExecDestroy(g.exec);
     if (ctx->capture_stream) hipStreamDestroy(ctx->capture_stream);

@@ -62,6 +62,9 @@ struct mv2_context {
     mv2::Scratch env_moved;               // envelope tiles: interior changed in launch k, [2][tiles] (int32)
     mv2::Scratch nn_mask;                 // warm start: cells moved by the first mass projection (uint8)
     mv2::Scratch host_result;             // pinned host staging
+    mv2::Scratch host_stamps;             // pinned: one completion stamp per workgroup of a call's last launch
+    unsigned stamp_seq = 0;               // last stamp handed to a launch
+    hipEvent_t ev_wait = nullptr;         // marker that stream_wait polls
     mv2::Scratch label_aux;               // labelling partials
     mv2::Scratch nn_partials;             // weight-gradient partials of the surrogate
     mv2::Scratch nn_loss;                 // loss / projection partials
@@ -101,8 +104,19 @@ namespace mv2 {
 int set_error(mv2_context *ctx, int code, const char *fmt, ...);
 int check_hip(mv2_context *ctx, hipError_t err, const char *what);
 int ensure_scratch(mv2_context *ctx, Scratch &s, size_t bytes, bool host_pinned = false);
-// poll a stream until its work is done (blocking sync after 2 ms)
+// poll a marker behind the stream's work until it is done (blocking sync after 2 ms)
 int stream_wait(mv2_context *ctx, hipStream_t st);
+// Completion stamps.  The last launch of a call writes its results to pinned
+// host memory; every thread that wrote one then stamps its own word with the
+// call's number (`host_stamp`), and the host spins on those words instead of
+// on the stream.  next_stamp returns 0 (no stamps: the caller waits on the
+// stream) when more words are asked for than exist.
+constexpr int kStampWords = 256;
+unsigned next_stamp(mv2_context *ctx, int words);
+unsigned *stamp_words(mv2_context *ctx);
+// wait until words 0 .. words - 1 carry `stamp` (stream_wait for stamp 0,
+// after 2 ms, or on a device error)
+int stamp_wait(mv2_context *ctx, hipStream_t st, unsigned stamp, int words);
 // device address of the allocation-free ledger's file-scope partials (mv2_exchange.hip)
 int noalloc_scratch_address(mv2_context *ctx, double **out);
 
@@ -424,6 +438,15 @@ __device__ inline double spread_read_max(const unsigned long long *slots, double
 __device__ inline void spread_clear(unsigned long long *slots)
 {
     if (threadIdx.x < kSpread) slots[threadIdx.x * kSpreadStride] = 0ull;
+}
+
+// Completion stamp, written by a thread after its writes to pinned host
+// memory: the system-scope release makes those writes visible to the host
+// before the stamp (stamp 0: no stamp).  Device memory the launch writes is
+// ordered for later work by the stream, not by the stamp.
+__device__ inline void host_stamp(unsigned *word, unsigned stamp)
+{
+    if (stamp) __hip_atomic_store(word, stamp, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
 }
 
 // ---- launch helpers --------------------------------------------------------------

@@ -388,7 +388,8 @@ __global__ void __launch_bounds__(kBlock) winner_stats_kernel(
 constexpr int kFinalBlock = 1024;   // the fold of up to kLabelParts partials per field
 
 __global__ void __launch_bounds__(kFinalBlock) winner_final_kernel(const double *partials, int nparts, LabelAux *aux,
-                                                                    LabelAux *host)
+                                                                    LabelAux *host, unsigned *stamp_word,
+                                                                    unsigned stamp)
 {
     constexpr int kNW = kFinalBlock / kWave;
     __shared__ double smem[kWinF * kNW];
@@ -418,6 +419,7 @@ __global__ void __launch_bounds__(kFinalBlock) winner_final_kernel(const double 
         for (int k = 0; k < 8; k++) r.diag_i[k] = aux->diag_i[k];
         for (int k = 0; k < 8; k++) r.diag_f[k] = aux->diag_f[k];
         *host = r;
+        host_stamp(stamp_word, stamp);
     }
 }
 
@@ -670,9 +672,11 @@ extern "C" int mv2_avalanche_label(mv2_context *ctx, const mv2_avalanche_config 
     // the result goes straight to pinned host memory (no copy launch)
     LabelAux *hres = (LabelAux *)((char *)ctx->host_result.ptr + 2048);
     static_assert(2048 + sizeof(LabelAux) <= 4096, "the labelling result in the pinned block");
-    hipLaunchKernelGGL(winner_final_kernel, dim3(1), dim3(kFinalBlock), 0, ctx->stream, partials, sgrid, aux, hres);
+    const unsigned stamp = next_stamp(ctx, 1);
+    hipLaunchKernelGGL(winner_final_kernel, dim3(1), dim3(kFinalBlock), 0, ctx->stream, partials, sgrid, aux, hres,
+                       stamp_words(ctx), stamp);
     MV2_HIP(ctx, hipGetLastError());
-    if (int rc_w = stream_wait(ctx, ctx->stream)) return rc_w;
+    if (int rc_w = stamp_wait(ctx, ctx->stream, stamp, 1)) return rc_w;
     LabelAux host;
     memcpy(&host, (const void *)hres, sizeof(LabelAux));
     memcpy(out->diag_i, host.diag_i, sizeof(host.diag_i));

@@ -214,7 +214,8 @@ __global__ void __launch_bounds__(G::BLOCK) relax_kernel(RelaxArgs<R> a, int k)
 // pinned host memory and clears the next call's first group (read by no
 // launch of this call).
 template <class R>
-__global__ void __launch_bounds__(kBlock) relax_end_kernel(RelaxArgs<R> a, int k)
+__global__ void __launch_bounds__(kBlock) relax_end_kernel(RelaxArgs<R> a, int k, unsigned *stamp_word,
+                                                           unsigned stamp)
 {
     __shared__ double smem[1];
     const RelaxControl C = relax_resolve(a, k, smem);
@@ -233,6 +234,7 @@ __global__ void __launch_bounds__(kBlock) relax_end_kernel(RelaxArgs<R> a, int k
             r.max_excess_last = C.last;
             r.complete = C.done;
             *a.out = r;
+            host_stamp(stamp_word, stamp);
         }
     }
 }
@@ -306,9 +308,10 @@ static int launch_relax(mv2_context *ctx, void *height, int ny, int nx, const do
                     hipLaunchKernelGGL((relax_kernel<R, RelaxPair>), dim3(a.tiles), dim3(RelaxPair::BLOCK), 0, st, a, k);
             }
         }
-        hipLaunchKernelGGL(relax_end_kernel<R>, dim3(gend), dim3(kBlock), 0, st, a, k);
+        const unsigned stamp = next_stamp(ctx, 1);
+        hipLaunchKernelGGL(relax_end_kernel<R>, dim3(gend), dim3(kBlock), 0, st, a, k, stamp_words(ctx), stamp);
         MV2_HIP(ctx, hipGetLastError());
-        if (int rc_w = stream_wait(ctx, st)) return rc_w;
+        if (int rc_w = stamp_wait(ctx, st, stamp, 1)) return rc_w;
         if (hres->complete) break;
         // each launch applies at least one iteration: max_iterations + 1
         // launches always end the loop (the last may be a replay)

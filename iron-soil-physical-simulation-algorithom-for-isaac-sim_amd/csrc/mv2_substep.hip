@@ -921,6 +921,7 @@ constexpr int kFoldQ = 2 * kClasses;           // int64 fields: quanta per weigh
 constexpr unsigned kFoldMax = (1u << 3) | (1u << (kSum + 3)) | (3u << (2 * kSum + kDiag));
 constexpr unsigned kFoldMin = (1u << 7) | (1u << (kSum + 7));
 constexpr int kFoldWaves = kFoldF + kFoldQ + 1;
+static_assert(kFoldWaves <= kStampWords, "one completion stamp per fold wave");
 constexpr int kFoldUnroll = 8;
 
 struct FoldOut {
@@ -949,7 +950,8 @@ __device__ inline T lane_fold(const T *src, int n, T ident)
 }
 
 template <class P>
-__global__ void __launch_bounds__(kBlock) v2_fold_kernel(V2Args<P> a, int k, int nparts, int ndiag, FoldOut *out)
+__global__ void __launch_bounds__(kBlock) v2_fold_kernel(V2Args<P> a, int k, int nparts, int ndiag, FoldOut *out,
+                                                          unsigned *stamps, unsigned stamp)
 {
     const int w = blockIdx.x * (kBlock / kWave) + wave_index();
     const int lane = threadIdx.x & (kWave - 1);
@@ -979,6 +981,8 @@ __global__ void __launch_bounds__(kBlock) v2_fold_kernel(V2Args<P> a, int k, int
         for (int i = lane; i < (int)(sizeof(SyncBlock) / 4); i += kWave) ps[i] = 0u;
         for (int i = lane; i < kStepSpreadSets * kSpread; i += kWave) a.spread_next[(int64_t)i * kSpreadStride] = 0ull;
     }
+    // each wave's lane 0 wrote one field of the host record: stamp word w
+    if (w < kFoldWaves && lane == 0) host_stamp(&stamps[w], stamp);
 }
 
 // the step record from the folded values (host side, after the stream sync)
@@ -1072,12 +1076,15 @@ static int enqueue_substeps(mv2_context *ctx, const V2Args<P> &a, const StepLaun
 }
 
 // after launch k - 1: FP32's first launch past the loop's end did the end pass;
-// FP64 runs it as a launch of its own.  Then the fold into the host record.
+// FP64 runs it as a launch of its own.  Then the fold into the host record,
+// whose workgroups stamp `stamps` with `stamp` (0: no stamps).
 template <class P>
-static void enqueue_close(const V2Args<P> &a, const StepLaunch &L, hipStream_t st, int k)
+static void enqueue_close(const V2Args<P> &a, const StepLaunch &L, hipStream_t st, int k, unsigned *stamps,
+                          unsigned stamp)
 {
     if constexpr (!P::kQuant) hipLaunchKernelGGL(v2_end_kernel<P>, dim3(L.gs), dim3(kBlock), 0, st, a, k);
-    hipLaunchKernelGGL(v2_fold_kernel<P>, dim3(L.gfold), dim3(kBlock), 0, st, a, k, L.gs, L.gu, L.fold);
+    hipLaunchKernelGGL(v2_fold_kernel<P>, dim3(L.gfold), dim3(kBlock), 0, st, a, k, L.gs, L.gu, L.fold, stamps,
+                       stamp);
 }
 
 constexpr size_t kStepGraphsMax = 16;   // instantiated sequences kept per context
@@ -1126,7 +1133,7 @@ static int replay_step(mv2_context *ctx, const V2Args<P> &a, const StepLaunch &L
         MV2_HIP(ctx, hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
         enqueue_begin<P>(a, L, cs);
         enqueue_substeps<P>(ctx, a, L, cs, 0, chunk, eager_until, false);
-        enqueue_close<P>(a, L, cs, chunk);
+        enqueue_close<P>(a, L, cs, chunk, nullptr, 0u);
         hipGraph_t graph = nullptr;
         hipError_t e = hipStreamEndCapture(cs, &graph);
         if (e == hipSuccess) e = hipGetLastError();
@@ -1229,7 +1236,11 @@ static int launch_step(mv2_context *ctx, const mv2_step_config *cfg, mv2_step_re
     a.clamp_quarter = cfg->cfl <= 0.25 ? 1 : 0;
     hipStream_t st = ctx->stream;
     const bool timing = ctx->timing != 0;
-    MV2_HIP(ctx, hipEventRecord(ctx->ev_start, st));
+    // plain launches wait on the fold's completion stamps; timing mode (whose
+    // events must have completed before they are read) and graph replays
+    // (whose captured fold carries no stamp) wait on the stream
+    const bool stamped = !timing && !(ctx->graphs && fixed_chunk == 0);
+    if (timing) MV2_HIP(ctx, hipEventRecord(ctx->ev_start, st));
     // this parity's loop state is zero if the previous step completed; an
     // error anywhere below leaves v2_clean at -1 and the next step resets
     if (ctx->v2_clean != parity) hipLaunchKernelGGL(v2_reset_kernel, dim3(1), dim3(kBlock), 0, st, a.sync, a.spread);
@@ -1247,6 +1258,7 @@ static int launch_step(mv2_context *ctx, const mv2_step_config *cfg, mv2_step_re
     if (replayed) k = chunk;
     else enqueue_begin<P>(a, L, st);
     MV2_HIP(ctx, hipGetLastError());
+    unsigned stamp = 0u;
     for (;;) {
         if (!replayed) {
             if (timing && (int)ctx->sub_events.size() < 2) {
@@ -1258,12 +1270,15 @@ static int launch_step(mv2_context *ctx, const mv2_step_config *cfg, mv2_step_re
             }
             if (int rc = enqueue_substeps<P>(ctx, a, L, st, k, chunk, eager_until, timing)) return rc;
             k += chunk;
-            enqueue_close<P>(a, L, st, k);
+            stamp = stamped ? next_stamp(ctx, kFoldWaves) : 0u;
+            enqueue_close<P>(a, L, st, k, stamp_words(ctx), stamp);
             MV2_HIP(ctx, hipGetLastError());
+        } else {
+            stamp = 0u;
         }
         replayed = false;
-        MV2_HIP(ctx, hipEventRecord(ctx->ev_stop, st));
-        if (int rc = stream_wait(ctx, st)) return rc;
+        if (timing) MV2_HIP(ctx, hipEventRecord(ctx->ev_stop, st));
+        if (int rc = stamp_wait(ctx, st, stamp, kFoldWaves)) return rc;
         if (fold->ctl.done && (!P::kQuant || fold->ctl.end_done)) break;
         // more substeps than enqueued: the state so far is valid, enqueue more
         chunk = chunk * 2 > 4 ? chunk * 2 : 4;
@@ -1276,10 +1291,13 @@ static int launch_step(mv2_context *ctx, const mv2_step_config *cfg, mv2_step_re
     ctx->v2_last_sync = parity;
     if (fixed_chunk == 0) ctx->v2_predicted = out->substeps;
     out->launches = k;
-    float ms = 0.0f;
-    MV2_HIP(ctx, hipEventElapsedTime(&ms, ctx->ev_start, ctx->ev_stop));
-    out->kernel_ms = ms;
+    out->kernel_ms = 0.0;
     out->substep_kernel_ms = 0.0;
+    if (timing) {
+        float ms = 0.0f;
+        MV2_HIP(ctx, hipEventElapsedTime(&ms, ctx->ev_start, ctx->ev_stop));
+        out->kernel_ms = ms;
+    }
     if (timing && bracket > 0 && out->substeps == bracket) {
         // substeps are the first launches: the bracket held exactly the working ones
         float e = 0.0f;
