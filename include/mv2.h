@@ -298,6 +298,18 @@ int mv2_relax(mv2_context *ctx, int precision, void *height, int ny, int nx,
               const double crit_diff[4], int max_iterations, double tolerance,
               mv2_relax_result *out);
 
+/* mv2_relax that also reports the maximum excess of iteration mark_iteration
+ * (>= 1), or of the last iteration when the loop ended before it.  It serves
+ * the warm start's "at most N exact sweeps, then fall back to the plain
+ * relaxation if the residual exceeds the threshold" (the learned avalanche
+ * warm start has no counterpart in the reference; its fallback is
+ * relax_critical_slope, slope_model.py:273-343): one call with mark_iteration
+ * = N gives the field, iteration count and convergence of the sweeps followed
+ * by the fallback, and the residual after the sweeps, without a second call. */
+int mv2_relax_marked(mv2_context *ctx, int precision, void *height, int ny, int nx,
+                     const double crit_diff[4], int max_iterations, double tolerance, int mark_iteration,
+                     mv2_relax_result *out, double *max_excess_at_mark);
+
 /* the plain relaxation of mv2_relax (FP32) applied to every instance of the
  * bound batch at once, each instance with its own iteration count and
  * convergence test as if relaxed alone: `field` has the batch's packed layout

@@ -62,7 +62,7 @@ struct mv2_context {
     mv2::Scratch env_moved;               // envelope tiles: interior changed in launch k, [2][tiles] (int32)
     mv2::Scratch nn_mask;                 // warm start: cells moved by the first mass projection (uint8)
     mv2::Scratch host_result;             // pinned host staging
-    mv2::Scratch host_stamps;             // pinned: one completion stamp per workgroup of a call's last launch
+    mv2::Scratch host_stamps;             // pinned: completion stamps of a call's last launch
     unsigned stamp_seq = 0;               // last stamp handed to a launch
     hipEvent_t ev_wait = nullptr;         // marker that stream_wait polls
     mv2::Scratch label_aux;               // labelling partials
@@ -86,6 +86,8 @@ struct mv2_context {
     int relax_clean = 0;                  // the relaxation's loop state and next first group are zero
     unsigned relax_calls = 0;             // completed relaxation calls (parity picks the first group)
     int frontier_predicted = 0;           // frontier rounds to enqueue up front
+    int env_gen = 1;                      // generation of the envelope's launch-0 tile flags (last used)
+    int env_predicted[2] = {0, 0};        // envelope launches the last call needed: batch, single field
     mv2_batch_state *batch = nullptr;     // batched terrains bound to this context
     double *noalloc_scratch = nullptr;    // device address of the allocation-free ledger's partials
     mv2::Scratch patch;                   // FailureZone activation: per-cell flags, partials, result

@@ -53,10 +53,18 @@ struct RelaxControl {
     int prev_src;        // input buffer of the previous launch
     int prev_iterations; // iterations applied before the previous launch
     int replay;          // > 0: re-apply this many iterations to `src`, then end the loop
+    int marked_set;      // the maximum excess of iteration `mark` has been tested
     double last;         // maximum excess of the last applied iteration
+    double marked;       // maximum excess of iteration `mark`
 };
 struct RelaxSync {
     RelaxControl ctl[2];
+};
+
+// the end pass's record in pinned host memory
+struct RelaxHostOut {
+    mv2_relax_result r;
+    double marked;       // maximum excess of iteration `mark`, or of the last one if the loop ended before
 };
 
 template <class R>
@@ -69,7 +77,8 @@ struct RelaxArgs {
     RelaxSync *sync;
     unsigned long long *spread;   // (kRelaxGroups + 2) x kRelaxKMax spread-maximum sets
     int first_group;              // group of launch 0 (3 or 4); the end pass clears the other
-    mv2_relax_result *out;        // pinned host memory: the end pass writes the result there
+    int mark;                     // iteration whose maximum excess is reported (0: none)
+    RelaxHostOut *out;            // pinned host memory: the end pass writes the result there
     int tiles_x, tiles;
     int32_t *tchanged;            // [2][tiles]: the tile's interior changed in launch k (parity k & 1)
     int32_t *titers;              // [tiles]: iterations of the tile's last computed launch
@@ -109,6 +118,10 @@ __device__ inline RelaxControl relax_resolve(const RelaxArgs<R> &a, int k, doubl
                 mfirst = m;
             }
             mlast = m;
+            if (C.prev_iterations + i + 1 == a.mark && (first < 0 || first == i)) {
+                C.marked = m;
+                C.marked_set = 1;
+            }
         }
         if (first == C.n_prev - 1) {
             C.converged = 1;
@@ -228,11 +241,12 @@ __global__ void __launch_bounds__(kBlock) relax_end_kernel(RelaxArgs<R> a, int k
     if (blockIdx.x == 0) {
         for (int i = 0; i < kRelaxKMax; i++) spread_clear(relax_set(a.spread, a.first_group == 3 ? 4 : 3, i));
         if (threadIdx.x == 0) {
-            mv2_relax_result r{};
-            r.iterations = C.iterations;
-            r.converged = C.converged;
-            r.max_excess_last = C.last;
-            r.complete = C.done;
+            RelaxHostOut r{};
+            r.r.iterations = C.iterations;
+            r.r.converged = C.converged;
+            r.r.max_excess_last = C.last;
+            r.r.complete = C.done;
+            r.marked = C.marked_set ? C.marked : C.last;
             *a.out = r;
             host_stamp(stamp_word, stamp);
         }
@@ -249,7 +263,8 @@ __global__ void zero_relax_sync(RelaxSync *s, unsigned long long *spread)
 
 template <class R>
 static int launch_relax(mv2_context *ctx, void *height, int ny, int nx, const double crit_diff[4],
-                        int max_iterations, double tolerance, mv2_relax_result *out)
+                        int max_iterations, double tolerance, mv2_relax_result *out, int mark = 0,
+                        double *marked_out = nullptr)
 {
     const size_t n = (size_t)ny * nx;
     if (int rc = ensure_scratch(ctx, ctx->relax_alt, n * sizeof(R))) return rc;
@@ -266,8 +281,9 @@ static int launch_relax(mv2_context *ctx, void *height, int ny, int nx, const do
     static_assert(sizeof(RelaxSync) <= 1024, "relaxation loop state");
     if (int rc = ensure_scratch(ctx, ctx->spread, kSpreadScratchBytes)) return rc;
     a.spread = (unsigned long long *)ctx->spread.ptr + (size_t)kRelaxSpreadSet * kSpreadWords;
-    a.out = (mv2_relax_result *)((char *)ctx->host_result.ptr + 2048);
-    static_assert(2048 + sizeof(mv2_relax_result) <= 4096, "the relaxation result in the pinned block");
+    a.mark = mark;
+    a.out = (RelaxHostOut *)((char *)ctx->host_result.ptr + 2048);
+    static_assert(2048 + sizeof(RelaxHostOut) <= 3072, "the relaxation result in the pinned block");
     a.first_group = 3 + (ctx->relax_calls & 1);
     a.tiles_x = (nx + kRelaxTileW - 1) / kRelaxTileW;
     a.tiles = a.tiles_x * ((ny + kRelaxTileH - 1) / kRelaxTileH);
@@ -291,7 +307,8 @@ static int launch_relax(mv2_context *ctx, void *height, int ny, int nx, const do
     int chunk = ctx->relax_predicted <= 1 ? 1 : ctx->relax_predicted + 1;
     if (chunk > 32) chunk = 32;
     int k = 0;
-    const volatile mv2_relax_result *hres = a.out;
+    const volatile RelaxHostOut *hout = a.out;
+    const volatile mv2_relax_result *hres = &hout->r;
     // two iterations per launch only while every workgroup of a launch runs at
     // once (at most 2 per CU; 701^2: 240 tiles): a larger grid runs in rounds,
     // where the two-iteration tile's 1.4x larger LDS area costs more than the
@@ -324,6 +341,7 @@ static int launch_relax(mv2_context *ctx, void *height, int ny, int nx, const do
     out->max_excess_last = hres->max_excess_last;
     out->complete = hres->complete;
     out->launches = k;
+    if (marked_out) *marked_out = hout->marked;
     ctx->relax_clean = 1;
     ctx->relax_calls++;
     // launches that applied iterations: one, then pairs (FP32, small grids) or
@@ -402,6 +420,26 @@ extern "C" int mv2_relax(mv2_context *ctx, int precision, void *height, int ny, 
     return precision == MV2_FP64
                ? launch_relax<double>(ctx, height, ny, nx, crit_diff, max_iterations, tolerance, out)
                : launch_relax<float>(ctx, height, ny, nx, crit_diff, max_iterations, tolerance, out);
+}
+
+extern "C" int mv2_relax_marked(mv2_context *ctx, int precision, void *height, int ny, int nx,
+                                const double crit_diff[4], int max_iterations, double tolerance, int mark_iteration,
+                                mv2_relax_result *out, double *max_excess_at_mark)
+{
+    if (!max_excess_at_mark) return set_error(ctx, MV2_EINVAL, "[MiniSlope] null argument");
+    if (mark_iteration < 1) return set_error(ctx, MV2_EINVAL, "[MiniSlope] mark_iteration must be >= 1");
+    if (!ctx) return set_error(nullptr, MV2_EINVAL, "[MiniSlope] null context");
+    if (!height || !crit_diff || !out) return set_error(ctx, MV2_EINVAL, "[MiniSlope] null argument");
+    if (ny < 2 || nx < 2)
+        return set_error(ctx, MV2_EINVAL, "height must be a 2-D array with both dimensions >= 2");
+    if (max_iterations < 1) return set_error(ctx, MV2_EINVAL, "[MiniSlope] max_iterations must be >= 1");
+    if (precision != MV2_FP64 && precision != MV2_FP32)
+        return set_error(ctx, MV2_EINVAL, "[MiniSlope] unknown precision %d", precision);
+    hipSetDevice(ctx->device);
+    return precision == MV2_FP64 ? launch_relax<double>(ctx, height, ny, nx, crit_diff, max_iterations, tolerance,
+                                                        out, mark_iteration, max_excess_at_mark)
+                                 : launch_relax<float>(ctx, height, ny, nx, crit_diff, max_iterations, tolerance,
+                                                       out, mark_iteration, max_excess_at_mark);
 }
 
 extern "C" int mv2_frontier_relax(mv2_context *ctx, double *height, const double *weights, int32_t *reached,

@@ -842,29 +842,62 @@ __global__ void __launch_bounds__(kBlock) delta_stats_kernel(const float *b, Pro
     }
 }
 
-// fold the partials of segment blockIdx.x (one workgroup, fixed order) into its shift
-__global__ void __launch_bounds__(kBlock) delta_shift_kernel(const double *partials, int nparts, float *shift)
+constexpr int kProjectParts = 1024;           // workgroups per segment of the projection passes at most
+constexpr int kProjectPer = kProjectParts / kBlock;
+
+// The shift of the current pass, folded by every workgroup from the previous
+// pass's partials in one fixed order (thread t sums partials t, t + kBlock,
+// ... from zero, then the block tree), so all workgroups of a segment hold the
+// same bits.  correction == 0: the first shift, sum of the change / moved
+// cells; otherwise the Newton step on the gained mass from the previous shift
+// (kept in shifts[(correction - 1) * segments + segment]): shift += gain /
+// cells above zero.  Workgroup 0 of the segment stores the result in slot
+// `correction` for the next pass.  Every thread receives it.
+__device__ inline float fold_shift(const double *partials, int nparts, float *shifts, int correction, double *smem)
 {
-    __shared__ double smem[8];
-    partials += (int64_t)blockIdx.x * 2 * nparts;
+    __shared__ float shift_lds;    // its own word: smem is reused by the caller's reduction
+    const int seg = blockIdx.y, segments = gridDim.y;
+    partials += (int64_t)seg * 2 * nparts;
+    double v0[kProjectPer], v1[kProjectPer];
+#pragma unroll
+    for (int u = 0; u < kProjectPer; u++) {
+        const int k = threadIdx.x + u * kBlock;
+        v0[u] = k < nparts ? partials[k] : 0.0;
+        v1[u] = k < nparts ? partials[nparts + k] : 0.0;
+    }
     double s[2] = {0.0, 0.0};
-    for (int k = threadIdx.x; k < nparts; k += blockDim.x) {
-        s[0] += partials[k];
-        s[1] += partials[nparts + k];
+#pragma unroll
+    for (int u = 0; u < kProjectPer; u++) {
+        s[0] += v0[u];
+        s[1] += v1[u];
     }
     block_reduce<double, 2, 0>(s, smem);
-    if (threadIdx.x == 0) shift[blockIdx.x] = s[1] > 0.0 ? (float)(s[0] / s[1]) : 0.0f;
+    if (threadIdx.x == 0) {
+        float shift;
+        if (correction == 0) {
+            shift = s[1] > 0.0 ? (float)(s[0] / s[1]) : 0.0f;
+        } else {
+            shift = shifts[(correction - 1) * segments + seg];
+            if (s[1] > 0.0) shift = (float)((double)shift + s[0] / s[1]);
+        }
+        if (blockIdx.x == 0) shifts[correction * segments + seg] = shift;
+        shift_lds = shift;
+    }
+    __syncthreads();
+    return shift_lds;
 }
 
 // mass the field gains at the current shift (clamp at zero included; masked:
 // with the fixed change outside M), and the moved cells that stay above zero
-// (same arithmetic as apply_delta_kernel)
+// (same arithmetic as apply_delta_kernel); the shift comes from the previous
+// pass's partials (`correction` as in fold_shift)
 template <bool FROM_SURFACE, bool MASKED>
-__global__ void __launch_bounds__(kBlock) clamp_stats_kernel(const float *b, ProjectArgs a, const float *shift_ptr,
+__global__ void __launch_bounds__(kBlock) clamp_stats_kernel(const float *b, ProjectArgs a, float *shifts,
+                                                             int correction, const double *partials_in,
                                                              double *partials)
 {
     __shared__ double smem[8];
-    const float shift = shift_ptr[blockIdx.y];
+    const float shift = fold_shift(partials_in, gridDim.x, shifts, correction, smem);
     const int64_t base = (int64_t)blockIdx.y * a.n;
     b += base;
     const float *src = a.src + base;
@@ -888,24 +921,12 @@ __global__ void __launch_bounds__(kBlock) clamp_stats_kernel(const float *b, Pro
     }
 }
 
-// Newton step on the gained mass of segment blockIdx.x: shift += gain / cells above zero
-__global__ void __launch_bounds__(kBlock) clamp_shift_kernel(const double *partials, int nparts, float *shift)
+template <bool FROM_SURFACE, bool MASKED>
+__global__ void __launch_bounds__(kBlock) apply_delta_kernel(float *b, ProjectArgs a, float *shifts, int correction,
+                                                             const double *partials_in)
 {
     __shared__ double smem[8];
-    partials += (int64_t)blockIdx.x * 2 * nparts;
-    double s[2] = {0.0, 0.0};
-    for (int k = threadIdx.x; k < nparts; k += blockDim.x) {
-        s[0] += partials[k];
-        s[1] += partials[nparts + k];
-    }
-    block_reduce<double, 2, 0>(s, smem);
-    if (threadIdx.x == 0 && s[1] > 0.0) shift[blockIdx.x] = (float)((double)shift[blockIdx.x] + s[0] / s[1]);
-}
-
-template <bool FROM_SURFACE, bool MASKED>
-__global__ void __launch_bounds__(kBlock) apply_delta_kernel(float *b, ProjectArgs a, const float *shift_ptr)
-{
-    const float shift = shift_ptr[blockIdx.y];
+    const float shift = fold_shift(partials_in, gridDim.x, shifts, correction, smem);
     const int64_t base = (int64_t)blockIdx.y * a.n;
     b += base;
     const float *src = a.src + base;
@@ -1040,7 +1061,7 @@ __global__ void __launch_bounds__(kEnvWaves * kWave) envelope_kernel(float *z, i
                                                                       float c2, float c3, int tiles_x, int tiles,
                                                                       EnvControl *ctl, unsigned long long *spread,
                                                                       int32_t *tile_moved, int k, int seam,
-                                                                      int max_cycles)
+                                                                      int max_cycles, int first_gen)
 {
     __shared__ float lds[kEnvWaves][kEnvRegion * kEnvStride];
     __shared__ double smem[1];
@@ -1070,16 +1091,20 @@ __global__ void __launch_bounds__(kEnvWaves * kWave) envelope_kernel(float *z, i
     // no interior of those tiles changed in launch k - 1, nothing wrote into
     // the region during it, so its input equals that of the last launch that
     // processed it, which changed nothing: it is skipped.  Launch 0 reads the
-    // flags of envelope_flags_kernel instead (parity 1): a region with no cell
-    // above a neighbour plus its limit in its tile neighbourhood changes
-    // nothing.
+    // flags of envelope_prep_kernel instead (parity 1, the call's generation
+    // number first_gen): a region with no cell above a neighbour plus its
+    // limit in its tile neighbourhood changes nothing.  first_gen == 0: launch
+    // 0 runs every region.
     bool quiet = false;
-    if (tile < tiles) {
+    if (tile < tiles && !(k == 0 && first_gen == 0)) {
         const int ty = tiles / tiles_x, trow = tile / tiles_x, tcol = tile % tiles_x;
         int flag = 0;
         if (lane < 9) {
             const int rr = trow + lane / 3 - 1, cc = tcol + lane % 3 - 1;
-            if (rr >= 0 && rr < ty && cc >= 0 && cc < tiles_x) flag = tile_moved[((k - 1) & 1) * tiles + rr * tiles_x + cc];
+            if (rr >= 0 && rr < ty && cc >= 0 && cc < tiles_x) {
+                const int32_t f = tile_moved[((k - 1) & 1) * tiles + rr * tiles_x + cc];
+                flag = k == 0 ? (f == first_gen) : f;
+            }
         }
         quiet = !__any(flag);
     }
@@ -1122,50 +1147,53 @@ __global__ void __launch_bounds__(kEnvWaves * kWave) envelope_kernel(float *z, i
     if (threadIdx.x == 0 && changed) spread_max(spread + (k % 3) * kSpreadWords, 1.0);
 }
 
-// launch 0's flags (parity 1): cleared to 0 and then set for every tile with a
-// cell that an update would lower (z(q) + lim < z(p) for a neighbour q across
-// an edge the envelope uses; same float arithmetic), or all set to 1 (every
-// region runs)
-__global__ void __launch_bounds__(kBlock) envelope_flags_clear_kernel(int32_t *flags, int tiles, int value)
+// The call's set-up in one launch: z = b + delta when delta is given (else z
+// holds the surface already); the loop state and spread sets zeroed
+// (workgroup 0); with gen != 0, launch 0's flags (parity 1) set to gen for
+// every tile with a cell that an update would lower (z(q) + lim < z(p) for a
+// neighbour q across an edge the envelope uses; same float arithmetic, the
+// neighbours' z formed from b + delta as above).  Flags of other tiles keep
+// older values, which differ from gen: the generations of a context never
+// repeat 0 or 1, the values the envelope launches write.
+__global__ void __launch_bounds__(kBlock) envelope_prep_kernel(const float *b, const float *delta, float *z, int ny,
+                                                               int nx, float c0, float c1, float c2, float c3,
+                                                               int tiles_x, int tiles, int seam, int32_t *flags,
+                                                               int gen, EnvControl *ctl, unsigned long long *spread)
 {
-    for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < tiles; t += gridDim.x * blockDim.x)
-        flags[tiles + t] = value;
-}
-
-__global__ void __launch_bounds__(kBlock) envelope_flags_kernel(const float *z, int ny, int nx, float c0, float c1,
-                                                                float c2, float c3, int tiles_x, int tiles,
-                                                                int seam, int32_t *flags)
-{
+    if (blockIdx.x == 0) {
+        if (threadIdx.x < 2) ctl[threadIdx.x] = EnvControl{0, 0, {0, 0}};
+        for (int i = threadIdx.x; i < 3 * kSpread; i += blockDim.x) spread[(int64_t)i * kSpreadStride] = 0ull;
+    }
     const int64_t n = (int64_t)ny * nx;
+    auto at = [&](int64_t q) -> float { return delta ? b[q] + delta[q] : z[q]; };
     for (int64_t p = (int64_t)xcd_block() * kBlock + threadIdx.x; p < n; p += (int64_t)gridDim.x * kBlock) {
+        const float v = at(p);
+        if (delta) z[p] = v;
+        if (!gen) continue;
         const int r = (int)(p / nx), c = (int)(p - (int64_t)r * nx);
-        const float v = z[p];
         const bool up_ok = r > 0 && !(seam > 0 && r % seam == 0);
         const bool dn_ok = r + 1 < ny && !(seam > 0 && (r + 1) % seam == 0);
         bool low = false;
-        if (c > 0) low |= z[p - 1] + c0 < v;
-        if (c + 1 < nx) low |= z[p + 1] + c0 < v;
+        if (c > 0) low |= at(p - 1) + c0 < v;
+        if (c + 1 < nx) low |= at(p + 1) + c0 < v;
         if (up_ok) {
-            low |= z[p - nx] + c1 < v;
-            if (c > 0) low |= z[p - nx - 1] + c2 < v;
-            if (c + 1 < nx) low |= z[p - nx + 1] + c3 < v;
+            low |= at(p - nx) + c1 < v;
+            if (c > 0) low |= at(p - nx - 1) + c2 < v;
+            if (c + 1 < nx) low |= at(p - nx + 1) + c3 < v;
         }
         if (dn_ok) {
-            low |= z[p + nx] + c1 < v;
-            if (c + 1 < nx) low |= z[p + nx + 1] + c2 < v;
-            if (c > 0) low |= z[p + nx - 1] + c3 < v;
+            low |= at(p + nx) + c1 < v;
+            if (c + 1 < nx) low |= at(p + nx + 1) + c2 < v;
+            if (c > 0) low |= at(p + nx - 1) + c3 < v;
         }
-        if (low) flags[tiles + (r / kEnvTile) * tiles_x + c / kEnvTile] = 1;   // every writer stores 1
+        if (low) flags[tiles + (r / kEnvTile) * tiles_x + c / kEnvTile] = gen;   // every writer stores gen
     }
 }
 
-__global__ void envelope_reset_kernel(EnvControl *ctl, unsigned long long *spread)
-{
-    if (threadIdx.x < 2) ctl[threadIdx.x] = EnvControl{0, 0, {0, 0}};
-    for (int i = threadIdx.x; i < 3 * kSpread; i += blockDim.x) spread[(int64_t)i * kSpreadStride] = 0ull;
-}
-
-__global__ void envelope_result_kernel(const EnvControl *ctl, const unsigned long long *spread, int k, int *out)
+// the loop's state after launch k - 1, written to pinned host memory, then the
+// completion stamp
+__global__ void envelope_result_kernel(const EnvControl *ctl, const unsigned long long *spread, int k, int *out,
+                                       unsigned *stamp_word, unsigned stamp)
 {
     __shared__ double smem[1];
     EnvControl C = ctl[k & 1];
@@ -1173,6 +1201,7 @@ __global__ void envelope_result_kernel(const EnvControl *ctl, const unsigned lon
     if (threadIdx.x == 0) {
         out[0] = C.launches;
         out[1] = C.done;
+        host_stamp(stamp_word, stamp);
     }
 }
 
@@ -1365,26 +1394,30 @@ extern "C" int mv2_relax_patches(mv2_context *ctx, float *h, int npatch, int H, 
 
 constexpr int kClampCorrections = 4;
 
+// Mass projection in kClampCorrections + 2 launches: the change statistics,
+// kClampCorrections clamp passes and the apply pass, each of which folds the
+// shift it uses from the previous pass's partials (ping-pong partial buffers,
+// one shift slot per pass) instead of a one-workgroup shift launch between
+// the passes.
 template <bool FROM_SURFACE, bool MASKED = false>
 static int project_change(mv2_context *ctx, float *b, const float *src, int64_t n, double eps, int segments = 1,
                           const uint8_t *mask_in = nullptr, uint8_t *mask_out = nullptr)
 {
-    const int grid = grid_for(n, kBlock, 1024);     // per segment: the same partials as one field
+    const int grid = grid_for(n, kBlock, kProjectParts);    // per segment: the same partials as one field
     const size_t parts_n = (size_t)2 * grid * segments;
-    if (int rc = ensure_scratch(ctx, ctx->nn_loss, sizeof(double) * parts_n + sizeof(float) * segments + 64)) return rc;
-    double *parts = (double *)ctx->nn_loss.ptr;
-    float *shift = (float *)(parts + parts_n);
+    const size_t shift_n = (size_t)(kClampCorrections + 1) * segments;
+    if (int rc = ensure_scratch(ctx, ctx->nn_loss, sizeof(double) * 2 * parts_n + sizeof(float) * shift_n + 64))
+        return rc;
+    double *parts[2] = {(double *)ctx->nn_loss.ptr, (double *)ctx->nn_loss.ptr + parts_n};
+    float *shifts = (float *)(parts[1] + parts_n);
     const ProjectArgs a{src, n, (float)eps, mask_in, mask_out};
     hipLaunchKernelGGL((delta_stats_kernel<FROM_SURFACE, MASKED>), dim3(grid, segments), dim3(kBlock), 0, ctx->stream,
-                       b, a, parts);
-    hipLaunchKernelGGL(delta_shift_kernel, dim3(segments), dim3(kBlock), 0, ctx->stream, parts, grid, shift);
-    for (int c = 0; c < kClampCorrections; c++) {
+                       b, a, parts[0]);
+    for (int c = 0; c < kClampCorrections; c++)
         hipLaunchKernelGGL((clamp_stats_kernel<FROM_SURFACE, MASKED>), dim3(grid, segments), dim3(kBlock), 0,
-                           ctx->stream, b, a, shift, parts);
-        hipLaunchKernelGGL(clamp_shift_kernel, dim3(segments), dim3(kBlock), 0, ctx->stream, parts, grid, shift);
-    }
+                           ctx->stream, b, a, shifts, c, parts[c & 1], parts[(c + 1) & 1]);
     hipLaunchKernelGGL((apply_delta_kernel<FROM_SURFACE, MASKED>), dim3(grid, segments), dim3(kBlock), 0, ctx->stream,
-                       b, a, shift);
+                       b, a, shifts, kClampCorrections, parts[kClampCorrections & 1]);
     MV2_HIP(ctx, hipGetLastError());
     return MV2_OK;
 }
@@ -1485,25 +1518,29 @@ static int envelope_run(mv2_context *ctx, const float *b, const float *delta, fl
     if (int rc = ensure_scratch(ctx, ctx->spread, kSpreadScratchBytes)) return rc;
     unsigned long long *spread = (unsigned long long *)ctx->spread.ptr + (size_t)kEnvSpreadSet * kSpreadWords;
     EnvControl *ctl = (EnvControl *)((char *)ctx->sync.ptr + 2048);
-    int *dres = (int *)((char *)ctx->result.ptr + 3800);
-    if (delta) hipLaunchKernelGGL(add_surface_kernel, dim3(grid_for(n, kBlock, 2048)), dim3(kBlock), 0, st, b, delta, z, n);
-    hipLaunchKernelGGL(envelope_reset_kernel, dim3(1), dim3(kBlock), 0, st, ctl, spread);
     const int tiles_x = (nx + kEnvTile - 1) / kEnvTile;
     const int tiles = tiles_x * ((ny + kEnvTile - 1) / kEnvTile);
     const int blocks = (tiles + kEnvWaves - 1) / kEnvWaves;
     if (int rc = ensure_scratch(ctx, ctx->env_moved, 2 * sizeof(int32_t) * (size_t)tiles)) return rc;
     int32_t *tile_moved = (int32_t *)ctx->env_moved.ptr;
     // launch 0 of one field runs only the regions next to a cell that an update
-    // would lower (a flag pass: 701^2 learned relax-to-rest 0.50 against
-    // 0.52 ms); for a batch of fields, whose predictions leave such cells in
-    // most regions, the pass costs more than it skips (64 x 256^2: 22 us per
-    // pass against no shorter first launch), so every region runs
-    hipLaunchKernelGGL(envelope_flags_clear_kernel, dim3((tiles + kBlock - 1) / kBlock), dim3(kBlock), 0, st,
-                       tile_moved, tiles, prior ? 0 : 1);
-    if (prior)
-        hipLaunchKernelGGL(envelope_flags_kernel, dim3(grid_for(n, kBlock, 4096)), dim3(kBlock), 0, st, z, ny, nx,
-                           (float)crit_diff[0], (float)crit_diff[1], (float)crit_diff[2], (float)crit_diff[3],
-                           tiles_x, tiles, seam, tile_moved);
+    // would lower (flags from the prep pass: 701^2 learned relax-to-rest 0.50
+    // against 0.52 ms); for a batch of fields, whose predictions leave such
+    // cells in most regions, the flag test costs more than it skips (64 x
+    // 256^2: 22 us per pass against no shorter first launch), so every region
+    // runs (generation 0)
+    int gen = 0;
+    if (prior) {
+        gen = ++ctx->env_gen;
+        if (gen < 2) gen = ctx->env_gen = 2;       // 0 and 1 are the values the envelope launches write
+    }
+    // the prep pass reads b + delta at the neighbours of a cell while writing
+    // z at the cell: with z aliasing delta that needs the surface in place first
+    const bool alias = delta && z == delta && gen != 0;
+    if (alias) hipLaunchKernelGGL(add_surface_kernel, dim3(grid_for(n, kBlock, 2048)), dim3(kBlock), 0, st, b, delta, z, n);
+    hipLaunchKernelGGL(envelope_prep_kernel, dim3(grid_for(n, kBlock, 4096)), dim3(kBlock), 0, st, alias ? nullptr : b,
+                       alias ? nullptr : delta, z, ny, nx, (float)crit_diff[0], (float)crit_diff[1],
+                       (float)crit_diff[2], (float)crit_diff[3], tiles_x, tiles, seam, tile_moved, gen, ctl, spread);
     // cycles of the four passes per region and launch: more settle a region
     // in one launch, fewer bound the launch by its slowest region (a launch
     // lasts as long as its busiest region); MV2_ENV_CYCLES = 1..6 overrides
@@ -1512,21 +1549,28 @@ static int envelope_run(mv2_context *ctx, const float *b, const float *delta, fl
         const int forced = atoi(e);
         if (forced >= 1 && forced <= kEnvMaxCycles) cycles = forced;
     }
-    int k = 0, chunk = 4;
-    const int *hres = (const int *)ctx->host_result.ptr;
+    // as many launches as the last call of the same kind needed (the result
+    // pass after them finds the loop ended when the last one moved nothing),
+    // then doubling rounds
+    int &predicted = ctx->env_predicted[prior ? 1 : 0];
+    int k = 0, chunk = predicted > 0 ? predicted : 4;
+    int *hres = (int *)((char *)ctx->host_result.ptr + 3072);
+    static_assert(3072 + 2 * sizeof(int) <= 4096, "the envelope result in the pinned block");
     for (;;) {
         if (chunk > max_launches - k) chunk = max_launches - k;
         for (int j = 0; j < chunk; j++, k++)
             hipLaunchKernelGGL(envelope_kernel, dim3(blocks), dim3(kEnvWaves * kWave), 0, st, z, ny, nx,
                                (float)crit_diff[0], (float)crit_diff[1], (float)crit_diff[2], (float)crit_diff[3],
-                               tiles_x, tiles, ctl, spread, tile_moved, k, seam, cycles);
-        hipLaunchKernelGGL(envelope_result_kernel, dim3(1), dim3(kWave), 0, st, ctl, spread, k, dres);
+                               tiles_x, tiles, ctl, spread, tile_moved, k, seam, cycles, gen);
+        const unsigned stamp = next_stamp(ctx, 1);
+        hipLaunchKernelGGL(envelope_result_kernel, dim3(1), dim3(kWave), 0, st, ctl, spread, k, hres,
+                           stamp_words(ctx), stamp);
         MV2_HIP(ctx, hipGetLastError());
-        MV2_HIP(ctx, hipMemcpyAsync(ctx->host_result.ptr, dres, 2 * sizeof(int), hipMemcpyDeviceToHost, st));
-        if (int rc_w = stream_wait(ctx, st)) return rc_w;
+        if (int rc_w = stamp_wait(ctx, st, stamp, 1)) return rc_w;
         if (hres[1] || k >= max_launches) break;
         chunk = 2 * chunk;
     }
+    if (hres[1]) predicted = hres[0];
     if (launches_out) launches_out[0] = hres[0];
     if (!hres[1]) return set_error(ctx, MV2_ENUMERIC, "[WarmStart] envelope did not converge in %d launches", max_launches);
     return MV2_OK;

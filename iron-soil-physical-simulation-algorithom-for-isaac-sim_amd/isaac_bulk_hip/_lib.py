@@ -149,6 +149,7 @@ SIGNATURES: dict[str, tuple[object, list[object]]] = {
     "mv2_surface_gather_tiles": (_I, [_P, _P, _I, _I, _I, _I, _P]),
     "mv2_deposition_step": (_I, [_P, ctypes.POINTER(DepositionConfig), _P, _P, _P, _P, _P, _P]),
     "mv2_relax": (_I, [_P, _I, _P, _I, _I, _P, _I, _D, ctypes.POINTER(RelaxResult)]),
+    "mv2_relax_marked": (_I, [_P, _I, _P, _I, _I, _P, _I, _D, _I, ctypes.POINTER(RelaxResult), _P]),
     "mv2_frontier_relax": (_I, [_P, _P, _P, _P, _P, _P, _P, _P, _I, _I, _I, _I, _I, _I, _D, _P,
                                 ctypes.POINTER(_I32)]),
     "mv2_frontier_scan": (_I, [_P, _P, _P, _P, _P, _P, _P, _P, _I, _I, _I, _D, _D, _P]),

@@ -26,7 +26,7 @@ from dataclasses import dataclass
 import numpy as np
 
 from . import _lib
-from .relaxation import legacy_critical_differences, relax_device
+from .relaxation import legacy_critical_differences
 from .runtime import HipRuntime
 
 DILATIONS = (1, 2, 4, 8, 16)
@@ -105,6 +105,7 @@ class WarmStartNet:
                 "feat": torch.zeros((n, 4), dtype=torch.float32, device=dev),
                 "y": [None] + [torch.zeros((n, 32), dtype=bf, device=dev) for _ in range(4)],
                 "out": torch.zeros(n, dtype=torch.float32, device=dev),
+                "surface": torch.zeros(n, dtype=torch.float32, device=dev),
                 "g5": torch.zeros(n, dtype=torch.float32, device=dev),
                 "g5x": torch.zeros((n, 16), dtype=bf, device=dev),
                 "dz": [torch.zeros((n, 32), dtype=bf, device=dev) for _ in range(2)],
@@ -263,6 +264,22 @@ class WarmStartResult:
     settle_launches: int = 0
 
 
+def _relax_marked(rt: HipRuntime, b_field, dx: float, critical_angle_deg: float, max_iterations: int,
+                  tolerance: float, mark: int) -> tuple[int, bool, float]:
+    """Plain relaxation of a device float32 [H, W] field in place (relax_device
+    semantics, closed boundary); returns (iterations, converged, max excess of
+    iteration `mark`, or of the last one when the loop ended before it)."""
+    H, W = b_field.shape
+    crit = legacy_critical_differences(dx, dx, critical_angle_deg)
+    res = _lib.RelaxResult()
+    marked = ctypes.c_double(0.0)
+    rt.bind_stream()
+    rt.check(rt.lib.mv2_relax_marked(rt.ctx, _lib.MV2_FP32, b_field.data_ptr(), int(H), int(W), crit.ctypes.data,
+                                     int(max_iterations), float(tolerance), int(mark), ctypes.byref(res),
+                                     ctypes.byref(marked)))
+    return int(res.iterations), bool(res.converged), float(marked.value)
+
+
 def warm_start_relax(net: WarmStartNet, b_field, h_field, dx: float, critical_angle_deg: float,
                      tolerance: float, *, max_iterations: int = 20000, sweeps: int = 4,
                      fallback_threshold: float = 1e-6, eps: float = 1e-5,
@@ -293,20 +310,22 @@ def warm_start_relax(net: WarmStartNet, b_field, h_field, dx: float, critical_an
     env_deg = critical_angle_deg if envelope_angle_deg is None else envelope_angle_deg
     crit = legacy_critical_differences(dx, dx, env_deg)
     launches = (ctypes.c_int * 1)()
-    zbuf = pred.view(H, W)                      # the surface overwrites the prediction in place
+    # the surface gets its own buffer: an envelope whose surface aliases the
+    # prediction needs one more pass (b + delta in place before the flag test)
+    zbuf = net._buffers(1, H, W)["surface"].view(H, W)
     rt.check(rt.lib.mv2_envelope(rt.ctx, b_field.data_ptr(), pred.data_ptr(), zbuf.data_ptr(), H, W,
                                  crit.ctypes.data, 256, launches))
     settle = (ctypes.c_int * 1)()
     rt.check(rt.lib.mv2_nn_commit_settled(rt.ctx, b_field.data_ptr(), zbuf.data_ptr(), H, W, crit.ctypes.data, eps,
                                           int(settle_rounds), 256, settle))
-    it, conv, last = relax_device(b_field, dx, dx, critical_angle_deg, sweeps, tolerance, runtime=rt)
-    fb_it = 0
-    used = False
-    if not conv and last > fallback_threshold:
-        used = True
-    if not conv:
-        fb_it, conv, last2 = relax_device(b_field, dx, dx, critical_angle_deg, max_iterations, tolerance,
-                                          runtime=rt)
+    # the sweeps and the fallback in one relaxation: the same iterations as a
+    # call limited to `sweeps` followed by one from its result (each
+    # iteration depends on the field alone), plus the residual after the sweeps
+    total, conv, last = _relax_marked(rt, b_field, dx, critical_angle_deg, max_iterations + sweeps, tolerance,
+                                      sweeps)
+    it = min(total, sweeps)
+    fb_it = total - it
+    used = total > sweeps and last > fallback_threshold
     torch.cuda.synchronize(rt.device)
     return WarmStartResult(it, fb_it, conv, used, last, (time.perf_counter() - t0) * 1e3, int(launches[0]),
                            int(settle[0]))

@@ -83,6 +83,41 @@ def test_relax_iteration_limits(precision):
         np.testing.assert_array_equal(t.cpu().numpy(), o)
 
 
+@pytest.mark.parametrize("precision", ["fp32", "fp64"])
+def test_relax_marked_reports_the_excess_of_the_marked_iteration(precision):
+    """mv2_relax_marked: the same field, iteration count and convergence as
+    the plain relaxation, and the maximum excess of iteration `mark` -- what a
+    call limited to `mark` iterations returns as its last excess (the oracle
+    with that limit), also when the loop converges before or at the mark."""
+    import ctypes
+
+    import torch
+    from isaac_bulk_hip import _lib
+    from isaac_bulk_hip.relaxation import _runtime, legacy_critical_differences
+
+    rt = _runtime("cuda:0")
+    dtype = torch.float64 if precision == "fp64" else torch.float32
+    npdt = np.float64 if precision == "fp64" else np.float32
+    prec = _lib.MV2_FP64 if precision == "fp64" else _lib.MV2_FP32
+    crit_t = float(np.tan(np.deg2rad(30.0)))
+    crit = legacy_critical_differences(0.05, 0.05, 30.0)
+    h = np.random.default_rng(12).uniform(0.0, 1.5, size=(61, 90)).astype(npdt)
+    tol = 1e-3
+    o_full, it_full, conv_full, last_full = oracle.relax_yx(h, 0.05, 0.05, crit_t, 5000, tol, dtype=npdt)
+    assert conv_full and it_full > 6
+    for mark in (1, 2, 3, 4, 5, it_full - 1, it_full, it_full + 3):
+        t = _device(h, dtype)
+        res = _lib.RelaxResult()
+        marked = ctypes.c_double(-1.0)
+        rt.bind_stream()
+        rt.check(rt.lib.mv2_relax_marked(rt.ctx, prec, t.data_ptr(), 61, 90, crit.ctypes.data, 5000, tol, mark,
+                                         ctypes.byref(res), ctypes.byref(marked)))
+        assert (res.iterations, bool(res.converged), res.max_excess_last) == (it_full, conv_full, last_full), mark
+        np.testing.assert_array_equal(t.cpu().numpy(), o_full)
+        _, _, _, last_m = oracle.relax_yx(h, 0.05, 0.05, crit_t, mark, tol, dtype=npdt)
+        assert marked.value == last_m, (mark, marked.value, last_m)
+
+
 def test_relax_toe_701_iteration_count_and_field():
     import torch
     from isaac_bulk_hip import relax_device
