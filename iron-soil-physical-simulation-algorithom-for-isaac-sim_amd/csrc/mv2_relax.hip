@@ -96,10 +96,26 @@ __device__ inline int relax_group(const RelaxArgs<R> &a, int k)
     return k == 0 ? a.first_group : k % kRelaxGroups;
 }
 
-// loop state for launch k: test the previous launch's iterations in order.
-// Every thread of the block calls it.
+// this thread's slot of each spread set the previous launch reduced into
+// (threads 0..63; loaded before the loop state, whose test reads them)
 template <class R>
-__device__ inline RelaxControl relax_resolve(const RelaxArgs<R> &a, int k, double *smem1)
+__device__ inline void relax_prev_slots(const RelaxArgs<R> &a, int k, double (&slot)[kRelaxKMax])
+{
+#pragma unroll
+    for (int i = 0; i < kRelaxKMax; i++) slot[i] = 0.0;
+    if (k > 0 && threadIdx.x < kSpread) {
+        const int g = relax_group(a, k - 1);
+#pragma unroll
+        for (int i = 0; i < kRelaxKMax; i++)
+            slot[i] = __longlong_as_double((long long)relax_set(a.spread, g, i)[threadIdx.x * kSpreadStride]);
+    }
+}
+
+// loop state for launch k: test the previous launch's iterations in order
+// (slot: relax_prev_slots).  Every thread of the block calls it.
+template <class R>
+__device__ inline RelaxControl relax_resolve(const RelaxArgs<R> &a, int k, double *smem1,
+                                             const double (&slot)[kRelaxKMax])
 {
     RelaxControl C;
     if (k == 0) {
@@ -108,11 +124,20 @@ __device__ inline RelaxControl relax_resolve(const RelaxArgs<R> &a, int k, doubl
     }
     C = a.sync->ctl[k & 1];      // written by an earlier launch: plain load
     if (!C.done && C.replay == 0 && C.n_prev > 0) {
-        const int g = relax_group(a, k - 1);
         int first = -1;
         double mfirst = 0.0, mlast = 0.0;
         for (int i = 0; i < C.n_prev; i++) {
-            const double m = spread_read_max(relax_set(a.spread, g, i), smem1);
+            double m = 0.0;
+#pragma unroll
+            for (int j = 0; j < kRelaxKMax; j++)
+                if (j == i) m = slot[j];
+            if (threadIdx.x < kWave) {
+                m = wave_max(m);
+                if (threadIdx.x == 0) *smem1 = m;
+            }
+            __syncthreads();
+            m = *smem1;
+            __syncthreads();
             if (first < 0 && m <= a.tolerance) {
                 first = i;
                 mfirst = m;
@@ -149,7 +174,28 @@ __global__ void __launch_bounds__(G::BLOCK) relax_kernel(RelaxArgs<R> a, int k)
     __shared__ double smem[kRelaxKMax * (G::BLOCK / kWave)];
     RelaxSync *s = a.sync;
     const bool lead = blockIdx.x == 0 && threadIdx.x == 0;
-    const RelaxControl C = relax_resolve(a, k, smem);
+    const int t = blockIdx.x;
+    const int trow = t / a.tiles_x, tcol = t % a.tiles_x;
+    const int r0 = trow * G::TH, c0 = tcol * G::TW;
+    // Requests that do not depend on the loop state, issued together so that
+    // they share one memory round trip: the previous launch's maxima, the
+    // quiet-tile flags of the 3 x 3 tile neighbourhood and this tile's
+    // iteration count there, and the tile itself from buffer k & 1 -- the
+    // input of launch k whenever every launch before it applied iterations
+    // (a replay reads the other buffer and reloads below).
+    double slot[kRelaxKMax];
+    relax_prev_slots(a, k, slot);
+    int qflag = 0;
+    if (k > 0 && threadIdx.x < 9) {
+        const int ty = a.tiles / a.tiles_x;
+        const int rr = trow + (int)threadIdx.x / 3 - 1, cc = tcol + (int)threadIdx.x % 3 - 1;
+        if (rr >= 0 && rr < ty && cc >= 0 && cc < a.tiles_x)
+            qflag = a.tchanged[((k - 1) & 1) * a.tiles + rr * a.tiles_x + cc];
+    }
+    const int titer = k > 0 ? a.titers[t] : 0;
+    R v[G::kStage];
+    relax_tile_load<R, G>(a.h[k & 1], a.ny, a.nx, r0, c0, v);
+    const RelaxControl C = relax_resolve(a, k, smem, slot);
     if (C.done) {
         if (lead) s->ctl[(k + 1) & 1] = C;
         return;
@@ -174,9 +220,6 @@ __global__ void __launch_bounds__(G::BLOCK) relax_kernel(RelaxArgs<R> a, int k)
         }
         s->ctl[(k + 1) & 1] = nc;
     }
-    const int t = blockIdx.x;
-    const int trow = t / a.tiles_x, tcol = t % a.tiles_x;
-    const int r0 = trow * G::TH, c0 = tcol * G::TW;
     unsigned long long *sets = relax_set(a.spread, relax_group(a, k), 0);
     // Quiet tiles: the iterations of a tile read its interior and a halo of
     // at most one tile, i.e. its 3 x 3 tile neighbourhood.  When none of
@@ -185,14 +228,7 @@ __global__ void __launch_bounds__(G::BLOCK) relax_kernel(RelaxArgs<R> a, int k)
     // reproduce that launch's field (already in both buffers) and maxima.
     // Launch 0 and a replay compute every tile.
     if (k > 0 && C.replay == 0) {
-        const int ty = a.tiles / a.tiles_x;
-        int flag = 0;
-        if (threadIdx.x < 9) {
-            const int rr = trow + (int)threadIdx.x / 3 - 1, cc = tcol + (int)threadIdx.x % 3 - 1;
-            if (rr >= 0 && rr < ty && cc >= 0 && cc < a.tiles_x)
-                flag = a.tchanged[((k - 1) & 1) * a.tiles + rr * a.tiles_x + cc];
-        }
-        if (!__syncthreads_or(flag) && a.titers[t] == n) {
+        if (!__syncthreads_or(qflag) && titer == n) {
             if (threadIdx.x == 0) {
                 a.tchanged[(k & 1) * a.tiles + t] = 0;
                 for (int i = 0; i < n; i++)
@@ -205,7 +241,8 @@ __global__ void __launch_bounds__(G::BLOCK) relax_kernel(RelaxArgs<R> a, int k)
     R tm[G::K];
 #pragma unroll
     for (int i = 0; i < G::K; i++) tm[i] = (R)0;
-    relax_tile_iterations<R, G>(a.h[src], a.h[src ^ 1], a.ny, a.nx, a.crit, r0, c0, tile, n, tm, &changed);
+    if (src != (k & 1)) relax_tile_load<R, G>(a.h[src], a.ny, a.nx, r0, c0, v);
+    relax_tile_iterations_staged<R, G>(v, a.h[src ^ 1], a.ny, a.nx, a.crit, r0, c0, tile, n, tm, &changed);
     double m[G::K];
 #pragma unroll
     for (int i = 0; i < G::K; i++) m[i] = (double)tm[i];
@@ -226,17 +263,29 @@ __global__ void __launch_bounds__(G::BLOCK) relax_kernel(RelaxArgs<R> a, int k)
 // buffers alone: the next launches continue from them.  Writes the result to
 // pinned host memory and clears the next call's first group (read by no
 // launch of this call).
+constexpr int kRelaxEndAhead = 2;    // copy-back cells per thread read before the loop state arrives
+
 template <class R>
 __global__ void __launch_bounds__(kBlock) relax_end_kernel(RelaxArgs<R> a, int k, unsigned *stamp_word,
                                                            unsigned stamp)
 {
     __shared__ double smem[1];
-    const RelaxControl C = relax_resolve(a, k, smem);
+    // the copy-back's first cells are read with the maxima, before the loop
+    // state says whether they are needed (the final field is in buffer 1)
+    const int64_t n = (int64_t)a.ny * a.nx;
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    const int64_t i0 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    double slot[kRelaxKMax];
+    relax_prev_slots(a, k, slot);
+    R ahead[kRelaxEndAhead];
+#pragma unroll
+    for (int u = 0; u < kRelaxEndAhead; u++) ahead[u] = i0 + u * stride < n ? a.h[1][i0 + u * stride] : (R)0;
+    const RelaxControl C = relax_resolve(a, k, smem, slot);
     if (C.done && C.src == 1) {
-        const int64_t n = (int64_t)a.ny * a.nx;
-        for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
-             i += (int64_t)gridDim.x * blockDim.x)
-            a.h[0][i] = a.h[1][i];
+#pragma unroll
+        for (int u = 0; u < kRelaxEndAhead; u++)
+            if (i0 + u * stride < n) a.h[0][i0 + u * stride] = ahead[u];
+        for (int64_t i = i0 + kRelaxEndAhead * stride; i < n; i += stride) a.h[0][i] = a.h[1][i];
     }
     if (blockIdx.x == 0) {
         for (int i = 0; i < kRelaxKMax; i++) spread_clear(relax_set(a.spread, a.first_group == 3 ? 4 : 3, i));

@@ -62,37 +62,45 @@ template <class R> __device__ inline R relax_nan();
 template <> __device__ inline float relax_nan<float>() { return __builtin_nanf(""); }
 template <> __device__ inline double relax_nan<double>() { return __builtin_nan(""); }
 
+// this thread's share of the padded tile with interior origin (r0, c0), read
+// from hs into registers (NaN outside the computed region and the grid)
+template <class R, class G>
+__device__ inline void relax_tile_load(const R *hs, int ny, int nx, int r0, int c0, R (&v)[G::kStage])
+{
+    constexpr int H = G::H, LH = G::LH, LW = G::LW, PW = G::PW, BLOCK = G::BLOCK;
+    const int R0 = r0 - H - 1, C0 = c0 - H - 1;         // global cell of padded (0, 0)
+#pragma unroll
+    for (int j = 0; j < G::kStage; j++) {
+        const int e = threadIdx.x + j * BLOCK;
+        const int pr = e / PW, pc = e - pr * PW;
+        const int gr = R0 + pr, gc = C0 + pc;
+        const bool inner = pr >= 1 && pr <= LH && pc >= 1 && pc <= LW;
+        v[j] = relax_nan<R>();
+        if (e < G::kCells && inner && gr >= 0 && gr < ny && gc >= 0 && gc < nx) v[j] = hs[gr * nx + gc];
+    }
+}
+
 // n (1 <= n <= G::K) iterations on the tile with interior origin (r0, c0) of
 // the field hs (ny x nx), written to hd; tmaxk[i] = this thread's maximum
 // excess over the owned edges in iteration i, taken before each sub-phase.
-// Per edge the arithmetic is relax_tile_iteration's: diff = a - b, excess =
+// Per edge (slope_model.py:318-331): diff = a - b, excess =
 // max(|diff| - crit, 0), transfer (excess / 2) sgn(diff) -- written as
 // diff < 0 ? -half : half, equal to it for every diff (half is +0 whenever
 // diff is 0 or NaN).  All threads of the block must call it.  With `changed`,
 // the thread reports whether an interior cell it writes differs from its
 // staged value.
+// The _staged form takes the tile as read into registers by relax_tile_load
+// (the caller may issue those loads early).
 template <class R, class G>
-__device__ inline void relax_tile_iterations(const R *hs, R *hd, int ny, int nx, const R (&crits)[4], int r0,
-                                             int c0, R *tile, int n, R (&tmaxk)[G::K], bool *changed = nullptr)
+__device__ inline void relax_tile_iterations_staged(const R (&v)[G::kStage], R *hd, int ny, int nx,
+                                                    const R (&crits)[4], int r0, int c0, R *tile, int n,
+                                                    R (&tmaxk)[G::K], bool *changed = nullptr)
 {
     constexpr int H = G::H, LH = G::LH, LW = G::LW, PW = G::PW, TH = G::TH, TW = G::TW, BLOCK = G::BLOCK;
-    const int R0 = r0 - H - 1, C0 = c0 - H - 1;         // global cell of padded (0, 0)
-    {
-        R v[G::kStage];
 #pragma unroll
-        for (int j = 0; j < G::kStage; j++) {
-            const int e = threadIdx.x + j * BLOCK;
-            const int pr = e / PW, pc = e - pr * PW;
-            const int gr = R0 + pr, gc = C0 + pc;
-            const bool inner = pr >= 1 && pr <= LH && pc >= 1 && pc <= LW;
-            v[j] = relax_nan<R>();
-            if (e < G::kCells && inner && gr >= 0 && gr < ny && gc >= 0 && gc < nx) v[j] = hs[gr * nx + gc];
-        }
-#pragma unroll
-        for (int j = 0; j < G::kStage; j++) {
-            const int e = threadIdx.x + j * BLOCK;
-            if (e < G::kCells) tile[e] = v[j];
-        }
+    for (int j = 0; j < G::kStage; j++) {
+        const int e = threadIdx.x + j * BLOCK;
+        if (e < G::kCells) tile[e] = v[j];
     }
     // per-slot anchors and ownership (anchor in the interior) of the active
     // edge and of the opposite-parity edge continuing from its second cell
@@ -196,6 +204,15 @@ __device__ inline void relax_tile_iterations(const R *hs, R *hd, int ny, int nx,
             if (changed && v != orig[j]) *changed = true;
         }
     }
+}
+
+template <class R, class G>
+__device__ inline void relax_tile_iterations(const R *hs, R *hd, int ny, int nx, const R (&crits)[4], int r0,
+                                             int c0, R *tile, int n, R (&tmaxk)[G::K], bool *changed = nullptr)
+{
+    R v[G::kStage];
+    relax_tile_load<R, G>(hs, ny, nx, r0, c0, v);
+    relax_tile_iterations_staged<R, G>(v, hd, ny, nx, crits, r0, c0, tile, n, tmaxk, changed);
 }
 
 }  // namespace mv2
