@@ -43,59 +43,76 @@ struct InitArgs {
     mv2_avalanche_fields f;
 };
 
+// a field value of the state's precision as double (FP32: float fields,
+// int32 mobile quanta), known at compile time so that a cell's loads form one
+// straight-line group
+template <bool F32>
+__device__ inline double ldf(const void *p, int64_t i)
+{
+    return F32 ? (double)((const float *)p)[i] : ((const double *)p)[i];
+}
+template <bool F32>
+__device__ inline double ldm(const void *p, int64_t i)
+{
+    return F32 ? (double)((const int32_t *)p)[i] * kQuantum : ((const double *)p)[i];
+}
+
 // initialize (gpu_large_avalanche.py:59-162) of one cell: gradient, slope,
 // cohesive start / stop margins, severity, the unstable flag and the latch
 // release; zeroes the cell's per-root count.  Returns the unstable flag.
+// Every value the cell needs is read first (one memory round trip for the
+// group), then the arithmetic runs in the reference's order.
+template <bool F32>
 __device__ inline bool init_cell(const InitArgs &A, int64_t index)
 {
-    const void *z_base = A.z_base, *resting = A.resting, *mobile = A.mobile, *qx = A.qx, *qy = A.qy;
-    const int f32 = A.f32, ny = A.ny, nx = A.nx;
+    const int ny = A.ny, nx = A.nx;
     const double dx = A.dx, dy = A.dy, area = A.area, f6 = A.f6;
     const mv2_avalanche_config &cfg = A.cfg;
     const mv2_avalanche_fields &f = A.f;
-    {
-        f.counts[index] = 0;
-        const int row = (int)index / nx, col = (int)index - row * nx;
-        int64_t left = index, right = index, down = index, up = index;
-        double denom_x = dx, denom_y = dy;
-        if (col > 0) left = index - 1;
-        if (col + 1 < nx) right = index + 1;
-        if (col > 0 && col + 1 < nx) denom_x = 2.0 * dx;
-        if (row > 0) down = index - nx;
-        if (row + 1 < ny) up = index + nx;
-        if (row > 0 && row + 1 < ny) denom_y = 2.0 * dy;
-        double gx = (ld_field(resting, f32, right) + ld_mobile(mobile, f32, right) - ld_field(resting, f32, left) -
-                     ld_mobile(mobile, f32, left)) / denom_x;
-        double gy = (ld_field(resting, f32, up) + ld_mobile(mobile, f32, up) - ld_field(resting, f32, down) -
-                     ld_mobile(mobile, f32, down)) / denom_y;
-        double magnitude = sqrt(gx * gx + gy * gy);
-        f.grad_x[index] = gx;
-        f.grad_y[index] = gy;
-        f.slope[index] = atan(magnitude);
-        double rest = ld_field(resting, f32, index);
-        double layer_depth = fmin(fmax(rest - ld_field(z_base, f32, index), 0.0), cfg.mobilization_depth);
-        double normalization = sqrt(1.0 + magnitude * magnitude);
-        double drive = cfg.density * cfg.gravity * layer_depth * magnitude / normalization;
-        double normal = cfg.density * cfg.gravity * layer_depth / normalization;
-        double start_margin = drive - (cfg.cohesion + normal * cfg.tan_start);
-        double stop_margin = drive - (cfg.cohesion + normal * cfg.tan_stop);
-        double denom = normal * (cfg.tan_start - cfg.tan_stop);
-        f.severity[index] = clampd(stop_margin / (denom >= 1.0e-12 ? denom : 1.0e-12), 0.0, 1.0);
-        const bool active = layer_depth > cfg.dry_tolerance && start_margin > 0.0;
-        f.unstable[index] = active ? 1 : 0;
-        double m = ld_mobile(mobile, f32, index);
-        double speed = 0.0;
-        if (m > cfg.dry_tolerance) {
-            double vx = ld_field(qx, f32, index) / m, vy = ld_field(qy, f32, index) / m;
-            speed = sqrt(vx * vx + vy * vy);
-        }
-        const double w = cell_weight(cfg.weight_mode, ny, nx, area, f6, row, col);
-        double exported = f.export_cumulative[index] - f.owned_export_baseline[index];
-        bool resolved = exported > cfg.dry_tolerance * w;
-        if (cfg.release_settled_latches && speed <= cfg.activity_speed && (resolved || stop_margin <= 0.0))
-            f.latch[index] = 0;
-        return active;
+    const int row = (int)index / nx, col = (int)index - row * nx;
+    int64_t left = index, right = index, down = index, up = index;
+    double denom_x = dx, denom_y = dy;
+    if (col > 0) left = index - 1;
+    if (col + 1 < nx) right = index + 1;
+    if (col > 0 && col + 1 < nx) denom_x = 2.0 * dx;
+    if (row > 0) down = index - nx;
+    if (row + 1 < ny) up = index + nx;
+    if (row > 0 && row + 1 < ny) denom_y = 2.0 * dy;
+    const double rr = ldf<F32>(A.resting, right), rl = ldf<F32>(A.resting, left);
+    const double ru = ldf<F32>(A.resting, up), rd = ldf<F32>(A.resting, down);
+    const double mr = ldm<F32>(A.mobile, right), ml = ldm<F32>(A.mobile, left);
+    const double mu = ldm<F32>(A.mobile, up), md = ldm<F32>(A.mobile, down);
+    const double rest = ldf<F32>(A.resting, index), zb = ldf<F32>(A.z_base, index);
+    const double m = ldm<F32>(A.mobile, index);
+    const double qxv = ldf<F32>(A.qx, index), qyv = ldf<F32>(A.qy, index);
+    const double exported = f.export_cumulative[index] - f.owned_export_baseline[index];
+    f.counts[index] = 0;
+    const double gx = (rr + mr - rl - ml) / denom_x;
+    const double gy = (ru + mu - rd - md) / denom_y;
+    const double magnitude = sqrt(gx * gx + gy * gy);
+    f.grad_x[index] = gx;
+    f.grad_y[index] = gy;
+    f.slope[index] = atan(magnitude);
+    const double layer_depth = fmin(fmax(rest - zb, 0.0), cfg.mobilization_depth);
+    const double normalization = sqrt(1.0 + magnitude * magnitude);
+    const double drive = cfg.density * cfg.gravity * layer_depth * magnitude / normalization;
+    const double normal = cfg.density * cfg.gravity * layer_depth / normalization;
+    const double start_margin = drive - (cfg.cohesion + normal * cfg.tan_start);
+    const double stop_margin = drive - (cfg.cohesion + normal * cfg.tan_stop);
+    const double denom = normal * (cfg.tan_start - cfg.tan_stop);
+    f.severity[index] = clampd(stop_margin / (denom >= 1.0e-12 ? denom : 1.0e-12), 0.0, 1.0);
+    const bool active = layer_depth > cfg.dry_tolerance && start_margin > 0.0;
+    f.unstable[index] = active ? 1 : 0;
+    double speed = 0.0;
+    if (m > cfg.dry_tolerance) {
+        const double vx = qxv / m, vy = qyv / m;
+        speed = sqrt(vx * vx + vy * vy);
     }
+    const double w = cell_weight(cfg.weight_mode, ny, nx, area, f6, row, col);
+    const bool resolved = exported > cfg.dry_tolerance * w;
+    if (cfg.release_settled_latches && speed <= cfg.activity_speed && (resolved || stop_margin <= 0.0))
+        f.latch[index] = 0;
+    return active;
 }
 
 // The failure mask fused into the tile-local union-find: one workgroup per
@@ -105,6 +122,7 @@ __device__ inline bool init_cell(const InitArgs &A, int64_t index)
 // statistics record and the label spread sets for the launches that follow.
 constexpr int kLocalBlock = kCclCells;   // one thread per cell of the 32 x 32 tile
 
+template <bool F32>
 __global__ void __launch_bounds__(kLocalBlock) avalanche_local_kernel(InitArgs A, int tiles_x, LabelAux *aux,
                                                                       int32_t n_plus_one, unsigned long long *spread)
 {
@@ -123,7 +141,7 @@ __global__ void __launch_bounds__(kLocalBlock) avalanche_local_kernel(InitArgs A
     const int lr = e / kCcl, lc = e % kCcl;
     const int gr = tr0 + lr, gc = tc0 + lc;
     const bool inside = gr < ny && gc < nx;
-    const bool masked = inside && init_cell(A, (int64_t)gr * nx + gc);
+    const bool masked = inside && init_cell<F32>(A, (int64_t)gr * nx + gc);
     lp[e] = masked ? e : -1;
     __syncthreads();
     if (masked) {
@@ -342,34 +360,43 @@ __device__ inline int32_t resolve_winner(const unsigned long long *spread, int64
     return (int32_t)win;
 }
 
-// aggregate + winner_diagnostics (:201-231, :255-281) restricted to the winner
+// aggregate + winner_diagnostics (:201-231, :255-281) restricted to the
+// winner.  The first cell's label is read with the statistics the winner
+// comes from, and a winner cell's fields are read together.
+template <bool F32>
 __global__ void __launch_bounds__(kBlock) winner_stats_kernel(
-    const int32_t *labels, const void *z_base, const void *resting, int f32, const int32_t *latch,
+    const int32_t *labels, const void *z_base, const void *resting, const int32_t *latch,
     const int32_t *previous, const double *slope, const double *severity, int ny, int nx,
     mv2_avalanche_config cfg, double area, double f6, LabelAux *aux, const unsigned long long *spread,
     double *partials)
 {
     __shared__ double smem[kWinF * 4];
-    const int32_t winner = resolve_winner(spread, (int64_t)ny * nx, aux);
+    const int64_t n = (int64_t)ny * nx;
+    const int64_t i0 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    const int32_t lab0 = i0 < n ? labels[i0] : -1;
+    const int32_t winner = resolve_winner(spread, n, aux);
     double s[5] = {0.0, 0.0, 0.0, 0.0, 0.0};   // sum es, sum et, area, volume, root excess
     double m[2] = {0.0, 0.0};                   // max es, max et
     int prev = 0;
-    const int64_t n = (int64_t)ny * nx;
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
-        if (labels[i] < 0 || labels[i] != winner) continue;
+    for (int64_t i = i0; i < n; i += stride) {
+        const int32_t lab = i == i0 ? lab0 : labels[i];
+        if (lab < 0 || lab != winner) continue;
+        const int pv = previous ? previous[i] : 0;
+        const double sl = slope[i], sev = severity[i];
+        const int lt = latch[i];
+        const double rs = ldf<F32>(resting, i), zb = ldf<F32>(z_base, i);
         const int r = (int)i / nx, c = (int)i - r * nx;
         const double w = cell_weight(cfg.weight_mode, ny, nx, area, f6, r, c);
-        prev += previous ? previous[i] : 0;
-        double es = fmax(slope[i] - cfg.start_rad, 0.0);
-        double et = fmax(slope[i] - cfg.stop_rad, 0.0);
+        prev += pv;
+        double es = fmax(sl - cfg.start_rad, 0.0);
+        double et = fmax(sl - cfg.stop_rad, 0.0);
         s[0] += es;
         s[1] += et;
         m[0] = fmax(m[0], es);
         m[1] = fmax(m[1], et);
         double transfer = 0.0;
-        if (latch[i] == 0)
-            transfer = fmin(fmax(ld_field(resting, f32, i) - ld_field(z_base, f32, i), 0.0),
-                            cfg.mobilization_depth * severity[i]);
+        if (lt == 0) transfer = fmin(fmax(rs - zb, 0.0), cfg.mobilization_depth * sev);
         s[2] += w;
         s[3] += transfer * w;
         s[4] += es;
@@ -662,13 +689,22 @@ extern "C" int mv2_avalanche_label(mv2_context *ctx, const mv2_avalanche_config 
                         *cfg, area, f6, *f};
     const int tiles_x = (nx + kCcl - 1) / kCcl;
     const int tiles = tiles_x * ((ny + kCcl - 1) / kCcl);
-    hipLaunchKernelGGL(avalanche_local_kernel, dim3(tiles), dim3(kLocalBlock), 0, ctx->stream, init, tiles_x, aux,
-                       (int32_t)(n + 1), spread);
+    if (f32)
+        hipLaunchKernelGGL(avalanche_local_kernel<true>, dim3(tiles), dim3(kLocalBlock), 0, ctx->stream, init, tiles_x,
+                           aux, (int32_t)(n + 1), spread);
+    else
+        hipLaunchKernelGGL(avalanche_local_kernel<false>, dim3(tiles), dim3(kLocalBlock), 0, ctx->stream, init,
+                           tiles_x, aux, (int32_t)(n + 1), spread);
     if (int rc = label_global(ctx, f->labels, ny, nx, cfg->connectivity, f->counts, spread)) return rc;
     const int sgrid = grid_for(n, kBlock, kLabelParts);
-    hipLaunchKernelGGL(winner_stats_kernel, dim3(sgrid), dim3(kBlock), 0, ctx->stream, f->labels, ctx->z_base,
-                       ctx->b_eff, f32, f->latch, f->previous, f->slope, f->severity, ny, nx, *cfg, area, f6, aux,
-                       spread, partials);
+    if (f32)
+        hipLaunchKernelGGL(winner_stats_kernel<true>, dim3(sgrid), dim3(kBlock), 0, ctx->stream, f->labels,
+                           ctx->z_base, ctx->b_eff, f->latch, f->previous, f->slope, f->severity, ny, nx, *cfg, area,
+                           f6, aux, spread, partials);
+    else
+        hipLaunchKernelGGL(winner_stats_kernel<false>, dim3(sgrid), dim3(kBlock), 0, ctx->stream, f->labels,
+                           ctx->z_base, ctx->b_eff, f->latch, f->previous, f->slope, f->severity, ny, nx, *cfg, area,
+                           f6, aux, spread, partials);
     // the result goes straight to pinned host memory (no copy launch)
     LabelAux *hres = (LabelAux *)((char *)ctx->host_result.ptr + 2048);
     static_assert(2048 + sizeof(LabelAux) <= 4096, "the labelling result in the pinned block");

@@ -976,6 +976,8 @@ constexpr int kEnvStride = kEnvRegion + 1;              // LDS row stride: colum
 constexpr int kEnvWaves = 2;                            // regions per workgroup
 constexpr int kEnvMaxCycles = 6;                        // upper limit of MV2_ENV_CYCLES
 constexpr int kEnvCycles = 1;                           // default cycles per region and launch
+constexpr int kEnvLoadBatch = 16;                       // region rows whose loads are in flight together
+static_assert(kEnvRegion % kEnvLoadBatch == 0, "whole batches of region rows");
 
 struct EnvControl {
     int launches, done, pad[2];
@@ -1117,9 +1119,24 @@ __global__ void __launch_bounds__(kEnvWaves * kWave) envelope_kernel(float *z, i
         const uint64_t valid_rows = __ballot(row_in), valid_cols = __ballot(col_in);
         const uint64_t cut_above = __ballot(row_in && seam > 0 && grow % seam == 0);
         const uint64_t cut_below = __ballot(row_in && seam > 0 && (grow + 1) % seam == 0);
-        for (int r = 0; r < kEnvRegion; r++)
-            t[r * kEnvStride + lane] = ((valid_rows >> r) & 1) && col_in ? z[(int64_t)(R0 + r) * nx + gcol]
-                                                                         : __builtin_huge_valf();
+        // the region's rows in batches of loads issued back to back: every
+        // lane reads an in-grid cell (row and column clamped into the grid)
+        // and cells outside the grid become +inf afterwards
+        const int gc_in = min(max(gcol, 0), nx - 1);
+#pragma unroll 1
+        for (int r0 = 0; r0 < kEnvRegion; r0 += kEnvLoadBatch) {
+            float v[kEnvLoadBatch];
+#pragma unroll
+            for (int u = 0; u < kEnvLoadBatch; u++) {
+                const int gr_in = min(max(R0 + r0 + u, 0), ny - 1);
+                v[u] = z[(int64_t)gr_in * nx + gc_in];
+            }
+#pragma unroll
+            for (int u = 0; u < kEnvLoadBatch; u++) {
+                const int r = r0 + u;
+                t[r * kEnvStride + lane] = ((valid_rows >> r) & 1) && col_in ? v[u] : __builtin_huge_valf();
+            }
+        }
         __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
         __builtin_amdgcn_wave_barrier();
         for (int cycle = 0; cycle < max_cycles; cycle++) {
@@ -1165,27 +1182,46 @@ __global__ void __launch_bounds__(kBlock) envelope_prep_kernel(const float *b, c
         for (int i = threadIdx.x; i < 3 * kSpread; i += blockDim.x) spread[(int64_t)i * kSpreadStride] = 0ull;
     }
     const int64_t n = (int64_t)ny * nx;
-    auto at = [&](int64_t q) -> float { return delta ? b[q] + delta[q] : z[q]; };
     for (int64_t p = (int64_t)xcd_block() * kBlock + threadIdx.x; p < n; p += (int64_t)gridDim.x * kBlock) {
-        const float v = at(p);
-        if (delta) z[p] = v;
-        if (!gen) continue;
+        if (!gen) {
+            if (delta) z[p] = b[p] + delta[p];
+            continue;
+        }
         const int r = (int)(p / nx), c = (int)(p - (int64_t)r * nx);
+        const bool has_l = c > 0, has_r = c + 1 < nx;
         const bool up_ok = r > 0 && !(seam > 0 && r % seam == 0);
         const bool dn_ok = r + 1 < ny && !(seam > 0 && (r + 1) % seam == 0);
+        // the cell and its 8 neighbours, read together (a neighbour outside
+        // the grid reads the cell itself and is not tested): centre, left,
+        // right, up-left, up, up-right, down-left, down, down-right
+        const int64_t cl = has_l ? -1 : 0, cr = has_r ? 1 : 0;
+        const int64_t ru = r > 0 ? -(int64_t)nx : 0, rd = r + 1 < ny ? (int64_t)nx : 0;
+        const int64_t q[9] = {p, p + cl, p + cr, p + ru + cl, p + ru, p + ru + cr, p + rd + cl, p + rd, p + rd + cr};
+        float zn[9];
+        if (delta) {
+            float bb[9], dd[9];
+#pragma unroll
+            for (int i = 0; i < 9; i++) {
+                bb[i] = b[q[i]];
+                dd[i] = delta[q[i]];
+            }
+#pragma unroll
+            for (int i = 0; i < 9; i++) zn[i] = bb[i] + dd[i];
+            z[p] = zn[0];
+        } else {
+#pragma unroll
+            for (int i = 0; i < 9; i++) zn[i] = z[q[i]];
+        }
+        const float v = zn[0];
         bool low = false;
-        if (c > 0) low |= at(p - 1) + c0 < v;
-        if (c + 1 < nx) low |= at(p + 1) + c0 < v;
-        if (up_ok) {
-            low |= at(p - nx) + c1 < v;
-            if (c > 0) low |= at(p - nx - 1) + c2 < v;
-            if (c + 1 < nx) low |= at(p - nx + 1) + c3 < v;
-        }
-        if (dn_ok) {
-            low |= at(p + nx) + c1 < v;
-            if (c + 1 < nx) low |= at(p + nx + 1) + c2 < v;
-            if (c > 0) low |= at(p + nx - 1) + c3 < v;
-        }
+        low |= has_l && zn[1] + c0 < v;
+        low |= has_r && zn[2] + c0 < v;
+        low |= up_ok && zn[4] + c1 < v;
+        low |= up_ok && has_l && zn[3] + c2 < v;
+        low |= up_ok && has_r && zn[5] + c3 < v;
+        low |= dn_ok && zn[7] + c1 < v;
+        low |= dn_ok && has_r && zn[8] + c2 < v;
+        low |= dn_ok && has_l && zn[6] + c3 < v;
         if (low) flags[tiles + (r / kEnvTile) * tiles_x + c / kEnvTile] = gen;   // every writer stores gen
     }
 }
