@@ -211,8 +211,43 @@ __global__ void __launch_bounds__(kBlock) ccl_local_kernel(int32_t *p, int ny, i
     }
 }
 
+// Union of the components holding a and b, starting from ancestors ra of a
+// and rb of b (their parents as read by the caller): both chains are followed
+// to their roots in lockstep, one pair of loads per step, then the larger root
+// is hooked onto the smaller as in unite.  Parents only ever decrease, so a
+// stale ancestor still leads to the current root.  (Pointing the two starting
+// ancestors at the roots found shortened the flatten pass by what it cost
+// here: 701^2, 10.8 + 6.2 against 9.8 + 7.0 us.)
+__device__ inline void unite_from(int32_t *p, int32_t ra, int32_t rb)
+{
+    for (;;) {
+        const int32_t na = load_parent(p, ra), nb = load_parent(p, rb);
+        if (na == ra && nb == rb) break;
+        ra = na;
+        rb = nb;
+    }
+    while (ra != rb) {
+        if (ra < rb) {
+            int32_t expected = rb;
+            if (__hip_atomic_compare_exchange_strong(&p[rb], &expected, ra, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
+                                                     __HIP_MEMORY_SCOPE_AGENT))
+                return;
+            rb = find_root(p, expected);
+        } else {
+            int32_t expected = ra;
+            if (__hip_atomic_compare_exchange_strong(&p[ra], &expected, rb, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
+                                                     __HIP_MEMORY_SCOPE_AGENT))
+                return;
+            ra = find_root(p, expected);
+        }
+    }
+}
+
 // edges crossing tile boundaries: per tile 32 right-column cells (E, SE),
-// 32 bottom-row cells (S, SE, SW) and 32 left-column cells (SW)
+// 32 bottom-row cells (S, SE, SW) and 32 left-column cells (SW).  A cell reads
+// its own parent and those of its neighbours across the border in one group
+// (a neighbour outside the grid or not on this cell's edges reads the cell
+// itself and is not used).
 __global__ void __launch_bounds__(kBlock) ccl_border_kernel(int32_t *p, int ny, int nx, int connectivity,
                                                              int tiles_x, int tiles)
 {
@@ -226,16 +261,22 @@ __global__ void __launch_bounds__(kBlock) ccl_border_kernel(int32_t *p, int ny, 
         else { r = tr0 + q - 2 * kCcl; c = tc0; }
         if (r >= ny || c >= nx) continue;
         const int i = r * nx + c;
-        if (load_parent(p, i) < 0) continue;
         const bool right = q < kCcl, bottom = q >= kCcl && q < 2 * kCcl, left = q >= 2 * kCcl;
-        if (right && c + 1 < nx && load_parent(p, i + 1) >= 0) unite(p, i, i + 1);
-        if (r + 1 < ny) {
-            if (bottom && load_parent(p, i + nx) >= 0) unite(p, i, i + nx);
-            if (connectivity == 8) {
-                if ((right || bottom) && c + 1 < nx && load_parent(p, i + nx + 1) >= 0) unite(p, i, i + nx + 1);
-                if ((left || bottom) && c > 0 && load_parent(p, i + nx - 1) >= 0) unite(p, i, i + nx - 1);
-            }
-        }
+        const bool has_d = r + 1 < ny, eight = connectivity == 8;
+        const bool use_e = right && c + 1 < nx;
+        const bool use_s = bottom && has_d;
+        const bool use_se = eight && (right || bottom) && has_d && c + 1 < nx;
+        const bool use_sw = eight && (left || bottom) && has_d && c > 0;
+        const int32_t pi = load_parent(p, i);
+        const int32_t pe = load_parent(p, use_e ? i + 1 : i);
+        const int32_t ps = load_parent(p, use_s ? i + nx : i);
+        const int32_t pse = load_parent(p, use_se ? i + nx + 1 : i);
+        const int32_t psw = load_parent(p, use_sw ? i + nx - 1 : i);
+        if (pi < 0) continue;
+        if (use_e && pe >= 0) unite_from(p, pi, pe);
+        if (use_s && ps >= 0) unite_from(p, pi, ps);
+        if (use_se && pse >= 0) unite_from(p, pi, pse);
+        if (use_sw && psw >= 0) unite_from(p, pi, psw);
     }
 }
 
