@@ -68,6 +68,9 @@ typedef struct mv2_step_config {
     int limit_mode;                   /* 0: mobile_v2_reference.py:237, 1: warp_mobile_v2.py:215 */
     int weight_mode;                  /* 0: uniform dx*dy, 1: Triangle-A-C vertex weights */
     int static_exit;                  /* apply the global static early exit */
+    int diagnostics;                  /* per-substep sums kept: 0 all ten (mobile_v2_reference.py ledgers),
+                                         1 the five of the production record, 0 1 4 5 7
+                                         (warp_mobile_v2.py:178-241); the others read 0 */
 } mv2_step_config;
 
 /* summary[]: 0 mass(m^3), 1 px, 2 py, 3 max speed, 4 kinetic, 5 gravitational,

@@ -252,8 +252,9 @@ __device__ inline void strip_load(const A &a, int src, int r0, int c0, StripRows
 // One substep of the wave's band into buffer src ^ 1.  Per lane: dg gathers
 // the substep diagnostics of its cells, wv the CFL wave speed of the new
 // state, hn_min the smallest new thickness (quanta) before clamping, ovf the
-// int32 range failure.
-template <int SH, class A>
+// int32 range failure.  DIAGF 1 gathers only the sums the production record
+// reads (0, 1, 4, 5, 7); the others stay 0.
+template <int SH, class A, int DIAGF = 0>
 __device__ inline void strip_update(const A &a, const StripRows<SH> &R, int src, int r0, int c0, double sub_dt,
                                     float (&dg)[kDiag], float &wv, long long &hn_min, int &ovf)
 {
@@ -348,15 +349,19 @@ __device__ inline void strip_update(const A &a, const StripRows<SH> &R, int src,
                 qax = depth * vax;
                 qay = depth * vay;
                 wave_v = speed * factor;
-                dg[2] += depth * (vex - vbx);
-                dg[3] += depth * (vey - vby);
-                dg[6] += depth * (0.5f * (vbx + vex) * ax + 0.5f * (vby + vey) * ay) * dt;
+                if constexpr (DIAGF == 0) {
+                    dg[2] += depth * (vex - vbx);
+                    dg[3] += depth * (vey - vby);
+                    dg[6] += depth * (0.5f * (vbx + vex) * ax + 0.5f * (vby + vey) * ay) * dt;
+                }
                 dg[4] += qax - depth * vex;
                 dg[5] += qay - depth * vey;
                 dg[7] += depth * (speed * speed - (vax * vax + vay * vay));
             }
-            dg[8] += (wet ? qxn : 0.0f) - qxi;
-            dg[9] += (wet ? qyn : 0.0f) - qyi;
+            if constexpr (DIAGF == 0) {
+                dg[8] += (wet ? qxn : 0.0f) - qxi;
+                dg[9] += (wet ? qyn : 0.0f) - qyi;
+            }
             const int i = gr * nx + gc;
             oh[i] = hq;
             oqx[i] = qax;

@@ -85,7 +85,6 @@ constexpr int kClasses = 9;              // weight classes of the exact FP32 mas
 // consecutive workgroups with coalesced loads.
 constexpr int kLedgerF = 2 * kSum;        // doubles per workgroup: before[8], after[8]
 constexpr int kLedgerQ = 2 * kClasses;    // int64 per workgroup: quanta before[9], after[9]
-constexpr int kLedger32D = 7;             // FP32 strips: ledger 1..7 of warp_mobile_v2.py:41-54 per workgroup
 
 struct F64P {
     using real = double;
@@ -515,9 +514,9 @@ __global__ void __launch_bounds__(kBlock) v2_begin_kernel(V2Args<P> a)
 constexpr int kSubFold = 3 + kDiag;   // values folded per workgroup at the end of a substep
 
 template <int SH>
-__device__ inline void end32_body(const V2Args<F32P> &a, int fsrc, double *lds);
+__device__ inline void end32_body(const V2Args<F32P> &a, int fsrc);
 
-template <class P, int SH = kSH>
+template <class P, int SH = kSH, int DIAGF = 0>
 __global__ void __launch_bounds__(kBlock) v2_substep_kernel(V2Args<P> a, int k, int eager)
 {
     constexpr int kRows = SH * (kBlock / kWave);     // rows of a tile
@@ -527,29 +526,58 @@ __global__ void __launch_bounds__(kBlock) v2_substep_kernel(V2Args<P> a, int k, 
     const mv2_step_config &cfg = a.cfg;
     const bool lead = blockIdx.x == 0 && threadIdx.x == 0;
     const int src = k & 1;
+    const int lane = threadIdx.x & (kWave - 1);
     const int ny = a.ny, nx = a.nx;
+    if constexpr (P::kQuant) {
+        // Every kernel argument the prologue uses is made ready here, before
+        // the first branch, so the scalar loads go out as one batch with one
+        // wait; read at first use they would each be waited for in turn
+        // ahead of the tile's loads.
+        __asm__ volatile("" ::"s"(ny), "s"(nx), "s"(a.b), "s"(a.h[0]), "s"(a.h[1]), "s"(a.qx[0]), "s"(a.qx[1]),
+                         "s"(a.qy[0]), "s"(a.qy[1]), "s"(a.spread), "s"(a.diag_part), "s"(a.diag_groups),
+                         "s"(a.sync), "s"(a.ext_x), "s"(k), "s"(eager), "s"((int)gridDim.x), "s"(a.cfg.density),
+                         "s"(a.clamp_quarter));
+    }
     const int tiles_x = (nx + kStripW - 1) / kStripW;
     const int tiles = tiles_x * ((ny + kRows - 1) / kRows);
     int t = 0;
     if constexpr (P::kQuant) {
-        int last;
-        xcd_tile_range(tiles, t, last);
+        // one tile per workgroup (the launch's own grid) needs no division
+        if (gridDim.x == tiles) {
+            t = xcd_block();
+        } else {
+            int last;
+            xcd_tile_range(tiles, t, last);
+        }
     }
     // this wave's band of the tile
     const int r0 = (t / tiles_x) * kRows + wave_index() * SH;
     const int c0 = (t % tiles_x) * kStripW;
-    // requests that do not depend on the loop state
-    double slot = 0.0;
-    if (threadIdx.x < kWave)
-        slot = __longlong_as_double((long long)a.spread[(k % kWaveSets) * kSpreadWords + threadIdx.x * kSpreadStride]);
-    // the workgroup's diagnostic slot accumulates over the working launches;
-    // launch 0 is the first of them and starts it from zero
-    double dprev[kDiag];
-    if (threadIdx.x == 0)
-        for (int d = 0; d < kDiag; d++) dprev[d] = k == 0 ? 0.0 : a.diag_part[(int64_t)d * a.diag_groups + blockIdx.x];
+    // FP32: each wave keeps its own diagnostic partials (global wave index)
+    const int gw = blockIdx.x * (kBlock / kWave) + wave_index();
+    // Requests that do not depend on the loop state, the tile first.  The
+    // spread slots and the diagnostic slot are vector loads (lane-dependent
+    // addresses): a scalar load here would be waited for before the tile's
+    // loads could issue.
     StripRows<SH> rows;
     if constexpr (P::kQuant) {
         if (eager) strip_load<SH>(a, src, r0, c0, rows);
+    }
+    // FP32: every wave folds the 64 slots of the previous launch's maximum
+    // itself (no barrier); FP64: the first wave, through LDS
+    double slot = 0.0;
+    if (P::kQuant || threadIdx.x < kWave)
+        slot = __longlong_as_double((long long)a.spread[(k % kWaveSets) * kSpreadWords + lane * kSpreadStride]);
+    // the diagnostic slots accumulate over the working launches; launch 0 is
+    // the first of them and starts them from zero.  Lane d holds field d: of
+    // every wave (FP32) or of the workgroup's first wave (FP64).
+    double dprev = 0.0;
+    if (k != 0) {
+        if constexpr (P::kQuant) {
+            if (lane < kDiag) dprev = a.diag_part[(int64_t)lane * a.diag_groups + gw];
+        } else {
+            if (threadIdx.x < kDiag) dprev = a.diag_part[(int64_t)threadIdx.x * a.diag_groups + blockIdx.x];
+        }
     }
     Control C = resolve_control(s, k);
     if (!C.done && k == 0 && cfg.static_exit) {
@@ -570,8 +598,7 @@ __global__ void __launch_bounds__(kBlock) v2_substep_kernel(V2Args<P> a, int k, 
     if (C.done) {
         if constexpr (P::kQuant) {
             // the first launch past the loop's end: copy back + ledger after
-            static_assert(kLedger32D <= kSubFold, "the ledger fold fits the substep's LDS");
-            if (!C.end_done && !C.static_exit) end32_body<SH>(a, C.src, fold_lds);
+            if (!C.end_done && !C.static_exit) end32_body<SH>(a, C.src);
             C.end_done = 1;
         }
         if (lead) s->ctl[(k + 1) & 1] = C;
@@ -581,12 +608,17 @@ __global__ void __launch_bounds__(kBlock) v2_substep_kernel(V2Args<P> a, int k, 
         if (!eager) strip_load<SH>(a, src, r0, c0, rows);
     }
     // CFL maximum of the current state (mobile_v2_reference.py:239-251)
-    if (threadIdx.x < kWave) {
-        const double v = wave_max(slot);
-        if (threadIdx.x == 0) smem[0] = v;
+    double maxw;
+    if constexpr (P::kQuant) {
+        maxw = wave_max(slot);
+    } else {
+        if (threadIdx.x < kWave) {
+            const double v = wave_max(slot);
+            if (threadIdx.x == 0) smem[0] = v;
+        }
+        __syncthreads();
+        maxw = smem[0];
     }
-    __syncthreads();
-    const double maxw = smem[0];
     const double mind = a.dx <= a.dy ? a.dx : a.dy;
     const double stable = maxw <= 1.0e-14 ? C.remaining : cfg.cfl * mind / maxw;
     const double sub_dt = npmin(C.remaining, stable);
@@ -613,50 +645,79 @@ __global__ void __launch_bounds__(kBlock) v2_substep_kernel(V2Args<P> a, int k, 
         }
         s->ctl[(k + 1) & 1] = nc;
     }
-    Acc<P> acc;
-    for (int d = 0; d < kDiag; d++) acc.diag[d] = 0.0;
-    double wmax[1] = {0.0};
-    double hmin = 0.0;
-    int ovf = 0;
     if constexpr (P::kQuant) {
         float dg[kDiag];
 #pragma unroll
         for (int d = 0; d < kDiag; d++) dg[d] = 0.0f;
         float wv = 0.0f;
         long long hn_min = 0;
-        strip_update<SH>(a, rows, src, r0, c0, sub_dt, dg, wv, hn_min, ovf);
+        int ovf = 0;
+        strip_update<SH, V2Args<P>, DIAGF>(a, rows, src, r0, c0, sub_dt, dg, wv, hn_min, ovf);
+        // Wave-level epilogue, no barrier: the wave's wave-speed maximum goes
+        // to a spread slot of its own, a failure (negative thickness before
+        // clamping, int32 range) to the flags, and its diagnostic sums (a
+        // fixed-order DPP tree) to its partials, which the fold pass sums in
+        // fixed order.
+        const float wmx = wave_max(wv);
+        const bool neg = __ballot(hn_min < 0) != 0ull;
+        const bool rng = __ballot(ovf != 0) != 0ull;
+        float ds[kDiag];
 #pragma unroll
-        for (int d = 0; d < kDiag; d++) acc.diag[d] += (double)dg[d];
-        hmin = (double)hn_min * kQuantum;
-        wmax[0] = (double)wv;
-    } else {
-        const int n = ny * nx;
-        for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-            const int r = i / nx, c = i - r * nx;
-            CellOut o = cell64(a, src, r, c, sub_dt, acc);
-            wmax[0] = npmax(wmax[0], o.wave);
-            hmin = npmin(hmin, o.hmin);
-            ovf |= o.overflow;
+        for (int d = 0; d < kDiag; d++) {
+            const bool kept = DIAGF == 0 || d == 0 || d == 1 || d == 4 || d == 5 || d == 7;
+            ds[d] = kept ? wave_sum(dg[d]) : 0.0f;
         }
-    }
-    // one field-parallel fold of the workgroup's values: 0 wave-speed maximum,
-    // 1 overflow flag (maximum), 2 minimum new depth, 3.. diagnostic sums
-    double fv[kSubFold];
-    fv[0] = wmax[0];
-    fv[1] = ovf ? 1.0 : 0.0;
-    fv[2] = hmin;
+        if (lane == 0) {
+            atomic_max_nonneg(&a.spread[((k + 1) % kWaveSets) * kSpreadWords + (gw % kSpread) * kSpreadStride],
+                              (double)wmx);
+            const int f = (neg ? 1 : 0) | (rng ? 2 : 0);
+            if (f) atomicOr(&s->flags[(k + 1) & 1], f);
+        }
+        if (lane < kDiag) {
+            float v = ds[0];
 #pragma unroll
-    for (int d = 0; d < kDiag; d++) fv[3 + d] = acc.diag[d];
-    block_fold<double, kSubFold, 0x3u, 0x4u>(fv, fold_lds);
-    if (threadIdx.x == 0) {
-        spread_max(a.spread + ((k + 1) % kWaveSets) * kSpreadWords, fv[0]);
-        const double tol = P::kQuant ? -1.0e-300 : -2.0e-13;
-        int f = 0;
-        if (fv[2] < tol) f |= 1;
-        if (fv[1] != 0.0) f |= 2;
-        if (f) atomicOr(&s->flags[(k + 1) & 1], f);
-        // each workgroup owns its diagnostic slot across the substep launches
-        for (int d = 0; d < kDiag; d++) a.diag_part[(int64_t)d * a.diag_groups + blockIdx.x] = dprev[d] + fv[3 + d];
+            for (int d = 1; d < kDiag; d++) v = lane == d ? ds[d] : v;
+            const bool kept = DIAGF == 0 || lane == 0 || lane == 1 || lane == 4 || lane == 5 || lane == 7;
+            a.diag_part[(int64_t)lane * a.diag_groups + gw] = kept ? dprev + (double)v : 0.0;
+        }
+    } else {
+        Acc<P> acc;
+        for (int d = 0; d < kDiag; d++) acc.diag[d] = 0.0;
+        double wmax[1] = {0.0};
+        double hmin = 0.0;
+        int ovf = 0;
+        {
+            const int n = ny * nx;
+            for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+                const int r = i / nx, c = i - r * nx;
+                CellOut o = cell64(a, src, r, c, sub_dt, acc);
+                wmax[0] = npmax(wmax[0], o.wave);
+                hmin = npmin(hmin, o.hmin);
+                ovf |= o.overflow;
+            }
+        }
+        // one field-parallel fold of the workgroup's values: 0 wave-speed
+        // maximum, 1 overflow flag (maximum), 2 minimum new depth, 3..
+        // diagnostic sums
+        double fv[kSubFold];
+        fv[0] = wmax[0];
+        fv[1] = ovf ? 1.0 : 0.0;
+        fv[2] = hmin;
+#pragma unroll
+        for (int d = 0; d < kDiag; d++) fv[3 + d] = acc.diag[d];
+        block_fold<double, kSubFold, 0x3u, 0x4u>(fv, fold_lds);
+        if (threadIdx.x == 0) {
+            spread_max(a.spread + ((k + 1) % kWaveSets) * kSpreadWords, fv[0]);
+            int f = 0;
+            if (fv[2] < -2.0e-13) f |= 1;
+            if (fv[1] != 0.0) f |= 2;
+            if (f) atomicOr(&s->flags[(k + 1) & 1], f);
+        }
+        // each workgroup owns its diagnostic slot across the substep
+        // launches; the folded sums stay in the fold's LDS (field f at f x kBlock)
+        if (threadIdx.x < kDiag)
+            a.diag_part[(int64_t)threadIdx.x * a.diag_groups + blockIdx.x] =
+                dprev + fold_lds[(3 + threadIdx.x) * kBlock];
     }
 }
 
@@ -759,29 +820,50 @@ __device__ inline float cell_weight32(int mode, int ny, int nx, float area, floa
     return (float)n * f6;
 }
 
-// fold the workgroup's ledger and store it as partial g (before or after)
-__device__ inline void store_ledger32(const V2Args<F32P> &a, const Ledger32 &L, int gc, bool owner, bool before,
-                                      double *lds)
+// 64-bit value of lane `src` (uniform) of the wave
+__device__ inline long long readlane64(long long v, int src)
+{
+    const unsigned long long u = (unsigned long long)v;
+    const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)u, src);
+    const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(u >> 32), src);
+    return (long long)(((unsigned long long)hi << 32) | lo);
+}
+
+// The wave's ledger as partial gw (before or after) of a.groups: fixed-order
+// DPP trees over the lanes (sums and extrema in double, the quanta of each
+// row class as integers), no barrier.  Weight class = row class (first,
+// inner, last row) x column class (first, inner, last column); the first and
+// last columns are at most one lane each and are read back from it.
+__device__ inline void store_ledger32(const V2Args<F32P> &a, const Ledger32 &L, int gc, bool owner, bool before)
 {
     const double rho = a.cfg.density, g = a.cfg.gravity, K = a.cfg.earth_pressure;
-    double f[kLedger32D] = {(double)L.qx, (double)L.qy, (double)L.vmax, (double)L.ke, (double)L.pot, (double)L.prs,
-                            (double)L.hmin};
-    block_fold<double, kLedger32D, 1u << 2, 1u << 6>(f, lds);
-    // quanta per weight class: the lane's column class, the band's row class
-    const int cc = gc == 0 ? 0 : (gc == a.nx - 1 ? 2 : 1);
+    const double f0 = wave_sum((double)L.qx), f1 = wave_sum((double)L.qy);
+    const double f2 = wave_max((double)L.vmax);
+    const double f3 = wave_sum((double)L.ke), f4 = wave_sum((double)L.pot), f5 = wave_sum((double)L.prs);
+    const double f6 = wave_min((double)L.hmin);
+    const unsigned long long first_col = __ballot(owner && gc == 0);
+    const unsigned long long last_col = __ballot(owner && gc == a.nx - 1 && gc != 0);
+    const int lf = first_col ? __ffsll((long long)first_col) - 1 : 0;
+    const int ll = last_col ? __ffsll((long long)last_col) - 1 : 0;
     long long mq[kClasses];
 #pragma unroll
-    for (int q = 0; q < kClasses; q++)
-        mq[q] = (owner && q % 3 == cc) ? (q / 3 == 0 ? L.q_first : (q / 3 == 1 ? L.q_inner : L.q_last)) : 0;
-    block_fold<long long, kClasses>(mq, reinterpret_cast<long long *>(lds));
-    if (threadIdx.x == 0) {
-        const int G = a.groups, gi = blockIdx.x;
-        double *dst = a.ledger_f + (before ? 0 : kSum) * G + gi;
-        const double vals[kSum] = {0.0, rho * f[0], rho * f[1], f[2], 0.5 * rho * f[3], rho * g * f[4],
-                                   rho * 0.5 * K * g * f[5], f[6]};
+    for (int r = 0; r < 3; r++) {
+        const long long v = owner ? (r == 0 ? L.q_first : (r == 1 ? L.q_inner : L.q_last)) : 0;
+        const long long tot = wave_sum(v);
+        const long long e0 = first_col ? readlane64(v, lf) : 0;
+        const long long e2 = last_col ? readlane64(v, ll) : 0;
+        mq[3 * r] = e0;
+        mq[3 * r + 1] = tot - e0 - e2;
+        mq[3 * r + 2] = e2;
+    }
+    if ((threadIdx.x & (kWave - 1)) == 0) {
+        const int G = a.groups, gw = blockIdx.x * (kBlock / kWave) + wave_index();
+        double *dst = a.ledger_f + (before ? 0 : kSum) * G + gw;
+        const double vals[kSum] = {0.0, rho * f0, rho * f1, f2, 0.5 * rho * f3, rho * g * f4,
+                                   rho * 0.5 * K * g * f5, f6};
 #pragma unroll
         for (int q = 0; q < kSum; q++) dst[q * G] = vals[q];
-        long long *qd = a.ledger_q + (before ? 0 : kClasses) * G + gi;
+        long long *qd = a.ledger_q + (before ? 0 : kClasses) * G + gw;
 #pragma unroll
         for (int q = 0; q < kClasses; q++) qd[q * G] = mq[q];
     }
@@ -792,8 +874,12 @@ __device__ inline void store_ledger32(const V2Args<F32P> &a, const Ledger32 &L, 
 template <int SH>
 __global__ void __launch_bounds__(kBlock) v2_begin32_kernel(V2Args<F32P> a)
 {
-    __shared__ double lds[kLedger32D * kBlock];
     const int ny = a.ny, nx = a.nx;
+    // the kernel arguments of the prologue in one batch of scalar loads
+    __asm__ volatile("" ::"s"(ny), "s"(nx), "s"(a.b), "s"(a.h[0]), "s"(a.qx[0]), "s"(a.qy[0]),
+                     "s"(a.cfg.static_exit), "s"(a.cfg.weight_mode), "s"(a.cfg.dry_tolerance),
+                     "s"(a.cfg.earth_pressure), "s"(a.cfg.gravity), "s"(a.cfg.tan_start), "s"(a.dx), "s"(a.dy),
+                     "s"((int)gridDim.x));
     const int tiles_x = (nx + kStripW - 1) / kStripW;
     const int t = xcd_block();
     const int r0 = (t / tiles_x) * (SH * (kBlock / kWave)) + wave_index() * SH, c0 = (t % tiles_x) * kStripW;
@@ -837,26 +923,32 @@ __global__ void __launch_bounds__(kBlock) v2_begin32_kernel(V2Args<F32P> a)
             m[1] = npmax(m[1], (double)fmaxf(fabsf(qxv), fabsf(qyv)));
         }
     }
-    store_ledger32(a, L, gc, owner, true, lds);
-    block_fold<double, 3, 0x7u>(m, lds);
-    if (threadIdx.x == 0) {
+    store_ledger32(a, L, gc, owner, true);
+    // the wave's maxima, one spread slot per wave (no barrier)
+    const double g2 = wave_max(m[0]), qm = wave_max(m[1]), wm = wave_max(m[2]);
+    if (lane == 0) {
+        const int slot = ((blockIdx.x * (kBlock / kWave) + wave_index()) % kSpread) * kSpreadStride;
         if (stat) {
             // the square root is monotonic: the root of the largest square is the largest root
-            spread_max(a.spread + kSetStaticGrad * kSpreadWords, sqrt(m[0]));
-            spread_max(a.spread + kSetStaticQ * kSpreadWords, m[1]);
+            atomic_max_nonneg(&a.spread[kSetStaticGrad * kSpreadWords + slot], sqrt(g2));
+            atomic_max_nonneg(&a.spread[kSetStaticQ * kSpreadWords + slot], qm);
         }
-        spread_max(a.spread, m[2]);
-        if (blockIdx.x == 0) a.sync->ctl[0].remaining = a.cfg.dt;
+        atomic_max_nonneg(&a.spread[slot], wm);
+        if (blockIdx.x == 0 && threadIdx.x == 0) a.sync->ctl[0].remaining = a.cfg.dt;
     }
 }
 
 // copy back when the final state sits in scratch; ledger after (FP32 strips).
 // The body runs in the first substep launch that finds the loop ended (same
-// tile grid as the substeps); lds holds kLedger32D x kBlock doubles.
+// tile grid as the substeps); each wave stores its own ledger partial.
 template <int SH>
-__device__ inline void end32_body(const V2Args<F32P> &a, int fsrc, double *lds)
+__device__ inline void end32_body(const V2Args<F32P> &a, int fsrc)
 {
     const int ny = a.ny, nx = a.nx;
+    // the configuration the ledger needs, in one batch of scalar loads
+    __asm__ volatile("" ::"s"(a.cfg.weight_mode), "s"(a.cfg.dry_tolerance), "s"(a.cfg.density),
+                     "s"(a.cfg.earth_pressure), "s"(a.cfg.gravity), "s"(a.dx), "s"(a.dy), "s"(a.groups),
+                     "s"(a.ledger_f), "s"(a.ledger_q));
     const int tiles_x = (nx + kStripW - 1) / kStripW;
     const int t = xcd_block();
     const int r0 = (t / tiles_x) * (SH * (kBlock / kWave)) + wave_index() * SH, c0 = (t % tiles_x) * kStripW;
@@ -895,16 +987,15 @@ __device__ inline void end32_body(const V2Args<F32P> &a, int fsrc, double *lds)
         }
         L.add(r, ny, hq[j], qxv[j], qyv[j], bv[j], cell_weight32(a.cfg.weight_mode, ny, nx, area, f6, r, gc), dry);
     }
-    store_ledger32(a, L, gc, owner, false, lds);
+    store_ledger32(a, L, gc, owner, false);
 }
 
 template <int SH>
 __global__ void __launch_bounds__(kBlock) v2_end32_kernel(V2Args<F32P> a, int k)
 {
-    __shared__ double lds[kLedger32D * kBlock];
     const Control C = resolve_control(a.sync, k);
     if (C.static_exit) return;
-    end32_body<SH>(a, C.src, lds);
+    end32_body<SH>(a, C.src);
 }
 
 // Fold the partial ledgers and diagnostics in fixed order.  One wave per
@@ -931,20 +1022,22 @@ struct FoldOut {
 };
 static_assert(sizeof(FoldOut) <= 1024, "the fold output fits the pinned result block");
 
-template <int OP, class T>
+// a wave's reduction of n values, lane l taking l, l + 64, ...; U loads per
+// lane are in flight at once
+template <int OP, class T, int U = kFoldUnroll>
 __device__ inline T lane_fold(const T *src, int n, T ident)
 {
     const int lane = threadIdx.x & (kWave - 1);
     T acc = ident;
-    for (int p0 = lane; p0 < n; p0 += kWave * kFoldUnroll) {
-        T x[kFoldUnroll];
+    for (int p0 = lane; p0 < n; p0 += kWave * U) {
+        T x[U];
 #pragma unroll
-        for (int u = 0; u < kFoldUnroll; u++) {
+        for (int u = 0; u < U; u++) {
             const int p = p0 + u * kWave;
             x[u] = p < n ? src[p] : ident;
         }
 #pragma unroll
-        for (int u = 0; u < kFoldUnroll; u++) acc = red_op<OP>(acc, x[u]);
+        for (int u = 0; u < U; u++) acc = red_op<OP>(acc, x[u]);
     }
     return wave_reduce<OP>(acc);
 }
@@ -958,13 +1051,15 @@ __global__ void __launch_bounds__(kBlock) v2_fold_kernel(V2Args<P> a, int k, int
     if (w < 2 * kSum) {
         const double *src = a.ledger_f + (int64_t)w * nparts;
         double v;
-        if ((kFoldMax >> w) & 1u) v = lane_fold<1>(src, nparts, 0.0);
-        else if ((kFoldMin >> w) & 1u) v = lane_fold<2>(src, nparts, 1e300);
-        else v = lane_fold<0>(src, nparts, 0.0);
+        // FP32: one partial per wave of the ledger passes (1 728 at 701 x 701)
+        if ((kFoldMax >> w) & 1u) v = lane_fold<1, double, 32>(src, nparts, 0.0);
+        else if ((kFoldMin >> w) & 1u) v = lane_fold<2, double, 32>(src, nparts, 1e300);
+        else v = lane_fold<0, double, 32>(src, nparts, 0.0);
         if (lane == 0) out->f[w] = v;
     } else if (w < 2 * kSum + kDiag) {
         const int d = w - 2 * kSum;
-        const double v = lane_fold<0>(a.diag_part + (int64_t)d * ndiag, ndiag, 0.0);
+        // FP32: one partial per substep wave (1 728 at 701 x 701)
+        const double v = lane_fold<0, double, 32>(a.diag_part + (int64_t)d * ndiag, ndiag, 0.0);
         if (lane == 0) out->f[w] = v;
     } else if (w < kFoldF) {
         const int set = w == 2 * kSum + kDiag ? kSetStaticGrad : kSetStaticQ;
@@ -973,7 +1068,7 @@ __global__ void __launch_bounds__(kBlock) v2_fold_kernel(V2Args<P> a, int k, int
         if (lane == 0) out->f[w] = v;
     } else if (w < kFoldF + kFoldQ) {
         const int q = w - kFoldF;
-        const long long v = lane_fold<0>(a.ledger_q + (int64_t)q * nparts, nparts, 0ll);
+        const long long v = lane_fold<0, long long, 32>(a.ledger_q + (int64_t)q * nparts, nparts, 0ll);
         if (lane == 0) out->q[q] = v;
     } else if (w == kFoldWaves - 1) {
         if (lane == 0) out->ctl = resolve_control(a.sync, k);
@@ -1049,6 +1144,17 @@ static void enqueue_begin(const V2Args<P> &a, const StepLaunch &L, hipStream_t s
     }
 }
 
+// one FP32 substep launch of band height SH, all ten diagnostic sums or the
+// production record's five
+template <class P, int SH>
+static void launch_substep32(const V2Args<P> &a, int grid, hipStream_t st, int k, int eager)
+{
+    if (a.cfg.diagnostics == 1)
+        hipLaunchKernelGGL((v2_substep_kernel<P, SH, 1>), dim3(grid), dim3(kBlock), 0, st, a, k, eager);
+    else
+        hipLaunchKernelGGL((v2_substep_kernel<P, SH, 0>), dim3(grid), dim3(kBlock), 0, st, a, k, eager);
+}
+
 // substep launches k0 .. k0 + count - 1.  Timing mode brackets the launches
 // expected to work (0 .. eager_until - 1 of the first round) with one event
 // pair: events between launches would add their own latency to every launch.
@@ -1061,11 +1167,11 @@ static int enqueue_substeps(mv2_context *ctx, const V2Args<P> &a, const StepLaun
         const int eager = k < eager_until ? 1 : 0;
         if constexpr (P::kQuant) {
             switch (L.band) {
-            case 2: hipLaunchKernelGGL((v2_substep_kernel<P, 2>), dim3(L.gu), dim3(kBlock), 0, st, a, k, eager); break;
-            case 3: hipLaunchKernelGGL((v2_substep_kernel<P, 3>), dim3(L.gu), dim3(kBlock), 0, st, a, k, eager); break;
-            case 5: hipLaunchKernelGGL((v2_substep_kernel<P, 5>), dim3(L.gu), dim3(kBlock), 0, st, a, k, eager); break;
-            case 6: hipLaunchKernelGGL((v2_substep_kernel<P, 6>), dim3(L.gu), dim3(kBlock), 0, st, a, k, eager); break;
-            default: hipLaunchKernelGGL((v2_substep_kernel<P, 4>), dim3(L.gu), dim3(kBlock), 0, st, a, k, eager); break;
+            case 2: launch_substep32<P, 2>(a, L.gu, st, k, eager); break;
+            case 3: launch_substep32<P, 3>(a, L.gu, st, k, eager); break;
+            case 5: launch_substep32<P, 5>(a, L.gu, st, k, eager); break;
+            case 6: launch_substep32<P, 6>(a, L.gu, st, k, eager); break;
+            default: launch_substep32<P, 4>(a, L.gu, st, k, eager); break;
             }
         } else {
             hipLaunchKernelGGL(v2_substep_kernel<P>, dim3(L.gu), dim3(kBlock), 0, st, a, k, eager);
@@ -1083,7 +1189,7 @@ static void enqueue_close(const V2Args<P> &a, const StepLaunch &L, hipStream_t s
                           unsigned stamp)
 {
     if constexpr (!P::kQuant) hipLaunchKernelGGL(v2_end_kernel<P>, dim3(L.gs), dim3(kBlock), 0, st, a, k);
-    hipLaunchKernelGGL(v2_fold_kernel<P>, dim3(L.gfold), dim3(kBlock), 0, st, a, k, L.gs, L.gu, L.fold, stamps,
+    hipLaunchKernelGGL(v2_fold_kernel<P>, dim3(L.gfold), dim3(kBlock), 0, st, a, k, a.groups, a.diag_groups, L.fold, stamps,
                        stamp);
 }
 
@@ -1197,9 +1303,14 @@ static int launch_step(mv2_context *ctx, const mv2_step_config *cfg, mv2_step_re
     const int gl = P::kQuant ? ((ctx->nx + kStripW - 1) / kStripW) * ((ctx->ny + rows_sub - 1) / rows_sub) : 0;
     // begin / end passes: the substep tiles (FP32), a grid stride (FP64)
     const int gs = P::kQuant ? gl : grid_for(n, kBlock * kLedgerU, kLedgerGroupsMax);
-    if (int rc = ensure_scratch(ctx, ctx->partials, (sizeof(double) * kLedgerF + sizeof(long long) * kLedgerQ) * (size_t)gs))
+    // ledger partials: one per wave of the FP32 begin / end passes, one per workgroup in FP64
+    const int gpart = P::kQuant ? gs * (kBlock / kWave) : gs;
+    if (int rc = ensure_scratch(ctx, ctx->partials,
+                                (sizeof(double) * kLedgerF + sizeof(long long) * kLedgerQ) * (size_t)gpart))
         return rc;
-    if (int rc = ensure_scratch(ctx, ctx->step_diag, sizeof(double) * kDiag * (size_t)gu)) return rc;
+    // diagnostic partials: one per substep wave (FP32) or workgroup (FP64)
+    const int gdiag = P::kQuant ? gu * (kBlock / kWave) : gu;
+    if (int rc = ensure_scratch(ctx, ctx->step_diag, sizeof(double) * kDiag * (size_t)gdiag)) return rc;
     if (int rc = ensure_scratch(ctx, ctx->spread, kSpreadScratchBytes)) return rc;
     V2Args<P> a;
     std::memset(&a, 0, sizeof(a));      // padding bytes are part of the graph key
@@ -1225,10 +1336,10 @@ static int launch_step(mv2_context *ctx, const mv2_step_config *cfg, mv2_step_re
     a.sync_next = (SyncBlock *)((char *)ctx->sync.ptr + (parity ^ 1) * kSyncParityBytes);
     a.spread_next = parity ? spread_even : spread_odd;
     a.ledger_f = (double *)ctx->partials.ptr;
-    a.ledger_q = (long long *)(a.ledger_f + (size_t)kLedgerF * gs);
-    a.groups = gs;
+    a.ledger_q = (long long *)(a.ledger_f + (size_t)kLedgerF * gpart);
+    a.groups = gpart;
     a.diag_part = (double *)ctx->step_diag.ptr;
-    a.diag_groups = gu;
+    a.diag_groups = gdiag;
     a.spread = parity ? spread_odd : spread_even;
     // the fold pass writes its results straight into pinned host memory
     // (device-visible): no copy launch before the stream sync
@@ -1339,6 +1450,8 @@ extern "C" int mv2_step(mv2_context *ctx, const mv2_step_config *cfg, mv2_step_r
     if (!(cfg->cfl > 0.0) || !(cfg->earth_pressure > 0.0) || !(cfg->gravity > 0.0) ||
         !(cfg->dry_tolerance > 0.0) || !(cfg->basal_friction > 0.0))
         return set_error(ctx, MV2_EINVAL, "MobileV2 configuration must be finite/positive");
+    if (cfg->diagnostics != 0 && cfg->diagnostics != 1)
+        return set_error(ctx, MV2_EINVAL, "[MobileV2] diagnostics must be 0 (all sums) or 1 (production record)");
     if (hipSetDevice(ctx->device) != hipSuccess) return set_error(ctx, MV2_EDEVICE, "hipSetDevice failed");
     int rc = ctx->precision == MV2_FP64 ? launch_step<F64P>(ctx, cfg, out)
                                         : launch_step<F32P>(ctx, cfg, out);

@@ -31,6 +31,7 @@ class StepConfig(ctypes.Structure):
         ("dry_tolerance", _D), ("basal_friction", _D), ("tan_start", _D),
         ("external_ax", _D), ("external_ay", _D), ("density", _D),
         ("maximum_substeps", _I), ("limit_mode", _I), ("weight_mode", _I), ("static_exit", _I),
+        ("diagnostics", _I),
     ]
 
 

@@ -101,14 +101,16 @@ class MobileV2Step:
 
 
 def _step_config(config: MobileV2Config, dt: float, *, density: float, limit_mode: int,
-                 weight_mode: int, external=(0.0, 0.0)) -> _lib.StepConfig:
+                 weight_mode: int, external=(0.0, 0.0), diagnostics: int = 0) -> _lib.StepConfig:
+    """C step configuration.  ``diagnostics`` 1 keeps only the five per-substep
+    sums the production record reads (warp_mobile_v2.py:220-241)."""
     return _lib.StepConfig(
         dt=float(dt), gravity=config.gravity_m_s2, earth_pressure=config.earth_pressure_coefficient,
         cfl=config.cfl, dry_tolerance=config.dry_tolerance_m,
         basal_friction=config.basal_friction_coefficient, tan_start=config.tan_start,
         external_ax=float(external[0]), external_ay=float(external[1]), density=float(density),
         maximum_substeps=int(config.maximum_substeps), limit_mode=int(limit_mode),
-        weight_mode=int(weight_mode), static_exit=1)
+        weight_mode=int(weight_mode), static_exit=1, diagnostics=int(diagnostics))
 
 
 _RECORD_BYTES = ctypes.sizeof(_lib.StepRecord)
@@ -116,13 +118,14 @@ _CONFIG_CACHE: dict = {}
 
 
 def run_step(state: DeviceBulkState, config: MobileV2Config, dt: float, *, density: float,
-             limit_mode: int, weight_mode: int, external=(0.0, 0.0),
+             limit_mode: int, weight_mode: int, external=(0.0, 0.0), diagnostics: int = 0,
              error_prefix: str = "") -> _lib.StepRecord:
     """Run one device step on ``state`` and return the raw record."""
     rt = state.runtime
     rt.bind_stream()
     # configs are frozen dataclasses: the C struct of a (config, dt, ...) key is reused
-    key = (config, float(dt), float(density), int(limit_mode), int(weight_mode), tuple(external))
+    key = (config, float(dt), float(density), int(limit_mode), int(weight_mode), tuple(external),
+           int(diagnostics))
     try:
         cfg = _CONFIG_CACHE.get(key)
     except TypeError:                      # an unhashable caller config: no caching
@@ -131,7 +134,7 @@ def run_step(state: DeviceBulkState, config: MobileV2Config, dt: float, *, densi
         if len(_CONFIG_CACHE) > 64:
             _CONFIG_CACHE.clear()
         cfg = _step_config(config, dt, density=density, limit_mode=limit_mode,
-                           weight_mode=weight_mode, external=external)
+                           weight_mode=weight_mode, external=external, diagnostics=diagnostics)
         if key is not None:
             _CONFIG_CACHE[key] = cfg
     rec = _lib.StepRecord()
@@ -255,7 +258,7 @@ class HipProductionMobileV2Solver:
         state = self.state
         density = float(self.material.assumed_bulk_density_kg_m3)
         rec = run_step(state, self.config, float(dt_s), density=density, limit_mode=1,
-                       weight_mode=1, error_prefix="[MobileV2Production] ")
+                       weight_mode=1, diagnostics=1, error_prefix="[MobileV2Production] ")
         self.last_record = rec
         return step_from_record(rec, state.grid.dx * state.grid.dy, density)
 

@@ -135,6 +135,50 @@ def test_production_step_resident_matches_oracle_and_ledgers():
     assert g[0] == pytest.approx(orec["diag"][0] * 1370.0 * 0.0025, rel=1e-9, abs=1e-12)
 
 
+def test_fp32_production_diagnostics_mode():
+    """FP32 steps with the production record's five per-substep sums
+    (diagnostics = 1, warp_mobile_v2.py:220-241) against the full mode on the
+    same frames: fields and substep counts identical, the five sums identical
+    bit for bit, the other five 0; the production record's impulses and
+    friction work track the FP64 oracle's within FP32 tolerance."""
+    from isaac_bulk_hip import DeviceBulkState, MobileV2Config, TerrainGrid
+    from isaac_bulk_hip.mobile_v2 import run_step
+    from isaac_bulk_hip.workloads import pile_with_patch
+
+    n, dt = 96, 1.0 / 60.0
+    b, h, qx, qy = pile_with_patch(n, seed=11)
+    b = b.astype(np.float32).astype(np.float64)
+    h = np.rint(h * 2.0 ** 24) * 2.0 ** -24
+    cfg = MobileV2Config(start_angle_deg=38.0, stop_angle_deg=30.0, basal_friction_coefficient=0.35)
+    runs = {}
+    for mode in (0, 1):
+        state = DeviceBulkState(TerrainGrid(n, n, 0.05, 0.05), b, precision="fp32")
+        state.set_fields(mobile=h)
+        recs = [run_step(state, cfg, dt, density=1.0, limit_mode=1, weight_mode=1, diagnostics=mode)
+                for _ in range(3)]
+        view = state.explicit_host_view(source="acceptance")
+        runs[mode] = (view.H_mobile_m.copy(), view.mobile_momentum_m2_s.copy(),
+                      [np.array(r.diag[:]) for r in recs], [r.substeps for r in recs])
+    kept, dropped = [0, 1, 4, 5, 7], [2, 3, 6, 8, 9]
+    np.testing.assert_array_equal(runs[0][0], runs[1][0])
+    np.testing.assert_array_equal(runs[0][1], runs[1][1])
+    assert runs[0][3] == runs[1][3] and min(runs[0][3]) > 0
+    for d0, d1 in zip(runs[0][2], runs[1][2]):
+        np.testing.assert_array_equal(d0[kept], d1[kept])
+        assert np.all(d1[dropped] == 0.0)
+    # the first frame's sums against the FP64 oracle from the same state (the
+    # device's sum 7 is twice the kinetic energy removed by friction, the
+    # record halves it)
+    _, _, _, orec = oracle.v2_step(b, h, qx, qy, dt, mu=0.35, start_angle_deg=38.0, limit_mode=1)
+    for d in kept:
+        got = runs[1][2][0][d] * (0.5 if d == 7 else 1.0)
+        assert got == pytest.approx(orec["diag"][d], rel=2e-3, abs=1e-9), d
+    with pytest.raises(ValueError, match="diagnostics"):
+        state = DeviceBulkState(TerrainGrid(n, n, 0.05, 0.05), b, precision="fp32")
+        state.set_fields(mobile=h)
+        run_step(state, cfg, dt, density=1.0, limit_mode=1, weight_mode=1, diagnostics=2)
+
+
 def test_static_state_takes_zero_substeps():
     from isaac_bulk_hip import MobileV2State
 
