@@ -70,6 +70,20 @@ __device__ inline double from_prev_lane_d(double v)
 
 struct Face32 { int h; float qxl, qyl, qxr, qyr; };
 
+// Element at a 32-bit byte offset from a wave-uniform base: the address is
+// the scalar base plus the unsigned per-lane offset, one global access with
+// no 64-bit address arithmetic per lane (fields hold fewer than 2^30 cells)
+template <class T>
+__device__ inline T &at_bytes(T *base, unsigned off)
+{
+    return *reinterpret_cast<T *>(reinterpret_cast<char *>(base) + off);
+}
+template <class T>
+__device__ inline const T &at_bytes(const T *base, unsigned off)
+{
+    return *reinterpret_cast<const T *>(reinterpret_cast<const char *>(base) + off);
+}
+
 // FP32 constants of one substep, rounded once from the FP64 configuration and
 // kept in scalar registers
 struct Consts32 {
@@ -223,7 +237,11 @@ struct StripRows {
 
 // Load the wave's band (first row r0, first column c0) with its halo.  Every
 // load of the lane is issued before any is used (one memory round trip);
-// cells outside the terrain read as zero and feed no face that is used.
+// cells outside the terrain read as zero and feed no face that is used.  The
+// loads are unconditional (indices clamped into the terrain, the value then
+// replaced by zero): a bounds branch around each row would split the loads
+// into blocks, across which the compiler's waits can no longer count on
+// loads returning in issue order and fall back to waiting for all of them.
 template <int SH, class A>
 __device__ inline void strip_load(const A &a, int src, int r0, int c0, StripRows<SH> &R)
 {
@@ -231,6 +249,7 @@ __device__ inline void strip_load(const A &a, int src, int r0, int c0, StripRows
     const int ny = a.ny, nx = a.nx;
     const int gc = c0 - 1 + lane;
     const bool col_in = gc >= 0 && gc < nx;
+    const int cc = min(max(gc, 0), nx - 1);
     const float *gb = a.b;
     const int32_t *gh = (src ? a.h[1] : a.h[0]);
     const float *gqx = (src ? a.qx[1] : a.qx[0]);
@@ -238,14 +257,14 @@ __device__ inline void strip_load(const A &a, int src, int r0, int c0, StripRows
 #pragma unroll
     for (int j = 0; j < SH + 2; j++) {
         const int gr = r0 - 1 + j;
-        R.b[j] = 0.0f; R.qx[j] = 0.0f; R.qy[j] = 0.0f; R.hq[j] = 0;
-        if (col_in && gr >= 0 && gr < ny) {
-            const int i = gr * nx + gc;
-            R.b[j] = gb[i];
-            R.hq[j] = gh[i];
-            R.qx[j] = gqx[i];
-            R.qy[j] = gqy[i];
-        }
+        const bool in = col_in && gr >= 0 && gr < ny;
+        const unsigned o = 4u * (unsigned)(min(max(gr, 0), ny - 1) * nx + cc);
+        const float bv = at_bytes(gb, o), qxv = at_bytes(gqx, o), qyv = at_bytes(gqy, o);
+        const int hv = at_bytes(gh, o);
+        R.b[j] = in ? bv : 0.0f;
+        R.hq[j] = in ? hv : 0;
+        R.qx[j] = in ? qxv : 0.0f;
+        R.qy[j] = in ? qyv : 0.0f;
     }
 }
 
@@ -253,15 +272,23 @@ __device__ inline void strip_load(const A &a, int src, int r0, int c0, StripRows
 // the substep diagnostics of its cells, wv the CFL wave speed of the new
 // state, hn_min the smallest new thickness (quanta) before clamping, ovf the
 // int32 range failure.  DIAGF 1 gathers only the sums the production record
-// reads (0, 1, 4, 5, 7); the others stay 0.
-template <int SH, class A, int DIAGF = 0>
+// reads (0, 1, 4, 5, 7); the others stay 0.  INTERIOR: the band touches no
+// edge of the terrain (strip_interior), so every cell has all four faces and
+// no bounds test is needed.
+template <int SH>
+__device__ inline bool strip_interior(int ny, int nx, int r0, int c0)
+{
+    return r0 > 0 && r0 + SH < ny && c0 > 0 && c0 + kStripW < nx;
+}
+
+template <int SH, class A, int DIAGF = 0, bool INTERIOR = false>
 __device__ inline void strip_update(const A &a, const StripRows<SH> &R, int src, int r0, int c0, double sub_dt,
                                     float (&dg)[kDiag], float &wv, long long &hn_min, int &ovf)
 {
     const int lane = threadIdx.x & (kWave - 1);
     const int ny = a.ny, nx = a.nx;
     const int gc = c0 - 1 + lane;
-    const bool owner = lane >= 1 && lane <= kStripW && gc < nx;
+    const bool owner = lane >= 1 && lane <= kStripW && (INTERIOR || gc < nx);
     const Consts32 k = consts32(a.cfg, sub_dt, a.dx, a.dy, a.clamp_quarter);
     const float dt = k.dt, decrement = k.decrement;
     const int dst = src ^ 1;
@@ -269,26 +296,28 @@ __device__ inline void strip_update(const A &a, const StripRows<SH> &R, int src,
     float *oqx = (dst ? a.qx[1] : a.qx[0]);
     float *oqy = (dst ? a.qy[1] : a.qy[0]);
 
-    // the sides of every row (band and halo)
-    Side32 sd[SH + 2];
-#pragma unroll
-    for (int j = 0; j < SH + 2; j++) sd[j] = side32(R.b[j], R.hq[j], R.qx[j], R.qy[j], k);
     // per direction: dt / spacing, half of it, half of it in quanta per metre
     const float hfx = 0.5f * k.fxf, hfy = 0.5f * k.fyf;
     const float qfx = uniform(hfx * (float)kInvQuantum), qfy = uniform(hfy * (float)kInvQuantum);
+    // The sides of the rows are formed as the loop reaches them (row j + 1
+    // in step j), so the rows' loads are waited for one by one and the work
+    // of the first rows overlaps the arrival of the later ones.
+    Side32 cur = side32(R.b[1], R.hq[1], R.qx[1], R.qy[1], k);
     // y-face between halo row 0 and band row 1
-    Face32 fdown = face32v<1>(sd[0], sd[1], k, k.fyf, hfy, qfy);
+    Face32 fdown = face32v<1>(side32(R.b[0], R.hq[0], R.qx[0], R.qy[0], k), cur, k, k.fyf, hfy, qfy);
 #pragma unroll
     for (int j = 1; j <= SH; j++) {
         const int gr = r0 - 1 + j;
-        const Face32 fup = face32v<1>(sd[j], sd[j + 1], k, k.fyf, hfy, qfy);
+        const Side32 next = side32(R.b[j + 1], R.hq[j + 1], R.qx[j + 1], R.qy[j + 1], k);
+        const Face32 fup = face32v<1>(cur, next, k, k.fyf, hfy, qfy);
         // x-face between this lane's column and the next
-        const Face32 fx = face32v<0>(sd[j], side_from_next_lane(sd[j]), k, k.fxf, hfx, qfx);
+        const Face32 fx = face32v<0>(cur, side_from_next_lane(cur), k, k.fxf, hfx, qfx);
         // the x-face on the left of this lane's column, from the lane before
         const int lq = from_prev_lane(fx.h);
         const float lqx = from_prev_lane(fx.qxr), lqy = from_prev_lane(fx.qyr);
-        if (owner && gr < ny) {
-            const bool has_r = gc + 1 < nx, has_l = gc > 0, has_u = gr + 1 < ny, has_d = gr > 0;
+        if (owner && (INTERIOR || gr < ny)) {
+            const bool has_r = INTERIOR || gc + 1 < nx, has_l = INTERIOR || gc > 0;
+            const bool has_u = INTERIOR || gr + 1 < ny, has_d = INTERIOR || gr > 0;
             float dqx = 0.0f, dqy = 0.0f;
             if (has_r) {
                 dqx = dqx - fx.qxl; dqy = dqy - fx.qyl;
@@ -337,8 +366,8 @@ __device__ inline void strip_update(const A &a, const StripRows<SH> &R, int src,
             if (wet) {
                 float ax = k.ax, ay = k.ay;
                 if (a.ext_x) {                          // per-cell external acceleration
-                    ax = a.ext_x[gr * nx + gc];
-                    ay = a.ext_y[gr * nx + gc];
+                    ax = at_bytes(a.ext_x, 4u * (unsigned)(gr * nx + gc));
+                    ay = at_bytes(a.ext_y, 4u * (unsigned)(gr * nx + gc));
                 }
                 const float inv = rcpf(depth);
                 const float vbx = qxn * inv, vby = qyn * inv;
@@ -362,13 +391,14 @@ __device__ inline void strip_update(const A &a, const StripRows<SH> &R, int src,
                 dg[8] += (wet ? qxn : 0.0f) - qxi;
                 dg[9] += (wet ? qyn : 0.0f) - qyi;
             }
-            const int i = gr * nx + gc;
-            oh[i] = hq;
-            oqx[i] = qax;
-            oqy[i] = qay;
+            const unsigned o = 4u * (unsigned)(gr * nx + gc);
+            at_bytes(oh, o) = hq;
+            at_bytes(oqx, o) = qax;
+            at_bytes(oqy, o) = qay;
             wv = fmaxf(wv, wave_v + sqrtf_hw(k.Kg * depth));
         }
         fdown = fup;
+        cur = next;
     }
 }
 

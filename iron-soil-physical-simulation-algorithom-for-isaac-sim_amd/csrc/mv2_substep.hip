@@ -418,10 +418,23 @@ __device__ inline void store_ledger(const V2Args<P> &a, LedgerAcc &L, bool befor
 // the loop ends with the error); flags of older launches were already folded
 // in by the launch after them (rolled == 1).  Both words were written by
 // earlier launches of the stream, so plain loads see them.
+// The record and flags of launch k were written by earlier launches (launch
+// k writes only the other slot), so they are read through the constant
+// address space: scalar loads, counted apart from the vector loads a launch
+// issues ahead of them.
+typedef const __attribute__((address_space(4))) unsigned ConstWord;
+
 __device__ inline Control resolve_control(const SyncBlock *s, int k)
 {
-    Control C = s->ctl[k & 1];
-    const int f = s->flags[k & 1];
+    static_assert(sizeof(Control) % 4 == 0, "the record is read as 32-bit words");
+    constexpr int kWords = (int)(sizeof(Control) / 4);
+    ConstWord *src = (ConstWord *)&s->ctl[k & 1];
+    unsigned w[kWords];
+#pragma unroll
+    for (int i = 0; i < kWords; i++) w[i] = src[i];
+    Control C;
+    __builtin_memcpy(&C, w, sizeof(Control));
+    const int f = (int)*(ConstWord *)&s->flags[k & 1];
     if (f && !C.rolled) {
         C.numeric = (f & 2) ? 3 : 1;
         C.src = C.last_src;
@@ -507,17 +520,21 @@ __global__ void __launch_bounds__(kBlock) v2_begin_kernel(V2Args<P> a)
 // the diagnostics of the substep.
 //
 // A working launch k always finds the current state in buffer k & 1 (every
-// earlier launch worked and flipped it).  With `eager` set, the launch stages
-// its tile from that buffer before the loop state and the CFL maximum arrive,
-// so all three loads share one memory round trip; launches expected to find
-// the loop ended (eager = 0) read the loop state first and skip the staging.
+// earlier launch worked and flipped it).  An EAGER launch stages its tile
+// from that buffer before the loop state and the CFL maximum arrive, so all
+// three loads share one memory round trip; launches expected to find the loop
+// ended (EAGER = 0) read the loop state first and skip the staging.  EAGER is
+// a template parameter: a run-time branch around the staging would join two
+// paths with different loads in flight, and the compiler's wait at the join
+// would hold the eager path's first use of the CFL maximum until its whole
+// tile had arrived.
 constexpr int kSubFold = 3 + kDiag;   // values folded per workgroup at the end of a substep
 
 template <int SH>
 __device__ inline void end32_body(const V2Args<F32P> &a, int fsrc);
 
-template <class P, int SH = kSH, int DIAGF = 0>
-__global__ void __launch_bounds__(kBlock) v2_substep_kernel(V2Args<P> a, int k, int eager)
+template <class P, int SH = kSH, int DIAGF = 0, int EAGER = 0>
+__global__ void __launch_bounds__(kBlock) v2_substep_kernel(V2Args<P> a, int k)
 {
     constexpr int kRows = SH * (kBlock / kWave);     // rows of a tile
     __shared__ double smem[64 * 4];
@@ -535,7 +552,7 @@ __global__ void __launch_bounds__(kBlock) v2_substep_kernel(V2Args<P> a, int k, 
         // ahead of the tile's loads.
         __asm__ volatile("" ::"s"(ny), "s"(nx), "s"(a.b), "s"(a.h[0]), "s"(a.h[1]), "s"(a.qx[0]), "s"(a.qx[1]),
                          "s"(a.qy[0]), "s"(a.qy[1]), "s"(a.spread), "s"(a.diag_part), "s"(a.diag_groups),
-                         "s"(a.sync), "s"(a.ext_x), "s"(k), "s"(eager), "s"((int)gridDim.x), "s"(a.cfg.density),
+                         "s"(a.sync), "s"(a.ext_x), "s"(k), "s"((int)gridDim.x), "s"(a.cfg.density),
                          "s"(a.clamp_quarter));
     }
     const int tiles_x = (nx + kStripW - 1) / kStripW;
@@ -555,27 +572,32 @@ __global__ void __launch_bounds__(kBlock) v2_substep_kernel(V2Args<P> a, int k, 
     const int c0 = (t % tiles_x) * kStripW;
     // FP32: each wave keeps its own diagnostic partials (global wave index)
     const int gw = blockIdx.x * (kBlock / kWave) + wave_index();
-    // Requests that do not depend on the loop state, the tile first.  The
-    // spread slots and the diagnostic slot are vector loads (lane-dependent
-    // addresses): a scalar load here would be waited for before the tile's
-    // loads could issue.
-    StripRows<SH> rows;
-    if constexpr (P::kQuant) {
-        if (eager) strip_load<SH>(a, src, r0, c0, rows);
-    }
+    // Requests that do not depend on the loop state.  Loads return in issue
+    // order, so the spread slots (needed first, for the substep's dt) go out
+    // before the tile, whose rows are then waited for one by one as the
+    // update reaches them; the diagnostic slots (needed last) go out after
+    // it.  All are vector loads (lane-dependent addresses): a scalar load
+    // here would be waited for before the tile's loads could issue.
     // FP32: every wave folds the 64 slots of the previous launch's maximum
     // itself (no barrier); FP64: the first wave, through LDS
     double slot = 0.0;
     if (P::kQuant || threadIdx.x < kWave)
         slot = __longlong_as_double((long long)a.spread[(k % kWaveSets) * kSpreadWords + lane * kSpreadStride]);
+    StripRows<SH> rows;
+    if constexpr (P::kQuant) {
+        if constexpr (EAGER) strip_load<SH>(a, src, r0, c0, rows);
+    }
     // the diagnostic slots accumulate over the working launches; launch 0 is
     // the first of them and starts them from zero.  Lane d holds field d: of
     // every wave (FP32) or of the workgroup's first wave (FP64).
     double dprev = 0.0;
+    if constexpr (P::kQuant) {
+        // unconditional (clamped) for the same reason as the tile's loads
+        const double v = a.diag_part[(int64_t)min(lane, kDiag - 1) * a.diag_groups + gw];
+        dprev = k != 0 && lane < kDiag ? v : 0.0;
+    }
     if (k != 0) {
-        if constexpr (P::kQuant) {
-            if (lane < kDiag) dprev = a.diag_part[(int64_t)lane * a.diag_groups + gw];
-        } else {
+        if constexpr (!P::kQuant) {
             if (threadIdx.x < kDiag) dprev = a.diag_part[(int64_t)threadIdx.x * a.diag_groups + blockIdx.x];
         }
     }
@@ -605,7 +627,7 @@ __global__ void __launch_bounds__(kBlock) v2_substep_kernel(V2Args<P> a, int k, 
         return;
     }
     if constexpr (P::kQuant) {
-        if (!eager) strip_load<SH>(a, src, r0, c0, rows);
+        if constexpr (!EAGER) strip_load<SH>(a, src, r0, c0, rows);
     }
     // CFL maximum of the current state (mobile_v2_reference.py:239-251)
     double maxw;
@@ -652,7 +674,10 @@ __global__ void __launch_bounds__(kBlock) v2_substep_kernel(V2Args<P> a, int k, 
         float wv = 0.0f;
         long long hn_min = 0;
         int ovf = 0;
-        strip_update<SH, V2Args<P>, DIAGF>(a, rows, src, r0, c0, sub_dt, dg, wv, hn_min, ovf);
+        if (strip_interior<SH>(ny, nx, r0, c0))
+            strip_update<SH, V2Args<P>, DIAGF, true>(a, rows, src, r0, c0, sub_dt, dg, wv, hn_min, ovf);
+        else
+            strip_update<SH, V2Args<P>, DIAGF, false>(a, rows, src, r0, c0, sub_dt, dg, wv, hn_min, ovf);
         // Wave-level epilogue, no barrier: the wave's wave-speed maximum goes
         // to a spread slot of its own, a failure (negative thickness before
         // clamping, int32 range) to the flags, and its diagnostic sums (a
@@ -1149,10 +1174,11 @@ static void enqueue_begin(const V2Args<P> &a, const StepLaunch &L, hipStream_t s
 template <class P, int SH>
 static void launch_substep32(const V2Args<P> &a, int grid, hipStream_t st, int k, int eager)
 {
-    if (a.cfg.diagnostics == 1)
-        hipLaunchKernelGGL((v2_substep_kernel<P, SH, 1>), dim3(grid), dim3(kBlock), 0, st, a, k, eager);
-    else
-        hipLaunchKernelGGL((v2_substep_kernel<P, SH, 0>), dim3(grid), dim3(kBlock), 0, st, a, k, eager);
+    const int d = a.cfg.diagnostics == 1 ? 1 : 0;
+    if (d && eager) hipLaunchKernelGGL((v2_substep_kernel<P, SH, 1, 1>), dim3(grid), dim3(kBlock), 0, st, a, k);
+    else if (d) hipLaunchKernelGGL((v2_substep_kernel<P, SH, 1, 0>), dim3(grid), dim3(kBlock), 0, st, a, k);
+    else if (eager) hipLaunchKernelGGL((v2_substep_kernel<P, SH, 0, 1>), dim3(grid), dim3(kBlock), 0, st, a, k);
+    else hipLaunchKernelGGL((v2_substep_kernel<P, SH, 0, 0>), dim3(grid), dim3(kBlock), 0, st, a, k);
 }
 
 // substep launches k0 .. k0 + count - 1.  Timing mode brackets the launches
@@ -1174,7 +1200,7 @@ static int enqueue_substeps(mv2_context *ctx, const V2Args<P> &a, const StepLaun
             default: launch_substep32<P, 4>(a, L.gu, st, k, eager); break;
             }
         } else {
-            hipLaunchKernelGGL(v2_substep_kernel<P>, dim3(L.gu), dim3(kBlock), 0, st, a, k, eager);
+            hipLaunchKernelGGL(v2_substep_kernel<P>, dim3(L.gu), dim3(kBlock), 0, st, a, k);
         }
         if (timing && k0 == 0 && k == eager_until - 1) MV2_HIP(ctx, hipEventRecord(ctx->sub_events[1], st));
     }
