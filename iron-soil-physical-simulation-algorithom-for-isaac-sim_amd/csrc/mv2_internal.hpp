@@ -397,6 +397,26 @@ __device__ inline int xcd_block()
     const int x = b % 8, j = b / 8;
     return x * (G / 8) + min(x, G % 8) + j;
 }
+// A record written by an earlier launch (and not by this one), read through
+// the constant address space as 32-bit words: scalar loads, counted apart
+// from the vector loads a launch issues ahead of them.  A plain load may
+// become a vector load, whose wait would then also cover every vector load
+// issued before it.
+template <class T>
+__device__ inline T load_launch_record(const T *p)
+{
+    static_assert(sizeof(T) % 4 == 0, "the record is read as 32-bit words");
+    constexpr int kWords = (int)(sizeof(T) / 4);
+    typedef const __attribute__((address_space(4))) unsigned ConstWord;
+    ConstWord *src = (ConstWord *)p;
+    unsigned w[kWords];
+#pragma unroll
+    for (int i = 0; i < kWords; i++) w[i] = src[i];
+    T v;
+    __builtin_memcpy(&v, w, sizeof(T));
+    return v;
+}
+
 // ---- spread maxima -----------------------------------------------------------------
 // Device-scope atomics on one address serialise (~12 ns each on MI355X, so a
 // launch of ~1000 workgroups pays ~12 us for one global maximum).  A spread

@@ -96,33 +96,33 @@ __device__ inline int relax_group(const RelaxArgs<R> &a, int k)
     return k == 0 ? a.first_group : k % kRelaxGroups;
 }
 
-// this thread's slot of each spread set the previous launch reduced into
-// (threads 0..63; loaded before the loop state, whose test reads them)
+// this lane's slot of each spread set the previous launch reduced into, in
+// every wave (loaded before the loop state, whose test reads them; every
+// wave folds the 64 slots itself, so the test needs no barrier)
 template <class R>
 __device__ inline void relax_prev_slots(const RelaxArgs<R> &a, int k, double (&slot)[kRelaxKMax])
 {
+    const int g = relax_group(a, k > 0 ? k - 1 : 0);
+    const int lane = threadIdx.x & (kWave - 1);
 #pragma unroll
-    for (int i = 0; i < kRelaxKMax; i++) slot[i] = 0.0;
-    if (k > 0 && threadIdx.x < kSpread) {
-        const int g = relax_group(a, k - 1);
-#pragma unroll
-        for (int i = 0; i < kRelaxKMax; i++)
-            slot[i] = __longlong_as_double((long long)relax_set(a.spread, g, i)[threadIdx.x * kSpreadStride]);
+    for (int i = 0; i < kRelaxKMax; i++) {
+        const double v = __longlong_as_double((long long)relax_set(a.spread, g, i)[lane * kSpreadStride]);
+        slot[i] = k > 0 ? v : 0.0;
     }
 }
 
 // loop state for launch k: test the previous launch's iterations in order
-// (slot: relax_prev_slots).  Every thread of the block calls it.
+// (slot: relax_prev_slots).  Each wave reaches the same decision from the
+// same slots (a fixed DPP tree), with no barrier.
 template <class R>
-__device__ inline RelaxControl relax_resolve(const RelaxArgs<R> &a, int k, double *smem1,
-                                             const double (&slot)[kRelaxKMax])
+__device__ inline RelaxControl relax_resolve(const RelaxArgs<R> &a, int k, const double (&slot)[kRelaxKMax])
 {
     RelaxControl C;
     if (k == 0) {
         C = RelaxControl{};
         return C;
     }
-    C = a.sync->ctl[k & 1];      // written by an earlier launch: plain load
+    C = load_launch_record(&a.sync->ctl[k & 1]);     // written by an earlier launch
     if (!C.done && C.replay == 0 && C.n_prev > 0) {
         int first = -1;
         double mfirst = 0.0, mlast = 0.0;
@@ -131,13 +131,7 @@ __device__ inline RelaxControl relax_resolve(const RelaxArgs<R> &a, int k, doubl
 #pragma unroll
             for (int j = 0; j < kRelaxKMax; j++)
                 if (j == i) m = slot[j];
-            if (threadIdx.x < kWave) {
-                m = wave_max(m);
-                if (threadIdx.x == 0) *smem1 = m;
-            }
-            __syncthreads();
-            m = *smem1;
-            __syncthreads();
+            m = wave_max(m);
             if (first < 0 && m <= a.tolerance) {
                 first = i;
                 mfirst = m;
@@ -172,6 +166,7 @@ __global__ void __launch_bounds__(G::BLOCK) relax_kernel(RelaxArgs<R> a, int k)
 {
     __shared__ R tile[G::kCells];
     __shared__ double smem[kRelaxKMax * (G::BLOCK / kWave)];
+    __shared__ int wflag[G::BLOCK / kWave];
     RelaxSync *s = a.sync;
     const bool lead = blockIdx.x == 0 && threadIdx.x == 0;
     const int t = blockIdx.x;
@@ -185,17 +180,26 @@ __global__ void __launch_bounds__(G::BLOCK) relax_kernel(RelaxArgs<R> a, int k)
     // (a replay reads the other buffer and reloads below).
     double slot[kRelaxKMax];
     relax_prev_slots(a, k, slot);
-    int qflag = 0;
-    if (k > 0 && threadIdx.x < 9) {
+    // Every load of the prologue is unconditional (clamped indices, values
+    // selected afterwards), so the prologue is one block whose waits can
+    // count on loads returning in issue order: loads under branches are
+    // waited for in full at the join.  Lanes 0..8 of every wave read the
+    // changed flags of the 3 x 3 tile neighbourhood; each wave tests them
+    // with a ballot (no barrier).
+    const int lane = threadIdx.x & (kWave - 1);
+    int qraw;
+    bool qown;
+    {
         const int ty = a.tiles / a.tiles_x;
-        const int rr = trow + (int)threadIdx.x / 3 - 1, cc = tcol + (int)threadIdx.x % 3 - 1;
-        if (rr >= 0 && rr < ty && cc >= 0 && cc < a.tiles_x)
-            qflag = a.tchanged[((k - 1) & 1) * a.tiles + rr * a.tiles_x + cc];
+        const int q = lane % 9;
+        const int rr = trow + q / 3 - 1, cc = tcol + q % 3 - 1;
+        qraw = a.tchanged[((k - 1) & 1) * a.tiles + min(max(rr, 0), ty - 1) * a.tiles_x + min(max(cc, 0), a.tiles_x - 1)];
+        qown = lane < 9 && rr >= 0 && rr < ty && cc >= 0 && cc < a.tiles_x;
     }
-    const int titer = k > 0 ? a.titers[t] : 0;
+    const int titer_raw = a.titers[t];
     R v[G::kStage];
-    relax_tile_load<R, G>(a.h[k & 1], a.ny, a.nx, r0, c0, v);
-    const RelaxControl C = relax_resolve(a, k, smem, slot);
+    unsigned vmask = relax_tile_load<R, G>(a.h[k & 1], a.ny, a.nx, r0, c0, v);
+    const RelaxControl C = relax_resolve(a, k, slot);
     if (C.done) {
         if (lead) s->ctl[(k + 1) & 1] = C;
         return;
@@ -228,7 +232,8 @@ __global__ void __launch_bounds__(G::BLOCK) relax_kernel(RelaxArgs<R> a, int k)
     // reproduce that launch's field (already in both buffers) and maxima.
     // Launch 0 and a replay compute every tile.
     if (k > 0 && C.replay == 0) {
-        if (!__syncthreads_or(qflag) && titer == n) {
+        const bool qany = __ballot(qown && qraw != 0) != 0ull;
+        if (!qany && titer_raw == n) {
             if (threadIdx.x == 0) {
                 a.tchanged[(k & 1) * a.tiles + t] = 0;
                 for (int i = 0; i < n; i++)
@@ -241,20 +246,36 @@ __global__ void __launch_bounds__(G::BLOCK) relax_kernel(RelaxArgs<R> a, int k)
     R tm[G::K];
 #pragma unroll
     for (int i = 0; i < G::K; i++) tm[i] = (R)0;
-    if (src != (k & 1)) relax_tile_load<R, G>(a.h[src], a.ny, a.nx, r0, c0, v);
-    relax_tile_iterations_staged<R, G>(v, a.h[src ^ 1], a.ny, a.nx, a.crit, r0, c0, tile, n, tm, &changed);
+    if (src != (k & 1)) vmask = relax_tile_load<R, G>(a.h[src], a.ny, a.nx, r0, c0, v);
+    relax_tile_iterations_staged<R, G>(v, vmask, a.h[src ^ 1], a.ny, a.nx, a.crit, r0, c0, tile, n, tm, &changed);
+    // one LDS round and one barrier: each wave's maxima and changed flag,
+    // folded by the first wave (lane w reads wave w; the maximum is exact in
+    // any order)
+    constexpr int NW = G::BLOCK / kWave;
+    static_assert(NW <= kWave, "the first wave folds one value per wave");
+    const int wave = (int)(threadIdx.x / kWave);
     double m[G::K];
 #pragma unroll
-    for (int i = 0; i < G::K; i++) m[i] = (double)tm[i];
-    block_reduce<double, G::K, 1, G::BLOCK / kWave>(m, smem);
-    const int any = __syncthreads_or(changed ? 1 : 0);
-    if (threadIdx.x == 0) {
-        for (int i = 0; i < n; i++) {
-            spread_max(sets + (size_t)i * kSpreadWords, m[i]);
-            a.texcess[(size_t)t * kRelaxKMax + i] = m[i];
+    for (int i = 0; i < G::K; i++) m[i] = wave_max((double)tm[i]);
+    const bool wchanged = __ballot(changed) != 0ull;
+    if (lane == 0) {
+#pragma unroll
+        for (int i = 0; i < G::K; i++) smem[i * NW + wave] = m[i];
+        wflag[wave] = wchanged ? 1 : 0;
+    }
+    __syncthreads();
+    if (wave == 0) {
+#pragma unroll
+        for (int i = 0; i < G::K; i++) m[i] = wave_max(lane < NW ? smem[i * NW + lane] : 0.0);
+        const bool any = __ballot(lane < NW && wflag[lane] != 0) != 0ull;
+        if (lane == 0) {
+            for (int i = 0; i < n; i++) {
+                spread_max(sets + (size_t)i * kSpreadWords, m[i]);
+                a.texcess[(size_t)t * kRelaxKMax + i] = m[i];
+            }
+            a.titers[t] = n;
+            a.tchanged[(k & 1) * a.tiles + t] = any ? 1 : 0;
         }
-        a.titers[t] = n;
-        a.tchanged[(k & 1) * a.tiles + t] = any;
     }
 }
 
@@ -269,7 +290,6 @@ template <class R>
 __global__ void __launch_bounds__(kBlock) relax_end_kernel(RelaxArgs<R> a, int k, unsigned *stamp_word,
                                                            unsigned stamp)
 {
-    __shared__ double smem[1];
     // the copy-back's first cells are read with the maxima, before the loop
     // state says whether they are needed (the final field is in buffer 1)
     const int64_t n = (int64_t)a.ny * a.nx;
@@ -280,7 +300,7 @@ __global__ void __launch_bounds__(kBlock) relax_end_kernel(RelaxArgs<R> a, int k
     R ahead[kRelaxEndAhead];
 #pragma unroll
     for (int u = 0; u < kRelaxEndAhead; u++) ahead[u] = i0 + u * stride < n ? a.h[1][i0 + u * stride] : (R)0;
-    const RelaxControl C = relax_resolve(a, k, smem, slot);
+    const RelaxControl C = relax_resolve(a, k, slot);
     if (C.done && C.src == 1) {
 #pragma unroll
         for (int u = 0; u < kRelaxEndAhead; u++)

@@ -63,21 +63,31 @@ template <> __device__ inline float relax_nan<float>() { return __builtin_nanf("
 template <> __device__ inline double relax_nan<double>() { return __builtin_nan(""); }
 
 // this thread's share of the padded tile with interior origin (r0, c0), read
-// from hs into registers (NaN outside the computed region and the grid)
+// from hs into registers; bit j of the returned mask is set where v[j] is a
+// grid cell of the computed region (the staging writes NaN elsewhere).
+// The loads are unconditional (indices clamped into the grid) and the NaN
+// selection happens at the staging, in a later block: a select next to the
+// load is turned into a load under a branch, and the compiler's waits after
+// such branches fall back to waiting for every load in flight.
 template <class R, class G>
-__device__ inline void relax_tile_load(const R *hs, int ny, int nx, int r0, int c0, R (&v)[G::kStage])
+__device__ inline unsigned relax_tile_load(const R *hs, int ny, int nx, int r0, int c0, R (&v)[G::kStage])
 {
     constexpr int H = G::H, LH = G::LH, LW = G::LW, PW = G::PW, BLOCK = G::BLOCK;
+    static_assert(G::kStage <= 32, "one validity bit per staged cell");
     const int R0 = r0 - H - 1, C0 = c0 - H - 1;         // global cell of padded (0, 0)
+    unsigned mask = 0u;
 #pragma unroll
     for (int j = 0; j < G::kStage; j++) {
         const int e = threadIdx.x + j * BLOCK;
         const int pr = e / PW, pc = e - pr * PW;
         const int gr = R0 + pr, gc = C0 + pc;
         const bool inner = pr >= 1 && pr <= LH && pc >= 1 && pc <= LW;
-        v[j] = relax_nan<R>();
-        if (e < G::kCells && inner && gr >= 0 && gr < ny && gc >= 0 && gc < nx) v[j] = hs[gr * nx + gc];
+        const bool in = e < G::kCells && inner && gr >= 0 && gr < ny && gc >= 0 && gc < nx;
+        const unsigned o = (unsigned)(min(max(gr, 0), ny - 1) * nx + min(max(gc, 0), nx - 1)) * (unsigned)sizeof(R);
+        v[j] = *reinterpret_cast<const R *>(reinterpret_cast<const char *>(hs) + o);
+        mask |= in ? 1u << j : 0u;
     }
+    return mask;
 }
 
 // n (1 <= n <= G::K) iterations on the tile with interior origin (r0, c0) of
@@ -92,7 +102,7 @@ __device__ inline void relax_tile_load(const R *hs, int ny, int nx, int r0, int 
 // The _staged form takes the tile as read into registers by relax_tile_load
 // (the caller may issue those loads early).
 template <class R, class G>
-__device__ inline void relax_tile_iterations_staged(const R (&v)[G::kStage], R *hd, int ny, int nx,
+__device__ inline void relax_tile_iterations_staged(const R (&v)[G::kStage], unsigned vmask, R *hd, int ny, int nx,
                                                     const R (&crits)[4], int r0, int c0, R *tile, int n,
                                                     R (&tmaxk)[G::K], bool *changed = nullptr)
 {
@@ -100,7 +110,7 @@ __device__ inline void relax_tile_iterations_staged(const R (&v)[G::kStage], R *
 #pragma unroll
     for (int j = 0; j < G::kStage; j++) {
         const int e = threadIdx.x + j * BLOCK;
-        if (e < G::kCells) tile[e] = v[j];
+        if (e < G::kCells) tile[e] = (vmask >> j) & 1u ? v[j] : relax_nan<R>();
     }
     // per-slot anchors and ownership (anchor in the interior) of the active
     // edge and of the opposite-parity edge continuing from its second cell
@@ -211,8 +221,8 @@ __device__ inline void relax_tile_iterations(const R *hs, R *hd, int ny, int nx,
                                              int c0, R *tile, int n, R (&tmaxk)[G::K], bool *changed = nullptr)
 {
     R v[G::kStage];
-    relax_tile_load<R, G>(hs, ny, nx, r0, c0, v);
-    relax_tile_iterations_staged<R, G>(v, hd, ny, nx, crits, r0, c0, tile, n, tmaxk, changed);
+    const unsigned vmask = relax_tile_load<R, G>(hs, ny, nx, r0, c0, v);
+    relax_tile_iterations_staged<R, G>(v, vmask, hd, ny, nx, crits, r0, c0, tile, n, tmaxk, changed);
 }
 
 }  // namespace mv2

@@ -419,22 +419,11 @@ __device__ inline void store_ledger(const V2Args<P> &a, LedgerAcc &L, bool befor
 // in by the launch after them (rolled == 1).  Both words were written by
 // earlier launches of the stream, so plain loads see them.
 // The record and flags of launch k were written by earlier launches (launch
-// k writes only the other slot), so they are read through the constant
-// address space: scalar loads, counted apart from the vector loads a launch
-// issues ahead of them.
-typedef const __attribute__((address_space(4))) unsigned ConstWord;
-
+// k writes only the other slot): scalar reads (load_launch_record).
 __device__ inline Control resolve_control(const SyncBlock *s, int k)
 {
-    static_assert(sizeof(Control) % 4 == 0, "the record is read as 32-bit words");
-    constexpr int kWords = (int)(sizeof(Control) / 4);
-    ConstWord *src = (ConstWord *)&s->ctl[k & 1];
-    unsigned w[kWords];
-#pragma unroll
-    for (int i = 0; i < kWords; i++) w[i] = src[i];
-    Control C;
-    __builtin_memcpy(&C, w, sizeof(Control));
-    const int f = (int)*(ConstWord *)&s->flags[k & 1];
+    Control C = load_launch_record(&s->ctl[k & 1]);
+    const int f = load_launch_record(&s->flags[k & 1]);
     if (f && !C.rolled) {
         C.numeric = (f & 2) ? 3 : 1;
         C.src = C.last_src;
