@@ -122,6 +122,15 @@ __device__ inline double inst_read_max(const unsigned long long *slots, double *
     return r;
 }
 
+// the same fold in one wave (every wave of the block folds the slots
+// itself: no barrier)
+__device__ inline double inst_read_max_wave(const unsigned long long *slots)
+{
+    const int lane = threadIdx.x & (kWave - 1);
+    const double v = __longlong_as_double((long long)slots[(lane % kInstSlots) * kInstStride]);
+    return wave_max(lane < kInstSlots ? v : 0.0);
+}
+
 __device__ inline BControl resolve_bcontrol(const BatchArgs &a, int k, int i)
 {
     BControl C = a.ctl[(k & 1) * a.count + i];
@@ -268,8 +277,8 @@ __global__ void __launch_bounds__(kBlock) batch_substep_kernel(BatchArgs a, int 
     const int src = C.src;
     const int r0 = a.tile_r0[g] + wave_index() * SH, c0 = a.tile_c0[g];
     StripRows<SH> rows;
+    const double maxw = inst_read_max_wave(inst_set(a.wave, k % kWaveSetsB, a.count, i));
     strip_load<SH>(v, src, r0, c0, rows);
-    const double maxw = inst_read_max(inst_set(a.wave, k % kWaveSetsB, a.count, i), smem);
     const double mind = a.dx <= a.dy ? a.dx : a.dy;
     const double stable = maxw <= 1.0e-14 ? C.remaining : cfg.cfl * mind / maxw;
     const double sub_dt = npmin(C.remaining, stable);
@@ -301,16 +310,21 @@ __global__ void __launch_bounds__(kBlock) batch_substep_kernel(BatchArgs a, int 
     float wv = 0.0f;
     long long hn_min = 0;
     int ovf = 0;
-    strip_update<SH>(v, rows, src, r0, c0, sub_dt, dg, wv, hn_min, ovf);
-    // one field-parallel fold: wave-speed maximum, overflow flag, minimum depth
-    __shared__ double fold_lds[3 * kBlock];
-    double fv[3] = {(double)wv, ovf ? 1.0 : 0.0, (double)hn_min * kQuantum};
-    block_fold<double, 3, 0x3u, 0x4u>(fv, fold_lds);
-    if (threadIdx.x == 0) {
-        inst_max(inst_set(a.wave, (k + 1) % kWaveSetsB, a.count, i), fv[0]);
-        int f = 0;
-        if (fv[2] < -1.0e-300) f |= 1;
-        if (fv[1] != 0.0) f |= 2;
+    if (strip_interior<SH>(v.ny, v.nx, r0, c0))
+        strip_update<SH, InstView, 0, true>(v, rows, src, r0, c0, sub_dt, dg, wv, hn_min, ovf);
+    else
+        strip_update<SH, InstView, 0, false>(v, rows, src, r0, c0, sub_dt, dg, wv, hn_min, ovf);
+    // per wave, no barrier: the wave-speed maximum into one of the instance's
+    // slots, a failure (negative thickness before clamping, int32 range) into
+    // its flags
+    const float wmx = wave_max(wv);
+    const bool neg = __ballot(hn_min < 0) != 0ull;
+    const bool rng = __ballot(ovf != 0) != 0ull;
+    if ((threadIdx.x & (kWave - 1)) == 0) {
+        unsigned long long *slots = inst_set(a.wave, (k + 1) % kWaveSetsB, a.count, i);
+        const int slot = (g * (kBlock / kWave) + wave_index()) % kInstSlots;
+        atomic_max_nonneg(&slots[slot * kInstStride], (double)wmx);
+        const int f = (neg ? 1 : 0) | (rng ? 2 : 0);
         if (f) atomicOr(&a.flags[((k + 1) & 1) * a.count + i], f);
     }
 }
