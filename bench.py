@@ -763,7 +763,7 @@ def main() -> int:
         v2_sub += subs
         if solver.last_record.substep_kernel_ms > 0.0:
             sub_ms += solver.last_record.substep_kernel_ms
-            timed_sub += subs
+            timed_sub += solver.last_record.timed_launches
             timed_frames += 1
     solver.set_kernel_timing(False)
     torch.cuda.synchronize()

@@ -96,7 +96,7 @@ typedef struct mv2_step_record {
     double kernel_ms;                 /* device time of the whole call (HIP events) */
     double substep_kernel_ms;         /* sum over the substep launches (timing mode only, else 0) */
     int launches;                     /* substep launches enqueued (substeps + ended-loop launches) */
-    int pad;
+    int timed_launches;               /* substep launches inside substep_kernel_ms (timing mode) */
 } mv2_step_record;
 
 /* One step_resident call (warp_mobile_v2.py:148-245) or standalone step
@@ -133,6 +133,21 @@ int mv2_set_timing(mv2_context *ctx, int enable);
  * mv2_graph_replays returns how many calls were served by a replay. */
 int mv2_set_graphs(mv2_context *ctx, int enable);
 long long mv2_graph_replays(const mv2_context *ctx);
+
+/* Speculative FP32 steps (no counterpart in the reference; its step begins
+ * with a host-synchronised CFL reduction): the end pass of a step also
+ * records the CFL maximum of its result, and the next step's first launch
+ * runs the begin pass and substep 0 together with that predicted maximum;
+ * the second launch keeps the substep only if the true maximum of the input
+ * equals the prediction bit for bit (and the input is not static), else it
+ * restarts from the untouched input.  Results are identical either way.  A
+ * step speculates when the previous one left a prediction and no library
+ * call has written mobile thickness or momentum since; on by default.
+ * mv2_set_speculation(ctx, 0) restores a separate begin launch;
+ * mv2_speculation_stats gives the steps that speculated (out[0]) and those
+ * whose prediction did not hold (out[1]). */
+int mv2_set_speculation(mv2_context *ctx, int enable);
+int mv2_speculation_stats(const mv2_context *ctx, long long out[2]);
 
 /* ledger reduction only (warp_mobile_v2.py:28-54 summarize) */
 int mv2_summary(mv2_context *ctx, const mv2_step_config *cfg, double out[8]);

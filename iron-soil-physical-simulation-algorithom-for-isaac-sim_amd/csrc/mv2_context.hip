@@ -265,5 +265,6 @@ extern "C" int mv2_bind(mv2_context *ctx, int precision, int ny, int nx, double 
     ctx->qx = momentum_x;
     ctx->qy = momentum_y;
     ctx->bound = 1;
+    state_written(ctx);
     return MV2_OK;
 }

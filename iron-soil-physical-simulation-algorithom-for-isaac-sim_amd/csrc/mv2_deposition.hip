@@ -213,6 +213,7 @@ extern "C" int mv2_deposition_step(mv2_context *ctx, const mv2_deposition_config
     if (!ctx) return set_error(nullptr, MV2_EINVAL, "[WarpDeposition] null context");
     if (!ctx->bound) return set_error(ctx, MV2_ESTATE, "[WarpDeposition] bind DeviceBulkState first");
     if (!cfg || !out) return set_error(ctx, MV2_EINVAL, "[WarpDeposition] null config/result");
+    state_written(ctx);
     if (!deposited || !removed_x || !removed_y || !m2r_cumulative)
         return set_error(ctx, MV2_EINVAL,
                          "[WarpDeposition] missing shared fields: deposition_work, "

@@ -226,6 +226,7 @@ extern "C" int mv2_entrain(mv2_context *ctx, const int32_t *idx, const double *d
     if (int rc = check_bound(ctx, "[DeviceBulkState]")) return rc;
     if (n < 0 || (n > 0 && (!idx || !depth)) || !volume_out)
         return set_error(ctx, MV2_EINVAL, "[DeviceBulkState] invalid entrainment request");
+    state_written(ctx);
     *volume_out = 0.0;
     if (n == 0) return MV2_OK;
     if (int rc = ensure_scratch(ctx, ctx->partials, sizeof(double) * 4 * kExGrid)) return rc;
@@ -249,6 +250,7 @@ extern "C" int mv2_deposit(mv2_context *ctx, const int32_t *idx, const double *d
     if (int rc = check_bound(ctx, "[DeviceBulkState]")) return rc;
     if (n < 0 || (n > 0 && (!idx || !depth)) || !out || !(density > 0.0))
         return set_error(ctx, MV2_EINVAL, "[DeviceBulkState] invalid deposition indices/density");
+    state_written(ctx);
     out[0] = out[1] = out[2] = 0.0;
     if (n == 0) return MV2_OK;
     if (int rc = ensure_scratch(ctx, ctx->partials, sizeof(double) * 4 * kExGrid)) return rc;

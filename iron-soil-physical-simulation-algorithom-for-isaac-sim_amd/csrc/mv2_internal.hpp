@@ -77,6 +77,11 @@ struct mv2_context {
     int v2_predicted = 0;                 // substep launches to enqueue up front
     int v2_frame = 0;                     // steps taken (the parity picks the loop-state block)
     int v2_clean = -1;                    // parity whose loop state is known zeroed, -1 none
+    int v2_spec = 1;                      // FP32 steps may speculate (mv2_set_speculation)
+    int v2_pred_ready = 0;                // the last step's end pass left a CFL prediction and no library
+                                          // call has written mobile or momentum since
+    long long spec_steps = 0;             // steps that speculated, and those that restarted
+    long long spec_restarts = 0;
     int v2_last_sync = 0;                 // parity of the last completed step
     double v2_static[2] = {0.0, 0.0};     // static diagnostic of the last step's input: max |grad H|, max |q|
     const void *ext_x = nullptr;          // per-cell external acceleration (state precision), or null
@@ -119,6 +124,9 @@ unsigned *stamp_words(mv2_context *ctx);
 // wait until words 0 .. words - 1 carry `stamp` (stream_wait for stamp 0,
 // after 2 ms, or on a device error)
 int stamp_wait(mv2_context *ctx, hipStream_t st, unsigned stamp, int words);
+// a library call wrote mobile thickness or momentum: the next step takes its
+// CFL maximum from a begin pass instead of the previous step's prediction
+inline void state_written(mv2_context *ctx) { ctx->v2_pred_ready = 0; }
 // device address of the allocation-free ledger's file-scope partials (mv2_exchange.hip)
 int noalloc_scratch_address(mv2_context *ctx, double **out);
 
@@ -427,8 +435,12 @@ __device__ inline T load_launch_record(const T *p)
 constexpr int kSpread = 64;
 constexpr int kSpreadStride = 16;                    // u64 words between slots
 constexpr int kSpreadWords = kSpread * kSpreadStride;
-// the step uses sets 0..4 (even frames) and 16..20 (odd frames), the envelope
-// 5..7, the labelling 12..14, the relaxation 21..30
+// the step uses sets 0..4 (even frames) and 16..20 (odd frames) and, for its
+// speculation, 8..9 (true maximum of the input, by frame parity) and 10..11
+// (predicted maximum, by the parity of the frame that reads it), the
+// envelope 5..7, the labelling 12..14, the relaxation 21..30
+constexpr int kSpecTrueSet = 8;
+constexpr int kSpecPredSet = 10;
 constexpr int kEnvSpreadSet = 5;
 constexpr int kLabelSpreadSet = 12;
 constexpr int kStepOddSpreadSet = 16;

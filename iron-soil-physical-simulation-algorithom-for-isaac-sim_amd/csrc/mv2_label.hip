@@ -786,6 +786,7 @@ extern "C" int mv2_avalanche_mobilize(mv2_context *ctx, const mv2_avalanche_conf
 {
     if (int rc = check_ctx(ctx)) return rc;
     if (!cfg || !f || !diag_f) return set_error(ctx, MV2_EINVAL, "[GpuLargeAvalanche] null argument");
+    state_written(ctx);
     const int ny = ctx->ny, nx = ctx->nx;
     const int64_t n = (int64_t)ny * nx;
     const int f32 = ctx->precision == MV2_FP32;

@@ -266,6 +266,7 @@ extern "C" int mv2_patch_scatter(mv2_context *ctx, int field, int row0, int row1
     if (int rc = bound(ctx)) return rc;
     if (field < kFieldB || field > kFieldQy) return set_error(ctx, MV2_EINVAL, "[DeviceBulkState] unsupported mutable field");
     if (int rc = check_bbox(ctx, row0, row1, col0, col1, true, "patch")) return rc;
+    state_written(ctx);
     const int64_t n = (int64_t)(row1 - row0) * (col1 - col0);
     if (n == 0) return MV2_OK;
     if (!values) return set_error(ctx, MV2_EINVAL, "[DeviceBulkState] patch values missing");
@@ -294,6 +295,7 @@ extern "C" int mv2_index_add(mv2_context *ctx, int field, const int32_t *idx, co
     if (field < kFieldB || field > kFieldQy) return set_error(ctx, MV2_EINVAL, "[DeviceBulkState] unsupported compact float field");
     if (n < 0 || (n > 0 && (!idx || !values)))
         return set_error(ctx, MV2_EINVAL, "[DeviceBulkState] invalid compact add");
+    state_written(ctx);
     if (n == 0) return MV2_OK;
     hipLaunchKernelGGL(index_add_kernel, dim3(grid_for(n, kBlock, 4096)), dim3(kBlock), 0, ctx->stream,
                        context_view(ctx), field, idx, values, n);
@@ -319,6 +321,7 @@ extern "C" int mv2_failure_activate(mv2_context *ctx, int row0, int row1, int co
 {
     if (int rc = bound(ctx)) return rc;
     if (int rc = check_bbox(ctx, row0, row1, col0, col1, false, "activation patch")) return rc;
+    state_written(ctx);
     if (!activated || !affected || !material_mask || !forcing_out || !out)
         return set_error(ctx, MV2_EINVAL, "[GpuFailureBridge] null argument");
     const int h = row1 - row0, w = col1 - col0;

@@ -56,6 +56,11 @@ struct Control {
     int cfl_limited, static_exit;
     int rolled;                // 1 once a failed substep was rolled back
     int end_done;              // FP32: a launch after the loop's end copied back and stored the ledger after
+    // FP32 speculative step: launch 0 took substep 0 with the CFL maximum
+    // `pred` predicted by the previous step's end pass; launch 1 checks it
+    // (restarted: it did not hold, and the step began again from its input)
+    int spec, restarted;
+    double pred;
 };
 
 struct SyncBlock {
@@ -122,6 +127,12 @@ struct V2Args {
     int diag_groups;
     unsigned long long *spread;          // kStepSpreadSets spread-maximum sets
     int clamp_quarter;                   // FP32: cap outflow per face at h/4
+    // FP32 speculation (spread sets outside the parity groups): the true CFL
+    // maximum of the input from launch 0 (this parity's set; the other
+    // parity's is zeroed by the fold pass), the predicted maximum this step
+    // reads (zeroed by its fold pass) and the one its end pass writes
+    unsigned long long *spec_true, *spec_true_next;
+    unsigned long long *pred, *pred_next;
 };
 
 
@@ -420,10 +431,11 @@ __device__ inline void store_ledger(const V2Args<P> &a, LedgerAcc &L, bool befor
 // earlier launches of the stream, so plain loads see them.
 // The record and flags of launch k were written by earlier launches (launch
 // k writes only the other slot): scalar reads (load_launch_record).
-__device__ inline Control resolve_control(const SyncBlock *s, int k)
+// A failure flagged by the previous launch (bit 0 positivity, bit 1 int32
+// range) rolls the loop state back to before the failing substep and ends
+// the loop.
+__device__ inline void apply_failure_flags(Control &C, int f)
 {
-    Control C = load_launch_record(&s->ctl[k & 1]);
-    const int f = load_launch_record(&s->flags[k & 1]);
     if (f && !C.rolled) {
         C.numeric = (f & 2) ? 3 : 1;
         C.src = C.last_src;
@@ -432,15 +444,37 @@ __device__ inline Control resolve_control(const SyncBlock *s, int k)
         C.done = 1;
         C.rolled = 1;
     }
+}
+
+__device__ inline Control resolve_control(const SyncBlock *s, int k)
+{
+    Control C = load_launch_record(&s->ctl[k & 1]);
+    apply_failure_flags(C, load_launch_record(&s->flags[k & 1]));
     return C;
 }
 
-// zero one parity's loop state and spread sets (first step, or after an error)
-__global__ void v2_reset_kernel(SyncBlock *s, unsigned long long *spread)
+// the loop state of a step's first substep: the whole frame dt still to go
+__device__ inline Control fresh_control(double dt)
+{
+    Control C;
+    __builtin_memset(&C, 0, sizeof(C));
+    C.remaining = dt;
+    return C;
+}
+
+// zero one parity's loop state and spread sets (first step, or after an
+// error), the true-maximum set of its launch 0 and the prediction set its end
+// pass fills
+__global__ void v2_reset_kernel(SyncBlock *s, unsigned long long *spread, unsigned long long *spec_true,
+                                unsigned long long *pred_next)
 {
     unsigned *ps = reinterpret_cast<unsigned *>(s);
     for (int i = threadIdx.x; i < (int)(sizeof(SyncBlock) / 4); i += blockDim.x) ps[i] = 0u;
     for (int i = threadIdx.x; i < kStepSpreadSets * kSpread; i += blockDim.x) spread[(int64_t)i * kSpreadStride] = 0ull;
+    for (int i = threadIdx.x; i < kSpread; i += blockDim.x) {
+        spec_true[(int64_t)i * kSpreadStride] = 0ull;
+        pred_next[(int64_t)i * kSpreadStride] = 0ull;
+    }
 }
 
 // One pass over the input state: ledger before, static diagnostic
@@ -522,7 +556,88 @@ constexpr int kSubFold = 3 + kDiag;   // values folded per workgroup at the end 
 template <int SH>
 __device__ inline void end32_body(const V2Args<F32P> &a, int fsrc);
 
-template <class P, int SH = kSH, int DIAGF = 0, int EAGER = 0>
+// One FP32 substep of the wave's band, staged from buffer C.src: dt from the
+// CFL maximum maxw, the successor loop state (workgroup 0; spec_pred >= 0
+// marks it speculative, with that predicted maximum, for launch 1 to check),
+// the update into the other buffer and the wave-level epilogue.  dprev is
+// this lane's diagnostic slot before the substep (0 for a step's first).
+template <int SH, int DIAGF>
+__device__ inline void substep32_work(const V2Args<F32P> &a, const StripRows<SH> &rows, const Control &C,
+                                      double maxw, int k, int r0, int c0, double dprev, double spec_pred)
+{
+    const mv2_step_config &cfg = a.cfg;
+    SyncBlock *s = a.sync;
+    const int lane = threadIdx.x & (kWave - 1);
+    const int gw = blockIdx.x * (kBlock / kWave) + wave_index();
+    const double mind = a.dx <= a.dy ? a.dx : a.dy;
+    const double stable = maxw <= 1.0e-14 ? C.remaining : cfg.cfl * mind / maxw;
+    const double sub_dt = npmin(C.remaining, stable);
+    if (blockIdx.x == 0) spread_clear(a.spread + ((k + 2) % kWaveSets) * kSpreadWords);
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        Control nc = C;
+        nc.last_remaining = C.remaining;
+        nc.last_substeps = C.substeps;
+        nc.last_src = C.src;
+        nc.max_cfl = npmax(C.max_cfl, maxw * sub_dt / mind);
+        if (sub_dt < C.remaining - 1.0e-14) nc.cfl_limited = 1;
+        nc.remaining = C.remaining - sub_dt;
+        nc.substeps = C.substeps + 1;
+        nc.src = C.src ^ 1;
+        // the loop's own exits, in the reference's order
+        if (cfg.limit_mode == 1 && nc.substeps > cfg.maximum_substeps) {
+            nc.numeric = 2;
+            nc.done = 1;
+        } else if (!(nc.remaining > 1.0e-14)) {
+            nc.done = 1;
+        } else if (cfg.limit_mode == 0 && nc.substeps >= cfg.maximum_substeps) {
+            nc.numeric = 2;
+            nc.done = 1;
+        }
+        if (spec_pred >= 0.0) {
+            nc.spec = 1;
+            nc.pred = spec_pred;
+        }
+        s->ctl[(k + 1) & 1] = nc;
+    }
+    float dg[kDiag];
+#pragma unroll
+    for (int d = 0; d < kDiag; d++) dg[d] = 0.0f;
+    float wv = 0.0f;
+    long long hn_min = 0;
+    int ovf = 0;
+    if (strip_interior<SH>(a.ny, a.nx, r0, c0))
+        strip_update<SH, V2Args<F32P>, DIAGF, true>(a, rows, C.src, r0, c0, sub_dt, dg, wv, hn_min, ovf);
+    else
+        strip_update<SH, V2Args<F32P>, DIAGF, false>(a, rows, C.src, r0, c0, sub_dt, dg, wv, hn_min, ovf);
+    // Wave-level epilogue, no barrier: the wave's wave-speed maximum goes to a
+    // spread slot of its own, a failure (negative thickness before clamping,
+    // int32 range) to the flags, and its diagnostic sums (a fixed-order DPP
+    // tree) to its partials, which the fold pass sums in fixed order.
+    const float wmx = wave_max(wv);
+    const bool neg = __ballot(hn_min < 0) != 0ull;
+    const bool rng = __ballot(ovf != 0) != 0ull;
+    float ds[kDiag];
+#pragma unroll
+    for (int d = 0; d < kDiag; d++) {
+        const bool kept = DIAGF == 0 || d == 0 || d == 1 || d == 4 || d == 5 || d == 7;
+        ds[d] = kept ? wave_sum(dg[d]) : 0.0f;
+    }
+    if (lane == 0) {
+        atomic_max_nonneg(&a.spread[((k + 1) % kWaveSets) * kSpreadWords + (gw % kSpread) * kSpreadStride],
+                          (double)wmx);
+        const int f = (neg ? 1 : 0) | (rng ? 2 : 0);
+        if (f) atomicOr(&s->flags[(k + 1) & 1], f);
+    }
+    if (lane < kDiag) {
+        float v = ds[0];
+#pragma unroll
+        for (int d = 1; d < kDiag; d++) v = lane == d ? ds[d] : v;
+        const bool kept = DIAGF == 0 || lane == 0 || lane == 1 || lane == 4 || lane == 5 || lane == 7;
+        a.diag_part[(int64_t)lane * a.diag_groups + gw] = kept ? dprev + (double)v : 0.0;
+    }
+}
+
+template <class P, int SH = kSH, int DIAGF = 0, int EAGER = 0, int CHECK = 1>
 __global__ void __launch_bounds__(kBlock) v2_substep_kernel(V2Args<P> a, int k)
 {
     constexpr int kRows = SH * (kBlock / kWave);     // rows of a tile
@@ -542,7 +657,7 @@ __global__ void __launch_bounds__(kBlock) v2_substep_kernel(V2Args<P> a, int k)
         __asm__ volatile("" ::"s"(ny), "s"(nx), "s"(a.b), "s"(a.h[0]), "s"(a.h[1]), "s"(a.qx[0]), "s"(a.qx[1]),
                          "s"(a.qy[0]), "s"(a.qy[1]), "s"(a.spread), "s"(a.diag_part), "s"(a.diag_groups),
                          "s"(a.sync), "s"(a.ext_x), "s"(k), "s"((int)gridDim.x), "s"(a.cfg.density),
-                         "s"(a.clamp_quarter));
+                         "s"(a.clamp_quarter), "s"(a.spec_true));
     }
     const int tiles_x = (nx + kStripW - 1) / kStripW;
     const int tiles = tiles_x * ((ny + kRows - 1) / kRows);
@@ -568,37 +683,100 @@ __global__ void __launch_bounds__(kBlock) v2_substep_kernel(V2Args<P> a, int k)
     // it.  All are vector loads (lane-dependent addresses): a scalar load
     // here would be waited for before the tile's loads could issue.
     // FP32: every wave folds the 64 slots of the previous launch's maximum
-    // itself (no barrier); FP64: the first wave, through LDS
-    double slot = 0.0;
-    if (P::kQuant || threadIdx.x < kWave)
-        slot = __longlong_as_double((long long)a.spread[(k % kWaveSets) * kSpreadWords + lane * kSpreadStride]);
+    // itself (no barrier); FP64: the first wave, through LDS.  FP32 also
+    // reads the true maximum of the step's input that a speculative launch 0
+    // left for launch 1.
+    // FP32: every wave reads and folds the slots itself (no barrier; a
+    // first-wave fold shared through LDS measured slower).  CHECK launches
+    // (the first two of a step) also read the sets of the input's static
+    // maxima and true CFL maximum, which the loop-state test of a step's
+    // first substep reads; every load is unconditional (loads under a branch
+    // make the compiler wait for every load at the join).
+    double slot = 0.0, tslot = 0.0, gslot = 0.0, qslot = 0.0;
+    if constexpr (P::kQuant) {
+        const int sl = lane * kSpreadStride;
+        slot = __longlong_as_double((long long)a.spread[(k % kWaveSets) * kSpreadWords + sl]);
+        if constexpr (CHECK) {
+            tslot = __longlong_as_double((long long)a.spec_true[sl]);
+            gslot = __longlong_as_double((long long)a.spread[kSetStaticGrad * kSpreadWords + sl]);
+            qslot = __longlong_as_double((long long)a.spread[kSetStaticQ * kSpreadWords + sl]);
+        }
+    } else {
+        if (threadIdx.x < kWave)
+            slot = __longlong_as_double((long long)a.spread[(k % kWaveSets) * kSpreadWords + lane * kSpreadStride]);
+    }
     StripRows<SH> rows;
     if constexpr (P::kQuant) {
         if constexpr (EAGER) strip_load<SH>(a, src, r0, c0, rows);
     }
-    // the diagnostic slots accumulate over the working launches; launch 0 is
-    // the first of them and starts them from zero.  Lane d holds field d: of
-    // every wave (FP32) or of the workgroup's first wave (FP64).
+    // the diagnostic slots accumulate over the working launches; a step's
+    // first substep starts them from zero.  Lane d holds field d: of every
+    // wave (FP32) or of the workgroup's first wave (FP64).
     double dprev = 0.0;
     if constexpr (P::kQuant) {
         // unconditional (clamped) for the same reason as the tile's loads
-        const double v = a.diag_part[(int64_t)min(lane, kDiag - 1) * a.diag_groups + gw];
-        dprev = k != 0 && lane < kDiag ? v : 0.0;
+        dprev = a.diag_part[(int64_t)min(lane, kDiag - 1) * a.diag_groups + gw];
     }
     if (k != 0) {
         if constexpr (!P::kQuant) {
             if (threadIdx.x < kDiag) dprev = a.diag_part[(int64_t)threadIdx.x * a.diag_groups + blockIdx.x];
         }
     }
-    Control C = resolve_control(s, k);
-    if (!C.done && k == 0 && cfg.static_exit) {
-        // warp_mobile_v2.py:148-160 - globally static state below the start angle
-        const double ext_max = a.ext_x ? a.ext_max : npmax(0.0, npmax(fabs(cfg.external_ax), fabs(cfg.external_ay)));
-        const double qmax = spread_read_max(a.spread + kSetStaticQ * kSpreadWords, smem);
-        const double gmax = spread_read_max(a.spread + kSetStaticGrad * kSpreadWords, smem);
-        if (ext_max <= 1.0e-14 && qmax <= 1.0e-14 && gmax <= cfg.tan_start + 1.0e-14) {
+    Control C = load_launch_record(&s->ctl[k & 1]);
+    const int flags = load_launch_record(&s->flags[k & 1]);
+    // FP32: each wave's folds of the slots (the same in every wave)
+    double fold_true = 0.0, fold_grad = 0.0, fold_q = 0.0;
+    if constexpr (P::kQuant && CHECK) {
+        fold_true = wave_max(tslot);
+        fold_grad = wave_max(gslot);
+        fold_q = wave_max(qslot);
+    }
+    // FP32 speculative step, launch 1: launch 0 took substep 0 with the
+    // predicted CFL maximum.  A static input ends the step (the input in
+    // buffer 0 is untouched); a true maximum other than the prediction
+    // restarts the step from buffer 0 here (the failure flags of the
+    // discarded substep are dropped); otherwise the loop goes on.
+    bool restart = false, spec_static = false;
+    double maxw_true = 0.0;
+    if (P::kQuant && CHECK && C.spec) {
+        const double pred = C.pred;
+        bool is_static = false;
+        if (cfg.static_exit) {
+            // warp_mobile_v2.py:148-160 - globally static state below the start angle
+            const double ext_max = a.ext_x ? a.ext_max : npmax(0.0, npmax(fabs(cfg.external_ax), fabs(cfg.external_ay)));
+            is_static = ext_max <= 1.0e-14 && fold_q <= 1.0e-14 && fold_grad <= cfg.tan_start + 1.0e-14;
+        }
+        if (is_static) {
+            C = fresh_control(cfg.dt);
             C.static_exit = 1;
             C.done = 1;
+            spec_static = true;
+        } else {
+            maxw_true = fold_true;
+            if (maxw_true != pred) {
+                C = fresh_control(cfg.dt);
+                C.restarted = 1;
+                restart = true;
+                // the flags launch 0's discarded substep left (read above by
+                // every workgroup of this launch and dropped); launch 2 ORs
+                // its own into this word
+                if (lead) s->flags[k & 1] = 0;
+            } else {
+                C.spec = 0;
+                apply_failure_flags(C, flags);
+            }
+        }
+    } else {
+        apply_failure_flags(C, flags);
+        if (!C.done && k == 0 && cfg.static_exit && (!P::kQuant || CHECK)) {
+            // warp_mobile_v2.py:148-160 - globally static state below the start angle
+            const double ext_max = a.ext_x ? a.ext_max : npmax(0.0, npmax(fabs(cfg.external_ax), fabs(cfg.external_ay)));
+            const double qmax = P::kQuant ? fold_q : spread_read_max(a.spread + kSetStaticQ * kSpreadWords, smem);
+            const double gmax = P::kQuant ? fold_grad : spread_read_max(a.spread + kSetStaticGrad * kSpreadWords, smem);
+            if (ext_max <= 1.0e-14 && qmax <= 1.0e-14 && gmax <= cfg.tan_start + 1.0e-14) {
+                C.static_exit = 1;
+                C.done = 1;
+            }
         }
     }
     if (!C.done && !(C.remaining > 1.0e-14)) C.done = 1;
@@ -611,90 +789,68 @@ __global__ void __launch_bounds__(kBlock) v2_substep_kernel(V2Args<P> a, int k)
             // the first launch past the loop's end: copy back + ledger after
             if (!C.end_done && !C.static_exit) end32_body<SH>(a, C.src);
             C.end_done = 1;
+            // a static input after a speculative substep 0: its diagnostics are void
+            if (spec_static && lane < kDiag) a.diag_part[(int64_t)lane * a.diag_groups + gw] = 0.0;
         }
         if (lead) s->ctl[(k + 1) & 1] = C;
         return;
     }
     if constexpr (P::kQuant) {
-        if constexpr (!EAGER) strip_load<SH>(a, src, r0, c0, rows);
-    }
-    // CFL maximum of the current state (mobile_v2_reference.py:239-251)
-    double maxw;
-    if constexpr (P::kQuant) {
-        maxw = wave_max(slot);
+        // CFL maximum of the current state (mobile_v2_reference.py:239-251)
+        const double maxw = restart ? maxw_true : wave_max(slot);
+        if (EAGER && C.src != src) {
+            // The state is in the other buffer (a restarted speculative
+            // step): pass the loop state on unchanged, with the maximum in the
+            // next launch's set, so that launch k + 1 finds the state in
+            // buffer (k + 1) & 1 as its early staging assumes.  (A reload
+            // here would join two paths with different loads in flight, and
+            // the compiler's wait at the join would hold every launch's first
+            // row until its whole tile had arrived.)
+            if (blockIdx.x == 0) spread_clear(a.spread + ((k + 2) % kWaveSets) * kSpreadWords);
+            if (lead) {
+                atomic_max_nonneg(&a.spread[((k + 1) % kWaveSets) * kSpreadWords], maxw);
+                s->ctl[(k + 1) & 1] = C;
+            }
+            return;
+        }
+        // launches expected to find the loop ended stage the state only now
+        if constexpr (!EAGER) strip_load<SH>(a, C.src, r0, c0, rows);
+        // a step's first substep starts its diagnostic slots from zero
+        substep32_work<SH, DIAGF>(a, rows, C, maxw, k, r0, c0, C.substeps > 0 && lane < kDiag ? dprev : 0.0, -1.0);
     } else {
+        // CFL maximum of the current state (mobile_v2_reference.py:239-251)
         if (threadIdx.x < kWave) {
             const double v = wave_max(slot);
             if (threadIdx.x == 0) smem[0] = v;
         }
         __syncthreads();
-        maxw = smem[0];
-    }
-    const double mind = a.dx <= a.dy ? a.dx : a.dy;
-    const double stable = maxw <= 1.0e-14 ? C.remaining : cfg.cfl * mind / maxw;
-    const double sub_dt = npmin(C.remaining, stable);
-    if (blockIdx.x == 0) spread_clear(a.spread + ((k + 2) % kWaveSets) * kSpreadWords);
-    if (lead) {
-        Control nc = C;
-        nc.last_remaining = C.remaining;
-        nc.last_substeps = C.substeps;
-        nc.last_src = C.src;
-        nc.max_cfl = npmax(C.max_cfl, maxw * sub_dt / mind);
-        if (sub_dt < C.remaining - 1.0e-14) nc.cfl_limited = 1;
-        nc.remaining = C.remaining - sub_dt;
-        nc.substeps = C.substeps + 1;
-        nc.src = C.src ^ 1;
-        // the loop's own exits, in the reference's order
-        if (cfg.limit_mode == 1 && nc.substeps > cfg.maximum_substeps) {
-            nc.numeric = 2;
-            nc.done = 1;
-        } else if (!(nc.remaining > 1.0e-14)) {
-            nc.done = 1;
-        } else if (cfg.limit_mode == 0 && nc.substeps >= cfg.maximum_substeps) {
-            nc.numeric = 2;
-            nc.done = 1;
+        const double maxw = smem[0];
+        const double mind = a.dx <= a.dy ? a.dx : a.dy;
+        const double stable = maxw <= 1.0e-14 ? C.remaining : cfg.cfl * mind / maxw;
+        const double sub_dt = npmin(C.remaining, stable);
+        if (blockIdx.x == 0) spread_clear(a.spread + ((k + 2) % kWaveSets) * kSpreadWords);
+        if (lead) {
+            Control nc = C;
+            nc.last_remaining = C.remaining;
+            nc.last_substeps = C.substeps;
+            nc.last_src = C.src;
+            nc.max_cfl = npmax(C.max_cfl, maxw * sub_dt / mind);
+            if (sub_dt < C.remaining - 1.0e-14) nc.cfl_limited = 1;
+            nc.remaining = C.remaining - sub_dt;
+            nc.substeps = C.substeps + 1;
+            nc.src = C.src ^ 1;
+            // the loop's own exits, in the reference's order
+            if (cfg.limit_mode == 1 && nc.substeps > cfg.maximum_substeps) {
+                nc.numeric = 2;
+                nc.done = 1;
+            } else if (!(nc.remaining > 1.0e-14)) {
+                nc.done = 1;
+            } else if (cfg.limit_mode == 0 && nc.substeps >= cfg.maximum_substeps) {
+                nc.numeric = 2;
+                nc.done = 1;
+            }
+            s->ctl[(k + 1) & 1] = nc;
         }
-        s->ctl[(k + 1) & 1] = nc;
-    }
-    if constexpr (P::kQuant) {
-        float dg[kDiag];
-#pragma unroll
-        for (int d = 0; d < kDiag; d++) dg[d] = 0.0f;
-        float wv = 0.0f;
-        long long hn_min = 0;
-        int ovf = 0;
-        if (strip_interior<SH>(ny, nx, r0, c0))
-            strip_update<SH, V2Args<P>, DIAGF, true>(a, rows, src, r0, c0, sub_dt, dg, wv, hn_min, ovf);
-        else
-            strip_update<SH, V2Args<P>, DIAGF, false>(a, rows, src, r0, c0, sub_dt, dg, wv, hn_min, ovf);
-        // Wave-level epilogue, no barrier: the wave's wave-speed maximum goes
-        // to a spread slot of its own, a failure (negative thickness before
-        // clamping, int32 range) to the flags, and its diagnostic sums (a
-        // fixed-order DPP tree) to its partials, which the fold pass sums in
-        // fixed order.
-        const float wmx = wave_max(wv);
-        const bool neg = __ballot(hn_min < 0) != 0ull;
-        const bool rng = __ballot(ovf != 0) != 0ull;
-        float ds[kDiag];
-#pragma unroll
-        for (int d = 0; d < kDiag; d++) {
-            const bool kept = DIAGF == 0 || d == 0 || d == 1 || d == 4 || d == 5 || d == 7;
-            ds[d] = kept ? wave_sum(dg[d]) : 0.0f;
-        }
-        if (lane == 0) {
-            atomic_max_nonneg(&a.spread[((k + 1) % kWaveSets) * kSpreadWords + (gw % kSpread) * kSpreadStride],
-                              (double)wmx);
-            const int f = (neg ? 1 : 0) | (rng ? 2 : 0);
-            if (f) atomicOr(&s->flags[(k + 1) & 1], f);
-        }
-        if (lane < kDiag) {
-            float v = ds[0];
-#pragma unroll
-            for (int d = 1; d < kDiag; d++) v = lane == d ? ds[d] : v;
-            const bool kept = DIAGF == 0 || lane == 0 || lane == 1 || lane == 4 || lane == 5 || lane == 7;
-            a.diag_part[(int64_t)lane * a.diag_groups + gw] = kept ? dprev + (double)v : 0.0;
-        }
-    } else {
         Acc<P> acc;
         for (int d = 0; d < kDiag; d++) acc.diag[d] = 0.0;
         double wmax[1] = {0.0};
@@ -788,6 +944,21 @@ __global__ void __launch_bounds__(kBlock) v2_end_kernel(V2Args<P> a, int k)
 // summed per lane in float over the band, then in double; the mobile volume
 // itself comes exactly from the quanta per weight class.
 // ---------------------------------------------------------------------------
+
+// CFL wave speed of one cell of an FP32 state, in the arithmetic of the
+// substep: |v| + sqrt(K g h) with the cell velocity from the hardware
+// reciprocal (zero where the cell is dry)
+__device__ inline float cfl_speed32(int hq, float qxv, float qyv, float dry, float Kg)
+{
+    const float d = (float)hq * (float)kQuantum;
+    float vx = 0.0f, vy = 0.0f;
+    if (d > dry) {
+        const float inv = rcpf(d);
+        vx = qxv * inv;
+        vy = qyv * inv;
+    }
+    return sqrtf_hw(vx * vx + vy * vy) + sqrtf_hw(Kg * d);
+}
 
 struct Ledger32 {
     float qx, qy, ke, pot, prs, vmax, hmin;   // sums weighted by the cell weight; max speed, min depth
@@ -883,26 +1054,19 @@ __device__ inline void store_ledger32(const V2Args<F32P> &a, const Ledger32 &L, 
     }
 }
 
-// One pass over the input state: ledger before, static diagnostic and the
-// CFL maximum of the first substep (as v2_begin_kernel, FP32 strip layout).
+// The begin pass over the wave's band of the input state (rows staged with
+// their halo, buffer 0): ledger before, static diagnostic and the CFL maximum
+// of the first substep, which goes to the spread set `cfl_set` (as
+// v2_begin_kernel, FP32 strip layout).
 template <int SH>
-__global__ void __launch_bounds__(kBlock) v2_begin32_kernel(V2Args<F32P> a)
+__device__ inline void begin32_work(const V2Args<F32P> &a, const StripRows<SH> &R, int r0, int c0,
+                                    unsigned long long *cfl_set)
 {
     const int ny = a.ny, nx = a.nx;
-    // the kernel arguments of the prologue in one batch of scalar loads
-    __asm__ volatile("" ::"s"(ny), "s"(nx), "s"(a.b), "s"(a.h[0]), "s"(a.qx[0]), "s"(a.qy[0]),
-                     "s"(a.cfg.static_exit), "s"(a.cfg.weight_mode), "s"(a.cfg.dry_tolerance),
-                     "s"(a.cfg.earth_pressure), "s"(a.cfg.gravity), "s"(a.cfg.tan_start), "s"(a.dx), "s"(a.dy),
-                     "s"((int)gridDim.x));
-    const int tiles_x = (nx + kStripW - 1) / kStripW;
-    const int t = xcd_block();
-    const int r0 = (t / tiles_x) * (SH * (kBlock / kWave)) + wave_index() * SH, c0 = (t % tiles_x) * kStripW;
     const int lane = threadIdx.x & (kWave - 1);
     const int gc = c0 - 1 + lane;
     const bool owner = lane >= 1 && lane <= kStripW && gc < nx;
     const bool stat = a.cfg.static_exit != 0;
-    StripRows<SH> R;
-    strip_load<SH>(a, 0, r0, c0, R);
     double H[SH + 2];
 #pragma unroll
     for (int j = 0; j < SH + 2; j++) H[j] = (double)R.b[j] + (double)((float)R.hq[j] * (float)kQuantum);
@@ -921,15 +1085,7 @@ __global__ void __launch_bounds__(kBlock) v2_begin32_kernel(V2Args<F32P> a)
         const int hq = R.hq[j];
         const float qxv = R.qx[j], qyv = R.qy[j];
         L.add(r, ny, hq, qxv, qyv, R.b[j], cell_weight32(a.cfg.weight_mode, ny, nx, area, f6, r, gc), dry);
-        // CFL wave speed of the state, in the arithmetic of the substep
-        const float d = (float)hq * (float)kQuantum;
-        float vx = 0.0f, vy = 0.0f;
-        if (d > dry) {
-            const float inv = rcpf(d);
-            vx = qxv * inv;
-            vy = qyv * inv;
-        }
-        m[2] = npmax(m[2], (double)(sqrtf_hw(vx * vx + vy * vy) + sqrtf_hw(Kg * d)));
+        m[2] = npmax(m[2], (double)cfl_speed32(hq, qxv, qyv, dry, Kg));
         if (stat) {
             const double hl = gc > 0 ? HL : H[j], hr = gc + 1 < nx ? HR : H[j];
             const double hd = r > 0 ? H[j - 1] : H[j], hu = r + 1 < ny ? H[j + 1] : H[j];
@@ -947,9 +1103,54 @@ __global__ void __launch_bounds__(kBlock) v2_begin32_kernel(V2Args<F32P> a)
             atomic_max_nonneg(&a.spread[kSetStaticGrad * kSpreadWords + slot], sqrt(g2));
             atomic_max_nonneg(&a.spread[kSetStaticQ * kSpreadWords + slot], qm);
         }
-        atomic_max_nonneg(&a.spread[slot], wm);
-        if (blockIdx.x == 0 && threadIdx.x == 0) a.sync->ctl[0].remaining = a.cfg.dt;
+        atomic_max_nonneg(&cfl_set[slot], wm);
     }
+}
+
+// One pass over the input state (a step without speculation).
+template <int SH>
+__global__ void __launch_bounds__(kBlock) v2_begin32_kernel(V2Args<F32P> a)
+{
+    const int ny = a.ny, nx = a.nx;
+    // the kernel arguments of the prologue in one batch of scalar loads
+    __asm__ volatile("" ::"s"(ny), "s"(nx), "s"(a.b), "s"(a.h[0]), "s"(a.qx[0]), "s"(a.qy[0]),
+                     "s"(a.cfg.static_exit), "s"(a.cfg.weight_mode), "s"(a.cfg.dry_tolerance),
+                     "s"(a.cfg.earth_pressure), "s"(a.cfg.gravity), "s"(a.cfg.tan_start), "s"(a.dx), "s"(a.dy),
+                     "s"((int)gridDim.x));
+    const int tiles_x = (nx + kStripW - 1) / kStripW;
+    const int t = xcd_block();
+    const int r0 = (t / tiles_x) * (SH * (kBlock / kWave)) + wave_index() * SH, c0 = (t % tiles_x) * kStripW;
+    StripRows<SH> R;
+    strip_load<SH>(a, 0, r0, c0, R);
+    begin32_work<SH>(a, R, r0, c0, a.spread);
+    if (blockIdx.x == 0 && threadIdx.x == 0) a.sync->ctl[0].remaining = a.cfg.dt;
+}
+
+// Launch 0 of a speculative FP32 step: the begin pass over the input (ledger
+// before, static diagnostic, the true CFL maximum into the spec_true set) and
+// substep 0 with the CFL maximum the previous step's end pass predicted, in
+// one launch.  The input in buffer 0 is not written; launch 1 keeps the
+// substep if the true maximum equals the prediction and the input is not
+// static, else restarts from buffer 0 or ends the step.
+template <int SH, int DIAGF>
+__global__ void __launch_bounds__(kBlock) v2_first32_kernel(V2Args<F32P> a)
+{
+    const int ny = a.ny, nx = a.nx;
+    __asm__ volatile("" ::"s"(ny), "s"(nx), "s"(a.b), "s"(a.h[0]), "s"(a.h[1]), "s"(a.qx[0]), "s"(a.qx[1]),
+                     "s"(a.qy[0]), "s"(a.qy[1]), "s"(a.spread), "s"(a.diag_part), "s"(a.diag_groups),
+                     "s"(a.sync), "s"(a.ext_x), "s"((int)gridDim.x), "s"(a.pred), "s"(a.spec_true),
+                     "s"(a.cfg.static_exit), "s"(a.cfg.weight_mode), "s"(a.cfg.dry_tolerance),
+                     "s"(a.cfg.earth_pressure), "s"(a.cfg.gravity), "s"(a.cfg.tan_start), "s"(a.dx), "s"(a.dy));
+    const int tiles_x = (nx + kStripW - 1) / kStripW;
+    const int t = xcd_block();
+    const int r0 = (t / tiles_x) * (SH * (kBlock / kWave)) + wave_index() * SH, c0 = (t % tiles_x) * kStripW;
+    const int lane = threadIdx.x & (kWave - 1);
+    const double pslot = __longlong_as_double((long long)a.pred[lane * kSpreadStride]);
+    StripRows<SH> rows;
+    strip_load<SH>(a, 0, r0, c0, rows);
+    begin32_work<SH>(a, rows, r0, c0, a.spec_true);
+    const double pred = wave_max(pslot);
+    substep32_work<SH, DIAGF>(a, rows, fresh_control(a.cfg.dt), pred, 0, r0, c0, 0.0, pred);
 }
 
 // copy back when the final state sits in scratch; ledger after (FP32 strips).
@@ -987,8 +1188,10 @@ __device__ inline void end32_body(const V2Args<F32P> &a, int fsrc)
     }
     const float area = uniform((float)(a.dx * a.dy)), f6 = uniform((float)(a.dx * a.dy / 6.0));
     const float dry = uniform((float)a.cfg.dry_tolerance);
+    const float Kg = uniform((float)(a.cfg.earth_pressure * a.cfg.gravity));
     Ledger32 L;
     L.init();
+    double pred = 0.0;
 #pragma unroll
     for (int j = 0; j < SH; j++) {
         const int r = r0 + j;
@@ -1000,8 +1203,15 @@ __device__ inline void end32_body(const V2Args<F32P> &a, int fsrc)
             a.qy[0][i] = qyv[j];
         }
         L.add(r, ny, hq[j], qxv[j], qyv[j], bv[j], cell_weight32(a.cfg.weight_mode, ny, nx, area, f6, r, gc), dry);
+        pred = npmax(pred, (double)cfl_speed32(hq[j], qxv[j], qyv[j], dry, Kg));
     }
     store_ledger32(a, L, gc, owner, false);
+    // the CFL maximum of the result, as the next step's begin pass would
+    // compute it: that step's launch 0 takes its first substep with it and
+    // launch 1 checks it against the true maximum of that step's input
+    const double pm = wave_max(pred);
+    if (lane == 0)
+        atomic_max_nonneg(&a.pred_next[((blockIdx.x * (kBlock / kWave) + wave_index()) % kSpread) * kSpreadStride], pm);
 }
 
 template <int SH>
@@ -1089,6 +1299,11 @@ __global__ void __launch_bounds__(kBlock) v2_fold_kernel(V2Args<P> a, int k, int
         unsigned *ps = reinterpret_cast<unsigned *>(a.sync_next);
         for (int i = lane; i < (int)(sizeof(SyncBlock) / 4); i += kWave) ps[i] = 0u;
         for (int i = lane; i < kStepSpreadSets * kSpread; i += kWave) a.spread_next[(int64_t)i * kSpreadStride] = 0ull;
+        if constexpr (P::kQuant) {
+            // the next step's true-maximum set, and this step's (consumed) prediction
+            a.spec_true_next[lane * kSpreadStride] = 0ull;
+            a.pred[lane * kSpreadStride] = 0ull;
+        }
     }
     // each wave's lane 0 wrote one field of the host record: stamp word w
     if (w < kFoldWaves && lane == 0) host_stamp(&stamps[w], stamp);
@@ -1158,16 +1373,45 @@ static void enqueue_begin(const V2Args<P> &a, const StepLaunch &L, hipStream_t s
     }
 }
 
+// launch 0 of a speculative FP32 step (begin pass + substep 0)
+template <class P>
+static void enqueue_first(const V2Args<P> &a, const StepLaunch &L, hipStream_t st)
+{
+    if constexpr (P::kQuant) {
+        const bool d = a.cfg.diagnostics == 1;
+        switch (L.band) {
+        case 2: if (d) hipLaunchKernelGGL((v2_first32_kernel<2, 1>), dim3(L.gu), dim3(kBlock), 0, st, a);
+                else hipLaunchKernelGGL((v2_first32_kernel<2, 0>), dim3(L.gu), dim3(kBlock), 0, st, a); break;
+        case 3: if (d) hipLaunchKernelGGL((v2_first32_kernel<3, 1>), dim3(L.gu), dim3(kBlock), 0, st, a);
+                else hipLaunchKernelGGL((v2_first32_kernel<3, 0>), dim3(L.gu), dim3(kBlock), 0, st, a); break;
+        case 5: if (d) hipLaunchKernelGGL((v2_first32_kernel<5, 1>), dim3(L.gu), dim3(kBlock), 0, st, a);
+                else hipLaunchKernelGGL((v2_first32_kernel<5, 0>), dim3(L.gu), dim3(kBlock), 0, st, a); break;
+        case 6: if (d) hipLaunchKernelGGL((v2_first32_kernel<6, 1>), dim3(L.gu), dim3(kBlock), 0, st, a);
+                else hipLaunchKernelGGL((v2_first32_kernel<6, 0>), dim3(L.gu), dim3(kBlock), 0, st, a); break;
+        default: if (d) hipLaunchKernelGGL((v2_first32_kernel<4, 1>), dim3(L.gu), dim3(kBlock), 0, st, a);
+                 else hipLaunchKernelGGL((v2_first32_kernel<4, 0>), dim3(L.gu), dim3(kBlock), 0, st, a); break;
+        }
+    }
+}
+
 // one FP32 substep launch of band height SH, all ten diagnostic sums or the
 // production record's five
+template <class P, int SH, int DIAGF, int EAGER>
+static void launch_substep32_c(const V2Args<P> &a, int grid, hipStream_t st, int k)
+{
+    // the first two launches of a step test the static exit / the speculation
+    if (k <= 1) hipLaunchKernelGGL((v2_substep_kernel<P, SH, DIAGF, EAGER, 1>), dim3(grid), dim3(kBlock), 0, st, a, k);
+    else hipLaunchKernelGGL((v2_substep_kernel<P, SH, DIAGF, EAGER, 0>), dim3(grid), dim3(kBlock), 0, st, a, k);
+}
+
 template <class P, int SH>
 static void launch_substep32(const V2Args<P> &a, int grid, hipStream_t st, int k, int eager)
 {
     const int d = a.cfg.diagnostics == 1 ? 1 : 0;
-    if (d && eager) hipLaunchKernelGGL((v2_substep_kernel<P, SH, 1, 1>), dim3(grid), dim3(kBlock), 0, st, a, k);
-    else if (d) hipLaunchKernelGGL((v2_substep_kernel<P, SH, 1, 0>), dim3(grid), dim3(kBlock), 0, st, a, k);
-    else if (eager) hipLaunchKernelGGL((v2_substep_kernel<P, SH, 0, 1>), dim3(grid), dim3(kBlock), 0, st, a, k);
-    else hipLaunchKernelGGL((v2_substep_kernel<P, SH, 0, 0>), dim3(grid), dim3(kBlock), 0, st, a, k);
+    if (d && eager) launch_substep32_c<P, SH, 1, 1>(a, grid, st, k);
+    else if (d) launch_substep32_c<P, SH, 1, 0>(a, grid, st, k);
+    else if (eager) launch_substep32_c<P, SH, 0, 1>(a, grid, st, k);
+    else launch_substep32_c<P, SH, 0, 0>(a, grid, st, k);
 }
 
 // substep launches k0 .. k0 + count - 1.  Timing mode brackets the launches
@@ -1175,10 +1419,10 @@ static void launch_substep32(const V2Args<P> &a, int grid, hipStream_t st, int k
 // pair: events between launches would add their own latency to every launch.
 template <class P>
 static int enqueue_substeps(mv2_context *ctx, const V2Args<P> &a, const StepLaunch &L, hipStream_t st, int k0,
-                            int count, int eager_until, bool timing)
+                            int count, int eager_until, bool timing, int first_timed = 0)
 {
     for (int k = k0; k < k0 + count; k++) {
-        if (timing && k == 0) MV2_HIP(ctx, hipEventRecord(ctx->sub_events[0], st));
+        if (timing && k == first_timed) MV2_HIP(ctx, hipEventRecord(ctx->sub_events[0], st));
         const int eager = k < eager_until ? 1 : 0;
         if constexpr (P::kQuant) {
             switch (L.band) {
@@ -1191,7 +1435,7 @@ static int enqueue_substeps(mv2_context *ctx, const V2Args<P> &a, const StepLaun
         } else {
             hipLaunchKernelGGL(v2_substep_kernel<P>, dim3(L.gu), dim3(kBlock), 0, st, a, k);
         }
-        if (timing && k0 == 0 && k == eager_until - 1) MV2_HIP(ctx, hipEventRecord(ctx->sub_events[1], st));
+        if (timing && k0 == first_timed && k == eager_until - 1) MV2_HIP(ctx, hipEventRecord(ctx->sub_events[1], st));
     }
     return MV2_OK;
 }
@@ -1356,6 +1600,10 @@ static int launch_step(mv2_context *ctx, const mv2_step_config *cfg, mv2_step_re
     a.diag_part = (double *)ctx->step_diag.ptr;
     a.diag_groups = gdiag;
     a.spread = parity ? spread_odd : spread_even;
+    a.spec_true = spread_even + (size_t)(kSpecTrueSet + parity) * kSpreadWords;
+    a.spec_true_next = spread_even + (size_t)(kSpecTrueSet + (parity ^ 1)) * kSpreadWords;
+    a.pred = spread_even + (size_t)(kSpecPredSet + parity) * kSpreadWords;
+    a.pred_next = spread_even + (size_t)(kSpecPredSet + (parity ^ 1)) * kSpreadWords;
     // the fold pass writes its results straight into pinned host memory
     // (device-visible): no copy launch before the stream sync
     FoldOut *fold = (FoldOut *)ctx->host_result.ptr;
@@ -1369,20 +1617,39 @@ static int launch_step(mv2_context *ctx, const mv2_step_config *cfg, mv2_step_re
     if (timing) MV2_HIP(ctx, hipEventRecord(ctx->ev_start, st));
     // this parity's loop state is zero if the previous step completed; an
     // error anywhere below leaves v2_clean at -1 and the next step resets
-    if (ctx->v2_clean != parity) hipLaunchKernelGGL(v2_reset_kernel, dim3(1), dim3(kBlock), 0, st, a.sync, a.spread);
+    // Speculation (FP32): the previous step left a CFL prediction of this
+    // state (its end pass, and no library call wrote mobile or momentum
+    // since) and this parity's loop state is clean; not under graph replay
+    // or a fixed chunk (the summary pass).
+    const bool spec = P::kQuant && ctx->v2_spec && ctx->v2_pred_ready && ctx->v2_clean == parity &&
+                      !ctx->graphs && fixed_chunk == 0;
+    ctx->v2_pred_ready = 0;
+    if (ctx->v2_clean != parity)
+        hipLaunchKernelGGL(v2_reset_kernel, dim3(1), dim3(kBlock), 0, st, a.sync, a.spread, a.spec_true, a.pred_next);
     ctx->v2_clean = -1;
     int k = 0;
     int chunk = fixed_chunk > 0 ? fixed_chunk : (ctx->v2_predicted > 0 ? ctx->v2_predicted + 1 : 4);
     // launches expected to work stage their tile before the loop state arrives
     int eager_until = fixed_chunk > 0 ? 0 : (ctx->v2_predicted > 0 ? ctx->v2_predicted : 4);
-    const int bracket = eager_until;        // timing mode: launches 0 .. bracket - 1 between the event pair
+    // timing mode: launches first .. bracket - 1 between the event pair (a
+    // speculative step's launch 0 also carries the begin pass: left out)
+    const int first_timed = spec ? 1 : 0;
+    const int bracket = eager_until;
     const StepLaunch L{band, gs, gu, (kFoldWaves + kBlock / kWave - 1) / (kBlock / kWave), fold};
     // the first round (begin, predicted substeps, end / fold) as one graph launch
     bool replayed = false;
     if (ctx->graphs && !timing && fixed_chunk == 0)
         if (int rc = replay_step<P>(ctx, a, L, chunk, eager_until, st, &replayed)) return rc;
-    if (replayed) k = chunk;
-    else enqueue_begin<P>(a, L, st);
+    if (replayed) {
+        k = chunk;
+    } else if (spec) {
+        // launch 0: begin pass and the speculative substep 0
+        enqueue_first<P>(a, L, st);
+        k = 1;
+        chunk -= 1;
+    } else {
+        enqueue_begin<P>(a, L, st);
+    }
     MV2_HIP(ctx, hipGetLastError());
     unsigned stamp = 0u;
     for (;;) {
@@ -1394,7 +1661,7 @@ static int launch_step(mv2_context *ctx, const mv2_step_config *cfg, mv2_step_re
                     ctx->sub_events.push_back(e);
                 }
             }
-            if (int rc = enqueue_substeps<P>(ctx, a, L, st, k, chunk, eager_until, timing)) return rc;
+            if (int rc = enqueue_substeps<P>(ctx, a, L, st, k, chunk, eager_until, timing, first_timed)) return rc;
             k += chunk;
             stamp = stamped ? next_stamp(ctx, kFoldWaves) : 0u;
             enqueue_close<P>(a, L, st, k, stamp_words(ctx), stamp);
@@ -1416,6 +1683,13 @@ static int launch_step(mv2_context *ctx, const mv2_step_config *cfg, mv2_step_re
     ctx->v2_frame++;
     ctx->v2_last_sync = parity;
     if (fixed_chunk == 0) ctx->v2_predicted = out->substeps;
+    // the end pass ran (no static exit): it left the CFL prediction of this
+    // result for the next step
+    if (P::kQuant && !out->static_exit) ctx->v2_pred_ready = 1;
+    if (spec) {
+        ctx->spec_steps++;
+        if (fold->ctl.restarted) ctx->spec_restarts++;
+    }
     out->launches = k;
     out->kernel_ms = 0.0;
     out->substep_kernel_ms = 0.0;
@@ -1424,11 +1698,13 @@ static int launch_step(mv2_context *ctx, const mv2_step_config *cfg, mv2_step_re
         MV2_HIP(ctx, hipEventElapsedTime(&ms, ctx->ev_start, ctx->ev_stop));
         out->kernel_ms = ms;
     }
-    if (timing && bracket > 0 && out->substeps == bracket) {
+    out->timed_launches = 0;
+    if (timing && bracket > first_timed && out->substeps == bracket) {
         // substeps are the first launches: the bracket held exactly the working ones
         float e = 0.0f;
         MV2_HIP(ctx, hipEventElapsedTime(&e, ctx->sub_events[0], ctx->sub_events[1]));
         out->substep_kernel_ms = e;
+        out->timed_launches = bracket - first_timed;
     }
     return MV2_OK;
 }
@@ -1542,3 +1818,18 @@ extern "C" int mv2_set_graphs(mv2_context *ctx, int enable)
 }
 
 extern "C" long long mv2_graph_replays(const mv2_context *ctx) { return ctx ? ctx->graph_replays : -1; }
+
+extern "C" int mv2_set_speculation(mv2_context *ctx, int enable)
+{
+    if (!ctx) return set_error(nullptr, MV2_EINVAL, "[MobileV2] null context");
+    ctx->v2_spec = enable ? 1 : 0;
+    return MV2_OK;
+}
+
+extern "C" int mv2_speculation_stats(const mv2_context *ctx, long long out[2])
+{
+    if (!ctx || !out) return set_error(nullptr, MV2_EINVAL, "[MobileV2] null argument");
+    out[0] = ctx->spec_steps;
+    out[1] = ctx->spec_restarts;
+    return MV2_OK;
+}

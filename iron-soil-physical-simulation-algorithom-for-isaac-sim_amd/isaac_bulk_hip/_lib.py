@@ -42,7 +42,7 @@ class StepRecord(ctypes.Structure):
         ("maximum_cfl", _D), ("remaining", _D),
         ("summary_before", _D * 8), ("summary_after", _D * 8), ("diag", _D * 10),
         ("mass_quanta_before", _D), ("mass_quanta_after", _D), ("kernel_ms", _D),
-        ("substep_kernel_ms", _D), ("launches", _I), ("pad", _I),
+        ("substep_kernel_ms", _D), ("launches", _I), ("timed_launches", _I),
     ]
 
 
@@ -126,6 +126,8 @@ SIGNATURES: dict[str, tuple[object, list[object]]] = {
     "mv2_static_diagnostic": (_I, [_P, _P]),
     "mv2_set_timing": (_I, [_P, _I]),
     "mv2_set_graphs": (_I, [_P, _I]),
+    "mv2_set_speculation": (_I, [_P, _I]),
+    "mv2_speculation_stats": (_I, [_P, _P]),
     "mv2_graph_replays": (ctypes.c_longlong, [_P]),
     "mv2_set_external": (_I, [_P, _P, _P, _D]),
     "mv2_batch_bind": (_I, [_P, _I, _P, _P, _P, _D, _D, _P, _P, _P, _P]),
