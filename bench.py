@@ -39,7 +39,7 @@ HBM_PEAK_GBS = 8000.0
 # from rocprofv3 FETCH_SIZE / WRITE_SIZE scaled by a 64 MiB copy
 # (profiles/v2_substep_701x701_cutface_traffic.md); counters cannot be read
 # from inside the timed run.
-MEASURED_TRAFFIC_BYTES = {"fp32": 15.04e6}
+MEASURED_TRAFFIC_BYTES = {"fp32": 15.20e6}
 # the same for a full launch of the batched substep (256 terrains, 18 274 304
 # cells; profiles/batch_substep_256_terrains.md)
 MEASURED_TRAFFIC_BATCHED_FULL = 515.1e6

@@ -252,6 +252,21 @@ class HipProductionMobileV2Solver:
         rt = self.runtime if self.runtime is not None else self.state.runtime
         return int(rt.lib.mv2_graph_replays(rt.ctx))
 
+    def set_speculation(self, enable: bool) -> None:
+        """FP32: run the next step's begin pass and first substep in one launch on
+        the CFL maximum predicted by this step's end pass (default on).  A
+        prediction that does not hold bit for bit restarts the step from its
+        input, so results are identical either way."""
+        rt = self.runtime if self.runtime is not None else self.state.runtime
+        rt.check(rt.lib.mv2_set_speculation(rt.ctx, int(bool(enable))))
+
+    def speculation_stats(self) -> tuple[int, int]:
+        """(steps that speculated, steps whose prediction did not hold) on this runtime."""
+        rt = self.runtime if self.runtime is not None else self.state.runtime
+        out = (ctypes.c_longlong * 2)()
+        rt.check(rt.lib.mv2_speculation_stats(rt.ctx, out))
+        return int(out[0]), int(out[1])
+
     def step_resident(self, dt_s: float, **_) -> WarpMobileStep:
         if self.state is None:
             raise RuntimeError("[MobileV2Production] bind state first")

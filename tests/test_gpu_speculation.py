@@ -52,6 +52,19 @@ def _stats(state):
     return int(out[0]), int(out[1])
 
 
+def test_solver_switch_and_counters():
+    runs = _runs()
+    for _ in range(3):
+        _frame(runs)
+    assert runs[0][1].speculation_stats() == _stats(runs[0][0]) == (2, 0)
+    runs[0][1].set_speculation(False)
+    _frame(runs)
+    assert runs[0][1].speculation_stats() == (2, 0)
+    runs[0][1].set_speculation(True)
+    _frame(runs)                                 # the step after a plain one left a prediction again
+    assert runs[0][1].speculation_stats() == (3, 0)
+
+
 def _frame(runs, relax=True):
     """One frame on both states; fields and records must agree bit for bit."""
     import torch
