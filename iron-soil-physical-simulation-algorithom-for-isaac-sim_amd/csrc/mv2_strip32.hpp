@@ -281,9 +281,17 @@ __device__ inline bool strip_interior(int ny, int nx, int r0, int c0)
     return r0 > 0 && r0 + SH < ny && c0 > 0 && c0 + kStripW < nx;
 }
 
-template <int SH, class A, int DIAGF = 0, bool INTERIOR = false>
+// per-cell consumer of the new state that does nothing (the default)
+struct NoCellSink {
+    __device__ void operator()(int, int, int, float, float, float) const {}
+};
+
+// `sink(row, column, quanta, qx, qy, b)` sees every owned cell's new state
+// as it is stored (a step's last substep forms the closing ledger there).
+template <int SH, class A, int DIAGF = 0, bool INTERIOR = false, class Sink = NoCellSink>
 __device__ inline void strip_update(const A &a, const StripRows<SH> &R, int src, int r0, int c0, double sub_dt,
-                                    float (&dg)[kDiag], float &wv, long long &hn_min, int &ovf)
+                                    float (&dg)[kDiag], float &wv, long long &hn_min, int &ovf,
+                                    Sink &&sink = NoCellSink{})
 {
     const int lane = threadIdx.x & (kWave - 1);
     const int ny = a.ny, nx = a.nx;
@@ -396,6 +404,7 @@ __device__ inline void strip_update(const A &a, const StripRows<SH> &R, int src,
             at_bytes(oqx, o) = qax;
             at_bytes(oqy, o) = qay;
             wv = fmaxf(wv, wave_v + sqrtf_hw(k.Kg * depth));
+            sink(gr, gc, hq, qax, qay, R.b[j]);
         }
         fdown = fup;
         cur = next;

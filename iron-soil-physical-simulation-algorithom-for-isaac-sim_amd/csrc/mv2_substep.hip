@@ -55,7 +55,8 @@ struct Control {
     int numeric;               // 0 ok, 1 positivity, 2 substep limit, 3 int32 range
     int cfl_limited, static_exit;
     int rolled;                // 1 once a failed substep was rolled back
-    int end_done;              // FP32: a launch after the loop's end copied back and stored the ledger after
+    int end_done;              // FP32 ledger after: 1 stored by a launch past the loop's end (state copied
+                               // back), 2 stored by the last substep (the fold pass copies back)
     // FP32 speculative step: launch 0 took substep 0 with the CFL maximum
     // `pred` predicted by the previous step's end pass; launch 1 checks it
     // (restarted: it did not hold, and the step began again from its input)
@@ -443,6 +444,7 @@ __device__ inline void apply_failure_flags(Control &C, int f)
         C.substeps = C.last_substeps;
         C.done = 1;
         C.rolled = 1;
+        C.end_done = 0;    // a closing ledger of the failed substep is void
     }
 }
 
@@ -536,6 +538,124 @@ __global__ void __launch_bounds__(kBlock) v2_begin_kernel(V2Args<P> a)
     }
 }
 
+// ---------------------------------------------------------------------------
+// FP32 ledger passes in the strip layout of the substep (one tile of the
+// substep grid per workgroup, one band of SH rows per wave): x-neighbours
+// by DPP, y-neighbours from the rows in registers, no integer division.  Per
+// cell the ledger terms of warp_mobile_v2.py:41-54 are formed in float and
+// summed per lane in float over the band, then in double; the mobile volume
+// itself comes exactly from the quanta per weight class.
+// ---------------------------------------------------------------------------
+
+// CFL wave speed of one cell of an FP32 state, in the arithmetic of the
+// substep: |v| + sqrt(K g h) with the cell velocity from the hardware
+// reciprocal (zero where the cell is dry)
+__device__ inline float cfl_speed32(int hq, float qxv, float qyv, float dry, float Kg)
+{
+    const float d = (float)hq * (float)kQuantum;
+    float vx = 0.0f, vy = 0.0f;
+    if (d > dry) {
+        const float inv = rcpf(d);
+        vx = qxv * inv;
+        vy = qyv * inv;
+    }
+    return sqrtf_hw(vx * vx + vy * vy) + sqrtf_hw(Kg * d);
+}
+
+struct Ledger32 {
+    float qx, qy, ke, pot, prs, vmax, hmin;   // sums weighted by the cell weight; max speed, min depth
+    // quanta of the band's cells by row class: first row, inner rows, last row
+    // (three scalars, not an array: a select between array slots would be
+    // turned into a dynamically indexed private array)
+    long long q_first, q_inner, q_last;
+    __device__ void init()
+    {
+        qx = qy = ke = pot = prs = vmax = 0.0f;
+        hmin = 3.0e38f;
+        q_first = q_inner = q_last = 0;
+    }
+    // one cell at row r (uniform over the wave), weight w, terrain b
+    __device__ void add(int r, int ny, int hq, float qxv, float qyv, float b, float w, float dry)
+    {
+        const float d = (float)hq * (float)kQuantum;
+        const long long u = (unsigned)hq;
+        q_first += r == 0 ? u : 0;
+        q_last += r == ny - 1 ? u : 0;
+        q_inner += (r != 0 && r != ny - 1) ? u : 0;
+        qx += qxv * w;
+        qy += qyv * w;
+        if (d > dry) {
+            const float inv = rcpf(d);
+            const float vx = qxv * inv, vy = qyv * inv;
+            const float s2 = vx * vx + vy * vy;
+            vmax = fmaxf(vmax, sqrtf_hw(s2));
+            ke += d * s2 * w;
+        }
+        pot += d * b * w;
+        prs += d * d * w;
+        hmin = fminf(hmin, d);
+    }
+};
+
+// weight of cell (r, c): the cell area, or the Triangle-A-C vertex weight
+// (bulk_state_authority.py:624-634) as a multiple of area / 6
+__device__ inline float cell_weight32(int mode, int ny, int nx, float area, float f6, int r, int c)
+{
+    if (!mode) return area;
+    const int n = (r < ny - 1 && c < nx - 1 ? 2 : 0) + (r < ny - 1 && c > 0 ? 1 : 0) + (r > 0 && c > 0 ? 2 : 0) +
+                  (r > 0 && c < nx - 1 ? 1 : 0);
+    return (float)n * f6;
+}
+
+// 64-bit value of lane `src` (uniform) of the wave
+__device__ inline long long readlane64(long long v, int src)
+{
+    const unsigned long long u = (unsigned long long)v;
+    const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)u, src);
+    const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(u >> 32), src);
+    return (long long)(((unsigned long long)hi << 32) | lo);
+}
+
+// The wave's ledger as partial gw (before or after) of a.groups: fixed-order
+// DPP trees over the lanes (sums and extrema in double, the quanta of each
+// row class as integers), no barrier.  Weight class = row class (first,
+// inner, last row) x column class (first, inner, last column); the first and
+// last columns are at most one lane each and are read back from it.
+__device__ inline void store_ledger32(const V2Args<F32P> &a, const Ledger32 &L, int gc, bool owner, bool before)
+{
+    const double rho = a.cfg.density, g = a.cfg.gravity, K = a.cfg.earth_pressure;
+    const double f0 = wave_sum((double)L.qx), f1 = wave_sum((double)L.qy);
+    const double f2 = wave_max((double)L.vmax);
+    const double f3 = wave_sum((double)L.ke), f4 = wave_sum((double)L.pot), f5 = wave_sum((double)L.prs);
+    const double f6 = wave_min((double)L.hmin);
+    const unsigned long long first_col = __ballot(owner && gc == 0);
+    const unsigned long long last_col = __ballot(owner && gc == a.nx - 1 && gc != 0);
+    const int lf = first_col ? __ffsll((long long)first_col) - 1 : 0;
+    const int ll = last_col ? __ffsll((long long)last_col) - 1 : 0;
+    long long mq[kClasses];
+#pragma unroll
+    for (int r = 0; r < 3; r++) {
+        const long long v = owner ? (r == 0 ? L.q_first : (r == 1 ? L.q_inner : L.q_last)) : 0;
+        const long long tot = wave_sum(v);
+        const long long e0 = first_col ? readlane64(v, lf) : 0;
+        const long long e2 = last_col ? readlane64(v, ll) : 0;
+        mq[3 * r] = e0;
+        mq[3 * r + 1] = tot - e0 - e2;
+        mq[3 * r + 2] = e2;
+    }
+    if ((threadIdx.x & (kWave - 1)) == 0) {
+        const int G = a.groups, gw = blockIdx.x * (kBlock / kWave) + wave_index();
+        double *dst = a.ledger_f + (before ? 0 : kSum) * G + gw;
+        const double vals[kSum] = {0.0, rho * f0, rho * f1, f2, 0.5 * rho * f3, rho * g * f4,
+                                   rho * 0.5 * K * g * f5, f6};
+#pragma unroll
+        for (int q = 0; q < kSum; q++) dst[q * G] = vals[q];
+        long long *qd = a.ledger_q + (before ? 0 : kClasses) * G + gw;
+#pragma unroll
+        for (int q = 0; q < kClasses; q++) qd[q * G] = mq[q];
+    }
+}
+
 // Substep k of the loop of mobile_v2_reference.py:236-307: decide from the
 // loop state whether the loop has ended (then return at once), otherwise take
 // sub_dt from the CFL maximum of the current state, update every cell into the
@@ -556,11 +676,59 @@ constexpr int kSubFold = 3 + kDiag;   // values folded per workgroup at the end 
 template <int SH>
 __device__ inline void end32_body(const V2Args<F32P> &a, int fsrc);
 
+// Per-cell sink of a step's last substep: the closing ledger of the new state
+// and the CFL maximum the next step's first launch will use as its prediction,
+// each cell's terms formed exactly as the end pass forms them from the stored
+// state (end32_body), so the partials are the same bits.
+struct LedgerSink {
+    Ledger32 L;
+    double pred;
+    int mode, ny, nx;
+    float area, f6, dry, Kg;
+    __device__ void operator()(int r, int gc, int hq, float qxv, float qyv, float b)
+    {
+        L.add(r, ny, hq, qxv, qyv, b, cell_weight32(mode, ny, nx, area, f6, r, gc), dry);
+        pred = npmax(pred, (double)cfl_speed32(hq, qxv, qyv, dry, Kg));
+    }
+};
+
+// The loop state after a substep of sub_dt taken from state C with CFL
+// maximum maxw; the loop's own exits in the reference's order
+// (mobile_v2_reference.py:236-307).  Uniform: every wave forms it.
+__device__ inline Control next_control(const Control &C, const mv2_step_config &cfg, double maxw, double sub_dt,
+                                       double mind)
+{
+    Control nc = C;
+    nc.last_remaining = C.remaining;
+    nc.last_substeps = C.substeps;
+    nc.last_src = C.src;
+    nc.max_cfl = npmax(C.max_cfl, maxw * sub_dt / mind);
+    if (sub_dt < C.remaining - 1.0e-14) nc.cfl_limited = 1;
+    nc.remaining = C.remaining - sub_dt;
+    nc.substeps = C.substeps + 1;
+    nc.src = C.src ^ 1;
+    if (cfg.limit_mode == 1 && nc.substeps > cfg.maximum_substeps) {
+        nc.numeric = 2;
+        nc.done = 1;
+    } else if (!(nc.remaining > 1.0e-14)) {
+        nc.done = 1;
+    } else if (cfg.limit_mode == 0 && nc.substeps >= cfg.maximum_substeps) {
+        nc.numeric = 2;
+        nc.done = 1;
+    }
+    return nc;
+}
+
 // One FP32 substep of the wave's band, staged from buffer C.src: dt from the
-// CFL maximum maxw, the successor loop state (workgroup 0; spec_pred >= 0
-// marks it speculative, with that predicted maximum, for launch 1 to check),
-// the update into the other buffer and the wave-level epilogue.  dprev is
-// this lane's diagnostic slot before the substep (0 for a step's first).
+// CFL maximum maxw, the successor loop state (written by workgroup 0;
+// spec_pred >= 0 marks it speculative, with that predicted maximum, for
+// launch 1 to check), the update into the other buffer and the wave-level
+// epilogue.  dprev is this lane's diagnostic slot before the substep (0 for
+// a step's first).  A substep that ends the loop and is not speculative also
+// stores the closing ledger and the next step's CFL prediction from the new
+// state in its registers (end_done = 2); the fold pass then copies the state
+// back when it sits in the second buffer.  A failure flagged by this substep
+// rolls the loop back and voids that ledger (apply_failure_flags).
 template <int SH, int DIAGF>
 __device__ inline void substep32_work(const V2Args<F32P> &a, const StripRows<SH> &rows, const Control &C,
                                       double maxw, int k, int r0, int c0, double dprev, double spec_pred)
@@ -572,43 +740,46 @@ __device__ inline void substep32_work(const V2Args<F32P> &a, const StripRows<SH>
     const double mind = a.dx <= a.dy ? a.dx : a.dy;
     const double stable = maxw <= 1.0e-14 ? C.remaining : cfg.cfl * mind / maxw;
     const double sub_dt = npmin(C.remaining, stable);
-    if (blockIdx.x == 0) spread_clear(a.spread + ((k + 2) % kWaveSets) * kSpreadWords);
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
-        Control nc = C;
-        nc.last_remaining = C.remaining;
-        nc.last_substeps = C.substeps;
-        nc.last_src = C.src;
-        nc.max_cfl = npmax(C.max_cfl, maxw * sub_dt / mind);
-        if (sub_dt < C.remaining - 1.0e-14) nc.cfl_limited = 1;
-        nc.remaining = C.remaining - sub_dt;
-        nc.substeps = C.substeps + 1;
-        nc.src = C.src ^ 1;
-        // the loop's own exits, in the reference's order
-        if (cfg.limit_mode == 1 && nc.substeps > cfg.maximum_substeps) {
-            nc.numeric = 2;
-            nc.done = 1;
-        } else if (!(nc.remaining > 1.0e-14)) {
-            nc.done = 1;
-        } else if (cfg.limit_mode == 0 && nc.substeps >= cfg.maximum_substeps) {
-            nc.numeric = 2;
-            nc.done = 1;
-        }
-        if (spec_pred >= 0.0) {
-            nc.spec = 1;
-            nc.pred = spec_pred;
-        }
-        s->ctl[(k + 1) & 1] = nc;
+    Control nc = next_control(C, cfg, maxw, sub_dt, mind);
+    const bool last = spec_pred < 0.0 && nc.done;
+    if (last) nc.end_done = 2;
+    if (spec_pred >= 0.0) {
+        nc.spec = 1;
+        nc.pred = spec_pred;
     }
+    if (blockIdx.x == 0) spread_clear(a.spread + ((k + 2) % kWaveSets) * kSpreadWords);
+    if (blockIdx.x == 0 && threadIdx.x == 0) s->ctl[(k + 1) & 1] = nc;
     float dg[kDiag];
 #pragma unroll
     for (int d = 0; d < kDiag; d++) dg[d] = 0.0f;
     float wv = 0.0f;
     long long hn_min = 0;
     int ovf = 0;
-    if (strip_interior<SH>(a.ny, a.nx, r0, c0))
+    const bool interior = strip_interior<SH>(a.ny, a.nx, r0, c0);
+    if (last) {
+        LedgerSink sink;
+        sink.L.init();
+        sink.pred = 0.0;
+        sink.mode = cfg.weight_mode;
+        sink.ny = a.ny;
+        sink.nx = a.nx;
+        sink.area = uniform((float)(a.dx * a.dy));
+        sink.f6 = uniform((float)(a.dx * a.dy / 6.0));
+        sink.dry = uniform((float)cfg.dry_tolerance);
+        sink.Kg = uniform((float)(cfg.earth_pressure * cfg.gravity));
+        if (interior)
+            strip_update<SH, V2Args<F32P>, DIAGF, true>(a, rows, C.src, r0, c0, sub_dt, dg, wv, hn_min, ovf, sink);
+        else
+            strip_update<SH, V2Args<F32P>, DIAGF, false>(a, rows, C.src, r0, c0, sub_dt, dg, wv, hn_min, ovf, sink);
+        const int gc = c0 - 1 + lane;
+        store_ledger32(a, sink.L, gc, lane >= 1 && lane <= kStripW && gc < a.nx, false);
+        const double pm = wave_max(sink.pred);
+        if (lane == 0) atomic_max_nonneg(&a.pred_next[(gw % kSpread) * kSpreadStride], pm);
+    } else if (interior) {
         strip_update<SH, V2Args<F32P>, DIAGF, true>(a, rows, C.src, r0, c0, sub_dt, dg, wv, hn_min, ovf);
-    else
+    } else {
         strip_update<SH, V2Args<F32P>, DIAGF, false>(a, rows, C.src, r0, c0, sub_dt, dg, wv, hn_min, ovf);
+    }
     // Wave-level epilogue, no barrier: the wave's wave-speed maximum goes to a
     // spread slot of its own, a failure (negative thickness before clamping,
     // int32 range) to the flags, and its diagnostic sums (a fixed-order DPP
@@ -786,9 +957,13 @@ __global__ void __launch_bounds__(kBlock) v2_substep_kernel(V2Args<P> a, int k)
     }
     if (C.done) {
         if constexpr (P::kQuant) {
-            // the first launch past the loop's end: copy back + ledger after
-            if (!C.end_done && !C.static_exit) end32_body<SH>(a, C.src);
-            C.end_done = 1;
+            // the first launch past the loop's end: copy back + ledger after,
+            // unless the last substep stored the ledger (end_done 2: the fold
+            // pass copies back)
+            if (C.end_done == 0) {
+                if (!C.static_exit) end32_body<SH>(a, C.src);
+                C.end_done = 1;
+            }
             // a static input after a speculative substep 0: its diagnostics are void
             if (spec_static && lane < kDiag) a.diag_part[(int64_t)lane * a.diag_groups + gw] = 0.0;
         }
@@ -934,124 +1109,6 @@ __global__ void __launch_bounds__(kBlock) v2_end_kernel(V2Args<P> a, int k)
         }
     }
     store_ledger<P>(a, L, false, smem);
-}
-
-// ---------------------------------------------------------------------------
-// FP32 ledger passes in the strip layout of the substep (one tile of the
-// substep grid per workgroup, one band of SH rows per wave): x-neighbours
-// by DPP, y-neighbours from the rows in registers, no integer division.  Per
-// cell the ledger terms of warp_mobile_v2.py:41-54 are formed in float and
-// summed per lane in float over the band, then in double; the mobile volume
-// itself comes exactly from the quanta per weight class.
-// ---------------------------------------------------------------------------
-
-// CFL wave speed of one cell of an FP32 state, in the arithmetic of the
-// substep: |v| + sqrt(K g h) with the cell velocity from the hardware
-// reciprocal (zero where the cell is dry)
-__device__ inline float cfl_speed32(int hq, float qxv, float qyv, float dry, float Kg)
-{
-    const float d = (float)hq * (float)kQuantum;
-    float vx = 0.0f, vy = 0.0f;
-    if (d > dry) {
-        const float inv = rcpf(d);
-        vx = qxv * inv;
-        vy = qyv * inv;
-    }
-    return sqrtf_hw(vx * vx + vy * vy) + sqrtf_hw(Kg * d);
-}
-
-struct Ledger32 {
-    float qx, qy, ke, pot, prs, vmax, hmin;   // sums weighted by the cell weight; max speed, min depth
-    // quanta of the band's cells by row class: first row, inner rows, last row
-    // (three scalars, not an array: a select between array slots would be
-    // turned into a dynamically indexed private array)
-    long long q_first, q_inner, q_last;
-    __device__ void init()
-    {
-        qx = qy = ke = pot = prs = vmax = 0.0f;
-        hmin = 3.0e38f;
-        q_first = q_inner = q_last = 0;
-    }
-    // one cell at row r (uniform over the wave), weight w, terrain b
-    __device__ void add(int r, int ny, int hq, float qxv, float qyv, float b, float w, float dry)
-    {
-        const float d = (float)hq * (float)kQuantum;
-        const long long u = (unsigned)hq;
-        q_first += r == 0 ? u : 0;
-        q_last += r == ny - 1 ? u : 0;
-        q_inner += (r != 0 && r != ny - 1) ? u : 0;
-        qx += qxv * w;
-        qy += qyv * w;
-        if (d > dry) {
-            const float inv = rcpf(d);
-            const float vx = qxv * inv, vy = qyv * inv;
-            const float s2 = vx * vx + vy * vy;
-            vmax = fmaxf(vmax, sqrtf_hw(s2));
-            ke += d * s2 * w;
-        }
-        pot += d * b * w;
-        prs += d * d * w;
-        hmin = fminf(hmin, d);
-    }
-};
-
-// weight of cell (r, c): the cell area, or the Triangle-A-C vertex weight
-// (bulk_state_authority.py:624-634) as a multiple of area / 6
-__device__ inline float cell_weight32(int mode, int ny, int nx, float area, float f6, int r, int c)
-{
-    if (!mode) return area;
-    const int n = (r < ny - 1 && c < nx - 1 ? 2 : 0) + (r < ny - 1 && c > 0 ? 1 : 0) + (r > 0 && c > 0 ? 2 : 0) +
-                  (r > 0 && c < nx - 1 ? 1 : 0);
-    return (float)n * f6;
-}
-
-// 64-bit value of lane `src` (uniform) of the wave
-__device__ inline long long readlane64(long long v, int src)
-{
-    const unsigned long long u = (unsigned long long)v;
-    const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)u, src);
-    const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(u >> 32), src);
-    return (long long)(((unsigned long long)hi << 32) | lo);
-}
-
-// The wave's ledger as partial gw (before or after) of a.groups: fixed-order
-// DPP trees over the lanes (sums and extrema in double, the quanta of each
-// row class as integers), no barrier.  Weight class = row class (first,
-// inner, last row) x column class (first, inner, last column); the first and
-// last columns are at most one lane each and are read back from it.
-__device__ inline void store_ledger32(const V2Args<F32P> &a, const Ledger32 &L, int gc, bool owner, bool before)
-{
-    const double rho = a.cfg.density, g = a.cfg.gravity, K = a.cfg.earth_pressure;
-    const double f0 = wave_sum((double)L.qx), f1 = wave_sum((double)L.qy);
-    const double f2 = wave_max((double)L.vmax);
-    const double f3 = wave_sum((double)L.ke), f4 = wave_sum((double)L.pot), f5 = wave_sum((double)L.prs);
-    const double f6 = wave_min((double)L.hmin);
-    const unsigned long long first_col = __ballot(owner && gc == 0);
-    const unsigned long long last_col = __ballot(owner && gc == a.nx - 1 && gc != 0);
-    const int lf = first_col ? __ffsll((long long)first_col) - 1 : 0;
-    const int ll = last_col ? __ffsll((long long)last_col) - 1 : 0;
-    long long mq[kClasses];
-#pragma unroll
-    for (int r = 0; r < 3; r++) {
-        const long long v = owner ? (r == 0 ? L.q_first : (r == 1 ? L.q_inner : L.q_last)) : 0;
-        const long long tot = wave_sum(v);
-        const long long e0 = first_col ? readlane64(v, lf) : 0;
-        const long long e2 = last_col ? readlane64(v, ll) : 0;
-        mq[3 * r] = e0;
-        mq[3 * r + 1] = tot - e0 - e2;
-        mq[3 * r + 2] = e2;
-    }
-    if ((threadIdx.x & (kWave - 1)) == 0) {
-        const int G = a.groups, gw = blockIdx.x * (kBlock / kWave) + wave_index();
-        double *dst = a.ledger_f + (before ? 0 : kSum) * G + gw;
-        const double vals[kSum] = {0.0, rho * f0, rho * f1, f2, 0.5 * rho * f3, rho * g * f4,
-                                   rho * 0.5 * K * g * f5, f6};
-#pragma unroll
-        for (int q = 0; q < kSum; q++) dst[q * G] = vals[q];
-        long long *qd = a.ledger_q + (before ? 0 : kClasses) * G + gw;
-#pragma unroll
-        for (int q = 0; q < kClasses; q++) qd[q * G] = mq[q];
-    }
 }
 
 // The begin pass over the wave's band of the input state (rows staged with
@@ -1229,6 +1286,10 @@ __global__ void __launch_bounds__(kBlock) v2_end32_kernel(V2Args<F32P> a, int k)
 // The results go straight to pinned host memory (FoldOut); the last wave
 // copies out the resolved loop state and zeroes the next step's loop state
 // (the other parity's block and spread sets, unused during this step).
+// FP32 launches carry more workgroups after the fold's, which copy the final
+// state back into buffer 0 when the last substep left it in buffer 1.  The
+// host sees the record before those copies end; later work on the stream is
+// ordered after them.
 constexpr int kFoldF = 2 * kSum + kDiag + 2;   // double fields: ledgers, diagnostics, static maxima
 constexpr int kFoldQ = 2 * kClasses;           // int64 fields: quanta per weight class
 // field 3 and kSum + 3 are maxima (speed), 7 and kSum + 7 minima (depth), the
@@ -1238,6 +1299,12 @@ constexpr unsigned kFoldMin = (1u << 7) | (1u << (kSum + 7));
 constexpr int kFoldWaves = kFoldF + kFoldQ + 1;
 static_assert(kFoldWaves <= kStampWords, "one completion stamp per fold wave");
 constexpr int kFoldUnroll = 8;
+constexpr int kFoldBlocks = (kFoldWaves + kBlock / kWave - 1) / (kBlock / kWave);
+// FP32: workgroups after the fold's copy the final state back into buffer 0
+// when the last substep stored the closing ledger and left the state in
+// buffer 1, kCopyU cells per thread (kBlock x kCopyU consecutive cells per
+// workgroup)
+constexpr int kCopyU = 8;
 
 struct FoldOut {
     double f[kFoldF];
@@ -1270,6 +1337,35 @@ template <class P>
 __global__ void __launch_bounds__(kBlock) v2_fold_kernel(V2Args<P> a, int k, int nparts, int ndiag, FoldOut *out,
                                                           unsigned *stamps, unsigned stamp)
 {
+    if constexpr (P::kQuant) {
+        if (blockIdx.x >= kFoldBlocks) {
+            const Control C = resolve_control(a.sync, k);
+            if (!(C.done && C.end_done == 2 && C.src == 1)) return;
+            const int n = a.ny * a.nx;
+            const int base = (blockIdx.x - kFoldBlocks) * (kBlock * kCopyU) + threadIdx.x;
+            int hv[kCopyU];
+            float xv[kCopyU], yv[kCopyU];
+#pragma unroll
+            for (int u = 0; u < kCopyU; u++) {
+                const int i = base + u * kBlock;
+                if (i < n) {
+                    hv[u] = a.h[1][i];
+                    xv[u] = a.qx[1][i];
+                    yv[u] = a.qy[1][i];
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < kCopyU; u++) {
+                const int i = base + u * kBlock;
+                if (i < n) {
+                    a.h[0][i] = hv[u];
+                    a.qx[0][i] = xv[u];
+                    a.qy[0][i] = yv[u];
+                }
+            }
+            return;
+        }
+    }
     const int w = blockIdx.x * (kBlock / kWave) + wave_index();
     const int lane = threadIdx.x & (kWave - 1);
     if (w < 2 * kSum) {
@@ -1628,14 +1724,24 @@ static int launch_step(mv2_context *ctx, const mv2_step_config *cfg, mv2_step_re
         hipLaunchKernelGGL(v2_reset_kernel, dim3(1), dim3(kBlock), 0, st, a.sync, a.spread, a.spec_true, a.pred_next);
     ctx->v2_clean = -1;
     int k = 0;
-    int chunk = fixed_chunk > 0 ? fixed_chunk : (ctx->v2_predicted > 0 ? ctx->v2_predicted + 1 : 4);
+    // Substep launches of the first round, for a step that takes as many
+    // substeps as the previous one: FP32's last substep stores the closing
+    // ledger itself (a speculative launch 0 cannot: a step of one substep
+    // closes in launch 1, after the check); FP64 closes in a launch of its own
+    // after one more that finds the loop ended.
+    const int pred_n = ctx->v2_predicted;
+    int chunk = fixed_chunk > 0 ? fixed_chunk
+                : pred_n <= 0 ? 4
+                : !P::kQuant ? pred_n + 1
+                : spec ? (pred_n > 2 ? pred_n : 2) : pred_n;
     // launches expected to work stage their tile before the loop state arrives
-    int eager_until = fixed_chunk > 0 ? 0 : (ctx->v2_predicted > 0 ? ctx->v2_predicted : 4);
+    int eager_until = fixed_chunk > 0 ? 0 : (pred_n > 0 ? (P::kQuant ? chunk : pred_n) : 4);
     // timing mode: launches first .. bracket - 1 between the event pair (a
     // speculative step's launch 0 also carries the begin pass: left out)
     const int first_timed = spec ? 1 : 0;
     const int bracket = eager_until;
-    const StepLaunch L{band, gs, gu, (kFoldWaves + kBlock / kWave - 1) / (kBlock / kWave), fold};
+    const int gcopy = P::kQuant ? (int)((n + kBlock * kCopyU - 1) / (kBlock * kCopyU)) : 0;
+    const StepLaunch L{band, gs, gu, kFoldBlocks + gcopy, fold};
     // the first round (begin, predicted substeps, end / fold) as one graph launch
     bool replayed = false;
     if (ctx->graphs && !timing && fixed_chunk == 0)
