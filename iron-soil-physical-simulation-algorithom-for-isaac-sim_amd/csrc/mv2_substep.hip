@@ -547,21 +547,6 @@ __global__ void __launch_bounds__(kBlock) v2_begin_kernel(V2Args<P> a)
 // itself comes exactly from the quanta per weight class.
 // ---------------------------------------------------------------------------
 
-// CFL wave speed of one cell of an FP32 state, in the arithmetic of the
-// substep: |v| + sqrt(K g h) with the cell velocity from the hardware
-// reciprocal (zero where the cell is dry)
-__device__ inline float cfl_speed32(int hq, float qxv, float qyv, float dry, float Kg)
-{
-    const float d = (float)hq * (float)kQuantum;
-    float vx = 0.0f, vy = 0.0f;
-    if (d > dry) {
-        const float inv = rcpf(d);
-        vx = qxv * inv;
-        vy = qyv * inv;
-    }
-    return sqrtf_hw(vx * vx + vy * vy) + sqrtf_hw(Kg * d);
-}
-
 struct Ledger32 {
     float qx, qy, ke, pot, prs, vmax, hmin;   // sums weighted by the cell weight; max speed, min depth
     // quanta of the band's cells by row class: first row, inner rows, last row
@@ -574,26 +559,36 @@ struct Ledger32 {
         hmin = 3.0e38f;
         q_first = q_inner = q_last = 0;
     }
-    // one cell at row r (uniform over the wave), weight w, terrain b
-    __device__ void add(int r, int ny, int hq, float qxv, float qyv, float b, float w, float dry)
+    // One cell at row r (uniform over the wave), weight w, terrain b.
+    // Returns the cell's CFL wave speed in the arithmetic of the substep:
+    // |v| + sqrt(K g h), the velocity from the hardware reciprocal and zero
+    // where the cell is dry.  Every pass that forms a CFL maximum from a
+    // stored state calls this, so a prediction and the true maximum it is
+    // checked against are the same bits.
+    __device__ float add(int r, int ny, int hq, float qxv, float qyv, float b, float w, float dry, float Kg)
     {
         const float d = (float)hq * (float)kQuantum;
         const long long u = (unsigned)hq;
+        // selects, not branches: this runs inside the substep's row loop,
+        // where a join would make the compiler wait for every row in flight
         q_first += r == 0 ? u : 0;
         q_last += r == ny - 1 ? u : 0;
         q_inner += (r != 0 && r != ny - 1) ? u : 0;
         qx += qxv * w;
         qy += qyv * w;
+        float sp = 0.0f;
         if (d > dry) {
             const float inv = rcpf(d);
             const float vx = qxv * inv, vy = qyv * inv;
             const float s2 = vx * vx + vy * vy;
-            vmax = fmaxf(vmax, sqrtf_hw(s2));
+            sp = sqrtf_hw(s2);
+            vmax = fmaxf(vmax, sp);
             ke += d * s2 * w;
         }
         pot += d * b * w;
         prs += d * d * w;
         hmin = fminf(hmin, d);
+        return sp + sqrtf_hw(Kg * d);
     }
 };
 
@@ -606,6 +601,26 @@ __device__ inline float cell_weight32(int mode, int ny, int nx, float area, floa
                   (r > 0 && c < nx - 1 ? 1 : 0);
     return (float)n * f6;
 }
+
+// The weights of one column (a lane's) for the three row classes: a row's
+// weight then depends only on whether it is the first, the last or an inner
+// row, a uniform choice per row.
+struct ColWeights {
+    float first, inner, last;
+    __device__ ColWeights(int mode, int ny, int nx, float area, float f6, int c)
+        : first(cell_weight32(mode, ny, nx, area, f6, 0, c)),
+          inner(cell_weight32(mode, ny, nx, area, f6, ny > 2 ? 1 : 0, c)),
+          last(cell_weight32(mode, ny, nx, area, f6, ny - 1, c)) {}
+    // bit selects on the values: a select between the members themselves
+    // becomes a select of their addresses and puts the struct in scratch
+    __device__ float at(int r, int ny) const
+    {
+        const unsigned m0 = r == 0 ? ~0u : 0u, ml = r == ny - 1 ? ~0u : 0u;
+        unsigned w = (__float_as_uint(first) & m0) | (__float_as_uint(inner) & ~m0);
+        w = (__float_as_uint(last) & ml) | (w & ~ml);
+        return __uint_as_float(w);
+    }
+};
 
 // 64-bit value of lane `src` (uniform) of the wave
 __device__ inline long long readlane64(long long v, int src)
@@ -682,13 +697,13 @@ __device__ inline void end32_body(const V2Args<F32P> &a, int fsrc);
 // state (end32_body), so the partials are the same bits.
 struct LedgerSink {
     Ledger32 L;
-    double pred;
-    int mode, ny, nx;
-    float area, f6, dry, Kg;
-    __device__ void operator()(int r, int gc, int hq, float qxv, float qyv, float b)
+    ColWeights cw;
+    float pred;          // a float maximum: the same value as the end pass's double one
+    int ny;
+    float dry, Kg;
+    __device__ void operator()(int r, int, int hq, float qxv, float qyv, float b)
     {
-        L.add(r, ny, hq, qxv, qyv, b, cell_weight32(mode, ny, nx, area, f6, r, gc), dry);
-        pred = npmax(pred, (double)cfl_speed32(hq, qxv, qyv, dry, Kg));
+        pred = fmaxf(pred, L.add(r, ny, hq, qxv, qyv, b, cw.at(r, ny), dry, Kg));
     }
 };
 
@@ -729,7 +744,7 @@ __device__ inline Control next_control(const Control &C, const mv2_step_config &
 // state in its registers (end_done = 2); the fold pass then copies the state
 // back when it sits in the second buffer.  A failure flagged by this substep
 // rolls the loop back and voids that ledger (apply_failure_flags).
-template <int SH, int DIAGF>
+template <int SH, int DIAGF, bool CAN_CLOSE = true>
 __device__ inline void substep32_work(const V2Args<F32P> &a, const StripRows<SH> &rows, const Control &C,
                                       double maxw, int k, int r0, int c0, double dprev, double spec_pred)
 {
@@ -741,7 +756,9 @@ __device__ inline void substep32_work(const V2Args<F32P> &a, const StripRows<SH>
     const double stable = maxw <= 1.0e-14 ? C.remaining : cfg.cfl * mind / maxw;
     const double sub_dt = npmin(C.remaining, stable);
     Control nc = next_control(C, cfg, maxw, sub_dt, mind);
-    const bool last = spec_pred < 0.0 && nc.done;
+    // CAN_CLOSE false: a speculative launch 0, whose substep never closes
+    // the step (the close path and its registers are left out)
+    const bool last = CAN_CLOSE && spec_pred < 0.0 && nc.done;
     if (last) nc.end_done = 2;
     if (spec_pred >= 0.0) {
         nc.spec = 1;
@@ -757,23 +774,19 @@ __device__ inline void substep32_work(const V2Args<F32P> &a, const StripRows<SH>
     int ovf = 0;
     const bool interior = strip_interior<SH>(a.ny, a.nx, r0, c0);
     if (last) {
-        LedgerSink sink;
+        const int gcl = c0 - 1 + lane;
+        LedgerSink sink{Ledger32{}, ColWeights(cfg.weight_mode, a.ny, a.nx, uniform((float)(a.dx * a.dy)),
+                                               uniform((float)(a.dx * a.dy / 6.0)), gcl),
+                        0.0f, a.ny, uniform((float)cfg.dry_tolerance),
+                        uniform((float)(cfg.earth_pressure * cfg.gravity))};
         sink.L.init();
-        sink.pred = 0.0;
-        sink.mode = cfg.weight_mode;
-        sink.ny = a.ny;
-        sink.nx = a.nx;
-        sink.area = uniform((float)(a.dx * a.dy));
-        sink.f6 = uniform((float)(a.dx * a.dy / 6.0));
-        sink.dry = uniform((float)cfg.dry_tolerance);
-        sink.Kg = uniform((float)(cfg.earth_pressure * cfg.gravity));
         if (interior)
             strip_update<SH, V2Args<F32P>, DIAGF, true>(a, rows, C.src, r0, c0, sub_dt, dg, wv, hn_min, ovf, sink);
         else
             strip_update<SH, V2Args<F32P>, DIAGF, false>(a, rows, C.src, r0, c0, sub_dt, dg, wv, hn_min, ovf, sink);
         const int gc = c0 - 1 + lane;
         store_ledger32(a, sink.L, gc, lane >= 1 && lane <= kStripW && gc < a.nx, false);
-        const double pm = wave_max(sink.pred);
+        const double pm = wave_max((double)sink.pred);
         if (lane == 0) atomic_max_nonneg(&a.pred_next[(gw % kSpread) * kSpreadStride], pm);
     } else if (interior) {
         strip_update<SH, V2Args<F32P>, DIAGF, true>(a, rows, C.src, r0, c0, sub_dt, dg, wv, hn_min, ovf);
@@ -1131,6 +1144,7 @@ __device__ inline void begin32_work(const V2Args<F32P> &a, const StripRows<SH> &
     const float dry = uniform((float)a.cfg.dry_tolerance);
     const float Kg = uniform((float)(a.cfg.earth_pressure * a.cfg.gravity));
     const StaticGrad sg(a.cfg.tan_start, a.dx, a.dy, gc > 0 && gc + 1 < nx);
+    const ColWeights cw(a.cfg.weight_mode, ny, nx, area, f6, gc);
     Ledger32 L;
     L.init();
     double m[3] = {0.0, 0.0, 0.0};              // max |grad H|^2, max |q|, max wave speed
@@ -1141,8 +1155,7 @@ __device__ inline void begin32_work(const V2Args<F32P> &a, const StripRows<SH> &
         if (!owner || r >= ny) continue;
         const int hq = R.hq[j];
         const float qxv = R.qx[j], qyv = R.qy[j];
-        L.add(r, ny, hq, qxv, qyv, R.b[j], cell_weight32(a.cfg.weight_mode, ny, nx, area, f6, r, gc), dry);
-        m[2] = npmax(m[2], (double)cfl_speed32(hq, qxv, qyv, dry, Kg));
+        m[2] = npmax(m[2], (double)L.add(r, ny, hq, qxv, qyv, R.b[j], cw.at(r, ny), dry, Kg));
         if (stat) {
             const double hl = gc > 0 ? HL : H[j], hr = gc + 1 < nx ? HR : H[j];
             const double hd = r > 0 ? H[j - 1] : H[j], hu = r + 1 < ny ? H[j + 1] : H[j];
@@ -1207,7 +1220,7 @@ __global__ void __launch_bounds__(kBlock) v2_first32_kernel(V2Args<F32P> a)
     strip_load<SH>(a, 0, r0, c0, rows);
     begin32_work<SH>(a, rows, r0, c0, a.spec_true);
     const double pred = wave_max(pslot);
-    substep32_work<SH, DIAGF>(a, rows, fresh_control(a.cfg.dt), pred, 0, r0, c0, 0.0, pred);
+    substep32_work<SH, DIAGF, false>(a, rows, fresh_control(a.cfg.dt), pred, 0, r0, c0, 0.0, pred);
 }
 
 // copy back when the final state sits in scratch; ledger after (FP32 strips).
@@ -1246,6 +1259,7 @@ __device__ inline void end32_body(const V2Args<F32P> &a, int fsrc)
     const float area = uniform((float)(a.dx * a.dy)), f6 = uniform((float)(a.dx * a.dy / 6.0));
     const float dry = uniform((float)a.cfg.dry_tolerance);
     const float Kg = uniform((float)(a.cfg.earth_pressure * a.cfg.gravity));
+    const ColWeights cw(a.cfg.weight_mode, ny, nx, area, f6, gc);
     Ledger32 L;
     L.init();
     double pred = 0.0;
@@ -1259,8 +1273,7 @@ __device__ inline void end32_body(const V2Args<F32P> &a, int fsrc)
             a.qx[0][i] = qxv[j];
             a.qy[0][i] = qyv[j];
         }
-        L.add(r, ny, hq[j], qxv[j], qyv[j], bv[j], cell_weight32(a.cfg.weight_mode, ny, nx, area, f6, r, gc), dry);
-        pred = npmax(pred, (double)cfl_speed32(hq[j], qxv[j], qyv[j], dry, Kg));
+        pred = npmax(pred, (double)L.add(r, ny, hq[j], qxv[j], qyv[j], bv[j], cw.at(r, ny), dry, Kg));
     }
     store_ledger32(a, L, gc, owner, false);
     // the CFL maximum of the result, as the next step's begin pass would
