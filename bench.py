@@ -35,11 +35,12 @@ N = 701
 DX = 0.05
 FRAME_DT = 1.0 / 60.0
 HBM_PEAK_GBS = 8000.0
-# HBM bytes per working substep launch of the FP32 kernel on this workload,
+# HBM bytes per timed substep launch of the FP32 kernel on this workload,
 # from rocprofv3 FETCH_SIZE / WRITE_SIZE scaled by a 64 MiB copy
-# (profiles/v2_substep_701x701_cutface_traffic.md); counters cannot be read
-# from inside the timed run.
-MEASURED_TRAFFIC_BYTES = {"fp32": 15.20e6}
+# (profiles/v2_substep_701x701_cutface_traffic.md): the mean of the checking
+# launch (15.46 MB) and the closing substep (15.63 MB), the launches a
+# 3-substep step times; counters cannot be read from inside the timed run.
+MEASURED_TRAFFIC_BYTES = {"fp32": 15.5e6}
 # the same for a full launch of the batched substep (256 terrains, 18 274 304
 # cells; profiles/batch_substep_256_terrains.md)
 MEASURED_TRAFFIC_BATCHED_FULL = 515.1e6
