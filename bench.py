@@ -804,7 +804,11 @@ def main() -> int:
                      "frac": achieved / HBM_PEAK_GBS, "traffic": MEASURED_TRAFFIC_BYTES.get(args.precision),
                      "kernel": "v2_substep_kernel", "bytes_per_cell_substep": bpc,
                      "cells_per_launch": cells, "avg_launch_ms": avg_substep_ms,
-                     "launches_timed": timed_sub, "frames_timed": timed_frames},
+                     "launches_timed": timed_sub, "frames_timed": timed_frames,
+                     "note": "one HIP event pair per step around its substep launches after launch 0 "
+                             "(which also runs the begin pass): the launch that checks the CFL "
+                             "prediction, plain working launches and the substep that ends the loop "
+                             "and stores the closing ledger, gaps between dependent launches included"},
     }
     if args.precision == "fp32" and not args.skip_configs:
         out["c3_batched"] = batched_result(256, 120, 3, args.seed)          # 2 s simulated
