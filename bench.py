@@ -43,7 +43,7 @@ HBM_PEAK_GBS = 8000.0
 MEASURED_TRAFFIC_BYTES = {"fp32": 15.5e6}
 # the same for a full launch of the batched substep (256 terrains, 18 274 304
 # cells; profiles/batch_substep_256_terrains.md)
-MEASURED_TRAFFIC_BATCHED_FULL = 515.1e6
+MEASURED_TRAFFIC_BATCHED_FULL = 519.5e6
 BYTES_PER_CELL_SUBSTEP = {"fp32": 28, "fp64": 56}
 
 
