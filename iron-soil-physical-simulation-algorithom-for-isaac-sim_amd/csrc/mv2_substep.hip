@@ -639,10 +639,12 @@ __device__ inline long long readlane64(long long v, int src)
 __device__ inline void store_ledger32(const V2Args<F32P> &a, const Ledger32 &L, int gc, bool owner, bool before)
 {
     const double rho = a.cfg.density, g = a.cfg.gravity, K = a.cfg.earth_pressure;
-    const double f0 = wave_sum((double)L.qx), f1 = wave_sum((double)L.qy);
-    const double f2 = wave_max((double)L.vmax);
-    const double f3 = wave_sum((double)L.ke), f4 = wave_sum((double)L.pot), f5 = wave_sum((double)L.prs);
-    const double f6 = wave_min((double)L.hmin);
+    // float DPP trees (one instruction per step; the lanes' sums are float
+    // already), widened per wave; the fold sums the waves in double
+    const double f0 = (double)wave_sum(L.qx), f1 = (double)wave_sum(L.qy);
+    const double f2 = (double)wave_max(L.vmax);
+    const double f3 = (double)wave_sum(L.ke), f4 = (double)wave_sum(L.pot), f5 = (double)wave_sum(L.prs);
+    const double f6 = (double)wave_min(L.hmin);
     const unsigned long long first_col = __ballot(owner && gc == 0);
     const unsigned long long last_col = __ballot(owner && gc == a.nx - 1 && gc != 0);
     const int lf = first_col ? __ffsll((long long)first_col) - 1 : 0;
