@@ -823,7 +823,10 @@ __device__ inline void substep32_work(const V2Args<F32P> &a, const StripRows<SH>
     }
 }
 
-template <class P, int SH = kSH, int DIAGF = 0, int EAGER = 0, int CHECK = 1>
+// CLOSE: the launch may close the step (FP32: the last launch the host
+// enqueues in a round; a step that ends in an earlier launch is closed by the
+// end pass of the launch after it, which the round holds)
+template <class P, int SH = kSH, int DIAGF = 0, int EAGER = 0, int CHECK = 1, bool CLOSE = true>
 __global__ void __launch_bounds__(kBlock) v2_substep_kernel(V2Args<P> a, int k)
 {
     constexpr int kRows = SH * (kBlock / kWave);     // rows of a tile
@@ -1006,7 +1009,8 @@ __global__ void __launch_bounds__(kBlock) v2_substep_kernel(V2Args<P> a, int k)
         // launches expected to find the loop ended stage the state only now
         if constexpr (!EAGER) strip_load<SH>(a, C.src, r0, c0, rows);
         // a step's first substep starts its diagnostic slots from zero
-        substep32_work<SH, DIAGF>(a, rows, C, maxw, k, r0, c0, C.substeps > 0 && lane < kDiag ? dprev : 0.0, -1.0);
+        substep32_work<SH, DIAGF, CLOSE>(a, rows, C, maxw, k, r0, c0, C.substeps > 0 && lane < kDiag ? dprev : 0.0,
+                                         -1.0);
     } else {
         // CFL maximum of the current state (mobile_v2_reference.py:239-251)
         if (threadIdx.x < kWave) {
@@ -1508,21 +1512,26 @@ static void enqueue_first(const V2Args<P> &a, const StepLaunch &L, hipStream_t s
 // one FP32 substep launch of band height SH, all ten diagnostic sums or the
 // production record's five
 template <class P, int SH, int DIAGF, int EAGER>
-static void launch_substep32_c(const V2Args<P> &a, int grid, hipStream_t st, int k)
+static void launch_substep32_c(const V2Args<P> &a, int grid, hipStream_t st, int k, bool close)
 {
     // the first two launches of a step test the static exit / the speculation
-    if (k <= 1) hipLaunchKernelGGL((v2_substep_kernel<P, SH, DIAGF, EAGER, 1>), dim3(grid), dim3(kBlock), 0, st, a, k);
-    else hipLaunchKernelGGL((v2_substep_kernel<P, SH, DIAGF, EAGER, 0>), dim3(grid), dim3(kBlock), 0, st, a, k);
+    if (k <= 1) {
+        if (close) hipLaunchKernelGGL((v2_substep_kernel<P, SH, DIAGF, EAGER, 1, true>), dim3(grid), dim3(kBlock), 0, st, a, k);
+        else hipLaunchKernelGGL((v2_substep_kernel<P, SH, DIAGF, EAGER, 1, false>), dim3(grid), dim3(kBlock), 0, st, a, k);
+    } else {
+        if (close) hipLaunchKernelGGL((v2_substep_kernel<P, SH, DIAGF, EAGER, 0, true>), dim3(grid), dim3(kBlock), 0, st, a, k);
+        else hipLaunchKernelGGL((v2_substep_kernel<P, SH, DIAGF, EAGER, 0, false>), dim3(grid), dim3(kBlock), 0, st, a, k);
+    }
 }
 
 template <class P, int SH>
-static void launch_substep32(const V2Args<P> &a, int grid, hipStream_t st, int k, int eager)
+static void launch_substep32(const V2Args<P> &a, int grid, hipStream_t st, int k, int eager, bool close)
 {
     const int d = a.cfg.diagnostics == 1 ? 1 : 0;
-    if (d && eager) launch_substep32_c<P, SH, 1, 1>(a, grid, st, k);
-    else if (d) launch_substep32_c<P, SH, 1, 0>(a, grid, st, k);
-    else if (eager) launch_substep32_c<P, SH, 0, 1>(a, grid, st, k);
-    else launch_substep32_c<P, SH, 0, 0>(a, grid, st, k);
+    if (d && eager) launch_substep32_c<P, SH, 1, 1>(a, grid, st, k, close);
+    else if (d) launch_substep32_c<P, SH, 1, 0>(a, grid, st, k, close);
+    else if (eager) launch_substep32_c<P, SH, 0, 1>(a, grid, st, k, close);
+    else launch_substep32_c<P, SH, 0, 0>(a, grid, st, k, close);
 }
 
 // substep launches k0 .. k0 + count - 1.  Timing mode brackets the launches
@@ -1536,12 +1545,14 @@ static int enqueue_substeps(mv2_context *ctx, const V2Args<P> &a, const StepLaun
         if (timing && k == first_timed) MV2_HIP(ctx, hipEventRecord(ctx->sub_events[0], st));
         const int eager = k < eager_until ? 1 : 0;
         if constexpr (P::kQuant) {
+            // only the round's last launch carries the close path
+            const bool close = k == k0 + count - 1;
             switch (L.band) {
-            case 2: launch_substep32<P, 2>(a, L.gu, st, k, eager); break;
-            case 3: launch_substep32<P, 3>(a, L.gu, st, k, eager); break;
-            case 5: launch_substep32<P, 5>(a, L.gu, st, k, eager); break;
-            case 6: launch_substep32<P, 6>(a, L.gu, st, k, eager); break;
-            default: launch_substep32<P, 4>(a, L.gu, st, k, eager); break;
+            case 2: launch_substep32<P, 2>(a, L.gu, st, k, eager, close); break;
+            case 3: launch_substep32<P, 3>(a, L.gu, st, k, eager, close); break;
+            case 5: launch_substep32<P, 5>(a, L.gu, st, k, eager, close); break;
+            case 6: launch_substep32<P, 6>(a, L.gu, st, k, eager, close); break;
+            default: launch_substep32<P, 4>(a, L.gu, st, k, eager, close); break;
             }
         } else {
             hipLaunchKernelGGL(v2_substep_kernel<P>, dim3(L.gu), dim3(kBlock), 0, st, a, k);
